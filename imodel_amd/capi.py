@@ -1,0 +1,146 @@
+"""ctypes view of the libimx C ABI (include/imx_swm.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` into
+``imodel_amd/libimx.so``. Loading fails loudly when it is missing: there is
+no CPU fallback for any product entry point.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libimx.so")
+
+IMX_MAXNB = 6
+PREC_FP64_REF, PREC_FP64, PREC_FP32 = 0, 1, 2
+POS_EQS, POS_POL = 0, 1
+NDIAG = 8
+DIAG_NAMES = ("mass", "penergy", "kenergy", "tenergy", "aenergy", "penstrophy",
+              "rmsdiv", "badflag")
+FIELDS = ("h_ed", "uh", "eta", "h_tr", "q_tr", "q_ed", "divuh", "ke_hx", "ghbk",
+          "grad_ghbk", "uhq_perp", "divu", "zeta")
+FIELD_POS = {"h_ed": "e", "uh": "e", "eta": "t", "h_tr": "t", "q_tr": "t",
+             "q_ed": "e", "divuh": "v", "ke_hx": "v", "ghbk": "v",
+             "grad_ghbk": "e", "uhq_perp": "e", "divu": "v", "zeta": "t"}
+
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_F64P = ctypes.POINTER(ctypes.c_double)
+
+
+class MeshDesc(ctypes.Structure):
+    """imx_mesh_desc: flat arrays of the reference grid_structure (1-based)."""
+    _fields_ = [
+        ("nv", ctypes.c_int32), ("ne", ctypes.c_int32), ("nt", ctypes.c_int32),
+        ("glevel", ctypes.c_int32),
+        ("v_nnb", _I32P), ("v_nb", _I32P), ("v_ed", _I32P), ("v_tr", _I32P),
+        ("hx_ttgout", _I32P), ("hx_tnrccw", _I32P), ("v_p", _F64P),
+        ("hx_areag", _F64P),
+        ("ed_v", _I32P), ("ed_sh", _I32P), ("ed_leng", _F64P), ("edhx_leng", _F64P),
+        ("ed_c", _F64P), ("edhx_c", _F64P),
+        ("tr_v", _I32P), ("tr_ed", _I32P), ("tr_tg", _I32P), ("tr_areag", _F64P),
+        ("tr_c", _F64P), ("tr_clat", _F64P),
+        ("tr_trhx_area", _F64P), ("hx_hxtr_area", _F64P), ("hx_trskw", _F64P),
+    ]
+
+
+DESC_ARRAYS = [f for f, _ in MeshDesc._fields_[4:]]
+OPTIONAL_ARRAYS = ("tr_trhx_area", "hx_hxtr_area", "hx_trskw")
+
+
+def make_desc(arrays):
+    """Build a MeshDesc over a dict of numpy arrays (IMXBIN record names).
+
+    Returns (desc, keepalive); keep ``keepalive`` referenced while the
+    descriptor is in use.
+    """
+    nv, ne, nt = (int(x) for x in arrays["dims"][:3])
+    glevel = int(arrays["dims"][3]) if len(arrays["dims"]) > 3 else -1
+    keep = {}
+    d = MeshDesc(nv=nv, ne=ne, nt=nt, glevel=glevel)
+    for name, ctype in MeshDesc._fields_[4:]:
+        if name not in arrays:
+            if name in OPTIONAL_ARRAYS:
+                continue
+            raise KeyError(f"mesh record {name!r} missing")
+        if ctype is _I32P:
+            a = np.ascontiguousarray(arrays[name], dtype=np.int32)
+        else:
+            a = np.ascontiguousarray(arrays[name], dtype=np.float64)
+        keep[name] = a
+        setattr(d, name, a.ctypes.data_as(ctype))
+    return d, keep
+
+
+class SwmOpts(ctypes.Structure):
+    """imx_swm_opts."""
+    _fields_ = [("precision", ctypes.c_int32), ("testcase", ctypes.c_int32),
+                ("use_sfc", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("erad", ctypes.c_double), ("grav", ctypes.c_double),
+                ("omega", ctypes.c_double), ("dt", ctypes.c_double),
+                ("h_ref", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libimx (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libimx not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        sig = {
+            "imx_last_error": (ctypes.c_char_p, []),
+            "imx_mesh_build_icos": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(vp)]),
+            "imx_mesh_desc_of": (ctypes.c_int, [vp, ctypes.POINTER(MeshDesc)]),
+            "imx_mesh_gridprop": (ctypes.c_int, [vp]),
+            "imx_mesh_write": (ctypes.c_int, [vp, ctypes.c_char_p]),
+            "imx_mesh_destroy": (None, [vp]),
+            "imx_gridprop": (ctypes.c_int, [ctypes.POINTER(MeshDesc), _F64P, _F64P, _F64P]),
+            "imx_ic_tc2": (ctypes.c_int, [vp, ctypes.c_double, _F64P, _F64P]),
+            "imx_ic_spectral": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_uint64,
+                                               ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double, _F64P, _F64P]),
+            "imx_swm_default_opts": (None, [ctypes.POINTER(SwmOpts)]),
+            "imx_swm_create": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.POINTER(SwmOpts),
+                                              ctypes.POINTER(vp)]),
+            "imx_swm_set_state": (ctypes.c_int, [vp, _F64P, _F64P]),
+            "imx_swm_get_state": (ctypes.c_int, [vp, _F64P, _F64P]),
+            "imx_swm_tendency": (ctypes.c_int, [vp, _F64P, _F64P, _F64P, _F64P]),
+            "imx_swm_get_field": (ctypes.c_int, [vp, ctypes.c_int32, _F64P]),
+            "imx_swm_step": (ctypes.c_int, [vp, ctypes.c_int32]),
+            "imx_swm_step_timed": (ctypes.c_int, [vp, ctypes.c_int32, _F64P]),
+            "imx_swm_profile": (ctypes.c_int, [vp, ctypes.c_int32, _F64P, _I32P]),
+            "imx_swm_kernel_name": (ctypes.c_char_p, [vp, ctypes.c_int32]),
+            "imx_swm_diagnostics": (ctypes.c_int, [vp, _F64P]),
+            "imx_swm_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
+            "imx_swm_destroy": (None, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class ImxError(RuntimeError):
+    """A libimx call returned a non-zero code."""
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().imx_last_error().decode(errors="replace")
+        raise ImxError(f"{what}: libimx error {rc}: {msg}")
+
+
+def f64p(a):
+    return a.ctypes.data_as(_F64P)
+
+
+def i32p(a):
+    return a.ctypes.data_as(_I32P)

@@ -1,0 +1,58 @@
+// Internal declarations shared by the host-side translation units of libimx.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/imx_swm.h"
+
+namespace imx {
+
+// Thread-local last error; every C entry point returns a code and leaves the
+// message here.
+int set_error(int code, const std::string &msg);
+
+// Host mesh in the reference's conventions (1-based indices, per-cell lists
+// padded to IMX_MAXNB). Field names follow grid_structure.
+struct HostMesh {
+  int32_t nv = 0, ne = 0, nt = 0, glevel = -1;
+  std::vector<double> v_p, v_lat, v_lon, hx_areag;
+  std::vector<int32_t> v_nnb, v_nb, v_ed, v_tr, hx_ttgout, hx_tnrccw, hx_tg, hx_nr;
+  std::vector<int32_t> ed_v, ed_sh;
+  std::vector<double> ed_leng, edhx_leng, ed_c, ed_clat, ed_clon, ed_tg, ed_nr;
+  std::vector<double> edhx_c, edhx_nr, edhx_tg;
+  std::vector<int32_t> tr_v, tr_ed, tr_tg, tr_nr;
+  std::vector<double> tr_areag, tr_c, tr_clat, tr_clon;
+  std::vector<double> tr_trhx_area, hx_hxtr_area, hx_trskw;  // gridprop
+};
+
+// Spherical geometry with the reference's expression order
+// (src/smeshpack.f90: arclen :4624, sphtriarea :4710, cross_product :6560,
+// det :6580, norm :7061, cart2sph :4279). Evaluated without FMA contraction.
+double dot3(const double *a, const double *b);
+void cross3(const double *a, const double *b, double *c);
+double norm3(const double *a);
+double arclen(const double *p, const double *q);
+double sphtriarea(const double *p1, const double *p2, const double *p3);
+double det3(const double *p1, const double *p2, const double *p3);
+void cart2sph(double x, double y, double z, double *lon, double *lat);
+
+void gridprop(const imx_mesh_desc &d, double *trhx, double *hxtr, double *trskw);
+void desc_of(const HostMesh &m, imx_mesh_desc &d);
+
+// IMXBIN container: "IMXBIN01", then records (name[24], dtype[4], int64 n, data).
+struct BinWriter {
+  FILE *f = nullptr;
+  explicit BinWriter(const char *path);
+  ~BinWriter();
+  bool ok() const { return f != nullptr; }
+  void put(const char *name, const std::vector<int32_t> &a);
+  void put(const char *name, const std::vector<double> &a);
+  void put_i4(const char *name, const int32_t *a, int64_t n);
+  void put_f8(const char *name, const double *a, int64_t n);
+};
+
+}  // namespace imx
+
+// The opaque handle of the C ABI is the host mesh itself.
+struct imx_mesh : public imx::HostMesh {};

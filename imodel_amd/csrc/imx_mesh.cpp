@@ -1,0 +1,735 @@
+// Host-side icosahedral mesh builder, reference-convention structuring and
+// grid-property precomputation for libimx. Compiled with -ffp-contract=off so
+// that every FP64 expression rounds exactly as written.
+//
+// What is kept alike with the reference, and why:
+//  * the 12 base nodes (eqs/pol) and the refinement rule "normalised mean of
+//    the two end points" (src/smeshpack.f90:1234-1279, :1659-1665): they fix
+//    the node coordinates, which are part of the mesh definition;
+//  * the numbering RULES of meshstruct (src/smeshpack.f90:706-886): edge l
+//    runs from its lower to its higher node, triangles are owned by their
+//    lowest node, ed%sh is filled in triangle order; the sign tables
+//    tr%tg/nr, hx%tg/nr, hx%ttgout, hx%tnrccw (:840-860, :1071-1104). The C-grid
+//    operators are written against these conventions;
+//  * the geometry formulas (arclen, sphtriarea, circumcentre, sphhxarea,
+//    midpoints) and initialize_gridprop (kite areas, TRiSK weights of
+//    Thuburn et al. 2009 eq. 33): they define the discretisation.
+// The triangulation itself is NOT computed with a Delaunay insertion as in
+// the reference (STRIPACK trmesh): the icosahedral subdivision is carried as
+// explicit triangles, so neighbour rotations start at the lowest-numbered
+// neighbour. The resulting mesh equals the reference's up to a renumbering.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <unordered_map>
+
+#include "imx_internal.h"
+
+namespace imx {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+// ---------------------------------------------------------------- geometry --
+// Fortran dot_product over 3 elements: ((a1 b1 + a2 b2) + a3 b3).
+double dot3(const double *a, const double *b) {
+  double s = a[0] * b[0];
+  s = s + a[1] * b[1];
+  s = s + a[2] * b[2];
+  return s;
+}
+
+void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+double norm3(const double *a) { return std::sqrt(dot3(a, a)); }
+
+// arclen (src/smeshpack.f90:4624-4657): geodesic distance on the unit sphere.
+double arclen(const double *p, const double *q) {
+  double s[3] = {p[0] + q[0], p[1] + q[1], p[2] + q[2]};
+  double d = dot3(s, s);
+  if (d == 0.0) return 3.1415927410125732;  // the reference's single-rounded pi
+  if (d >= 4.0) return 0.0;
+  return 2.0 * std::atan(std::sqrt((4.0 - d) / d));
+}
+
+// epsilon(1.) of the reference (constants.f90:45): single-precision epsilon.
+static const double kEpsSingle = 1.1920928955078125e-07;
+
+static inline void sub3(const double *a, const double *b, double *c) {
+  c[0] = a[0] - b[0];
+  c[1] = a[1] - b[1];
+  c[2] = a[2] - b[2];
+}
+
+// sphtriarea (src/smeshpack.f90:4710-4762): L'Huilier's formula.
+double sphtriarea(const double *p1, const double *p2, const double *p3) {
+  double t[3];
+  sub3(p1, p2, t);
+  double n12 = norm3(t);
+  sub3(p2, p3, t);
+  double n23 = norm3(t);
+  sub3(p1, p3, t);
+  double n13 = norm3(t);
+  if (n12 < kEpsSingle / 100.0 || n23 < kEpsSingle / 100.0 || n13 < kEpsSingle / 100.0)
+    return 0.0;
+  double a = arclen(p1, p2);
+  double b = arclen(p2, p3);
+  double c = arclen(p3, p1);
+  double s = (a + b + c) / 2.0;
+  double tmp = std::tan(s / 2.0) * std::tan((s - a) / 2.0);
+  tmp = tmp * std::tan((s - b) / 2.0);
+  tmp = tmp * std::tan((s - c) / 2.0);
+  if (tmp < 0.0) return 0.0;
+  return 4.0 * std::atan(std::sqrt(tmp));
+}
+
+// det (src/smeshpack.f90:6580): (p1 x p2) . p3
+double det3(const double *p1, const double *p2, const double *p3) {
+  double c[3];
+  cross3(p1, p2, c);
+  return dot3(c, p3);
+}
+
+// cart2sph (src/smeshpack.f90:4279-4314)
+void cart2sph(double x, double y, double z, double *lon, double *lat) {
+  double pnrm = x * x;
+  pnrm = pnrm + y * y;
+  pnrm = pnrm + z * z;
+  pnrm = std::sqrt(pnrm);
+  *lon = (x != 0.0 || y != 0.0) ? std::atan2(y, x) : 0.0;
+  *lat = (pnrm != 0.0) ? std::asin(z / pnrm) : 0.0;
+}
+
+static inline double fsign1(double x) { return std::signbit(x) ? -1.0 : 1.0; }
+
+// p <- p / norm(p), element by element (as `p=p/norm(p)`).
+static inline void normalize3(double *p) {
+  double n = norm3(p);
+  p[0] = p[0] / n;
+  p[1] = p[1] / n;
+  p[2] = p[2] / n;
+}
+
+// ------------------------------------------------------------- gridprop ----
+// initialize_gridprop for HTC/trsk10 (src/swm.f90:979-1041):
+//   trhx_intersec_areas (src/smeshpack.f90:5080-5154),
+//   hxtr_intersec_areas (src/smeshpack.f90:4991-5078) normalised by areag,
+//   calc_trisk_weights  (src/interpack.f90:3978-4098).
+void gridprop(const imx_mesh_desc &d, double *trhx, double *hxtr, double *trskw) {
+  const int MX = IMX_MAXNB;
+  for (int k = 0; k < d.nt; ++k) {
+    const double *pc = d.tr_c + 3 * k;
+    for (int j = 0; j < 3; ++j) {
+      int jm = (j + 2) % 3;  // modint(j-1,3) in 1-based terms
+      int ed1 = d.tr_ed[3 * k + jm] - 1;
+      int ed2 = d.tr_ed[3 * k + j] - 1;
+      const double *ped1 = d.ed_c + 3 * ed1;
+      const double *ped2 = d.ed_c + 3 * ed2;
+      const double *pv = d.v_p + 3 * (d.tr_v[3 * k + j] - 1);
+      double area1 = sphtriarea(pc, ped1, pv);
+      if (det3(pc, ped1, pv) < 0.0) area1 = -area1;
+      double area2 = sphtriarea(pc, pv, ped2);
+      if (det3(pc, pv, ped2) < 0.0) area2 = -area2;
+      trhx[3 * k + j] = area1 + area2;
+    }
+  }
+  for (int i = 0; i < d.nv; ++i) {
+    int n = d.v_nnb[i];
+    const double *pc = d.v_p + 3 * i;
+    double *R = hxtr + MX * i;
+    for (int j = 0; j < MX; ++j) R[j] = 0.0;
+    for (int j = 0; j < n; ++j) {
+      int ed1 = d.v_ed[MX * i + j] - 1;
+      int ed2 = d.v_ed[MX * i + (j + 1) % n] - 1;
+      const double *ped1 = d.edhx_c + 3 * ed1;
+      const double *ped2 = d.edhx_c + 3 * ed2;
+      const double *pv = d.tr_c + 3 * (d.v_tr[MX * i + j] - 1);
+      double area1 = sphtriarea(pc, ped1, pv);
+      if (det3(pc, ped1, pv) < 0.0) area1 = -area1;
+      double area2 = sphtriarea(pc, pv, ped2);
+      if (det3(pc, pv, ped2) < 0.0) area2 = -area2;
+      R[j] = area1 + area2;
+    }
+    for (int j = 0; j < n; ++j) R[j] = R[j] / d.hx_areag[i];
+    double *W = trskw + MX * MX * i;
+    for (int j = 0; j < MX * MX; ++j) W[j] = 0.0;
+    // w(a,b) = sum of R over kites a..b-1 (cyclic) - 1/2, w(a,a) = 0
+    for (int a = 0; a < n; ++a) {
+      for (int b = 0; b < n; ++b) {
+        if (a == b) {
+          W[MX * a + b] = 0.0;
+          continue;
+        }
+        double rs = 0.0;
+        if (b < a) {
+          for (int k = a; k < n; ++k) rs = rs + R[k];
+          for (int k = 0; k < b; ++k) rs = rs + R[k];
+        } else {
+          for (int k = a; k < b; ++k) rs = rs + R[k];
+        }
+        W[MX * a + b] = rs - 0.5;
+      }
+    }
+  }
+}
+
+void desc_of(const HostMesh &m, imx_mesh_desc &d) {
+  std::memset(&d, 0, sizeof(d));
+  d.nv = m.nv;
+  d.ne = m.ne;
+  d.nt = m.nt;
+  d.glevel = m.glevel;
+  d.v_nnb = m.v_nnb.data();
+  d.v_nb = m.v_nb.data();
+  d.v_ed = m.v_ed.data();
+  d.v_tr = m.v_tr.data();
+  d.hx_ttgout = m.hx_ttgout.data();
+  d.hx_tnrccw = m.hx_tnrccw.data();
+  d.v_p = m.v_p.data();
+  d.hx_areag = m.hx_areag.data();
+  d.ed_v = m.ed_v.data();
+  d.ed_sh = m.ed_sh.data();
+  d.ed_leng = m.ed_leng.data();
+  d.edhx_leng = m.edhx_leng.data();
+  d.ed_c = m.ed_c.data();
+  d.edhx_c = m.edhx_c.data();
+  d.tr_v = m.tr_v.data();
+  d.tr_ed = m.tr_ed.data();
+  d.tr_tg = m.tr_tg.data();
+  d.tr_areag = m.tr_areag.data();
+  d.tr_c = m.tr_c.data();
+  d.tr_clat = m.tr_clat.data();
+  d.tr_trhx_area = m.tr_trhx_area.empty() ? nullptr : m.tr_trhx_area.data();
+  d.hx_hxtr_area = m.hx_hxtr_area.empty() ? nullptr : m.hx_hxtr_area.data();
+  d.hx_trskw = m.hx_trskw.empty() ? nullptr : m.hx_trskw.data();
+}
+
+// ----------------------------------------------------------------- IMXBIN --
+BinWriter::BinWriter(const char *path) {
+  f = std::fopen(path, "wb");
+  if (f) std::fwrite("IMXBIN01", 1, 8, f);
+}
+BinWriter::~BinWriter() {
+  if (f) std::fclose(f);
+}
+static void put_hdr(FILE *f, const char *name, const char *dt, int64_t n) {
+  char nm[24];
+  std::memset(nm, ' ', sizeof(nm));
+  std::memcpy(nm, name, std::min<size_t>(std::strlen(name), sizeof(nm)));
+  std::fwrite(nm, 1, 24, f);
+  std::fwrite(dt, 1, 4, f);
+  std::fwrite(&n, 8, 1, f);
+}
+void BinWriter::put_i4(const char *name, const int32_t *a, int64_t n) {
+  put_hdr(f, name, "i4  ", n);
+  std::fwrite(a, 4, (size_t)n, f);
+}
+void BinWriter::put_f8(const char *name, const double *a, int64_t n) {
+  put_hdr(f, name, "f8  ", n);
+  std::fwrite(a, 8, (size_t)n, f);
+}
+void BinWriter::put(const char *name, const std::vector<int32_t> &a) {
+  put_i4(name, a.data(), (int64_t)a.size());
+}
+void BinWriter::put(const char *name, const std::vector<double> &a) {
+  put_f8(name, a.data(), (int64_t)a.size());
+}
+
+// ------------------------------------------------------- icosahedral build --
+// Base icosahedron (src/smeshpack.f90:1197-1279).
+static void icos_base(int pos, double x[12], double y[12], double z[12]) {
+  if (pos == IMX_POS_EQS) {
+    const double radius = 1.0;
+    const double sqrt5 = std::sqrt(5.0);
+    const double phi = (1.0 + sqrt5) * 0.5;
+    const double ratio = std::sqrt(10.0 + (2.0 * sqrt5)) / (4.0 * phi);
+    const double a = (radius / ratio) * 0.5;
+    const double b = (radius / ratio) / (2.0 * phi);
+    const double tab[12][3] = {{0, b, -a},  {0, -b, -a}, {a, 0, -b},  {-a, 0, -b},
+                               {b, a, 0},   {-b, a, 0},  {-a, 0, b},  {a, 0, b},
+                               {b, -a, 0},  {-b, -a, 0}, {0, b, a},   {0, -b, a}};
+    for (int i = 0; i < 12; ++i) {
+      x[i] = tab[i][0];
+      y[i] = tab[i][1];
+      z[i] = tab[i][2];
+    }
+  } else {
+    // Nodes on the poles. pi is the reference's single-rounded value and the
+    // longitude step 2/5.*pi takes 2/5 in single precision (constants.f90:31,
+    // smeshpack.f90:1261-1278).
+    const double pi = 3.1415927410125732;
+    const double dlon = (double)(2.0f / 5.0f) * pi;
+    double lat = std::atan(0.5);
+    x[0] = 0; y[0] = 0; z[0] = 1;
+    double lon = -pi;
+    for (int i = 1; i <= 5; ++i) {
+      double cl = std::cos(lat);
+      x[i] = cl * std::cos(lon);
+      y[i] = cl * std::sin(lon);
+      z[i] = std::sin(lat);
+      lon = lon + dlon;
+    }
+    x[6] = 0; y[6] = 0; z[6] = -1;
+    lat = -lat;
+    lon = -4.0 * pi / 5.0;
+    for (int i = 7; i <= 11; ++i) {
+      double cl = std::cos(lat);
+      x[i] = cl * std::cos(lon);
+      y[i] = cl * std::sin(lon);
+      z[i] = std::sin(lat);
+      lon = lon + dlon;
+    }
+  }
+}
+
+using Tri = std::array<int32_t, 3>;  // 0-based node ids, CCW seen from outside
+
+// The 20 faces of the convex hull of the 12 base nodes, oriented outward.
+static std::vector<Tri> icos_faces(const std::vector<double> &p) {
+  std::vector<Tri> f;
+  for (int i = 0; i < 12; ++i)
+    for (int j = i + 1; j < 12; ++j)
+      for (int k = j + 1; k < 12; ++k) {
+        double e1[3], e2[3], n[3];
+        sub3(&p[3 * j], &p[3 * i], e1);
+        sub3(&p[3 * k], &p[3 * i], e2);
+        cross3(e1, e2, n);
+        int pos = 0, neg = 0;
+        for (int q = 0; q < 12; ++q) {
+          if (q == i || q == j || q == k) continue;
+          double w[3];
+          sub3(&p[3 * q], &p[3 * i], w);
+          double s = dot3(n, w);
+          if (s > 1e-12) ++pos;
+          else if (s < -1e-12) ++neg;
+        }
+        if (pos == 0 || neg == 0) {
+          // outward when n . p_i > 0
+          if (dot3(n, &p[3 * i]) > 0) f.push_back({i, j, k});
+          else f.push_back({i, k, j});
+        }
+      }
+  return f;
+}
+
+static inline uint64_t ekey(int32_t a, int32_t b) {
+  if (a > b) std::swap(a, b);
+  return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+}
+
+// CCW neighbour rings from CCW triangles, each ring started at the
+// lowest-numbered neighbour.
+static bool rings(int nv, const std::vector<Tri> &tris, std::vector<int32_t> &nnb,
+                  std::vector<int32_t> &nb) {
+  const int MX = IMX_MAXNB;
+  std::vector<int32_t> cnt(nv, 0);
+  std::vector<int32_t> succ_from((size_t)nv * MX, -1), succ_to((size_t)nv * MX, -1);
+  for (const Tri &t : tris) {
+    for (int e = 0; e < 3; ++e) {
+      int a = t[e], b = t[(e + 1) % 3], c = t[(e + 2) % 3];
+      if (cnt[a] >= MX) return false;
+      succ_from[(size_t)a * MX + cnt[a]] = b;
+      succ_to[(size_t)a * MX + cnt[a]] = c;
+      ++cnt[a];
+    }
+  }
+  nnb.assign(nv, 0);
+  nb.assign((size_t)nv * MX, 0);
+  for (int a = 0; a < nv; ++a) {
+    int n = cnt[a];
+    int start = INT32_MAX;
+    for (int s = 0; s < n; ++s) start = std::min(start, succ_from[(size_t)a * MX + s]);
+    int cur = start;
+    for (int s = 0; s < n; ++s) {
+      nb[(size_t)a * MX + s] = cur;
+      int nxt = -1;
+      for (int q = 0; q < n; ++q)
+        if (succ_from[(size_t)a * MX + q] == cur) nxt = succ_to[(size_t)a * MX + q];
+      if (nxt < 0) return false;
+      cur = nxt;
+    }
+    if (cur != start) return false;
+    nnb[a] = n;
+  }
+  return true;
+}
+
+static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
+  const int MX = IMX_MAXNB;
+  std::vector<double> p(36);
+  {
+    double x[12], y[12], z[12];
+    icos_base(pos, x, y, z);
+    for (int i = 0; i < 12; ++i) {
+      p[3 * i] = x[i];
+      p[3 * i + 1] = y[i];
+      p[3 * i + 2] = z[i];
+    }
+  }
+  std::vector<Tri> tris = icos_faces(p);
+  if (tris.size() != 20) {
+    err = "icosahedron hull has " + std::to_string(tris.size()) + " faces";
+    return false;
+  }
+  int nv = 12;
+  for (int lev = 1; lev <= glevel; ++lev) {
+    // Append midpoints node by node, neighbours in ring order (higher ids only)
+    std::vector<int32_t> nnb, nb;
+    if (!rings(nv, tris, nnb, nb)) {
+      err = "bad neighbour ring at level " + std::to_string(lev);
+      return false;
+    }
+    std::unordered_map<uint64_t, int32_t> mid;
+    mid.reserve((size_t)3 * nv);
+    int nn = nv;
+    p.resize((size_t)3 * (4 * (size_t)nv - 6));
+    for (int i = 0; i < nv; ++i)
+      for (int j = 0; j < nnb[i]; ++j) {
+        int k = nb[(size_t)i * MX + j];
+        if (k <= i) continue;
+        double q[3];
+        q[0] = 0.5 * (p[3 * i] + p[3 * k]);
+        q[1] = 0.5 * (p[3 * i + 1] + p[3 * k + 1]);
+        q[2] = 0.5 * (p[3 * i + 2] + p[3 * k + 2]);
+        double r = q[0] * q[0];
+        r = r + q[1] * q[1];
+        r = r + q[2] * q[2];
+        r = std::sqrt(r);
+        p[3 * nn] = q[0] / r;
+        p[3 * nn + 1] = q[1] / r;
+        p[3 * nn + 2] = q[2] / r;
+        mid[ekey(i, k)] = nn;
+        ++nn;
+      }
+    std::vector<Tri> nt;
+    nt.reserve(tris.size() * 4);
+    for (const Tri &t : tris) {
+      int a = t[0], b = t[1], c = t[2];
+      int ab = mid[ekey(a, b)], bc = mid[ekey(b, c)], ca = mid[ekey(c, a)];
+      nt.push_back({a, ab, ca});
+      nt.push_back({ab, b, bc});
+      nt.push_back({ca, bc, c});
+      nt.push_back({ab, bc, ca});
+    }
+    tris.swap(nt);
+    nv = nn;
+  }
+
+  // ---- meshstruct-equivalent structuring (src/smeshpack.f90:370-1195) ----
+  std::vector<int32_t> nnb, nb0;
+  if (!rings(nv, tris, nnb, nb0)) {
+    err = "bad neighbour ring";
+    return false;
+  }
+  m.nv = nv;
+  m.nt = 2 * nv - 4;
+  m.ne = 3 * nv - 6;
+  m.glevel = glevel;
+  const int ne = m.ne, nt = m.nt;
+  m.v_p = p;
+  m.v_p.resize((size_t)3 * nv);
+  m.v_lat.resize(nv);
+  m.v_lon.resize(nv);
+  for (int i = 0; i < nv; ++i)
+    cart2sph(m.v_p[3 * i], m.v_p[3 * i + 1], m.v_p[3 * i + 2], &m.v_lon[i], &m.v_lat[i]);
+  m.v_nnb = nnb;
+  m.v_nb.assign((size_t)nv * MX, 0);
+  for (size_t q = 0; q < (size_t)nv * MX; ++q) m.v_nb[q] = (q % MX) < (size_t)nnb[q / MX] ? nb0[q] + 1 : 0;
+
+  // Edges: edge l runs v1 -> v2 with v1 < v2, numbered in (node, ring) order.
+  m.v_ed.assign((size_t)nv * MX, 0);
+  m.ed_v.assign((size_t)2 * ne, 0);
+  m.ed_sh.assign((size_t)2 * ne, 0);
+  m.ed_c.assign((size_t)3 * ne, 0);
+  m.ed_nr.assign((size_t)3 * ne, 0);
+  m.ed_tg.assign((size_t)3 * ne, 0);
+  m.ed_leng.assign(ne, 0);
+  m.ed_clat.assign(ne, 0);
+  m.ed_clon.assign(ne, 0);
+  std::unordered_map<uint64_t, int32_t> edge_of;
+  edge_of.reserve((size_t)ne * 2);
+  int l = 0;
+  for (int i = 0; i < nv; ++i)
+    for (int j = 0; j < nnb[i]; ++j) {
+      int k = nb0[(size_t)i * MX + j];
+      if (k > i) {
+        m.v_ed[(size_t)i * MX + j] = l + 1;
+        m.ed_v[2 * l] = i + 1;
+        m.ed_v[2 * l + 1] = k + 1;
+        const double *pi = &m.v_p[3 * i], *pk = &m.v_p[3 * k];
+        double c[3] = {(pi[0] + pk[0]) / 2.0, (pi[1] + pk[1]) / 2.0, (pi[2] + pk[2]) / 2.0};
+        normalize3(c);
+        std::memcpy(&m.ed_c[3 * l], c, sizeof(c));
+        cart2sph(c[0], c[1], c[2], &m.ed_clon[l], &m.ed_clat[l]);
+        double nr[3], tg[3];
+        cross3(pi, pk, nr);
+        normalize3(nr);
+        cross3(nr, c, tg);
+        normalize3(tg);
+        std::memcpy(&m.ed_nr[3 * l], nr, sizeof(nr));
+        std::memcpy(&m.ed_tg[3 * l], tg, sizeof(tg));
+        m.ed_leng[l] = arclen(pi, pk);
+        edge_of[ekey(i, k)] = l;
+        ++l;
+      }
+    }
+  if (l != ne) {
+    err = "edge count mismatch";
+    return false;
+  }
+  for (int i = 0; i < nv; ++i)
+    for (int j = 0; j < nnb[i]; ++j) {
+      int k = nb0[(size_t)i * MX + j];
+      if (k < i) m.v_ed[(size_t)i * MX + j] = edge_of[ekey(i, k)] + 1;
+    }
+
+  // Triangles: (i, nb(j), nb(j+1)) is created by its lowest node i.
+  m.v_tr.assign((size_t)nv * MX, 0);
+  m.tr_v.assign((size_t)3 * nt, 0);
+  std::map<std::array<int32_t, 3>, int32_t> tri_of;
+  int kt = 0;
+  for (int i = 0; i < nv; ++i) {
+    int n = nnb[i];
+    for (int j = 0; j < n; ++j) {
+      int b = nb0[(size_t)i * MX + j], c = nb0[(size_t)i * MX + (j + 1) % n];
+      std::array<int32_t, 3> key = {i, b, c};
+      std::sort(key.begin(), key.end());
+      if (std::min(b, c) > i) {
+        m.v_tr[(size_t)i * MX + j] = kt + 1;
+        m.tr_v[3 * kt] = i + 1;
+        m.tr_v[3 * kt + 1] = b + 1;
+        m.tr_v[3 * kt + 2] = c + 1;
+        tri_of[key] = kt;
+        ++kt;
+      } else {
+        auto it = tri_of.find(key);
+        if (it == tri_of.end()) {
+          err = "triangle lookup failed";
+          return false;
+        }
+        m.v_tr[(size_t)i * MX + j] = it->second + 1;
+      }
+    }
+  }
+  if (kt != nt) {
+    err = "triangle count mismatch";
+    return false;
+  }
+  m.tr_ed.assign((size_t)3 * nt, 0);
+  m.tr_tg.assign((size_t)3 * nt, 0);
+  m.tr_nr.assign((size_t)3 * nt, 0);
+  for (int k = 0; k < nt; ++k)
+    for (int e = 0; e < 3; ++e) {
+      int a = m.tr_v[3 * k + e] - 1, b = m.tr_v[3 * k + (e + 1) % 3] - 1;
+      int ed = edge_of[ekey(a, b)];
+      m.tr_ed[3 * k + e] = ed + 1;
+      if (m.ed_sh[2 * ed] == 0) m.ed_sh[2 * ed] = k + 1;
+      else m.ed_sh[2 * ed + 1] = k + 1;
+      double nr[3];
+      cross3(&m.v_p[3 * a], &m.v_p[3 * b], nr);
+      nr[0] = -nr[0];
+      nr[1] = -nr[1];
+      nr[2] = -nr[2];
+      normalize3(nr);
+      double dp = dot3(nr, &m.ed_nr[3 * ed]);
+      if (std::fabs(std::fabs(dp) - 1.0) > kEpsSingle * 100) {
+        err = "triangle and edge normals not colinear";
+        return false;
+      }
+      m.tr_nr[3 * k + e] = dp > 0 ? 1 : -1;
+      m.tr_tg[3 * k + e] = dp > 0 ? -1 : 1;
+    }
+
+  // Circumcentres and areas (trcircumc :4659, sphtriarea :4710).
+  m.tr_c.assign((size_t)3 * nt, 0);
+  m.tr_clat.assign(nt, 0);
+  m.tr_clon.assign(nt, 0);
+  m.tr_areag.assign(nt, 0);
+  for (int k = 0; k < nt; ++k) {
+    const double *p1 = &m.v_p[3 * (m.tr_v[3 * k] - 1)];
+    const double *p2 = &m.v_p[3 * (m.tr_v[3 * k + 1] - 1)];
+    const double *p3 = &m.v_p[3 * (m.tr_v[3 * k + 2] - 1)];
+    double e1[3], e2[3], c[3];
+    sub3(p2, p1, e1);
+    sub3(p3, p1, e2);
+    cross3(e1, e2, c);
+    normalize3(c);
+    std::memcpy(&m.tr_c[3 * k], c, sizeof(c));
+    cart2sph(c[0], c[1], c[2], &m.tr_clon[k], &m.tr_clat[k]);
+    m.tr_areag[k] = sphtriarea(p1, p2, p3);
+  }
+
+  // Voronoi edges (src/smeshpack.f90:1016-1049).
+  m.edhx_c.assign((size_t)3 * ne, 0);
+  m.edhx_nr.assign((size_t)3 * ne, 0);
+  m.edhx_tg.assign((size_t)3 * ne, 0);
+  m.edhx_leng.assign(ne, 0);
+  for (int e = 0; e < ne; ++e) {
+    const double *c1 = &m.tr_c[3 * (m.ed_sh[2 * e] - 1)];
+    const double *c2 = &m.tr_c[3 * (m.ed_sh[2 * e + 1] - 1)];
+    double c[3] = {(c1[0] + c2[0]) / 2.0, (c1[1] + c2[1]) / 2.0, (c1[2] + c2[2]) / 2.0};
+    normalize3(c);
+    std::memcpy(&m.edhx_c[3 * e], c, sizeof(c));
+    double nr[3], tg[3];
+    cross3(c1, c2, nr);
+    normalize3(nr);
+    cross3(nr, c, tg);
+    normalize3(tg);
+    std::memcpy(&m.edhx_nr[3 * e], nr, sizeof(nr));
+    std::memcpy(&m.edhx_tg[3 * e], tg, sizeof(tg));
+    m.edhx_leng[e] = arclen(c1, c2);
+  }
+
+  // Voronoi cells: sign tables and areas (src/smeshpack.f90:1071-1151, sphhxarea :4765).
+  m.hx_tg.assign((size_t)nv * MX, 0);
+  m.hx_nr.assign((size_t)nv * MX, 0);
+  m.hx_ttgout.assign((size_t)nv * MX, 0);
+  m.hx_tnrccw.assign((size_t)nv * MX, 0);
+  m.hx_areag.assign(nv, 0);
+  for (int i = 0; i < nv; ++i) {
+    int n = nnb[i];
+    double area = 0.0;
+    for (int j = 0; j < n; ++j) {
+      int j1 = m.v_tr[(size_t)i * MX + (j + n - 1) % n] - 1;
+      int j2 = m.v_tr[(size_t)i * MX + j] - 1;
+      int ed = m.v_ed[(size_t)i * MX + j] - 1;
+      double nr[3];
+      cross3(&m.tr_c[3 * j1], &m.tr_c[3 * j2], nr);
+      nr[0] = -nr[0];
+      nr[1] = -nr[1];
+      nr[2] = -nr[2];
+      normalize3(nr);
+      double dp = dot3(nr, &m.edhx_nr[3 * ed]);
+      if (std::fabs(std::fabs(dp) - 1.0) > kEpsSingle * 100) {
+        err = "hexagon edge normal not colinear";
+        return false;
+      }
+      int hnr = dp > 0 ? 1 : -1, htg = dp > 0 ? -1 : 1;
+      m.hx_nr[(size_t)i * MX + j] = hnr;
+      m.hx_tg[(size_t)i * MX + j] = htg;
+      double w[3];
+      for (int q = 0; q < 3; ++q) w[q] = m.edhx_nr[3 * ed + q] * (double)hnr;
+      m.hx_ttgout[(size_t)i * MX + j] = (int)fsign1(dot3(&m.ed_tg[3 * ed], w));
+      for (int q = 0; q < 3; ++q) w[q] = m.edhx_tg[3 * ed + q] * (double)htg;
+      m.hx_tnrccw[(size_t)i * MX + j] = (int)fsign1(dot3(&m.ed_nr[3 * ed], w));
+      area = area + sphtriarea(&m.v_p[3 * i], &m.tr_c[3 * j1], &m.tr_c[3 * j2]);
+    }
+    m.hx_areag[i] = area;
+  }
+  return true;
+}
+
+}  // namespace imx
+
+// ------------------------------------------------------------------ C ABI --
+extern "C" {
+
+const char *imx_last_error(void) { return imx::g_last_error.c_str(); }
+
+int imx_mesh_build_icos(int32_t glevel, int32_t pos, imx_mesh **out) {
+  if (!out) return imx::set_error(IMX_EINVAL, "imx_mesh_build_icos: out is NULL");
+  *out = nullptr;
+  if (glevel < 0 || glevel > 11)
+    return imx::set_error(IMX_EINVAL, "imx_mesh_build_icos: glevel must be in 0..11");
+  if (pos != IMX_POS_EQS && pos != IMX_POS_POL)
+    return imx::set_error(IMX_EINVAL, "imx_mesh_build_icos: pos must be IMX_POS_EQS or IMX_POS_POL");
+  imx_mesh *m = new (std::nothrow) imx_mesh();
+  if (!m) return imx::set_error(IMX_ENOMEM, "imx_mesh_build_icos: out of memory");
+  std::string err;
+  try {
+    if (!imx::build_icos(glevel, pos, *m, err)) {
+      delete m;
+      return imx::set_error(IMX_EINVAL, "imx_mesh_build_icos: " + err);
+    }
+  } catch (const std::bad_alloc &) {
+    delete m;
+    return imx::set_error(IMX_ENOMEM, "imx_mesh_build_icos: out of memory");
+  }
+  *out = m;
+  return IMX_OK;
+}
+
+int imx_mesh_desc_of(const imx_mesh *m, imx_mesh_desc *out) {
+  if (!m || !out) return imx::set_error(IMX_EINVAL, "imx_mesh_desc_of: NULL argument");
+  imx::desc_of(*m, *out);
+  return IMX_OK;
+}
+
+int imx_mesh_gridprop(imx_mesh *m) {
+  if (!m) return imx::set_error(IMX_EINVAL, "imx_mesh_gridprop: NULL mesh");
+  m->tr_trhx_area.assign((size_t)3 * m->nt, 0.0);
+  m->hx_hxtr_area.assign((size_t)IMX_MAXNB * m->nv, 0.0);
+  m->hx_trskw.assign((size_t)IMX_MAXNB * IMX_MAXNB * m->nv, 0.0);
+  imx_mesh_desc d;
+  imx::desc_of(*m, d);
+  imx::gridprop(d, m->tr_trhx_area.data(), m->hx_hxtr_area.data(), m->hx_trskw.data());
+  return IMX_OK;
+}
+
+int imx_gridprop(const imx_mesh_desc *d, double *trhx, double *hxtr, double *trskw) {
+  if (!d || !trhx || !hxtr || !trskw) return imx::set_error(IMX_EINVAL, "imx_gridprop: NULL argument");
+  imx::gridprop(*d, trhx, hxtr, trskw);
+  return IMX_OK;
+}
+
+int imx_mesh_write(const imx_mesh *m, const char *path) {
+  if (!m || !path) return imx::set_error(IMX_EINVAL, "imx_mesh_write: NULL argument");
+  imx::BinWriter w(path);
+  if (!w.ok()) return imx::set_error(IMX_EIO, std::string("imx_mesh_write: cannot open ") + path);
+  std::vector<int32_t> dims = {m->nv, m->ne, m->nt, m->glevel};
+  w.put("dims", dims);
+  w.put("v_p", m->v_p);
+  w.put("v_lat", m->v_lat);
+  w.put("v_lon", m->v_lon);
+  w.put("v_nnb", m->v_nnb);
+  w.put("v_nb", m->v_nb);
+  w.put("v_ed", m->v_ed);
+  w.put("v_tr", m->v_tr);
+  w.put("hx_ttgout", m->hx_ttgout);
+  w.put("hx_tnrccw", m->hx_tnrccw);
+  w.put("hx_tg", m->hx_tg);
+  w.put("hx_nr", m->hx_nr);
+  w.put("hx_areag", m->hx_areag);
+  w.put("ed_v", m->ed_v);
+  w.put("ed_sh", m->ed_sh);
+  w.put("ed_leng", m->ed_leng);
+  w.put("edhx_leng", m->edhx_leng);
+  w.put("ed_c", m->ed_c);
+  w.put("ed_clat", m->ed_clat);
+  w.put("ed_clon", m->ed_clon);
+  w.put("ed_tg", m->ed_tg);
+  w.put("ed_nr", m->ed_nr);
+  w.put("edhx_c", m->edhx_c);
+  w.put("edhx_nr", m->edhx_nr);
+  w.put("edhx_tg", m->edhx_tg);
+  w.put("tr_v", m->tr_v);
+  w.put("tr_ed", m->tr_ed);
+  w.put("tr_tg", m->tr_tg);
+  w.put("tr_nr", m->tr_nr);
+  w.put("tr_areag", m->tr_areag);
+  w.put("tr_c", m->tr_c);
+  w.put("tr_clat", m->tr_clat);
+  w.put("tr_clon", m->tr_clon);
+  if (!m->tr_trhx_area.empty()) {
+    w.put("tr_trhx_area", m->tr_trhx_area);
+    w.put("hx_hxtr_area", m->hx_hxtr_area);
+    w.put("hx_trskw", m->hx_trskw);
+  }
+  return IMX_OK;
+}
+
+void imx_mesh_destroy(imx_mesh *m) { delete m; }
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+/*
+ * imx_swm.h -- C ABI of libimx, the MI355X-native shallow-water time step
+ * (TRiSK/HTC C-grid on icosahedral Voronoi/Delaunay meshes, classical RK4).
+ *
+ * Drop-in for the reference Fortran path
+ *     swm_tests -> ode_rk4 -> tendency -> swm_operators
+ * (src/swm.f90:102-380, :2416-2683, src/swm_operators.f90). The reference has
+ * no handle and no return codes: all state is module-global in swm_data
+ * (src/swm_data.f90:31-334) and every error is `print*; stop`. Here every
+ * call returns 0 on success or a negative IMX_E* code, and imx_last_error()
+ * gives the message; the Fortran wrapper turns a non-zero code back into
+ * `print*, msg; stop`, which reproduces the reference's convention.
+ *
+ * Numbering: every index handed across this boundary is in the CALLER's
+ * numbering and 1-based, exactly as in the reference grid_structure
+ * (src/datastruct.f90:403-519). Host arrays are caller-owned and never
+ * retained past the call. One host thread per handle; calls are ordered on
+ * the handle's HIP stream; *_get_*, *_diagnostics and *_tendency block.
+ */
+#ifndef IMX_SWM_H
+#define IMX_SWM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMX_OK        0
+#define IMX_EINVAL   (-1)  /* bad argument / inconsistent mesh                */
+#define IMX_ENOMEM   (-2)  /* host or device allocation failed                */
+#define IMX_EDEVICE  (-3)  /* HIP runtime error                               */
+#define IMX_ESTATE   (-4)  /* numerical guard tripped (h_tr ~ 0, NaN state)   */
+#define IMX_EIO      (-5)  /* file I/O                                        */
+#define IMX_ENOTSUP  (-6)  /* option not supported by this build              */
+
+#define IMX_MAXNB 6        /* cells of icosahedral grids have 5 or 6 edges    */
+
+/* Arithmetic of the step. */
+#define IMX_PREC_FP64_REF  0  /* FP64, the reference's operation order, one
+                                 kernel per reference operator; bit-for-bit
+                                 with the reference built -ffp-contract=off   */
+#define IMX_PREC_FP64      1  /* FP64, fused cell/edge/triangle passes        */
+#define IMX_PREC_FP32      2  /* FP32 stencil; FP64 geometry, compensated FP64
+                                 mass-flux divergence, FP64 reductions        */
+
+/* Mesh positions of the icosahedron (src/smeshpack.f90:1234-1279). */
+#define IMX_POS_EQS 0
+#define IMX_POS_POL 1
+
+/*
+ * Flat view of the reference grid_structure: the arrays the trsk10/HTC
+ * tendency, initialize_gridprop and the diagnostics read. Per-cell lists are
+ * padded to IMX_MAXNB with 0. Vectors are (x,y,z) per entity, unit sphere.
+ * Replaces passing `type(grid_structure)` (src/swm.f90:102-107, :152).
+ */
+typedef struct imx_mesh_desc {
+  int32_t nv, ne, nt;          /* cells (Voronoi nodes), edges, triangles      */
+  int32_t glevel;              /* icosahedral level, -1 if not icosahedral     */
+  /* cells: v(i)%..., hx(i)%...                                               */
+  const int32_t *v_nnb;        /* [nv]          v%nnb                          */
+  const int32_t *v_nb;         /* [nv*6]        v%nb   (neighbour cells, CCW)  */
+  const int32_t *v_ed;         /* [nv*6]        v%ed   (edges, CCW)            */
+  const int32_t *v_tr;         /* [nv*6]        v%tr   (triangles, CCW)        */
+  const int32_t *hx_ttgout;    /* [nv*6]  +-1   hx%ttgout                      */
+  const int32_t *hx_tnrccw;    /* [nv*6]  +-1   hx%tnrccw                      */
+  const double  *v_p;          /* [nv*3]        v%p                            */
+  const double  *hx_areag;     /* [nv]          hx%areag (unit sphere)         */
+  /* edges: ed(l)%..., edhx(l)%...  (edhx%sh == ed%v, edhx%v == ed%sh)        */
+  const int32_t *ed_v;         /* [ne*2]        ed%v   (cells, v1 -> v2)       */
+  const int32_t *ed_sh;        /* [ne*2]        ed%sh  (triangles)             */
+  const double  *ed_leng;      /* [ne]          ed%leng   (primal, geodesic)   */
+  const double  *edhx_leng;    /* [ne]          edhx%leng (dual, geodesic)     */
+  const double  *ed_c;         /* [ne*3]        ed%c%p    (primal midpoint)    */
+  const double  *edhx_c;       /* [ne*3]        edhx%c%p  (dual midpoint)      */
+  /* triangles: tr(k)%...                                                     */
+  const int32_t *tr_v;         /* [nt*3]        tr%v                           */
+  const int32_t *tr_ed;        /* [nt*3]        tr%ed                          */
+  const int32_t *tr_tg;        /* [nt*3]  +-1   tr%tg                          */
+  const double  *tr_areag;     /* [nt]          tr%areag                       */
+  const double  *tr_c;         /* [nt*3]        tr%c%p (circumcentre)          */
+  const double  *tr_clat;      /* [nt]          tr%c%lat                       */
+  /* Optional initialize_gridprop products (src/swm.f90:979-1041). NULL: the
+     library computes them on the host in FP64 with the reference formulas. */
+  const double  *tr_trhx_area; /* [nt*3]        tr%trhx_area                   */
+  const double  *hx_hxtr_area; /* [nv*6]        hx%hxtr_area (/areag)          */
+  const double  *hx_trskw;     /* [nv*6*6]      hx%trskw(i,j) at [c][i][j]     */
+} imx_mesh_desc;
+
+typedef struct imx_swm_opts {
+  int32_t precision;   /* IMX_PREC_*                                          */
+  int32_t testcase;    /* reference semantics: <2 -> q=0 (swm.f90:2464),
+                          <=1 -> momeq=0 (swm.f90:2592)                       */
+  int32_t use_sfc;     /* reorder cells/edges/triangles along a Hilbert curve */
+  int32_t use_graph;   /* capture the RK4 step in a hipGraph                  */
+  double  erad;        /* sphere radius [m]        (constants.f90:55)         */
+  double  grav;        /* gravity [m s^-2]         (constants.f90:61)         */
+  double  omega;       /* rotation rate [s^-1]     (constants.f90:67)         */
+  double  dt;          /* time step [s]                                       */
+  double  h_ref;       /* FP32 mode stores h - h_ref; 0 -> area mean of the
+                          first state set                                     */
+} imx_swm_opts;
+
+typedef struct imx_mesh imx_mesh;  /* host mesh built by the library */
+typedef struct imx_swm  imx_swm;   /* one SWM instance; owns device memory */
+
+/* Diagnostics vector written by imx_swm_diagnostics (all FP64). */
+#define IMX_DIAG_MASS      0  /* sumf_areas(h)               swm.f90:3394      */
+#define IMX_DIAG_PENERGY   1  /* sum A g h (h/2 + b)         swm.f90:2704      */
+#define IMX_DIAG_KENERGY   2  /* sum 1/2 l d h_e u^2         swm.f90:2739      */
+#define IMX_DIAG_TENERGY   3  /* P + K                       swm.f90:2745      */
+#define IMX_DIAG_AENERGY   4  /* T - unavailable P           swm.f90:2747      */
+#define IMX_DIAG_PENSTROPHY 5 /* sum_t 1/2 A_t a^2 h_t q_t^2 (not in reference) */
+#define IMX_DIAG_RMSDIV    6  /* sqrt(sumfsq_areas(div u))   swm.f90:352       */
+#define IMX_DIAG_BADFLAG   7  /* 1 if h or u holds NaN/Inf, else 0             */
+#define IMX_NDIAG          8
+
+/* Intermediate fields of the last imx_swm_tendency call (names of the
+   reference module globals in swm_data, written by tendency). */
+#define IMX_FIELD_H_ED      0  /* [ne] */
+#define IMX_FIELD_UH        1  /* [ne] */
+#define IMX_FIELD_ETA       2  /* [nt] */
+#define IMX_FIELD_H_TR      3  /* [nt] */
+#define IMX_FIELD_Q_TR      4  /* [nt] */
+#define IMX_FIELD_Q_ED      5  /* [ne] */
+#define IMX_FIELD_DIVUH     6  /* [nv] */
+#define IMX_FIELD_KE_HX     7  /* [nv] */
+#define IMX_FIELD_GHBK      8  /* [nv] */
+#define IMX_FIELD_GRAD_GHBK 9  /* [ne] */
+#define IMX_FIELD_UHQ_PERP 10  /* [ne] */
+#define IMX_FIELD_DIVU     11  /* [nv] */
+#define IMX_FIELD_ZETA     12  /* [nt] */
+
+/* ---- errors -------------------------------------------------------------- */
+const char *imx_last_error(void);
+
+/* ---- mesh (replaces meshbuild for icos/nopt grids, src/smeshpack.f90:77-368,
+        and initialize_gridprop, src/swm.f90:979-1041) ------------------------ */
+int  imx_mesh_build_icos(int32_t glevel, int32_t pos, imx_mesh **out);
+int  imx_mesh_desc_of(const imx_mesh *m, imx_mesh_desc *out);
+int  imx_mesh_gridprop(imx_mesh *m);             /* fills the optional arrays */
+int  imx_mesh_write(const imx_mesh *m, const char *path);  /* IMXBIN records  */
+void imx_mesh_destroy(imx_mesh *m);
+/* initialize_gridprop on caller arrays: trhx_intersec_areas
+   (src/smeshpack.f90:5080-5154), hxtr_intersec_areas (:4991-5078) and
+   calc_trisk_weights (src/interpack.f90:3978-4098). */
+int  imx_gridprop(const imx_mesh_desc *d, double *tr_trhx_area,
+                  double *hx_hxtr_area, double *hx_trskw);
+
+/* ---- initial states -------------------------------------------------------- */
+/* Williamson TC2 as initialize_fields case(2) with HTC (src/swm.f90:1205-1247)
+   on a library-built mesh (needs ed%c%lat, ed%tg and v%lat). */
+int  imx_ic_tc2(const imx_mesh *m, double erad, double *h, double *u);
+/* Seeded spectral state: real spherical harmonics l=1..lmax, coefficients
+   N(0,1)*l^-2 from SplitMix64 + Box-Muller; h = h0 + h', h' area-rms =
+   hfrac*h0; u = -(psi(t2)-psi(t1))/(l_e a) * sign, scaled to area-rms
+   speed urms. Not in the reference (new test case). */
+int  imx_ic_spectral(const imx_mesh_desc *d, uint64_t seed, int32_t lmax,
+                     double h0, double hfrac, double urms, double erad,
+                     double *h, double *u);
+
+/* ---- the time step (module swm) ------------------------------------------- */
+void imx_swm_default_opts(imx_swm_opts *o);
+int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out);
+int  imx_swm_set_state(imx_swm *s, const double *h, const double *u);
+int  imx_swm_get_state(imx_swm *s, double *h, double *u);
+/* subroutine tendency(h, u, masseq, momeq)       src/swm.f90:2416-2598 */
+int  imx_swm_tendency(imx_swm *s, const double *h, const double *u,
+                      double *masseq, double *momeq);
+int  imx_swm_get_field(imx_swm *s, int32_t which, double *out);
+/* nsteps x ode_rk4 on the resident state            src/swm.f90:2600-2683 */
+int  imx_swm_step(imx_swm *s, int32_t nsteps);
+/* as imx_swm_step, returns device time (HIP events on the handle's stream)  */
+int  imx_swm_step_timed(imx_swm *s, int32_t nsteps, double *ms);
+/* per-kernel average device time [ms] over nsteps, one entry per kernel of
+   the step (names via imx_swm_kernel_name); returns the count in *nk        */
+int  imx_swm_profile(imx_swm *s, int32_t nsteps, double *ms, int32_t *nk);
+const char *imx_swm_kernel_name(imx_swm *s, int32_t k);
+int  imx_swm_diagnostics(imx_swm *s, double out[IMX_NDIAG]);
+int  imx_swm_info(imx_swm *s, int64_t out[8]);   /* nv ne nt bytes_dev ... */
+void imx_swm_destroy(imx_swm *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMX_SWM_H */

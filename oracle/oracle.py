@@ -1,0 +1,143 @@
+"""TEST INFRASTRUCTURE ONLY: Python access to the C oracle (swm_oracle.c).
+
+Imported by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+bench.py, and nowhere else; it is the checker, never the thing measured or
+shipped.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from imodel_amd import capi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+REF_HARNESS = os.path.join(_HERE, "_ref", "ref_harness")
+
+ERAD_REF = 6.37122e6      # constants.f90:55
+GRAV = 9.80616            # constants.f90:61
+OMEGA = 7.292e-5          # constants.f90:67
+
+_F = ("h_ed", "uh", "eta", "h_tr", "q_tr", "q_ed", "eta_ed", "divuh", "ke_hx", "ghbk",
+      "hbt", "grad_ghbk", "uhq_perp", "divu", "zeta", "v_hx", "massf", "momf", "h_new",
+      "u_new")
+_F64P = ctypes.POINTER(ctypes.c_double)
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("erad", ctypes.c_double), ("grav", ctypes.c_double),
+                ("omega", ctypes.c_double), ("testcase", ctypes.c_int),
+                ("trhx_area", _F64P), ("trskw", _F64P)]
+
+
+class _Fields(ctypes.Structure):
+    _fields_ = [(n, _F64P) for n in _F]
+
+
+def build():
+    """Compile the C oracle (gcc, no FMA contraction, OpenMP)."""
+    src = os.path.join(_HERE, "swm_oracle.c")
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
+                           "-std=c11", src, "-o", LIB, "-lm"])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"oracle not built: {LIB} missing")
+        L = ctypes.CDLL(LIB)
+        L.oracle_fields_alloc.argtypes = [ctypes.POINTER(capi.MeshDesc), ctypes.POINTER(_Fields), ctypes.c_int]
+        L.oracle_fields_free.argtypes = [ctypes.POINTER(_Fields)]
+        L.oracle_tendency.argtypes = [ctypes.POINTER(capi.MeshDesc), ctypes.POINTER(_Params), _F64P,
+                                      _F64P, _F64P, _F64P, _F64P, ctypes.POINTER(_Fields)]
+        L.oracle_rk4.argtypes = [ctypes.POINTER(capi.MeshDesc), ctypes.POINTER(_Params), ctypes.c_double,
+                                 _F64P, _F64P, _F64P, ctypes.POINTER(_Fields)]
+        L.oracle_diagnostics.argtypes = [ctypes.POINTER(capi.MeshDesc), ctypes.POINTER(_Params), _F64P,
+                                         _F64P, _F64P, ctypes.POINTER(_Fields), _F64P]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """FP64 reference-order CPU model on one mesh (dict of IMXBIN arrays).
+
+    The mesh dict must hold the gridprop arrays (tr_trhx_area, hx_trskw).
+    """
+
+    def __init__(self, mesh, erad=ERAD_REF, grav=GRAV, omega=OMEGA, testcase=2,
+                 perot=False):
+        self.mesh = mesh
+        self.desc, self._keep = capi.make_desc(mesh)
+        self.nv, self.ne, self.nt = self.desc.nv, self.desc.ne, self.desc.nt
+        self._trhx = np.ascontiguousarray(mesh["tr_trhx_area"], dtype=np.float64)
+        self._trskw = np.ascontiguousarray(mesh["hx_trskw"], dtype=np.float64)
+        self.p = _Params(erad=erad, grav=grav, omega=omega, testcase=testcase,
+                         trhx_area=capi.f64p(self._trhx), trskw=capi.f64p(self._trskw))
+        self.f = _Fields()
+        if lib().oracle_fields_alloc(ctypes.byref(self.desc), ctypes.byref(self.f), int(perot)) != 0:
+            raise MemoryError("oracle_fields_alloc")
+        self.bt = np.zeros(self.nv)
+
+    def __del__(self):
+        try:
+            lib().oracle_fields_free(ctypes.byref(self.f))
+        except Exception:
+            pass
+
+    def field(self, name):
+        n = {"v": self.nv, "e": self.ne, "t": self.nt}[capi.FIELD_POS.get(name, "v")]
+        if name in ("eta_ed",):
+            n = self.ne
+        if name == "hbt":
+            n = self.nv
+        return np.ctypeslib.as_array(getattr(self.f, name), shape=(n,)).copy()
+
+    def tendency(self, h, u):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        m = np.zeros(self.nv)
+        f = np.zeros(self.ne)
+        rc = lib().oracle_tendency(ctypes.byref(self.desc), ctypes.byref(self.p), capi.f64p(h),
+                                   capi.f64p(u), capi.f64p(self.bt), capi.f64p(m), capi.f64p(f),
+                                   ctypes.byref(self.f))
+        if rc:
+            raise FloatingPointError(f"oracle_tendency rc={rc}")
+        return m, f
+
+    def step(self, h, u, dt, nsteps=1):
+        """nsteps of ode_rk4 on copies of (h, u); returns the new (h, u)."""
+        h = np.array(h, dtype=np.float64)
+        u = np.array(u, dtype=np.float64)
+        for _ in range(nsteps):
+            rc = lib().oracle_rk4(ctypes.byref(self.desc), ctypes.byref(self.p), dt, capi.f64p(h),
+                                  capi.f64p(u), capi.f64p(self.bt), ctypes.byref(self.f))
+            if rc:
+                raise FloatingPointError(f"oracle_rk4 rc={rc}")
+        return h, u
+
+    def diagnostics(self, h, u, refresh=True):
+        """Diagnostics vector (mass, P, K, T, A, pot. enstrophy, RMSdiv, bad).
+
+        refresh=True evaluates the tendency on (h, u) first, so the edge and
+        triangle fields belong to the given state; refresh=False reproduces the
+        reference time loop, which reads the fields of its last RK stage.
+        """
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if refresh:
+            self.tendency(h, u)
+        out = np.zeros(8)
+        lib().oracle_diagnostics(ctypes.byref(self.desc), ctypes.byref(self.p), capi.f64p(h),
+                                 capi.f64p(u), capi.f64p(self.bt), ctypes.byref(self.f),
+                                 capi.f64p(out))
+        return out
+
+
+def ref_harness_available():
+    return os.access(REF_HARNESS, os.X_OK)
