@@ -38,6 +38,9 @@ double det3(const double *p1, const double *p2, const double *p3);
 void cart2sph(double x, double y, double z, double *lon, double *lat);
 
 void gridprop(const imx_mesh_desc &d, double *trhx, double *hxtr, double *trskw);
+// Caller (0-based) -> internal index maps along a Hilbert curve (or identity).
+void sfc_order(const imx_mesh_desc &d, bool use_sfc, std::vector<int32_t> &cell_new,
+               std::vector<int32_t> &edge_new, std::vector<int32_t> &tri_new);
 void desc_of(const HostMesh &m, imx_mesh_desc &d);
 
 // IMXBIN container: "IMXBIN01", then records (name[24], dtype[4], int64 n, data).

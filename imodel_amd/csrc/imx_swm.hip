@@ -1,0 +1,1275 @@
+// libimx device side: the TRiSK/HTC shallow-water step on MI355X (gfx950).
+//
+// Two kernel families share one handle:
+//  * REF  (IMX_PREC_FP64_REF): one kernel per reference operator, FP64, the
+//    reference's expression order and no FMA contraction (the library is
+//    built with -ffp-contract=off). Bit-for-bit with the reference Fortran
+//    built the same way. Used for parity and for FP64 callers.
+//  * FAST (IMX_PREC_FP32 / IMX_PREC_FP64): three fused passes per RK stage,
+//      triangle pass  q_t = (zeta_t + f_t) / h_t
+//      cell pass      mass-flux divergence (FP64, compensated), kinetic
+//                     energy, Bernoulli potential, and the cell's TRiSK
+//                     partial sums for each of its edges (a 6x6 local
+//                     product using weights rebuilt from the 6 kite areas)
+//      edge pass      grad of the Bernoulli potential + the two Coriolis
+//                     partials, RK update
+//    with the RK4 state resident in HBM and one step captured in a hipGraph.
+// Tables are structure-of-arrays in an internal numbering along a Hilbert
+// curve; blocks are remapped so each XCD walks a contiguous stretch of it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "imx_internal.h"
+
+#define TPB 256
+#define NXCD 8
+#define MX IMX_MAXNB
+
+// --------------------------------------------------------------- helpers ---
+static inline int grid_for(int n) {
+  int nb = (n + TPB - 1) / TPB;
+  if (nb < 1) nb = 1;
+  return ((nb + NXCD - 1) / NXCD) * NXCD;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs. Give XCD x the
+// contiguous logical range [x*per, (x+1)*per) so that SFC-neighbouring work
+// shares one L2. The grid is always a multiple of NXCD blocks.
+__device__ __forceinline__ int xcd_index() {
+  const int per = gridDim.x / NXCD;
+  const int lb = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+  return lb * TPB + threadIdx.x;
+}
+
+// epsilon(1.)/1e5 of the reference zero guard (swm_operators.f90:1603)
+#define HTR_GUARD (1.1920928955078125e-07 / 100000.0)
+
+// ============================================================ REF kernels ==
+// scalar_hx2ed + scalar_elem_product (swm_operators.f90:88-98, :1567-1573)
+__global__ __launch_bounds__(TPB) void ref_edge_flux(int ne, const int32_t *__restrict__ e_v,
+                                                     const double *__restrict__ h,
+                                                     const double *__restrict__ u,
+                                                     double *__restrict__ h_ed,
+                                                     double *__restrict__ uh) {
+  const int l = xcd_index();
+  if (l >= ne) return;
+  const double he = (h[e_v[l]] + h[e_v[ne + l]]) * 0.5;
+  h_ed[l] = he;
+  uh[l] = u[l] * he;
+}
+
+// vort_tr + rel_vort_tr + scalar_hx2tr + scalar_elem_divide
+// (swm_operators.f90:551-611, :780-827, :258-263, :1579-1615)
+__global__ __launch_bounds__(TPB) void ref_tri(int nt, const int32_t *__restrict__ t_ed,
+                                               const int8_t *__restrict__ t_tg,
+                                               const int32_t *__restrict__ t_v,
+                                               const double *__restrict__ t_area,
+                                               const double *__restrict__ t_f,
+                                               const double *__restrict__ t_trhx,
+                                               const double *__restrict__ e_leng,
+                                               const double *__restrict__ u,
+                                               const double *__restrict__ h, double erad,
+                                               int use_pv, double *__restrict__ eta,
+                                               double *__restrict__ zeta,
+                                               double *__restrict__ h_tr,
+                                               double *__restrict__ q_tr, int *__restrict__ flag) {
+  const int k = xcd_index();
+  if (k >= nt) return;
+  double e = 0.0;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const int ed = t_ed[l * nt + k];
+    e = e + u[ed] * e_leng[ed] * (double)t_tg[l * nt + k];
+  }
+  const double z = e / t_area[k] / erad;
+  zeta[k] = z;
+  const double et = z + t_f[k];
+  eta[k] = et;
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) s = s + t_trhx[i * nt + k] * h[t_v[i * nt + k]];
+  const double ht = s / t_area[k];
+  h_tr[k] = ht;
+  double q = 0.0;
+  if (use_pv) {
+    if (fabs(ht) < HTR_GUARD) flag[0] = 1;
+    else q = et / ht;
+  }
+  q_tr[k] = q;
+}
+
+// scalar_tr2ed on q (swm_operators.f90:152-153)
+__global__ __launch_bounds__(TPB) void ref_edge_q(int ne, const int32_t *__restrict__ e_sh,
+                                                  const double *__restrict__ q_tr,
+                                                  double *__restrict__ q_ed) {
+  const int l = xcd_index();
+  if (l >= ne) return;
+  q_ed[l] = 0.5 * (q_tr[e_sh[l]] + q_tr[e_sh[ne + l]]);
+}
+
+// div_hx(uh), div_hx(u), kinetic_energy_hx, g(h+b)+K
+// (swm_operators.f90:613-661, :1132-1217; swm.f90:2533-2536)
+__global__ __launch_bounds__(TPB) void ref_cell(int nv, const int32_t *__restrict__ c_nnb,
+                                                const int32_t *__restrict__ c_ed,
+                                                const int8_t *__restrict__ c_ttg,
+                                                const double *__restrict__ c_area,
+                                                const double *__restrict__ e_leng,
+                                                const double *__restrict__ e_dleng,
+                                                const double *__restrict__ u,
+                                                const double *__restrict__ uh,
+                                                const double *__restrict__ h,
+                                                const double *__restrict__ bt, double erad,
+                                                double grav, double *__restrict__ divuh,
+                                                double *__restrict__ masseq,
+                                                double *__restrict__ divu,
+                                                double *__restrict__ ke,
+                                                double *__restrict__ ghbk) {
+  const int i = xcd_index();
+  if (i >= nv) return;
+  const int n = c_nnb[i];
+  double s = 0.0, sd = 0.0, k = 0.0, ca = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const int l = c_ed[j * nv + i];
+    const double sg = (double)c_ttg[j * nv + i];
+    s = s + sg * uh[l] * e_dleng[l];
+    sd = sd + sg * u[l] * e_dleng[l];
+    const double ed_area = 0.25 * e_leng[l] * e_dleng[l];
+    ca = ca + ed_area;
+    k = k + ed_area * (u[l] * u[l]);
+  }
+  const double dv = s / c_area[i] / erad;
+  divuh[i] = dv;
+  masseq[i] = -dv;
+  divu[i] = sd / c_area[i] / erad;
+  const double kk = k / ca;
+  ke[i] = kk;
+  ghbk[i] = grav * (h[i] + bt[i]) + kk;
+}
+
+// grad_ed (HTC) + coriolis_ed (trsk, PV form) -> momeq
+// (swm_operators.f90:535-542, :1283-1309; swm.f90:2542, :2567)
+__global__ __launch_bounds__(TPB) void ref_edge_mom(
+    int ne, int nv, const int32_t *__restrict__ e_v, const int8_t *__restrict__ e_loc,
+    const double *__restrict__ e_leng, const double *__restrict__ e_dleng,
+    const int32_t *__restrict__ c_nnb, const int32_t *__restrict__ c_ed,
+    const int8_t *__restrict__ c_ttg, const int8_t *__restrict__ c_tnr,
+    const double *__restrict__ c_trskw, const double *__restrict__ ghbk,
+    const double *__restrict__ uh, const double *__restrict__ q_ed, double erad,
+    int zero_mom, double *__restrict__ grad, double *__restrict__ uhq,
+    double *__restrict__ momeq) {
+  const int l = xcd_index();
+  if (l >= ne) return;
+  const double g = (ghbk[e_v[ne + l]] - ghbk[e_v[l]]) / e_leng[l] / erad;
+  grad[l] = g;
+  double s = 0.0;
+  const double ql = q_ed[l];
+  for (int ic = 0; ic < 2; ++ic) {
+    const int node = e_v[ic * ne + l];
+    const int ei = e_loc[ic * ne + l];
+    const int n = c_nnb[node];
+    const double tn = (double)c_tnr[ei * nv + node];
+    for (int j = 0; j < n; ++j) {
+      const int k = c_ed[j * nv + node];
+      const double sg = 1.0 * (double)c_ttg[j * nv + node] * tn;
+      const double signcor = sg < 0.0 ? -1.0 : 1.0;
+      const double qtmp = 0.5 * (ql + q_ed[k]);
+      s = s + uh[k] * qtmp * e_dleng[k] * c_trskw[(ei * MX + j) * nv + node] * signcor;
+    }
+  }
+  const double p = s / e_leng[l];
+  uhq[l] = p;
+  momeq[l] = zero_mom ? 0.0 : (-g) - p;
+}
+
+// ode_rk4 combinations (swm.f90:2647-2678)
+__global__ __launch_bounds__(TPB) void ref_rk_stage(int n, const double *__restrict__ y,
+                                                    const double *__restrict__ k, double dt,
+                                                    int half, double *__restrict__ yn) {
+  const int i = xcd_index();
+  if (i >= n) return;
+  yn[i] = half ? y[i] + dt * k[i] / 2.0 : y[i] + dt * k[i];
+}
+
+__global__ __launch_bounds__(TPB) void ref_rk_final(int n, double *__restrict__ y,
+                                                    const double *__restrict__ k0,
+                                                    const double *__restrict__ k1,
+                                                    const double *__restrict__ k2,
+                                                    const double *__restrict__ k3, double dt) {
+  const int i = xcd_index();
+  if (i >= n) return;
+  y[i] = y[i] + dt * (k0[i] + 2.0 * k1[i] + 2.0 * k2[i] + k3[i]) / 6.0;
+}
+
+// =========================================================== FAST kernels ==
+// Cell bit field: nnb (bits 0-2) | ttgout<0 (3-8) | tnrccw<0 (9-14) |
+//                 cell is the edge's 2nd end (15-20)
+#define BIT_TTG 3
+#define BIT_TNR 9
+#define BIT_SIDE 15
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void fast_tri(int nt, const int32_t *__restrict__ t_ed,
+                                                const int32_t *__restrict__ t_v,
+                                                const T *__restrict__ t_ceta,
+                                                const T *__restrict__ t_kap,
+                                                const T *__restrict__ t_f,
+                                                const T *__restrict__ u,
+                                                const T *__restrict__ h, T href, int use_pv,
+                                                T *__restrict__ q, int *__restrict__ flag) {
+  const int k = xcd_index();
+  if (k >= nt) return;
+  T eta = t_f[k];
+  T hs = (T)0;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    eta += u[t_ed[l * nt + k]] * t_ceta[l * nt + k];
+    hs += t_kap[l * nt + k] * h[t_v[l * nt + k]];
+  }
+  const T ht = href + hs;
+  T qq = (T)0;
+  if (use_pv) {
+    if (fabs((double)ht) < HTR_GUARD) flag[0] = 1;
+    else qq = eta / ht;
+  }
+  q[k] = qq;
+}
+
+// Neumaier's compensated summation step.
+__device__ __forceinline__ void kadd(double &s, double &c, double x) {
+  const double t = s + x;
+  if (fabs(s) >= fabs(x)) c += (s - t) + x;
+  else c += (x - t) + s;
+  s = t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void fast_cell(
+    int nv, int ne, const int32_t *__restrict__ c_ed, const int32_t *__restrict__ c_nb,
+    const int32_t *__restrict__ c_tr, const int32_t *__restrict__ c_bits,
+    const T *__restrict__ c_R, const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
+    const double *__restrict__ e_dl, const T *__restrict__ e_kw, const T *__restrict__ q,
+    const T *__restrict__ hs, const T *__restrict__ us, double href, T grav, T inv_a,
+    T *__restrict__ phi, T *__restrict__ Pe, int stage, double dt,
+    const T *h0, T *hout, T *__restrict__ acch,
+    double *__restrict__ dh_out) {
+  const int c = xcd_index();
+  if (c >= nv) return;
+  const int bits = c_bits[c];
+  const int n = bits & 7;
+  const T hc = hs[c];
+  T G[MX], qt[MX], qe[MX], R[MX];
+  int ed[MX];
+  double dv = 0.0, dvc = 0.0;
+  T ke = (T)0;
+#pragma unroll
+  for (int j = 0; j < MX; ++j) {
+    G[j] = (T)0;
+    qt[j] = (T)0;
+    R[j] = (T)0;
+    ed[j] = 0;
+    if (j < n) {
+      const int e = c_ed[j * nv + c];
+      ed[j] = e;
+      const T ue = us[e];
+      const T hn = hs[c_nb[j * nv + c]];
+      const double he = href + 0.5 * ((double)hc + (double)hn);
+      double g = (double)ue * he * e_dl[e];
+      if ((bits >> (BIT_TTG + j)) & 1) g = -g;
+      kadd(dv, dvc, g);
+      G[j] = (T)g;
+      ke += e_kw[e] * ue * ue;
+      qt[j] = q[c_tr[j * nv + c]];
+      R[j] = c_R[j * nv + c];
+    }
+  }
+  const double dh = -(dv + dvc) * c_iaa[c];
+  const T K = ke * c_ika[c];
+  phi[c] = (grav * hc + K) * inv_a;
+
+  // PV at the cell's edges: edge j lies between ring triangles j-1 and j
+#pragma unroll
+  for (int j = 0; j < MX; ++j) {
+    const int jm = (j == 0) ? n - 1 : j - 1;
+    T qm = (T)0;
+#pragma unroll
+    for (int t = 0; t < MX; ++t)
+      if (t == jm) qm = qt[t];
+    qe[j] = (j < n) ? (T)0.5 * (qm + qt[j]) : (T)0;
+  }
+  // prefix sums of kite areas: w(i,j) = S_j - S_i - 1/2 (j>i), tot - S_i + S_j - 1/2 (j<i)
+  T S[MX + 1];
+  S[0] = (T)0;
+#pragma unroll
+  for (int j = 0; j < MX; ++j) S[j + 1] = S[j] + R[j];
+  T tot = (T)0;
+#pragma unroll
+  for (int j = 0; j <= MX; ++j)
+    if (j == n) tot = S[j];
+#pragma unroll
+  for (int i = 0; i < MX; ++i) {
+    if (i < n) {
+      T acc = (T)0;
+#pragma unroll
+      for (int j = 0; j < MX; ++j) {
+        if (j < n && j != i) {
+          const T w = ((j > i) ? (S[j] - S[i]) : (tot - S[i] + S[j])) - (T)0.5;
+          acc += G[j] * (qe[i] + qe[j]) * w;
+        }
+      }
+      T P = (T)0.5 * acc;
+      if ((bits >> (BIT_TNR + i)) & 1) P = -P;
+      const int side = (bits >> (BIT_SIDE + i)) & 1;
+      Pe[side * ne + ed[i]] = P;
+    }
+  }
+
+  if (stage < 0) {
+    dh_out[c] = dh;
+  } else if (stage < 3) {
+    const double a = (stage == 2) ? dt : 0.5 * dt;
+    const double b = (stage == 0) ? dt / 6.0 : dt / 3.0;
+    const double y0 = (double)h0[c];
+    hout[c] = (T)(y0 + a * dh);
+    acch[c] = (stage == 0) ? (T)(b * dh) : (T)((double)acch[c] + b * dh);
+  } else {
+    hout[c] = (T)((double)h0[c] + ((double)acch[c] + (dt / 6.0) * dh));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void fast_edge(int ne, const int32_t *__restrict__ e_c,
+                                                 const T *__restrict__ e_il,
+                                                 const T *__restrict__ phi,
+                                                 const T *__restrict__ Pe, int zero_mom,
+                                                 int stage, T dt, const T *u0,
+                                                 T *uout, T *__restrict__ accu,
+                                                 T *__restrict__ du_out) {
+  const int e = xcd_index();
+  if (e >= ne) return;
+  T du = -((phi[e_c[ne + e]] - phi[e_c[e]]) + (Pe[e] + Pe[ne + e])) * e_il[e];
+  if (zero_mom) du = (T)0;
+  if (stage < 0) {
+    du_out[e] = du;
+  } else if (stage < 3) {
+    const T a = (stage == 2) ? dt : (T)0.5 * dt;
+    const T b = (stage == 0) ? dt / (T)6 : dt / (T)3;
+    uout[e] = u0[e] + a * du;
+    accu[e] = (stage == 0) ? b * du : accu[e] + b * du;
+  } else {
+    uout[e] = u0[e] + (accu[e] + (dt / (T)6) * du);
+  }
+}
+
+// ============================================================ diagnostics ==
+// FP64 reductions of the current state (h = href + hs, u) with wavefront
+// shuffles + LDS, one partial per block, then a one-block final pass.
+#define NPART 8
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ void block_store(double (&v)[NPART], double *part) {
+  __shared__ double red[TPB / 64][NPART];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NPART; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) red[wid][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NPART) {
+    double s = 0.0;
+    for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
+    part[(size_t)blockIdx.x * NPART + threadIdx.x] = s;
+  }
+}
+
+// slot 0: sum A h, 1: sum A, 2: sum A g h (h/2), 3: sum A divu^2, 7: bad
+template <typename T>
+__global__ __launch_bounds__(TPB) void diag_cell(int nv, const int32_t *__restrict__ c_nnb,
+                                                 const int32_t *__restrict__ c_ed,
+                                                 const int8_t *__restrict__ c_ttg,
+                                                 const double *__restrict__ c_area,
+                                                 const double *__restrict__ e_dleng,
+                                                 const T *__restrict__ hs,
+                                                 const T *__restrict__ us, double href,
+                                                 double grav, double erad,
+                                                 double *__restrict__ part) {
+  double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i < nv) {
+    const double h = href + (double)hs[i];
+    const double A = c_area[i];
+    double s = 0.0;
+    for (int j = 0; j < c_nnb[i]; ++j) {
+      const int l = c_ed[j * nv + i];
+      s = s + (double)c_ttg[j * nv + i] * (double)us[l] * e_dleng[l];
+    }
+    const double dvu = s / A / erad;
+    v[0] = A * h;
+    v[1] = A;
+    v[2] = A * grav * h * (h * 0.5);
+    v[3] = A * (dvu * dvu);
+    v[7] = isfinite(h) ? 0.0 : 1.0;
+  }
+  block_store(v, part);
+}
+
+// slot 4: sum 1/2 l d h_e u^2, 7: bad
+template <typename T>
+__global__ __launch_bounds__(TPB) void diag_edge(int ne, const int32_t *__restrict__ e_v,
+                                                 const double *__restrict__ e_leng,
+                                                 const double *__restrict__ e_dleng,
+                                                 const T *__restrict__ hs,
+                                                 const T *__restrict__ us, double href,
+                                                 double *__restrict__ part) {
+  double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int l = blockIdx.x * TPB + threadIdx.x;
+  if (l < ne) {
+    const double he = (href + (double)hs[e_v[l]] + href + (double)hs[e_v[ne + l]]) * 0.5;
+    const double u = (double)us[l];
+    v[4] = 0.5 * e_leng[l] * e_dleng[l] * he * (u * u);
+    v[7] = isfinite(u) ? 0.0 : 1.0;
+  }
+  block_store(v, part);
+}
+
+// slot 5: sum 1/2 A_t a^2 h_t q_t^2 (potential enstrophy)
+template <typename T>
+__global__ __launch_bounds__(TPB) void diag_tri(int nt, const int32_t *__restrict__ t_ed,
+                                                const int8_t *__restrict__ t_tg,
+                                                const int32_t *__restrict__ t_v,
+                                                const double *__restrict__ t_area,
+                                                const double *__restrict__ t_f,
+                                                const double *__restrict__ t_trhx,
+                                                const double *__restrict__ e_leng,
+                                                const T *__restrict__ hs,
+                                                const T *__restrict__ us, double href,
+                                                double erad, double *__restrict__ part) {
+  double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int k = blockIdx.x * TPB + threadIdx.x;
+  if (k < nt) {
+    double e = 0.0, s = 0.0;
+    for (int l = 0; l < 3; ++l) {
+      const int ed = t_ed[l * nt + k];
+      e += (double)us[ed] * e_leng[ed] * (double)t_tg[l * nt + k];
+      s += t_trhx[l * nt + k] * (href + (double)hs[t_v[l * nt + k]]);
+    }
+    const double eta = e / t_area[k] / erad + t_f[k];
+    const double ht = s / t_area[k];
+    const double qq = eta / ht;
+    v[5] = 0.5 * t_area[k] * erad * erad * ht * qq * qq;
+  }
+  block_store(v, part);
+}
+
+__global__ __launch_bounds__(TPB) void diag_final(int nparts, const double *__restrict__ part,
+                                                  double *__restrict__ out) {
+  double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nparts; b += TPB)
+#pragma unroll
+    for (int k = 0; k < NPART; ++k) v[k] += part[(size_t)b * NPART + k];
+  __shared__ double red[TPB / 64][NPART];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NPART; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) red[wid][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NPART) {
+    double s = 0.0;
+    for (int w = 0; w < TPB / 64; ++w) s += red[w][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// =============================================================== handle ====
+namespace {
+
+struct Alloc {
+  std::vector<void *> ptrs;
+  size_t bytes = 0;
+  hipError_t err = hipSuccess;
+  template <typename X>
+  X *get(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(X));
+    if (e != hipSuccess) {
+      err = e;
+      return nullptr;
+    }
+    ptrs.push_back(p);
+    bytes += n * sizeof(X);
+    return (X *)p;
+  }
+  template <typename X>
+  X *put(const std::vector<X> &h) {
+    X *p = get<X>(h.size());
+    if (p && !h.empty()) {
+      hipError_t e = hipMemcpy(p, h.data(), h.size() * sizeof(X), hipMemcpyHostToDevice);
+      if (e != hipSuccess) err = e;
+    }
+    return p;
+  }
+  void release() {
+    for (void *p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+  }
+};
+
+struct KTimer {
+  const char *name;
+};
+
+}  // namespace
+
+struct imx_swm {
+  imx_swm_opts o{};
+  int nv = 0, ne = 0, nt = 0;
+  std::vector<int32_t> cell_new, edge_new, tri_new;  // caller -> internal
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Alloc A;
+  bool fast = false;
+  bool single = false;  // FAST with T=float
+  double href = 0.0;
+  bool href_set = false;
+
+  // geometry / REF tables (always present)
+  int32_t *c_nnb = nullptr, *c_ed = nullptr;
+  int8_t *c_ttg = nullptr, *c_tnr = nullptr;
+  double *c_area = nullptr, *c_trskw = nullptr;
+  int32_t *e_v = nullptr, *e_sh = nullptr;
+  int8_t *e_loc = nullptr;
+  double *e_leng = nullptr, *e_dleng = nullptr;
+  int32_t *t_ed = nullptr, *t_v = nullptr;
+  int8_t *t_tg = nullptr;
+  double *t_area = nullptr, *t_f = nullptr, *t_trhx = nullptr;
+  double *bt = nullptr;
+  int *flag = nullptr;
+  double *part = nullptr, *diag = nullptr;
+  int nparts = 0;
+
+  // REF state and fields
+  double *rh = nullptr, *ru = nullptr, *rhn = nullptr, *run = nullptr;
+  double *massf = nullptr, *momf = nullptr;
+  double *f_h_ed = nullptr, *f_uh = nullptr, *f_eta = nullptr, *f_h_tr = nullptr,
+         *f_q_tr = nullptr, *f_q_ed = nullptr, *f_divuh = nullptr, *f_ke = nullptr,
+         *f_ghbk = nullptr, *f_grad = nullptr, *f_uhq = nullptr, *f_divu = nullptr,
+         *f_zeta = nullptr;
+
+  // FAST tables and state (typed by precision; void* holds T*)
+  int32_t *x_c_ed = nullptr, *x_c_nb = nullptr, *x_c_tr = nullptr, *x_c_bits = nullptr;
+  void *x_c_R = nullptr, *x_c_ika = nullptr;
+  double *x_c_iaa = nullptr;
+  double *x_e_dl = nullptr;
+  void *x_e_kw = nullptr, *x_e_il = nullptr;
+  int32_t *x_e_c = nullptr;
+  int32_t *x_t_ed = nullptr, *x_t_v = nullptr;
+  void *x_t_ceta = nullptr, *x_t_kap = nullptr, *x_t_f = nullptr;
+  void *h0 = nullptr, *hA = nullptr, *hB = nullptr, *u0 = nullptr, *uA = nullptr, *uB = nullptr;
+  void *acch = nullptr, *accu = nullptr, *q = nullptr, *phi = nullptr, *Pe = nullptr;
+  double *dh_out = nullptr;
+  void *du_out = nullptr;
+};
+
+namespace imx {
+
+static int hip_fail(hipError_t e, const char *what) {
+  return set_error(IMX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPTRY(expr, what)                 \
+  do {                                     \
+    hipError_t _e = (expr);                \
+    if (_e != hipSuccess) return imx::hip_fail(_e, what); \
+  } while (0)
+
+template <typename T>
+static std::vector<T> cvt(const std::vector<double> &a) {
+  return std::vector<T>(a.begin(), a.end());
+}
+
+// --------------------------------------------------------------- launches --
+static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, double *masseq,
+                                double *momeq) {
+  const int nv = s->nv, ne = s->ne, nt = s->nt;
+  hipStream_t st = s->stream;
+  ref_edge_flux<<<grid_for(ne), TPB, 0, st>>>(ne, s->e_v, h, u, s->f_h_ed, s->f_uh);
+  ref_tri<<<grid_for(nt), TPB, 0, st>>>(nt, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f,
+                                        s->t_trhx, s->e_leng, u, h, s->o.erad,
+                                        s->o.testcase >= 2, s->f_eta, s->f_zeta, s->f_h_tr,
+                                        s->f_q_tr, s->flag);
+  ref_edge_q<<<grid_for(ne), TPB, 0, st>>>(ne, s->e_sh, s->f_q_tr, s->f_q_ed);
+  ref_cell<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area, s->e_leng,
+                                         s->e_dleng, u, s->f_uh, h, s->bt, s->o.erad, s->o.grav,
+                                         s->f_divuh, masseq, s->f_divu, s->f_ke, s->f_ghbk);
+  ref_edge_mom<<<grid_for(ne), TPB, 0, st>>>(ne, nv, s->e_v, s->e_loc, s->e_leng, s->e_dleng,
+                                             s->c_nnb, s->c_ed, s->c_ttg, s->c_tnr, s->c_trskw,
+                                             s->f_ghbk, s->f_uh, s->f_q_ed, s->o.erad,
+                                             s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
+}
+
+static void launch_ref_step(imx_swm *s) {
+  const int nv = s->nv, ne = s->ne;
+  const double dt = s->o.dt;
+  hipStream_t st = s->stream;
+  double *m[4], *f[4];
+  for (int k = 0; k < 4; ++k) {
+    m[k] = s->massf + (size_t)k * nv;
+    f[k] = s->momf + (size_t)k * ne;
+  }
+  launch_ref_tendency(s, s->rh, s->ru, m[0], f[0]);
+  for (int k = 0; k < 3; ++k) {
+    const int half = (k < 2);
+    ref_rk_stage<<<grid_for(ne), TPB, 0, st>>>(ne, s->ru, f[k], dt, half, s->run);
+    ref_rk_stage<<<grid_for(nv), TPB, 0, st>>>(nv, s->rh, m[k], dt, half, s->rhn);
+    launch_ref_tendency(s, s->rhn, s->run, m[k + 1], f[k + 1]);
+  }
+  ref_rk_final<<<grid_for(ne), TPB, 0, st>>>(ne, s->ru, f[0], f[1], f[2], f[3], dt);
+  ref_rk_final<<<grid_for(nv), TPB, 0, st>>>(nv, s->rh, m[0], m[1], m[2], m[3], dt);
+}
+
+// One FAST stage: stage<0 evaluates the tendency of (hs, us) into dh_out/du_out.
+template <typename T>
+static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T *hout,
+                              T *uout, hipEvent_t *evs) {
+  const int nv = s->nv, ne = s->ne, nt = s->nt;
+  hipStream_t st = s->stream;
+  const T href = (T)s->href;
+  if (evs) (void)hipEventRecord(evs[0], st);
+  fast_tri<T><<<grid_for(nt), TPB, 0, st>>>(nt, s->x_t_ed, s->x_t_v, (const T *)s->x_t_ceta,
+                                            (const T *)s->x_t_kap, (const T *)s->x_t_f, us, hs,
+                                            href, s->o.testcase >= 2, (T *)s->q, s->flag);
+  if (evs) (void)hipEventRecord(evs[1], st);
+  fast_cell<T><<<grid_for(nv), TPB, 0, st>>>(
+      nv, ne, s->x_c_ed, s->x_c_nb, s->x_c_tr, s->x_c_bits, (const T *)s->x_c_R, s->x_c_iaa,
+      (const T *)s->x_c_ika, s->x_e_dl, (const T *)s->x_e_kw, (const T *)s->q, hs, us, s->href,
+      (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pe, stage, s->o.dt,
+      (const T *)s->h0, hout, (T *)s->acch, s->dh_out);
+  if (evs) (void)hipEventRecord(evs[2], st);
+  fast_edge<T><<<grid_for(ne), TPB, 0, st>>>(ne, s->x_e_c, (const T *)s->x_e_il, (const T *)s->phi,
+                                             (const T *)s->Pe, s->o.testcase <= 1, stage,
+                                             (T)s->o.dt, (const T *)s->u0, uout, (T *)s->accu,
+                                             (T *)s->du_out);
+  if (evs) (void)hipEventRecord(evs[3], st);
+}
+
+template <typename T>
+static void launch_fast_step(imx_swm *s, std::vector<hipEvent_t> *evs) {
+  T *h0 = (T *)s->h0, *hA = (T *)s->hA, *hB = (T *)s->hB;
+  T *u0 = (T *)s->u0, *uA = (T *)s->uA, *uB = (T *)s->uB;
+  const T *hin[4] = {h0, hA, hB, hA};
+  const T *uin[4] = {u0, uA, uB, uA};
+  T *hout[4] = {hA, hB, hA, h0};
+  T *uout[4] = {uA, uB, uA, u0};
+  for (int st = 0; st < 4; ++st)
+    launch_fast_stage<T>(s, st, hin[st], uin[st], hout[st], uout[st],
+                         evs ? &(*evs)[4 * st] : nullptr);
+}
+
+static void launch_step(imx_swm *s) {
+  if (!s->fast) launch_ref_step(s);
+  else if (s->single) launch_fast_step<float>(s, nullptr);
+  else launch_fast_step<double>(s, nullptr);
+}
+
+template <typename T>
+static void launch_diag(imx_swm *s, const T *hs, const T *us, double href) {
+  hipStream_t st = s->stream;
+  const int gv = (s->nv + TPB - 1) / TPB, ge = (s->ne + TPB - 1) / TPB,
+            gt = (s->nt + TPB - 1) / TPB;
+  diag_cell<T><<<gv, TPB, 0, st>>>(s->nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area, s->e_dleng, hs,
+                                   us, href, s->o.grav, s->o.erad, s->part);
+  diag_edge<T><<<ge, TPB, 0, st>>>(s->ne, s->e_v, s->e_leng, s->e_dleng, hs, us, href,
+                                   s->part + (size_t)gv * NPART);
+  diag_tri<T><<<gt, TPB, 0, st>>>(s->nt, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f, s->t_trhx,
+                                  s->e_leng, hs, us, href, s->o.erad,
+                                  s->part + (size_t)(gv + ge) * NPART);
+  diag_final<<<1, TPB, 0, st>>>(gv + ge + gt, s->part, s->diag);
+}
+
+// ------------------------------------------------------------ create -------
+template <typename T>
+static int build_fast(imx_swm *s, const imx_mesh_desc &d, const double *trhx, const double *hxtr) {
+  const int nv = s->nv, ne = s->ne, nt = s->nt;
+  const double a = s->o.erad;
+  std::vector<int32_t> c_ed((size_t)MX * nv, 0), c_nb((size_t)MX * nv, 0), c_tr((size_t)MX * nv, 0),
+      c_bits(nv, 0);
+  std::vector<double> c_R((size_t)MX * nv, 0.0), c_iaa(nv), c_ika(nv);
+  for (int co = 0; co < nv; ++co) {
+    const int c = s->cell_new[co];
+    const int n = d.v_nnb[co];
+    int bits = n;
+    double ksum = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const int eo = d.v_ed[MX * co + j] - 1;
+      c_ed[(size_t)j * nv + c] = s->edge_new[eo];
+      c_nb[(size_t)j * nv + c] = s->cell_new[d.v_nb[MX * co + j] - 1];
+      c_tr[(size_t)j * nv + c] = s->tri_new[d.v_tr[MX * co + j] - 1];
+      if (d.hx_ttgout[MX * co + j] < 0) bits |= 1 << (BIT_TTG + j);
+      if (d.hx_tnrccw[MX * co + j] < 0) bits |= 1 << (BIT_TNR + j);
+      if (d.ed_v[2 * eo] - 1 != co) bits |= 1 << (BIT_SIDE + j);
+      c_R[(size_t)j * nv + c] = hxtr[MX * co + j];
+      ksum += 0.25 * d.ed_leng[eo] * d.edhx_leng[eo];
+    }
+    c_bits[c] = bits;
+    c_iaa[c] = 1.0 / (d.hx_areag[co] * a);
+    c_ika[c] = 1.0 / ksum;
+  }
+  std::vector<int32_t> e_c((size_t)2 * ne);
+  std::vector<double> e_dl(ne), e_kw(ne), e_il(ne);
+  for (int eo = 0; eo < ne; ++eo) {
+    const int e = s->edge_new[eo];
+    e_c[e] = s->cell_new[d.ed_v[2 * eo] - 1];
+    e_c[(size_t)ne + e] = s->cell_new[d.ed_v[2 * eo + 1] - 1];
+    e_dl[e] = d.edhx_leng[eo];
+    e_kw[e] = 0.25 * d.ed_leng[eo] * d.edhx_leng[eo];
+    e_il[e] = 1.0 / d.ed_leng[eo];
+  }
+  std::vector<int32_t> t_ed((size_t)3 * nt), t_v((size_t)3 * nt);
+  std::vector<double> t_ceta((size_t)3 * nt), t_kap((size_t)3 * nt), t_f(nt);
+  for (int ko = 0; ko < nt; ++ko) {
+    const int k = s->tri_new[ko];
+    for (int l = 0; l < 3; ++l) {
+      const int eo = d.tr_ed[3 * ko + l] - 1;
+      t_ed[(size_t)l * nt + k] = s->edge_new[eo];
+      t_v[(size_t)l * nt + k] = s->cell_new[d.tr_v[3 * ko + l] - 1];
+      t_ceta[(size_t)l * nt + k] = d.ed_leng[eo] * (double)d.tr_tg[3 * ko + l] / (d.tr_areag[ko] * a);
+      t_kap[(size_t)l * nt + k] = trhx[3 * ko + l] / d.tr_areag[ko];
+    }
+    t_f[k] = 2.0 * s->o.omega * std::sin(d.tr_clat[ko]);
+  }
+  Alloc &A = s->A;
+  s->x_c_ed = A.put(c_ed);
+  s->x_c_nb = A.put(c_nb);
+  s->x_c_tr = A.put(c_tr);
+  s->x_c_bits = A.put(c_bits);
+  s->x_c_R = A.put(cvt<T>(c_R));
+  s->x_c_iaa = A.put(c_iaa);
+  s->x_c_ika = A.put(cvt<T>(c_ika));
+  s->x_e_c = A.put(e_c);
+  s->x_e_dl = A.put(e_dl);
+  s->x_e_kw = A.put(cvt<T>(e_kw));
+  s->x_e_il = A.put(cvt<T>(e_il));
+  s->x_t_ed = A.put(t_ed);
+  s->x_t_v = A.put(t_v);
+  s->x_t_ceta = A.put(cvt<T>(t_ceta));
+  s->x_t_kap = A.put(cvt<T>(t_kap));
+  s->x_t_f = A.put(cvt<T>(t_f));
+  s->h0 = A.get<T>(nv);
+  s->hA = A.get<T>(nv);
+  s->hB = A.get<T>(nv);
+  s->u0 = A.get<T>(ne);
+  s->uA = A.get<T>(ne);
+  s->uB = A.get<T>(ne);
+  s->acch = A.get<T>(nv);
+  s->accu = A.get<T>(ne);
+  s->q = A.get<T>(nt);
+  s->phi = A.get<T>(nv);
+  s->Pe = A.get<T>((size_t)2 * ne);
+  s->dh_out = A.get<double>(nv);
+  s->du_out = A.get<T>(ne);
+  if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_create (fast tables)");
+  return IMX_OK;
+}
+
+static int validate_desc(const imx_mesh_desc &d) {
+  if (d.nv <= 0 || d.ne <= 0 || d.nt <= 0) return set_error(IMX_EINVAL, "mesh: empty mesh");
+  if (!d.v_nnb || !d.v_nb || !d.v_ed || !d.v_tr || !d.hx_ttgout || !d.hx_tnrccw || !d.v_p ||
+      !d.hx_areag || !d.ed_v || !d.ed_sh || !d.ed_leng || !d.edhx_leng || !d.ed_c || !d.edhx_c ||
+      !d.tr_v || !d.tr_ed || !d.tr_tg || !d.tr_areag || !d.tr_c || !d.tr_clat)
+    return set_error(IMX_EINVAL, "mesh: a required array is NULL");
+  for (int i = 0; i < d.nv; ++i) {
+    const int n = d.v_nnb[i];
+    if (n < 3 || n > MX) return set_error(IMX_EINVAL, "mesh: cell with nnb outside 3..6");
+    for (int j = 0; j < n; ++j) {
+      const int e = d.v_ed[MX * i + j], t = d.v_tr[MX * i + j], c = d.v_nb[MX * i + j];
+      if (e < 1 || e > d.ne || t < 1 || t > d.nt || c < 1 || c > d.nv)
+        return set_error(IMX_EINVAL, "mesh: cell list index out of range");
+    }
+  }
+  for (int l = 0; l < d.ne; ++l)
+    for (int k = 0; k < 2; ++k) {
+      const int c = d.ed_v[2 * l + k], t = d.ed_sh[2 * l + k];
+      if (c < 1 || c > d.nv || t < 1 || t > d.nt)
+        return set_error(IMX_EINVAL, "mesh: edge index out of range");
+    }
+  for (int k = 0; k < d.nt; ++k)
+    for (int l = 0; l < 3; ++l) {
+      const int e = d.tr_ed[3 * k + l], c = d.tr_v[3 * k + l];
+      if (e < 1 || e > d.ne || c < 1 || c > d.nv)
+        return set_error(IMX_EINVAL, "mesh: triangle index out of range");
+    }
+  // every edge must appear in the rings of both its end cells
+  for (int l = 0; l < d.ne; ++l)
+    for (int k = 0; k < 2; ++k) {
+      const int c = d.ed_v[2 * l + k] - 1;
+      bool found = false;
+      for (int j = 0; j < d.v_nnb[c]; ++j) found |= (d.v_ed[MX * c + j] == l + 1);
+      if (!found) return set_error(IMX_EINVAL, "mesh: edge missing from the ring of its cell");
+    }
+  return IMX_OK;
+}
+
+static int create(const imx_mesh_desc &d, const imx_swm_opts &o, imx_swm *s) {
+  int rc = validate_desc(d);
+  if (rc) return rc;
+  if (o.precision < IMX_PREC_FP64_REF || o.precision > IMX_PREC_FP32)
+    return set_error(IMX_EINVAL, "opts: unknown precision");
+  if (!(o.erad > 0) || !(o.grav > 0)) return set_error(IMX_EINVAL, "opts: erad and grav must be > 0");
+  s->o = o;
+  s->nv = d.nv;
+  s->ne = d.ne;
+  s->nt = d.nt;
+  s->fast = o.precision != IMX_PREC_FP64_REF;
+  s->single = o.precision == IMX_PREC_FP32;
+  if (o.h_ref != 0.0) {
+    s->href = o.h_ref;
+    s->href_set = true;
+  }
+  const int nv = d.nv, ne = d.ne, nt = d.nt;
+
+  // gridprop: caller's arrays or computed here (reference formulas, FP64)
+  std::vector<double> trhx_v, hxtr_v, trskw_v;
+  const double *trhx = d.tr_trhx_area, *hxtr = d.hx_hxtr_area, *trskw = d.hx_trskw;
+  if (!trhx || !hxtr || !trskw) {
+    trhx_v.assign((size_t)3 * nt, 0.0);
+    hxtr_v.assign((size_t)MX * nv, 0.0);
+    trskw_v.assign((size_t)MX * MX * nv, 0.0);
+    gridprop(d, trhx_v.data(), hxtr_v.data(), trskw_v.data());
+    trhx = trhx_v.data();
+    hxtr = hxtr_v.data();
+    trskw = trskw_v.data();
+  }
+
+  sfc_order(d, o.use_sfc != 0, s->cell_new, s->edge_new, s->tri_new);
+
+  HIPTRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+  HIPTRY(hipEventCreate(&s->ev0), "hipEventCreate");
+  HIPTRY(hipEventCreate(&s->ev1), "hipEventCreate");
+
+  // geometry tables in internal numbering (slot-major [j][entity])
+  std::vector<int32_t> c_nnb(nv), c_ed((size_t)MX * nv, 0);
+  std::vector<int8_t> c_ttg((size_t)MX * nv, 0), c_tnr((size_t)MX * nv, 0);
+  std::vector<double> c_area(nv), c_trskw((size_t)MX * MX * nv, 0.0);
+  for (int co = 0; co < nv; ++co) {
+    const int c = s->cell_new[co];
+    const int n = d.v_nnb[co];
+    c_nnb[c] = n;
+    c_area[c] = d.hx_areag[co];
+    for (int j = 0; j < n; ++j) {
+      c_ed[(size_t)j * nv + c] = s->edge_new[d.v_ed[MX * co + j] - 1];
+      c_ttg[(size_t)j * nv + c] = (int8_t)(d.hx_ttgout[MX * co + j] < 0 ? -1 : 1);
+      c_tnr[(size_t)j * nv + c] = (int8_t)(d.hx_tnrccw[MX * co + j] < 0 ? -1 : 1);
+      for (int k = 0; k < n; ++k)
+        c_trskw[((size_t)j * MX + k) * nv + c] = trskw[MX * MX * co + MX * j + k];
+    }
+  }
+  std::vector<int32_t> e_v((size_t)2 * ne), e_sh((size_t)2 * ne);
+  std::vector<int8_t> e_loc((size_t)2 * ne);
+  std::vector<double> e_leng(ne), e_dleng(ne);
+  for (int eo = 0; eo < ne; ++eo) {
+    const int e = s->edge_new[eo];
+    for (int k = 0; k < 2; ++k) {
+      const int co = d.ed_v[2 * eo + k] - 1;
+      e_v[(size_t)k * ne + e] = s->cell_new[co];
+      e_sh[(size_t)k * ne + e] = s->tri_new[d.ed_sh[2 * eo + k] - 1];
+      int loc = -1;  // getedindexonhx (smeshpack.f90:4099-4125)
+      for (int j = 0; j < d.v_nnb[co]; ++j)
+        if (d.v_ed[MX * co + j] == eo + 1) {
+          loc = j;
+          break;
+        }
+      e_loc[(size_t)k * ne + e] = (int8_t)loc;
+    }
+    e_leng[e] = d.ed_leng[eo];
+    e_dleng[e] = d.edhx_leng[eo];
+  }
+  std::vector<int32_t> t_ed((size_t)3 * nt), t_v((size_t)3 * nt);
+  std::vector<int8_t> t_tg((size_t)3 * nt);
+  std::vector<double> t_area(nt), t_f(nt), t_trhx((size_t)3 * nt);
+  for (int ko = 0; ko < nt; ++ko) {
+    const int k = s->tri_new[ko];
+    for (int l = 0; l < 3; ++l) {
+      t_ed[(size_t)l * nt + k] = s->edge_new[d.tr_ed[3 * ko + l] - 1];
+      t_v[(size_t)l * nt + k] = s->cell_new[d.tr_v[3 * ko + l] - 1];
+      t_tg[(size_t)l * nt + k] = (int8_t)(d.tr_tg[3 * ko + l] < 0 ? -1 : 1);
+      t_trhx[(size_t)l * nt + k] = trhx[3 * ko + l];
+    }
+    t_area[k] = d.tr_areag[ko];
+    // 2.*Omega*dsin(lat) of vort_tr (swm_operators.f90:597), libm sin on the host
+    t_f[k] = 2.0 * o.omega * std::sin(d.tr_clat[ko]);
+  }
+  Alloc &A = s->A;
+  s->c_nnb = A.put(c_nnb);
+  s->c_ed = A.put(c_ed);
+  s->c_ttg = A.put(c_ttg);
+  s->c_tnr = A.put(c_tnr);
+  s->c_area = A.put(c_area);
+  s->e_v = A.put(e_v);
+  s->e_sh = A.put(e_sh);
+  s->e_loc = A.put(e_loc);
+  s->e_leng = A.put(e_leng);
+  s->e_dleng = A.put(e_dleng);
+  s->t_ed = A.put(t_ed);
+  s->t_v = A.put(t_v);
+  s->t_tg = A.put(t_tg);
+  s->t_area = A.put(t_area);
+  s->t_f = A.put(t_f);
+  s->t_trhx = A.put(t_trhx);
+  s->bt = A.put(std::vector<double>(nv, 0.0));
+  s->flag = A.put(std::vector<int>(1, 0));
+  s->nparts = (nv + TPB - 1) / TPB + (ne + TPB - 1) / TPB + (nt + TPB - 1) / TPB;
+  s->part = A.get<double>((size_t)s->nparts * NPART);
+  s->diag = A.get<double>(NPART);
+  if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_create (geometry)");
+
+  if (!s->fast) {
+    s->c_trskw = A.put(c_trskw);
+    s->rh = A.get<double>(nv);
+    s->ru = A.get<double>(ne);
+    s->rhn = A.get<double>(nv);
+    s->run = A.get<double>(ne);
+    s->massf = A.get<double>((size_t)4 * nv);
+    s->momf = A.get<double>((size_t)4 * ne);
+    s->f_h_ed = A.get<double>(ne);
+    s->f_uh = A.get<double>(ne);
+    s->f_eta = A.get<double>(nt);
+    s->f_h_tr = A.get<double>(nt);
+    s->f_q_tr = A.get<double>(nt);
+    s->f_q_ed = A.get<double>(ne);
+    s->f_divuh = A.get<double>(nv);
+    s->f_ke = A.get<double>(nv);
+    s->f_ghbk = A.get<double>(nv);
+    s->f_grad = A.get<double>(ne);
+    s->f_uhq = A.get<double>(ne);
+    s->f_divu = A.get<double>(nv);
+    s->f_zeta = A.get<double>(nt);
+    if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_create (ref fields)");
+  } else {
+    rc = s->single ? build_fast<float>(s, d, trhx, hxtr) : build_fast<double>(s, d, trhx, hxtr);
+    if (rc) return rc;
+  }
+  HIPTRY(hipStreamSynchronize(s->stream), "imx_swm_create");
+  return IMX_OK;
+}
+
+static int check_flag(imx_swm *s, const char *what) {
+  int f = 0;
+  HIPTRY(hipMemcpyAsync(&f, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream), what);
+  HIPTRY(hipStreamSynchronize(s->stream), what);
+  if (f) {
+    int z = 0;
+    HIPTRY(hipMemcpy(s->flag, &z, sizeof(int), hipMemcpyHostToDevice), what);
+    return set_error(IMX_ESTATE, std::string(what) +
+                                     ": division by zero, |h_tr| < epsilon/1e5 "
+                                     "(swm_operators.f90:1603)");
+  }
+  return IMX_OK;
+}
+
+template <typename T>
+static int upload_state(imx_swm *s, const double *h, const double *u, T *hd, T *ud) {
+  std::vector<T> hb(s->nv), ub(s->ne);
+  for (int i = 0; i < s->nv; ++i) hb[s->cell_new[i]] = (T)(h[i] - s->href);
+  for (int l = 0; l < s->ne; ++l) ub[s->edge_new[l]] = (T)u[l];
+  HIPTRY(hipMemcpyAsync(hd, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, s->stream),
+         "upload h");
+  HIPTRY(hipMemcpyAsync(ud, ub.data(), ub.size() * sizeof(T), hipMemcpyHostToDevice, s->stream),
+         "upload u");
+  HIPTRY(hipStreamSynchronize(s->stream), "upload");
+  return IMX_OK;
+}
+
+template <typename T>
+static int download_state(imx_swm *s, double *h, double *u, const T *hd, const T *ud) {
+  std::vector<T> hb(s->nv), ub(s->ne);
+  HIPTRY(hipMemcpyAsync(hb.data(), hd, hb.size() * sizeof(T), hipMemcpyDeviceToHost, s->stream),
+         "download h");
+  HIPTRY(hipMemcpyAsync(ub.data(), ud, ub.size() * sizeof(T), hipMemcpyDeviceToHost, s->stream),
+         "download u");
+  HIPTRY(hipStreamSynchronize(s->stream), "download");
+  if (h)
+    for (int i = 0; i < s->nv; ++i) h[i] = (double)hb[s->cell_new[i]] + s->href;
+  if (u)
+    for (int l = 0; l < s->ne; ++l) u[l] = (double)ub[s->edge_new[l]];
+  return IMX_OK;
+}
+
+static void set_href(imx_swm *s, const double *h) {
+  if (s->href_set || !s->fast) return;
+  double sa = 0.0, sh = 0.0;
+  std::vector<double> area(s->nv);
+  // area-weighted mean of the first state (cell areas live on the device)
+  (void)hipMemcpy(area.data(), s->c_area, s->nv * sizeof(double), hipMemcpyDeviceToHost);
+  for (int i = 0; i < s->nv; ++i) {
+    const double A = area[s->cell_new[i]];
+    sa += A;
+    sh += A * h[i];
+  }
+  s->href = sh / sa;
+  s->href_set = true;
+}
+
+static void drop_graph(imx_swm *s) {
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+}
+
+static int ensure_graph(imx_swm *s) {
+  if (s->gexec) return IMX_OK;
+  HIPTRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+  launch_step(s);
+  HIPTRY(hipStreamEndCapture(s->stream, &s->graph), "graph capture end");
+  HIPTRY(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0), "graph instantiate");
+  return IMX_OK;
+}
+
+static int run_steps(imx_swm *s, int nsteps) {
+  if (s->o.use_graph) {
+    int rc = ensure_graph(s);
+    if (rc) return rc;
+    for (int k = 0; k < nsteps; ++k) HIPTRY(hipGraphLaunch(s->gexec, s->stream), "hipGraphLaunch");
+  } else {
+    for (int k = 0; k < nsteps; ++k) launch_step(s);
+    HIPTRY(hipGetLastError(), "step launch");
+  }
+  return IMX_OK;
+}
+
+}  // namespace imx
+
+// ================================================================== C ABI ==
+extern "C" {
+
+void imx_swm_default_opts(imx_swm_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->precision = IMX_PREC_FP32;
+  o->testcase = 2;
+  o->use_sfc = 1;
+  o->use_graph = 1;
+  o->erad = 6.37122e6;
+  o->grav = 9.80616;
+  o->omega = 7.292e-5;
+  o->dt = 900.0;
+  o->h_ref = 0.0;
+}
+
+int imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out) {
+  if (!d || !o || !out) return imx::set_error(IMX_EINVAL, "imx_swm_create: NULL argument");
+  *out = nullptr;
+  imx_swm *s = new (std::nothrow) imx_swm();
+  if (!s) return imx::set_error(IMX_ENOMEM, "imx_swm_create: out of memory");
+  int rc;
+  try {
+    rc = imx::create(*d, *o, s);
+  } catch (const std::bad_alloc &) {
+    rc = imx::set_error(IMX_ENOMEM, "imx_swm_create: host out of memory");
+  }
+  if (rc) {
+    imx_swm_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return IMX_OK;
+}
+
+int imx_swm_set_state(imx_swm *s, const double *h, const double *u) {
+  if (!s || !h || !u) return imx::set_error(IMX_EINVAL, "imx_swm_set_state: NULL argument");
+  if (!s->fast) return imx::upload_state<double>(s, h, u, s->rh, s->ru);
+  imx::set_href(s, h);
+  if (s->single) return imx::upload_state<float>(s, h, u, (float *)s->h0, (float *)s->u0);
+  return imx::upload_state<double>(s, h, u, (double *)s->h0, (double *)s->u0);
+}
+
+int imx_swm_get_state(imx_swm *s, double *h, double *u) {
+  if (!s) return imx::set_error(IMX_EINVAL, "imx_swm_get_state: NULL handle");
+  if (!s->fast) return imx::download_state<double>(s, h, u, s->rh, s->ru);
+  if (s->single) return imx::download_state<float>(s, h, u, (float *)s->h0, (float *)s->u0);
+  return imx::download_state<double>(s, h, u, (double *)s->h0, (double *)s->u0);
+}
+
+int imx_swm_tendency(imx_swm *s, const double *h, const double *u, double *masseq,
+                     double *momeq) {
+  if (!s || !h || !u || !masseq || !momeq)
+    return imx::set_error(IMX_EINVAL, "imx_swm_tendency: NULL argument");
+  int rc;
+  std::vector<double> mb(s->nv), fb(s->ne);
+  if (!s->fast) {
+    if ((rc = imx::upload_state<double>(s, h, u, s->rhn, s->run))) return rc;
+    imx::launch_ref_tendency(s, s->rhn, s->run, s->massf, s->momf);
+    if ((rc = imx::check_flag(s, "imx_swm_tendency"))) return rc;
+    HIPTRY(hipMemcpy(mb.data(), s->massf, s->nv * sizeof(double), hipMemcpyDeviceToHost), "tendency");
+    HIPTRY(hipMemcpy(fb.data(), s->momf, s->ne * sizeof(double), hipMemcpyDeviceToHost), "tendency");
+  } else {
+    imx::set_href(s, h);
+    if (s->single) {
+      if ((rc = imx::upload_state<float>(s, h, u, (float *)s->hA, (float *)s->uA))) return rc;
+      imx::launch_fast_stage<float>(s, -1, (const float *)s->hA, (const float *)s->uA, nullptr,
+                                    nullptr, nullptr);
+      if ((rc = imx::check_flag(s, "imx_swm_tendency"))) return rc;
+      std::vector<float> ff(s->ne);
+      HIPTRY(hipMemcpy(ff.data(), s->du_out, s->ne * sizeof(float), hipMemcpyDeviceToHost), "tendency");
+      for (int l = 0; l < s->ne; ++l) fb[l] = ff[l];
+    } else {
+      if ((rc = imx::upload_state<double>(s, h, u, (double *)s->hA, (double *)s->uA))) return rc;
+      imx::launch_fast_stage<double>(s, -1, (const double *)s->hA, (const double *)s->uA, nullptr,
+                                     nullptr, nullptr);
+      if ((rc = imx::check_flag(s, "imx_swm_tendency"))) return rc;
+      HIPTRY(hipMemcpy(fb.data(), s->du_out, s->ne * sizeof(double), hipMemcpyDeviceToHost), "tendency");
+    }
+    HIPTRY(hipMemcpy(mb.data(), s->dh_out, s->nv * sizeof(double), hipMemcpyDeviceToHost), "tendency");
+  }
+  for (int i = 0; i < s->nv; ++i) masseq[i] = mb[s->cell_new[i]];
+  for (int l = 0; l < s->ne; ++l) momeq[l] = fb[s->edge_new[l]];
+  return IMX_OK;
+}
+
+int imx_swm_get_field(imx_swm *s, int32_t which, double *out) {
+  if (!s || !out) return imx::set_error(IMX_EINVAL, "imx_swm_get_field: NULL argument");
+  if (s->fast) return imx::set_error(IMX_ENOTSUP, "imx_swm_get_field: only IMX_PREC_FP64_REF keeps the intermediates");
+  double *src = nullptr;
+  int kind = 0;  // 0 cell, 1 edge, 2 tri
+  switch (which) {
+    case IMX_FIELD_H_ED: src = s->f_h_ed; kind = 1; break;
+    case IMX_FIELD_UH: src = s->f_uh; kind = 1; break;
+    case IMX_FIELD_ETA: src = s->f_eta; kind = 2; break;
+    case IMX_FIELD_H_TR: src = s->f_h_tr; kind = 2; break;
+    case IMX_FIELD_Q_TR: src = s->f_q_tr; kind = 2; break;
+    case IMX_FIELD_Q_ED: src = s->f_q_ed; kind = 1; break;
+    case IMX_FIELD_DIVUH: src = s->f_divuh; kind = 0; break;
+    case IMX_FIELD_KE_HX: src = s->f_ke; kind = 0; break;
+    case IMX_FIELD_GHBK: src = s->f_ghbk; kind = 0; break;
+    case IMX_FIELD_GRAD_GHBK: src = s->f_grad; kind = 1; break;
+    case IMX_FIELD_UHQ_PERP: src = s->f_uhq; kind = 1; break;
+    case IMX_FIELD_DIVU: src = s->f_divu; kind = 0; break;
+    case IMX_FIELD_ZETA: src = s->f_zeta; kind = 2; break;
+    default: return imx::set_error(IMX_EINVAL, "imx_swm_get_field: unknown field");
+  }
+  const int n = kind == 0 ? s->nv : kind == 1 ? s->ne : s->nt;
+  const std::vector<int32_t> &perm = kind == 0 ? s->cell_new : kind == 1 ? s->edge_new : s->tri_new;
+  std::vector<double> b(n);
+  HIPTRY(hipMemcpy(b.data(), src, n * sizeof(double), hipMemcpyDeviceToHost), "get_field");
+  for (int i = 0; i < n; ++i) out[i] = b[perm[i]];
+  return IMX_OK;
+}
+
+int imx_swm_step(imx_swm *s, int32_t nsteps) {
+  if (!s) return imx::set_error(IMX_EINVAL, "imx_swm_step: NULL handle");
+  if (nsteps < 0) return imx::set_error(IMX_EINVAL, "imx_swm_step: nsteps < 0");
+  int rc = imx::run_steps(s, nsteps);
+  if (rc) return rc;
+  return imx::check_flag(s, "imx_swm_step");
+}
+
+int imx_swm_step_timed(imx_swm *s, int32_t nsteps, double *ms) {
+  if (!s || !ms) return imx::set_error(IMX_EINVAL, "imx_swm_step_timed: NULL argument");
+  HIPTRY(hipEventRecord(s->ev0, s->stream), "event");
+  int rc = imx::run_steps(s, nsteps);
+  if (rc) return rc;
+  HIPTRY(hipEventRecord(s->ev1, s->stream), "event");
+  HIPTRY(hipEventSynchronize(s->ev1), "event");
+  float f = 0.f;
+  HIPTRY(hipEventElapsedTime(&f, s->ev0, s->ev1), "event");
+  *ms = f;
+  return imx::check_flag(s, "imx_swm_step_timed");
+}
+
+static const char *kFastNames[3] = {"fast_tri", "fast_cell", "fast_edge"};
+
+int imx_swm_profile(imx_swm *s, int32_t nsteps, double *ms, int32_t *nk) {
+  if (!s || !ms || !nk) return imx::set_error(IMX_EINVAL, "imx_swm_profile: NULL argument");
+  if (!s->fast) return imx::set_error(IMX_ENOTSUP, "imx_swm_profile: FAST precisions only");
+  std::vector<hipEvent_t> ev(16);
+  for (auto &e : ev) HIPTRY(hipEventCreate(&e), "event");
+  double acc[3] = {0, 0, 0};
+  for (int k = 0; k < nsteps; ++k) {
+    if (s->single) imx::launch_fast_step<float>(s, &ev);
+    else imx::launch_fast_step<double>(s, &ev);
+    HIPTRY(hipEventSynchronize(ev[15]), "event");
+    for (int st = 0; st < 4; ++st)
+      for (int j = 0; j < 3; ++j) {
+        float f = 0.f;
+        HIPTRY(hipEventElapsedTime(&f, ev[4 * st + j], ev[4 * st + j + 1]), "event");
+        acc[j] += f;
+      }
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  for (int j = 0; j < 3; ++j) ms[j] = acc[j] / (4.0 * (nsteps > 0 ? nsteps : 1));
+  *nk = 3;
+  return imx::check_flag(s, "imx_swm_profile");
+}
+
+const char *imx_swm_kernel_name(imx_swm *s, int32_t k) {
+  (void)s;
+  if (k < 0 || k > 2) return "";
+  return kFastNames[k];
+}
+
+int imx_swm_diagnostics(imx_swm *s, double out[IMX_NDIAG]) {
+  if (!s || !out) return imx::set_error(IMX_EINVAL, "imx_swm_diagnostics: NULL argument");
+  if (!s->fast) imx::launch_diag<double>(s, s->rh, s->ru, 0.0);
+  else if (s->single) imx::launch_diag<float>(s, (const float *)s->h0, (const float *)s->u0, s->href);
+  else imx::launch_diag<double>(s, (const double *)s->h0, (const double *)s->u0, s->href);
+  double v[NPART];
+  HIPTRY(hipMemcpyAsync(v, s->diag, sizeof(v), hipMemcpyDeviceToHost, s->stream), "diagnostics");
+  HIPTRY(hipStreamSynchronize(s->stream), "diagnostics");
+  const double mass = v[0] / v[1];
+  const double pe = v[2];
+  const double ke = v[4];
+  // unavailable potential energy of the flat state at the mean height (b = 0)
+  const double upe = v[1] * s->o.grav * mass * (mass * 0.5);
+  out[IMX_DIAG_MASS] = mass;
+  out[IMX_DIAG_PENERGY] = pe;
+  out[IMX_DIAG_KENERGY] = ke;
+  out[IMX_DIAG_TENERGY] = pe + ke;
+  out[IMX_DIAG_AENERGY] = (pe + ke) - upe;
+  out[IMX_DIAG_PENSTROPHY] = v[5];
+  out[IMX_DIAG_RMSDIV] = std::sqrt(v[3] / v[1]);
+  out[IMX_DIAG_BADFLAG] = v[7] > 0 ? 1.0 : 0.0;
+  return IMX_OK;
+}
+
+int imx_swm_info(imx_swm *s, int64_t out[8]) {
+  if (!s || !out) return imx::set_error(IMX_EINVAL, "imx_swm_info: NULL argument");
+  out[0] = s->nv;
+  out[1] = s->ne;
+  out[2] = s->nt;
+  out[3] = (int64_t)s->A.bytes;
+  out[4] = s->o.precision;
+  out[5] = s->o.use_sfc;
+  out[6] = s->o.use_graph;
+  out[7] = 0;
+  return IMX_OK;
+}
+
+void imx_swm_destroy(imx_swm *s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  imx::drop_graph(s);
+  s->A.release();
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+}  // extern "C"

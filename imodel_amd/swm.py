@@ -1,0 +1,224 @@
+"""Python host mirroring the reference ``swm`` module over libimx.
+
+Reference call surface (src/swm.f90): ``tendency(h, u, masseq, momeq)``
+(:2416-2598), ``ode_rk4(t, h, u, h_new, u_new, dt)`` (:2600-2683) and the
+swm_tests time loop (:238-370). Errors are raised as ``capi.ImxError``
+carrying libimx's message (the reference prints and stops).
+"""
+import ctypes
+import os
+import tempfile
+
+import numpy as np
+
+from . import capi, imxbin
+
+PREC = {"fp64_ref": capi.PREC_FP64_REF, "fp64": capi.PREC_FP64, "fp32": capi.PREC_FP32}
+ERAD_REF = 6.37122e6   # constants.f90:55
+GRAV = 9.80616         # constants.f90:61
+OMEGA = 7.292e-5       # constants.f90:67
+H0_GEO = 2.94e4 / GRAV  # mean depth for gH0 = 2.94e4 m^2/s^2
+
+_SHAPES = {
+    "v_nnb": ("nv", 1), "v_nb": ("nv", 6), "v_ed": ("nv", 6), "v_tr": ("nv", 6),
+    "hx_ttgout": ("nv", 6), "hx_tnrccw": ("nv", 6), "v_p": ("nv", 3), "hx_areag": ("nv", 1),
+    "ed_v": ("ne", 2), "ed_sh": ("ne", 2), "ed_leng": ("ne", 1), "edhx_leng": ("ne", 1),
+    "ed_c": ("ne", 3), "edhx_c": ("ne", 3), "tr_v": ("nt", 3), "tr_ed": ("nt", 3),
+    "tr_tg": ("nt", 3), "tr_areag": ("nt", 1), "tr_c": ("nt", 3), "tr_clat": ("nt", 1),
+    "tr_trhx_area": ("nt", 3), "hx_hxtr_area": ("nv", 6), "hx_trskw": ("nv", 36),
+}
+
+
+class Mesh:
+    """A mesh in the reference's flat layout (dict of IMXBIN record arrays)."""
+
+    def __init__(self, arrays, handle=None):
+        self.arrays = arrays
+        self._handle = handle
+        self.nv, self.ne, self.nt = (int(x) for x in arrays["dims"][:3])
+        self.glevel = int(arrays["dims"][3]) if len(arrays["dims"]) > 3 else -1
+        self._desc = None
+
+    @classmethod
+    def icos(cls, glevel, pos="eqs", gridprop=True):
+        """Icosahedral non-optimised grid (meshbuild for icos/nopt)."""
+        L = capi.lib()
+        h = ctypes.c_void_p()
+        capi.check(L.imx_mesh_build_icos(int(glevel), capi.POS_EQS if pos == "eqs" else capi.POS_POL,
+                                         ctypes.byref(h)), "imx_mesh_build_icos")
+        if gridprop:
+            capi.check(L.imx_mesh_gridprop(h), "imx_mesh_gridprop")
+        d = capi.MeshDesc()
+        capi.check(L.imx_mesh_desc_of(h, ctypes.byref(d)), "imx_mesh_desc_of")
+        n = {"nv": d.nv, "ne": d.ne, "nt": d.nt}
+        arrays = {"dims": np.array([d.nv, d.ne, d.nt, d.glevel], dtype=np.int32)}
+        for name, (ent, w) in _SHAPES.items():
+            ptr = getattr(d, name)
+            if not ptr:
+                continue
+            arrays[name] = np.ctypeslib.as_array(ptr, shape=(n[ent] * w,)).copy()
+        m = cls(arrays, handle=h)
+        return m
+
+    @classmethod
+    def load(cls, path):
+        return cls(imxbin.read(path))
+
+    def write(self, path):
+        """Full IMXBIN record set (library meshes carry every record)."""
+        if self._handle is not None:
+            capi.check(capi.lib().imx_mesh_write(self._handle, path.encode()), "imx_mesh_write")
+        else:
+            imxbin.write(path, self.arrays)
+
+    def full_arrays(self):
+        """Every IMXBIN record, including those outside imx_mesh_desc."""
+        if self._handle is None:
+            return self.arrays
+        with tempfile.TemporaryDirectory() as td:
+            p = os.path.join(td, "m.bin")
+            self.write(p)
+            return imxbin.read(p)
+
+    def desc(self):
+        if self._desc is None:
+            self._desc = capi.make_desc(self.arrays)
+        return self._desc[0]
+
+    def tc2(self, erad=ERAD_REF):
+        """Williamson TC2 state (initialize_fields case 2) on a library mesh."""
+        if self._handle is None:
+            raise ValueError("tc2 needs a library-built mesh")
+        h = np.zeros(self.nv)
+        u = np.zeros(self.ne)
+        capi.check(capi.lib().imx_ic_tc2(self._handle, erad, capi.f64p(h), capi.f64p(u)), "imx_ic_tc2")
+        return h, u
+
+    def spectral(self, seed, lmax=16, h0=H0_GEO, hfrac=0.05, urms=20.0, erad=ERAD_REF):
+        """Seeded spectral state (h, u)."""
+        h = np.zeros(self.nv)
+        u = np.zeros(self.ne)
+        capi.check(capi.lib().imx_ic_spectral(ctypes.byref(self.desc()), int(seed), int(lmax), h0, hfrac,
+                                              urms, erad, capi.f64p(h), capi.f64p(u)), "imx_ic_spectral")
+        return h, u
+
+    def gridprop(self):
+        """initialize_gridprop products for this mesh (trhx, hxtr, trskw)."""
+        trhx = np.zeros(3 * self.nt)
+        hxtr = np.zeros(6 * self.nv)
+        trskw = np.zeros(36 * self.nv)
+        capi.check(capi.lib().imx_gridprop(ctypes.byref(self.desc()), capi.f64p(trhx), capi.f64p(hxtr),
+                                           capi.f64p(trskw)), "imx_gridprop")
+        return trhx, hxtr, trskw
+
+    def __del__(self):
+        if getattr(self, "_handle", None) is not None:
+            try:
+                capi.lib().imx_mesh_destroy(self._handle)
+            except Exception:
+                pass
+            self._handle = None
+
+
+class Swm:
+    """One shallow-water instance resident on the GPU (libimx handle)."""
+
+    def __init__(self, mesh, precision="fp32", dt=900.0, erad=ERAD_REF, grav=GRAV, omega=OMEGA,
+                 testcase=2, use_sfc=True, use_graph=True, h_ref=0.0):
+        L = capi.lib()
+        self.mesh = mesh
+        self.precision = precision
+        o = capi.SwmOpts()
+        L.imx_swm_default_opts(ctypes.byref(o))
+        o.precision = PREC[precision]
+        o.testcase = int(testcase)
+        o.use_sfc = int(bool(use_sfc))
+        o.use_graph = int(bool(use_graph))
+        o.erad, o.grav, o.omega, o.dt, o.h_ref = erad, grav, omega, float(dt), float(h_ref)
+        self.opts = o
+        self._h = ctypes.c_void_p()
+        capi.check(L.imx_swm_create(ctypes.byref(mesh.desc()), ctypes.byref(o), ctypes.byref(self._h)),
+                   "imx_swm_create")
+        self.nv, self.ne, self.nt = mesh.nv, mesh.ne, mesh.nt
+
+    def close(self):
+        if self._h:
+            capi.lib().imx_swm_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, h, u):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        capi.check(capi.lib().imx_swm_set_state(self._h, capi.f64p(h), capi.f64p(u)), "imx_swm_set_state")
+
+    def get_state(self):
+        h = np.zeros(self.nv)
+        u = np.zeros(self.ne)
+        capi.check(capi.lib().imx_swm_get_state(self._h, capi.f64p(h), capi.f64p(u)), "imx_swm_get_state")
+        return h, u
+
+    def tendency(self, h, u):
+        """(masseq, momeq) = RHS of the mass and momentum equations."""
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        m = np.zeros(self.nv)
+        f = np.zeros(self.ne)
+        capi.check(capi.lib().imx_swm_tendency(self._h, capi.f64p(h), capi.f64p(u), capi.f64p(m),
+                                               capi.f64p(f)), "imx_swm_tendency")
+        return m, f
+
+    def field(self, name):
+        n = {"v": self.nv, "e": self.ne, "t": self.nt}[capi.FIELD_POS[name]]
+        out = np.zeros(n)
+        capi.check(capi.lib().imx_swm_get_field(self._h, capi.FIELDS.index(name), capi.f64p(out)),
+                   "imx_swm_get_field")
+        return out
+
+    def step(self, nsteps=1):
+        capi.check(capi.lib().imx_swm_step(self._h, int(nsteps)), "imx_swm_step")
+
+    def step_timed(self, nsteps):
+        ms = ctypes.c_double()
+        capi.check(capi.lib().imx_swm_step_timed(self._h, int(nsteps), ctypes.byref(ms)), "imx_swm_step_timed")
+        return ms.value
+
+    def profile(self, nsteps):
+        ms = np.zeros(16)
+        nk = ctypes.c_int32()
+        capi.check(capi.lib().imx_swm_profile(self._h, int(nsteps), capi.f64p(ms), ctypes.byref(nk)),
+                   "imx_swm_profile")
+        return {capi.lib().imx_swm_kernel_name(self._h, k).decode(): ms[k] for k in range(nk.value)}
+
+    def diagnostics(self):
+        out = np.zeros(capi.NDIAG)
+        capi.check(capi.lib().imx_swm_diagnostics(self._h, capi.f64p(out)), "imx_swm_diagnostics")
+        return dict(zip(capi.DIAG_NAMES, out))
+
+    def info(self):
+        out = (ctypes.c_int64 * 8)()
+        capi.check(capi.lib().imx_swm_info(self._h, out), "imx_swm_info")
+        return list(out)
+
+
+def tendency(swm, h, u):
+    """subroutine tendency(h, u, masseq, momeq) (src/swm.f90:2416)."""
+    return swm.tendency(h, u)
+
+
+def ode_rk4(swm, h, u):
+    """subroutine ode_rk4(t, h, u, h_new, u_new, dt) (src/swm.f90:2600): one step."""
+    swm.set_state(h, u)
+    swm.step(1)
+    return swm.get_state()
