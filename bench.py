@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Throughput of the TRiSK/HTC shallow-water step on one MI355X.
+
+Workload (the glevel-8 configuration): icosahedral eqs/nopt grid, glevel 8
+(655,362 cells, 1,966,080 edges, 1,310,720 triangles), seeded spectral state
+(h' 5 % of H0 = 2.94e4/g, 20 m/s rms wind), RK4 with dt = 112.5 s, FP32
+stencil with FP64 geometry, FP64 compensated mass-flux divergence and FP64
+reductions. A "step" is one RK4 step; --steps 768 is one simulated day.
+
+Prints ONE JSON line: cell-steps/s, the HBM roofline of the step (CANON-1
+algorithmic bytes per step / measured step time vs 8 TB/s), per-kernel
+timings, accuracy against the FP64 path (bit-identical to the reference) and
+the reference CPU path timed on this host's cores.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def canon_step_bytes(nv, ne, nt, fp64=False):
+    """CANON-1 algorithmic bytes of one RK4 step (each array once per stage)."""
+    P = 30 * nv - 120
+    if fp64:
+        return 4 * (40 * nv + 66 * ne + 55 * nt + 12 * P) + 48 * (nv + ne)
+    return 4 * (24 * nv + 50 * ne + 35 * nt + 8 * P) + 24 * (nv + ne)
+
+
+def impl_kernel_bytes(nv, ne, nt):
+    """Bytes each FP32 fused kernel must move once per launch with its layout
+    (every table and state array read or written exactly once)."""
+    # triangle: 2 vertex ids, edge bits, 3 eta coefficients, 3 kite weights,
+    #           f, q out; gathers u (edges) and h (cells) once
+    # cell:     6 neighbour ids, 2 bit words, 6 kite areas, 1/(A a) (FP64),
+    #           K normaliser, h/h0/hout/acc/phi; per edge FP64 dual length, K
+    #           weight, u and the two Coriolis partials out; q per triangle
+    # edge:     other-cell id, 1/l, 2 partials, u0, acc (r+w), u out; phi
+    return {
+        "fast_tri": nt * (8 + 4 + 12 + 12 + 4 + 4) + ne * 4 + nv * 4,
+        "fast_cell": nv * (24 + 4 + 4 + 24 + 8 + 4 + 4 + 4 + 8 + 4 + 4) + ne * (8 + 4 + 4 + 8) + nt * 4,
+        "fast_edge": ne * (4 + 4 + 4 + 4 + 4 + 8 + 4) + nv * 4,
+    }
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(mesh, h0, u0, dt, nsteps):
+    """The reference Fortran loop (oracle/_ref, built from the reference
+    sources) on this host's cores; falls back to the C oracle."""
+    from oracle import oracle as orc
+    from imodel_amd import imxbin
+
+    threads = os.cpu_count() or 1
+    if orc.ref_harness_available():
+        with tempfile.TemporaryDirectory() as td:
+            mesh.write(os.path.join(td, "mesh.bin"))
+            imxbin.write(os.path.join(td, "state.bin"), {"h": h0, "u": u0})
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                       OMP_PLACES="cores")
+            subprocess.run([orc.REF_HARNESS, "run", "mesh.bin", "state.bin", str(nsteps), repr(dt), "0",
+                            "out.bin"], cwd=td, env=env, check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, timeout=900)
+            r = imxbin.read(os.path.join(td, "out.bin"))
+        wall, nthr = float(r["wall"][0]), int(r["wall"][1])
+        kind = "reference"
+        what = "reference ode_rk4 loop (src/swm.f90:238-370) built by oracle/Makefile.ref " \
+               "(amdflang -O2 -fopenmp -ffp-contract=off)"
+    else:
+        o = orc.Oracle(mesh.arrays)
+        t0 = time.perf_counter()
+        o.step(h0, u0, dt, nsteps)
+        wall, nthr, kind = time.perf_counter() - t0, threads, "port"
+        what = "C oracle (oracle/swm_oracle.c, OpenMP)"
+    return {"value": mesh.nv * nsteps / wall, "unit": "cell-steps/s", "cores": nthr, "kind": kind,
+            "sample": f"glevel {mesh.glevel}, {nsteps} RK4 steps of dt={dt} s from the same seeded "
+                      f"state, FP64, {what}; {wall:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=768)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--glevel", type=int, default=8)
+    ap.add_argument("--dt", type=float, default=112.5)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp64_ref"])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design (DESIGN.md), not implemented", file=sys.stderr)
+        sys.exit(2)
+
+    from imodel_amd.swm import Mesh, Swm
+
+    t = time.perf_counter()
+    mesh = Mesh.icos(args.glevel)
+    h0, u0 = mesh.spectral(args.seed)
+    log(f"mesh g{args.glevel}: nv={mesh.nv} ne={mesh.ne} nt={mesh.nt} ({time.perf_counter() - t:.1f} s)")
+    nv, ne, nt = mesh.nv, mesh.ne, mesh.nt
+
+    s = Swm(mesh, precision=args.precision, dt=args.dt)
+    s.set_state(h0, u0)
+    s.step(args.warmup)
+    s.set_state(h0, u0)
+    t0 = time.perf_counter()
+    ev_ms = s.step_timed(args.steps)
+    wall = time.perf_counter() - t0
+    hK, uK = s.get_state()
+    diag = s.diagnostics()
+    kern = s.profile(args.profile_steps) if args.precision != "fp64_ref" else {}
+    s.close()
+    ms_step = 1e3 * wall / args.steps
+    ev_step = ev_ms / args.steps
+    value = nv * args.steps / wall
+    log(f"{args.steps} steps: wall {wall * 1e3:.2f} ms, events {ev_ms:.2f} ms, {value:.3e} cell-steps/s")
+
+    fp64 = args.precision != "fp32"
+    B = canon_step_bytes(nv, ne, nt, fp64)
+    achieved = B / (ev_step * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_g{args.glevel}_{args.precision}.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_step")
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "kernel": "RK4 step (one hipGraph launch: 4 x fast_tri, fast_cell, fast_edge)",
+                "algorithmic_bytes_per_launch": B, "launch_ms": ev_step}
+    kernels = {}
+    if kern:
+        kb = impl_kernel_bytes(nv, ne, nt)
+        for k, ms in kern.items():
+            kernels[k] = {"ms": ms, "impl_bytes": kb.get(k), "GBs": kb.get(k, 0) / (ms * 1e-3) / 1e9}
+
+    accuracy = None
+    if not args.no_accuracy and args.precision == "fp32":
+        with Swm(mesh, precision="fp64_ref", dt=args.dt) as r:
+            r.set_state(h0, u0)
+            r.step(args.steps)
+            hR, uR = r.get_state()
+            dR = r.diagnostics()
+        A = mesh.arrays["hx_areag"]
+        L = mesh.arrays["ed_leng"] * mesh.arrays["edhx_leng"] * 0.5
+        m0 = np.sum(A * h0)
+        accuracy = {
+            "vs": "FP64 GPU path bit-identical to the reference (same state, same steps)",
+            "h_l2_rel": float(np.sqrt(np.sum(A * (hK - hR) ** 2) / np.sum(A * hR ** 2))),
+            "h_linf_rel": float(np.max(np.abs(hK - hR)) / np.max(np.abs(hR))),
+            "hprime_l2_rel": float(np.sqrt(np.sum(A * (hK - hR) ** 2) / np.sum(A * (hR - hR.mean()) ** 2))),
+            "u_l2_rel": float(np.sqrt(np.sum(L * (uK - uR) ** 2) / np.sum(L * uR ** 2))),
+            "mass_drift": float((np.sum(A * hK) - m0) / m0),
+            "mass_drift_fp64": float((np.sum(A * hR) - m0) / m0),
+            "simulated_s": args.steps * args.dt,
+        }
+        log("accuracy", accuracy)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(mesh, h0, u0, args.dt, args.cpu_steps)
+            log("cpu baseline", cpu)
+        except Exception as e:  # reported, never silently replaced
+            cpu = {"value": None, "error": str(e)[:300]}
+
+    out = {
+        "metric": "cell-steps/s at glevel 8 FP32 vs FP64; time-to-accuracy learned vs plain",
+        "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic (seeded spectral state, spherical harmonics l=1..16)",
+        "config": {"workload": f"TRiSK/HTC SWM RK4 step, icos eqs nopt glevel {args.glevel}",
+                   "glevel": args.glevel, "cells": nv, "edges": ne, "triangles": nt, "dt": args.dt,
+                   "precision": args.precision, "seed": args.seed},
+        "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels, "accuracy": accuracy,
+        "diagnostics_end": {k: float(v) for k, v in diag.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

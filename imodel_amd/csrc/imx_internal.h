@@ -38,9 +38,14 @@ double det3(const double *p1, const double *p2, const double *p3);
 void cart2sph(double x, double y, double z, double *lon, double *lat);
 
 void gridprop(const imx_mesh_desc &d, double *trhx, double *hxtr, double *trskw);
-// Caller (0-based) -> internal index maps along a Hilbert curve (or identity).
-void sfc_order(const imx_mesh_desc &d, bool use_sfc, std::vector<int32_t> &cell_new,
-               std::vector<int32_t> &edge_new, std::vector<int32_t> &tri_new);
+// Internal numbering: cells along a Hilbert curve (or caller order); edge of
+// owning cell c -> 3c+k, triangle of owning cell c -> 2c+k. Maps are caller
+// (0-based) -> internal; NE = 3 nv and NT = 2 nv slots (some are dummies).
+struct Layout {
+  std::vector<int32_t> cell_new, edge_new, tri_new;
+  int NE = 0, NT = 0;
+};
+bool build_layout(const imx_mesh_desc &d, bool use_sfc, Layout &L, std::string &err);
 void desc_of(const HostMesh &m, imx_mesh_desc &d);
 
 // IMXBIN container: "IMXBIN01", then records (name[24], dtype[4], int64 n, data).

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(TPB) void ref_edge_flux(int ne, const int32_t *__re
                                                      double *__restrict__ h_ed,
                                                      double *__restrict__ uh) {
   const int l = xcd_index();
-  if (l >= ne) return;
+  if (l >= ne || e_v[l] < 0) return;
   const double he = (h[e_v[l]] + h[e_v[ne + l]]) * 0.5;
   h_ed[l] = he;
   uh[l] = u[l] * he;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(TPB) void ref_tri(int nt, const int32_t *__restrict
                                                double *__restrict__ h_tr,
                                                double *__restrict__ q_tr, int *__restrict__ flag) {
   const int k = xcd_index();
-  if (k >= nt) return;
+  if (k >= nt || t_ed[k] < 0) return;
   double e = 0.0;
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(TPB) void ref_edge_q(int ne, const int32_t *__restr
                                                   const double *__restrict__ q_tr,
                                                   double *__restrict__ q_ed) {
   const int l = xcd_index();
-  if (l >= ne) return;
+  if (l >= ne || e_sh[l] < 0) return;
   q_ed[l] = 0.5 * (q_tr[e_sh[l]] + q_tr[e_sh[ne + l]]);
 }
 
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(TPB) void ref_edge_mom(
     int zero_mom, double *__restrict__ grad, double *__restrict__ uhq,
     double *__restrict__ momeq) {
   const int l = xcd_index();
-  if (l >= ne) return;
+  if (l >= ne || e_v[l] < 0) return;
   const double g = (ghbk[e_v[ne + l]] - ghbk[e_v[l]]) / e_leng[l] / erad;
   grad[l] = g;
   double s = 0.0;
@@ -208,37 +208,57 @@ __global__ __launch_bounds__(TPB) void ref_rk_final(int n, double *__restrict__ 
 }
 
 // =========================================================== FAST kernels ==
-// Cell bit field: nnb (bits 0-2) | ttgout<0 (3-8) | tnrccw<0 (9-14) |
-//                 cell is the edge's 2nd end (15-20)
-#define BIT_TTG 3
-#define BIT_TNR 9
-#define BIT_SIDE 15
+// Internal numbering: edge of owning cell c is 3c+k, triangle of owning cell
+// c is 2c+k (k < 3, k < 2). A cell rebuilds the ids of its ring edges and
+// triangles from the neighbour ids it holds plus per-slot bits:
+//   bitsA: nnb (0-2) | ttgout<0 (3-8) | tnrccw<0 (9-14) | cell owns edge j (15-20)
+//   bitsB: k of edge j (2 bits at 2j) | owner of triangle j: 0 cell, 1 nb_j,
+//          2 nb_{j+1} (2 bits at 12+2j) | k of triangle j (bit 24+j)
+// Pentagons carry a padded 6th slot repeating slot 0, masked out of all sums,
+// so every lane issues the same independent gathers and no load sits behind
+// a branch.
+#define BA_TTG 3
+#define BA_TNR 9
+#define BA_OWN 15
+#define BB_TSEL 12
+#define BB_KT 24
+#define OTH_MASK 0x3FFFFFFF
+#define OTH_V2 30
 
 template <typename T>
-__global__ __launch_bounds__(TPB) void fast_tri(int nt, const int32_t *__restrict__ t_ed,
-                                                const int32_t *__restrict__ t_v,
+__global__ __launch_bounds__(TPB) void fast_tri(int NT, const int32_t *__restrict__ t_ab,
+                                                const int32_t *__restrict__ t_eb,
                                                 const T *__restrict__ t_ceta,
                                                 const T *__restrict__ t_kap,
                                                 const T *__restrict__ t_f,
                                                 const T *__restrict__ u,
                                                 const T *__restrict__ h, T href, int use_pv,
                                                 T *__restrict__ q, int *__restrict__ flag) {
-  const int k = xcd_index();
-  if (k >= nt) return;
-  T eta = t_f[k];
-  T hs = (T)0;
-#pragma unroll
-  for (int l = 0; l < 3; ++l) {
-    eta += u[t_ed[l * nt + k]] * t_ceta[l * nt + k];
-    hs += t_kap[l * nt + k] * h[t_v[l * nt + k]];
-  }
-  const T ht = href + hs;
+  const int t = xcd_index();
+  if (t >= NT) return;
+  const int a = t_ab[t];
+  if (a < 0) return;
+  const int o = t >> 1;
+  const int b = t_ab[NT + t];
+  const int eb = t_eb[t];
+  // edges (o,a), (a,b), (b,o): bit 0 of each 3-bit field = first vertex owns it
+  const int e0 = 3 * ((eb & 1) ? o : a) + ((eb >> 1) & 3);
+  const int e1 = 3 * (((eb >> 3) & 1) ? a : b) + ((eb >> 4) & 3);
+  const int e2 = 3 * (((eb >> 6) & 1) ? b : o) + ((eb >> 7) & 3);
+  T eta = t_f[t];
+  eta += u[e0] * t_ceta[t];
+  eta += u[e1] * t_ceta[NT + t];
+  eta += u[e2] * t_ceta[2 * NT + t];
+  T hsum = t_kap[t] * h[o];
+  hsum += t_kap[NT + t] * h[a];
+  hsum += t_kap[2 * NT + t] * h[b];
+  const T ht = href + hsum;
   T qq = (T)0;
   if (use_pv) {
     if (fabs((double)ht) < HTR_GUARD) flag[0] = 1;
     else qq = eta / ht;
   }
-  q[k] = qq;
+  q[t] = qq;
 }
 
 // Neumaier's compensated summation step.
@@ -251,82 +271,89 @@ __device__ __forceinline__ void kadd(double &s, double &c, double x) {
 
 template <typename T>
 __global__ __launch_bounds__(TPB) void fast_cell(
-    int nv, int ne, const int32_t *__restrict__ c_ed, const int32_t *__restrict__ c_nb,
-    const int32_t *__restrict__ c_tr, const int32_t *__restrict__ c_bits,
-    const T *__restrict__ c_R, const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
+    int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
+    const int32_t *__restrict__ c_bitsB, const T *__restrict__ c_R,
+    const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
     const double *__restrict__ e_dl, const T *__restrict__ e_kw, const T *__restrict__ q,
     const T *__restrict__ hs, const T *__restrict__ us, double href, T grav, T inv_a,
-    T *__restrict__ phi, T *__restrict__ Pe, int stage, double dt,
-    const T *h0, T *hout, T *__restrict__ acch,
-    double *__restrict__ dh_out) {
+    T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt,
+    const T *h0, T *hout, T *__restrict__ acch, double *__restrict__ dh_out) {
   const int c = xcd_index();
   if (c >= nv) return;
-  const int bits = c_bits[c];
-  const int n = bits & 7;
-  const T hc = hs[c];
-  T G[MX], qt[MX], qe[MX], R[MX];
-  int ed[MX];
-  double dv = 0.0, dvc = 0.0;
-  T ke = (T)0;
+  const int A = c_bitsA[c];
+  const int B = c_bitsB[c];
+  const bool hex = ((A & 7) == MX);
+  int nb[MX];
+#pragma unroll
+  for (int j = 0; j < MX; ++j) nb[j] = c_nb[j * nv + c];
+  T R[MX];
+#pragma unroll
+  for (int j = 0; j < MX; ++j) R[j] = c_R[j * nv + c];
+  int ed[MX], tr[MX];
 #pragma unroll
   for (int j = 0; j < MX; ++j) {
-    G[j] = (T)0;
-    qt[j] = (T)0;
-    R[j] = (T)0;
-    ed[j] = 0;
-    if (j < n) {
-      const int e = c_ed[j * nv + c];
-      ed[j] = e;
-      const T ue = us[e];
-      const T hn = hs[c_nb[j * nv + c]];
-      const double he = href + 0.5 * ((double)hc + (double)hn);
-      double g = (double)ue * he * e_dl[e];
-      if ((bits >> (BIT_TTG + j)) & 1) g = -g;
-      kadd(dv, dvc, g);
-      G[j] = (T)g;
-      ke += e_kw[e] * ue * ue;
-      qt[j] = q[c_tr[j * nv + c]];
-      R[j] = c_R[j * nv + c];
-    }
+    const int eo = ((A >> (BA_OWN + j)) & 1) ? c : nb[j];
+    ed[j] = 3 * eo + ((B >> (2 * j)) & 3);
+    const int ts = (B >> (BB_TSEL + 2 * j)) & 3;
+    const int to = (ts == 0) ? c : (ts == 1) ? nb[j] : nb[(j + 1) % MX];
+    tr[j] = 2 * to + ((B >> (BB_KT + j)) & 1);
+  }
+  const T hc = hs[c];
+  T ue[MX], hn[MX], qt[MX], kw[MX];
+  double dl[MX];
+#pragma unroll
+  for (int j = 0; j < MX; ++j) {
+    ue[j] = us[ed[j]];
+    hn[j] = hs[nb[j]];
+    qt[j] = q[tr[j]];
+    dl[j] = e_dl[ed[j]];
+    kw[j] = e_kw[ed[j]];
+  }
+  double dv = 0.0, dvc = 0.0;
+  T ke = (T)0;
+  T G[MX];
+#pragma unroll
+  for (int j = 0; j < MX; ++j) {
+    const bool valid = (j < MX - 1) || hex;
+    const double he = href + 0.5 * ((double)hc + (double)hn[j]);
+    double g = (double)ue[j] * he * dl[j];
+    if ((A >> (BA_TTG + j)) & 1) g = -g;
+    if (!valid) g = 0.0;
+    kadd(dv, dvc, g);
+    G[j] = (T)g;
+    ke += valid ? kw[j] * ue[j] * ue[j] : (T)0;
   }
   const double dh = -(dv + dvc) * c_iaa[c];
   const T K = ke * c_ika[c];
   phi[c] = (grav * hc + K) * inv_a;
 
   // PV at the cell's edges: edge j lies between ring triangles j-1 and j
+  T qe[MX];
+  qe[0] = (T)0.5 * ((hex ? qt[MX - 1] : qt[MX - 2]) + qt[0]);
 #pragma unroll
-  for (int j = 0; j < MX; ++j) {
-    const int jm = (j == 0) ? n - 1 : j - 1;
-    T qm = (T)0;
-#pragma unroll
-    for (int t = 0; t < MX; ++t)
-      if (t == jm) qm = qt[t];
-    qe[j] = (j < n) ? (T)0.5 * (qm + qt[j]) : (T)0;
-  }
-  // prefix sums of kite areas: w(i,j) = S_j - S_i - 1/2 (j>i), tot - S_i + S_j - 1/2 (j<i)
+  for (int j = 1; j < MX; ++j) qe[j] = (T)0.5 * (qt[j - 1] + qt[j]);
+  // TRiSK weights from prefix sums of the kite areas:
+  // w(i,j) = S_j - S_i - 1/2 (j>i), tot - S_i + S_j - 1/2 (j<i), w(i,i) = 0
   T S[MX + 1];
   S[0] = (T)0;
 #pragma unroll
   for (int j = 0; j < MX; ++j) S[j + 1] = S[j] + R[j];
-  T tot = (T)0;
-#pragma unroll
-  for (int j = 0; j <= MX; ++j)
-    if (j == n) tot = S[j];
+  const T tot = S[MX];
 #pragma unroll
   for (int i = 0; i < MX; ++i) {
-    if (i < n) {
-      T acc = (T)0;
+    T acc = (T)0;
 #pragma unroll
-      for (int j = 0; j < MX; ++j) {
-        if (j < n && j != i) {
-          const T w = ((j > i) ? (S[j] - S[i]) : (tot - S[i] + S[j])) - (T)0.5;
-          acc += G[j] * (qe[i] + qe[j]) * w;
-        }
+    for (int j = 0; j < MX; ++j) {
+      if (j != i) {
+        const T w = ((j > i) ? (S[j] - S[i]) : (tot - S[i] + S[j])) - (T)0.5;
+        acc += G[j] * (qe[i] + qe[j]) * w;
       }
-      T P = (T)0.5 * acc;
-      if ((bits >> (BIT_TNR + i)) & 1) P = -P;
-      const int side = (bits >> (BIT_SIDE + i)) & 1;
-      Pe[side * ne + ed[i]] = P;
+    }
+    T P = (T)0.5 * acc;
+    if ((A >> (BA_TNR + i)) & 1) P = -P;
+    if (i < MX - 1 || hex) {
+      if ((A >> (BA_OWN + i)) & 1) Pown[ed[i]] = P;
+      else Poth[ed[i]] = P;
     }
   }
 
@@ -344,16 +371,22 @@ __global__ __launch_bounds__(TPB) void fast_cell(
 }
 
 template <typename T>
-__global__ __launch_bounds__(TPB) void fast_edge(int ne, const int32_t *__restrict__ e_c,
+__global__ __launch_bounds__(TPB) void fast_edge(int NE, const int32_t *__restrict__ e_oth,
                                                  const T *__restrict__ e_il,
                                                  const T *__restrict__ phi,
-                                                 const T *__restrict__ Pe, int zero_mom,
-                                                 int stage, T dt, const T *u0,
-                                                 T *uout, T *__restrict__ accu,
-                                                 T *__restrict__ du_out) {
+                                                 const T *__restrict__ Pown,
+                                                 const T *__restrict__ Poth, int zero_mom,
+                                                 int stage, T dt, const T *u0, T *uout,
+                                                 T *__restrict__ accu, T *__restrict__ du_out) {
   const int e = xcd_index();
-  if (e >= ne) return;
-  T du = -((phi[e_c[ne + e]] - phi[e_c[e]]) + (Pe[e] + Pe[ne + e])) * e_il[e];
+  if (e >= NE) return;
+  const int oth = e_oth[e];
+  if (oth < 0) return;
+  const int o = e / 3;
+  const int c2 = oth & OTH_MASK;
+  const bool own_v2 = (oth >> OTH_V2) & 1;
+  const int v1 = own_v2 ? c2 : o, v2 = own_v2 ? o : c2;
+  T du = -((phi[v2] - phi[v1]) + (Pown[e] + Poth[e])) * e_il[e];
   if (zero_mom) du = (T)0;
   if (stage < 0) {
     du_out[e] = du;
@@ -435,7 +468,7 @@ __global__ __launch_bounds__(TPB) void diag_edge(int ne, const int32_t *__restri
                                                  double *__restrict__ part) {
   double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int l = blockIdx.x * TPB + threadIdx.x;
-  if (l < ne) {
+  if (l < ne && e_v[l] >= 0) {
     const double he = (href + (double)hs[e_v[l]] + href + (double)hs[e_v[ne + l]]) * 0.5;
     const double u = (double)us[l];
     v[4] = 0.5 * e_leng[l] * e_dleng[l] * he * (u * u);
@@ -458,7 +491,7 @@ __global__ __launch_bounds__(TPB) void diag_tri(int nt, const int32_t *__restric
                                                 double erad, double *__restrict__ part) {
   double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int k = blockIdx.x * TPB + threadIdx.x;
-  if (k < nt) {
+  if (k < nt && t_ed[k] >= 0) {
     double e = 0.0, s = 0.0;
     for (int l = 0; l < 3; ++l) {
       const int ed = t_ed[l * nt + k];
@@ -510,6 +543,8 @@ struct Alloc {
       err = e;
       return nullptr;
     }
+    e = hipMemset(p, 0, n * sizeof(X));  // dummy slots read as zeros
+    if (e != hipSuccess) err = e;
     ptrs.push_back(p);
     bytes += n * sizeof(X);
     return (X *)p;
@@ -529,16 +564,13 @@ struct Alloc {
   }
 };
 
-struct KTimer {
-  const char *name;
-};
-
 }  // namespace
 
 struct imx_swm {
   imx_swm_opts o{};
-  int nv = 0, ne = 0, nt = 0;
-  std::vector<int32_t> cell_new, edge_new, tri_new;  // caller -> internal
+  int nv = 0, ne = 0, nt = 0;  // caller counts
+  int NE = 0, NT = 0;          // internal edge/triangle slots (3 nv, 2 nv)
+  imx::Layout lay;             // caller -> internal maps
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -549,7 +581,7 @@ struct imx_swm {
   double href = 0.0;
   bool href_set = false;
 
-  // geometry / REF tables (always present)
+  // geometry / REF tables (always present), internal numbering
   int32_t *c_nnb = nullptr, *c_ed = nullptr;
   int8_t *c_ttg = nullptr, *c_tnr = nullptr;
   double *c_area = nullptr, *c_trskw = nullptr;
@@ -573,16 +605,17 @@ struct imx_swm {
          *f_zeta = nullptr;
 
   // FAST tables and state (typed by precision; void* holds T*)
-  int32_t *x_c_ed = nullptr, *x_c_nb = nullptr, *x_c_tr = nullptr, *x_c_bits = nullptr;
+  int32_t *x_c_nb = nullptr, *x_c_bitsA = nullptr, *x_c_bitsB = nullptr;
   void *x_c_R = nullptr, *x_c_ika = nullptr;
   double *x_c_iaa = nullptr;
   double *x_e_dl = nullptr;
   void *x_e_kw = nullptr, *x_e_il = nullptr;
-  int32_t *x_e_c = nullptr;
-  int32_t *x_t_ed = nullptr, *x_t_v = nullptr;
+  int32_t *x_e_oth = nullptr;
+  int32_t *x_t_ab = nullptr, *x_t_eb = nullptr;
   void *x_t_ceta = nullptr, *x_t_kap = nullptr, *x_t_f = nullptr;
   void *h0 = nullptr, *hA = nullptr, *hB = nullptr, *u0 = nullptr, *uA = nullptr, *uB = nullptr;
-  void *acch = nullptr, *accu = nullptr, *q = nullptr, *phi = nullptr, *Pe = nullptr;
+  void *acch = nullptr, *accu = nullptr, *q = nullptr, *phi = nullptr, *Pown = nullptr,
+       *Poth = nullptr;
   double *dh_out = nullptr;
   void *du_out = nullptr;
 };
@@ -593,9 +626,9 @@ static int hip_fail(hipError_t e, const char *what) {
   return set_error(IMX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIPTRY(expr, what)                 \
-  do {                                     \
-    hipError_t _e = (expr);                \
+#define HIPTRY(expr, what)                                 \
+  do {                                                     \
+    hipError_t _e = (expr);                                \
     if (_e != hipSuccess) return imx::hip_fail(_e, what); \
   } while (0)
 
@@ -607,40 +640,40 @@ static std::vector<T> cvt(const std::vector<double> &a) {
 // --------------------------------------------------------------- launches --
 static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, double *masseq,
                                 double *momeq) {
-  const int nv = s->nv, ne = s->ne, nt = s->nt;
+  const int nv = s->nv, NE = s->NE, NT = s->NT;
   hipStream_t st = s->stream;
-  ref_edge_flux<<<grid_for(ne), TPB, 0, st>>>(ne, s->e_v, h, u, s->f_h_ed, s->f_uh);
-  ref_tri<<<grid_for(nt), TPB, 0, st>>>(nt, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f,
+  ref_edge_flux<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, h, u, s->f_h_ed, s->f_uh);
+  ref_tri<<<grid_for(NT), TPB, 0, st>>>(NT, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f,
                                         s->t_trhx, s->e_leng, u, h, s->o.erad,
                                         s->o.testcase >= 2, s->f_eta, s->f_zeta, s->f_h_tr,
                                         s->f_q_tr, s->flag);
-  ref_edge_q<<<grid_for(ne), TPB, 0, st>>>(ne, s->e_sh, s->f_q_tr, s->f_q_ed);
+  ref_edge_q<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_sh, s->f_q_tr, s->f_q_ed);
   ref_cell<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area, s->e_leng,
                                          s->e_dleng, u, s->f_uh, h, s->bt, s->o.erad, s->o.grav,
                                          s->f_divuh, masseq, s->f_divu, s->f_ke, s->f_ghbk);
-  ref_edge_mom<<<grid_for(ne), TPB, 0, st>>>(ne, nv, s->e_v, s->e_loc, s->e_leng, s->e_dleng,
+  ref_edge_mom<<<grid_for(NE), TPB, 0, st>>>(NE, nv, s->e_v, s->e_loc, s->e_leng, s->e_dleng,
                                              s->c_nnb, s->c_ed, s->c_ttg, s->c_tnr, s->c_trskw,
                                              s->f_ghbk, s->f_uh, s->f_q_ed, s->o.erad,
                                              s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
 }
 
 static void launch_ref_step(imx_swm *s) {
-  const int nv = s->nv, ne = s->ne;
+  const int nv = s->nv, NE = s->NE;
   const double dt = s->o.dt;
   hipStream_t st = s->stream;
   double *m[4], *f[4];
   for (int k = 0; k < 4; ++k) {
     m[k] = s->massf + (size_t)k * nv;
-    f[k] = s->momf + (size_t)k * ne;
+    f[k] = s->momf + (size_t)k * NE;
   }
   launch_ref_tendency(s, s->rh, s->ru, m[0], f[0]);
   for (int k = 0; k < 3; ++k) {
     const int half = (k < 2);
-    ref_rk_stage<<<grid_for(ne), TPB, 0, st>>>(ne, s->ru, f[k], dt, half, s->run);
+    ref_rk_stage<<<grid_for(NE), TPB, 0, st>>>(NE, s->ru, f[k], dt, half, s->run);
     ref_rk_stage<<<grid_for(nv), TPB, 0, st>>>(nv, s->rh, m[k], dt, half, s->rhn);
     launch_ref_tendency(s, s->rhn, s->run, m[k + 1], f[k + 1]);
   }
-  ref_rk_final<<<grid_for(ne), TPB, 0, st>>>(ne, s->ru, f[0], f[1], f[2], f[3], dt);
+  ref_rk_final<<<grid_for(NE), TPB, 0, st>>>(NE, s->ru, f[0], f[1], f[2], f[3], dt);
   ref_rk_final<<<grid_for(nv), TPB, 0, st>>>(nv, s->rh, m[0], m[1], m[2], m[3], dt);
 }
 
@@ -648,23 +681,24 @@ static void launch_ref_step(imx_swm *s) {
 template <typename T>
 static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T *hout,
                               T *uout, hipEvent_t *evs) {
-  const int nv = s->nv, ne = s->ne, nt = s->nt;
+  const int nv = s->nv, NE = s->NE, NT = s->NT;
   hipStream_t st = s->stream;
   const T href = (T)s->href;
   if (evs) (void)hipEventRecord(evs[0], st);
-  fast_tri<T><<<grid_for(nt), TPB, 0, st>>>(nt, s->x_t_ed, s->x_t_v, (const T *)s->x_t_ceta,
+  fast_tri<T><<<grid_for(NT), TPB, 0, st>>>(NT, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta,
                                             (const T *)s->x_t_kap, (const T *)s->x_t_f, us, hs,
                                             href, s->o.testcase >= 2, (T *)s->q, s->flag);
   if (evs) (void)hipEventRecord(evs[1], st);
   fast_cell<T><<<grid_for(nv), TPB, 0, st>>>(
-      nv, ne, s->x_c_ed, s->x_c_nb, s->x_c_tr, s->x_c_bits, (const T *)s->x_c_R, s->x_c_iaa,
+      nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_R, s->x_c_iaa,
       (const T *)s->x_c_ika, s->x_e_dl, (const T *)s->x_e_kw, (const T *)s->q, hs, us, s->href,
-      (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pe, stage, s->o.dt,
-      (const T *)s->h0, hout, (T *)s->acch, s->dh_out);
+      (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,
+      s->o.dt, (const T *)s->h0, hout, (T *)s->acch, s->dh_out);
   if (evs) (void)hipEventRecord(evs[2], st);
-  fast_edge<T><<<grid_for(ne), TPB, 0, st>>>(ne, s->x_e_c, (const T *)s->x_e_il, (const T *)s->phi,
-                                             (const T *)s->Pe, s->o.testcase <= 1, stage,
-                                             (T)s->o.dt, (const T *)s->u0, uout, (T *)s->accu,
+  fast_edge<T><<<grid_for(NE), TPB, 0, st>>>(NE, s->x_e_oth, (const T *)s->x_e_il, (const T *)s->phi,
+                                             (const T *)s->Pown, (const T *)s->Poth,
+                                             s->o.testcase <= 1, stage, (T)s->o.dt,
+                                             (const T *)s->u0, uout, (T *)s->accu,
                                              (T *)s->du_out);
   if (evs) (void)hipEventRecord(evs[3], st);
 }
@@ -691,105 +725,141 @@ static void launch_step(imx_swm *s) {
 template <typename T>
 static void launch_diag(imx_swm *s, const T *hs, const T *us, double href) {
   hipStream_t st = s->stream;
-  const int gv = (s->nv + TPB - 1) / TPB, ge = (s->ne + TPB - 1) / TPB,
-            gt = (s->nt + TPB - 1) / TPB;
+  const int gv = (s->nv + TPB - 1) / TPB, ge = (s->NE + TPB - 1) / TPB,
+            gt = (s->NT + TPB - 1) / TPB;
   diag_cell<T><<<gv, TPB, 0, st>>>(s->nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area, s->e_dleng, hs,
                                    us, href, s->o.grav, s->o.erad, s->part);
-  diag_edge<T><<<ge, TPB, 0, st>>>(s->ne, s->e_v, s->e_leng, s->e_dleng, hs, us, href,
+  diag_edge<T><<<ge, TPB, 0, st>>>(s->NE, s->e_v, s->e_leng, s->e_dleng, hs, us, href,
                                    s->part + (size_t)gv * NPART);
-  diag_tri<T><<<gt, TPB, 0, st>>>(s->nt, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f, s->t_trhx,
+  diag_tri<T><<<gt, TPB, 0, st>>>(s->NT, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f, s->t_trhx,
                                   s->e_leng, hs, us, href, s->o.erad,
                                   s->part + (size_t)(gv + ge) * NPART);
   diag_final<<<1, TPB, 0, st>>>(gv + ge + gt, s->part, s->diag);
 }
 
 // ------------------------------------------------------------ create -------
+// Tables of the fused kernels in the ownership numbering (see fast_cell).
 template <typename T>
 static int build_fast(imx_swm *s, const imx_mesh_desc &d, const double *trhx, const double *hxtr) {
-  const int nv = s->nv, ne = s->ne, nt = s->nt;
+  const int nv = s->nv, NE = s->NE, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
   const double a = s->o.erad;
-  std::vector<int32_t> c_ed((size_t)MX * nv, 0), c_nb((size_t)MX * nv, 0), c_tr((size_t)MX * nv, 0),
-      c_bits(nv, 0);
+  std::vector<int32_t> c_nb((size_t)MX * nv, 0), bitsA(nv, 0), bitsB(nv, 0);
   std::vector<double> c_R((size_t)MX * nv, 0.0), c_iaa(nv), c_ika(nv);
   for (int co = 0; co < nv; ++co) {
-    const int c = s->cell_new[co];
+    const int c = cn[co];
     const int n = d.v_nnb[co];
-    int bits = n;
+    if (n < MX - 1) return set_error(IMX_ENOTSUP, "fused kernels need cells with 5 or 6 edges");
+    int A = n, B = 0;
     double ksum = 0.0;
-    for (int j = 0; j < n; ++j) {
-      const int eo = d.v_ed[MX * co + j] - 1;
-      c_ed[(size_t)j * nv + c] = s->edge_new[eo];
-      c_nb[(size_t)j * nv + c] = s->cell_new[d.v_nb[MX * co + j] - 1];
-      c_tr[(size_t)j * nv + c] = s->tri_new[d.v_tr[MX * co + j] - 1];
-      if (d.hx_ttgout[MX * co + j] < 0) bits |= 1 << (BIT_TTG + j);
-      if (d.hx_tnrccw[MX * co + j] < 0) bits |= 1 << (BIT_TNR + j);
-      if (d.ed_v[2 * eo] - 1 != co) bits |= 1 << (BIT_SIDE + j);
-      c_R[(size_t)j * nv + c] = hxtr[MX * co + j];
-      ksum += 0.25 * d.ed_leng[eo] * d.edhx_leng[eo];
+    for (int j = 0; j < MX; ++j) {
+      const int js = (j < n) ? j : 0;  // padded slot of a pentagon repeats slot 0
+      const int nbo = d.v_nb[MX * co + js] - 1;
+      const int nbi = cn[nbo];
+      c_nb[(size_t)j * nv + c] = nbi;
+      const int eo = d.v_ed[MX * co + js] - 1;
+      const int ei = en[eo];
+      const int owner = ei / 3;
+      if (owner != c && owner != nbi) return set_error(IMX_EINVAL, "layout: edge owner is not an end cell");
+      if (owner == c) A |= 1 << (BA_OWN + j);
+      B |= (ei % 3) << (2 * j);
+      const int ti = tn[d.v_tr[MX * co + js] - 1];
+      const int town = ti / 2;
+      const int nbn = cn[d.v_nb[MX * co + (js + 1) % n] - 1];
+      const int ts = (town == c) ? 0 : (town == nbi) ? 1 : (town == nbn) ? 2 : -1;
+      if (ts < 0) return set_error(IMX_EINVAL, "layout: triangle owner is not a vertex");
+      B |= ts << (BB_TSEL + 2 * j);
+      B |= (ti % 2) << (BB_KT + j);
+      if (j < n) {
+        if (d.hx_ttgout[MX * co + j] < 0) A |= 1 << (BA_TTG + j);
+        if (d.hx_tnrccw[MX * co + j] < 0) A |= 1 << (BA_TNR + j);
+        c_R[(size_t)j * nv + c] = hxtr[MX * co + j];
+        ksum += 0.25 * d.ed_leng[eo] * d.edhx_leng[eo];
+      }
     }
-    c_bits[c] = bits;
+    bitsA[c] = A;
+    bitsB[c] = B;
     c_iaa[c] = 1.0 / (d.hx_areag[co] * a);
     c_ika[c] = 1.0 / ksum;
   }
-  std::vector<int32_t> e_c((size_t)2 * ne);
-  std::vector<double> e_dl(ne), e_kw(ne), e_il(ne);
-  for (int eo = 0; eo < ne; ++eo) {
-    const int e = s->edge_new[eo];
-    e_c[e] = s->cell_new[d.ed_v[2 * eo] - 1];
-    e_c[(size_t)ne + e] = s->cell_new[d.ed_v[2 * eo + 1] - 1];
+  std::vector<int32_t> e_oth(NE, -1);
+  std::vector<double> e_dl(NE, 0.0), e_kw(NE, 0.0), e_il(NE, 0.0);
+  for (int eo = 0; eo < d.ne; ++eo) {
+    const int e = en[eo];
+    const int o = e / 3;
+    const int v1 = cn[d.ed_v[2 * eo] - 1], v2 = cn[d.ed_v[2 * eo + 1] - 1];
+    if (o == v1) e_oth[e] = v2;
+    else e_oth[e] = v1 | (1 << OTH_V2);
     e_dl[e] = d.edhx_leng[eo];
     e_kw[e] = 0.25 * d.ed_leng[eo] * d.edhx_leng[eo];
     e_il[e] = 1.0 / d.ed_leng[eo];
   }
-  std::vector<int32_t> t_ed((size_t)3 * nt), t_v((size_t)3 * nt);
-  std::vector<double> t_ceta((size_t)3 * nt), t_kap((size_t)3 * nt), t_f(nt);
-  for (int ko = 0; ko < nt; ++ko) {
-    const int k = s->tri_new[ko];
-    for (int l = 0; l < 3; ++l) {
-      const int eo = d.tr_ed[3 * ko + l] - 1;
-      t_ed[(size_t)l * nt + k] = s->edge_new[eo];
-      t_v[(size_t)l * nt + k] = s->cell_new[d.tr_v[3 * ko + l] - 1];
-      t_ceta[(size_t)l * nt + k] = d.ed_leng[eo] * (double)d.tr_tg[3 * ko + l] / (d.tr_areag[ko] * a);
-      t_kap[(size_t)l * nt + k] = trhx[3 * ko + l] / d.tr_areag[ko];
+  std::vector<int32_t> t_ab((size_t)2 * NT, -1), t_eb(NT, 0);
+  std::vector<double> t_ceta((size_t)3 * NT, 0.0), t_kap((size_t)3 * NT, 0.0), t_f(NT, 0.0);
+  for (int ko = 0; ko < d.nt; ++ko) {
+    const int t = tn[ko];
+    const int o = t / 2;
+    int r = -1;  // rotate the CCW vertex list so the owner comes first
+    for (int l = 0; l < 3; ++l)
+      if (cn[d.tr_v[3 * ko + l] - 1] == o) r = l;
+    if (r < 0) return set_error(IMX_EINVAL, "layout: triangle owner not found");
+    int vv[3];
+    for (int m = 0; m < 3; ++m) vv[m] = cn[d.tr_v[3 * ko + (r + m) % 3] - 1];
+    t_ab[t] = vv[1];
+    t_ab[(size_t)NT + t] = vv[2];
+    int eb = 0;
+    for (int m = 0; m < 3; ++m) {
+      const int lr = (r + m) % 3;  // reference edge l runs tr%v(l) -> tr%v(l+1)
+      const int eo = d.tr_ed[3 * ko + lr] - 1;
+      const int ei = en[eo];
+      const int first = vv[m], second = vv[(m + 1) % 3];
+      const int owner = ei / 3;
+      if (owner != first && owner != second) return set_error(IMX_EINVAL, "layout: bad triangle edge");
+      eb |= ((owner == first) ? 1 : 0) << (3 * m);
+      eb |= (ei % 3) << (3 * m + 1);
+      t_ceta[(size_t)m * NT + t] = d.ed_leng[eo] * (double)d.tr_tg[3 * ko + lr] / (d.tr_areag[ko] * a);
+      t_kap[(size_t)m * NT + t] = trhx[3 * ko + (r + m) % 3] / d.tr_areag[ko];
     }
-    t_f[k] = 2.0 * s->o.omega * std::sin(d.tr_clat[ko]);
+    t_eb[t] = eb;
+    t_f[t] = 2.0 * s->o.omega * std::sin(d.tr_clat[ko]);
   }
   Alloc &A = s->A;
-  s->x_c_ed = A.put(c_ed);
   s->x_c_nb = A.put(c_nb);
-  s->x_c_tr = A.put(c_tr);
-  s->x_c_bits = A.put(c_bits);
+  s->x_c_bitsA = A.put(bitsA);
+  s->x_c_bitsB = A.put(bitsB);
   s->x_c_R = A.put(cvt<T>(c_R));
   s->x_c_iaa = A.put(c_iaa);
   s->x_c_ika = A.put(cvt<T>(c_ika));
-  s->x_e_c = A.put(e_c);
+  s->x_e_oth = A.put(e_oth);
   s->x_e_dl = A.put(e_dl);
   s->x_e_kw = A.put(cvt<T>(e_kw));
   s->x_e_il = A.put(cvt<T>(e_il));
-  s->x_t_ed = A.put(t_ed);
-  s->x_t_v = A.put(t_v);
+  s->x_t_ab = A.put(t_ab);
+  s->x_t_eb = A.put(t_eb);
   s->x_t_ceta = A.put(cvt<T>(t_ceta));
   s->x_t_kap = A.put(cvt<T>(t_kap));
   s->x_t_f = A.put(cvt<T>(t_f));
   s->h0 = A.get<T>(nv);
   s->hA = A.get<T>(nv);
   s->hB = A.get<T>(nv);
-  s->u0 = A.get<T>(ne);
-  s->uA = A.get<T>(ne);
-  s->uB = A.get<T>(ne);
+  s->u0 = A.get<T>(NE);
+  s->uA = A.get<T>(NE);
+  s->uB = A.get<T>(NE);
   s->acch = A.get<T>(nv);
-  s->accu = A.get<T>(ne);
-  s->q = A.get<T>(nt);
+  s->accu = A.get<T>(NE);
+  s->q = A.get<T>(NT);
   s->phi = A.get<T>(nv);
-  s->Pe = A.get<T>((size_t)2 * ne);
+  s->Pown = A.get<T>(NE);
+  s->Poth = A.get<T>(NE);
   s->dh_out = A.get<double>(nv);
-  s->du_out = A.get<T>(ne);
+  s->du_out = A.get<T>(NE);
   if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_create (fast tables)");
   return IMX_OK;
 }
 
 static int validate_desc(const imx_mesh_desc &d) {
   if (d.nv <= 0 || d.ne <= 0 || d.nt <= 0) return set_error(IMX_EINVAL, "mesh: empty mesh");
+  if (d.nv >= (1 << OTH_V2)) return set_error(IMX_EINVAL, "mesh: too many cells (max 2^30 - 1)");
   if (!d.v_nnb || !d.v_nb || !d.v_ed || !d.v_tr || !d.hx_ttgout || !d.hx_tnrccw || !d.v_p ||
       !d.hx_areag || !d.ed_v || !d.ed_sh || !d.ed_leng || !d.edhx_leng || !d.ed_c || !d.edhx_c ||
       !d.tr_v || !d.tr_ed || !d.tr_tg || !d.tr_areag || !d.tr_c || !d.tr_clat)
@@ -842,74 +912,80 @@ static int create(const imx_mesh_desc &d, const imx_swm_opts &o, imx_swm *s) {
     s->href = o.h_ref;
     s->href_set = true;
   }
-  const int nv = d.nv, ne = d.ne, nt = d.nt;
 
   // gridprop: caller's arrays or computed here (reference formulas, FP64)
   std::vector<double> trhx_v, hxtr_v, trskw_v;
   const double *trhx = d.tr_trhx_area, *hxtr = d.hx_hxtr_area, *trskw = d.hx_trskw;
   if (!trhx || !hxtr || !trskw) {
-    trhx_v.assign((size_t)3 * nt, 0.0);
-    hxtr_v.assign((size_t)MX * nv, 0.0);
-    trskw_v.assign((size_t)MX * MX * nv, 0.0);
+    trhx_v.assign((size_t)3 * d.nt, 0.0);
+    hxtr_v.assign((size_t)MX * d.nv, 0.0);
+    trskw_v.assign((size_t)MX * MX * d.nv, 0.0);
     gridprop(d, trhx_v.data(), hxtr_v.data(), trskw_v.data());
     trhx = trhx_v.data();
     hxtr = hxtr_v.data();
     trskw = trskw_v.data();
   }
 
-  sfc_order(d, o.use_sfc != 0, s->cell_new, s->edge_new, s->tri_new);
+  std::string err;
+  if (!build_layout(d, o.use_sfc != 0, s->lay, err)) return set_error(IMX_EINVAL, "layout: " + err);
+  s->NE = s->lay.NE;
+  s->NT = s->lay.NT;
+  const int nv = d.nv, NE = s->NE, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
 
   HIPTRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
   HIPTRY(hipEventCreate(&s->ev0), "hipEventCreate");
   HIPTRY(hipEventCreate(&s->ev1), "hipEventCreate");
 
-  // geometry tables in internal numbering (slot-major [j][entity])
+  // geometry / REF tables in the internal numbering (slot-major [j][entity])
   std::vector<int32_t> c_nnb(nv), c_ed((size_t)MX * nv, 0);
   std::vector<int8_t> c_ttg((size_t)MX * nv, 0), c_tnr((size_t)MX * nv, 0);
-  std::vector<double> c_area(nv), c_trskw((size_t)MX * MX * nv, 0.0);
+  std::vector<double> c_area(nv), c_trskw;
+  if (!s->fast) c_trskw.assign((size_t)MX * MX * nv, 0.0);
   for (int co = 0; co < nv; ++co) {
-    const int c = s->cell_new[co];
+    const int c = cn[co];
     const int n = d.v_nnb[co];
     c_nnb[c] = n;
     c_area[c] = d.hx_areag[co];
     for (int j = 0; j < n; ++j) {
-      c_ed[(size_t)j * nv + c] = s->edge_new[d.v_ed[MX * co + j] - 1];
+      c_ed[(size_t)j * nv + c] = en[d.v_ed[MX * co + j] - 1];
       c_ttg[(size_t)j * nv + c] = (int8_t)(d.hx_ttgout[MX * co + j] < 0 ? -1 : 1);
       c_tnr[(size_t)j * nv + c] = (int8_t)(d.hx_tnrccw[MX * co + j] < 0 ? -1 : 1);
-      for (int k = 0; k < n; ++k)
-        c_trskw[((size_t)j * MX + k) * nv + c] = trskw[MX * MX * co + MX * j + k];
+      if (!s->fast)
+        for (int k = 0; k < n; ++k)
+          c_trskw[((size_t)j * MX + k) * nv + c] = trskw[MX * MX * co + MX * j + k];
     }
   }
-  std::vector<int32_t> e_v((size_t)2 * ne), e_sh((size_t)2 * ne);
-  std::vector<int8_t> e_loc((size_t)2 * ne);
-  std::vector<double> e_leng(ne), e_dleng(ne);
-  for (int eo = 0; eo < ne; ++eo) {
-    const int e = s->edge_new[eo];
+  std::vector<int32_t> e_v((size_t)2 * NE, -1), e_sh((size_t)2 * NE, -1);
+  std::vector<int8_t> e_loc((size_t)2 * NE, 0);
+  std::vector<double> e_leng(NE, 1.0), e_dleng(NE, 0.0);
+  for (int eo = 0; eo < d.ne; ++eo) {
+    const int e = en[eo];
     for (int k = 0; k < 2; ++k) {
       const int co = d.ed_v[2 * eo + k] - 1;
-      e_v[(size_t)k * ne + e] = s->cell_new[co];
-      e_sh[(size_t)k * ne + e] = s->tri_new[d.ed_sh[2 * eo + k] - 1];
+      e_v[(size_t)k * NE + e] = cn[co];
+      e_sh[(size_t)k * NE + e] = tn[d.ed_sh[2 * eo + k] - 1];
       int loc = -1;  // getedindexonhx (smeshpack.f90:4099-4125)
       for (int j = 0; j < d.v_nnb[co]; ++j)
         if (d.v_ed[MX * co + j] == eo + 1) {
           loc = j;
           break;
         }
-      e_loc[(size_t)k * ne + e] = (int8_t)loc;
+      e_loc[(size_t)k * NE + e] = (int8_t)loc;
     }
     e_leng[e] = d.ed_leng[eo];
     e_dleng[e] = d.edhx_leng[eo];
   }
-  std::vector<int32_t> t_ed((size_t)3 * nt), t_v((size_t)3 * nt);
-  std::vector<int8_t> t_tg((size_t)3 * nt);
-  std::vector<double> t_area(nt), t_f(nt), t_trhx((size_t)3 * nt);
-  for (int ko = 0; ko < nt; ++ko) {
-    const int k = s->tri_new[ko];
+  std::vector<int32_t> t_ed((size_t)3 * NT, -1), t_v((size_t)3 * NT, 0);
+  std::vector<int8_t> t_tg((size_t)3 * NT, 0);
+  std::vector<double> t_area(NT, 1.0), t_f(NT, 0.0), t_trhx((size_t)3 * NT, 0.0);
+  for (int ko = 0; ko < d.nt; ++ko) {
+    const int k = tn[ko];
     for (int l = 0; l < 3; ++l) {
-      t_ed[(size_t)l * nt + k] = s->edge_new[d.tr_ed[3 * ko + l] - 1];
-      t_v[(size_t)l * nt + k] = s->cell_new[d.tr_v[3 * ko + l] - 1];
-      t_tg[(size_t)l * nt + k] = (int8_t)(d.tr_tg[3 * ko + l] < 0 ? -1 : 1);
-      t_trhx[(size_t)l * nt + k] = trhx[3 * ko + l];
+      t_ed[(size_t)l * NT + k] = en[d.tr_ed[3 * ko + l] - 1];
+      t_v[(size_t)l * NT + k] = cn[d.tr_v[3 * ko + l] - 1];
+      t_tg[(size_t)l * NT + k] = (int8_t)(d.tr_tg[3 * ko + l] < 0 ? -1 : 1);
+      t_trhx[(size_t)l * NT + k] = trhx[3 * ko + l];
     }
     t_area[k] = d.tr_areag[ko];
     // 2.*Omega*dsin(lat) of vort_tr (swm_operators.f90:597), libm sin on the host
@@ -934,7 +1010,7 @@ static int create(const imx_mesh_desc &d, const imx_swm_opts &o, imx_swm *s) {
   s->t_trhx = A.put(t_trhx);
   s->bt = A.put(std::vector<double>(nv, 0.0));
   s->flag = A.put(std::vector<int>(1, 0));
-  s->nparts = (nv + TPB - 1) / TPB + (ne + TPB - 1) / TPB + (nt + TPB - 1) / TPB;
+  s->nparts = (nv + TPB - 1) / TPB + (NE + TPB - 1) / TPB + (NT + TPB - 1) / TPB;
   s->part = A.get<double>((size_t)s->nparts * NPART);
   s->diag = A.get<double>(NPART);
   if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_create (geometry)");
@@ -942,30 +1018,30 @@ static int create(const imx_mesh_desc &d, const imx_swm_opts &o, imx_swm *s) {
   if (!s->fast) {
     s->c_trskw = A.put(c_trskw);
     s->rh = A.get<double>(nv);
-    s->ru = A.get<double>(ne);
+    s->ru = A.get<double>(NE);
     s->rhn = A.get<double>(nv);
-    s->run = A.get<double>(ne);
+    s->run = A.get<double>(NE);
     s->massf = A.get<double>((size_t)4 * nv);
-    s->momf = A.get<double>((size_t)4 * ne);
-    s->f_h_ed = A.get<double>(ne);
-    s->f_uh = A.get<double>(ne);
-    s->f_eta = A.get<double>(nt);
-    s->f_h_tr = A.get<double>(nt);
-    s->f_q_tr = A.get<double>(nt);
-    s->f_q_ed = A.get<double>(ne);
+    s->momf = A.get<double>((size_t)4 * NE);
+    s->f_h_ed = A.get<double>(NE);
+    s->f_uh = A.get<double>(NE);
+    s->f_eta = A.get<double>(NT);
+    s->f_h_tr = A.get<double>(NT);
+    s->f_q_tr = A.get<double>(NT);
+    s->f_q_ed = A.get<double>(NE);
     s->f_divuh = A.get<double>(nv);
     s->f_ke = A.get<double>(nv);
     s->f_ghbk = A.get<double>(nv);
-    s->f_grad = A.get<double>(ne);
-    s->f_uhq = A.get<double>(ne);
+    s->f_grad = A.get<double>(NE);
+    s->f_uhq = A.get<double>(NE);
     s->f_divu = A.get<double>(nv);
-    s->f_zeta = A.get<double>(nt);
+    s->f_zeta = A.get<double>(NT);
     if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_create (ref fields)");
   } else {
     rc = s->single ? build_fast<float>(s, d, trhx, hxtr) : build_fast<double>(s, d, trhx, hxtr);
     if (rc) return rc;
   }
-  HIPTRY(hipStreamSynchronize(s->stream), "imx_swm_create");
+  HIPTRY(hipDeviceSynchronize(), "imx_swm_create");
   return IMX_OK;
 }
 
@@ -985,9 +1061,9 @@ static int check_flag(imx_swm *s, const char *what) {
 
 template <typename T>
 static int upload_state(imx_swm *s, const double *h, const double *u, T *hd, T *ud) {
-  std::vector<T> hb(s->nv), ub(s->ne);
-  for (int i = 0; i < s->nv; ++i) hb[s->cell_new[i]] = (T)(h[i] - s->href);
-  for (int l = 0; l < s->ne; ++l) ub[s->edge_new[l]] = (T)u[l];
+  std::vector<T> hb(s->nv, (T)0), ub(s->NE, (T)0);
+  for (int i = 0; i < s->nv; ++i) hb[s->lay.cell_new[i]] = (T)(h[i] - s->href);
+  for (int l = 0; l < s->ne; ++l) ub[s->lay.edge_new[l]] = (T)u[l];
   HIPTRY(hipMemcpyAsync(hd, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice, s->stream),
          "upload h");
   HIPTRY(hipMemcpyAsync(ud, ub.data(), ub.size() * sizeof(T), hipMemcpyHostToDevice, s->stream),
@@ -998,16 +1074,16 @@ static int upload_state(imx_swm *s, const double *h, const double *u, T *hd, T *
 
 template <typename T>
 static int download_state(imx_swm *s, double *h, double *u, const T *hd, const T *ud) {
-  std::vector<T> hb(s->nv), ub(s->ne);
+  std::vector<T> hb(s->nv), ub(s->NE);
   HIPTRY(hipMemcpyAsync(hb.data(), hd, hb.size() * sizeof(T), hipMemcpyDeviceToHost, s->stream),
          "download h");
   HIPTRY(hipMemcpyAsync(ub.data(), ud, ub.size() * sizeof(T), hipMemcpyDeviceToHost, s->stream),
          "download u");
   HIPTRY(hipStreamSynchronize(s->stream), "download");
   if (h)
-    for (int i = 0; i < s->nv; ++i) h[i] = (double)hb[s->cell_new[i]] + s->href;
+    for (int i = 0; i < s->nv; ++i) h[i] = (double)hb[s->lay.cell_new[i]] + s->href;
   if (u)
-    for (int l = 0; l < s->ne; ++l) u[l] = (double)ub[s->edge_new[l]];
+    for (int l = 0; l < s->ne; ++l) u[l] = (double)ub[s->lay.edge_new[l]];
   return IMX_OK;
 }
 
@@ -1015,12 +1091,12 @@ static void set_href(imx_swm *s, const double *h) {
   if (s->href_set || !s->fast) return;
   double sa = 0.0, sh = 0.0;
   std::vector<double> area(s->nv);
-  // area-weighted mean of the first state (cell areas live on the device)
+  // area-weighted mean of the first state set (cell areas live on the device)
   (void)hipMemcpy(area.data(), s->c_area, s->nv * sizeof(double), hipMemcpyDeviceToHost);
   for (int i = 0; i < s->nv; ++i) {
-    const double A = area[s->cell_new[i]];
-    sa += A;
-    sh += A * h[i];
+    const double Ar = area[s->lay.cell_new[i]];
+    sa += Ar;
+    sh += Ar * h[i];
   }
   s->href = sh / sa;
   s->href_set = true;
@@ -1112,13 +1188,13 @@ int imx_swm_tendency(imx_swm *s, const double *h, const double *u, double *masse
   if (!s || !h || !u || !masseq || !momeq)
     return imx::set_error(IMX_EINVAL, "imx_swm_tendency: NULL argument");
   int rc;
-  std::vector<double> mb(s->nv), fb(s->ne);
+  std::vector<double> mb(s->nv), fb(s->NE);
   if (!s->fast) {
     if ((rc = imx::upload_state<double>(s, h, u, s->rhn, s->run))) return rc;
     imx::launch_ref_tendency(s, s->rhn, s->run, s->massf, s->momf);
     if ((rc = imx::check_flag(s, "imx_swm_tendency"))) return rc;
     HIPTRY(hipMemcpy(mb.data(), s->massf, s->nv * sizeof(double), hipMemcpyDeviceToHost), "tendency");
-    HIPTRY(hipMemcpy(fb.data(), s->momf, s->ne * sizeof(double), hipMemcpyDeviceToHost), "tendency");
+    HIPTRY(hipMemcpy(fb.data(), s->momf, s->NE * sizeof(double), hipMemcpyDeviceToHost), "tendency");
   } else {
     imx::set_href(s, h);
     if (s->single) {
@@ -1126,26 +1202,27 @@ int imx_swm_tendency(imx_swm *s, const double *h, const double *u, double *masse
       imx::launch_fast_stage<float>(s, -1, (const float *)s->hA, (const float *)s->uA, nullptr,
                                     nullptr, nullptr);
       if ((rc = imx::check_flag(s, "imx_swm_tendency"))) return rc;
-      std::vector<float> ff(s->ne);
-      HIPTRY(hipMemcpy(ff.data(), s->du_out, s->ne * sizeof(float), hipMemcpyDeviceToHost), "tendency");
-      for (int l = 0; l < s->ne; ++l) fb[l] = ff[l];
+      std::vector<float> ff(s->NE);
+      HIPTRY(hipMemcpy(ff.data(), s->du_out, s->NE * sizeof(float), hipMemcpyDeviceToHost), "tendency");
+      for (int l = 0; l < s->NE; ++l) fb[l] = ff[l];
     } else {
       if ((rc = imx::upload_state<double>(s, h, u, (double *)s->hA, (double *)s->uA))) return rc;
       imx::launch_fast_stage<double>(s, -1, (const double *)s->hA, (const double *)s->uA, nullptr,
                                      nullptr, nullptr);
       if ((rc = imx::check_flag(s, "imx_swm_tendency"))) return rc;
-      HIPTRY(hipMemcpy(fb.data(), s->du_out, s->ne * sizeof(double), hipMemcpyDeviceToHost), "tendency");
+      HIPTRY(hipMemcpy(fb.data(), s->du_out, s->NE * sizeof(double), hipMemcpyDeviceToHost), "tendency");
     }
     HIPTRY(hipMemcpy(mb.data(), s->dh_out, s->nv * sizeof(double), hipMemcpyDeviceToHost), "tendency");
   }
-  for (int i = 0; i < s->nv; ++i) masseq[i] = mb[s->cell_new[i]];
-  for (int l = 0; l < s->ne; ++l) momeq[l] = fb[s->edge_new[l]];
+  for (int i = 0; i < s->nv; ++i) masseq[i] = mb[s->lay.cell_new[i]];
+  for (int l = 0; l < s->ne; ++l) momeq[l] = fb[s->lay.edge_new[l]];
   return IMX_OK;
 }
 
 int imx_swm_get_field(imx_swm *s, int32_t which, double *out) {
   if (!s || !out) return imx::set_error(IMX_EINVAL, "imx_swm_get_field: NULL argument");
-  if (s->fast) return imx::set_error(IMX_ENOTSUP, "imx_swm_get_field: only IMX_PREC_FP64_REF keeps the intermediates");
+  if (s->fast)
+    return imx::set_error(IMX_ENOTSUP, "imx_swm_get_field: only IMX_PREC_FP64_REF keeps the intermediates");
   double *src = nullptr;
   int kind = 0;  // 0 cell, 1 edge, 2 tri
   switch (which) {
@@ -1165,9 +1242,11 @@ int imx_swm_get_field(imx_swm *s, int32_t which, double *out) {
     default: return imx::set_error(IMX_EINVAL, "imx_swm_get_field: unknown field");
   }
   const int n = kind == 0 ? s->nv : kind == 1 ? s->ne : s->nt;
-  const std::vector<int32_t> &perm = kind == 0 ? s->cell_new : kind == 1 ? s->edge_new : s->tri_new;
-  std::vector<double> b(n);
-  HIPTRY(hipMemcpy(b.data(), src, n * sizeof(double), hipMemcpyDeviceToHost), "get_field");
+  const int ni = kind == 0 ? s->nv : kind == 1 ? s->NE : s->NT;
+  const std::vector<int32_t> &perm =
+      kind == 0 ? s->lay.cell_new : kind == 1 ? s->lay.edge_new : s->lay.tri_new;
+  std::vector<double> b(ni);
+  HIPTRY(hipMemcpy(b.data(), src, ni * sizeof(double), hipMemcpyDeviceToHost), "get_field");
   for (int i = 0; i < n; ++i) out[i] = b[perm[i]];
   return IMX_OK;
 }
@@ -1182,6 +1261,11 @@ int imx_swm_step(imx_swm *s, int32_t nsteps) {
 
 int imx_swm_step_timed(imx_swm *s, int32_t nsteps, double *ms) {
   if (!s || !ms) return imx::set_error(IMX_EINVAL, "imx_swm_step_timed: NULL argument");
+  if (nsteps < 0) return imx::set_error(IMX_EINVAL, "imx_swm_step_timed: nsteps < 0");
+  if (s->o.use_graph) {
+    int rc = imx::ensure_graph(s);  // instantiation stays outside the timed region
+    if (rc) return rc;
+  }
   HIPTRY(hipEventRecord(s->ev0, s->stream), "event");
   int rc = imx::run_steps(s, nsteps);
   if (rc) return rc;
