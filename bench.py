@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=8)
+    ap.add_argument("--copy-probe-mib", type=int, default=1024,
+                    help="size of the streaming-copy bandwidth probe (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -131,6 +133,15 @@ def main():
     value = nv * args.steps / wall
     log(f"{args.steps} steps: wall {wall * 1e3:.2f} ms, events {ev_ms:.2f} ms, {value:.3e} cell-steps/s")
 
+    copy_gbs = None
+    if args.copy_probe_mib > 0:
+        import ctypes
+        from imodel_amd import capi
+        g = ctypes.c_double()
+        capi.check(capi.lib().imx_copy_probe(args.copy_probe_mib << 20, 10, ctypes.byref(g)), "imx_copy_probe")
+        copy_gbs = g.value
+        log(f"copy probe: {copy_gbs:.0f} GB/s")
+
     fp64 = args.precision != "fp32"
     B = canon_step_bytes(nv, ne, nt, fp64)
     achieved = B / (ev_step * 1e-3) / 1e9
@@ -141,7 +152,9 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "RK4 step (one hipGraph launch: 4 x fast_tri, fast_cell, fast_edge)",
-                "algorithmic_bytes_per_launch": B, "launch_ms": ev_step}
+                "algorithmic_bytes_per_launch": B, "launch_ms": ev_step,
+                "copy_probe_GBs": copy_gbs,
+                "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None}
     kernels = {}
     if kern:
         kb = impl_kernel_bytes(nv, ne, nt)

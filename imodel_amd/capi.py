@@ -119,6 +119,7 @@ def lib():
             "imx_swm_diagnostics": (ctypes.c_int, [vp, _F64P]),
             "imx_swm_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
             "imx_swm_destroy": (None, [vp]),
+            "imx_copy_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _F64P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

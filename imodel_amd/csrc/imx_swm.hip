@@ -527,6 +527,15 @@ __global__ __launch_bounds__(TPB) void diag_final(int nparts, const double *__re
   }
 }
 
+// ======================================================= bandwidth probe ==
+// Plain streaming copy, used to measure the attainable HBM bandwidth and to
+// scale the memory counters of a profiler (bytes moved are known exactly).
+__global__ __launch_bounds__(TPB) void copy_probe(const float4 *__restrict__ a,
+                                                  float4 *__restrict__ b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * TPB;
+  for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
 // =============================================================== handle ====
 namespace {
 
@@ -1343,6 +1352,36 @@ int imx_swm_info(imx_swm *s, int64_t out[8]) {
   out[6] = s->o.use_graph;
   out[7] = 0;
   return IMX_OK;
+}
+
+int imx_copy_probe(int64_t bytes, int32_t reps, double *gbs) {
+  if (!gbs || bytes < 16 || reps < 1) return imx::set_error(IMX_EINVAL, "imx_copy_probe: bad argument");
+  const size_t n = (size_t)bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = IMX_OK;
+  float ms = 0.f;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+    rc = imx::set_error(IMX_ENOMEM, "imx_copy_probe: hipMalloc failed");
+  } else if (hipMemset(a, 0, n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+             hipEventCreate(&e1) != hipSuccess) {
+    rc = imx::set_error(IMX_EDEVICE, "imx_copy_probe: setup failed");
+  } else {
+    const int blocks = 4096;
+    copy_probe<<<blocks, TPB>>>(a, b, n);
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) copy_probe<<<blocks, TPB>>>(a, b, n);
+    (void)hipEventRecord(e1, nullptr);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = imx::set_error(IMX_EDEVICE, "imx_copy_probe: kernel failed");
+    else
+      *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }
 
 void imx_swm_destroy(imx_swm *s) {

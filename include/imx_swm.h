@@ -180,6 +180,12 @@ int  imx_swm_diagnostics(imx_swm *s, double out[IMX_NDIAG]);
 int  imx_swm_info(imx_swm *s, int64_t out[8]);   /* nv ne nt bytes_dev ... */
 void imx_swm_destroy(imx_swm *s);
 
+/* ---- measurement utility (no counterpart in the reference) ----------------
+   Streams `bytes` from one device buffer to another `reps` times and returns
+   the achieved read+write bandwidth [GB/s]; also the known-size kernel used
+   to scale profiler memory counters. Allocates and frees its own buffers. */
+int  imx_copy_probe(int64_t bytes, int32_t reps, double *gbs);
+
 #ifdef __cplusplus
 }
 #endif
