@@ -1,0 +1,143 @@
+!=======================================================================
+! imx_c: ISO_C_BINDING interface to libimx (include/imx_swm.h).
+! Every libimx call returns 0 or a negative code; imx_check turns a failure
+! into `print*, message; stop`, the reference's error convention
+! (e.g. src/swm_operators.f90:1604-1606).
+!=======================================================================
+module imx_c
+  use iso_c_binding
+  implicit none
+
+  integer(c_int32_t), parameter :: IMX_PREC_FP64_REF = 0, IMX_PREC_FP64 = 1, IMX_PREC_FP32 = 2
+  integer(c_int32_t), parameter :: IMX_POS_EQS = 0, IMX_POS_POL = 1
+  integer, parameter :: IMX_NDIAG = 8
+
+  type, bind(C) :: imx_mesh_desc
+    integer(c_int32_t) :: nv, ne, nt, glevel
+    type(c_ptr) :: v_nnb, v_nb, v_ed, v_tr, hx_ttgout, hx_tnrccw, v_p, hx_areag
+    type(c_ptr) :: ed_v, ed_sh, ed_leng, edhx_leng, ed_c, edhx_c
+    type(c_ptr) :: tr_v, tr_ed, tr_tg, tr_areag, tr_c, tr_clat
+    type(c_ptr) :: tr_trhx_area, hx_hxtr_area, hx_trskw
+  end type imx_mesh_desc
+
+  type, bind(C) :: imx_swm_opts
+    integer(c_int32_t) :: precision, testcase, use_sfc, use_graph
+    real(c_double) :: erad, grav, omega, dt, h_ref
+  end type imx_swm_opts
+
+  interface
+    function imx_last_error() bind(C, name='imx_last_error') result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+    function imx_mesh_build_icos(glevel, pos, out) bind(C, name='imx_mesh_build_icos') result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      integer(c_int32_t), value :: glevel, pos
+      type(c_ptr) :: out
+      integer(c_int) :: rc
+    end function
+    function imx_mesh_gridprop(m) bind(C, name='imx_mesh_gridprop') result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int) :: rc
+    end function
+    function imx_mesh_desc_of(m, d) bind(C, name='imx_mesh_desc_of') result(rc)
+      import :: c_int, c_ptr, imx_mesh_desc
+      type(c_ptr), value :: m
+      type(imx_mesh_desc) :: d
+      integer(c_int) :: rc
+    end function
+    subroutine imx_mesh_destroy(m) bind(C, name='imx_mesh_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: m
+    end subroutine
+    function imx_ic_tc2(m, erad, h, u) bind(C, name='imx_ic_tc2') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: m
+      real(c_double), value :: erad
+      real(c_double) :: h(*), u(*)
+      integer(c_int) :: rc
+    end function
+    function imx_ic_spectral(d, seed, lmax, h0, hfrac, urms, erad, h, u) &
+        bind(C, name='imx_ic_spectral') result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double, imx_mesh_desc
+      type(imx_mesh_desc) :: d
+      integer(c_int64_t), value :: seed
+      integer(c_int32_t), value :: lmax
+      real(c_double), value :: h0, hfrac, urms, erad
+      real(c_double) :: h(*), u(*)
+      integer(c_int) :: rc
+    end function
+    subroutine imx_swm_default_opts(o) bind(C, name='imx_swm_default_opts')
+      import :: imx_swm_opts
+      type(imx_swm_opts) :: o
+    end subroutine
+    function imx_swm_create(d, o, out) bind(C, name='imx_swm_create') result(rc)
+      import :: c_int, c_ptr, imx_mesh_desc, imx_swm_opts
+      type(imx_mesh_desc) :: d
+      type(imx_swm_opts) :: o
+      type(c_ptr) :: out
+      integer(c_int) :: rc
+    end function
+    function imx_swm_set_state(s, h, u) bind(C, name='imx_swm_set_state') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: s
+      real(c_double) :: h(*), u(*)
+      integer(c_int) :: rc
+    end function
+    function imx_swm_get_state(s, h, u) bind(C, name='imx_swm_get_state') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: s
+      real(c_double) :: h(*), u(*)
+      integer(c_int) :: rc
+    end function
+    function imx_swm_tendency(s, h, u, masseq, momeq) bind(C, name='imx_swm_tendency') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: s
+      real(c_double) :: h(*), u(*), masseq(*), momeq(*)
+      integer(c_int) :: rc
+    end function
+    function imx_swm_step(s, nsteps) bind(C, name='imx_swm_step') result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: s
+      integer(c_int32_t), value :: nsteps
+      integer(c_int) :: rc
+    end function
+    function imx_swm_diagnostics(s, out) bind(C, name='imx_swm_diagnostics') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: s
+      real(c_double) :: out(*)
+      integer(c_int) :: rc
+    end function
+    subroutine imx_swm_destroy(s) bind(C, name='imx_swm_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: s
+    end subroutine
+  end interface
+
+contains
+
+  subroutine imx_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= 0) then
+      print*, "ERROR in ", what, ": ", trim(imx_error_message())
+      stop 1
+    end if
+  end subroutine imx_check
+
+  function imx_error_message() result(msg)
+    character(len=512) :: msg
+    character(kind=c_char), pointer :: cs(:)
+    type(c_ptr) :: p
+    integer :: i
+    msg = ''
+    p = imx_last_error()
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, cs, [512])
+    do i = 1, 512
+      if (cs(i) == c_null_char) exit
+      msg(i:i) = cs(i)
+    end do
+  end function imx_error_message
+end module imx_c

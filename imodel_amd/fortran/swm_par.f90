@@ -1,0 +1,182 @@
+!=======================================================================
+! swm_par: the positional parameter files of the reference driver.
+!   par/mesh.par  as getparameters  (src/smeshpack.f90:7343-7416), argv
+!                 [meshpar] [nv] override as in the reference
+!   par/swm.par   as swmpars        (src/swm_data.f90:338-693), the
+!                 HTC/trsk10/geo/no-diffusion/no-APVM subset this build runs
+!   par/swm_gpu.par (new, optional): precision, seed, SFC and graph switches
+!=======================================================================
+module swm_par
+  use iso_c_binding, only: c_int32_t
+  implicit none
+  integer, parameter :: r8 = selected_real_kind(12, 100)
+  real(r8), parameter :: day2sec = 86400._r8
+
+  ! mesh.par
+  integer :: nv_req = 100, loadable = 0, showonscreen = 0, simulcase = 15, hrchy = 1
+  character(len=16) :: mkind = 'icos', mpos = 'eqs', moptm = 'nopt'
+  character(len=128) :: mname = ''
+  ! swm.par
+  integer :: testcase = 2, test_lterror = 0, ntime = 0, adjustntime = 0
+  integer :: nplots = 0, nprints = 1000000, iploterrors = 0, inopv = 0
+  real(r8) :: period = 1._r8, maxtime = 1._r8, dt = 0._r8, gasscoef = 1._r8
+  real(r8) :: k2_max = 0._r8, k4_max = 0._r8, hollgw = 2._r8, trskindmax = 0._r8, pvspar = 0.5_r8
+  character(len=16) :: stag, mtdwrapper, reconmtd, coriolis_reconmtd, sinterpol, gradmtd
+  character(len=16) :: areamtd, diffus, hyperdiffus, pv_stab
+  ! swm_gpu.par
+  character(len=16) :: precision = 'fp32'
+  integer :: seed = 1, use_sfc = 1, use_graph = 1, dump_state = 0
+
+contains
+
+  subroutine read_mesh_par()
+    character(len=256) :: filename, buffer, argv
+    integer :: fu, nargs, n
+    filename = 'par/mesh.par'
+    n = 0
+    nargs = command_argument_count()
+    if (nargs >= 1) call get_command_argument(1, filename)
+    if (nargs >= 2) then
+      call get_command_argument(2, argv)
+      read(argv, *) n
+    end if
+    open(newunit=fu, file=trim(filename), status='old')
+    read(fu, *) buffer
+    read(fu, *) buffer
+    read(fu, *) nv_req
+    read(fu, *) buffer
+    read(fu, *) mkind
+    read(fu, *) buffer
+    read(fu, *) mpos
+    read(fu, *) buffer
+    read(fu, *) moptm
+    read(fu, *) buffer
+    read(fu, *) loadable
+    read(fu, *) buffer
+    read(fu, *) showonscreen
+    read(fu, *) buffer
+    read(fu, *) simulcase
+    read(fu, *) buffer
+    read(fu, *) mname
+    read(fu, *) buffer
+    read(fu, *) hrchy
+    close(fu)
+    if (n > 0) nv_req = n
+  end subroutine read_mesh_par
+
+  ! nodesizeadj (src/smeshpack.f90:1497-1558): smallest icosahedral level
+  ! with at least nv_req nodes.
+  integer function glevel_for(nv) result(g)
+    integer, intent(in) :: nv
+    do g = 0, 11
+      if (nv <= 10*4**g + 2) return
+    end do
+    g = 11
+  end function glevel_for
+
+  subroutine read_swm_par(glevel)
+    integer, intent(in) :: glevel
+    character(len=300) :: buffer
+    integer :: fu, ios
+    logical :: ex
+    open(newunit=fu, file='par/swm.par', status='old')
+    read(fu, *) buffer
+    read(fu, *) buffer
+    read(fu, *) testcase, test_lterror
+    read(fu, *) buffer
+    read(fu, *) period, maxtime
+    read(fu, *) buffer
+    read(fu, *) dt, ntime, adjustntime
+    read(fu, *) buffer
+    read(fu, *) stag
+    read(fu, *) buffer
+    read(fu, *) mtdwrapper
+    read(fu, *) buffer
+    read(fu, *) reconmtd, gasscoef
+    read(fu, *) buffer
+    read(fu, *) coriolis_reconmtd
+    read(fu, *) buffer
+    read(fu, *) sinterpol
+    read(fu, *) buffer
+    read(fu, *) gradmtd
+    read(fu, *) buffer
+    read(fu, *) areamtd
+    read(fu, *) buffer
+    read(fu, *) nplots, nprints, iploterrors
+    read(fu, *) buffer
+    read(fu, *) k2_max, diffus
+    read(fu, *) buffer
+    read(fu, *) k4_max, hyperdiffus
+    read(fu, *) buffer
+    read(fu, *) hollgw
+    read(fu, *) buffer
+    read(fu, *) trskindmax
+    read(fu, *) buffer
+    read(fu, *) pv_stab, pvspar
+    read(fu, *) buffer
+    read(fu, *) inopv
+    close(fu)
+
+    ! the scheme this build runs (the reference defaults, par/swm.par)
+    if (trim(stag) /= 'HTC') call unsupported('staggering', stag)
+    if (trim(mtdwrapper) /= 'trsk10') then
+      if (trim(mtdwrapper) /= 'none' .or. trim(reconmtd) /= 'trsk' .or. &
+          trim(coriolis_reconmtd) /= 'trsk' .or. trim(sinterpol) /= 'trsk' .or. &
+          trim(gradmtd) /= 'trsk') call unsupported('method wrapper', mtdwrapper)
+    end if
+    if (areamtd(1:3) /= 'geo') call unsupported('area method', areamtd)
+    if (k2_max /= 0._r8 .or. k4_max /= 0._r8) call unsupported('diffusion', diffus)
+    if (trim(pv_stab) /= 'none') call unsupported('PV stabilisation', pv_stab)
+    if (inopv /= 0) call unsupported('level model (noPV)', 'noPV=1')
+    if (test_lterror /= 0) call unsupported('local truncation error test', '1')
+    if (testcase /= 2 .and. testcase /= 100) call unsupported('test case (2 or 100)', 'other')
+
+    ! dt / ntime (src/swm_data.f90:562-586)
+    if (period <= 0) stop "swmpars error: period should be positive and must be given in days"
+    period = period*day2sec
+    maxtime = maxtime*day2sec
+    if (dt > 0) then
+      if (adjustntime == 1) dt = dt*(2**(5 - glevel))
+      ntime = ceiling(maxtime/dt)
+    else if (dt == 0 .and. ntime > 0) then
+      if (adjustntime == 1) ntime = ntime*(2**(glevel - 5))
+      dt = maxtime/real(ntime, r8)
+    else
+      dt = 50
+      if (adjustntime == 1) dt = dt*(2**(5 - glevel))
+      ntime = ceiling(maxtime/dt)
+    end if
+    if (nprints <= 0) nprints = 1000000
+
+    inquire(file='par/swm_gpu.par', exist=ex)
+    if (ex) then
+      open(newunit=fu, file='par/swm_gpu.par', status='old')
+      read(fu, *, iostat=ios) buffer
+      read(fu, *, iostat=ios) precision
+      read(fu, *, iostat=ios) buffer
+      read(fu, *, iostat=ios) seed
+      read(fu, *, iostat=ios) buffer
+      read(fu, *, iostat=ios) use_sfc, use_graph
+      read(fu, *, iostat=ios) buffer
+      read(fu, *, iostat=ios) dump_state
+      close(fu)
+    end if
+  end subroutine read_swm_par
+
+  integer(c_int32_t) function precision_code() result(p)
+    select case (trim(precision))
+    case ('fp64_ref'); p = 0
+    case ('fp64'); p = 1
+    case ('fp32'); p = 2
+    case default
+      call unsupported('precision', precision)
+      p = 2
+    end select
+  end function precision_code
+
+  subroutine unsupported(what, val)
+    character(len=*), intent(in) :: what, val
+    print*, "ERROR: ", what, " '", trim(val), "' is not part of this build"
+    stop 1
+  end subroutine unsupported
+end module swm_par

@@ -1,0 +1,76 @@
+"""The Fortran host (imodel driver + swm module over ISO_C_BINDING) on the GPU.
+
+Runs imodel_amd/fortran/imodel with the reference's positional par files on
+glevel 3, TC2, dt = 900 s, one day, FP64_REF: its final state must equal the
+oracle's bit for bit, and its write_evol file must carry the reference's
+layout (src/swm.f90:3088-3152).
+"""
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from imodel_amd import imxbin
+from imodel_amd.build import IMODEL
+from imodel_amd.swm import Mesh
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _par(td, nv, testcase, dt, days, nprints, precision, dump=1):
+    os.makedirs(os.path.join(td, "par"))
+    src = os.path.join(ROOT, "par")
+    mesh = open(os.path.join(src, "mesh.par")).read().splitlines()
+    mesh[2] = str(nv)
+    open(os.path.join(td, "par", "mesh.par"), "w").write("\n".join(mesh) + "\n")
+    swm = open(os.path.join(src, "swm.par")).read().splitlines()
+    swm[2] = f"{testcase} 0"
+    swm[4] = f"{days} {days}"
+    swm[6] = f"{dt} 0 0"
+    swm[22] = f"0 {nprints} 0"
+    open(os.path.join(td, "par", "swm.par"), "w").write("\n".join(swm) + "\n")
+    gpu = open(os.path.join(src, "swm_gpu.par")).read().splitlines()
+    gpu[1] = precision
+    gpu[7] = str(dump)
+    open(os.path.join(td, "par", "swm_gpu.par"), "w").write("\n".join(gpu) + "\n")
+
+
+def test_fortran_driver_tc2_day_bitexact():
+    print("[gpu-case] fortran imodel g3 tc2 fp64_ref", flush=True)
+    assert os.access(IMODEL, os.X_OK), "Fortran host not built"
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 24, "fp64_ref")
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        st = imxbin.read(os.path.join(td, "data", "swm_tc2_dt900_HTC_trsk10_areageo_state_icos_eqs_nopt_3.bin"))
+        evol = open(os.path.join(td, "data", "swm_tc2_dt900_HTC_trsk10_areageo_mass_energy_icos_eqs_nopt_3.txt")).read().splitlines()
+    m = Mesh.icos(3)
+    h0, u0 = m.tc2()
+    h, u = Oracle(m.arrays).step(h0, u0, 900.0, 96)
+    assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
+    assert evol[0].split()[:3] == ["n", "mvdist", "tcase"]
+    rows = [line.split() for line in evol[1:]]
+    ks = [int(row[3]) for row in rows]
+    assert ks == [0, 1, 2, 24, 48, 72, 96]
+    assert all(int(row[0]) == 642 and int(row[4]) == 96 for row in rows)
+    assert abs(float(rows[-1][8])) < 1e-13  # relative mass change after one day, FP64
+
+
+def test_fortran_driver_seeded_fp32_runs():
+    print("[gpu-case] fortran imodel g5 seeded fp32", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 10242, 100, 900, 1, 96, "fp32", dump=0)
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        evol = open(os.path.join(td, "data", "swm_tc100_dt900_HTC_trsk10_areageo_mass_energy_icos_eqs_nopt_5.txt")).read().splitlines()
+    last = evol[-1].split()
+    assert int(last[3]) == 96
+    assert abs(float(last[8])) < 1e-8       # mass drift, FP32 state
+    assert abs(float(last[11])) < 1e-4      # total energy drift
