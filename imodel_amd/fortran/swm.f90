@@ -120,9 +120,11 @@ contains
     integer, intent(in) :: k
     real(r8), intent(in) :: time, tmass, penergy, kenergy, tenergy, availenergy, rmsdiv
     character(len=512) :: filename, buffer
-    integer, save :: iunit = -1
+    integer, save :: iunit = 0
+    logical, save :: opened = .false.
     filename = "data/"//trim(swmname)//"_mass_energy_"//trim(meshname)//".txt"
-    if (iunit < 0) then
+    if (.not. opened) then
+      opened = .true.
       call execute_command_line('mkdir -p data')
       buffer = "        n        mvdist  tcase       k       nt        dt(s)       "// &
                "          cfl                 time(dys)             mass               Penergy    "// &

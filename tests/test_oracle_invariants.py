@@ -73,3 +73,16 @@ def test_tc2_steady_and_converging():
         errs.append(np.max(np.abs(h2 - h)) / np.max(h))
     assert errs[0] < 5e-3
     assert errs[1] < 0.6 * errs[0]
+
+
+def test_potential_enstrophy_rest_state():
+    """Fluid at rest, h = H: q_t = f_t/H, so sum_t 1/2 A_t a^2 h_t q_t^2
+    = 1/2 a^2 (2 Omega)^2 / H * sum_t A_t z_t^2, and the triangle quadrature
+    of z^2 over the icosahedral sphere is exact: 4 pi / 3."""
+    H, a, Om = 1000.0, 6.37122e6, 7.292e-5
+    exact = 0.5 * a * a * (2 * Om) ** 2 / H * 4 * np.pi / 3
+    for g in (3, 4):
+        m = _mesh(g)
+        o = Oracle(m.arrays)
+        d = o.diagnostics(np.full(m.nv, H), np.zeros(m.ne))
+        assert abs(d[5] / exact - 1) < 1e-12
