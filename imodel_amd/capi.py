@@ -120,6 +120,17 @@ def lib():
             "imx_swm_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
             "imx_swm_destroy": (None, [vp]),
             "imx_copy_probe": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _F64P]),
+            "imx_swm_tune": (ctypes.c_int, [vp, ctypes.c_int32, _I32P]),
+            "imx_ens_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(MeshDesc), _F64P,
+                                              ctypes.POINTER(SwmOpts), ctypes.POINTER(vp)]),
+            "imx_ens_set_state": (ctypes.c_int, [vp, ctypes.c_int32, _F64P, _F64P]),
+            "imx_ens_get_state": (ctypes.c_int, [vp, ctypes.c_int32, _F64P, _F64P]),
+            "imx_ens_step": (ctypes.c_int, [vp, ctypes.c_int32]),
+            "imx_ens_step_timed": (ctypes.c_int, [vp, ctypes.c_int32, _F64P]),
+            "imx_ens_diagnostics": (ctypes.c_int, [vp, ctypes.c_int32, _F64P]),
+            "imx_ens_tune": (ctypes.c_int, [vp, ctypes.c_int32, _I32P]),
+            "imx_ens_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
+            "imx_ens_destroy": (None, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -45,7 +45,11 @@ struct Layout {
   std::vector<int32_t> cell_new, edge_new, tri_new;
   int NE = 0, NT = 0;
 };
-bool build_layout(const imx_mesh_desc &d, bool use_sfc, Layout &L, std::string &err);
+// group (optional, per caller cell): cells are ordered by (group, Hilbert key),
+// so each group's cells -- and the edges and triangles they own -- are
+// contiguous internal ranges.
+bool build_layout(const imx_mesh_desc &d, bool use_sfc, Layout &L, std::string &err,
+                  const int32_t *group = nullptr);
 void desc_of(const HostMesh &m, imx_mesh_desc &d);
 
 // IMXBIN container: "IMXBIN01", then records (name[24], dtype[4], int64 n, data).

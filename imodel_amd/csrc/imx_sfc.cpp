@@ -63,16 +63,20 @@ static uint64_t hilbert3(double x, double y, double z) {
   return key;
 }
 
-bool build_layout(const imx_mesh_desc &d, bool use_sfc, Layout &L, std::string &err) {
+bool build_layout(const imx_mesh_desc &d, bool use_sfc, Layout &L, std::string &err,
+                  const int32_t *group) {
   const int nv = d.nv, ne = d.ne, nt = d.nt, MX = IMX_MAXNB;
   // ---- cells
   std::vector<int32_t> cell_old(nv);
   std::iota(cell_old.begin(), cell_old.end(), 0);
-  if (use_sfc) {
-    std::vector<uint64_t> key(nv);
-    for (int i = 0; i < nv; ++i) key[i] = hilbert3(d.v_p[3 * i], d.v_p[3 * i + 1], d.v_p[3 * i + 2]);
-    std::stable_sort(cell_old.begin(), cell_old.end(),
-                     [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  if (use_sfc || group) {
+    std::vector<uint64_t> key(nv, 0);
+    if (use_sfc)
+      for (int i = 0; i < nv; ++i) key[i] = hilbert3(d.v_p[3 * i], d.v_p[3 * i + 1], d.v_p[3 * i + 2]);
+    std::stable_sort(cell_old.begin(), cell_old.end(), [&](int32_t a, int32_t b) {
+      if (group && group[a] != group[b]) return group[a] < group[b];
+      return key[a] < key[b];
+    });
   }
   L.cell_new.assign(nv, -1);
   for (int i = 0; i < nv; ++i) L.cell_new[cell_old[i]] = i;
@@ -163,7 +167,6 @@ bool build_layout(const imx_mesh_desc &d, bool use_sfc, Layout &L, std::string &
     }
   }
   std::vector<int32_t> par_t(nt, -1), par_v(nt, -1), seen(nt, -1);
-  std::vector<int32_t> held;  // triangles held by a vertex (scan of its ring)
   for (int t0 = 0; t0 < nt; ++t0) {
     if (own[t0] >= 0) continue;
     std::deque<int32_t> q;
@@ -196,17 +199,14 @@ bool build_layout(const imx_mesh_desc &d, bool use_sfc, Layout &L, std::string &
       err = "triangle ownership has no augmenting path";
       return false;
     }
+    // shift ownership back along the path: end_t takes the free vertex, each
+    // parent takes the vertex its child held
     int32_t cur = end_t;
-    int32_t oldv = own[cur];
     own[cur] = end_v;
     ++load[end_v];
     while (cur != t0) {
-      const int32_t p = par_t[cur];
-      const int32_t v = par_v[cur];
-      (void)oldv;
-      own[p] = v;
-      cur = p;
-      oldv = own[cur];
+      own[par_t[cur]] = par_v[cur];
+      cur = par_t[cur];
     }
   }
   L.tri_new.assign(nt, -1);

@@ -48,6 +48,22 @@ __device__ __forceinline__ int xcd_index() {
   return lb * TPB + threadIdx.x;
 }
 
+// Same with a compile-time block size; remap = 0 keeps the hardware order
+// (a choice the autotuner makes per kernel).
+template <int BS>
+__device__ __forceinline__ int ent_index(int remap) {
+  if (!remap) return blockIdx.x * BS + threadIdx.x;
+  const int per = gridDim.x / NXCD;
+  const int lb = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+  return lb * BS + threadIdx.x;
+}
+
+static inline int grid_bs(int n, int bs) {
+  int nb = (n + bs - 1) / bs;
+  if (nb < 1) nb = 1;
+  return ((nb + NXCD - 1) / NXCD) * NXCD;
+}
+
 // epsilon(1.)/1e5 of the reference zero guard (swm_operators.f90:1603)
 #define HTR_GUARD (1.1920928955078125e-07 / 100000.0)
 
@@ -225,8 +241,8 @@ __global__ __launch_bounds__(TPB) void ref_rk_final(int n, double *__restrict__ 
 #define OTH_MASK 0x3FFFFFFF
 #define OTH_V2 30
 
-template <typename T>
-__global__ __launch_bounds__(TPB) void fast_tri(int NT, const int32_t *__restrict__ t_ab,
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t *__restrict__ t_ab,
                                                 const int32_t *__restrict__ t_eb,
                                                 const T *__restrict__ t_ceta,
                                                 const T *__restrict__ t_kap,
@@ -234,7 +250,7 @@ __global__ __launch_bounds__(TPB) void fast_tri(int NT, const int32_t *__restric
                                                 const T *__restrict__ u,
                                                 const T *__restrict__ h, T href, int use_pv,
                                                 T *__restrict__ q, int *__restrict__ flag) {
-  const int t = xcd_index();
+  const int t = ent_index<BS>(remap);
   if (t >= NT) return;
   const int a = t_ab[t];
   if (a < 0) return;
@@ -269,16 +285,17 @@ __device__ __forceinline__ void kadd(double &s, double &c, double x) {
   s = t;
 }
 
-template <typename T>
-__global__ __launch_bounds__(TPB) void fast_cell(
-    int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void fast_cell(
+    int remap, int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
     const int32_t *__restrict__ c_bitsB, const T *__restrict__ c_R,
     const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
     const double *__restrict__ e_dl, const T *__restrict__ e_kw, const T *__restrict__ q,
     const T *__restrict__ hs, const T *__restrict__ us, double href, T grav, T inv_a,
-    T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt,
-    const T *h0, T *hout, T *__restrict__ acch, double *__restrict__ dh_out) {
-  const int c = xcd_index();
+    T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
+    const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
+    double *__restrict__ dh_out) {
+  const int c = ent_index<BS>(remap);
   if (c >= nv) return;
   const int A = c_bitsA[c];
   const int B = c_bitsB[c];
@@ -357,6 +374,7 @@ __global__ __launch_bounds__(TPB) void fast_cell(
     }
   }
 
+  const double dt = dtc ? dtc[c] : dt_all;  // ensembles: per-instance step
   if (stage < 0) {
     dh_out[c] = dh;
   } else if (stage < 3) {
@@ -370,15 +388,16 @@ __global__ __launch_bounds__(TPB) void fast_cell(
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(TPB) void fast_edge(int NE, const int32_t *__restrict__ e_oth,
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void fast_edge(int remap, int NE, const int32_t *__restrict__ e_oth,
                                                  const T *__restrict__ e_il,
                                                  const T *__restrict__ phi,
                                                  const T *__restrict__ Pown,
                                                  const T *__restrict__ Poth, int zero_mom,
-                                                 int stage, T dt, const T *u0, T *uout,
-                                                 T *__restrict__ accu, T *__restrict__ du_out) {
-  const int e = xcd_index();
+                                                 int stage, T dt_all, const T *__restrict__ dte,
+                                                 const T *u0, T *uout, T *__restrict__ accu,
+                                                 T *__restrict__ du_out) {
+  const int e = ent_index<BS>(remap);
   if (e >= NE) return;
   const int oth = e_oth[e];
   if (oth < 0) return;
@@ -388,6 +407,7 @@ __global__ __launch_bounds__(TPB) void fast_edge(int NE, const int32_t *__restri
   const int v1 = own_v2 ? c2 : o, v2 = own_v2 ? o : c2;
   T du = -((phi[v2] - phi[v1]) + (Pown[e] + Poth[e])) * e_il[e];
   if (zero_mom) du = (T)0;
+  const T dt = dte ? dte[e] : dt_all;
   if (stage < 0) {
     du_out[e] = du;
   } else if (stage < 3) {
@@ -429,7 +449,7 @@ __device__ void block_store(double (&v)[NPART], double *part) {
 
 // slot 0: sum A h, 1: sum A, 2: sum A g h (h/2), 3: sum A divu^2, 7: bad
 template <typename T>
-__global__ __launch_bounds__(TPB) void diag_cell(int nv, const int32_t *__restrict__ c_nnb,
+__global__ __launch_bounds__(TPB) void diag_cell(int off, int cnt, int nv, const int32_t *__restrict__ c_nnb,
                                                  const int32_t *__restrict__ c_ed,
                                                  const int8_t *__restrict__ c_ttg,
                                                  const double *__restrict__ c_area,
@@ -439,8 +459,8 @@ __global__ __launch_bounds__(TPB) void diag_cell(int nv, const int32_t *__restri
                                                  double grav, double erad,
                                                  double *__restrict__ part) {
   double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int i = blockIdx.x * TPB + threadIdx.x;
-  if (i < nv) {
+  const int i = off + blockIdx.x * TPB + threadIdx.x;
+  if (i < off + cnt) {
     const double h = href + (double)hs[i];
     const double A = c_area[i];
     double s = 0.0;
@@ -460,15 +480,15 @@ __global__ __launch_bounds__(TPB) void diag_cell(int nv, const int32_t *__restri
 
 // slot 4: sum 1/2 l d h_e u^2, 7: bad
 template <typename T>
-__global__ __launch_bounds__(TPB) void diag_edge(int ne, const int32_t *__restrict__ e_v,
+__global__ __launch_bounds__(TPB) void diag_edge(int off, int cnt, int ne, const int32_t *__restrict__ e_v,
                                                  const double *__restrict__ e_leng,
                                                  const double *__restrict__ e_dleng,
                                                  const T *__restrict__ hs,
                                                  const T *__restrict__ us, double href,
                                                  double *__restrict__ part) {
   double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int l = blockIdx.x * TPB + threadIdx.x;
-  if (l < ne && e_v[l] >= 0) {
+  const int l = off + blockIdx.x * TPB + threadIdx.x;
+  if (l < off + cnt && e_v[l] >= 0) {
     const double he = (href + (double)hs[e_v[l]] + href + (double)hs[e_v[ne + l]]) * 0.5;
     const double u = (double)us[l];
     v[4] = 0.5 * e_leng[l] * e_dleng[l] * he * (u * u);
@@ -479,7 +499,7 @@ __global__ __launch_bounds__(TPB) void diag_edge(int ne, const int32_t *__restri
 
 // slot 5: sum 1/2 A_t a^2 h_t q_t^2 (potential enstrophy)
 template <typename T>
-__global__ __launch_bounds__(TPB) void diag_tri(int nt, const int32_t *__restrict__ t_ed,
+__global__ __launch_bounds__(TPB) void diag_tri(int off, int cnt, int nt, const int32_t *__restrict__ t_ed,
                                                 const int8_t *__restrict__ t_tg,
                                                 const int32_t *__restrict__ t_v,
                                                 const double *__restrict__ t_area,
@@ -490,8 +510,8 @@ __global__ __launch_bounds__(TPB) void diag_tri(int nt, const int32_t *__restric
                                                 const T *__restrict__ us, double href,
                                                 double erad, double *__restrict__ part) {
   double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int k = blockIdx.x * TPB + threadIdx.x;
-  if (k < nt && t_ed[k] >= 0) {
+  const int k = off + blockIdx.x * TPB + threadIdx.x;
+  if (k < off + cnt && t_ed[k] >= 0) {
     double e = 0.0, s = 0.0;
     for (int l = 0; l < 3; ++l) {
       const int ed = t_ed[l * nt + k];
@@ -627,6 +647,13 @@ struct imx_swm {
        *Poth = nullptr;
   double *dh_out = nullptr;
   void *du_out = nullptr;
+  // ensembles: per-cell / per-edge-slot time step (nullptr: scalar o.dt)
+  double *dt_c = nullptr;
+  void *dt_e = nullptr;
+  // launch configuration per fused kernel (tri, cell, edge): block size and
+  // XCD remap, chosen by imx_swm_tune
+  int kbs[3] = {256, 256, 256};
+  int kremap[3] = {1, 1, 1};
 };
 
 namespace imx {
@@ -686,6 +713,15 @@ static void launch_ref_step(imx_swm *s) {
   ref_rk_final<<<grid_for(nv), TPB, 0, st>>>(nv, s->rh, m[0], m[1], m[2], m[3], dt);
 }
 
+// Block-size dispatch of the fused kernels (the autotuner's search space).
+#define IMX_BS_SWITCH(bs, CALL) \
+  switch (bs) {                 \
+    case 64: CALL(64); break;    \
+    case 128: CALL(128); break;  \
+    case 512: CALL(512); break;  \
+    default: CALL(256); break;   \
+  }
+
 // One FAST stage: stage<0 evaluates the tendency of (hs, us) into dh_out/du_out.
 template <typename T>
 static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T *hout,
@@ -694,21 +730,29 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
   hipStream_t st = s->stream;
   const T href = (T)s->href;
   if (evs) (void)hipEventRecord(evs[0], st);
-  fast_tri<T><<<grid_for(NT), TPB, 0, st>>>(NT, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta,
-                                            (const T *)s->x_t_kap, (const T *)s->x_t_f, us, hs,
-                                            href, s->o.testcase >= 2, (T *)s->q, s->flag);
+#define IMX_TRI(B)                                                                          \
+  fast_tri<T, B><<<grid_bs(NT, B), B, 0, st>>>(                                            \
+      s->kremap[0], NT, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta, (const T *)s->x_t_kap, \
+      (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, (T *)s->q, s->flag)
+  IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
+#undef IMX_TRI
   if (evs) (void)hipEventRecord(evs[1], st);
-  fast_cell<T><<<grid_for(nv), TPB, 0, st>>>(
-      nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_R, s->x_c_iaa,
-      (const T *)s->x_c_ika, s->x_e_dl, (const T *)s->x_e_kw, (const T *)s->q, hs, us, s->href,
-      (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,
-      s->o.dt, (const T *)s->h0, hout, (T *)s->acch, s->dh_out);
+#define IMX_CELL(B)                                                                           \
+  fast_cell<T, B><<<grid_bs(nv, B), B, 0, st>>>(                                             \
+      s->kremap[1], nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_R, s->x_c_iaa, \
+      (const T *)s->x_c_ika, s->x_e_dl, (const T *)s->x_e_kw, (const T *)s->q, hs, us, s->href,  \
+      (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
+      s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
+  IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
+#undef IMX_CELL
   if (evs) (void)hipEventRecord(evs[2], st);
-  fast_edge<T><<<grid_for(NE), TPB, 0, st>>>(NE, s->x_e_oth, (const T *)s->x_e_il, (const T *)s->phi,
-                                             (const T *)s->Pown, (const T *)s->Poth,
-                                             s->o.testcase <= 1, stage, (T)s->o.dt,
-                                             (const T *)s->u0, uout, (T *)s->accu,
-                                             (T *)s->du_out);
+#define IMX_EDGE(B)                                                                              \
+  fast_edge<T, B><<<grid_bs(NE, B), B, 0, st>>>(                                                \
+      s->kremap[2], NE, s->x_e_oth, (const T *)s->x_e_il, (const T *)s->phi, (const T *)s->Pown, \
+      (const T *)s->Poth, s->o.testcase <= 1, stage, (T)s->o.dt, (const T *)s->dt_e,             \
+      (const T *)s->u0, uout, (T *)s->accu, (T *)s->du_out)
+  IMX_BS_SWITCH(s->kbs[2], IMX_EDGE)
+#undef IMX_EDGE
   if (evs) (void)hipEventRecord(evs[3], st);
 }
 
@@ -731,17 +775,18 @@ static void launch_step(imx_swm *s) {
   else launch_fast_step<double>(s, nullptr);
 }
 
+// Diagnostics of the cells [c0, c1) and the edge/triangle slots they own.
 template <typename T>
-static void launch_diag(imx_swm *s, const T *hs, const T *us, double href) {
+static void launch_diag(imx_swm *s, const T *hs, const T *us, double href, int c0, int c1) {
   hipStream_t st = s->stream;
-  const int gv = (s->nv + TPB - 1) / TPB, ge = (s->NE + TPB - 1) / TPB,
-            gt = (s->NT + TPB - 1) / TPB;
-  diag_cell<T><<<gv, TPB, 0, st>>>(s->nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area, s->e_dleng, hs,
-                                   us, href, s->o.grav, s->o.erad, s->part);
-  diag_edge<T><<<ge, TPB, 0, st>>>(s->NE, s->e_v, s->e_leng, s->e_dleng, hs, us, href,
+  const int nc = c1 - c0, ne = 3 * nc, nt = 2 * nc;
+  const int gv = (nc + TPB - 1) / TPB, ge = (ne + TPB - 1) / TPB, gt = (nt + TPB - 1) / TPB;
+  diag_cell<T><<<gv, TPB, 0, st>>>(c0, nc, s->nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area,
+                                   s->e_dleng, hs, us, href, s->o.grav, s->o.erad, s->part);
+  diag_edge<T><<<ge, TPB, 0, st>>>(3 * c0, ne, s->NE, s->e_v, s->e_leng, s->e_dleng, hs, us, href,
                                    s->part + (size_t)gv * NPART);
-  diag_tri<T><<<gt, TPB, 0, st>>>(s->NT, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f, s->t_trhx,
-                                  s->e_leng, hs, us, href, s->o.erad,
+  diag_tri<T><<<gt, TPB, 0, st>>>(2 * c0, nt, s->NT, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f,
+                                  s->t_trhx, s->e_leng, hs, us, href, s->o.erad,
                                   s->part + (size_t)(gv + ge) * NPART);
   diag_final<<<1, TPB, 0, st>>>(gv + ge + gt, s->part, s->diag);
 }
@@ -905,7 +950,9 @@ static int validate_desc(const imx_mesh_desc &d) {
   return IMX_OK;
 }
 
-static int create(const imx_mesh_desc &d, const imx_swm_opts &o, imx_swm *s) {
+static int create(const imx_mesh_desc &d, const imx_swm_opts &o, imx_swm *s,
+                  const int32_t *group = nullptr, const double *dt_cell = nullptr,
+                  const double *dt_edge = nullptr, const Layout *pre = nullptr) {
   int rc = validate_desc(d);
   if (rc) return rc;
   if (o.precision < IMX_PREC_FP64_REF || o.precision > IMX_PREC_FP32)
@@ -936,7 +983,9 @@ static int create(const imx_mesh_desc &d, const imx_swm_opts &o, imx_swm *s) {
   }
 
   std::string err;
-  if (!build_layout(d, o.use_sfc != 0, s->lay, err)) return set_error(IMX_EINVAL, "layout: " + err);
+  if (pre) s->lay = *pre;
+  else if (!build_layout(d, o.use_sfc != 0, s->lay, err, group))
+    return set_error(IMX_EINVAL, "layout: " + err);
   s->NE = s->lay.NE;
   s->NT = s->lay.NT;
   const int nv = d.nv, NE = s->NE, NT = s->NT;
@@ -1049,6 +1098,15 @@ static int create(const imx_mesh_desc &d, const imx_swm_opts &o, imx_swm *s) {
   } else {
     rc = s->single ? build_fast<float>(s, d, trhx, hxtr) : build_fast<double>(s, d, trhx, hxtr);
     if (rc) return rc;
+    if (dt_cell && dt_edge) {  // per-instance time steps of an ensemble
+      std::vector<double> dc(nv), de(NE, 0.0);
+      for (int co = 0; co < nv; ++co) dc[cn[co]] = dt_cell[co];
+      for (int eo = 0; eo < d.ne; ++eo) de[en[eo]] = dt_edge[eo];
+      s->dt_c = A.put(dc);
+      if (s->single) s->dt_e = A.put(cvt<float>(de));
+      else s->dt_e = A.put(de);
+      if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_create (time steps)");
+    }
   }
   HIPTRY(hipDeviceSynchronize(), "imx_swm_create");
   return IMX_OK;
@@ -1137,6 +1195,88 @@ static int run_steps(imx_swm *s, int nsteps) {
     HIPTRY(hipGetLastError(), "step launch");
   }
   return IMX_OK;
+}
+
+// Diagnostics vector of the cells [c0, c1) (IMX_DIAG_* order, FP64).
+static int diag_range(imx_swm *s, int c0, int c1, double *out) {
+  if (!s->fast) launch_diag<double>(s, s->rh, s->ru, 0.0, c0, c1);
+  else if (s->single) launch_diag<float>(s, (const float *)s->h0, (const float *)s->u0, s->href, c0, c1);
+  else launch_diag<double>(s, (const double *)s->h0, (const double *)s->u0, s->href, c0, c1);
+  double v[NPART];
+  HIPTRY(hipMemcpyAsync(v, s->diag, sizeof(v), hipMemcpyDeviceToHost, s->stream), "diagnostics");
+  HIPTRY(hipStreamSynchronize(s->stream), "diagnostics");
+  const double mass = v[0] / v[1];
+  const double pe = v[2];
+  const double ke = v[4];
+  // unavailable potential energy of the flat state at the mean height (b = 0)
+  const double upe = v[1] * s->o.grav * mass * (mass * 0.5);
+  out[IMX_DIAG_MASS] = mass;
+  out[IMX_DIAG_PENERGY] = pe;
+  out[IMX_DIAG_KENERGY] = ke;
+  out[IMX_DIAG_TENERGY] = pe + ke;
+  out[IMX_DIAG_AENERGY] = (pe + ke) - upe;
+  out[IMX_DIAG_PENSTROPHY] = v[5];
+  out[IMX_DIAG_RMSDIV] = std::sqrt(v[3] / v[1]);
+  out[IMX_DIAG_BADFLAG] = v[7] > 0 ? 1.0 : 0.0;
+  return IMX_OK;
+}
+
+// Autotuner: for each fused kernel, time every (block size, XCD remap)
+// candidate over `nsteps` steps with the other kernels fixed, keep the
+// fastest. The state is restored afterwards.
+static int tune(imx_swm *s, int nsteps, int32_t *out) {
+  if (!s->fast) return set_error(IMX_ENOTSUP, "imx_swm_tune: FAST precisions only");
+  if (nsteps < 1) nsteps = 1;
+  const size_t es = s->single ? sizeof(float) : sizeof(double);
+  void *sh = nullptr, *su = nullptr;
+  HIPTRY(hipMalloc(&sh, es * s->nv), "tune");
+  HIPTRY(hipMalloc(&su, es * s->NE), "tune");
+  HIPTRY(hipMemcpyAsync(sh, s->h0, es * s->nv, hipMemcpyDeviceToDevice, s->stream), "tune");
+  HIPTRY(hipMemcpyAsync(su, s->u0, es * s->NE, hipMemcpyDeviceToDevice, s->stream), "tune");
+  std::vector<hipEvent_t> ev(16);
+  for (auto &e : ev) HIPTRY(hipEventCreate(&e), "event");
+  const int bss[4] = {64, 128, 256, 512};
+  for (int k = 0; k < 3; ++k) {
+    double best = 1e30;
+    int bb = s->kbs[k], br = s->kremap[k];
+    for (int bi = 0; bi < 4; ++bi)
+      for (int r = 0; r < 2; ++r) {
+        s->kbs[k] = bss[bi];
+        s->kremap[k] = r;
+        double acc = 0.0;
+        for (int it = 0; it <= nsteps; ++it) {  // first pass warms up
+          if (s->single) launch_fast_step<float>(s, &ev);
+          else launch_fast_step<double>(s, &ev);
+          HIPTRY(hipEventSynchronize(ev[15]), "event");
+          if (it == 0) continue;
+          for (int st = 0; st < 4; ++st) {
+            float f = 0.f;
+            HIPTRY(hipEventElapsedTime(&f, ev[4 * st + k], ev[4 * st + k + 1]), "event");
+            acc += f;
+          }
+        }
+        if (acc < best) {
+          best = acc;
+          bb = bss[bi];
+          br = r;
+        }
+      }
+    s->kbs[k] = bb;
+    s->kremap[k] = br;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  HIPTRY(hipMemcpyAsync(s->h0, sh, es * s->nv, hipMemcpyDeviceToDevice, s->stream), "tune");
+  HIPTRY(hipMemcpyAsync(s->u0, su, es * s->NE, hipMemcpyDeviceToDevice, s->stream), "tune");
+  HIPTRY(hipStreamSynchronize(s->stream), "tune");
+  (void)hipFree(sh);
+  (void)hipFree(su);
+  drop_graph(s);
+  if (out)
+    for (int k = 0; k < 3; ++k) {
+      out[2 * k] = s->kbs[k];
+      out[2 * k + 1] = s->kremap[k];
+    }
+  return check_flag(s, "imx_swm_tune");
 }
 
 }  // namespace imx
@@ -1319,26 +1459,7 @@ const char *imx_swm_kernel_name(imx_swm *s, int32_t k) {
 
 int imx_swm_diagnostics(imx_swm *s, double out[IMX_NDIAG]) {
   if (!s || !out) return imx::set_error(IMX_EINVAL, "imx_swm_diagnostics: NULL argument");
-  if (!s->fast) imx::launch_diag<double>(s, s->rh, s->ru, 0.0);
-  else if (s->single) imx::launch_diag<float>(s, (const float *)s->h0, (const float *)s->u0, s->href);
-  else imx::launch_diag<double>(s, (const double *)s->h0, (const double *)s->u0, s->href);
-  double v[NPART];
-  HIPTRY(hipMemcpyAsync(v, s->diag, sizeof(v), hipMemcpyDeviceToHost, s->stream), "diagnostics");
-  HIPTRY(hipStreamSynchronize(s->stream), "diagnostics");
-  const double mass = v[0] / v[1];
-  const double pe = v[2];
-  const double ke = v[4];
-  // unavailable potential energy of the flat state at the mean height (b = 0)
-  const double upe = v[1] * s->o.grav * mass * (mass * 0.5);
-  out[IMX_DIAG_MASS] = mass;
-  out[IMX_DIAG_PENERGY] = pe;
-  out[IMX_DIAG_KENERGY] = ke;
-  out[IMX_DIAG_TENERGY] = pe + ke;
-  out[IMX_DIAG_AENERGY] = (pe + ke) - upe;
-  out[IMX_DIAG_PENSTROPHY] = v[5];
-  out[IMX_DIAG_RMSDIV] = std::sqrt(v[3] / v[1]);
-  out[IMX_DIAG_BADFLAG] = v[7] > 0 ? 1.0 : 0.0;
-  return IMX_OK;
+  return imx::diag_range(s, 0, s->nv, out);
 }
 
 int imx_swm_info(imx_swm *s, int64_t out[8]) {
@@ -1393,6 +1514,302 @@ void imx_swm_destroy(imx_swm *s) {
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
+}
+
+int imx_swm_tune(imx_swm *s, int32_t nsteps, int32_t out[6]) {
+  if (!s) return imx::set_error(IMX_EINVAL, "imx_swm_tune: NULL handle");
+  return imx::tune(s, nsteps, out);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- ensembles --
+// n independent instances as one disjoint-union mesh: cells ordered by
+// (instance, Hilbert key), so every instance owns contiguous cell, edge and
+// triangle ranges; per-cell/per-edge time steps let each instance run at its
+// own dt while one launch per fused kernel advances all of them.
+struct imx_ens {
+  imx_swm *s = nullptr;
+  int n = 0;
+  std::vector<int32_t> nv, ne, nt, cv0, ce0, c0;  // counts, union caller offsets, internal cell offset
+  std::vector<double> area;                       // union cell areas (caller order)
+};
+
+namespace imx {
+
+template <typename T>
+static int ens_upload(imx_ens *E, int i, const double *h, const double *u) {
+  imx_swm *s = E->s;
+  const int nc = E->nv[i], c0 = E->c0[i];
+  std::vector<T> hb(nc, (T)0), ub((size_t)3 * nc, (T)0);
+  for (int k = 0; k < nc; ++k) hb[s->lay.cell_new[E->cv0[i] + k] - c0] = (T)(h[k] - s->href);
+  for (int l = 0; l < E->ne[i]; ++l) ub[s->lay.edge_new[E->ce0[i] + l] - 3 * c0] = (T)u[l];
+  HIPTRY(hipMemcpyAsync((T *)s->h0 + c0, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice,
+                        s->stream), "imx_ens_set_state");
+  HIPTRY(hipMemcpyAsync((T *)s->u0 + (size_t)3 * c0, ub.data(), ub.size() * sizeof(T),
+                        hipMemcpyHostToDevice, s->stream), "imx_ens_set_state");
+  HIPTRY(hipStreamSynchronize(s->stream), "imx_ens_set_state");
+  return IMX_OK;
+}
+
+template <typename T>
+static int ens_download(imx_ens *E, int i, double *h, double *u) {
+  imx_swm *s = E->s;
+  const int nc = E->nv[i], c0 = E->c0[i];
+  std::vector<T> hb(nc), ub((size_t)3 * nc);
+  HIPTRY(hipMemcpyAsync(hb.data(), (const T *)s->h0 + c0, hb.size() * sizeof(T),
+                        hipMemcpyDeviceToHost, s->stream), "imx_ens_get_state");
+  HIPTRY(hipMemcpyAsync(ub.data(), (const T *)s->u0 + (size_t)3 * c0, ub.size() * sizeof(T),
+                        hipMemcpyDeviceToHost, s->stream), "imx_ens_get_state");
+  HIPTRY(hipStreamSynchronize(s->stream), "imx_ens_get_state");
+  if (h)
+    for (int k = 0; k < nc; ++k) h[k] = (double)hb[s->lay.cell_new[E->cv0[i] + k] - c0] + s->href;
+  if (u)
+    for (int l = 0; l < E->ne[i]; ++l) u[l] = (double)ub[s->lay.edge_new[E->ce0[i] + l] - 3 * c0];
+  return IMX_OK;
+}
+
+}  // namespace imx
+
+extern "C" {
+
+int imx_ens_create(int32_t n, const imx_mesh_desc *descs, const double *dt, const imx_swm_opts *o,
+                   imx_ens **out) {
+  if (!out || !descs || !dt || !o || n < 1) return imx::set_error(IMX_EINVAL, "imx_ens_create: bad argument");
+  *out = nullptr;
+  if (o->precision == IMX_PREC_FP64_REF)
+    return imx::set_error(IMX_ENOTSUP, "imx_ens_create: ensembles run the fused FP32/FP64 kernels");
+  imx_ens *E = new (std::nothrow) imx_ens();
+  if (!E) return imx::set_error(IMX_ENOMEM, "imx_ens_create: out of memory");
+  int rc = IMX_OK;
+  try {
+    E->n = n;
+    int64_t NV = 0, NEd = 0, NTr = 0;
+    bool gp = true;
+    for (int i = 0; i < n; ++i) {
+      int r = imx::validate_desc(descs[i]);
+      if (r) {
+        delete E;
+        return r;
+      }
+      E->nv.push_back(descs[i].nv);
+      E->ne.push_back(descs[i].ne);
+      E->nt.push_back(descs[i].nt);
+      E->cv0.push_back((int32_t)NV);
+      E->ce0.push_back((int32_t)NEd);
+      NV += descs[i].nv;
+      NEd += descs[i].ne;
+      NTr += descs[i].nt;
+      gp = gp && descs[i].tr_trhx_area && descs[i].hx_hxtr_area && descs[i].hx_trskw;
+    }
+    if (NV >= (1 << 29)) {
+      delete E;
+      return imx::set_error(IMX_EINVAL, "imx_ens_create: too many cells in total");
+    }
+    const int MXn = IMX_MAXNB;
+    std::vector<int32_t> v_nnb(NV), v_nb(MXn * NV), v_ed(MXn * NV), v_tr(MXn * NV), ttg(MXn * NV),
+        tnr(MXn * NV), ed_v(2 * NEd), ed_sh(2 * NEd), tr_v(3 * NTr), tr_ed(3 * NTr), tr_tg(3 * NTr),
+        group(NV);
+    std::vector<double> v_p(3 * NV), areag(NV), leng(NEd), dleng(NEd), ed_c(3 * NEd), edhx_c(3 * NEd),
+        tr_area(NTr), tr_c(3 * NTr), tr_lat(NTr), dtc(NV), dte(NEd), trhx, hxtr, trskw;
+    if (gp) {
+      trhx.resize(3 * NTr);
+      hxtr.resize(MXn * NV);
+      trskw.resize((size_t)MXn * MXn * NV);
+    }
+    int64_t ov = 0, oe = 0, ot = 0;
+    for (int i = 0; i < n; ++i) {
+      const imx_mesh_desc &d = descs[i];
+      for (int c = 0; c < d.nv; ++c) {
+        v_nnb[ov + c] = d.v_nnb[c];
+        group[ov + c] = i;
+        dtc[ov + c] = dt[i];
+        areag[ov + c] = d.hx_areag[c];
+        for (int q = 0; q < 3; ++q) v_p[3 * (ov + c) + q] = d.v_p[3 * c + q];
+        for (int j = 0; j < MXn; ++j) {
+          const int64_t k = MXn * (ov + c) + j;
+          const bool on = j < d.v_nnb[c];
+          v_nb[k] = on ? d.v_nb[MXn * c + j] + (int32_t)ov : 0;
+          v_ed[k] = on ? d.v_ed[MXn * c + j] + (int32_t)oe : 0;
+          v_tr[k] = on ? d.v_tr[MXn * c + j] + (int32_t)ot : 0;
+          ttg[k] = d.hx_ttgout[MXn * c + j];
+          tnr[k] = d.hx_tnrccw[MXn * c + j];
+        }
+        if (gp) {
+          for (int j = 0; j < MXn; ++j) hxtr[MXn * (ov + c) + j] = d.hx_hxtr_area[MXn * c + j];
+          for (int j = 0; j < MXn * MXn; ++j) trskw[(size_t)MXn * MXn * (ov + c) + j] = d.hx_trskw[MXn * MXn * c + j];
+        }
+      }
+      for (int l = 0; l < d.ne; ++l) {
+        for (int k = 0; k < 2; ++k) {
+          ed_v[2 * (oe + l) + k] = d.ed_v[2 * l + k] + (int32_t)ov;
+          ed_sh[2 * (oe + l) + k] = d.ed_sh[2 * l + k] + (int32_t)ot;
+        }
+        leng[oe + l] = d.ed_leng[l];
+        dleng[oe + l] = d.edhx_leng[l];
+        dte[oe + l] = dt[i];
+        for (int q = 0; q < 3; ++q) {
+          ed_c[3 * (oe + l) + q] = d.ed_c[3 * l + q];
+          edhx_c[3 * (oe + l) + q] = d.edhx_c[3 * l + q];
+        }
+      }
+      for (int k = 0; k < d.nt; ++k) {
+        for (int q = 0; q < 3; ++q) {
+          tr_v[3 * (ot + k) + q] = d.tr_v[3 * k + q] + (int32_t)ov;
+          tr_ed[3 * (ot + k) + q] = d.tr_ed[3 * k + q] + (int32_t)oe;
+          tr_tg[3 * (ot + k) + q] = d.tr_tg[3 * k + q];
+          tr_c[3 * (ot + k) + q] = d.tr_c[3 * k + q];
+          if (gp) trhx[3 * (ot + k) + q] = d.tr_trhx_area[3 * k + q];
+        }
+        tr_area[ot + k] = d.tr_areag[k];
+        tr_lat[ot + k] = d.tr_clat[k];
+      }
+      ov += d.nv;
+      oe += d.ne;
+      ot += d.nt;
+    }
+    imx_mesh_desc U;
+    std::memset(&U, 0, sizeof(U));
+    U.nv = (int32_t)NV;
+    U.ne = (int32_t)NEd;
+    U.nt = (int32_t)NTr;
+    U.glevel = -1;
+    U.v_nnb = v_nnb.data();
+    U.v_nb = v_nb.data();
+    U.v_ed = v_ed.data();
+    U.v_tr = v_tr.data();
+    U.hx_ttgout = ttg.data();
+    U.hx_tnrccw = tnr.data();
+    U.v_p = v_p.data();
+    U.hx_areag = areag.data();
+    U.ed_v = ed_v.data();
+    U.ed_sh = ed_sh.data();
+    U.ed_leng = leng.data();
+    U.edhx_leng = dleng.data();
+    U.ed_c = ed_c.data();
+    U.edhx_c = edhx_c.data();
+    U.tr_v = tr_v.data();
+    U.tr_ed = tr_ed.data();
+    U.tr_tg = tr_tg.data();
+    U.tr_areag = tr_area.data();
+    U.tr_c = tr_c.data();
+    U.tr_clat = tr_lat.data();
+    if (gp) {
+      U.tr_trhx_area = trhx.data();
+      U.hx_hxtr_area = hxtr.data();
+      U.hx_trskw = trskw.data();
+    }
+    // one layout per distinct mesh (instances sharing a mesh share arrays),
+    // concatenated instance-major with offsets
+    imx::Layout UL;
+    UL.cell_new.resize(NV);
+    UL.edge_new.resize(NEd);
+    UL.tri_new.resize(NTr);
+    UL.NE = 3 * (int)NV;
+    UL.NT = 2 * (int)NV;
+    {
+      std::vector<const double *> keys;
+      std::vector<imx::Layout> lays;
+      int64_t c0 = 0;
+      for (int i = 0; i < n; ++i) {
+        int li = -1;
+        for (size_t q = 0; q < keys.size(); ++q)
+          if (keys[q] == descs[i].v_p) li = (int)q;
+        if (li < 0) {
+          imx::Layout L;
+          std::string err;
+          if (!imx::build_layout(descs[i], o->use_sfc != 0, L, err)) {
+            rc = imx::set_error(IMX_EINVAL, "layout: " + err);
+            break;
+          }
+          keys.push_back(descs[i].v_p);
+          lays.push_back(std::move(L));
+          li = (int)lays.size() - 1;
+        }
+        const imx::Layout &L = lays[li];
+        for (int k = 0; k < descs[i].nv; ++k) UL.cell_new[E->cv0[i] + k] = (int32_t)c0 + L.cell_new[k];
+        for (int l = 0; l < descs[i].ne; ++l) UL.edge_new[E->ce0[i] + l] = 3 * (int32_t)c0 + L.edge_new[l];
+        int64_t t0 = 0;
+        for (int j = 0; j < i; ++j) t0 += descs[j].nt;
+        for (int k = 0; k < descs[i].nt; ++k) UL.tri_new[t0 + k] = 2 * (int32_t)c0 + L.tri_new[k];
+        c0 += descs[i].nv;
+      }
+    }
+    if (!rc) {
+      E->s = new (std::nothrow) imx_swm();
+      if (!E->s) rc = imx::set_error(IMX_ENOMEM, "imx_ens_create: out of memory");
+      else rc = imx::create(U, *o, E->s, group.data(), dtc.data(), dte.data(), &UL);
+    }
+    if (!rc) {
+      int64_t acc = 0;
+      for (int i = 0; i < n; ++i) {
+        E->c0.push_back((int32_t)acc);
+        acc += E->nv[i];
+      }
+      E->area = areag;
+    }
+  } catch (const std::bad_alloc &) {
+    rc = imx::set_error(IMX_ENOMEM, "imx_ens_create: host out of memory");
+  }
+  if (rc) {
+    imx_ens_destroy(E);
+    return rc;
+  }
+  *out = E;
+  return IMX_OK;
+}
+
+int imx_ens_set_state(imx_ens *E, int32_t i, const double *h, const double *u) {
+  if (!E || !h || !u || i < 0 || i >= E->n) return imx::set_error(IMX_EINVAL, "imx_ens_set_state: bad argument");
+  imx_swm *s = E->s;
+  if (!s->href_set) {  // FP32 storage reference: area mean of the first state set
+    double sa = 0.0, sh = 0.0;
+    for (int k = 0; k < E->nv[i]; ++k) {
+      sa += E->area[E->cv0[i] + k];
+      sh += E->area[E->cv0[i] + k] * h[k];
+    }
+    s->href = sh / sa;
+    s->href_set = true;
+  }
+  return s->single ? imx::ens_upload<float>(E, i, h, u) : imx::ens_upload<double>(E, i, h, u);
+}
+
+int imx_ens_get_state(imx_ens *E, int32_t i, double *h, double *u) {
+  if (!E || i < 0 || i >= E->n) return imx::set_error(IMX_EINVAL, "imx_ens_get_state: bad argument");
+  return E->s->single ? imx::ens_download<float>(E, i, h, u) : imx::ens_download<double>(E, i, h, u);
+}
+
+int imx_ens_step(imx_ens *E, int32_t nsteps) {
+  if (!E) return imx::set_error(IMX_EINVAL, "imx_ens_step: NULL handle");
+  return imx_swm_step(E->s, nsteps);
+}
+
+int imx_ens_step_timed(imx_ens *E, int32_t nsteps, double *ms) {
+  if (!E) return imx::set_error(IMX_EINVAL, "imx_ens_step_timed: NULL handle");
+  return imx_swm_step_timed(E->s, nsteps, ms);
+}
+
+int imx_ens_diagnostics(imx_ens *E, int32_t i, double out[IMX_NDIAG]) {
+  if (!E || !out || i < 0 || i >= E->n) return imx::set_error(IMX_EINVAL, "imx_ens_diagnostics: bad argument");
+  return imx::diag_range(E->s, E->c0[i], E->c0[i] + E->nv[i], out);
+}
+
+int imx_ens_tune(imx_ens *E, int32_t nsteps, int32_t out[6]) {
+  if (!E) return imx::set_error(IMX_EINVAL, "imx_ens_tune: NULL handle");
+  return imx::tune(E->s, nsteps, out);
+}
+
+int imx_ens_info(imx_ens *E, int64_t out[8]) {
+  if (!E || !out) return imx::set_error(IMX_EINVAL, "imx_ens_info: NULL argument");
+  int rc = imx_swm_info(E->s, out);
+  out[7] = E->n;
+  return rc;
+}
+
+void imx_ens_destroy(imx_ens *E) {
+  if (!E) return;
+  imx_swm_destroy(E->s);
+  delete E;
 }
 
 }  // extern "C"

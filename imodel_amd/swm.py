@@ -211,6 +211,93 @@ class Swm:
         capi.check(capi.lib().imx_swm_info(self._h, out), "imx_swm_info")
         return list(out)
 
+    def tune(self, nsteps=3):
+        """Autotune block size and XCD remap of each fused kernel."""
+        out = np.zeros(6, dtype=np.int32)
+        capi.check(capi.lib().imx_swm_tune(self._h, int(nsteps), capi.i32p(out)), "imx_swm_tune")
+        return {k: (int(out[2 * i]), int(out[2 * i + 1])) for i, k in enumerate(("tri", "cell", "edge"))}
+
+
+class Ensemble:
+    """Independent instances (meshes, time steps) advanced together on the GPU."""
+
+    def __init__(self, meshes, dts, precision="fp32", erad=ERAD_REF, grav=GRAV, omega=OMEGA,
+                 testcase=2, use_sfc=True, use_graph=True, h_ref=0.0):
+        L = capi.lib()
+        self.meshes = list(meshes)
+        n = len(self.meshes)
+        descs = (capi.MeshDesc * n)(*[m.desc() for m in self.meshes])
+        self._dts = np.ascontiguousarray(dts, dtype=np.float64)
+        o = capi.SwmOpts()
+        L.imx_swm_default_opts(ctypes.byref(o))
+        o.precision = PREC[precision]
+        o.testcase = int(testcase)
+        o.use_sfc, o.use_graph = int(bool(use_sfc)), int(bool(use_graph))
+        o.erad, o.grav, o.omega, o.dt, o.h_ref = erad, grav, omega, float(self._dts[0]), float(h_ref)
+        self._h = ctypes.c_void_p()
+        capi.check(L.imx_ens_create(n, descs, capi.f64p(self._dts), ctypes.byref(o), ctypes.byref(self._h)),
+                   "imx_ens_create")
+        self.n = n
+
+    def close(self):
+        if self._h:
+            capi.lib().imx_ens_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, i, h, u):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        capi.check(capi.lib().imx_ens_set_state(self._h, i, capi.f64p(h), capi.f64p(u)), "imx_ens_set_state")
+
+    def get_state(self, i):
+        m = self.meshes[i]
+        h = np.zeros(m.nv)
+        u = np.zeros(m.ne)
+        capi.check(capi.lib().imx_ens_get_state(self._h, i, capi.f64p(h), capi.f64p(u)), "imx_ens_get_state")
+        return h, u
+
+    def step(self, nsteps=1):
+        capi.check(capi.lib().imx_ens_step(self._h, int(nsteps)), "imx_ens_step")
+
+    def step_timed(self, nsteps):
+        ms = ctypes.c_double()
+        capi.check(capi.lib().imx_ens_step_timed(self._h, int(nsteps), ctypes.byref(ms)), "imx_ens_step_timed")
+        return ms.value
+
+    def diagnostics(self, i):
+        out = np.zeros(capi.NDIAG)
+        capi.check(capi.lib().imx_ens_diagnostics(self._h, i, capi.f64p(out)), "imx_ens_diagnostics")
+        return dict(zip(capi.DIAG_NAMES, out))
+
+    def tune(self, nsteps=2):
+        out = np.zeros(6, dtype=np.int32)
+        capi.check(capi.lib().imx_ens_tune(self._h, int(nsteps), capi.i32p(out)), "imx_ens_tune")
+        return {k: (int(out[2 * i]), int(out[2 * i + 1])) for i, k in enumerate(("tri", "cell", "edge"))}
+
+    def info(self):
+        out = (ctypes.c_int64 * 8)()
+        capi.check(capi.lib().imx_ens_info(self._h, out), "imx_ens_info")
+        return list(out)
+
+
+def cfl_dt(mesh, u, cfl=0.8, erad=ERAD_REF, gh0=2.94e4):
+    """Time step at Courant number cfl for c = sqrt(g H0) + max|u| over the
+    shortest distance between cell centres (min ed%leng)."""
+    c = np.sqrt(gh0) + np.max(np.abs(u))
+    return cfl * np.min(mesh.arrays["ed_leng"]) * erad / c
+
 
 def tendency(swm, h, u):
     """subroutine tendency(h, u, masseq, momeq) (src/swm.f90:2416)."""

@@ -180,6 +180,30 @@ int  imx_swm_diagnostics(imx_swm *s, double out[IMX_NDIAG]);
 int  imx_swm_info(imx_swm *s, int64_t out[8]);   /* nv ne nt bytes_dev ... */
 void imx_swm_destroy(imx_swm *s);
 
+/* Autotuner (no counterpart in the reference): times every block size
+   (64/128/256/512) x XCD-remap choice of each fused kernel over nsteps steps
+   and keeps the fastest; out = {bs, remap} for tri, cell, edge. The state is
+   restored. FAST precisions only. */
+int  imx_swm_tune(imx_swm *s, int32_t nsteps, int32_t out[6]);
+
+/* ---- ensembles: many independent instances in one set of launches --------
+   (no counterpart in the reference; the batched configuration). Instance i
+   has its own mesh descs[i] and time step dt[i]; all instances advance
+   together, one launch per fused kernel per RK stage, one hipGraph per step.
+   FAST precisions only. States and diagnostics are per instance, in that
+   instance's own (caller, 1-based) numbering. */
+typedef struct imx_ens imx_ens;
+int  imx_ens_create(int32_t n, const imx_mesh_desc *descs, const double *dt,
+                    const imx_swm_opts *o, imx_ens **out);
+int  imx_ens_set_state(imx_ens *e, int32_t i, const double *h, const double *u);
+int  imx_ens_get_state(imx_ens *e, int32_t i, double *h, double *u);
+int  imx_ens_step(imx_ens *e, int32_t nsteps);
+int  imx_ens_step_timed(imx_ens *e, int32_t nsteps, double *ms);
+int  imx_ens_diagnostics(imx_ens *e, int32_t i, double out[IMX_NDIAG]);
+int  imx_ens_tune(imx_ens *e, int32_t nsteps, int32_t out[6]);
+int  imx_ens_info(imx_ens *e, int64_t out[8]);  /* as imx_swm_info, out[7] = n */
+void imx_ens_destroy(imx_ens *e);
+
 /* ---- measurement utility (no counterpart in the reference) ----------------
    Streams `bytes` from one device buffer to another `reps` times and returns
    the achieved read+write bandwidth [GB/s]; also the known-size kernel used
