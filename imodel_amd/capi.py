@@ -26,6 +26,7 @@ FIELD_POS = {"h_ed": "e", "uh": "e", "eta": "t", "h_tr": "t", "q_tr": "t",
 
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _F64P = ctypes.POINTER(ctypes.c_double)
+_F32P = ctypes.POINTER(ctypes.c_float)
 
 
 class MeshDesc(ctypes.Structure):
@@ -131,6 +132,15 @@ def lib():
             "imx_ens_tune": (ctypes.c_int, [vp, ctypes.c_int32, _I32P]),
             "imx_ens_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int64)]),
             "imx_ens_destroy": (None, [vp]),
+            "imx_mlp_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(vp)]),
+            "imx_mlp_nparams": (ctypes.c_int, []),
+            "imx_mlp_get_params": (ctypes.c_int, [vp, _F32P]),
+            "imx_mlp_set_params": (ctypes.c_int, [vp, _F32P]),
+            "imx_mlp_forward": (ctypes.c_int, [vp, ctypes.c_int64, _F32P, _F32P]),
+            "imx_mlp_train": (ctypes.c_int, [vp, ctypes.c_int64, _F32P, _F32P, _F32P, ctypes.c_float,
+                                             ctypes.c_float, ctypes.c_int32, _F64P, _F32P]),
+            "imx_mlp_forward_timed": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int32, _F64P]),
+            "imx_mlp_destroy": (None, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -156,3 +166,7 @@ def f64p(a):
 
 def i32p(a):
     return a.ctypes.data_as(_I32P)
+
+
+def f32p(a):
+    return a.ctypes.data_as(_F32P)

@@ -204,6 +204,27 @@ int  imx_ens_tune(imx_ens *e, int32_t nsteps, int32_t out[6]);
 int  imx_ens_info(imx_ens *e, int64_t out[8]);  /* as imx_swm_info, out[7] = n */
 void imx_ens_destroy(imx_ens *e);
 
+/* ---- learned correction (no counterpart in the reference) -----------------
+   Per-edge MLP 48 -> 128 -> 128 -> 128 -> 2, GELU (tanh form), BF16 weights
+   and activations on MFMA with FP32 accumulation; FP32 master parameters
+   updated by AdamW. Parameters are flat FP32: W1[48][128] b1[128]
+   W2[128][128] b2 W3[128][128] b3 W4[128][2] b4[2] (W stored [in][out]). */
+typedef struct imx_mlp imx_mlp;
+int  imx_mlp_create(uint64_t seed, imx_mlp **out);
+int  imx_mlp_nparams(void);
+int  imx_mlp_get_params(imx_mlp *m, float *p);
+int  imx_mlp_set_params(imx_mlp *m, const float *p);
+/* host rows X[n][48] -> Y[n][2] */
+int  imx_mlp_forward(imx_mlp *m, int64_t n, const float *X, float *Y);
+/* L = sum_e w[e] |Y[e] - T[e]|^2: gradient into grad[nparams] (may be NULL),
+   loss into *loss; one AdamW step (lr, decoupled weight decay wd on the
+   matrices) when apply != 0 */
+int  imx_mlp_train(imx_mlp *m, int64_t n, const float *X, const float *T,
+                   const float *w, float lr, float wd, int32_t apply,
+                   double *loss, float *grad);
+int  imx_mlp_forward_timed(imx_mlp *m, int64_t n, int32_t reps, double *ms);
+void imx_mlp_destroy(imx_mlp *m);
+
 /* ---- measurement utility (no counterpart in the reference) ----------------
    Streams `bytes` from one device buffer to another `reps` times and returns
    the achieved read+write bandwidth [GB/s]; also the known-size kernel used
