@@ -28,7 +28,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
 HOST_SRCS = ["imx_mesh.cpp", "imx_ic.cpp", "imx_sfc.cpp"]
-HIP_SRCS = ["imx_swm.hip", "imx_mlp.hip"]
+HIP_SRCS = ["imx_swm.hip", "imx_mlp.hip", "imx_learn.hip"]
 
 
 def _run(cmd):

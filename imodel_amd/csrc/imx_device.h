@@ -1,0 +1,127 @@
+// Device-side handle definitions shared by the HIP translation units of
+// libimx (the step, the MLP and the learned-correction coupling).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "imx_internal.h"
+
+namespace imx {
+
+struct Alloc {
+  std::vector<void *> ptrs;
+  size_t bytes = 0;
+  hipError_t err = hipSuccess;
+  template <typename X>
+  X *get(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(X));
+    if (e != hipSuccess) {
+      err = e;
+      return nullptr;
+    }
+    e = hipMemset(p, 0, n * sizeof(X));  // dummy slots read as zeros
+    if (e != hipSuccess) err = e;
+    ptrs.push_back(p);
+    bytes += n * sizeof(X);
+    return (X *)p;
+  }
+  template <typename X>
+  X *put(const std::vector<X> &h) {
+    X *p = get<X>(h.size());
+    if (p && !h.empty()) {
+      hipError_t e = hipMemcpy(p, h.data(), h.size() * sizeof(X), hipMemcpyHostToDevice);
+      if (e != hipSuccess) err = e;
+    }
+    return p;
+  }
+  void release() {
+    for (void *p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+  }
+};
+
+}  // namespace imx
+
+struct imx_swm {
+  imx_swm_opts o{};
+  int nv = 0, ne = 0, nt = 0;  // caller counts
+  int NE = 0, NT = 0;          // internal edge/triangle slots (3 nv, 2 nv)
+  imx::Layout lay;             // caller -> internal maps
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  imx::Alloc A;
+  bool fast = false;
+  bool single = false;  // FAST with T=float
+  double href = 0.0;
+  bool href_set = false;
+
+  // geometry / REF tables (always present), internal numbering
+  int32_t *c_nnb = nullptr, *c_ed = nullptr;
+  int8_t *c_ttg = nullptr, *c_tnr = nullptr;
+  double *c_area = nullptr, *c_trskw = nullptr;
+  int32_t *e_v = nullptr, *e_sh = nullptr;
+  int8_t *e_loc = nullptr;
+  double *e_leng = nullptr, *e_dleng = nullptr;
+  int32_t *t_ed = nullptr, *t_v = nullptr;
+  int8_t *t_tg = nullptr;
+  double *t_area = nullptr, *t_f = nullptr, *t_trhx = nullptr;
+  double *bt = nullptr;
+  int *flag = nullptr;
+  double *part = nullptr, *diag = nullptr;
+  int nparts = 0;
+
+  // REF state and fields
+  double *rh = nullptr, *ru = nullptr, *rhn = nullptr, *run = nullptr;
+  double *massf = nullptr, *momf = nullptr;
+  double *f_h_ed = nullptr, *f_uh = nullptr, *f_eta = nullptr, *f_h_tr = nullptr,
+         *f_q_tr = nullptr, *f_q_ed = nullptr, *f_divuh = nullptr, *f_ke = nullptr,
+         *f_ghbk = nullptr, *f_grad = nullptr, *f_uhq = nullptr, *f_divu = nullptr,
+         *f_zeta = nullptr;
+
+  // FAST tables and state (typed by precision; void* holds T*)
+  int32_t *x_c_nb = nullptr, *x_c_bitsA = nullptr, *x_c_bitsB = nullptr;
+  void *x_c_R = nullptr, *x_c_ika = nullptr;
+  double *x_c_iaa = nullptr;
+  double *x_e_dl = nullptr;
+  void *x_e_kw = nullptr, *x_e_il = nullptr;
+  int32_t *x_e_oth = nullptr;
+  int32_t *x_t_ab = nullptr, *x_t_eb = nullptr;
+  void *x_t_ceta = nullptr, *x_t_kap = nullptr, *x_t_f = nullptr;
+  void *h0 = nullptr, *hA = nullptr, *hB = nullptr, *u0 = nullptr, *uA = nullptr, *uB = nullptr;
+  void *acch = nullptr, *accu = nullptr, *q = nullptr, *phi = nullptr, *Pown = nullptr,
+       *Poth = nullptr;
+  double *dh_out = nullptr;
+  void *du_out = nullptr;
+  // ensembles: per-cell / per-edge-slot time step (nullptr: scalar o.dt)
+  double *dt_c = nullptr;
+  void *dt_e = nullptr;
+  // launch configuration per fused kernel (tri, cell, edge): block size and
+  // XCD remap, chosen by imx_swm_tune
+  int kbs[3] = {256, 256, 256};
+  int kremap[3] = {1, 1, 1};
+};
+
+// MLP handle: FP32 master parameters with AdamW moments, BF16 fragment
+// copies of the weights, and the scratch of the training pass.
+struct imx_mlp {
+  float *P = nullptr, *M = nullptr, *V = nullptr, *G = nullptr, *part = nullptr, *lossp = nullptr;
+  short *Ff = nullptr, *Fb = nullptr;
+  short *X = nullptr, *AT = nullptr, *DZT = nullptr;
+  float *T = nullptr, *w = nullptr, *Y = nullptr, *Xf = nullptr;
+  int64_t cap = 0;  // edges the scratch holds
+  int nsplit = 16;
+  int64_t t = 0;    // optimiser step
+  hipStream_t stream = nullptr;
+};
+
+namespace imx {
+int mlp_reserve(imx_mlp *m, int64_t n);
+int mlp_forward_dev(imx_mlp *m, int64_t n, const short *X, float *Y, hipStream_t st);
+int mlp_train_dev(imx_mlp *m, int64_t n, float lr, float wd, int apply, double *loss);
+}  // namespace imx

@@ -23,7 +23,7 @@
 #include <string>
 #include <vector>
 
-#include "imx_internal.h"
+#include "imx_device.h"
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -401,16 +401,6 @@ __global__ void f32_to_bf16(int64_t n, const float *__restrict__ x, short *__res
 }
 
 // ================================================================= handle ==
-struct imx_mlp {
-  float *P = nullptr, *M = nullptr, *V = nullptr, *G = nullptr, *part = nullptr, *lossp = nullptr;
-  short *Ff = nullptr, *Fb = nullptr;
-  short *X = nullptr, *AT = nullptr, *DZT = nullptr;
-  float *T = nullptr, *w = nullptr, *Y = nullptr, *Xf = nullptr;
-  int64_t cap = 0;  // edges the scratch holds
-  int nsplit = 16;
-  int64_t t = 0;    // optimiser step
-  hipStream_t stream = nullptr;
-};
 
 namespace imx {
 
@@ -423,7 +413,7 @@ static int mlp_fail(hipError_t e, const char *what) {
     if (_e != hipSuccess) return mlp_fail(_e, what); \
   } while (0)
 
-static int mlp_reserve(imx_mlp *m, int64_t n) {
+int mlp_reserve(imx_mlp *m, int64_t n) {
   const int64_t npad = ((n + 63) / 64) * 64;
   if (npad <= m->cap) return IMX_OK;
   (void)hipFree(m->X);

@@ -225,6 +225,24 @@ int  imx_mlp_train(imx_mlp *m, int64_t n, const float *X, const float *T,
 int  imx_mlp_forward_timed(imx_mlp *m, int64_t n, int32_t reps, double *ms);
 void imx_mlp_destroy(imx_mlp *m);
 
+/* Learned correction bound to a coarse FP32/FP64 fused handle (d: the mesh
+   the handle was created from). imx_learn_sample saves the state, takes one
+   plain step and builds the 48 raw features per edge (copied to raw[ne][48]
+   in caller edge order when raw != NULL). imx_learn_train fits the MLP to the
+   correction du[ne], dh[nv] of that step (`epochs` AdamW steps).
+   imx_learn_step runs corrected steps: plain step + MLP correction, the
+   plain result kept when the correction's RMS (scaled) exceeds `bound`. */
+typedef struct imx_learn imx_learn;
+int  imx_learn_create(imx_swm *coarse, imx_mlp *m, const imx_mesh_desc *d, imx_learn **out);
+int  imx_learn_sample(imx_learn *l, float *raw);
+int  imx_learn_set_scaling(imx_learn *l, const float mu[48], const float sig[48],
+                           float tscale_u, float tscale_h);
+int  imx_learn_train(imx_learn *l, const double *du, const double *dh, int32_t epochs,
+                     float lr, float wd, double *loss);
+int  imx_learn_step(imx_learn *l, int32_t nsteps, double bound, int32_t *nfallback);
+int  imx_learn_last_rms(imx_learn *l, double *rms);
+void imx_learn_destroy(imx_learn *l);
+
 /* ---- measurement utility (no counterpart in the reference) ----------------
    Streams `bytes` from one device buffer to another `reps` times and returns
    the achieved read+write bandwidth [GB/s]; also the known-size kernel used
