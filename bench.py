@@ -103,10 +103,15 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=8)
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the streaming-copy bandwidth probe (0: skip)")
-    ap.add_argument("--workload", default="single", choices=["single", "ensemble"],
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "learned"],
                     help="single: one instance at --glevel; ensemble: --instances instances, "
-                         "levels uniform in 4..7, CFL-0.8 time steps")
+                         "levels uniform in 4..7, CFL-0.8 time steps; learned: glevel-6 model with "
+                         "the learned correction vs plain glevel 6 and 7 against restricted glevel 8")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--train-seeds", type=int, default=24)
+    ap.add_argument("--train-samples", type=int, default=48)
+    ap.add_argument("--fit-steps", type=int, default=6000)
+    ap.add_argument("--eval-steps", type=int, default=192)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -117,6 +122,8 @@ def main():
 
     if args.workload == "ensemble":
         return bench_ensemble(args)
+    if args.workload == "learned":
+        return bench_learned(args)
 
     t = time.perf_counter()
     mesh = Mesh.icos(args.glevel)
@@ -266,6 +273,65 @@ def bench_ensemble(args):
                      "kernel": "ensemble RK4 step (one hipGraph launch)", "algorithmic_bytes_per_launch": B,
                      "launch_ms": ev_ms / args.steps},
         "instances_with_nan": int(bad),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_learned(args):
+    """Learned correction (glevel 6, dt 450 s) against the plain glevel-6 and
+    glevel-7 (dt 225 s) models, all started from the FP64 glevel-8 run
+    restricted to their grids and compared on glevel 6 after --eval-steps
+    coarse steps (192 = 1 day). Training: --train-seeds runs x
+    --train-samples one-step samples produced by this solver, stored on the
+    device, --fit-steps shuffled AdamW steps. Reports errors, wall times per
+    simulated interval, fallbacks and the MLP kernel's MFMA throughput."""
+    from imodel_amd.learn import Trainer, evaluate, restriction
+    from imodel_amd.swm import H0_GEO, Mesh, Swm
+
+    t0 = time.perf_counter()
+    tr = Trainer(gc=6, gf=8, dt_c=450.0, log=log)
+    tr.calibrate([1000, 1001], nsamples=8, spinup=4)
+    tc = time.perf_counter()
+    tr.collect(list(range(1, args.train_seeds + 1)), args.train_samples, spinup=4)
+    t_collect = time.perf_counter() - tc
+    tc = time.perf_counter()
+    fit = tr.fit(args.fit_steps, 1e-3, 5e-5, 1e-4)
+    t_fit = time.perf_counter() - tc
+    m7 = Mesh.icos(7)
+    mid = (Swm(m7, precision="fp32", dt=225.0, h_ref=H0_GEO),) + restriction(tr.fine_mesh, m7) + \
+        restriction(m7, tr.coarse_mesh)
+    res = [evaluate(tr, sd, args.eval_steps, spinup=4, mid=mid) for sd in (9999, 8888)]
+    for r in res:
+        log(json.dumps(r))
+    ne6 = 3 * tr.coarse_mesh.nv
+    fwd_ms = tr.mlp.forward_timed(ne6, reps=50)
+    flop = 2.0 * (48 * 128 + 2 * 128 * 128 + 128 * 2) * ne6
+    tflops = flop / (fwd_ms * 1e-3) / 1e12
+
+    def mean(k, f):
+        return float(np.mean([r[k][f] for r in res]))
+
+    speed = mean("mid", "wall_s") / mean("learned", "wall_s")
+    out = {
+        "metric": "learned glevel-6 vs plain glevel-7: error ratio and wall-time speedup (1 day)",
+        "value": speed, "unit": "x (plain glevel-7 wall time / learned glevel-6 wall time)",
+        "n_gpus": 1, "steps": args.eval_steps, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 state, bf16 MFMA network",
+        "data": "synthetic (seeded spectral states; truth = FP64 glevel-8 run of this solver)",
+        "config": {"workload": "learned correction, glevel 6 dt 450 s vs glevel 7 dt 225 s vs glevel 8 truth",
+                   "train_samples": args.train_seeds * args.train_samples, "fit_steps": args.fit_steps,
+                   "collect_s": t_collect, "fit_s": t_fit, "setup_s": time.perf_counter() - t0},
+        "errors": {k: {"h": mean(k, "h"), "u": mean(k, "u"), "wall_s": mean(k, "wall_s")}
+                   for k in ("plain", "learned", "mid")},
+        "learned_vs_mid_error": {"h": mean("learned", "h") / mean("mid", "h"),
+                                 "u": mean("learned", "u") / mean("mid", "u")},
+        "fallbacks": [r["learned"]["fallbacks"] for r in res],
+        "mass_drift_learned": [r["learned"]["mass_drift"] for r in res],
+        "fit_loss_last": fit[-1].tolist(),
+        "roofline": {"bound": "mfma", "kernel": "mlp_fwd (48-128-128-128-2, bf16 MFMA)",
+                     "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0,
+                     "traffic": None, "launch_ms": fwd_ms, "edges": ne6},
+        "runs": res,
     }
     print(json.dumps(out), flush=True)
 

@@ -138,7 +138,7 @@ def lib():
             "imx_mlp_set_params": (ctypes.c_int, [vp, _F32P]),
             "imx_mlp_forward": (ctypes.c_int, [vp, ctypes.c_int64, _F32P, _F32P]),
             "imx_mlp_train": (ctypes.c_int, [vp, ctypes.c_int64, _F32P, _F32P, _F32P, ctypes.c_float,
-                                             ctypes.c_float, ctypes.c_int32, _F64P, _F32P]),
+                                             ctypes.c_float, ctypes.c_int32, _F64P, _F32P, ctypes.c_int32]),
             "imx_mlp_forward_timed": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int32, _F64P]),
             "imx_mlp_destroy": (None, [vp]),
             "imx_learn_create": (ctypes.c_int, [vp, vp, ctypes.POINTER(MeshDesc), ctypes.POINTER(vp)]),
@@ -146,7 +146,11 @@ def lib():
             "imx_learn_set_scaling": (ctypes.c_int, [vp, _F32P, _F32P, ctypes.c_float, ctypes.c_float]),
             "imx_learn_train": (ctypes.c_int, [vp, _F64P, _F64P, ctypes.c_int32, ctypes.c_float,
                                                ctypes.c_float, _F64P]),
-            "imx_learn_step": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_double, _I32P]),
+            "imx_learn_reserve": (ctypes.c_int, [vp, ctypes.c_int32]),
+    "imx_learn_store": (ctypes.c_int, [vp, _F64P, _F64P, _I32P]),
+    "imx_learn_fit": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_uint64, _F64P]),
+    "imx_learn_step": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_double, _I32P]),
             "imx_learn_last_rms": (ctypes.c_int, [vp, _F64P]),
             "imx_learn_destroy": (None, [vp]),
         }

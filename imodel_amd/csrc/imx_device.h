@@ -123,5 +123,10 @@ struct imx_mlp {
 namespace imx {
 int mlp_reserve(imx_mlp *m, int64_t n);
 int mlp_forward_dev(imx_mlp *m, int64_t n, const short *X, float *Y, hipStream_t st);
-int mlp_train_dev(imx_mlp *m, int64_t n, float lr, float wd, int apply, double *loss);
+int mlp_train_dev(imx_mlp *m, int64_t n, const short *X, float lr, float wd, int apply, double *loss,
+                  int gmask);
+// one fused RK4 step of a FAST handle, enqueued on its stream (no graph)
+void swm_launch_step(imx_swm *s);
+// waits for the handle's stream; IMX_ESTATE if a step met a vanishing depth
+int swm_check_flag(imx_swm *s, const char *what);
 }  // namespace imx

@@ -12,8 +12,15 @@
 //   32-41  the plain step's increment of the same
 //   42-47  z and sqrt(1-z^2) of the edge midpoint, l_e, l~_e, A_c1, A_c2
 //          relative to their means
-// Output: (du_e, dh_e); the cell gets the mean dh_e of its edges. The
-// correction is skipped (plain step kept) when its RMS exceeds a bound.
+// Outputs per edge: du_e, added to the edge velocity, and a mass flux F_e
+// whose discrete divergence D F is added to the cell depths,
+//   (D F)_c = -kappa/A_c sum_{e in c} ttgout_ce l~_e F_e,
+// so the h correction moves mass between neighbours and the total mass
+// sum_c A_c h_c is unchanged. Training fits du to its target directly and
+// fits D F to the dh target through D (the gradient D^T r is handed to the
+// MLP backward pass). The correction is skipped (plain step kept) when its
+// RMS exceeds a bound. The corrected step (save, RK4 step, features, MLP,
+// correction) is one hipGraph launch.
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
@@ -33,26 +40,49 @@
 struct imx_learn {
   imx_swm *s = nullptr;
   imx_mlp *m = nullptr;
-  int nv = 0, NE = 0, nvalid = 0;
+  int nv = 0, NE = 0;
   imx::Alloc A;
   int32_t *st_e = nullptr;  // [NSE][NE], -1 pad / dummy
   float *st_s = nullptr;    // [NSE][NE]
   int32_t *st_c = nullptr;  // [NSC][NE]
   float *geo = nullptr;     // [6][NE]
+  float *dc = nullptr;      // [6][nv] coefficient of F_{c_ed[j][c]} in (D F)_c
+  float *de = nullptr;      // [2][NE] coefficient of F_e in (D F) at e_v[0/1][e]
   void *hn = nullptr, *un = nullptr;  // state before the step (T)
   float *raw = nullptr;     // [NE][NF]
   float *mu = nullptr, *sig = nullptr;
   float tsu = 1.f, tsh = 1.f;
   float *Y = nullptr;       // [NE][2]
-  float *wv = nullptr;      // [NE]
-  double *rms = nullptr;    // [1]
-  int *fb = nullptr;        // [1] fallback flag of the last step
-  double bound = 1e300;
+  float *wv = nullptr;      // [NE] 1/ne on real edges
+  float *dhc = nullptr;     // [nv] (D F)_c, or its residual in training
+  float *th = nullptr;      // [nv] scaled dh target
+  double *rms = nullptr;    // [1] correction RMS of the last step
+  double *bound = nullptr;  // [1]
+  double *lh = nullptr;     // [1] h loss of the last training epoch
+  int *nfb = nullptr;       // [1] fallback steps since the counter was cleared
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const short *gX = nullptr;  // MLP input buffer the graph was captured with
+  // device sample store: normalised features, scaled targets
+  int cap = 0, count = 0;
+  int64_t xstride = 0;      // shorts per stored feature matrix
+  short *Xb = nullptr;      // [cap][xstride]
+  float *tub = nullptr;     // [cap][NE] scaled du target, internal order
+  float *thb = nullptr;     // [cap][nv] scaled dh target
 };
 
 __device__ __forceinline__ short lf2bf(float x) {
   __hip_bfloat16 b = __float2bfloat16(x);
   return *reinterpret_cast<short *>(&b);
+}
+
+template <typename T>
+__global__ __launch_bounds__(LTPB) void learn_save(int nv, int NE, const T *__restrict__ h,
+                                                   const T *__restrict__ u, T *__restrict__ hn,
+                                                   T *__restrict__ un) {
+  const int i = blockIdx.x * LTPB + threadIdx.x;
+  if (i < NE) un[i] = u[i];
+  if (i < nv) hn[i] = h[i];
 }
 
 template <typename T>
@@ -97,6 +127,18 @@ __global__ __launch_bounds__(LTPB) void learn_feat(int NE, const int32_t *__rest
   for (int k = 0; k < 6; ++k) f[2 * NSE + 2 * NSC + k] = geo[k * NE + e];
 }
 
+// T[e] = (scaled du target, scale * (D^T r)_e) for the store's training step
+__global__ __launch_bounds__(LTPB) void learn_targets(int NE, const int32_t *__restrict__ e_v,
+                                                      const float *__restrict__ de,
+                                                      const float *__restrict__ r, float scale,
+                                                      const float *__restrict__ tu, float *__restrict__ T) {
+  const int e = blockIdx.x * LTPB + threadIdx.x;
+  if (e >= NE) return;
+  const int c1 = e_v[e], c2 = e_v[NE + e];
+  T[2 * e] = tu[e];
+  T[2 * e + 1] = (c1 < 0) ? 0.f : scale * (de[e] * r[c1] + de[NE + e] * r[c2]);
+}
+
 __global__ __launch_bounds__(LTPB) void learn_norm(int64_t n, const float *__restrict__ raw,
                                                    const float *__restrict__ mu,
                                                    const float *__restrict__ sig, short *__restrict__ X) {
@@ -106,11 +148,41 @@ __global__ __launch_bounds__(LTPB) void learn_norm(int64_t n, const float *__res
   X[i] = lf2bf((raw[i] - mu[j]) / sig[j]);
 }
 
-// weighted RMS of the velocity output (one block)
-__global__ __launch_bounds__(1024) void learn_rms(int NE, const float *__restrict__ Y,
-                                                  const float *__restrict__ w, double *__restrict__ out) {
+// out_c = (D F)_c - t_c (t may be NULL), F = the flux output Y[.][1]
+__global__ __launch_bounds__(LTPB) void learn_div(int nv, const int32_t *__restrict__ c_nnb,
+                                                  const int32_t *__restrict__ c_ed,
+                                                  const float *__restrict__ dc, const float *__restrict__ Y,
+                                                  const float *__restrict__ t, float *__restrict__ out) {
+  const int c = blockIdx.x * LTPB + threadIdx.x;
+  if (c >= nv) return;
+  const int n = c_nnb[c];
+  float v = 0.f;
+  for (int j = 0; j < n; ++j) v += dc[j * nv + c] * Y[2 * c_ed[j * nv + c] + 1];
+  out[c] = t ? v - t[c] : v;
+}
+
+// dL/dF_e = scale * (D^T r)_e into T[e][1] (the MLP's gradient input)
+__global__ __launch_bounds__(LTPB) void learn_divT(int NE, const int32_t *__restrict__ e_v,
+                                                   const float *__restrict__ de, const float *__restrict__ r,
+                                                   float scale, float *__restrict__ T) {
+  const int e = blockIdx.x * LTPB + threadIdx.x;
+  if (e >= NE) return;
+  const int c1 = e_v[e], c2 = e_v[NE + e];
+  T[2 * e + 1] = (c1 < 0) ? 0.f : scale * (de[e] * r[c1] + de[NE + e] * r[c2]);
+}
+
+// out = sqrt(sum_e w_e a_e^2 + sb sum_c b_c^2) (one block; a strided by sa)
+__global__ __launch_bounds__(1024) void learn_norm2(int na, const float *__restrict__ a, int sa,
+                                                    const float *__restrict__ w, int nb,
+                                                    const float *__restrict__ b, double sb, int take_sqrt,
+                                                    double *__restrict__ out) {
   double s = 0.0;
-  for (int e = threadIdx.x; e < NE; e += 1024) s += (double)w[e] * (double)Y[2 * e] * (double)Y[2 * e];
+  if (a)
+    for (int e = threadIdx.x; e < na; e += 1024) s += (double)w[e] * (double)a[sa * e] * (double)a[sa * e];
+  double t = 0.0;
+  if (b)
+    for (int c = threadIdx.x; c < nb; c += 1024) t += (double)b[c] * (double)b[c];
+  s += sb * t;
   __shared__ double red[1024];
   red[threadIdx.x] = s;
   __syncthreads();
@@ -118,33 +190,30 @@ __global__ __launch_bounds__(1024) void learn_rms(int NE, const float *__restric
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = sqrt(red[0]);
+  if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(red[0]) : red[0];
 }
 
 template <typename T>
 __global__ __launch_bounds__(LTPB) void learn_apply_u(int NE, const int32_t *__restrict__ st_e,
                                                       const float *__restrict__ Y, float inv_tsu,
-                                                      const double *__restrict__ rms, double bound,
-                                                      T *u, int *__restrict__ fb) {
+                                                      const double *__restrict__ rms,
+                                                      const double *__restrict__ bound, T *u,
+                                                      int *__restrict__ nfb) {
   const int e = blockIdx.x * LTPB + threadIdx.x;
   if (e >= NE) return;
-  const bool ok = rms[0] <= bound;
-  if (e == 0) fb[0] = ok ? 0 : 1;
+  const bool ok = rms[0] <= bound[0];
+  if (e == 0 && !ok) nfb[0] = nfb[0] + 1;  // one thread of the grid counts
   if (!ok || st_e[e] < 0) return;
   u[e] = (T)((float)u[e] + Y[2 * e] * inv_tsu);
 }
 
 template <typename T>
-__global__ __launch_bounds__(LTPB) void learn_apply_h(int nv, const int32_t *__restrict__ c_nnb,
-                                                      const int32_t *__restrict__ c_ed,
-                                                      const float *__restrict__ Y, float inv_tsh,
-                                                      const double *__restrict__ rms, double bound, T *h) {
+__global__ __launch_bounds__(LTPB) void learn_apply_h(int nv, const float *__restrict__ dhc, float inv_tsh,
+                                                      const double *__restrict__ rms,
+                                                      const double *__restrict__ bound, T *h) {
   const int c = blockIdx.x * LTPB + threadIdx.x;
-  if (c >= nv || !(rms[0] <= bound)) return;
-  const int n = c_nnb[c];
-  float s = 0.f;
-  for (int j = 0; j < n; ++j) s += Y[2 * c_ed[j * nv + c] + 1];
-  h[c] = (T)((float)h[c] + (s / (float)n) * inv_tsh);
+  if (c >= nv || !(rms[0] <= bound[0])) return;
+  h[c] = (T)((float)h[c] + dhc[c] * inv_tsh);
 }
 
 namespace imx {
@@ -158,57 +227,89 @@ static int lfail(hipError_t e, const char *what) {
     if (_e != hipSuccess) return imx::lfail(_e, what); \
   } while (0)
 
-// save x_n, one plain step, raw features of (x_n, x_plain)
-static int learn_sample(imx_learn *L) {
+static void learn_drop_graph(imx_learn *L) {
+  if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
+  if (L->graph) (void)hipGraphDestroy(L->graph);
+  L->gexec = nullptr;
+  L->graph = nullptr;
+}
+
+// enqueue: save x_n, one plain step, raw features of (x_n, x_plain)
+template <typename T>
+static void learn_enqueue_sample(imx_learn *L) {
   imx_swm *s = L->s;
-  const size_t es = s->single ? 4 : 8;
-  LTRY(hipMemcpyAsync(L->hn, s->h0, es * s->nv, hipMemcpyDeviceToDevice, s->stream), "learn save");
-  LTRY(hipMemcpyAsync(L->un, s->u0, es * s->NE, hipMemcpyDeviceToDevice, s->stream), "learn save");
-  int rc = imx_swm_step(s, 1);
-  if (rc) return rc;
+  hipStream_t st = s->stream;
   const int g = (L->NE + LTPB - 1) / LTPB;
-  if (s->single)
-    learn_feat<float><<<g, LTPB, 0, s->stream>>>(L->NE, L->st_e, L->st_s, L->st_c, L->geo,
-                                                (const float *)L->un, (const float *)L->hn,
-                                                (const float *)s->u0, (const float *)s->h0, L->raw);
-  else
-    learn_feat<double><<<g, LTPB, 0, s->stream>>>(L->NE, L->st_e, L->st_s, L->st_c, L->geo,
-                                                 (const double *)L->un, (const double *)L->hn,
-                                                 (const double *)s->u0, (const double *)s->h0, L->raw);
-  LTRY(hipGetLastError(), "learn_feat");
+  learn_save<T><<<g, LTPB, 0, st>>>(s->nv, L->NE, (const T *)s->h0, (const T *)s->u0, (T *)L->hn,
+                                    (T *)L->un);
+  swm_launch_step(s);
+  learn_feat<T><<<g, LTPB, 0, st>>>(L->NE, L->st_e, L->st_s, L->st_c, L->geo, (const T *)L->un,
+                                    (const T *)L->hn, (const T *)s->u0, (const T *)s->h0, L->raw);
+}
+
+static void learn_enqueue_norm(imx_learn *L, hipStream_t st, short *X = nullptr) {
+  const int64_t n = (int64_t)L->NE * NF;
+  learn_norm<<<(unsigned)((n + LTPB - 1) / LTPB), LTPB, 0, st>>>(L->NE, L->raw, L->mu, L->sig,
+                                                                 X ? X : L->m->X);
+}
+
+static void learn_free_store(imx_learn *L) {
+  (void)hipFree(L->Xb);
+  (void)hipFree(L->tub);
+  (void)hipFree(L->thb);
+  L->Xb = nullptr;
+  L->tub = L->thb = nullptr;
+  L->cap = L->count = 0;
+}
+
+// enqueue: features -> MLP -> correction (with the RMS bound) on the state
+template <typename T>
+static int learn_enqueue_correct(imx_learn *L) {
+  imx_swm *s = L->s;
+  hipStream_t st = s->stream;
+  learn_enqueue_norm(L, st);
+  int rc = mlp_forward_dev(L->m, L->NE, L->m->X, L->Y, st);
+  if (rc) return rc;
+  const int g = (L->NE + LTPB - 1) / LTPB, gc = (s->nv + LTPB - 1) / LTPB;
+  learn_div<<<gc, LTPB, 0, st>>>(s->nv, s->c_nnb, s->c_ed, L->dc, L->Y, nullptr, L->dhc);
+  learn_norm2<<<1, 1024, 0, st>>>(L->NE, L->Y, 2, L->wv, s->nv, L->dhc, 1.0 / s->nv, 1, L->rms);
+  learn_apply_u<T><<<g, LTPB, 0, st>>>(L->NE, L->st_e, L->Y, 1.f / L->tsu, L->rms, L->bound, (T *)s->u0,
+                                       L->nfb);
+  learn_apply_h<T><<<gc, LTPB, 0, st>>>(s->nv, L->dhc, 1.f / L->tsh, L->rms, L->bound, (T *)s->h0);
   return IMX_OK;
 }
 
-static int learn_normalise(imx_learn *L) {
+static int learn_sample(imx_learn *L) {
+  if (L->s->single) learn_enqueue_sample<float>(L);
+  else learn_enqueue_sample<double>(L);
+  LTRY(hipGetLastError(), "learn sample");
+  return IMX_OK;
+}
+
+static int learn_ensure_graph(imx_learn *L) {
   int rc = mlp_reserve(L->m, L->NE);
   if (rc) return rc;
-  const int64_t n = (int64_t)L->NE * NF;
-  learn_norm<<<(unsigned)((n + LTPB - 1) / LTPB), LTPB, 0, L->s->stream>>>(L->NE, L->raw, L->mu, L->sig,
-                                                                           L->m->X);
-  LTRY(hipGetLastError(), "learn_norm");
-  return IMX_OK;
-}
-
-static int learn_correct(imx_learn *L, double bound) {
-  imx_swm *s = L->s;
-  int rc = learn_normalise(L);
-  if (rc) return rc;
-  rc = mlp_forward_dev(L->m, L->NE, L->m->X, L->Y, s->stream);
-  if (rc) return rc;
-  learn_rms<<<1, 1024, 0, s->stream>>>(L->NE, L->Y, L->wv, L->rms);
-  const int g = (L->NE + LTPB - 1) / LTPB, gc = (s->nv + LTPB - 1) / LTPB;
-  if (s->single) {
-    learn_apply_u<float><<<g, LTPB, 0, s->stream>>>(L->NE, L->st_e, L->Y, 1.f / L->tsu, L->rms, bound,
-                                                   (float *)s->u0, L->fb);
-    learn_apply_h<float><<<gc, LTPB, 0, s->stream>>>(s->nv, s->c_nnb, s->c_ed, L->Y, 1.f / L->tsh, L->rms,
-                                                    bound, (float *)s->h0);
+  if (L->gexec && L->gX == L->m->X) return IMX_OK;
+  learn_drop_graph(L);
+  hipStream_t st = L->s->stream;
+  LTRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "learn graph capture");
+  if (L->s->single) {
+    learn_enqueue_sample<float>(L);
+    rc = learn_enqueue_correct<float>(L);
   } else {
-    learn_apply_u<double><<<g, LTPB, 0, s->stream>>>(L->NE, L->st_e, L->Y, 1.f / L->tsu, L->rms, bound,
-                                                    (double *)s->u0, L->fb);
-    learn_apply_h<double><<<gc, LTPB, 0, s->stream>>>(s->nv, s->c_nnb, s->c_ed, L->Y, 1.f / L->tsh, L->rms,
-                                                     bound, (double *)s->h0);
+    learn_enqueue_sample<double>(L);
+    rc = learn_enqueue_correct<double>(L);
   }
-  LTRY(hipGetLastError(), "learn_apply");
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &gr);
+  if (rc) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  LTRY(e, "learn graph capture end");
+  L->graph = gr;
+  LTRY(hipGraphInstantiate(&L->gexec, L->graph, nullptr, nullptr, 0), "learn graph instantiate");
+  L->gX = L->m->X;
   return IMX_OK;
 }
 
@@ -228,10 +329,11 @@ int imx_learn_create(imx_swm *s, imx_mlp *m, const imx_mesh_desc *d, imx_learn *
   L->m = m;
   L->nv = s->nv;
   L->NE = s->NE;
-  const int NE = s->NE, MX = IMX_MAXNB;
+  const int NE = s->NE, nv = s->nv, MX = IMX_MAXNB;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new;
   std::vector<int32_t> st_e((size_t)NSE * NE, -1), st_c((size_t)NSC * NE, -1);
   std::vector<float> st_s((size_t)NSE * NE, 0.f), geo((size_t)6 * NE, 0.f), wv(NE, 0.f);
+  std::vector<float> dc((size_t)6 * nv, 0.f), de((size_t)2 * NE, 0.f);
   double ml = 0, md = 0, ma = 0;
   for (int eo = 0; eo < d->ne; ++eo) {
     ml += d->ed_leng[eo];
@@ -241,6 +343,18 @@ int imx_learn_create(imx_swm *s, imx_mlp *m, const imx_mesh_desc *d, imx_learn *
   ml /= d->ne;
   md /= d->ne;
   ma /= d->nv;
+  // (D F)_c coefficients, scaled by kappa = mean area / mean dual length
+  const double kappa = ma / md;
+  for (int co = 0; co < d->nv; ++co) {
+    const int c = cn[co];
+    for (int j = 0; j < d->v_nnb[co]; ++j) {
+      const int eo = d->v_ed[MX * co + j] - 1, e = en[eo];
+      const double sg = d->hx_ttgout[MX * co + j] < 0 ? -1.0 : 1.0;
+      const float coef = (float)(-kappa * sg * d->edhx_leng[eo] / d->hx_areag[co]);
+      dc[(size_t)j * nv + c] = coef;
+      de[(size_t)((d->ed_v[2 * eo] - 1 == co) ? 0 : 1) * NE + e] = coef;
+    }
+  }
   for (int eo = 0; eo < d->ne; ++eo) {
     const int e = en[eo];
     int cell[2] = {d->ed_v[2 * eo] - 1, d->ed_v[2 * eo + 1] - 1};
@@ -300,22 +414,27 @@ int imx_learn_create(imx_swm *s, imx_mlp *m, const imx_mesh_desc *d, imx_learn *
     geo[(size_t)5 * NE + e] = (float)(d->hx_areag[cell[1]] / ma);
     wv[e] = 1.f / (float)d->ne;
   }
-  L->nvalid = d->ne;
   imx::Alloc &A = L->A;
   L->st_e = A.put(st_e);
   L->st_s = A.put(st_s);
   L->st_c = A.put(st_c);
   L->geo = A.put(geo);
   L->wv = A.put(wv);
+  L->dc = A.put(dc);
+  L->de = A.put(de);
   const size_t es = s->single ? 4 : 8;
-  L->hn = A.get<char>(es * s->nv);
+  L->hn = A.get<char>(es * nv);
   L->un = A.get<char>(es * NE);
   L->raw = A.get<float>((size_t)NE * NF);
   L->mu = A.put(std::vector<float>(NF, 0.f));
   L->sig = A.put(std::vector<float>(NF, 1.f));
   L->Y = A.get<float>((size_t)2 * NE);
+  L->dhc = A.get<float>(nv);
+  L->th = A.get<float>(nv);
   L->rms = A.get<double>(1);
-  L->fb = A.get<int>(1);
+  L->bound = A.put(std::vector<double>(1, 1e300));
+  L->lh = A.get<double>(1);
+  L->nfb = A.get<int>(1);
   if (A.err != hipSuccess) {
     int rc = imx::lfail(A.err, "imx_learn_create");
     imx_learn_destroy(L);
@@ -329,10 +448,10 @@ int imx_learn_sample(imx_learn *L, float *raw) {
   if (!L) return imx::set_error(IMX_EINVAL, "imx_learn_sample: NULL handle");
   int rc = imx::learn_sample(L);
   if (rc) return rc;
+  LTRY(hipStreamSynchronize(L->s->stream), "learn sample");
   if (raw) {  // caller edge order
     std::vector<float> b((size_t)L->NE * NF);
-    LTRY(hipMemcpyAsync(b.data(), L->raw, b.size() * 4, hipMemcpyDeviceToHost, L->s->stream), "learn raw");
-    LTRY(hipStreamSynchronize(L->s->stream), "learn raw");
+    LTRY(hipMemcpy(b.data(), L->raw, b.size() * 4, hipMemcpyDeviceToHost), "learn raw");
     for (int eo = 0; eo < L->s->ne; ++eo)
       std::memcpy(raw + (size_t)eo * NF, b.data() + (size_t)L->s->lay.edge_new[eo] * NF, NF * 4);
   }
@@ -342,41 +461,62 @@ int imx_learn_sample(imx_learn *L, float *raw) {
 int imx_learn_set_scaling(imx_learn *L, const float *mu, const float *sig, float tscale_u, float tscale_h) {
   if (!L || !mu || !sig || !(tscale_u > 0) || !(tscale_h > 0))
     return imx::set_error(IMX_EINVAL, "imx_learn_set_scaling: bad argument");
+  for (int j = 0; j < NF; ++j)
+    if (!(sig[j] > 0)) return imx::set_error(IMX_EINVAL, "imx_learn_set_scaling: sig must be positive");
   LTRY(hipMemcpy(L->mu, mu, NF * 4, hipMemcpyHostToDevice), "learn scaling");
   LTRY(hipMemcpy(L->sig, sig, NF * 4, hipMemcpyHostToDevice), "learn scaling");
   L->tsu = tscale_u;
   L->tsh = tscale_h;
+  imx::learn_drop_graph(L);  // the scales are launch arguments of the graph
   return IMX_OK;
 }
 
 // Train on the features of the last imx_learn_sample against the targets
 // du[ne], dh[nv] (caller numbering): `epochs` AdamW steps on all edges.
+// loss[0] = mean_e (du_e - t_e)^2, loss[1] = mean_c ((D F)_c - t_c)^2 in
+// the scaled units, before the last step.
 int imx_learn_train(imx_learn *L, const double *du, const double *dh, int32_t epochs, float lr, float wd,
                     double *loss) {
   if (!L || !du || !dh || epochs < 1) return imx::set_error(IMX_EINVAL, "imx_learn_train: bad argument");
   imx_swm *s = L->s;
-  int rc = imx::learn_normalise(L);
+  imx_mlp *m = L->m;
+  int rc = imx::mlp_reserve(m, L->NE);
   if (rc) return rc;
-  const int NE = L->NE;
-  std::vector<float> T((size_t)2 * NE, 0.f), w(NE, 0.f);
-  std::vector<double> dhi(s->nv);
-  for (int c = 0; c < s->nv; ++c) dhi[s->lay.cell_new[c]] = dh[c];
-  std::vector<int32_t> ev((size_t)2 * s->NE);
-  LTRY(hipMemcpy(ev.data(), s->e_v, ev.size() * 4, hipMemcpyDeviceToHost), "learn train");
+  const int NE = L->NE, nv = s->nv;
+  imx::learn_enqueue_norm(L, s->stream);
+  LTRY(hipGetLastError(), "learn_norm");
+  LTRY(hipStreamSynchronize(s->stream), "learn train");
+  std::vector<float> T((size_t)2 * NE, 0.f), w(NE, 0.f), th(nv);
+  for (int c = 0; c < nv; ++c) th[s->lay.cell_new[c]] = (float)(dh[c] * L->tsh);
   for (int eo = 0; eo < s->ne; ++eo) {
     const int e = s->lay.edge_new[eo];
     T[2 * e] = (float)(du[eo] * L->tsu);
-    T[2 * e + 1] = (float)(0.5 * (dhi[ev[e]] + dhi[ev[(size_t)s->NE + e]]) * L->tsh);
     w[e] = 1.f / (float)s->ne;
   }
-  LTRY(hipMemcpyAsync(L->m->T, T.data(), T.size() * 4, hipMemcpyHostToDevice, s->stream), "learn train");
-  LTRY(hipMemcpyAsync(L->m->w, w.data(), w.size() * 4, hipMemcpyHostToDevice, s->stream), "learn train");
-  LTRY(hipStreamSynchronize(s->stream), "learn train");
+  hipStream_t st = m->stream;
+  LTRY(hipMemcpyAsync(m->T, T.data(), T.size() * 4, hipMemcpyHostToDevice, st), "learn train");
+  LTRY(hipMemcpyAsync(m->w, w.data(), w.size() * 4, hipMemcpyHostToDevice, st), "learn train");
+  LTRY(hipMemcpyAsync(L->th, th.data(), th.size() * 4, hipMemcpyHostToDevice, st), "learn train");
+  const int g = (NE + LTPB - 1) / LTPB, gc = (nv + LTPB - 1) / LTPB;
+  double lu = 0.0;
   for (int k = 0; k < epochs; ++k) {
-    rc = imx::mlp_train_dev(L->m, NE, lr, wd, 1, (k == epochs - 1) ? loss : nullptr);
+    const bool last = (k == epochs - 1);
+    rc = imx::mlp_forward_dev(m, NE, m->X, L->Y, st);
+    if (rc) return rc;
+    learn_div<<<gc, LTPB, 0, st>>>(nv, s->c_nnb, s->c_ed, L->dc, L->Y, L->th, L->dhc);
+    if (last) learn_norm2<<<1, 1024, 0, st>>>(0, nullptr, 1, nullptr, nv, L->dhc, 1.0 / nv, 0, L->lh);
+    learn_divT<<<g, LTPB, 0, st>>>(NE, s->e_v, L->de, L->dhc, 2.f / (float)nv, m->T);
+    LTRY(hipGetLastError(), "learn train");
+    rc = imx::mlp_train_dev(m, NE, nullptr, lr, wd, 1, last ? &lu : nullptr, 2);
     if (rc) return rc;
   }
-  LTRY(hipStreamSynchronize(L->m->stream), "learn train");
+  double lh = 0.0;
+  LTRY(hipMemcpyAsync(&lh, L->lh, 8, hipMemcpyDeviceToHost, st), "learn train");
+  LTRY(hipStreamSynchronize(st), "learn train");
+  if (loss) {
+    loss[0] = lu;
+    loss[1] = lh;
+  }
   return IMX_OK;
 }
 
@@ -384,21 +524,107 @@ int imx_learn_train(imx_learn *L, const double *du, const double *dh, int32_t ep
 // exceeds `bound` (scaled units); *nfallback counts the plain-only steps.
 int imx_learn_step(imx_learn *L, int32_t nsteps, double bound, int32_t *nfallback) {
   if (!L || nsteps < 0) return imx::set_error(IMX_EINVAL, "imx_learn_step: bad argument");
+  imx_swm *s = L->s;
+  int rc = imx::learn_ensure_graph(L);
+  if (rc) return rc;
+  LTRY(hipMemcpyAsync(L->bound, &bound, 8, hipMemcpyHostToDevice, s->stream), "learn step");
+  LTRY(hipMemsetAsync(L->nfb, 0, 4, s->stream), "learn step");
+  for (int k = 0; k < nsteps; ++k) LTRY(hipGraphLaunch(L->gexec, s->stream), "learn step");
   int nfb = 0;
+  LTRY(hipMemcpyAsync(&nfb, L->nfb, 4, hipMemcpyDeviceToHost, s->stream), "learn step");
+  rc = imx::swm_check_flag(s, "imx_learn_step");
+  if (nfallback) *nfallback = nfb;
+  return rc;
+}
+
+// Device sample store (capacity `cap` samples; existing samples dropped).
+int imx_learn_reserve(imx_learn *L, int32_t cap) {
+  if (!L || cap < 1) return imx::set_error(IMX_EINVAL, "imx_learn_reserve: bad argument");
+  imx::learn_free_store(L);
+  const int64_t npad = ((int64_t)L->NE + 63) / 64 * 64;
+  L->xstride = npad * NF;
+  LTRY(hipMalloc(&L->Xb, (size_t)cap * L->xstride * 2), "imx_learn_reserve");
+  LTRY(hipMalloc(&L->tub, (size_t)cap * L->NE * 4), "imx_learn_reserve");
+  LTRY(hipMalloc(&L->thb, (size_t)cap * L->nv * 4), "imx_learn_reserve");
+  L->cap = cap;
+  return IMX_OK;
+}
+
+// Store the last sample's normalised features with its targets du[ne],
+// dh[nv] (caller numbering); the oldest sample is replaced when full.
+int imx_learn_store(imx_learn *L, const double *du, const double *dh, int32_t *slot) {
+  if (!L || !du || !dh) return imx::set_error(IMX_EINVAL, "imx_learn_store: bad argument");
+  if (L->cap < 1) return imx::set_error(IMX_ESTATE, "imx_learn_store: no store (imx_learn_reserve)");
+  imx_swm *s = L->s;
+  const int k = L->count % L->cap;
+  imx::learn_enqueue_norm(L, s->stream, L->Xb + (size_t)k * L->xstride);
+  LTRY(hipGetLastError(), "learn_norm");
+  std::vector<float> tu(L->NE, 0.f), th(s->nv);
+  for (int c = 0; c < s->nv; ++c) th[s->lay.cell_new[c]] = (float)(dh[c] * L->tsh);
+  for (int eo = 0; eo < s->ne; ++eo) tu[s->lay.edge_new[eo]] = (float)(du[eo] * L->tsu);
+  LTRY(hipMemcpyAsync(L->tub + (size_t)k * L->NE, tu.data(), tu.size() * 4, hipMemcpyHostToDevice,
+                      s->stream), "imx_learn_store");
+  LTRY(hipMemcpyAsync(L->thb + (size_t)k * s->nv, th.data(), th.size() * 4, hipMemcpyHostToDevice,
+                      s->stream), "imx_learn_store");
+  LTRY(hipStreamSynchronize(s->stream), "imx_learn_store");
+  ++L->count;
+  if (slot) *slot = k;
+  return IMX_OK;
+}
+
+// nsteps AdamW steps, each on one stored sample drawn uniformly (SplitMix64
+// from `seed`); the learning rate goes linearly from lr0 to lr1. loss[2]:
+// mean scaled (u, h) losses over the last min(nsteps, 16) steps.
+int imx_learn_fit(imx_learn *L, int32_t nsteps, float lr0, float lr1, float wd, uint64_t seed,
+                  double loss[2]) {
+  if (!L || nsteps < 1) return imx::set_error(IMX_EINVAL, "imx_learn_fit: bad argument");
+  const int n = L->count < L->cap ? L->count : L->cap;
+  if (n < 1) return imx::set_error(IMX_ESTATE, "imx_learn_fit: the store is empty");
+  imx_swm *s = L->s;
+  imx_mlp *m = L->m;
+  int rc = imx::mlp_reserve(m, L->NE);
+  if (rc) return rc;
+  const int NE = L->NE, nv = s->nv;
+  hipStream_t st = m->stream;
+  LTRY(hipStreamSynchronize(s->stream), "imx_learn_fit");
+  LTRY(hipMemcpyAsync(m->w, L->wv, (size_t)NE * 4, hipMemcpyDeviceToDevice, st), "imx_learn_fit");
+  const int g = (NE + LTPB - 1) / LTPB, gc = (nv + LTPB - 1) / LTPB;
+  const int tail = nsteps < 16 ? nsteps : 16;
+  double su = 0.0, sh = 0.0;
+  uint64_t x = seed;
   for (int k = 0; k < nsteps; ++k) {
-    int rc = imx::learn_sample(L);
+    x += 0x9E3779B97F4A7C15ull;  // SplitMix64
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const int j = (int)(z % (uint64_t)n);
+    const short *X = L->Xb + (size_t)j * L->xstride;
+    const float lr = nsteps > 1 ? lr0 + (lr1 - lr0) * (float)k / (float)(nsteps - 1) : lr0;
+    const bool last = k >= nsteps - tail;
+    rc = imx::mlp_forward_dev(m, NE, X, L->Y, st);
     if (rc) return rc;
-    rc = imx::learn_correct(L, bound);
+    learn_div<<<gc, LTPB, 0, st>>>(nv, s->c_nnb, s->c_ed, L->dc, L->Y, L->thb + (size_t)j * nv, L->dhc);
+    if (last) learn_norm2<<<1, 1024, 0, st>>>(0, nullptr, 1, nullptr, nv, L->dhc, 1.0 / nv, 0, L->lh);
+    learn_targets<<<g, LTPB, 0, st>>>(NE, s->e_v, L->de, L->dhc, 2.f / (float)nv,
+                                      L->tub + (size_t)j * NE, m->T);
+    LTRY(hipGetLastError(), "imx_learn_fit");
+    double lu = 0.0;
+    rc = imx::mlp_train_dev(m, NE, X, lr, wd, 1, last ? &lu : nullptr, 2);
     if (rc) return rc;
-    if (nfallback) {
-      int f = 0;
-      LTRY(hipMemcpyAsync(&f, L->fb, 4, hipMemcpyDeviceToHost, L->s->stream), "learn step");
-      LTRY(hipStreamSynchronize(L->s->stream), "learn step");
-      nfb += f;
+    if (last) {
+      double lh = 0.0;
+      LTRY(hipMemcpyAsync(&lh, L->lh, 8, hipMemcpyDeviceToHost, st), "imx_learn_fit");
+      LTRY(hipStreamSynchronize(st), "imx_learn_fit");
+      su += lu;
+      sh += lh;
     }
   }
-  LTRY(hipStreamSynchronize(L->s->stream), "learn step");
-  if (nfallback) *nfallback = nfb;
+  LTRY(hipStreamSynchronize(st), "imx_learn_fit");
+  if (loss) {
+    loss[0] = su / tail;
+    loss[1] = sh / tail;
+  }
   return IMX_OK;
 }
 
@@ -410,6 +636,8 @@ int imx_learn_last_rms(imx_learn *L, double *rms) {
 
 void imx_learn_destroy(imx_learn *L) {
   if (!L) return;
+  imx::learn_drop_graph(L);
+  imx::learn_free_store(L);
   L->A.release();
   delete L;
 }

@@ -156,9 +156,12 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void mlp_fwd(int64_t n, const short
 // backward through every layer; stores (feature-major, bf16) the layer
 // inputs A0^T = X^T .. A3^T and the pre-activation gradients dZ1^T .. dZ4^T
 // for the weight-gradient kernel, and one loss partial per block.
+// Output o with bit o of gmask set takes T[e][o] as dL/dy_eo itself (a loss
+// defined outside the network, e.g. through a divergence) and adds nothing
+// to the loss partial.
 __global__ __launch_bounds__(64 * MLP_WAVES) void mlp_train(
     int64_t n, int64_t npad, const short *__restrict__ X, const float *__restrict__ T,
-    const float *__restrict__ w, const v4s *__restrict__ Wf, const v4s *__restrict__ Wb,
+    const float *__restrict__ w, int gmask, const v4s *__restrict__ Wf, const v4s *__restrict__ Wb,
     const float *__restrict__ P, short *__restrict__ AT, short *__restrict__ DZT,
     float *__restrict__ loss_part) {
   const int lane = threadIdx.x & 63;
@@ -200,9 +203,9 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void mlp_train(
     if (lane < 16 && live) {
       const float we = w[e];
       const float r0 = y[0] - T[e * 2], r1 = y[1] - T[e * 2 + 1];
-      lsum = we * (r0 * r0 + r1 * r1);
-      dz4[0] = f2bf(2.f * we * r0);
-      dz4[1] = f2bf(2.f * we * r1);
+      lsum = ((gmask & 1) ? 0.f : we * r0 * r0) + ((gmask & 2) ? 0.f : we * r1 * r1);
+      dz4[0] = f2bf((gmask & 1) ? T[e * 2] : 2.f * we * r0);
+      dz4[1] = f2bf((gmask & 2) ? T[e * 2 + 1] : 2.f * we * r1);
     }
     // stores: feature-major [row][npad]; a wave writes its 16 edge columns
     const int64_t col = e0 + (lane % 16);
@@ -454,13 +457,14 @@ int mlp_forward_dev(imx_mlp *m, int64_t n, const short *X, float *Y, hipStream_t
   return IMX_OK;
 }
 
-// One training pass on the device batch already in m->X/T/w (n edges).
-int mlp_train_dev(imx_mlp *m, int64_t n, float lr, float wd, int apply, double *loss) {
+// One training pass on the device batch X (m->X when NULL), m->T, m->w (n edges).
+int mlp_train_dev(imx_mlp *m, int64_t n, const short *X, float lr, float wd, int apply, double *loss,
+                  int gmask) {
   const int64_t npad = ((n + 63) / 64) * 64;
   const int64_t blocks = npad / 64;
   hipStream_t st = m->stream;
   // zero the padding columns of the stored tensors (rows beyond n in the batch)
-  mlp_train<<<(unsigned)blocks, 64 * MLP_WAVES, 0, st>>>(n, npad, m->X, m->T, m->w, (const v4s *)m->Ff,
+  mlp_train<<<(unsigned)blocks, 64 * MLP_WAVES, 0, st>>>(n, npad, X ? X : m->X, m->T, m->w, gmask, (const v4s *)m->Ff,
                                                         (const v4s *)m->Fb, m->P, m->AT, m->DZT, m->lossp);
   MTRY(hipGetLastError(), "mlp_train");
   MTRY(hipMemsetAsync(m->part, 0, (size_t)m->nsplit * P_N * 4, st), "mlp part");
@@ -583,8 +587,9 @@ int imx_mlp_forward(imx_mlp *m, int64_t n, const float *X, float *Y) {
 // Gradient of L = sum_e w_e |y_e - t_e|^2 (no update when apply == 0);
 // grad receives the P_N parameter gradients (may be NULL).
 int imx_mlp_train(imx_mlp *m, int64_t n, const float *X, const float *T, const float *w, float lr,
-                  float wd, int32_t apply, double *loss, float *grad) {
-  if (!m || !X || !T || !w || n <= 0) return imx::set_error(IMX_EINVAL, "imx_mlp_train: bad argument");
+                  float wd, int32_t apply, double *loss, float *grad, int32_t gmask) {
+  if (!m || !X || !T || !w || n <= 0 || gmask < 0 || gmask > 3)
+    return imx::set_error(IMX_EINVAL, "imx_mlp_train: bad argument");
   int rc = imx::mlp_reserve(m, n);
   if (rc) return rc;
   hipStream_t st = m->stream;
@@ -593,7 +598,7 @@ int imx_mlp_train(imx_mlp *m, int64_t n, const float *X, const float *T, const f
       hipMemcpyAsync(m->w, w, n * 4, hipMemcpyHostToDevice, st) != hipSuccess)
     return imx::set_error(IMX_EDEVICE, "imx_mlp_train: upload");
   f32_to_bf16<<<(unsigned)((n * MLP_IN + 255) / 256), 256, 0, st>>>(n * MLP_IN, m->Xf, m->X);
-  rc = imx::mlp_train_dev(m, n, lr, wd, apply, loss);
+  rc = imx::mlp_train_dev(m, n, nullptr, lr, wd, apply, loss, gmask);
   if (rc) return rc;
   if (grad && (hipMemcpyAsync(grad, m->G, P_N * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
                hipStreamSynchronize(st) != hipSuccess))

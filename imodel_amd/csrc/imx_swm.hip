@@ -678,6 +678,8 @@ static void launch_step(imx_swm *s) {
   else launch_fast_step<double>(s, nullptr);
 }
 
+void swm_launch_step(imx_swm *s) { launch_step(s); }
+
 // Diagnostics of the cells [c0, c1) and the edge/triangle slots they own.
 template <typename T>
 static void launch_diag(imx_swm *s, const T *hs, const T *us, double href, int c0, int c1) {
@@ -1181,6 +1183,8 @@ static int tune(imx_swm *s, int nsteps, int32_t *out) {
     }
   return check_flag(s, "imx_swm_tune");
 }
+
+int swm_check_flag(imx_swm *s, const char *what) { return check_flag(s, what); }
 
 }  // namespace imx
 

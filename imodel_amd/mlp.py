@@ -47,8 +47,10 @@ class Mlp:
                    "imx_mlp_forward")
         return Y
 
-    def train_step(self, X, T, w=None, lr=1e-3, wd=0.0, apply=True):
-        """One AdamW step on L = sum w |y - t|^2; returns (loss, grad)."""
+    def train_step(self, X, T, w=None, lr=1e-3, wd=0.0, apply=True, gmask=0):
+        """One AdamW step on L = sum w |y - t|^2; returns (loss, grad).
+
+        Output o with bit o of gmask set takes T[:, o] as dL/dy itself."""
         X = np.ascontiguousarray(X, dtype=np.float32)
         T = np.ascontiguousarray(T, dtype=np.float32)
         w = np.ascontiguousarray(np.ones(X.shape[0]) if w is None else w, dtype=np.float32)
@@ -56,7 +58,7 @@ class Mlp:
         g = np.zeros(self.nparams, dtype=np.float32)
         capi.check(capi.lib().imx_mlp_train(self._h, X.shape[0], capi.f32p(X), capi.f32p(T), capi.f32p(w),
                                             float(lr), float(wd), int(bool(apply)), ctypes.byref(loss),
-                                            capi.f32p(g)), "imx_mlp_train")
+                                            capi.f32p(g), int(gmask)), "imx_mlp_train")
         return loss.value, g
 
     def forward_timed(self, n, reps=20):

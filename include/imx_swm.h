@@ -218,27 +218,46 @@ int  imx_mlp_set_params(imx_mlp *m, const float *p);
 int  imx_mlp_forward(imx_mlp *m, int64_t n, const float *X, float *Y);
 /* L = sum_e w[e] |Y[e] - T[e]|^2: gradient into grad[nparams] (may be NULL),
    loss into *loss; one AdamW step (lr, decoupled weight decay wd on the
-   matrices) when apply != 0 */
+   matrices) when apply != 0. Output o with bit o of gmask set takes T[e][o]
+   as dL/dY[e][o] itself (a loss defined outside the network) and adds
+   nothing to *loss. */
 int  imx_mlp_train(imx_mlp *m, int64_t n, const float *X, const float *T,
                    const float *w, float lr, float wd, int32_t apply,
-                   double *loss, float *grad);
+                   double *loss, float *grad, int32_t gmask);
 int  imx_mlp_forward_timed(imx_mlp *m, int64_t n, int32_t reps, double *ms);
 void imx_mlp_destroy(imx_mlp *m);
 
 /* Learned correction bound to a coarse FP32/FP64 fused handle (d: the mesh
-   the handle was created from). imx_learn_sample saves the state, takes one
-   plain step and builds the 48 raw features per edge (copied to raw[ne][48]
-   in caller edge order when raw != NULL). imx_learn_train fits the MLP to the
-   correction du[ne], dh[nv] of that step (`epochs` AdamW steps).
-   imx_learn_step runs corrected steps: plain step + MLP correction, the
-   plain result kept when the correction's RMS (scaled) exceeds `bound`. */
+   the handle was created from). No counterpart in the reference: the
+   coarse model with a learned per-step correction is a new capability.
+   imx_learn_sample saves the state, takes one plain step and builds the 48
+   raw features per edge (copied to raw[ne][48] in caller edge order when
+   raw != NULL). imx_learn_set_scaling gives the feature mean/std and the
+   target scales (scaled target = tscale * physical). imx_learn_train fits
+   the MLP to the correction du[ne], dh[nv] of that step (`epochs` AdamW
+   steps); the network's second output is an edge mass flux whose
+   divergence is the h correction, so corrections conserve total mass.
+   loss[2] receives the scaled u and h losses. imx_learn_step runs corrected
+   steps (one graph launch each): plain step + MLP correction, the plain
+   result kept when the correction's RMS (scaled units) exceeds `bound`;
+   *nfallback counts those steps. */
 typedef struct imx_learn imx_learn;
 int  imx_learn_create(imx_swm *coarse, imx_mlp *m, const imx_mesh_desc *d, imx_learn **out);
 int  imx_learn_sample(imx_learn *l, float *raw);
 int  imx_learn_set_scaling(imx_learn *l, const float mu[48], const float sig[48],
                            float tscale_u, float tscale_h);
 int  imx_learn_train(imx_learn *l, const double *du, const double *dh, int32_t epochs,
-                     float lr, float wd, double *loss);
+                     float lr, float wd, double loss[2]);
+/* Device sample store for shuffled training: imx_learn_reserve allocates
+   room for `cap` samples (normalised features + scaled targets, about
+   110 bytes per edge); imx_learn_store keeps the last sample with its
+   targets (ring buffer); imx_learn_fit runs nsteps AdamW steps, each on one
+   stored sample drawn uniformly from `seed`, lr linear from lr0 to lr1,
+   loss[2] = mean scaled (u, h) losses of the last min(nsteps,16) steps. */
+int  imx_learn_reserve(imx_learn *l, int32_t cap);
+int  imx_learn_store(imx_learn *l, const double *du, const double *dh, int32_t *slot);
+int  imx_learn_fit(imx_learn *l, int32_t nsteps, float lr0, float lr1, float wd, uint64_t seed,
+                   double loss[2]);
 int  imx_learn_step(imx_learn *l, int32_t nsteps, double bound, int32_t *nfallback);
 int  imx_learn_last_rms(imx_learn *l, double *rms);
 void imx_learn_destroy(imx_learn *l);

@@ -1,0 +1,125 @@
+"""Learned coarse-grid correction on the GPU (imx_learn_*): features and the
+corrected step against the numpy restatement, the RMS fallback, mass
+conservation of the flux-form h correction, and what a network trained
+with this solver's own finer run gains over the plain coarse step."""
+import numpy as np
+import pytest
+
+from imodel_amd.learn import Learn, Trainer, evaluate, restriction
+from imodel_amd.mlp import Mlp
+from imodel_amd.swm import H0_GEO, Mesh, Swm
+from oracle import learn_oracle as lo
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def g3():
+    return Mesh.icos(3)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_features_match_oracle(g3, prec):
+    say("learn features", prec)
+    s = Swm(g3, precision=prec, dt=3600.0, h_ref=H0_GEO)
+    L = Learn(s, Mlp(seed=1), g3)
+    s.set_state(*g3.spectral(4))
+    hn, un = s.get_state()          # the state as stored (FP32-rounded for fp32)
+    raw = L.sample(raw=True)
+    hp, up = s.get_state()
+    ref = lo.features(lo.stencils(g3.arrays), hn, un, hp, up, H0_GEO)
+    scale = np.abs(ref).max(axis=0) + 1e-30
+    tol = 2e-6 if prec == "fp32" else 1e-6
+    assert (np.abs(raw - ref) / scale).max() < tol
+
+
+def test_corrected_step_matches_oracle(g3):
+    """Plain step + MLP correction (BF16 network, flux-form h) vs the oracle
+    chain: features -> normalise -> network (BF16 emulated) -> D -> update."""
+    say("learn corrected step")
+    m = Mlp(seed=9)
+    p = m.params().astype(np.float64)
+    s = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
+    L = Learn(s, m, g3)
+    h0, u0 = g3.spectral(6)
+    s.set_state(h0, u0)
+    hn, un = s.get_state()
+    raw = L.sample(raw=True)
+    hp, up = s.get_state()
+    mu, sig = raw.mean(0), raw.std(0)
+    sig[sig < 1e-6] = 1.0
+    tsu, tsh = 50.0, 0.5
+    L.set_scaling(mu, sig, tsu, tsh)
+    s.set_state(hn, un)
+    nfb = L.step(1, bound=1e30)
+    assert nfb == 0
+    hc, uc = s.get_state()
+    X = (raw.astype(np.float64) - mu) / sig
+    Y, _ = mo.forward(p, X, emulate_bf16=True)
+    D = lo.div_matrix(g3.arrays)
+    ho, uo, _ = lo.correct(hp, up, Y, D, tsu, tsh, 1e30)
+    du, dh = uo - up, ho - hp
+    assert np.linalg.norm((uc - up) - du) < 2e-2 * np.linalg.norm(du)
+    assert np.linalg.norm((hc - hp) - dh) < 2e-2 * np.linalg.norm(dh)
+
+
+def test_fallback_bound_zero_is_plain(g3):
+    say("learn fallback")
+    s = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
+    p = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
+    L = Learn(s, Mlp(seed=2), g3)
+    h0, u0 = g3.spectral(7)
+    s.set_state(h0, u0)
+    p.set_state(h0, u0)
+    assert L.step(5, bound=0.0) == 5
+    p.step(5)
+    hs, us = s.get_state()
+    hp, up = p.get_state()
+    assert np.array_equal(hs, hp) and np.array_equal(us, up)
+    assert L.last_rms() > 0.0
+
+
+def test_correction_conserves_mass(g3):
+    """Large corrections (untrained network, small target scales) change h by
+    metres per step; total mass moves only by FP32 rounding."""
+    say("learn mass")
+    s = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
+    L = Learn(s, Mlp(seed=3), g3)
+    h0, u0 = g3.spectral(8)
+    s.set_state(h0, u0)
+    raw = L.sample(raw=True)
+    L.set_scaling(raw.mean(0), raw.std(0) + 1e-3, 1.0, 0.2)
+    s.set_state(h0, u0)
+    m0 = s.diagnostics()[0]
+    assert L.step(6, bound=1e30) == 0
+    h1, _ = s.get_state()
+    m1 = s.diagnostics()[0]
+    p = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
+    p.set_state(h0, u0)
+    p.step(6)
+    hp, _ = p.get_state()
+    assert np.abs(h1 - hp).max() > 1.0                 # the correction is large
+    assert abs(m1 - m0) / m0 < 1e-9, (m0, m1)
+
+
+def test_learned_beats_plain():
+    """Train on a glevel-6 FP64 run restricted to glevel 4 (1280 samples),
+    then run 48 steps (1 day) from a held-out seed: the corrected glevel-4
+    model tracks the restricted truth better than the plain one."""
+    say("learn train g4 from g6")
+    tr = Trainer(gc=4, gf=6, dt_c=1800.0, log=lambda *a: print(*a, flush=True))
+    tr.calibrate([1000, 1001], nsamples=8, spinup=4)
+    tr.collect(list(range(1, 41)), 32, spinup=4)
+    fit = tr.fit(4000, 1e-3, 5e-5, 1e-4)
+    assert fit[-1].sum() < 0.7 * fit[0].sum() or fit[-1].sum() < 1.0
+    r = evaluate(tr, 9999, 48, spinup=4)
+    say(r)
+    assert r["learned"]["h"] < 0.7 * r["plain"]["h"]
+    assert r["learned"]["u"] < 0.8 * r["plain"]["u"]
+    assert 0 <= r["learned"]["fallbacks"] <= 48
+    assert abs(r["learned"]["mass_drift"]) < 1e-8

@@ -58,3 +58,32 @@ def test_training_reduces_loss():
     w = np.full(len(X), 1.0 / len(X), dtype=np.float32)
     losses = [m.train_step(X, T, w, lr=3e-3)[0] for _ in range(200)]
     assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
+
+
+def test_gradient_input_mask_matches_oracle():
+    """gmask=2: T[:, 1] is dL/dy1 itself. The oracle gets the same gradient
+    from the pseudo-target t1 = y1 - g / (2 w) of its squared loss."""
+    say("mlp gradient input mask")
+    n = 2048 + 5
+    m = Mlp(seed=11)
+    p = m.params().astype(np.float64)
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal((n, 48)).astype(np.float32)
+    T = rng.standard_normal((n, 2)).astype(np.float32)
+    T[:, 1] = (rng.standard_normal(n) / n).astype(np.float32)
+    w = np.full(n, 0.5, dtype=np.float32)
+    loss, g = m.train_step(X, T, w, apply=False, gmask=2)
+    Xd = X.astype(np.float64)
+    Yo, _ = mo.forward(p, Xd, emulate_bf16=True)
+    To = T.astype(np.float64)
+    To[:, 1] = Yo[:, 1] - T[:, 1].astype(np.float64) / (2 * 0.5)
+    Lo, go = mo.loss_and_grad(p, Xd, To, w.astype(np.float64), emulate_bf16=True)
+    Lo0 = float(np.sum(0.5 * (Yo[:, 0] - To[:, 0]) ** 2))
+    assert abs(loss / Lo0 - 1) < 1e-2, (loss, Lo0)
+    assert Lo > Lo0  # the pseudo part is in the oracle's loss only
+    o = 0
+    for name, s in mo.SHAPES:
+        k = int(np.prod(s))
+        gb, gob = g[o:o + k], go[o:o + k]
+        assert np.linalg.norm(gb - gob) <= 3e-2 * np.linalg.norm(gob) + 1e-12, name
+        o += k
