@@ -108,6 +108,9 @@ def main():
                          "levels uniform in 4..7, CFL-0.8 time steps; learned: glevel-6 model with "
                          "the learned correction vs plain glevel 6 and 7 against restricted glevel 8")
     ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--compare-coarser", action="store_true",
+                    help="also run the plain model one glevel coarser (2 dt, steps/2) from its own "
+                         "sampling of the same spectral state and compare on the coarser grid")
     ap.add_argument("--train-seeds", type=int, default=24)
     ap.add_argument("--train-samples", type=int, default=48)
     ap.add_argument("--fit-steps", type=int, default=6000)
@@ -127,11 +130,15 @@ def main():
 
     t = time.perf_counter()
     mesh = Mesh.icos(args.glevel)
-    h0, u0 = mesh.spectral(args.seed)
     log(f"mesh g{args.glevel}: nv={mesh.nv} ne={mesh.ne} nt={mesh.nt} ({time.perf_counter() - t:.1f} s)")
+    t = time.perf_counter()
+    h0, u0 = mesh.spectral(args.seed)
+    log(f"spectral state ({time.perf_counter() - t:.1f} s)")
     nv, ne, nt = mesh.nv, mesh.ne, mesh.nt
 
+    t = time.perf_counter()
     s = Swm(mesh, precision=args.precision, dt=args.dt)
+    log(f"handle created ({time.perf_counter() - t:.1f} s)")
     s.set_state(h0, u0)
     s.step(args.warmup)
     s.set_state(h0, u0)
@@ -179,7 +186,9 @@ def main():
     if not args.no_accuracy and args.precision == "fp32":
         with Swm(mesh, precision="fp64_ref", dt=args.dt) as r:
             r.set_state(h0, u0)
+            t = time.perf_counter()
             r.step(args.steps)
+            log(f"FP64 reference-order run ({time.perf_counter() - t:.1f} s)")
             hR, uR = r.get_state()
             dR = r.diagnostics()
         A = mesh.arrays["hx_areag"]
@@ -196,6 +205,12 @@ def main():
             "simulated_s": args.steps * args.dt,
         }
         log("accuracy", accuracy)
+
+    coarser = None
+    if args.compare_coarser:
+        coarser = compare_coarser(args, mesh, hK, uK)
+        coarser["fine_gpu_ms_same_interval"] = ev_ms
+        log("coarser grid", coarser)
 
     cpu = None
     if not args.no_cpu:
@@ -215,9 +230,37 @@ def main():
                    "glevel": args.glevel, "cells": nv, "edges": ne, "triangles": nt, "dt": args.dt,
                    "precision": args.precision, "seed": args.seed},
         "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels, "accuracy": accuracy,
+        "coarser_grid": coarser,
         "diagnostics_end": {k: float(v) for k, v in diag.items()},
     }
     print(json.dumps(out), flush=True)
+
+
+def compare_coarser(args, mesh, hK, uK):
+    """Plain glevel-1 run over the same simulated time, compared with this
+    run's final state restricted to the coarser grid."""
+    from imodel_amd.learn import restriction
+    from imodel_amd.swm import Mesh, Swm
+
+    t = time.perf_counter()
+    mc = Mesh.icos(args.glevel - 1)
+    hc0, uc0 = mc.spectral(args.seed)
+    Rh, Ru = restriction(mesh, mc)
+    log(f"coarser mesh and restriction ({time.perf_counter() - t:.1f} s)")
+    nsteps = args.steps // 2
+    with Swm(mc, precision=args.precision, dt=2 * args.dt) as c:
+        c.set_state(hc0, uc0)
+        c.step(1)
+        c.set_state(hc0, uc0)
+        ms = c.step_timed(nsteps)
+        hc, uc = c.get_state()
+    ht, ut = Rh @ hK, Ru @ uK
+    A = mc.arrays["hx_areag"]
+    ha = ht - np.sum(A * ht) / np.sum(A)
+    return {"glevel": args.glevel - 1, "dt": 2 * args.dt, "steps": nsteps, "gpu_ms": ms,
+            "fine_gpu_ms_same_interval": None,
+            "hprime_l2_rel_vs_fine": float(np.sqrt(np.sum(A * (hc - ht) ** 2) / np.sum(A * ha ** 2))),
+            "u_l2_rel_vs_fine": float(np.linalg.norm(uc - ut) / np.linalg.norm(ut))}
 
 
 def bench_ensemble(args):

@@ -117,11 +117,13 @@ int imx_ic_spectral(const imx_mesh_desc *d, uint64_t seed, int32_t lmax, double 
   imx::SplitMix64 rng(seed);
   imx::SphExpansion hs(lmax, rng);
   imx::SphExpansion ps(lmax, rng);
-  std::vector<double> P;
   // h' at cell nodes, scaled to area-rms hfrac*h0
+  imx::parallel_ranges(d->nv, [&](int i0, int i1) {
+    std::vector<double> P;
+    for (int i = i0; i < i1; ++i) h[i] = hs.eval(d->v_p + 3 * i, P);
+  });
   double sa = 0.0, s2 = 0.0;
   for (int i = 0; i < d->nv; ++i) {
-    h[i] = hs.eval(d->v_p + 3 * i, P);
     sa += d->hx_areag[i];
     s2 += d->hx_areag[i] * h[i] * h[i];
   }
@@ -130,7 +132,10 @@ int imx_ic_spectral(const imx_mesh_desc *d, uint64_t seed, int32_t lmax, double 
   for (int i = 0; i < d->nv; ++i) h[i] = h0 + hscale * h[i];
   // streamfunction at circumcentres; u = -d psi / dn along ed%nr
   std::vector<double> psi(d->nt);
-  for (int k = 0; k < d->nt; ++k) psi[k] = ps.eval(d->tr_c + 3 * k, P);
+  imx::parallel_ranges(d->nt, [&](int k0, int k1) {
+    std::vector<double> P;
+    for (int k = k0; k < k1; ++k) psi[k] = ps.eval(d->tr_c + 3 * k, P);
+  });
   for (int l = 0; l < d->ne; ++l) {
     const double *p1 = d->v_p + 3 * (d->ed_v[2 * l] - 1);
     const double *p2 = d->v_p + 3 * (d->ed_v[2 * l + 1] - 1);

@@ -1,12 +1,33 @@
 // Internal declarations shared by the host-side translation units of libimx.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/imx_swm.h"
 
 namespace imx {
+
+// fn(i0, i1) over [0, n) split across up to 16 host threads; each index is
+// handled by exactly one thread, so per-element results do not depend on
+// the split (reductions stay serial in the callers).
+template <typename F>
+void parallel_ranges(int n, F fn) {
+  const int hw = (int)std::thread::hardware_concurrency();
+  const int nt = std::max(1, std::min({16, hw > 0 ? hw : 1, n / 4096 + 1}));
+  if (nt == 1) {
+    fn(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) {
+    const int i0 = (int)((int64_t)n * t / nt), i1 = (int)((int64_t)n * (t + 1) / nt);
+    th.emplace_back([=] { fn(i0, i1); });
+  }
+  for (auto &x : th) x.join();
+}
 
 // Thread-local last error; every C entry point returns a code and leaves the
 // message here.

@@ -25,7 +25,7 @@
 #include <cstring>
 #include <map>
 #include <new>
-#include <unordered_map>
+#include <atomic>
 
 #include "imx_internal.h"
 
@@ -129,7 +129,8 @@ static inline void normalize3(double *p) {
 //   calc_trisk_weights  (src/interpack.f90:3978-4098).
 void gridprop(const imx_mesh_desc &d, double *trhx, double *hxtr, double *trskw) {
   const int MX = IMX_MAXNB;
-  for (int k = 0; k < d.nt; ++k) {
+  parallel_ranges(d.nt, [&](int k0, int k1) {
+  for (int k = k0; k < k1; ++k) {
     const double *pc = d.tr_c + 3 * k;
     for (int j = 0; j < 3; ++j) {
       int jm = (j + 2) % 3;  // modint(j-1,3) in 1-based terms
@@ -145,7 +146,9 @@ void gridprop(const imx_mesh_desc &d, double *trhx, double *hxtr, double *trskw)
       trhx[3 * k + j] = area1 + area2;
     }
   }
-  for (int i = 0; i < d.nv; ++i) {
+  });
+  parallel_ranges(d.nv, [&](int i0, int i1) {
+  for (int i = i0; i < i1; ++i) {
     int n = d.v_nnb[i];
     const double *pc = d.v_p + 3 * i;
     double *R = hxtr + MX * i;
@@ -183,6 +186,7 @@ void gridprop(const imx_mesh_desc &d, double *trhx, double *hxtr, double *trskw)
       }
     }
   }
+  });
 }
 
 void desc_of(const HostMesh &m, imx_mesh_desc &d) {
@@ -324,11 +328,6 @@ static std::vector<Tri> icos_faces(const std::vector<double> &p) {
   return f;
 }
 
-static inline uint64_t ekey(int32_t a, int32_t b) {
-  if (a > b) std::swap(a, b);
-  return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
-}
-
 // CCW neighbour rings from CCW triangles, each ring started at the
 // lowest-numbered neighbour.
 static bool rings(int nv, const std::vector<Tri> &tris, std::vector<int32_t> &nnb,
@@ -391,8 +390,14 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
       err = "bad neighbour ring at level " + std::to_string(lev);
       return false;
     }
-    std::unordered_map<uint64_t, int32_t> mid;
-    mid.reserve((size_t)3 * nv);
+    // midpoint id of edge (i, nb(j)) kept at the lower node's ring slot
+    std::vector<int32_t> mid((size_t)nv * MX, -1);
+    auto mid_of = [&](int a, int b) -> int32_t {
+      const int lo = std::min(a, b), hi = std::max(a, b);
+      for (int j = 0; j < nnb[lo]; ++j)
+        if (nb[(size_t)lo * MX + j] == hi) return mid[(size_t)lo * MX + j];
+      return -1;
+    };
     int nn = nv;
     p.resize((size_t)3 * (4 * (size_t)nv - 6));
     for (int i = 0; i < nv; ++i)
@@ -410,14 +415,18 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
         p[3 * nn] = q[0] / r;
         p[3 * nn + 1] = q[1] / r;
         p[3 * nn + 2] = q[2] / r;
-        mid[ekey(i, k)] = nn;
+        mid[(size_t)i * MX + j] = nn;
         ++nn;
       }
     std::vector<Tri> nt;
     nt.reserve(tris.size() * 4);
     for (const Tri &t : tris) {
       int a = t[0], b = t[1], c = t[2];
-      int ab = mid[ekey(a, b)], bc = mid[ekey(b, c)], ca = mid[ekey(c, a)];
+      int ab = mid_of(a, b), bc = mid_of(b, c), ca = mid_of(c, a);
+      if (ab < 0 || bc < 0 || ca < 0) {
+        err = "midpoint lookup failed";
+        return false;
+      }
       nt.push_back({a, ab, ca});
       nt.push_back({ab, b, bc});
       nt.push_back({ca, bc, c});
@@ -458,8 +467,6 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
   m.ed_leng.assign(ne, 0);
   m.ed_clat.assign(ne, 0);
   m.ed_clon.assign(ne, 0);
-  std::unordered_map<uint64_t, int32_t> edge_of;
-  edge_of.reserve((size_t)ne * 2);
   int l = 0;
   for (int i = 0; i < nv; ++i)
     for (int j = 0; j < nnb[i]; ++j) {
@@ -481,7 +488,6 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
         std::memcpy(&m.ed_nr[3 * l], nr, sizeof(nr));
         std::memcpy(&m.ed_tg[3 * l], tg, sizeof(tg));
         m.ed_leng[l] = arclen(pi, pk);
-        edge_of[ekey(i, k)] = l;
         ++l;
       }
     }
@@ -489,37 +495,58 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
     err = "edge count mismatch";
     return false;
   }
+  // slot of node b in the ring of node a (-1 if absent)
+  auto slot_of = [&](int a, int b) -> int {
+    for (int j = 0; j < nnb[a]; ++j)
+      if (nb0[(size_t)a * MX + j] == b) return j;
+    return -1;
+  };
+  // edge id of (a, b), numbered at its lower node
+  auto edge_of = [&](int a, int b) -> int {
+    const int lo = std::min(a, b), j = slot_of(lo, std::max(a, b));
+    return j < 0 ? -1 : m.v_ed[(size_t)lo * MX + j] - 1;
+  };
   for (int i = 0; i < nv; ++i)
     for (int j = 0; j < nnb[i]; ++j) {
       int k = nb0[(size_t)i * MX + j];
-      if (k < i) m.v_ed[(size_t)i * MX + j] = edge_of[ekey(i, k)] + 1;
+      if (k < i) {
+        const int e = edge_of(i, k);
+        if (e < 0) {
+          err = "edge lookup failed";
+          return false;
+        }
+        m.v_ed[(size_t)i * MX + j] = e + 1;
+      }
     }
 
   // Triangles: (i, nb(j), nb(j+1)) is created by its lowest node i.
   m.v_tr.assign((size_t)nv * MX, 0);
   m.tr_v.assign((size_t)3 * nt, 0);
-  std::map<std::array<int32_t, 3>, int32_t> tri_of;
+  // A triangle met again at a higher node i is found in the ring of its
+  // lowest node lo, where it was numbered: rotating the CCW triple (i, b, c)
+  // to start at lo gives the two ring slots (x at j', y at j'+1).
   int kt = 0;
   for (int i = 0; i < nv; ++i) {
     int n = nnb[i];
     for (int j = 0; j < n; ++j) {
       int b = nb0[(size_t)i * MX + j], c = nb0[(size_t)i * MX + (j + 1) % n];
-      std::array<int32_t, 3> key = {i, b, c};
-      std::sort(key.begin(), key.end());
       if (std::min(b, c) > i) {
         m.v_tr[(size_t)i * MX + j] = kt + 1;
         m.tr_v[3 * kt] = i + 1;
         m.tr_v[3 * kt + 1] = b + 1;
         m.tr_v[3 * kt + 2] = c + 1;
-        tri_of[key] = kt;
         ++kt;
       } else {
-        auto it = tri_of.find(key);
-        if (it == tri_of.end()) {
+        const int lo = std::min(b, c), x = (lo == b) ? c : i, y = (lo == b) ? i : b;
+        const int nl = nnb[lo];
+        int found = -1;
+        for (int q = 0; q < nl && found < 0; ++q)
+          if (nb0[(size_t)lo * MX + q] == x && nb0[(size_t)lo * MX + (q + 1) % nl] == y) found = q;
+        if (found < 0 || m.v_tr[(size_t)lo * MX + found] == 0) {
           err = "triangle lookup failed";
           return false;
         }
-        m.v_tr[(size_t)i * MX + j] = it->second + 1;
+        m.v_tr[(size_t)i * MX + j] = m.v_tr[(size_t)lo * MX + found];
       }
     }
   }
@@ -533,7 +560,11 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
   for (int k = 0; k < nt; ++k)
     for (int e = 0; e < 3; ++e) {
       int a = m.tr_v[3 * k + e] - 1, b = m.tr_v[3 * k + (e + 1) % 3] - 1;
-      int ed = edge_of[ekey(a, b)];
+      int ed = edge_of(a, b);
+      if (ed < 0) {
+        err = "triangle edge lookup failed";
+        return false;
+      }
       m.tr_ed[3 * k + e] = ed + 1;
       if (m.ed_sh[2 * ed] == 0) m.ed_sh[2 * ed] = k + 1;
       else m.ed_sh[2 * ed + 1] = k + 1;
@@ -557,7 +588,8 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
   m.tr_clat.assign(nt, 0);
   m.tr_clon.assign(nt, 0);
   m.tr_areag.assign(nt, 0);
-  for (int k = 0; k < nt; ++k) {
+  parallel_ranges(nt, [&](int k0, int k1) {
+  for (int k = k0; k < k1; ++k) {
     const double *p1 = &m.v_p[3 * (m.tr_v[3 * k] - 1)];
     const double *p2 = &m.v_p[3 * (m.tr_v[3 * k + 1] - 1)];
     const double *p3 = &m.v_p[3 * (m.tr_v[3 * k + 2] - 1)];
@@ -570,13 +602,15 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
     cart2sph(c[0], c[1], c[2], &m.tr_clon[k], &m.tr_clat[k]);
     m.tr_areag[k] = sphtriarea(p1, p2, p3);
   }
+  });
 
   // Voronoi edges (src/smeshpack.f90:1016-1049).
   m.edhx_c.assign((size_t)3 * ne, 0);
   m.edhx_nr.assign((size_t)3 * ne, 0);
   m.edhx_tg.assign((size_t)3 * ne, 0);
   m.edhx_leng.assign(ne, 0);
-  for (int e = 0; e < ne; ++e) {
+  parallel_ranges(ne, [&](int e0, int e1) {
+  for (int e = e0; e < e1; ++e) {
     const double *c1 = &m.tr_c[3 * (m.ed_sh[2 * e] - 1)];
     const double *c2 = &m.tr_c[3 * (m.ed_sh[2 * e + 1] - 1)];
     double c[3] = {(c1[0] + c2[0]) / 2.0, (c1[1] + c2[1]) / 2.0, (c1[2] + c2[2]) / 2.0};
@@ -591,6 +625,7 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
     std::memcpy(&m.edhx_tg[3 * e], tg, sizeof(tg));
     m.edhx_leng[e] = arclen(c1, c2);
   }
+  });
 
   // Voronoi cells: sign tables and areas (src/smeshpack.f90:1071-1151, sphhxarea :4765).
   m.hx_tg.assign((size_t)nv * MX, 0);
@@ -598,7 +633,9 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
   m.hx_ttgout.assign((size_t)nv * MX, 0);
   m.hx_tnrccw.assign((size_t)nv * MX, 0);
   m.hx_areag.assign(nv, 0);
-  for (int i = 0; i < nv; ++i) {
+  std::atomic<int> bad{0};
+  parallel_ranges(nv, [&](int i0, int i1) {
+  for (int i = i0; i < i1; ++i) {
     int n = nnb[i];
     double area = 0.0;
     for (int j = 0; j < n; ++j) {
@@ -612,10 +649,7 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
       nr[2] = -nr[2];
       normalize3(nr);
       double dp = dot3(nr, &m.edhx_nr[3 * ed]);
-      if (std::fabs(std::fabs(dp) - 1.0) > kEpsSingle * 100) {
-        err = "hexagon edge normal not colinear";
-        return false;
-      }
+      if (std::fabs(std::fabs(dp) - 1.0) > kEpsSingle * 100) bad = 1;
       int hnr = dp > 0 ? 1 : -1, htg = dp > 0 ? -1 : 1;
       m.hx_nr[(size_t)i * MX + j] = hnr;
       m.hx_tg[(size_t)i * MX + j] = htg;
@@ -627,6 +661,11 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
       area = area + sphtriarea(&m.v_p[3 * i], &m.tr_c[3 * j1], &m.tr_c[3 * j2]);
     }
     m.hx_areag[i] = area;
+  }
+  });
+  if (bad) {
+    err = "hexagon edge normal not colinear";
+    return false;
   }
   return true;
 }

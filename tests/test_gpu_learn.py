@@ -95,10 +95,10 @@ def test_correction_conserves_mass(g3):
     raw = L.sample(raw=True)
     L.set_scaling(raw.mean(0), raw.std(0) + 1e-3, 1.0, 0.2)
     s.set_state(h0, u0)
-    m0 = s.diagnostics()[0]
+    m0 = s.diagnostics()["mass"]
     assert L.step(6, bound=1e30) == 0
     h1, _ = s.get_state()
-    m1 = s.diagnostics()[0]
+    m1 = s.diagnostics()["mass"]
     p = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
     p.set_state(h0, u0)
     p.step(6)
