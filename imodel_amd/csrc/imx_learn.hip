@@ -31,21 +31,21 @@
 #include <vector>
 
 #include "imx_device.h"
+#include "imx_mlp_dev.h"
 
 #define NF 48
 #define NSE 11  // stencil edges
 #define NSC 10  // stencil cells
 #define LTPB 256
+#define LPART 1024  // blocks of the correction-norm pass
+#define SREC 32    // words per edge stencil record (128 bytes)
 
 struct imx_learn {
   imx_swm *s = nullptr;
   imx_mlp *m = nullptr;
   int nv = 0, NE = 0;
   imx::Alloc A;
-  int32_t *st_e = nullptr;  // [NSE][NE], -1 pad / dummy
-  float *st_s = nullptr;    // [NSE][NE]
-  int32_t *st_c = nullptr;  // [NSC][NE]
-  float *geo = nullptr;     // [6][NE]
+  int32_t *stn = nullptr;   // [NE][SREC] stencil record per edge slot (see learn_featx)
   float *dc = nullptr;      // [6][nv] coefficient of F_{c_ed[j][c]} in (D F)_c
   float *de = nullptr;      // [2][NE] coefficient of F_e in (D F) at e_v[0/1][e]
   void *hn = nullptr, *un = nullptr;  // state before the step (T)
@@ -56,6 +56,7 @@ struct imx_learn {
   float *wv = nullptr;      // [NE] 1/ne on real edges
   float *dhc = nullptr;     // [nv] (D F)_c, or its residual in training
   float *th = nullptr;      // [nv] scaled dh target
+  double *part = nullptr;   // [LPART] partial sums of the correction norm
   double *rms = nullptr;    // [1] correction RMS of the last step
   double *bound = nullptr;  // [1]
   double *lh = nullptr;     // [1] h loss of the last training epoch
@@ -71,10 +72,6 @@ struct imx_learn {
   float *thb = nullptr;     // [cap][nv] scaled dh target
 };
 
-__device__ __forceinline__ short lf2bf(float x) {
-  __hip_bfloat16 b = __float2bfloat16(x);
-  return *reinterpret_cast<short *>(&b);
-}
 
 template <typename T>
 __global__ __launch_bounds__(LTPB) void learn_save(int nv, int NE, const T *__restrict__ h,
@@ -85,67 +82,81 @@ __global__ __launch_bounds__(LTPB) void learn_save(int nv, int NE, const T *__re
   if (i < nv) hn[i] = h[i];
 }
 
+// Features of (saved state un/hn, plain step up/hp), one thread per edge
+// slot: the 128-byte stencil record is read with vector loads and all 42
+// state gathers are issued before any is used, so a thread keeps its
+// loads in flight together (one feature per thread would chain a record
+// read and a gather for every feature). Record words: 0-10 ring edges
+// (id << 1 | sign bit, -1 pad; word 0 is the edge itself, -1 for a dummy
+// slot), 11-20 stencil cells (-1 pad), 21-26 geometry features (FP32
+// bits). Dummy slots get all-zero raw features.
 template <typename T>
-__global__ __launch_bounds__(LTPB) void learn_feat(int NE, const int32_t *__restrict__ st_e,
-                                                   const float *__restrict__ st_s,
-                                                   const int32_t *__restrict__ st_c,
-                                                   const float *__restrict__ geo,
-                                                   const T *__restrict__ un, const T *__restrict__ hn,
-                                                   const T *__restrict__ up, const T *__restrict__ hp,
-                                                   float *__restrict__ raw) {
+__global__ __launch_bounds__(LTPB) void learn_featx(int NE, const int32_t *__restrict__ stn,
+                                                    const T *__restrict__ un, const T *__restrict__ hn,
+                                                    const T *__restrict__ up, const T *__restrict__ hp,
+                                                    const float *__restrict__ mu,
+                                                    const float *__restrict__ sig,
+                                                    float *__restrict__ raw, short *__restrict__ X) {
   const int e = blockIdx.x * LTPB + threadIdx.x;
   if (e >= NE) return;
-  float *f = raw + (int64_t)e * NF;
-  if (st_e[e] < 0) {
-    for (int j = 0; j < NF; ++j) f[j] = 0.f;
-    return;
+  int32_t r[28];
+  const int4 *rv = reinterpret_cast<const int4 *>(stn + (int64_t)e * SREC);
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const int4 v = rv[q];
+    r[4 * q] = v.x;
+    r[4 * q + 1] = v.y;
+    r[4 * q + 2] = v.z;
+    r[4 * q + 3] = v.w;
   }
+  const bool real = r[0] >= 0;
+  float f[NF];
+  T a[NSE], b[NSE], c0[NSC], c1[NSC];
 #pragma unroll
   for (int k = 0; k < NSE; ++k) {
-    const int ek = st_e[k * NE + e];
-    float a = 0.f, d = 0.f;
-    if (ek >= 0) {
-      const float s = st_s[k * NE + e];
-      a = s * (float)un[ek];
-      d = s * (float)(up[ek] - un[ek]);
-    }
-    f[k] = a;
-    f[NSE + k] = d;
+    const int w = r[k];
+    const int ek = (real && w >= 0) ? (w >> 1) : 0;
+    a[k] = un[ek];
+    b[k] = up[ek];
   }
 #pragma unroll
   for (int k = 0; k < NSC; ++k) {
-    const int c = st_c[k * NE + e];
-    float a = 0.f, d = 0.f;
-    if (c >= 0) {
-      a = (float)hn[c];
-      d = (float)(hp[c] - hn[c]);
-    }
-    f[2 * NSE + k] = a;
-    f[2 * NSE + NSC + k] = d;
+    const int c = (real && r[NSE + k] >= 0) ? r[NSE + k] : 0;
+    c0[k] = hn[c];
+    c1[k] = hp[c];
   }
 #pragma unroll
-  for (int k = 0; k < 6; ++k) f[2 * NSE + 2 * NSC + k] = geo[k * NE + e];
-}
-
-// T[e] = (scaled du target, scale * (D^T r)_e) for the store's training step
-__global__ __launch_bounds__(LTPB) void learn_targets(int NE, const int32_t *__restrict__ e_v,
-                                                      const float *__restrict__ de,
-                                                      const float *__restrict__ r, float scale,
-                                                      const float *__restrict__ tu, float *__restrict__ T) {
-  const int e = blockIdx.x * LTPB + threadIdx.x;
-  if (e >= NE) return;
-  const int c1 = e_v[e], c2 = e_v[NE + e];
-  T[2 * e] = tu[e];
-  T[2 * e + 1] = (c1 < 0) ? 0.f : scale * (de[e] * r[c1] + de[NE + e] * r[c2]);
-}
-
-__global__ __launch_bounds__(LTPB) void learn_norm(int64_t n, const float *__restrict__ raw,
-                                                   const float *__restrict__ mu,
-                                                   const float *__restrict__ sig, short *__restrict__ X) {
-  const int64_t i = (int64_t)blockIdx.x * LTPB + threadIdx.x;
-  if (i >= n * NF) return;
-  const int j = (int)(i % NF);
-  X[i] = lf2bf((raw[i] - mu[j]) / sig[j]);
+  for (int k = 0; k < NSE; ++k) {
+    const int w = r[k];
+    const bool ok = real && w >= 0;
+    const float sg = (w & 1) ? -1.f : 1.f;
+    f[k] = ok ? sg * (float)a[k] : 0.f;
+    f[NSE + k] = ok ? sg * (float)(b[k] - a[k]) : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < NSC; ++k) {
+    const bool ok = real && r[NSE + k] >= 0;
+    f[2 * NSE + k] = ok ? (float)c0[k] : 0.f;
+    f[2 * NSE + NSC + k] = ok ? (float)(c1[k] - c0[k]) : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) f[2 * NSE + 2 * NSC + k] = real ? __int_as_float(r[NSE + NSC + k]) : 0.f;
+  const int64_t o = (int64_t)e * NF;
+  if (raw) {
+    float4 *rw = reinterpret_cast<float4 *>(raw + o);
+#pragma unroll
+    for (int q = 0; q < NF / 4; ++q) rw[q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+  }
+  if (X) {
+    v4s *xw = reinterpret_cast<v4s *>(X + o);
+#pragma unroll
+    for (int q = 0; q < NF / 4; ++q) {
+      v4s x;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[i] = f2bf((f[4 * q + i] - mu[4 * q + i]) / sig[4 * q + i]);
+      xw[q] = x;
+    }
+  }
 }
 
 // out_c = (D F)_c - t_c (t may be NULL), F = the flux output Y[.][1]
@@ -171,6 +182,18 @@ __global__ __launch_bounds__(LTPB) void learn_divT(int NE, const int32_t *__rest
   T[2 * e + 1] = (c1 < 0) ? 0.f : scale * (de[e] * r[c1] + de[NE + e] * r[c2]);
 }
 
+// T[e] = (scaled du target, scale * (D^T r)_e) for a stored sample's step
+__global__ __launch_bounds__(LTPB) void learn_targets(int NE, const int32_t *__restrict__ e_v,
+                                                      const float *__restrict__ de,
+                                                      const float *__restrict__ r, float scale,
+                                                      const float *__restrict__ tu, float *__restrict__ T) {
+  const int e = blockIdx.x * LTPB + threadIdx.x;
+  if (e >= NE) return;
+  const int c1 = e_v[e], c2 = e_v[NE + e];
+  T[2 * e] = tu[e];
+  T[2 * e + 1] = (c1 < 0) ? 0.f : scale * (de[e] * r[c1] + de[NE + e] * r[c2]);
+}
+
 // out = sqrt(sum_e w_e a_e^2 + sb sum_c b_c^2) (one block; a strided by sa)
 __global__ __launch_bounds__(1024) void learn_norm2(int na, const float *__restrict__ a, int sa,
                                                     const float *__restrict__ w, int nb,
@@ -193,27 +216,65 @@ __global__ __launch_bounds__(1024) void learn_norm2(int na, const float *__restr
   if (threadIdx.x == 0) out[0] = take_sqrt ? sqrt(red[0]) : red[0];
 }
 
-template <typename T>
-__global__ __launch_bounds__(LTPB) void learn_apply_u(int NE, const int32_t *__restrict__ st_e,
-                                                      const float *__restrict__ Y, float inv_tsu,
-                                                      const double *__restrict__ rms,
-                                                      const double *__restrict__ bound, T *u,
-                                                      int *__restrict__ nfb) {
-  const int e = blockIdx.x * LTPB + threadIdx.x;
-  if (e >= NE) return;
-  const bool ok = rms[0] <= bound[0];
-  if (e == 0 && !ok) nfb[0] = nfb[0] + 1;  // one thread of the grid counts
-  if (!ok || st_e[e] < 0) return;
-  u[e] = (T)((float)u[e] + Y[2 * e] * inv_tsu);
+// (D F)_c of the flux output into dhc, and per-block partial sums of the
+// correction's squared norm sum_e w_e Y0^2 + (1/nv) sum_c (D F)_c^2
+// (grid-stride over max(nv, NE), LPART blocks)
+__global__ __launch_bounds__(LTPB) void learn_div_part(int nv, int NE, const int32_t *__restrict__ c_nnb,
+                                                       const int32_t *__restrict__ c_ed,
+                                                       const float *__restrict__ dc,
+                                                       const float *__restrict__ Y,
+                                                       const float *__restrict__ w,
+                                                       float *__restrict__ dhc, double *__restrict__ part) {
+  double s = 0.0, t = 0.0;
+  const int nmax = nv > NE ? nv : NE;
+  for (int i = blockIdx.x * LTPB + threadIdx.x; i < nmax; i += gridDim.x * LTPB) {
+    if (i < nv) {
+      const int n = c_nnb[i];
+      float v = 0.f;
+      for (int j = 0; j < n; ++j) v += dc[j * nv + i] * Y[2 * c_ed[j * nv + i] + 1];
+      dhc[i] = v;
+      t += (double)v * (double)v;
+    }
+    if (i < NE) s += (double)w[i] * (double)Y[2 * i] * (double)Y[2 * i];
+  }
+  __shared__ double red[LTPB];
+  red[threadIdx.x] = s + t / nv;
+  __syncthreads();
+  for (int o = LTPB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// every block sums the LPART partials (fixed order: the same RMS in every
+// block), then applies u += Y0/tsu and h += (D F)/tsh unless RMS > bound
 template <typename T>
-__global__ __launch_bounds__(LTPB) void learn_apply_h(int nv, const float *__restrict__ dhc, float inv_tsh,
-                                                      const double *__restrict__ rms,
-                                                      const double *__restrict__ bound, T *h) {
-  const int c = blockIdx.x * LTPB + threadIdx.x;
-  if (c >= nv || !(rms[0] <= bound[0])) return;
-  h[c] = (T)((float)h[c] + dhc[c] * inv_tsh);
+__global__ __launch_bounds__(LTPB) void learn_apply(int nv, int NE, const float *__restrict__ wv,
+                                                    const float *__restrict__ Y,
+                                                    const float *__restrict__ dhc, float inv_tsu,
+                                                    float inv_tsh, const double *__restrict__ part,
+                                                    const double *__restrict__ bound, T *u, T *h,
+                                                    double *__restrict__ rms, int *__restrict__ nfb) {
+  __shared__ double red[LTPB];
+  double p = 0.0;
+  for (int k = threadIdx.x; k < LPART; k += LTPB) p += part[k];
+  red[threadIdx.x] = p;
+  __syncthreads();
+  for (int o = LTPB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = sqrt(red[0]);
+  const bool ok = r <= bound[0];
+  const int i = blockIdx.x * LTPB + threadIdx.x;
+  if (i == 0) {
+    rms[0] = r;
+    if (!ok) nfb[0] = nfb[0] + 1;  // one thread of the grid counts
+  }
+  if (!ok) return;
+  if (i < NE && wv[i] > 0.f) u[i] = (T)((float)u[i] + Y[2 * i] * inv_tsu);
+  if (i < nv) h[i] = (T)((float)h[i] + dhc[i] * inv_tsh);
 }
 
 namespace imx {
@@ -234,23 +295,28 @@ static void learn_drop_graph(imx_learn *L) {
   L->graph = nullptr;
 }
 
-// enqueue: save x_n, one plain step, raw features of (x_n, x_plain)
+// enqueue: save x_n, one plain step
 template <typename T>
 static void learn_enqueue_sample(imx_learn *L) {
   imx_swm *s = L->s;
-  hipStream_t st = s->stream;
   const int g = (L->NE + LTPB - 1) / LTPB;
-  learn_save<T><<<g, LTPB, 0, st>>>(s->nv, L->NE, (const T *)s->h0, (const T *)s->u0, (T *)L->hn,
-                                    (T *)L->un);
+  learn_save<T><<<g, LTPB, 0, s->stream>>>(s->nv, L->NE, (const T *)s->h0, (const T *)s->u0, (T *)L->hn,
+                                           (T *)L->un);
   swm_launch_step(s);
-  learn_feat<T><<<g, LTPB, 0, st>>>(L->NE, L->st_e, L->st_s, L->st_c, L->geo, (const T *)L->un,
-                                    (const T *)L->hn, (const T *)s->u0, (const T *)s->h0, L->raw);
 }
 
-static void learn_enqueue_norm(imx_learn *L, hipStream_t st, short *X = nullptr) {
-  const int64_t n = (int64_t)L->NE * NF;
-  learn_norm<<<(unsigned)((n + LTPB - 1) / LTPB), LTPB, 0, st>>>(L->NE, L->raw, L->mu, L->sig,
-                                                                 X ? X : L->m->X);
+// enqueue: features of (saved state, current state) into raw and/or X
+template <typename T>
+static void learn_enqueue_featx_t(imx_learn *L, hipStream_t st, float *raw, short *X) {
+  imx_swm *s = L->s;
+  learn_featx<T><<<(unsigned)((L->NE + LTPB - 1) / LTPB), LTPB, 0, st>>>(
+      L->NE, L->stn, (const T *)L->un, (const T *)L->hn, (const T *)s->u0, (const T *)s->h0, L->mu, L->sig,
+      raw, X);
+}
+
+static void learn_enqueue_featx(imx_learn *L, hipStream_t st, float *raw, short *X) {
+  if (L->s->single) learn_enqueue_featx_t<float>(L, st, raw, X);
+  else learn_enqueue_featx_t<double>(L, st, raw, X);
 }
 
 static void learn_free_store(imx_learn *L) {
@@ -267,15 +333,16 @@ template <typename T>
 static int learn_enqueue_correct(imx_learn *L) {
   imx_swm *s = L->s;
   hipStream_t st = s->stream;
-  learn_enqueue_norm(L, st);
+  learn_enqueue_featx_t<T>(L, st, nullptr, L->m->X);
   int rc = mlp_forward_dev(L->m, L->NE, L->m->X, L->Y, st);
   if (rc) return rc;
-  const int g = (L->NE + LTPB - 1) / LTPB, gc = (s->nv + LTPB - 1) / LTPB;
-  learn_div<<<gc, LTPB, 0, st>>>(s->nv, s->c_nnb, s->c_ed, L->dc, L->Y, nullptr, L->dhc);
-  learn_norm2<<<1, 1024, 0, st>>>(L->NE, L->Y, 2, L->wv, s->nv, L->dhc, 1.0 / s->nv, 1, L->rms);
-  learn_apply_u<T><<<g, LTPB, 0, st>>>(L->NE, L->st_e, L->Y, 1.f / L->tsu, L->rms, L->bound, (T *)s->u0,
-                                       L->nfb);
-  learn_apply_h<T><<<gc, LTPB, 0, st>>>(s->nv, L->dhc, 1.f / L->tsh, L->rms, L->bound, (T *)s->h0);
+  const int nmax = s->nv > L->NE ? s->nv : L->NE;
+  learn_div_part<<<LPART, LTPB, 0, st>>>(s->nv, L->NE, s->c_nnb, s->c_ed, L->dc, L->Y, L->wv, L->dhc,
+                                         L->part);
+  learn_apply<T><<<(nmax + LTPB - 1) / LTPB, LTPB, 0, st>>>(s->nv, L->NE, L->wv, L->Y, L->dhc,
+                                                           1.f / L->tsu, 1.f / L->tsh, L->part, L->bound,
+                                                           (T *)s->u0, (T *)s->h0, L->rms, L->nfb);
+  LTRY(hipGetLastError(), "learn_apply");
   return IMX_OK;
 }
 
@@ -331,8 +398,10 @@ int imx_learn_create(imx_swm *s, imx_mlp *m, const imx_mesh_desc *d, imx_learn *
   L->NE = s->NE;
   const int NE = s->NE, nv = s->nv, MX = IMX_MAXNB;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new;
-  std::vector<int32_t> st_e((size_t)NSE * NE, -1), st_c((size_t)NSC * NE, -1);
-  std::vector<float> st_s((size_t)NSE * NE, 0.f), geo((size_t)6 * NE, 0.f), wv(NE, 0.f);
+  std::vector<int32_t> stn((size_t)SREC * NE, 0);
+  for (int e = 0; e < NE; ++e)
+    for (int k = 0; k < NSE + NSC; ++k) stn[(size_t)e * SREC + k] = -1;
+  std::vector<float> wv(NE, 0.f);
   std::vector<float> dc((size_t)6 * nv, 0.f), de((size_t)2 * NE, 0.f);
   double ml = 0, md = 0, ma = 0;
   for (int eo = 0; eo < d->ne; ++eo) {
@@ -362,17 +431,16 @@ int imx_learn_create(imx_swm *s, imx_mlp *m, const imx_mesh_desc *d, imx_learn *
     for (int q = 0; q < 2; ++q)
       for (int j = 0; j < d->v_nnb[cell[q]]; ++j)
         if (d->v_ed[MX * cell[q] + j] == eo + 1) slot[q] = j;
-    st_e[e] = e;
-    st_s[e] = 1.f;
+    int32_t *rec = stn.data() + (size_t)e * SREC;
+    rec[0] = e << 1;
     int k = 1;
     for (int q = 0; q < 2; ++q) {
       const int c = cell[q], n = d->v_nnb[c], i = slot[q];
       for (int t = 1; t < 6; ++t, ++k) {
         if (t >= n) continue;  // pentagon: one slot stays padded
         const int j = (i + t) % n;
-        st_e[(size_t)k * NE + e] = en[d->v_ed[MX * c + j] - 1];
-        st_s[(size_t)k * NE + e] =
-            (float)(d->hx_ttgout[MX * c + j] * d->hx_tnrccw[MX * c + i] < 0 ? -1 : 1);
+        const int neg = d->hx_ttgout[MX * c + j] * d->hx_tnrccw[MX * c + i] < 0 ? 1 : 0;
+        rec[k] = (en[d->v_ed[MX * c + j] - 1] << 1) | neg;
       }
     }
     // cells: c1, c2, left apex, right apex, other ring cells of c1 and c2
@@ -404,21 +472,16 @@ int imx_learn_create(imx_swm *s, imx_mlp *m, const imx_mesh_desc *d, imx_learn *
         cl[kc] = d->v_nb[MX * c + (i + t) % n] - 1;
       }
     }
-    for (int q = 0; q < NSC; ++q) st_c[(size_t)q * NE + e] = cl[q] >= 0 ? cn[cl[q]] : -1;
+    for (int q = 0; q < NSC; ++q) rec[NSE + q] = cl[q] >= 0 ? cn[cl[q]] : -1;
     const double z = d->ed_c[3 * eo + 2];
-    geo[e] = (float)z;
-    geo[(size_t)NE + e] = (float)std::sqrt(std::max(0.0, 1 - z * z));
-    geo[(size_t)2 * NE + e] = (float)(d->ed_leng[eo] / ml);
-    geo[(size_t)3 * NE + e] = (float)(d->edhx_leng[eo] / md);
-    geo[(size_t)4 * NE + e] = (float)(d->hx_areag[cell[0]] / ma);
-    geo[(size_t)5 * NE + e] = (float)(d->hx_areag[cell[1]] / ma);
+    const float geo[6] = {(float)z, (float)std::sqrt(std::max(0.0, 1 - z * z)),
+                          (float)(d->ed_leng[eo] / ml), (float)(d->edhx_leng[eo] / md),
+                          (float)(d->hx_areag[cell[0]] / ma), (float)(d->hx_areag[cell[1]] / ma)};
+    std::memcpy(rec + NSE + NSC, geo, sizeof(geo));
     wv[e] = 1.f / (float)d->ne;
   }
   imx::Alloc &A = L->A;
-  L->st_e = A.put(st_e);
-  L->st_s = A.put(st_s);
-  L->st_c = A.put(st_c);
-  L->geo = A.put(geo);
+  L->stn = A.put(stn);
   L->wv = A.put(wv);
   L->dc = A.put(dc);
   L->de = A.put(de);
@@ -431,6 +494,7 @@ int imx_learn_create(imx_swm *s, imx_mlp *m, const imx_mesh_desc *d, imx_learn *
   L->Y = A.get<float>((size_t)2 * NE);
   L->dhc = A.get<float>(nv);
   L->th = A.get<float>(nv);
+  L->part = A.get<double>(LPART);
   L->rms = A.get<double>(1);
   L->bound = A.put(std::vector<double>(1, 1e300));
   L->lh = A.get<double>(1);
@@ -448,6 +512,8 @@ int imx_learn_sample(imx_learn *L, float *raw) {
   if (!L) return imx::set_error(IMX_EINVAL, "imx_learn_sample: NULL handle");
   int rc = imx::learn_sample(L);
   if (rc) return rc;
+  if (raw) imx::learn_enqueue_featx(L, L->s->stream, L->raw, nullptr);
+  LTRY(hipGetLastError(), "learn_featx");
   LTRY(hipStreamSynchronize(L->s->stream), "learn sample");
   if (raw) {  // caller edge order
     std::vector<float> b((size_t)L->NE * NF);
@@ -483,8 +549,8 @@ int imx_learn_train(imx_learn *L, const double *du, const double *dh, int32_t ep
   int rc = imx::mlp_reserve(m, L->NE);
   if (rc) return rc;
   const int NE = L->NE, nv = s->nv;
-  imx::learn_enqueue_norm(L, s->stream);
-  LTRY(hipGetLastError(), "learn_norm");
+  imx::learn_enqueue_featx(L, s->stream, nullptr, m->X);
+  LTRY(hipGetLastError(), "learn_featx");
   LTRY(hipStreamSynchronize(s->stream), "learn train");
   std::vector<float> T((size_t)2 * NE, 0.f), w(NE, 0.f), th(nv);
   for (int c = 0; c < nv; ++c) th[s->lay.cell_new[c]] = (float)(dh[c] * L->tsh);
@@ -557,8 +623,8 @@ int imx_learn_store(imx_learn *L, const double *du, const double *dh, int32_t *s
   if (L->cap < 1) return imx::set_error(IMX_ESTATE, "imx_learn_store: no store (imx_learn_reserve)");
   imx_swm *s = L->s;
   const int k = L->count % L->cap;
-  imx::learn_enqueue_norm(L, s->stream, L->Xb + (size_t)k * L->xstride);
-  LTRY(hipGetLastError(), "learn_norm");
+  imx::learn_enqueue_featx(L, s->stream, nullptr, L->Xb + (size_t)k * L->xstride);
+  LTRY(hipGetLastError(), "learn_featx");
   std::vector<float> tu(L->NE, 0.f), th(s->nv);
   for (int c = 0; c < s->nv; ++c) th[s->lay.cell_new[c]] = (float)(dh[c] * L->tsh);
   for (int eo = 0; eo < s->ne; ++eo) tu[s->lay.edge_new[eo]] = (float)(du[eo] * L->tsu);

@@ -1,5 +1,5 @@
 // libimx learned correction: per-edge MLP 48 -> 128 -> 128 -> 128 -> 2 with
-// GELU (tanh form), BF16 weights and activations on MFMA, FP32 accumulation;
+// hard-sigmoid GELU, BF16 weights and activations on MFMA, FP32 accumulation;
 // hand-written inference, training (forward + backward), weight-gradient and
 // AdamW kernels.
 //
@@ -25,60 +25,7 @@
 
 #include "imx_device.h"
 
-typedef short v4s __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-#define MLP_IN 48
-#define MLP_H 128
-#define MLP_OUT 2
-#define MLP_WAVES 4  // waves per block, 16 edges each
-
-// parameter offsets in the flat FP32 master vector
-#define P_W1 0
-#define P_B1 (P_W1 + MLP_IN * MLP_H)
-#define P_W2 (P_B1 + MLP_H)
-#define P_B2 (P_W2 + MLP_H * MLP_H)
-#define P_W3 (P_B2 + MLP_H)
-#define P_B3 (P_W3 + MLP_H * MLP_H)
-#define P_W4 (P_B3 + MLP_H)
-#define P_B4 (P_W4 + MLP_H * MLP_OUT)
-#define P_N (P_B4 + MLP_OUT)
-
-// forward fragments: layer l has (N/16) x (K/16) tiles of 64 lanes x 4 bf16
-#define FF_1 0
-#define FF_2 (FF_1 + MLP_IN * MLP_H)
-#define FF_3 (FF_2 + MLP_H * MLP_H)
-#define FF_4 (FF_3 + MLP_H * MLP_H)
-#define FF_N (FF_4 + MLP_H * 16)
-// backward fragments (layers 2..4, transposed)
-#define FB_2 0
-#define FB_3 (FB_2 + MLP_H * MLP_H)
-#define FB_4 (FB_3 + MLP_H * MLP_H)
-#define FB_N (FB_4 + MLP_H * 16)
-
-__device__ __forceinline__ short f2bf(float x) {
-  __hip_bfloat16 b = __float2bfloat16(x);
-  return *reinterpret_cast<short *>(&b);
-}
-__device__ __forceinline__ float bf2f(short s) {
-  __hip_bfloat16 b = *reinterpret_cast<__hip_bfloat16 *>(&s);
-  return __bfloat162float(b);
-}
-
-// GELU, tanh form: 0.5 x (1 + tanh(c (x + 0.044715 x^3))), c = sqrt(2/pi)
-__device__ __forceinline__ float gelu(float x) {
-  const float c = 0.7978845608028654f;
-  return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
-}
-__device__ __forceinline__ float gelu_d(float x) {
-  const float c = 0.7978845608028654f;
-  const float t = tanhf(c * (x + 0.044715f * x * x * x));
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x * x);
-}
-
-__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
+#include "imx_mlp_dev.h"
 
 // Y^T tiles (NT x 16 edges) = W^T (N x K) X^T (K x 16), from fragments
 // Wf[(n*KT + kc)*64 + lane] and input fragments xin[kc].
@@ -104,50 +51,6 @@ __device__ __forceinline__ void load_x(const short *__restrict__ X, int64_t e0, 
   for (int kc = 0; kc < 3; ++kc) {
     if (e < n) x[kc] = *reinterpret_cast<const v4s *>(X + e * MLP_IN + kc * 16 + 4 * (lane / 16));
     else x[kc] = v4s{0, 0, 0, 0};
-  }
-}
-
-__global__ __launch_bounds__(64 * MLP_WAVES) void mlp_fwd(int64_t n, const short *__restrict__ X,
-                                                          const v4s *__restrict__ Wf,
-                                                          const float *__restrict__ P,
-                                                          float *__restrict__ Y) {
-  const int lane = threadIdx.x & 63;
-  const int64_t e0 = ((int64_t)blockIdx.x * MLP_WAVES + (threadIdx.x >> 6)) * 16;
-  if (e0 >= n) return;
-  v4s x[3];
-  load_x(X, e0, n, lane, x);
-  v4f z1[8], z2[8], z3[8], z4[1];
-  v4s a1[8], a2[8], a3[8];
-  layer_fwd<3, 8>(Wf + FF_1 / 4, P + P_B1, x, z1, lane);
-#pragma unroll
-  for (int t = 0; t < 8; ++t)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) a1[t][i] = f2bf(gelu(z1[t][i]));
-  layer_fwd<8, 8>(Wf + FF_2 / 4, P + P_B2, a1, z2, lane);
-#pragma unroll
-  for (int t = 0; t < 8; ++t)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) a2[t][i] = f2bf(gelu(z2[t][i]));
-  layer_fwd<8, 8>(Wf + FF_3 / 4, P + P_B3, a2, z3, lane);
-#pragma unroll
-  for (int t = 0; t < 8; ++t)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) a3[t][i] = f2bf(gelu(z3[t][i]));
-  // output layer: feature tile of 16 rows, 2 real (bias read only for them)
-  {
-    v4f acc = {0.f, 0.f, 0.f, 0.f};
-    if (lane < 16) {
-      acc[0] = P[P_B4];
-      acc[1] = P[P_B4 + 1];
-    }
-#pragma unroll
-    for (int kc = 0; kc < 8; ++kc) acc = mfma(Wf[(FF_4 / 4) + kc * 64 + lane], a3[kc], acc);
-    z4[0] = acc;
-  }
-  const int64_t e = e0 + (lane % 16);
-  if (lane < 16 && e < n) {
-    Y[e * 2] = z4[0][0];
-    Y[e * 2 + 1] = z4[0][1];
   }
 }
 
@@ -451,9 +354,7 @@ int mlp_pack_launch(imx_mlp *m) {
 // Forward on device buffers (X bf16 [n][48]) -> Y fp32 [n][2].
 int mlp_forward_dev(imx_mlp *m, int64_t n, const short *X, float *Y, hipStream_t st) {
   if (n <= 0) return IMX_OK;
-  const int64_t blocks = (n + 16 * MLP_WAVES - 1) / (16 * MLP_WAVES);
-  mlp_fwd<<<(unsigned)blocks, 64 * MLP_WAVES, 0, st>>>(n, X, (const v4s *)m->Ff, m->P, Y);
-  MTRY(hipGetLastError(), "mlp_fwd");
+  MTRY(mlp_fwd_launch(m->Ff, m->P, n, RowLoader{X}, Y, st), "mlp_fwd_lds");
   return IMX_OK;
 }
 

@@ -205,7 +205,7 @@ int  imx_ens_info(imx_ens *e, int64_t out[8]);  /* as imx_swm_info, out[7] = n *
 void imx_ens_destroy(imx_ens *e);
 
 /* ---- learned correction (no counterpart in the reference) -----------------
-   Per-edge MLP 48 -> 128 -> 128 -> 128 -> 2, GELU (tanh form), BF16 weights
+   Per-edge MLP 48 -> 128 -> 128 -> 128 -> 2, activation x*clamp(0.5+0.4255x, 0, 1) (hard-sigmoid GELU), BF16 weights
    and activations on MFMA with FP32 accumulation; FP32 master parameters
    updated by AdamW. Parameters are flat FP32: W1[48][128] b1[128]
    W2[128][128] b2 W3[128][128] b3 W4[128][2] b4[2] (W stored [in][out]). */

@@ -1,5 +1,6 @@
 """TEST INFRASTRUCTURE ONLY: FP64 numpy reference of the learned-correction
-MLP (48 -> 128 -> 128 -> 128 -> 2, GELU tanh form), its loss
+MLP (48 -> 128 -> 128 -> 128 -> 2, activation g(x) = x clamp(0.5 + 0.4255 x, 0, 1),
+the hard-sigmoid form of GELU), its loss
 L = sum_e w_e |y_e - t_e|^2 and its parameter gradients, with optional
 emulation of the BF16 storage the GPU kernels use (weights, layer inputs and
 pre-activation gradients rounded to BF16, round-to-nearest-even).
@@ -11,7 +12,6 @@ IN, H, OUT = 48, 128, 2
 SHAPES = [("W1", (IN, H)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (H, H)),
           ("b3", (H,)), ("W4", (H, OUT)), ("b4", (OUT,))]
 NPARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
-C = np.sqrt(2.0 / np.pi)
 
 
 def unpack(p):
@@ -35,13 +35,20 @@ def bf16(x):
     return b.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+K_GELU = 0.4255
+
+
+def _s(x):
+    return np.clip(0.5 + K_GELU * x, 0.0, 1.0)
+
+
 def gelu(x):
-    return 0.5 * x * (1.0 + np.tanh(C * (x + 0.044715 * x ** 3)))
+    return x * _s(x)
 
 
 def gelu_d(x):
-    t = np.tanh(C * (x + 0.044715 * x ** 3))
-    return 0.5 * (1 + t) + 0.5 * x * (1 - t * t) * C * (1 + 3 * 0.044715 * x * x)
+    s = _s(x)
+    return np.where((s > 0.0) & (s < 1.0), s + K_GELU * x, s)
 
 
 def forward(p, X, emulate_bf16=False):
