@@ -103,10 +103,11 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=8)
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the streaming-copy bandwidth probe (0: skip)")
-    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "learned"],
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "learned", "mlp"],
                     help="single: one instance at --glevel; ensemble: --instances instances, "
                          "levels uniform in 4..7, CFL-0.8 time steps; learned: glevel-6 model with "
-                         "the learned correction vs plain glevel 6 and 7 against restricted glevel 8")
+                         "the learned correction vs plain glevel 6 and 7 against restricted glevel 8; "
+                         "mlp: MLP inference and corrected-step timing at --glevel (untrained network)")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--compare-coarser", action="store_true",
                     help="also run the plain model one glevel coarser (2 dt, steps/2) from its own "
@@ -127,6 +128,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "mlp":
+        return bench_mlp(args)
 
     t = time.perf_counter()
     mesh = Mesh.icos(args.glevel)
@@ -320,6 +323,55 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+MLP_FLOP_PER_EDGE = 2.0 * (48 * 128 + 2 * 128 * 128 + 128 * 2)
+
+
+def bench_mlp(args):
+    """Per-launch cost of the learned correction at --glevel: MLP inference
+    over the handle's edge slots (HIP events over --steps launches), and the
+    corrected step against the plain step (wall time per step). The network
+    is the seeded untrained one: timing does not depend on the weights."""
+    from imodel_amd.learn import Learn
+    from imodel_amd.mlp import Mlp
+    from imodel_amd.swm import H0_GEO, Mesh, Swm
+
+    mesh = Mesh.icos(args.glevel)
+    dt = args.dt
+    s = Swm(mesh, precision="fp32", dt=dt, h_ref=H0_GEO)
+    m = Mlp(seed=1)
+    L = Learn(s, m, mesh)
+    h0, u0 = mesh.spectral(args.seed)
+    s.set_state(h0, u0)
+    raw = L.sample(raw=True)
+    L.set_scaling(raw.mean(0), raw.std(0) + 1e-3, 1e4, 1e3)  # small corrections: the run stays smooth
+    ne_slots = 3 * mesh.nv
+    fwd_ms = m.forward_timed(ne_slots, reps=args.steps)
+    s.set_state(h0, u0)
+    L.step(args.warmup, 1e30)
+    t = time.perf_counter()
+    nfb = L.step(args.steps, 1e30)
+    learned_us = 1e6 * (time.perf_counter() - t) / args.steps
+    s.set_state(h0, u0)
+    s.step(args.warmup)
+    t = time.perf_counter()
+    s.step(args.steps)
+    plain_us = 1e6 * (time.perf_counter() - t) / args.steps
+    tflops = MLP_FLOP_PER_EDGE * ne_slots / (fwd_ms * 1e-3) / 1e12
+    out = {
+        "metric": "MLP inference TFLOP/s and corrected-step cost", "value": tflops, "unit": "TFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": learned_us / 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 MFMA, f32 state",
+        "data": "synthetic (seeded spectral state, seeded untrained network)",
+        "config": {"workload": f"learned-correction step, icos glevel {args.glevel}", "glevel": args.glevel,
+                   "edge_slots": ne_slots, "dt": dt},
+        "roofline": {"bound": "mfma", "kernel": "mlp_fwd_lds (48-128-128-128-2, bf16 MFMA)",
+                     "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0,
+                     "traffic": None, "launch_ms": fwd_ms, "flop_per_edge": MLP_FLOP_PER_EDGE},
+        "corrected_step_us": learned_us, "plain_step_us": plain_us, "fallbacks": nfb,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_learned(args):
     """Learned correction (glevel 6, dt 450 s) against the plain glevel-6 and
     glevel-7 (dt 225 s) models, all started from the FP64 glevel-8 run
@@ -348,8 +400,7 @@ def bench_learned(args):
         log(json.dumps(r))
     ne6 = 3 * tr.coarse_mesh.nv
     fwd_ms = tr.mlp.forward_timed(ne6, reps=50)
-    flop = 2.0 * (48 * 128 + 2 * 128 * 128 + 128 * 2) * ne6
-    tflops = flop / (fwd_ms * 1e-3) / 1e12
+    tflops = MLP_FLOP_PER_EDGE * ne6 / (fwd_ms * 1e-3) / 1e12
 
     def mean(k, f):
         return float(np.mean([r[k][f] for r in res]))
