@@ -100,6 +100,9 @@ def lib():
             "imx_mesh_desc_of": (ctypes.c_int, [vp, ctypes.POINTER(MeshDesc)]),
             "imx_mesh_gridprop": (ctypes.c_int, [vp]),
             "imx_mesh_write": (ctypes.c_int, [vp, ctypes.c_char_p]),
+            "imx_mesh_load_ref": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]),
+            "imx_mesh_write_ref": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p]),
+            "imx_make_dir": (ctypes.c_int, [ctypes.c_char_p]),
             "imx_mesh_destroy": (None, [vp]),
             "imx_gridprop": (ctypes.c_int, [ctypes.POINTER(MeshDesc), _F64P, _F64P, _F64P]),
             "imx_ic_tc2": (ctypes.c_int, [vp, ctypes.c_double, _F64P, _F64P]),

@@ -37,6 +37,7 @@ int set_error(int code, const std::string &msg);
 // padded to IMX_MAXNB). Field names follow grid_structure.
 struct HostMesh {
   int32_t nv = 0, ne = 0, nt = 0, glevel = -1;
+  std::string kind = "icos", pos = "eqs", optm = "nopt";  // grid-file header fields
   std::vector<double> v_p, v_lat, v_lon, hx_areag;
   std::vector<int32_t> v_nnb, v_nb, v_ed, v_tr, hx_ttgout, hx_tnrccw, hx_tg, hx_nr;
   std::vector<int32_t> ed_v, ed_sh;
@@ -72,6 +73,9 @@ struct Layout {
 bool build_layout(const imx_mesh_desc &d, bool use_sfc, Layout &L, std::string &err,
                   const int32_t *group = nullptr);
 void desc_of(const HostMesh &m, imx_mesh_desc &d);
+// Grid files in the reference's meshwrite/meshload layout (imx_gridio.cpp).
+bool load_ref(const std::string &dir, const std::string &name, HostMesh &m, std::string &err);
+bool write_ref(const HostMesh &m, const std::string &dir, const std::string &name, std::string &err);
 
 // IMXBIN container: "IMXBIN01", then records (name[24], dtype[4], int64 n, data).
 struct BinWriter {

@@ -446,6 +446,7 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
   m.nt = 2 * nv - 4;
   m.ne = 3 * nv - 6;
   m.glevel = glevel;
+  m.pos = (pos == IMX_POS_POL) ? "pol" : "eqs";
   const int ne = m.ne, nt = m.nt;
   m.v_p = p;
   m.v_p.resize((size_t)3 * nv);

@@ -1,7 +1,11 @@
 !=======================================================================
 ! imodel: driver of the MI355X shallow-water build (src/imodel.f90:1-170).
-! Reads par/mesh.par, builds the icosahedral non-optimised grid with
-! libimx (meshbuild for icos/nopt, src/smeshpack.f90:77-368) and runs
+! Reads par/mesh.par and gets the grid as the reference's meshbuild does
+! (src/smeshpack.f90:77-368): with loadable = 1 and grid/<name>_header.dat
+! present, the grid files are loaded (meshload layout, so grids the
+! reference built or optimised, e.g. SCVT, run unchanged); otherwise the
+! icosahedral non-optimised grid is built with libimx and, with
+! loadable = 1, stored in grid/ for later runs (meshstore). Then runs
 ! simulcase 15, the shallow-water model (src/imodel.f90:152-153).
 !=======================================================================
 program imodel
@@ -13,18 +17,30 @@ program imodel
   type(c_ptr) :: meshp = c_null_ptr
   integer :: g
   integer(c_int32_t) :: pos
+  character(len=128) :: gname
+  logical :: have
 
   call read_mesh_par()
-  if (trim(mkind) /= 'icos') call unsupported('mesh kind', mkind)
-  if (trim(moptm) /= 'nopt') call unsupported('mesh optimisation', moptm)
-  select case (trim(mpos))
-  case ('eqs'); pos = IMX_POS_EQS
-  case ('pol'); pos = IMX_POS_POL
-  case default; call unsupported('mesh position', mpos); pos = 0
-  end select
   g = glevel_for(nv_req)
-  print*, "Mesh: icos ", trim(mpos), " nopt, glevel", g, " nodes", 10*4**g + 2
-  call imx_check(imx_mesh_build_icos(int(g, c_int32_t), pos, meshp), 'meshbuild')
+  gname = grid_name(g)
+  inquire(file='grid/'//trim(gname)//'_header.dat', exist=have)
+  if (loadable == 1 .and. have) then
+    print*, "Mesh: loading grid/", trim(gname)
+    call imx_check(imx_mesh_load_ref('grid/'//c_null_char, trim(gname)//c_null_char, meshp), 'meshload')
+  else
+    if (trim(mkind) /= 'icos') call unsupported('mesh kind', mkind)
+    if (trim(moptm) /= 'nopt') call unsupported('mesh optimisation (without its grid files)', moptm)
+    select case (trim(mpos))
+    case ('eqs'); pos = IMX_POS_EQS
+    case ('pol'); pos = IMX_POS_POL
+    case default; call unsupported('mesh position', mpos); pos = 0
+    end select
+    print*, "Mesh: icos ", trim(mpos), " nopt, glevel", g, " nodes", 10*4**g + 2
+    call imx_check(imx_mesh_build_icos(int(g, c_int32_t), pos, meshp), 'meshbuild')
+    if (loadable == 1) then
+      call imx_check(imx_mesh_write_ref(meshp, 'grid/'//c_null_char, trim(gname)//c_null_char), 'meshstore')
+    end if
+  end if
   call imx_check(imx_mesh_gridprop(meshp), 'initialize_gridprop')
 
   select case (simulcase)

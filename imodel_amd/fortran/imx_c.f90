@@ -47,6 +47,24 @@ module imx_c
       type(imx_mesh_desc) :: d
       integer(c_int) :: rc
     end function
+    ! meshload / meshwrite grid files (<dir>/<name>_*.dat), C strings
+    function imx_mesh_load_ref(dir, name, out) bind(C, name='imx_mesh_load_ref') result(rc)
+      import :: c_int, c_ptr, c_char
+      character(kind=c_char) :: dir(*), name(*)
+      type(c_ptr) :: out
+      integer(c_int) :: rc
+    end function
+    function imx_mesh_write_ref(m, dir, name) bind(C, name='imx_mesh_write_ref') result(rc)
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: m
+      character(kind=c_char) :: dir(*), name(*)
+      integer(c_int) :: rc
+    end function
+    function imx_make_dir(path) bind(C, name='imx_make_dir') result(rc)
+      import :: c_int, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int) :: rc
+    end function
     subroutine imx_mesh_destroy(m) bind(C, name='imx_mesh_destroy')
       import :: c_ptr
       type(c_ptr), value :: m

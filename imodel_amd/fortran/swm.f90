@@ -125,7 +125,7 @@ contains
     filename = "data/"//trim(swmname)//"_mass_energy_"//trim(meshname)//".txt"
     if (.not. opened) then
       opened = .true.
-      call execute_command_line('mkdir -p data')
+      call imx_check(imx_make_dir('data'//c_null_char), 'write_evol')
       buffer = "        n        mvdist  tcase       k       nt        dt(s)       "// &
                "          cfl                 time(dys)             mass               Penergy    "// &
                "          Kenergy            Tenergy            Availenergy           RMSdiv"

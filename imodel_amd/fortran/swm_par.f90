@@ -64,6 +64,26 @@ contains
     if (n > 0) nv_req = n
   end subroutine read_mesh_par
 
+  ! namegrid (src/smeshpack.f90:7422-7464) for icos/octg grids
+  function grid_name(g) result(name)
+    integer, intent(in) :: g
+    character(len=128) :: name
+    character(len=16) :: an, ah
+    write(an, '(i8)') g
+    if (g == 0) then
+      write(ah, '(i2)') 0
+    else
+      write(ah, '(i2)') hrchy
+    end if
+    if (trim(moptm) /= 'nopt') then
+      name = trim(mkind)//'_'//trim(mpos)//'_'//trim(moptm)//'_h'//trim(adjustl(ah))//'_'//trim(adjustl(an))
+    else if (g == 0) then
+      name = trim(mkind)//'_'//trim(mpos)//'_'//trim(adjustl(an))
+    else
+      name = trim(mkind)//'_'//trim(mpos)//'_'//trim(moptm)//'_'//trim(adjustl(an))
+    end if
+  end function grid_name
+
   ! nodesizeadj (src/smeshpack.f90:1497-1558): smallest icosahedral level
   ! with at least nv_req nodes.
   integer function glevel_for(nv) result(g)

@@ -46,6 +46,11 @@ class Mesh:
         h = ctypes.c_void_p()
         capi.check(L.imx_mesh_build_icos(int(glevel), capi.POS_EQS if pos == "eqs" else capi.POS_POL,
                                          ctypes.byref(h)), "imx_mesh_build_icos")
+        return cls._from_handle(h, gridprop)
+
+    @classmethod
+    def _from_handle(cls, h, gridprop):
+        L = capi.lib()
         if gridprop:
             capi.check(L.imx_mesh_gridprop(h), "imx_mesh_gridprop")
         d = capi.MeshDesc()
@@ -63,6 +68,21 @@ class Mesh:
     @classmethod
     def load(cls, path):
         return cls(imxbin.read(path))
+
+    @classmethod
+    def load_ref(cls, griddir, name, gridprop=True):
+        """Grid files written by the reference's meshwrite (<griddir>/<name>_*.dat)."""
+        h = ctypes.c_void_p()
+        capi.check(capi.lib().imx_mesh_load_ref(griddir.encode(), name.encode(), ctypes.byref(h)),
+                   "imx_mesh_load_ref")
+        return cls._from_handle(h, gridprop)
+
+    def write_ref(self, griddir, name):
+        """Write the reference's 11 grid files (meshload reads them)."""
+        if self._handle is None:
+            raise ValueError("write_ref needs a library mesh (Mesh.icos or Mesh.load_ref)")
+        capi.check(capi.lib().imx_mesh_write_ref(self._handle, griddir.encode(), name.encode()),
+                   "imx_mesh_write_ref")
 
     def write(self, path):
         """Full IMXBIN record set (library meshes carry every record)."""

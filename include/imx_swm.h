@@ -140,6 +140,16 @@ int  imx_mesh_build_icos(int32_t glevel, int32_t pos, imx_mesh **out);
 int  imx_mesh_desc_of(const imx_mesh *m, imx_mesh_desc *out);
 int  imx_mesh_gridprop(imx_mesh *m);             /* fills the optional arrays */
 int  imx_mesh_write(const imx_mesh *m, const char *path);  /* IMXBIN records  */
+/* Grid files in the reference's binary layout (meshload / meshwrite,
+   src/smeshpack.f90:7702-8221): <dir>/<name>_header.dat and the ten data
+   files (Fortran unformatted sequential records). Loading fills every array
+   of imx_mesh_desc (call imx_mesh_gridprop after it); writing emits all 11
+   files so the reference's meshload can read a library mesh. */
+int  imx_mesh_load_ref(const char *dir, const char *name, imx_mesh **out);
+int  imx_mesh_write_ref(const imx_mesh *m, const char *dir, const char *name);
+/* mkdir -p for the hosts' output folders (grid/, data/): no counterpart in
+   the reference, whose runs expect the folders to exist. */
+int  imx_make_dir(const char *path);
 void imx_mesh_destroy(imx_mesh *m);
 /* initialize_gridprop on caller arrays: trhx_intersec_areas
    (src/smeshpack.f90:5080-5154), hxtr_intersec_areas (:4991-5078) and

@@ -328,7 +328,7 @@ program ref_harness
   use constants, only: i4, r8
   use imxbin
   use harness_mesh
-  use smeshpack, only: meshbuild, showonscreen
+  use smeshpack, only: meshbuild, meshload, showonscreen
   use swm_data
   use swm, only: tendency, ode_rk4, initialize_gridprop, initialize_fields, &
        calc_energies, sumf_areas, sumfsq_areas
@@ -336,6 +336,7 @@ program ref_harness
   implicit none
 
   character(len=512) :: cmd, a1, a2, a3, meshsrc, statesrc, outf
+  character(len=128) :: hdr
   integer :: glevel, nsteps, idiag, k, ios, pc1, pc2
   real(r8) :: dtin, time, wall0, wall1
   real(r8) :: pe0, ke0, te0, ae0, pe, ke, te, ae, mass0
@@ -359,6 +360,20 @@ program ref_harness
     call initialize_gridprop()
     call bin_open_w(trim(outf))
     call dump_mesh(mesh, .true.)
+    call bin_close()
+
+  case ('loadref')
+    ! ref_harness loadref <name> <out.bin>: meshload of grid/<name>_*.dat
+    call get_command_argument(2, a1)
+    call get_command_argument(3, outf)
+    mesh%kind = 'icos'
+    mesh%pos = 'eqs'   ! meshload leaves pos to the caller (printmesh checks it)
+    mesh%optm = 'nopt'
+    mesh%name = trim(a1)
+    hdr = 'grid/'//trim(a1)//'_header.dat'
+    call meshload(mesh, hdr)
+    call bin_open_w(trim(outf))
+    call dump_mesh(mesh, .false.)
     call bin_close()
 
   case ('run')
@@ -438,6 +453,7 @@ program ref_harness
   case default
     print*, 'usage: ref_harness build <glevel> <eqs|pol> <out.bin>'
     print*, '       ref_harness run <build:G:pos|mesh.bin> <tc2|state.bin> <nsteps> <dt> <diag> <out.bin>'
+    print*, '       ref_harness loadref <name> <out.bin>   (reads grid/<name>_*.dat)'
     stop 1
   end select
 

@@ -16,17 +16,18 @@ import pytest
 from imodel_amd import imxbin
 from imodel_amd.build import IMODEL
 from imodel_amd.swm import Mesh
-from oracle.oracle import Oracle
+from oracle.oracle import REF_HARNESS, Oracle, ref_harness_available
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _par(td, nv, testcase, dt, days, nprints, precision, dump=1):
-    os.makedirs(os.path.join(td, "par"))
+def _par(td, nv, testcase, dt, days, nprints, precision, dump=1, loadable=0):
+    os.makedirs(os.path.join(td, "par"), exist_ok=True)
     src = os.path.join(ROOT, "par")
     mesh = open(os.path.join(src, "mesh.par")).read().splitlines()
     mesh[2] = str(nv)
+    mesh[10] = str(loadable)
     open(os.path.join(td, "par", "mesh.par"), "w").write("\n".join(mesh) + "\n")
     swm = open(os.path.join(src, "swm.par")).read().splitlines()
     swm[2] = f"{testcase} 0"
@@ -74,3 +75,46 @@ def test_fortran_driver_seeded_fp32_runs():
     assert int(last[3]) == 96
     assert abs(float(last[8])) < 1e-8       # mass drift, FP32 state
     assert abs(float(last[11])) < 1e-4      # total energy drift
+
+
+STATE = "swm_tc2_dt900_HTC_trsk10_areageo_state_icos_eqs_nopt_3.bin"
+
+
+@pytest.mark.skipif(not ref_harness_available(), reason="reference harness not built (oracle/Makefile.ref)")
+def test_fortran_driver_runs_reference_grid_files():
+    """loadable = 1: the driver loads the grid files the reference's own
+    meshbuild stored (its numbering, not ours) and its one-day FP64_REF run
+    equals the oracle on the reference's in-memory mesh bit for bit."""
+    print("[gpu-case] fortran imodel g3 tc2 on reference grid files", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        subprocess.run([REF_HARNESS, "build", "3", "eqs", os.path.join(td, "ref.bin")], cwd=td, check=True,
+                       stdout=subprocess.DEVNULL)
+        ref = imxbin.read(os.path.join(td, "ref.bin"))
+        shutil.rmtree(os.path.join(td, "par"))
+        _par(td, 642, 2, 900, 1, 24, "fp64_ref", loadable=1)
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        assert "loading grid/icos_eqs_nopt_3" in r.stdout
+        st = imxbin.read(os.path.join(td, "data", STATE))
+        m = Mesh.load_ref(os.path.join(td, "grid"), "icos_eqs_nopt_3")
+    h0, u0 = m.tc2()
+    h, u = Oracle(ref).step(h0, u0, 900.0, 96)
+    assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
+
+
+def test_fortran_driver_stores_then_loads_grid():
+    """loadable = 1 without grid files: the driver builds and stores the grid
+    (meshstore); a second run loads it and reproduces the first bit for bit."""
+    print("[gpu-case] fortran imodel g3 store/load grid", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 24, "fp64_ref", loadable=1)
+        runs = []
+        for _ in range(2):
+            r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                               text=True, timeout=120)
+            assert r.returncode == 0, r.stdout[-2000:]
+            runs.append((r.stdout, imxbin.read(os.path.join(td, "data", STATE))))
+        assert len([f for f in os.listdir(os.path.join(td, "grid")) if f.endswith(".dat")]) == 11
+    assert "loading grid/" not in runs[0][0] and "loading grid/icos_eqs_nopt_3" in runs[1][0]
+    assert np.array_equal(runs[0][1]["h"], runs[1][1]["h"]) and np.array_equal(runs[0][1]["u"], runs[1][1]["u"])
