@@ -110,6 +110,25 @@ int imx_ic_tc2(const imx_mesh *m, double erad, double *h, double *u) {
   return IMX_OK;
 }
 
+// TC2's exact potential vorticity at triangle circumcentres
+// (src/swm.f90:1293-1296): eta_ct sin(lat) / (h0 - h_ct sin(lat)^2).
+int imx_ic_tc2_pv(const imx_mesh *m, double erad, double *q) {
+  if (!m || !q) return imx::set_error(IMX_EINVAL, "imx_ic_tc2_pv: NULL argument");
+  const double pi = 3.1415927410125732;  // constants.f90:31 (single-rounded)
+  const double pi2 = 2.0 * pi;
+  const double day2sec = 86400.0;
+  const double gravi = 1.0 / 9.80616, omega = 7.292e-5;
+  const double u0 = pi2 * erad / (12.0 * day2sec);
+  const double h0 = 2.94e4 * gravi;
+  const double h_ct = (erad * omega * u0 + (u0 * u0) / 2.0) * gravi;
+  const double eta_ct = 2.0 * u0 / erad + 2.0 * omega;
+  for (int k = 0; k < m->nt; ++k) {
+    const double s = std::sin(m->tr_clat[k]);
+    q[k] = eta_ct * s / (h0 - h_ct * (s * s));
+  }
+  return IMX_OK;
+}
+
 int imx_ic_spectral(const imx_mesh_desc *d, uint64_t seed, int32_t lmax, double h0,
                     double hfrac, double urms, double erad, double *h, double *u) {
   if (!d || !h || !u) return imx::set_error(IMX_EINVAL, "imx_ic_spectral: NULL argument");

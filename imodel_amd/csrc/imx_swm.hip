@@ -1277,8 +1277,22 @@ int imx_swm_tendency(imx_swm *s, const double *h, const double *u, double *masse
 
 int imx_swm_get_field(imx_swm *s, int32_t which, double *out) {
   if (!s || !out) return imx::set_error(IMX_EINVAL, "imx_swm_get_field: NULL argument");
-  if (s->fast)
-    return imx::set_error(IMX_ENOTSUP, "imx_swm_get_field: only IMX_PREC_FP64_REF keeps the intermediates");
+  if (s->fast) {
+    // the fused passes keep only q on triangles (from the last RK stage)
+    if (which != IMX_FIELD_Q_TR)
+      return imx::set_error(IMX_ENOTSUP,
+                            "imx_swm_get_field: fused handles keep only IMX_FIELD_Q_TR; use IMX_PREC_FP64_REF");
+    std::vector<double> b(s->NT);
+    if (s->single) {
+      std::vector<float> f(s->NT);
+      HIPTRY(hipMemcpy(f.data(), s->q, s->NT * sizeof(float), hipMemcpyDeviceToHost), "get_field");
+      for (int t = 0; t < s->NT; ++t) b[t] = f[t];
+    } else {
+      HIPTRY(hipMemcpy(b.data(), s->q, s->NT * sizeof(double), hipMemcpyDeviceToHost), "get_field");
+    }
+    for (int t = 0; t < s->nt; ++t) out[t] = b[s->lay.tri_new[t]];
+    return IMX_OK;
+  }
   double *src = nullptr;
   int kind = 0;  // 0 cell, 1 edge, 2 tri
   switch (which) {

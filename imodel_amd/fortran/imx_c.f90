@@ -9,6 +9,7 @@ module imx_c
   implicit none
 
   integer(c_int32_t), parameter :: IMX_PREC_FP64_REF = 0, IMX_PREC_FP64 = 1, IMX_PREC_FP32 = 2
+  integer(c_int32_t), parameter :: IMX_FIELD_Q_TR = 4, IMX_FIELD_KE_HX = 7, IMX_FIELD_DIVU = 11
   integer(c_int32_t), parameter :: IMX_POS_EQS = 0, IMX_POS_POL = 1
   integer, parameter :: IMX_NDIAG = 8
 
@@ -69,6 +70,20 @@ module imx_c
       import :: c_ptr
       type(c_ptr), value :: m
     end subroutine
+    function imx_ic_tc2_pv(m, erad, q) bind(C, name='imx_ic_tc2_pv') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: m
+      real(c_double), value :: erad
+      real(c_double) :: q(*)
+      integer(c_int) :: rc
+    end function
+    function imx_swm_get_field(s, which, out) bind(C, name='imx_swm_get_field') result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: s
+      integer(c_int32_t), value :: which
+      real(c_double) :: out(*)
+      integer(c_int) :: rc
+    end function
     function imx_ic_tc2(m, erad, h, u) bind(C, name='imx_ic_tc2') result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: m

@@ -15,9 +15,14 @@ module swm
   real(r8), parameter :: omega = 7.292e-5_r8   ! constants.f90:67
   real(r8), parameter :: rad2deg = 57.295779513082323_r8
 
+  real(r8), parameter :: sec2day = 1._r8/86400._r8
   type(c_ptr) :: handle = c_null_ptr
   type(imx_mesh_desc) :: mdesc
   real(r8), allocatable :: h(:), u(:)
+  ! analytic reference solution (RefSolAnal, src/swm_data.f90:685): TC2
+  logical :: refsol = .false.
+  integer :: blowup = 0
+  real(r8), allocatable :: h_exact(:), u_exact(:), q_exact(:), qtr(:)
   character(len=256) :: swmname = '', meshname = ''
   real(r8) :: cfl = 0._r8, meanvdist = 0._r8
 
@@ -69,14 +74,28 @@ contains
     call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
     call imx_check(imx_swm_set_state(handle, h, u), 'imx_swm_set_state')
     call imx_check(imx_swm_diagnostics(handle, d0), 'imx_swm_diagnostics')
-    call write_evol(0, 0._r8, d0(1), d0(2), d0(3), d0(4), d0(5), d0(7))
+    refsol = (testcase == 2)
+    if (refsol) then
+      allocate(h_exact(mdesc%nv), u_exact(mdesc%ne), q_exact(mdesc%nt), qtr(mdesc%nt))
+      h_exact = h
+      u_exact = u
+      call imx_check(imx_ic_tc2_pv(meshp, erad, q_exact), 'imx_ic_tc2_pv')
+    end if
+    ! swm.f90:226-231: both files start with the absolute initial values
+    call write_evol_error(0, 0._r8, d0(1), 0._r8, 0._r8, 0._r8, 0._r8, 0._r8, 0._r8, &
+                          d0(2), d0(3), d0(4), d0(5), 0._r8, 0._r8, 0._r8, 0._r8)
+    call write_evol(0, 0._r8, d0(1), d0(2), d0(3), d0(4), d0(5), 0._r8)
 
-    ! time loop of swm_tests (swm.f90:238-370): diagnostics at k<=2,
-    ! k==ntime and every nprints steps; the steps in between run on the GPU
-    ! without leaving it
+    ! time loop of swm_tests (swm.f90:238-370). With the analytic solution
+    ! (TC2) errors are written every nprints steps and at the end
+    ! (write_evol_error, write_errors); otherwise the diagnostics at k<=2,
+    ! k==ntime and every nprints steps (write_evol). The steps in between
+    ! run on the GPU without leaving it, so the reference's per-step
+    ! blow-up test is made at the written steps only.
     k = 0
+    blowup = 0
     do while (k < ntime)
-      if (k < 2) then
+      if (k < 2 .and. .not. refsol) then
         kn = k + 1
       else
         kn = min(ntime, (k/nprints + 1)*nprints)
@@ -89,8 +108,12 @@ contains
         print*, "Stopping due to NaN/Inf in the state at step", k
         exit
       end if
-      call write_evol(k, time, (d(1) - d0(1))/d0(1), (d(2) - d0(2))/d0(2), (d(3) - d0(3))/d0(3), &
-                      (d(4) - d0(4))/d0(4), (d(5) - d0(5))/d0(5), d(7))
+      if (refsol) then
+        if (.not. write_ref_errors(k, time, d, d0)) exit
+      else
+        call write_evol(k, time, (d(1) - d0(1))/d0(1), (d(2) - d0(2))/d0(2), (d(3) - d0(3))/d0(3), &
+                        (d(4) - d0(4))/d0(4), (d(5) - d0(5))/d0(5), d(7))
+      end if
     end do
 
     call imx_check(imx_swm_get_state(handle, h, u), 'imx_swm_get_state')
@@ -114,6 +137,186 @@ contains
     call imx_check(imx_swm_step(handle, 1_c_int32_t), 'ode_rk4')
     call imx_check(imx_swm_get_state(handle, h_new, u_new), 'ode_rk4')
   end subroutine ode_rk4
+
+  ! error_calc (src/swm.f90:3492-3543) with sumfsq_areas weights w
+  ! (:3442-3490): relative max and L2 errors of f against fref
+  subroutine error_calc(f, fref, w, errormaxrel, error2rel, errormax)
+    real(r8), intent(in) :: f(:), fref(:), w(:)
+    real(r8), intent(out) :: errormaxrel, error2rel, errormax
+    real(r8) :: refsum, refmax, difsum, sw
+    integer :: i
+    refsum = 0._r8; difsum = 0._r8; sw = 0._r8
+    do i = 1, size(f)
+      sw = sw + w(i)
+      refsum = refsum + w(i)*fref(i)**2
+      difsum = difsum + w(i)*(f(i) - fref(i))**2
+    end do
+    refsum = refsum/sw
+    difsum = difsum/sw
+    if (refsum > 0) then
+      error2rel = sqrt(difsum)/sqrt(refsum)
+    else
+      error2rel = sqrt(difsum)
+    end if
+    refmax = maxval(abs(fref))
+    errormax = maxval(abs(f - fref))
+    if (refmax > 0) then
+      errormaxrel = errormax/refmax
+    else
+      errormaxrel = errormax
+    end if
+  end subroutine error_calc
+
+  ! The analytic-solution branch of the time loop (src/swm.f90:291-349):
+  ! errors of h, u and q_tr against the exact fields, max |div u| and
+  ! max |grad K| over edges, written by write_evol_error every nprints
+  ! steps and by write_errors at the last step. With IMX_PREC_FP64_REF,
+  ! q_tr, div u and K are those the reference reads (its last RK stage);
+  ! fused handles give q_tr of the last stage and div u, K of the state.
+  ! Returns .false. when the run should stop (blow-up, swm.f90:331-340).
+  logical function write_ref_errors(k, time, d, d0) result(go_on)
+    integer, intent(in) :: k
+    real(r8), intent(in) :: time, d(:), d0(:)
+    real(r8), pointer :: areag(:), leng(:), dleng(:), trarea(:)
+    real(r8), allocatable :: we(:), divu(:), ke(:)
+    real(r8) :: emr_h, e2_h, em_h, emr_u, e2_u, em_u, emr_q, e2_q, em_q
+    real(r8) :: maxdiv, max_gradke, rmsdiv, sw
+    integer(c_int32_t), pointer :: ev(:)
+    integer :: l
+    go_on = .true.
+    call c_f_pointer(mdesc%hx_areag, areag, [mdesc%nv])
+    call c_f_pointer(mdesc%ed_leng, leng, [mdesc%ne])
+    call c_f_pointer(mdesc%edhx_leng, dleng, [mdesc%ne])
+    call c_f_pointer(mdesc%tr_areag, trarea, [mdesc%nt])
+    call c_f_pointer(mdesc%ed_v, ev, [2*mdesc%ne])
+    call imx_check(imx_swm_get_state(handle, h, u), 'imx_swm_get_state')
+    call imx_check(imx_swm_get_field(handle, IMX_FIELD_Q_TR, qtr), 'imx_swm_get_field')
+    allocate(we(mdesc%ne), divu(mdesc%nv), ke(mdesc%nv))
+    we = leng*dleng*0.5_r8
+    call error_calc(h, h_exact, areag, emr_h, e2_h, em_h)
+    call error_calc(u, u_exact, we, emr_u, e2_u, em_u)
+    call error_calc(qtr, q_exact, trarea, emr_q, e2_q, em_q)
+    if (precision_code() == IMX_PREC_FP64_REF) then
+      call imx_check(imx_swm_get_field(handle, IMX_FIELD_DIVU, divu), 'imx_swm_get_field')
+      call imx_check(imx_swm_get_field(handle, IMX_FIELD_KE_HX, ke), 'imx_swm_get_field')
+    else
+      call div_ke(u, divu, ke)
+    end if
+    sw = sum(areag)
+    rmsdiv = sqrt(sum(areag*divu**2)/sw)
+    maxdiv = maxval(abs(divu))
+    max_gradke = 0._r8
+    do l = 1, mdesc%ne   ! HTC: ed%v(1) -> ed%v(2) (swm.f90:318-326)
+      max_gradke = max(max_gradke, abs((ke(ev(2*l)) - ke(ev(2*l-1)))/leng(l)/erad))
+    end do
+    if (mod(k, nprints) == 0) &
+      call write_evol_error(k, time, (d(1) - d0(1))/d0(1), emr_h, e2_h, emr_u, e2_u, emr_q, e2_q, &
+                            (d(2) - d0(2))/d0(2), (d(3) - d0(3))/d0(3), (d(4) - d0(4))/d0(4), &
+                            (d(5) - d0(5))/d0(5), rmsdiv, maxdiv, max_gradke, 1._r8)
+    if ((emr_h > 20.0_r8 .or. isnan(emr_h)) .and. k > 3) then
+      blowup = blowup + 1
+      print*, "System might be unstable, large errors:", emr_h
+      if (blowup >= 5 .or. isnan(emr_h) .or. emr_h > 100000000.0_r8) then
+        print*, "Stopping due to large errors", emr_h
+        go_on = .false.
+        return
+      end if
+    end if
+    if (k == ntime) &
+      call write_errors(time, emr_h, e2_h, emr_u, e2_u, (d(1) - d0(1))/d0(1), (d(2) - d0(2))/d0(2), &
+                        (d(3) - d0(3))/d0(3), (d(4) - d0(4))/d0(4), 1._r8)
+  end function write_ref_errors
+
+  ! div_hx(u) and kinetic_energy_hx (trsk, src/swm_operators.f90:613-661,
+  ! :1132-1217) of the given state on the host, caller numbering
+  subroutine div_ke(uu, divu, ke)
+    real(r8), intent(in) :: uu(:)
+    real(r8), intent(out) :: divu(:), ke(:)
+    real(r8), pointer :: areag(:), leng(:), dleng(:)
+    integer(c_int32_t), pointer :: nnb(:), ved(:), ttg(:)
+    real(r8) :: s, k, ca, w
+    integer :: i, j, l
+    call c_f_pointer(mdesc%hx_areag, areag, [mdesc%nv])
+    call c_f_pointer(mdesc%ed_leng, leng, [mdesc%ne])
+    call c_f_pointer(mdesc%edhx_leng, dleng, [mdesc%ne])
+    call c_f_pointer(mdesc%v_nnb, nnb, [mdesc%nv])
+    call c_f_pointer(mdesc%v_ed, ved, [6*mdesc%nv])
+    call c_f_pointer(mdesc%hx_ttgout, ttg, [6*mdesc%nv])
+    do i = 1, mdesc%nv
+      s = 0._r8; k = 0._r8; ca = 0._r8
+      do j = 1, nnb(i)
+        l = ved(6*(i-1)+j)
+        s = s + real(ttg(6*(i-1)+j), r8)*uu(l)*dleng(l)
+        w = 0.25_r8*leng(l)*dleng(l)
+        ca = ca + w
+        k = k + w*uu(l)**2
+      end do
+      divu(i) = s/areag(i)/erad
+      ke(i) = k/ca
+    end do
+  end subroutine div_ke
+
+  ! write_evol_error (src/swm.f90:2999-3086): same file name, header, format
+  subroutine write_evol_error(k, time, tmass, errormax_h, error2_h, errormax_u, error2_u, &
+                              errormax_pv, error2_pv, penergy, kenergy, tenergy, availenergy, &
+                              rmsdiv, maxdiv, max_gradke, tmp)
+    integer, intent(in) :: k
+    real(r8), intent(in) :: time, tmass, errormax_h, error2_h, errormax_u, error2_u, errormax_pv, &
+                            error2_pv, penergy, kenergy, tenergy, availenergy, rmsdiv, maxdiv, &
+                            max_gradke, tmp
+    character(len=512) :: filename, buffer
+    integer, save :: iunit = 0
+    logical, save :: opened = .false.
+    filename = "data/"//trim(swmname)//"_evolution_"//trim(meshname)//".txt"
+    if (.not. opened) then
+      opened = .true.
+      call imx_check(imx_make_dir('data'//c_null_char), 'write_evol_error')
+      buffer = "        n        mvdist  tcase       k       nt    "// &
+               "     dt                 cfl               TimeDays  "// &
+               "        errormax_h       error2_h               errormax_u     "// &
+               "   error2_u               errormax_pv       error2_pv            "// &
+               " mass              Penergy      "// &
+               "     Kenergy             Tenergy          Availenergy      "// &
+               "     RMSdiv              maxdiv    "// &
+               "     max_gradke          tmp        SWMparameters"
+      open(newunit=iunit, file=trim(filename), status='replace')
+      write(iunit, '(a)') trim(buffer)
+    end if
+    write(iunit, '(i12, 1f12.4, i4, 2i10, 18e20.10 , a100)') mdesc%nv, meanvdist*rad2deg, testcase, &
+      k, ntime, dt, cfl, time*sec2day, errormax_h, error2_h, errormax_u, error2_u, errormax_pv, &
+      error2_pv, tmass, penergy, kenergy, tenergy, availenergy, rmsdiv, maxdiv, max_gradke, tmp, &
+      trim(adjustl(trim(swmname)))//"_"//trim(adjustl(trim(meshname)))
+    flush(iunit)
+  end subroutine write_evol_error
+
+  ! write_errors (src/swm.f90:2922-2997): appended to data/swm_errors.txt
+  subroutine write_errors(time, errormax_h, error2_h, errormax_u, error2_u, error_mass, &
+                          penergy, kenergy, tenergy, tmp)
+    real(r8), intent(in) :: time, errormax_h, error2_h, errormax_u, error2_u, error_mass, &
+                            penergy, kenergy, tenergy, tmp
+    character(len=512) :: buffer
+    character(len=*), parameter :: filename = "data/swm_errors.txt"
+    integer :: iunit
+    logical :: ifile
+    buffer = "       Mesh  TCase    nTime"// &
+             "     dt                   TimeDays  "// &
+             "        MaxError_h          RMSError_h            MaxError_u     "// &
+             "   RMSError_u               Mass              Penergy      "// &
+             "     Kenergy             Tenergy             ExtraData                    "// &
+             "                Methods  Grid    "
+    call imx_check(imx_make_dir('data'//c_null_char), 'write_errors')
+    inquire(file=filename, exist=ifile)
+    if (ifile) then
+      open(newunit=iunit, file=filename, status='old', position='append')
+    else
+      open(newunit=iunit, file=filename, status='replace')
+      write(iunit, '(a)') buffer
+    end if
+    write(iunit, '(i12, i4, i10, 11e20.10 , a120)') mdesc%nv, testcase, ntime, dt, time*sec2day, &
+      errormax_h, error2_h, errormax_u, error2_u, error_mass, penergy, kenergy, tenergy, tmp, &
+      trim(adjustl(trim(swmname)))//" "//trim(adjustl(trim(meshname)))
+    close(iunit)
+  end subroutine write_errors
 
   ! write_evol (src/swm.f90:3088-3152): same file name, header and format
   subroutine write_evol(k, time, tmass, penergy, kenergy, tenergy, availenergy, rmsdiv)

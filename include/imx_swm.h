@@ -161,6 +161,9 @@ int  imx_gridprop(const imx_mesh_desc *d, double *tr_trhx_area,
 /* Williamson TC2 as initialize_fields case(2) with HTC (src/swm.f90:1205-1247)
    on a library-built mesh (needs ed%c%lat, ed%tg and v%lat). */
 int  imx_ic_tc2(const imx_mesh *m, double erad, double *h, double *u);
+/* TC2's exact potential vorticity q_tr_exact at the triangle circumcentres
+   (src/swm.f90:1293-1296), for the error columns of write_evol_error. */
+int  imx_ic_tc2_pv(const imx_mesh *m, double erad, double *q_tr);
 /* Seeded spectral state: real spherical harmonics l=1..lmax, coefficients
    N(0,1)*l^-2 from SplitMix64 + Box-Muller; h = h0 + h', h' area-rms =
    hfrac*h0; u = -(psi(t2)-psi(t1))/(l_e a) * sign, scaled to area-rms
@@ -177,6 +180,9 @@ int  imx_swm_get_state(imx_swm *s, double *h, double *u);
 /* subroutine tendency(h, u, masseq, momeq)       src/swm.f90:2416-2598 */
 int  imx_swm_tendency(imx_swm *s, const double *h, const double *u,
                       double *masseq, double *momeq);
+/* An intermediate of the last tendency evaluation (after a step: its 4th
+   RK stage, the module globals the reference leaves behind). FP64_REF keeps
+   every IMX_FIELD_*; fused handles keep only IMX_FIELD_Q_TR. */
 int  imx_swm_get_field(imx_swm *s, int32_t which, double *out);
 /* nsteps x ode_rk4 on the resident state            src/swm.f90:2600-2683 */
 int  imx_swm_step(imx_swm *s, int32_t nsteps);

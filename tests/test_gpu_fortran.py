@@ -41,7 +41,17 @@ def _par(td, nv, testcase, dt, days, nprints, precision, dump=1, loadable=0):
     open(os.path.join(td, "par", "swm_gpu.par"), "w").write("\n".join(gpu) + "\n")
 
 
+def _error_calc(f, fref, w):
+    """error_calc (src/swm.f90:3492-3543) with sumfsq_areas weights."""
+    e2 = np.sqrt(np.sum(w * (f - fref) ** 2) / np.sum(w)) / np.sqrt(np.sum(w * fref ** 2) / np.sum(w))
+    return np.abs(f - fref).max() / np.abs(fref).max(), e2
+
+
 def test_fortran_driver_tc2_day_bitexact():
+    """TC2 has an analytic solution, so the driver writes the reference's
+    error outputs: the _evolution_ file every nprints steps (errors of h, u,
+    q_tr, max div, max grad K) and data/swm_errors.txt at the end; the
+    final state equals the oracle's bit for bit."""
     print("[gpu-case] fortran imodel g3 tc2 fp64_ref", flush=True)
     assert os.access(IMODEL, os.X_OK), "Fortran host not built"
     with tempfile.TemporaryDirectory() as td:
@@ -49,18 +59,43 @@ def test_fortran_driver_tc2_day_bitexact():
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)
         assert r.returncode == 0, r.stdout[-2000:]
-        st = imxbin.read(os.path.join(td, "data", "swm_tc2_dt900_HTC_trsk10_areageo_state_icos_eqs_nopt_3.bin"))
-        evol = open(os.path.join(td, "data", "swm_tc2_dt900_HTC_trsk10_areageo_mass_energy_icos_eqs_nopt_3.txt")).read().splitlines()
+        st = imxbin.read(os.path.join(td, "data", STATE))
+        base = "swm_tc2_dt900_HTC_trsk10_areageo"
+        evo = open(os.path.join(td, "data", f"{base}_evolution_icos_eqs_nopt_3.txt")).read().splitlines()
+        me = open(os.path.join(td, "data", f"{base}_mass_energy_icos_eqs_nopt_3.txt")).read().splitlines()
+        errs = open(os.path.join(td, "data", "swm_errors.txt")).read().splitlines()
     m = Mesh.icos(3)
+    a = m.arrays
     h0, u0 = m.tc2()
-    h, u = Oracle(m.arrays).step(h0, u0, 900.0, 96)
+    o = Oracle(a)
+    h, u = o.step(h0, u0, 900.0, 96)
     assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
-    assert evol[0].split()[:3] == ["n", "mvdist", "tcase"]
-    rows = [line.split() for line in evol[1:]]
-    ks = [int(row[3]) for row in rows]
-    assert ks == [0, 1, 2, 24, 48, 72, 96]
-    assert all(int(row[0]) == 642 and int(row[4]) == 96 for row in rows)
-    assert abs(float(rows[-1][8])) < 1e-13  # relative mass change after one day, FP64
+    # the mass/energy file holds only the initial absolute values (swm.f90:231)
+    assert len(me) == 2 and int(me[1].split()[3]) == 0
+    assert evo[0].split()[:4] == ["n", "mvdist", "tcase", "k"]
+    rows = [line.split() for line in evo[1:]]
+    assert [int(x[3]) for x in rows] == [0, 24, 48, 72, 96]
+    assert all(float(v) == 0.0 for v in rows[0][8:14])
+    q_exact = np.zeros(m.nt)
+    from imodel_amd import capi
+    capi.check(capi.lib().imx_ic_tc2_pv(m._handle, 6.37122e6, capi.f64p(q_exact)), "imx_ic_tc2_pv")
+    A, At = a["hx_areag"], a["tr_areag"]
+    We = a["ed_leng"] * a["edhx_leng"] * 0.5
+    ev = a["ed_v"].reshape(-1, 2) - 1
+    hk, uk = h0, u0
+    for row in rows[1:]:
+        hk, uk = o.step(hk, uk, 900.0, 24)
+        q, divu, ke = o.field("q_tr"), o.field("divu"), o.field("ke_hx")
+        want = [*_error_calc(hk, h0, A), *_error_calc(uk, u0, We), *_error_calc(q, q_exact, At),
+                np.sqrt(np.sum(A * divu ** 2) / np.sum(A)), np.abs(divu).max(),
+                np.abs((ke[ev[:, 1]] - ke[ev[:, 0]]) / a["ed_leng"] / 6.37122e6).max()]
+        got = [float(row[i]) for i in (8, 9, 10, 11, 12, 13, 19, 20, 21)]
+        assert np.allclose(got, want, rtol=1e-9, atol=0), (row[3], got, want)
+        assert float(row[22]) == 1.0
+    e = errs[1].split()
+    assert errs[0].split()[:3] == ["Mesh", "TCase", "nTime"] and len(errs) == 2
+    assert (int(e[0]), int(e[1]), int(e[2])) == (642, 2, 96)
+    assert np.allclose([float(x) for x in e[5:9]], [float(rows[-1][i]) for i in (8, 9, 10, 11)], rtol=1e-9)
 
 
 def test_fortran_driver_seeded_fp32_runs():
