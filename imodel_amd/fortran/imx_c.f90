@@ -9,7 +9,8 @@ module imx_c
   implicit none
 
   integer(c_int32_t), parameter :: IMX_PREC_FP64_REF = 0, IMX_PREC_FP64 = 1, IMX_PREC_FP32 = 2
-  integer(c_int32_t), parameter :: IMX_FIELD_Q_TR = 4, IMX_FIELD_KE_HX = 7, IMX_FIELD_DIVU = 11
+  integer(c_int32_t), parameter :: IMX_FIELD_ETA = 2, IMX_FIELD_Q_TR = 4, IMX_FIELD_Q_ED = 5, &
+                                   IMX_FIELD_DIVUH = 6, IMX_FIELD_KE_HX = 7, IMX_FIELD_DIVU = 11
   integer(c_int32_t), parameter :: IMX_POS_EQS = 0, IMX_POS_POL = 1
   integer, parameter :: IMX_NDIAG = 8
 
@@ -59,6 +60,14 @@ module imx_c
       import :: c_int, c_ptr, c_char
       type(c_ptr), value :: m
       character(kind=c_char) :: dir(*), name(*)
+      integer(c_int) :: rc
+    end function
+    function imx_remap_ll(d, pos, f, nlon, nlat, out) bind(C, name='imx_remap_ll') result(rc)
+      import :: c_int, c_int32_t, c_double, c_float, imx_mesh_desc
+      type(imx_mesh_desc) :: d
+      integer(c_int32_t), value :: pos, nlon, nlat
+      real(c_double) :: f(*)
+      real(c_float) :: out(*)
       integer(c_int) :: rc
     end function
     function imx_make_dir(path) bind(C, name='imx_make_dir') result(rc)

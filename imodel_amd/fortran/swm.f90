@@ -21,7 +21,7 @@ module swm
   real(r8), allocatable :: h(:), u(:)
   ! analytic reference solution (RefSolAnal, src/swm_data.f90:685): TC2
   logical :: refsol = .false.
-  integer :: blowup = 0
+  integer :: blowup = 0, plotsteps = 1
   real(r8), allocatable :: h_exact(:), u_exact(:), q_exact(:), qtr(:)
   character(len=256) :: swmname = '', meshname = ''
   real(r8) :: cfl = 0._r8, meanvdist = 0._r8
@@ -85,6 +85,9 @@ contains
     call write_evol_error(0, 0._r8, d0(1), 0._r8, 0._r8, 0._r8, 0._r8, 0._r8, 0._r8, &
                           d0(2), d0(3), d0(4), d0(5), 0._r8, 0._r8, 0._r8, 0._r8)
     call write_evol(0, 0._r8, d0(1), d0(2), d0(3), d0(4), d0(5), 0._r8)
+    plotsteps = 1
+    if (nplots > 0) plotsteps = max(1, ntime/nplots)   ! swm_data.f90:589-597
+    call plotfields(0, 0._r8)
 
     ! time loop of swm_tests (swm.f90:238-370). With the analytic solution
     ! (TC2) errors are written every nprints steps and at the end
@@ -95,11 +98,9 @@ contains
     k = 0
     blowup = 0
     do while (k < ntime)
-      if (k < 2 .and. .not. refsol) then
-        kn = k + 1
-      else
-        kn = min(ntime, (k/nprints + 1)*nprints)
-      end if
+      kn = min(ntime, (k/nprints + 1)*nprints)
+      if (k < 2 .and. .not. refsol) kn = k + 1
+      if (nplots > 0) kn = min(kn, (k/plotsteps + 1)*plotsteps)
       call imx_check(imx_swm_step(handle, int(kn - k, c_int32_t)), 'imx_swm_step')
       k = kn
       time = real(k, r8)*dt
@@ -109,11 +110,14 @@ contains
         exit
       end if
       if (refsol) then
-        if (.not. write_ref_errors(k, time, d, d0)) exit
-      else
+        if (mod(k, nprints) == 0 .or. k == ntime) then
+          if (.not. write_ref_errors(k, time, d, d0)) exit
+        end if
+      else if (k <= 2 .or. k == ntime .or. mod(k, nprints) == 0) then
         call write_evol(k, time, (d(1) - d0(1))/d0(1), (d(2) - d0(2))/d0(2), (d(3) - d0(3))/d0(3), &
                         (d(4) - d0(4))/d0(4), (d(5) - d0(5))/d0(5), d(7))
       end if
+      call plotfields(k, time)
     end do
 
     call imx_check(imx_swm_get_state(handle, h, u), 'imx_swm_get_state')
@@ -137,6 +141,70 @@ contains
     call imx_check(imx_swm_step(handle, 1_c_int32_t), 'ode_rk4')
     call imx_check(imx_swm_get_state(handle, h_new, u_new), 'ode_rk4')
   end subroutine ode_rk4
+
+  ! plotfields (src/swm.f90:2754-2904) for runs without the local
+  ! truncation-error test: at k = 0, every plotsteps steps and at the end,
+  ! h, eta, K, q_tr (pv), q_ed (pv_ed), div(uh) and, with the analytic
+  ! solution, the h, u and q_tr errors, each sampled on the 1440 x 720
+  ! lat-lon grid by imx_remap_ll (plot_scalarfield). eta, K, q_ed and div(uh)
+  ! are intermediates of the last RK stage that only IMX_PREC_FP64_REF keeps;
+  ! fused handles write h, pv and the errors.
+  subroutine plotfields(k, time)
+    integer, intent(in) :: k
+    real(r8), intent(in) :: time
+    character(len=60) :: atime
+    real(r8), allocatable :: f(:)
+    logical :: ref
+    if (nplots <= 0) return
+    if (.not. (k == ntime .or. mod(k, plotsteps) == 0)) return
+    write(atime, *) nint(time)
+    atime = adjustl(atime)
+    ref = precision_code() == IMX_PREC_FP64_REF
+    call imx_check(imx_swm_get_state(handle, h, u), 'plotfields')
+    call plot_scalarfield(h, 0, "_h_t"//trim(atime))
+    if (ref) then
+      allocate(f(max(mdesc%nv, mdesc%ne, mdesc%nt)))
+      call imx_check(imx_swm_get_field(handle, IMX_FIELD_ETA, f), 'plotfields')
+      call plot_scalarfield(f(1:mdesc%nt), 1, "_eta_t"//trim(atime))
+      call imx_check(imx_swm_get_field(handle, IMX_FIELD_KE_HX, f), 'plotfields')
+      call plot_scalarfield(f(1:mdesc%nv), 0, "_Kenergy_t"//trim(atime))
+    end if
+    if (.not. allocated(qtr)) allocate(qtr(mdesc%nt))
+    call imx_check(imx_swm_get_field(handle, IMX_FIELD_Q_TR, qtr), 'plotfields')
+    call plot_scalarfield(qtr, 1, "_pv_t"//trim(atime))
+    if (ref) then
+      call imx_check(imx_swm_get_field(handle, IMX_FIELD_Q_ED, f), 'plotfields')
+      call plot_scalarfield(f(1:mdesc%ne), 6, "_pv_ed_t"//trim(atime))
+      call imx_check(imx_swm_get_field(handle, IMX_FIELD_DIVUH, f), 'plotfields')
+      call plot_scalarfield(f(1:mdesc%nv), 0, "_divuh_t"//trim(atime))
+    end if
+    if (refsol) then
+      call plot_scalarfield(h - h_exact, 0, "_h_error_t"//trim(atime))
+      call plot_scalarfield(u - u_exact, 6, "_u_error_t"//trim(atime))
+      call plot_scalarfield(qtr - q_exact, 1, "_q_tr_error_t"//trim(atime))
+    end if
+  end subroutine plotfields
+
+  ! plot_scalarfield (src/interpack.f90:7376-7514): 720 x 1440 float32
+  ! (lon, lat, value) stream in data/<swmname><suffix>_<grid>.dat
+  subroutine plot_scalarfield(f, pos, suffix)
+    real(r8), intent(in) :: f(:)
+    integer, intent(in) :: pos
+    character(len=*), intent(in) :: suffix
+    integer, parameter :: nlat = 720, nlon = 1440
+    real(c_float), allocatable :: buf(:)
+    real(r8), allocatable :: fc(:)
+    character(len=512) :: filename
+    integer :: iunit
+    allocate(buf(3*nlat*nlon), fc(size(f)))
+    fc = f
+    call imx_check(imx_remap_ll(mdesc, int(pos, c_int32_t), fc, nlon, nlat, buf), 'plot_scalarfield')
+    call imx_check(imx_make_dir('data'//c_null_char), 'plot_scalarfield')
+    filename = "data/"//trim(swmname)//trim(suffix)//"_"//trim(meshname)//".dat"
+    open(newunit=iunit, file=trim(filename), status='replace', access='stream', form='unformatted')
+    write(iunit) buf
+    close(iunit)
+  end subroutine plot_scalarfield
 
   ! error_calc (src/swm.f90:3492-3543) with sumfsq_areas weights w
   ! (:3442-3490): relative max and L2 errors of f against fref

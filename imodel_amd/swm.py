@@ -105,6 +105,15 @@ class Mesh:
             self._desc = capi.make_desc(self.arrays)
         return self._desc[0]
 
+    def remap_ll(self, f, pos=0, nlon=1440, nlat=720):
+        """plot_scalarfield's lat-lon sampling: (nlat*nlon, 3) float32 rows
+        (lon, lat, value), the stream the reference writes."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.zeros((nlat * nlon, 3), dtype=np.float32)
+        capi.check(capi.lib().imx_remap_ll(ctypes.byref(self.desc()), int(pos), capi.f64p(f), int(nlon), int(nlat),
+                                           capi.f32p(out)), "imx_remap_ll")
+        return out
+
     def tc2(self, erad=ERAD_REF):
         """Williamson TC2 state (initialize_fields case 2) on a library mesh."""
         if self._handle is None:

@@ -147,6 +147,13 @@ int  imx_mesh_write(const imx_mesh *m, const char *path);  /* IMXBIN records  */
    files so the reference's meshload can read a library mesh. */
 int  imx_mesh_load_ref(const char *dir, const char *name, imx_mesh **out);
 int  imx_mesh_write_ref(const imx_mesh *m, const char *dir, const char *name);
+/* plot_scalarfield's lat-lon sampling (src/interpack.f90:7376-7514):
+   field f at position pos (0 cells, 1 triangles, 2/6 primal edges, 3 dual
+   edges; the reference's nearest rules) on nlat x nlon cell-centred
+   points; out[3*nlat*nlon] = (lon, lat, value) float32, longitude fastest,
+   the stream the reference writes to data/<name>_<grid>.dat. */
+int  imx_remap_ll(const imx_mesh_desc *d, int32_t pos, const double *f, int32_t nlon,
+                  int32_t nlat, float *out);
 /* mkdir -p for the hosts' output folders (grid/, data/): no counterpart in
    the reference, whose runs expect the folders to exist. */
 int  imx_make_dir(const char *path);

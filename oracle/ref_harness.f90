@@ -333,9 +333,12 @@ program ref_harness
   use swm, only: tendency, ode_rk4, initialize_gridprop, initialize_fields, &
        calc_energies, sumf_areas, sumfsq_areas
   use omp_lib
+  use interpack, only: plot_scalarfield
+  use datastruct, only: scalar_field
   implicit none
 
   character(len=512) :: cmd, a1, a2, a3, meshsrc, statesrc, outf
+  type(scalar_field) :: pf
   character(len=128) :: hdr
   integer :: glevel, nsteps, idiag, k, ios, pc1, pc2
   real(r8) :: dtin, time, wall0, wall1
@@ -361,6 +364,21 @@ program ref_harness
     call bin_open_w(trim(outf))
     call dump_mesh(mesh, .true.)
     call bin_close()
+
+  case ('plot')
+    ! ref_harness plot <glevel> <field.bin> <pos>: plot_scalarfield of the
+    ! record 'f' (reference numbering) on the reference's glevel/eqs mesh,
+    ! into data/plottest_<mesh name>.dat
+    call get_command_argument(2, a1); read(a1, *) glevel
+    call get_command_argument(3, a2)
+    call get_command_argument(4, a3); read(a3, *) pc1
+    call build_ref_mesh(glevel, 'eqs')
+    call bin_read(trim(a2))
+    pf%f = getr('f')
+    pf%n = size(pf%f)
+    pf%pos = pc1
+    pf%name = 'plottest'
+    call plot_scalarfield(pf, mesh)
 
   case ('loadref')
     ! ref_harness loadref <name> <out.bin>: meshload of grid/<name>_*.dat
@@ -454,6 +472,7 @@ program ref_harness
     print*, 'usage: ref_harness build <glevel> <eqs|pol> <out.bin>'
     print*, '       ref_harness run <build:G:pos|mesh.bin> <tc2|state.bin> <nsteps> <dt> <diag> <out.bin>'
     print*, '       ref_harness loadref <name> <out.bin>   (reads grid/<name>_*.dat)'
+    print*, '       ref_harness plot <glevel> <field.bin> <pos>   (data/plottest_<grid>.dat)'
     stop 1
   end select
 

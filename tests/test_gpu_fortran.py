@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _par(td, nv, testcase, dt, days, nprints, precision, dump=1, loadable=0):
+def _par(td, nv, testcase, dt, days, nprints, precision, dump=1, loadable=0, nplots=0):
     os.makedirs(os.path.join(td, "par"), exist_ok=True)
     src = os.path.join(ROOT, "par")
     mesh = open(os.path.join(src, "mesh.par")).read().splitlines()
@@ -33,7 +33,7 @@ def _par(td, nv, testcase, dt, days, nprints, precision, dump=1, loadable=0):
     swm[2] = f"{testcase} 0"
     swm[4] = f"{days} {days}"
     swm[6] = f"{dt} 0 0"
-    swm[22] = f"0 {nprints} 0"
+    swm[22] = f"{nplots} {nprints} 0"
     open(os.path.join(td, "par", "swm.par"), "w").write("\n".join(swm) + "\n")
     gpu = open(os.path.join(src, "swm_gpu.par")).read().splitlines()
     gpu[1] = precision
@@ -153,3 +153,30 @@ def test_fortran_driver_stores_then_loads_grid():
         assert len([f for f in os.listdir(os.path.join(td, "grid")) if f.endswith(".dat")]) == 11
     assert "loading grid/" not in runs[0][0] and "loading grid/icos_eqs_nopt_3" in runs[1][0]
     assert np.array_equal(runs[0][1]["h"], runs[1][1]["h"]) and np.array_equal(runs[0][1]["u"], runs[1][1]["u"])
+
+
+def test_fortran_driver_plot_files():
+    """nplots = 2: plotfields writes the reference's lat-lon streams at t = 0,
+    12 h and 24 h; h and pv at the end equal the lat-lon sampling of the
+    oracle's final state and last-stage q_tr."""
+    print("[gpu-case] fortran imodel g3 tc2 plots", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 24, "fp64_ref", nplots=2)
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        base = "swm_tc2_dt900_HTC_trsk10_areageo"
+        names = set(os.listdir(os.path.join(td, "data")))
+        for t in (0, 43200, 86400):
+            for f in ("h", "eta", "Kenergy", "pv", "pv_ed", "divuh", "h_error", "u_error", "q_tr_error"):
+                n = f"{base}_{f}_t{t}_icos_eqs_nopt_3.dat"
+                assert n in names, n
+                assert os.path.getsize(os.path.join(td, "data", n)) == 720 * 1440 * 3 * 4
+        hp = np.fromfile(os.path.join(td, "data", f"{base}_h_t86400_icos_eqs_nopt_3.dat"), np.float32).reshape(-1, 3)
+        qp = np.fromfile(os.path.join(td, "data", f"{base}_pv_t86400_icos_eqs_nopt_3.dat"), np.float32).reshape(-1, 3)
+    m = Mesh.icos(3)
+    h0, u0 = m.tc2()
+    o = Oracle(m.arrays)
+    h, u = o.step(h0, u0, 900.0, 96)
+    assert np.array_equal(hp, m.remap_ll(h, 0))
+    assert np.array_equal(qp, m.remap_ll(o.field("q_tr"), 1))
