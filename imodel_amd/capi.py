@@ -103,6 +103,9 @@ def lib():
             "imx_mesh_load_ref": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]),
             "imx_mesh_write_ref": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p]),
             "imx_make_dir": (ctypes.c_int, [ctypes.c_char_p]),
+            "imx_swm_jacobian": (ctypes.c_int, [vp, ctypes.POINTER(MeshDesc), _F64P, _F64P, ctypes.c_double,
+                                                ctypes.c_int64, _I32P, _I32P, _F64P,
+                                                ctypes.POINTER(ctypes.c_int64), _I32P]),
             "imx_remap_ll": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_int32, _F64P, ctypes.c_int32,
                                             ctypes.c_int32, _F32P]),
             "imx_mesh_destroy": (None, [vp]),

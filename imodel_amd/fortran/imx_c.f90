@@ -70,6 +70,20 @@ module imx_c
       real(c_float) :: out(*)
       integer(c_int) :: rc
     end function
+    function imx_swm_jacobian(s, d, h, u, pert, cap, row, col, val, nnz, ncolors) &
+        bind(C, name='imx_swm_jacobian') result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr, imx_mesh_desc
+      type(c_ptr), value :: s
+      type(imx_mesh_desc) :: d
+      real(c_double) :: h(*), u(*)
+      real(c_double), value :: pert
+      integer(c_int64_t), value :: cap
+      integer(c_int32_t) :: row(*), col(*)
+      real(c_double) :: val(*)
+      integer(c_int64_t) :: nnz
+      integer(c_int32_t) :: ncolors
+      integer(c_int) :: rc
+    end function
     function imx_make_dir(path) bind(C, name='imx_make_dir') result(rc)
       import :: c_int, c_char
       character(kind=c_char) :: path(*)

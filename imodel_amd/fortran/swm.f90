@@ -73,6 +73,12 @@ contains
     o%dt = dt
     call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
     call imx_check(imx_swm_set_state(handle, h, u), 'imx_swm_set_state')
+    if (normal_modes == 1) then
+      call swm_normal_mode_analysis()
+      call imx_swm_destroy(handle)
+      handle = c_null_ptr
+      return
+    end if
     call imx_check(imx_swm_diagnostics(handle, d0), 'imx_swm_diagnostics')
     refsol = (testcase == 2)
     if (refsol) then
@@ -125,6 +131,57 @@ contains
     call imx_swm_destroy(handle)
     handle = c_null_ptr
   end subroutine swm_tests
+
+  ! swm_normal_mode_analysis (src/swm.f90:738-976): the linearised operator
+  ! of the initial state by one-sided differences with pert = mean cell
+  ! area, written as the reference does: dense rows for n < 100, else the
+  ! (i, j, M(i,j)) triplets with |M| > epsilon(1.)/100. The library
+  ! evaluates one tendency per colour of columns with disjoint stencils;
+  ! every entry equals the one-column difference bit for bit.
+  subroutine swm_normal_mode_analysis()
+    real(r8), pointer :: areag(:)
+    integer(c_int32_t), allocatable :: row(:), col(:)
+    real(r8), allocatable :: val(:), dense(:, :)
+    integer(c_int64_t) :: nnz
+    integer(c_int32_t) :: ncolors
+    real(r8) :: pert
+    integer :: n, k, iunit
+    character(len=512) :: filename
+    character(len=100) :: fmt, atmp
+    call c_f_pointer(mdesc%hx_areag, areag, [mdesc%nv])
+    pert = sum(areag)/mdesc%nv            ! mesh%meanhxarea
+    n = mdesc%nv + mdesc%ne
+    print*, "Generating the linearised operator (", n, " unknowns)"
+    call imx_check(imx_swm_jacobian(handle, mdesc, h, u, pert, 0_c_int64_t, [0_c_int32_t], [0_c_int32_t], &
+                                    [0._r8], nnz, ncolors), 'swm_normal_mode_analysis')
+    allocate(row(nnz), col(nnz), val(nnz))
+    call imx_check(imx_swm_jacobian(handle, mdesc, h, u, pert, nnz, row, col, val, nnz, ncolors), &
+                   'swm_normal_mode_analysis')
+    print*, "Tendency evaluations:", ncolors + 1, " (one per unknown in the reference:", n, ")"
+    call imx_check(imx_make_dir('data'//c_null_char), 'swm_normal_mode_analysis')
+    if (n < 100) then
+      filename = "data/"//trim(swmname)//"_linearmatrix_"//trim(meshname)//".txt"
+      allocate(dense(n, n))
+      dense = 0._r8
+      do k = 1, int(nnz)
+        dense(row(k), col(k)) = val(k)
+      end do
+      write(atmp, '(i10)') n
+      fmt = '('//trim(adjustl(atmp))//'f24.8)'
+      open(newunit=iunit, file=trim(filename), status='replace')
+      do k = 1, n
+        write(iunit, fmt) dense(k, 1:n)
+      end do
+    else
+      filename = "data/"//trim(swmname)//"_linearmatrixsparse_"//trim(meshname)//".txt"
+      open(newunit=iunit, file=trim(filename), status='replace')
+      do k = 1, int(nnz)
+        if (abs(val(k)) > epsilon(1.)/100._r8) write(iunit, '(2i10, es26.16e3)') row(k), col(k), val(k)
+      end do
+    end if
+    close(iunit)
+    print*, "Matrix printed as file: ", trim(filename)
+  end subroutine swm_normal_mode_analysis
 
   ! subroutine tendency(h, u, masseq, momeq)  (src/swm.f90:2416)
   subroutine tendency(hh, uu, masseq, momeq)

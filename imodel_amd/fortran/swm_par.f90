@@ -18,6 +18,7 @@ module swm_par
   character(len=128) :: mname = ''
   ! swm.par
   integer :: testcase = 2, test_lterror = 0, ntime = 0, adjustntime = 0
+  integer :: normal_modes = 0
   integer :: nplots = 0, nprints = 1000000, iploterrors = 0, inopv = 0
   real(r8) :: period = 1._r8, maxtime = 1._r8, dt = 0._r8, gasscoef = 1._r8
   real(r8) :: k2_max = 0._r8, k4_max = 0._r8, hollgw = 2._r8, trskindmax = 0._r8, pvspar = 0.5_r8
@@ -179,6 +180,9 @@ contains
       read(fu, *, iostat=ios) use_sfc, use_graph
       read(fu, *, iostat=ios) buffer
       read(fu, *, iostat=ios) dump_state
+      read(fu, *, iostat=ios) buffer
+      read(fu, *, iostat=ios) normal_modes
+      if (ios /= 0) normal_modes = 0
       close(fu)
     end if
   end subroutine read_swm_par

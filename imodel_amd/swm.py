@@ -208,6 +208,31 @@ class Swm:
                                                capi.f64p(f)), "imx_swm_tendency")
         return m, f
 
+    def jacobian(self, h, u, pert=None):
+        """Linearised operator (swm_normal_mode_analysis): scipy CSR matrix
+        M[r, i] = (T(x + pert e_i) - T(x)) / pert, unknowns (h, u); pert
+        defaults to the mean cell area on the unit sphere (the reference's
+        mesh%meanhxarea). Returns (M, number of tendency evaluations)."""
+        from scipy.sparse import csr_matrix
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if pert is None:
+            pert = float(np.mean(self.mesh.arrays["hx_areag"]))
+        nnz = ctypes.c_int64()
+        ncol = np.zeros(1, dtype=np.int32)
+        L = capi.lib()
+        d = ctypes.byref(self.mesh.desc())
+        capi.check(L.imx_swm_jacobian(self._h, d, capi.f64p(h), capi.f64p(u), pert, 0, None, None, None,
+                                      ctypes.byref(nnz), capi.i32p(ncol)), "imx_swm_jacobian")
+        r = np.zeros(nnz.value, dtype=np.int32)
+        c = np.zeros(nnz.value, dtype=np.int32)
+        v = np.zeros(nnz.value)
+        capi.check(L.imx_swm_jacobian(self._h, d, capi.f64p(h), capi.f64p(u), pert, nnz.value, capi.i32p(r),
+                                      capi.i32p(c), capi.f64p(v), ctypes.byref(nnz), capi.i32p(ncol)),
+                   "imx_swm_jacobian")
+        n = self.nv + self.ne
+        return csr_matrix((v, (r - 1, c - 1)), shape=(n, n)), int(ncol[0]) + 1
+
     def field(self, name):
         n = {"v": self.nv, "e": self.ne, "t": self.nt}[capi.FIELD_POS[name]]
         out = np.zeros(n)

@@ -191,6 +191,17 @@ int  imx_swm_tendency(imx_swm *s, const double *h, const double *u,
    RK stage, the module globals the reference leaves behind). FP64_REF keeps
    every IMX_FIELD_*; fused handles keep only IMX_FIELD_Q_TR. */
 int  imx_swm_get_field(imx_swm *s, int32_t which, double *out);
+/* Linearised operator for normal-mode analysis (swm_normal_mode_analysis,
+   src/swm.f90:738-976): M(r, i) = (T(x + pert e_i) - T(x)) / pert over the
+   nv + ne unknowns (h then u; rows masseq then momeq), as 1-based COO
+   triplets sorted by row then column, exact zeros dropped. Columns with
+   disjoint stencils share one tendency evaluation (column colouring), which
+   leaves every entry bit-identical to the one-column-at-a-time difference.
+   With cap smaller than the entry count nothing is written to row, col
+   and val, and *nnz (and *ncolors) report the count: a size query. */
+int  imx_swm_jacobian(imx_swm *s, const imx_mesh_desc *d, const double *h, const double *u,
+                      double pert, int64_t cap, int32_t *row, int32_t *col, double *val,
+                      int64_t *nnz, int32_t *ncolors);
 /* nsteps x ode_rk4 on the resident state            src/swm.f90:2600-2683 */
 int  imx_swm_step(imx_swm *s, int32_t nsteps);
 /* as imx_swm_step, returns device time (HIP events on the handle's stream)  */
