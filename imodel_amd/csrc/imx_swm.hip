@@ -290,7 +290,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     int remap, int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
     const int32_t *__restrict__ c_bitsB, const T *__restrict__ c_R,
     const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
-    const double *__restrict__ e_dl, const T *__restrict__ e_kw, const T *__restrict__ q,
+    const double *__restrict__ c_dl, const T *__restrict__ c_kw, const T *__restrict__ q,
     const T *__restrict__ hs, const T *__restrict__ us, double href, T grav, T inv_a,
     T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
@@ -323,8 +323,8 @@ __global__ __launch_bounds__(BS) void fast_cell(
     ue[j] = us[ed[j]];
     hn[j] = hs[nb[j]];
     qt[j] = q[tr[j]];
-    dl[j] = e_dl[ed[j]];
-    kw[j] = e_kw[ed[j]];
+    dl[j] = c_dl[j * nv + c];
+    kw[j] = c_kw[j * nv + c];
   }
   double dv = 0.0, dvc = 0.0;
   T ke = (T)0;
@@ -643,7 +643,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
 #define IMX_CELL(B)                                                                           \
   fast_cell<T, B><<<grid_bs(nv, B), B, 0, st>>>(                                             \
       s->kremap[1], nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_R, s->x_c_iaa, \
-      (const T *)s->x_c_ika, s->x_e_dl, (const T *)s->x_e_kw, (const T *)s->q, hs, us, s->href,  \
+      (const T *)s->x_c_ika, s->x_c_dl, (const T *)s->x_c_kw, (const T *)s->q, hs, us, s->href,  \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
   IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
@@ -705,6 +705,9 @@ static int build_fast(imx_swm *s, const imx_mesh_desc &d, const double *trhx, co
   const double a = s->o.erad;
   std::vector<int32_t> c_nb((size_t)MX * nv, 0), bitsA(nv, 0), bitsB(nv, 0);
   std::vector<double> c_R((size_t)MX * nv, 0.0), c_iaa(nv), c_ika(nv);
+  // per-slot dual length and K weight of each cell's edges, slot-major so a
+  // cell reads them with coalesced loads instead of gathering them by edge
+  std::vector<double> c_dl((size_t)MX * nv, 0.0), c_kw((size_t)MX * nv, 0.0);
   for (int co = 0; co < nv; ++co) {
     const int c = cn[co];
     const int n = d.v_nnb[co];
@@ -719,6 +722,8 @@ static int build_fast(imx_swm *s, const imx_mesh_desc &d, const double *trhx, co
       const int eo = d.v_ed[MX * co + js] - 1;
       const int ei = en[eo];
       const int owner = ei / 3;
+      c_dl[(size_t)j * nv + c] = d.edhx_leng[eo];
+      c_kw[(size_t)j * nv + c] = 0.25 * d.ed_leng[eo] * d.edhx_leng[eo];
       if (owner != c && owner != nbi) return set_error(IMX_EINVAL, "layout: edge owner is not an end cell");
       if (owner == c) A |= 1 << (BA_OWN + j);
       B |= (ei % 3) << (2 * j);
@@ -742,15 +747,13 @@ static int build_fast(imx_swm *s, const imx_mesh_desc &d, const double *trhx, co
     c_ika[c] = 1.0 / ksum;
   }
   std::vector<int32_t> e_oth(NE, -1);
-  std::vector<double> e_dl(NE, 0.0), e_kw(NE, 0.0), e_il(NE, 0.0);
+  std::vector<double> e_il(NE, 0.0);
   for (int eo = 0; eo < d.ne; ++eo) {
     const int e = en[eo];
     const int o = e / 3;
     const int v1 = cn[d.ed_v[2 * eo] - 1], v2 = cn[d.ed_v[2 * eo + 1] - 1];
     if (o == v1) e_oth[e] = v2;
     else e_oth[e] = v1 | (1 << OTH_V2);
-    e_dl[e] = d.edhx_leng[eo];
-    e_kw[e] = 0.25 * d.ed_leng[eo] * d.edhx_leng[eo];
     e_il[e] = 1.0 / d.ed_leng[eo];
   }
   std::vector<int32_t> t_ab((size_t)2 * NT, -1), t_eb(NT, 0);
@@ -790,8 +793,8 @@ static int build_fast(imx_swm *s, const imx_mesh_desc &d, const double *trhx, co
   s->x_c_iaa = A.put(c_iaa);
   s->x_c_ika = A.put(cvt<T>(c_ika));
   s->x_e_oth = A.put(e_oth);
-  s->x_e_dl = A.put(e_dl);
-  s->x_e_kw = A.put(cvt<T>(e_kw));
+  s->x_c_dl = A.put(c_dl);
+  s->x_c_kw = A.put(cvt<T>(c_kw));
   s->x_e_il = A.put(cvt<T>(e_il));
   s->x_t_ab = A.put(t_ab);
   s->x_t_eb = A.put(t_eb);
