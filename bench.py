@@ -368,7 +368,7 @@ def bench_mlp(args):
         "roofline": {"bound": "mfma", "kernel": "mlp_fwd_lds (48-128-128-128-2, bf16 MFMA)",
                      "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0,
                      "traffic": None, "launch_ms": fwd_ms, "flop_per_edge": MLP_FLOP_PER_EDGE},
-        "corrected_step_us": learned_us, "plain_step_us": plain_us, "fallbacks": nfb,
+        "corrected_step_us": learned_us, "plain_step_us": plain_us, "limited": nfb,
     }
     print(json.dumps(out), flush=True)
 
@@ -380,7 +380,7 @@ def bench_learned(args):
     coarse steps (192 = 1 day). Training: --train-seeds runs x
     --train-samples one-step samples produced by this solver, stored on the
     device, --fit-steps shuffled AdamW steps. Reports errors, wall times per
-    simulated interval, fallbacks and the MLP kernel's MFMA throughput."""
+    simulated interval, limited steps and the MLP kernel's MFMA throughput."""
     from imodel_amd.learn import Trainer, evaluate, restriction
     from imodel_amd.swm import H0_GEO, Mesh, Swm
 
@@ -420,7 +420,7 @@ def bench_learned(args):
                    for k in ("plain", "learned", "mid")},
         "learned_vs_mid_error": {"h": mean("learned", "h") / mean("mid", "h"),
                                  "u": mean("learned", "u") / mean("mid", "u")},
-        "fallbacks": [r["learned"]["fallbacks"] for r in res],
+        "limited": [r["learned"]["limited"] for r in res],
         "mass_drift_learned": [r["learned"]["mass_drift"] for r in res],
         "fit_loss_last": fit[-1].tolist(),
         "roofline": {"bound": "mfma", "kernel": "mlp_fwd (48-128-128-128-2, bf16 MFMA)",

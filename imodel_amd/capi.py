@@ -145,6 +145,7 @@ def lib():
             "imx_mlp_nparams": (ctypes.c_int, []),
             "imx_mlp_get_params": (ctypes.c_int, [vp, _F32P]),
             "imx_mlp_set_params": (ctypes.c_int, [vp, _F32P]),
+            "imx_mlp_init": (ctypes.c_int, [vp, ctypes.c_uint64]),
             "imx_mlp_forward": (ctypes.c_int, [vp, ctypes.c_int64, _F32P, _F32P]),
             "imx_mlp_train": (ctypes.c_int, [vp, ctypes.c_int64, _F32P, _F32P, _F32P, ctypes.c_float,
                                              ctypes.c_float, ctypes.c_int32, _F64P, _F32P, ctypes.c_int32]),

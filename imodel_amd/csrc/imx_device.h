@@ -111,7 +111,8 @@ struct imx_swm {
 // copies of the weights, and the scratch of the training pass.
 struct imx_mlp {
   float *P = nullptr, *M = nullptr, *V = nullptr, *G = nullptr, *part = nullptr, *lossp = nullptr;
-  short *Ff = nullptr, *Fb = nullptr;
+  short *Ff = nullptr, *Fb = nullptr;  // K16 fragments of the training kernels
+  short *Ff32 = nullptr;                // K32 fragments of the inference kernel
   short *X = nullptr, *AT = nullptr, *DZT = nullptr;
   float *T = nullptr, *w = nullptr, *Y = nullptr, *Xf = nullptr;
   int64_t cap = 0;  // edges the scratch holds

@@ -60,7 +60,7 @@ struct imx_learn {
   double *rms = nullptr;    // [1] correction RMS of the last step
   double *bound = nullptr;  // [1]
   double *lh = nullptr;     // [1] h loss of the last training epoch
-  int *nfb = nullptr;       // [1] fallback steps since the counter was cleared
+  int *nfb = nullptr;       // [1] limited steps since the counter was cleared
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   const short *gX = nullptr;  // MLP input buffer the graph was captured with
@@ -265,16 +265,19 @@ __global__ __launch_bounds__(LTPB) void learn_apply(int nv, int NE, const float 
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
+  // limiter: a correction whose RMS exceeds the bound is scaled down to
+  // it; a non-finite one (or bound 0) leaves the plain step untouched
   const double r = sqrt(red[0]);
   const bool ok = r <= bound[0];
+  const float sc = ok ? 1.f : (r < 1e300 ? (float)(bound[0] / r) : 0.f);
   const int i = blockIdx.x * LTPB + threadIdx.x;
   if (i == 0) {
     rms[0] = r;
     if (!ok) nfb[0] = nfb[0] + 1;  // one thread of the grid counts
   }
-  if (!ok) return;
-  if (i < NE && wv[i] > 0.f) u[i] = (T)((float)u[i] + Y[2 * i] * inv_tsu);
-  if (i < nv) h[i] = (T)((float)h[i] + dhc[i] * inv_tsh);
+  if (!(sc > 0.f)) return;
+  if (i < NE && wv[i] > 0.f) u[i] = (T)((float)u[i] + sc * (Y[2 * i] * inv_tsu));
+  if (i < nv) h[i] = (T)((float)h[i] + sc * (dhc[i] * inv_tsh));
 }
 
 namespace imx {
@@ -586,9 +589,10 @@ int imx_learn_train(imx_learn *L, const double *du, const double *dh, int32_t ep
   return IMX_OK;
 }
 
-// nsteps corrected steps: plain RK4, then the MLP correction unless its RMS
-// exceeds `bound` (scaled units); *nfallback counts the plain-only steps.
-int imx_learn_step(imx_learn *L, int32_t nsteps, double bound, int32_t *nfallback) {
+// nsteps corrected steps: plain RK4, then the MLP correction, scaled down
+// to RMS `bound` (scaled units) where it exceeds it; *nlimited counts the
+// scaled steps.
+int imx_learn_step(imx_learn *L, int32_t nsteps, double bound, int32_t *nlimited) {
   if (!L || nsteps < 0) return imx::set_error(IMX_EINVAL, "imx_learn_step: bad argument");
   imx_swm *s = L->s;
   int rc = imx::learn_ensure_graph(L);
@@ -599,7 +603,7 @@ int imx_learn_step(imx_learn *L, int32_t nsteps, double bound, int32_t *nfallbac
   int nfb = 0;
   LTRY(hipMemcpyAsync(&nfb, L->nfb, 4, hipMemcpyDeviceToHost, s->stream), "learn step");
   rc = imx::swm_check_flag(s, "imx_learn_step");
-  if (nfallback) *nfallback = nfb;
+  if (nlimited) *nlimited = nfb;
   return rc;
 }
 

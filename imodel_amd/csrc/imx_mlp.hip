@@ -265,9 +265,42 @@ __global__ __launch_bounds__(256) void mlp_adamw(int nsplit, const float *__rest
 }
 
 // fragment copies from the FP32 master (forward: W^T tiles; backward: W tiles)
+// k-permutation of a K32 chunk: operand slot i of lane group g holds
+// feature kappa(g, i) of the chunk (the inference kernel's B operand is the
+// concatenation of two 16-feature activation tiles)
+__device__ __forceinline__ int kappa32(int g, int i) { return i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4); }
+
 __global__ __launch_bounds__(256) void mlp_pack(const float *__restrict__ P, short *__restrict__ Ff,
-                                                short *__restrict__ Fb) {
+                                                short *__restrict__ Fb, short *__restrict__ Ff32) {
   const int q = blockIdx.x * 256 + threadIdx.x;
+  // inference (K32) fragments: index -> (layer, output tile, chunk, lane, i)
+  if (q < FF_N) {
+    int row, col, N, poff;
+    if (q < FF_2) {  // layer 1: per output tile a K32 fragment then a K16 one
+      const int nn = q / 768, s = q % 768;
+      N = MLP_H;
+      poff = P_W1;
+      if (s < 512) {
+        const int lane = s / 8, i = s % 8;
+        row = kappa32(lane / 16, i);
+        col = nn * 16 + lane % 16;
+      } else {
+        const int lane = (s - 512) / 4, i = (s - 512) % 4;
+        row = 32 + 4 * (lane / 16) + i;
+        col = nn * 16 + lane % 16;
+      }
+    } else {
+      int base;
+      if (q < FF_3) { base = FF_2; N = MLP_H; poff = P_W2; }
+      else if (q < FF_4) { base = FF_3; N = MLP_H; poff = P_W3; }
+      else { base = FF_4; N = MLP_OUT; poff = P_W4; }
+      const int r = q - base;
+      const int nn = r / 2048, c = (r / 512) % 4, lane = (r / 8) % 64, i = r % 8;
+      row = 32 * c + kappa32(lane / 16, i);
+      col = nn * 16 + lane % 16;
+    }
+    Ff32[q] = (col < N) ? f2bf(P[poff + row * N + col]) : (short)0;
+  }
   // forward: index -> (layer, n, kc, lane, i)
   if (q < FF_N) {
     int base, K, N, poff, KT;
@@ -344,9 +377,33 @@ int mlp_reserve(imx_mlp *m, int64_t n) {
   return IMX_OK;
 }
 
+// He-normal init of the matrices, zero biases (SplitMix64 + Box-Muller)
+void mlp_initial_params(uint64_t seed, std::vector<float> &P) {
+  P.assign(P_N, 0.f);
+  uint64_t s = seed;
+  auto next = [&]() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  auto normal = [&]() {
+    const double u1 = ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  };
+  auto init = [&](int off, int fan_in, int count, double gain) {
+    for (int i = 0; i < count; ++i) P[off + i] = (float)(normal() * gain * std::sqrt(2.0 / fan_in));
+  };
+  init(P_W1, MLP_IN, MLP_IN * MLP_H, 1.0);
+  init(P_W2, MLP_H, MLP_H * MLP_H, 1.0);
+  init(P_W3, MLP_H, MLP_H * MLP_H, 1.0);
+  init(P_W4, MLP_H, MLP_H * MLP_OUT, 0.1);  // small output layer: starts near "no correction"
+}
+
 int mlp_pack_launch(imx_mlp *m) {
   const int n = FF_N > FB_N ? FF_N : FB_N;
-  mlp_pack<<<(n + 255) / 256, 256, 0, m->stream>>>(m->P, m->Ff, m->Fb);
+  mlp_pack<<<(n + 255) / 256, 256, 0, m->stream>>>(m->P, m->Ff, m->Fb, m->Ff32);
   MTRY(hipGetLastError(), "mlp_pack");
   return IMX_OK;
 }
@@ -354,7 +411,7 @@ int mlp_pack_launch(imx_mlp *m) {
 // Forward on device buffers (X bf16 [n][48]) -> Y fp32 [n][2].
 int mlp_forward_dev(imx_mlp *m, int64_t n, const short *X, float *Y, hipStream_t st) {
   if (n <= 0) return IMX_OK;
-  MTRY(mlp_fwd_launch(m->Ff, m->P, n, RowLoader{X}, Y, st), "mlp_fwd_lds");
+  MTRY(mlp_fwd_launch(m->Ff32, m->P, n, RowLoader{X}, Y, st), "mlp_fwd_lds");
   return IMX_OK;
 }
 
@@ -405,33 +462,15 @@ int imx_mlp_create(uint64_t seed, imx_mlp **out) {
   *out = nullptr;
   imx_mlp *m = new (std::nothrow) imx_mlp();
   if (!m) return imx::set_error(IMX_ENOMEM, "imx_mlp_create: out of memory");
-  // He-normal init of the matrices, zero biases (SplitMix64 + Box-Muller)
-  std::vector<float> P(P_N, 0.f);
-  uint64_t s = seed;
-  auto next = [&]() {
-    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-  };
-  auto normal = [&]() {
-    const double u1 = ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-    const double u2 = ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-  };
-  auto init = [&](int off, int fan_in, int count, double gain) {
-    for (int i = 0; i < count; ++i) P[off + i] = (float)(normal() * gain * std::sqrt(2.0 / fan_in));
-  };
-  init(P_W1, MLP_IN, MLP_IN * MLP_H, 1.0);
-  init(P_W2, MLP_H, MLP_H * MLP_H, 1.0);
-  init(P_W3, MLP_H, MLP_H * MLP_H, 1.0);
-  init(P_W4, MLP_H, MLP_H * MLP_OUT, 0.1);  // small output layer: starts near "no correction"
+  std::vector<float> P;
+  imx::mlp_initial_params(seed, P);
   int rc = IMX_OK;
   if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&m->P, P_N * 4) != hipSuccess || hipMalloc(&m->M, P_N * 4) != hipSuccess ||
       hipMalloc(&m->V, P_N * 4) != hipSuccess || hipMalloc(&m->G, P_N * 4) != hipSuccess ||
       hipMalloc(&m->part, (size_t)m->nsplit * P_N * 4) != hipSuccess ||
-      hipMalloc(&m->Ff, FF_N * 2) != hipSuccess || hipMalloc(&m->Fb, FB_N * 2) != hipSuccess) {
+      hipMalloc(&m->Ff, FF_N * 2) != hipSuccess || hipMalloc(&m->Fb, FB_N * 2) != hipSuccess ||
+      hipMalloc(&m->Ff32, FF_N * 2) != hipSuccess) {
     rc = imx::set_error(IMX_EDEVICE, "imx_mlp_create: device allocation failed");
   } else {
     (void)hipMemcpy(m->P, P.data(), P_N * 4, hipMemcpyHostToDevice);
@@ -466,6 +505,22 @@ int imx_mlp_set_params(imx_mlp *m, const float *p) {
   int rc = imx::mlp_pack_launch(m);
   if (rc) return rc;
   if (hipStreamSynchronize(m->stream) != hipSuccess) return imx::set_error(IMX_EDEVICE, "imx_mlp_set_params");
+  return IMX_OK;
+}
+
+int imx_mlp_init(imx_mlp *m, uint64_t seed) {
+  if (!m) return imx::set_error(IMX_EINVAL, "imx_mlp_init: NULL handle");
+  std::vector<float> P;
+  imx::mlp_initial_params(seed, P);
+  if (hipStreamSynchronize(m->stream) != hipSuccess ||
+      hipMemcpy(m->P, P.data(), P_N * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(m->M, 0, P_N * 4) != hipSuccess || hipMemset(m->V, 0, P_N * 4) != hipSuccess ||
+      hipMemset(m->G, 0, P_N * 4) != hipSuccess)
+    return imx::set_error(IMX_EDEVICE, "imx_mlp_init: device copy failed");
+  m->t = 0;
+  int rc = imx::mlp_pack_launch(m);
+  if (rc) return rc;
+  if (hipStreamSynchronize(m->stream) != hipSuccess) return imx::set_error(IMX_EDEVICE, "imx_mlp_init");
   return IMX_OK;
 }
 
@@ -535,7 +590,7 @@ void imx_mlp_destroy(imx_mlp *m) {
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   float *fs[] = {m->P, m->M, m->V, m->G, m->part, m->lossp, m->T, m->w, m->Y, m->Xf};
   for (float *p : fs) (void)hipFree(p);
-  short *ss[] = {m->Ff, m->Fb, m->X, m->AT, m->DZT};
+  short *ss[] = {m->Ff, m->Fb, m->Ff32, m->X, m->AT, m->DZT};
   for (short *p : ss) (void)hipFree(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;

@@ -171,10 +171,12 @@ class Learn:
         return float(loss[0]), float(loss[1])
 
     def step(self, nsteps, bound=1e300):
-        nfb = np.zeros(1, dtype=np.int32)
-        capi.check(capi.lib().imx_learn_step(self._h, int(nsteps), float(bound), capi.i32p(nfb)),
+        """nsteps corrected steps; corrections above RMS `bound` (scaled
+        units) are scaled down to it. Returns the number of scaled steps."""
+        nlim = np.zeros(1, dtype=np.int32)
+        capi.check(capi.lib().imx_learn_step(self._h, int(nsteps), float(bound), capi.i32p(nlim)),
                    "imx_learn_step")
-        return int(nfb[0])
+        return int(nlim[0])
 
     def last_rms(self):
         r = ctypes.c_double()
@@ -371,7 +373,7 @@ def evaluate(tr, seed, nsteps, spinup=0, bound=None, mid=None):
     nfb = [0]
     t = _sync_time(lambda: nfb.__setitem__(0, tr.learn.step(nsteps, bound)))
     hl, ul = tr.coarse.get_state()
-    out["learned"] = dict(errs(hl, ul), wall_s=t, fallbacks=nfb[0])
+    out["learned"] = dict(errs(hl, ul), wall_s=t, limited=nfb[0])
     M0 = np.sum(A * hc0)
     out["learned"]["mass_drift"] = float((np.sum(A * hl) - M0) / M0)
     out["plain"]["mass_drift"] = float((np.sum(A * hp) - M0) / M0)

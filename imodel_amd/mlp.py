@@ -40,6 +40,10 @@ class Mlp:
         p = np.ascontiguousarray(p, dtype=np.float32)
         capi.check(capi.lib().imx_mlp_set_params(self._h, capi.f32p(p)), "imx_mlp_set_params")
 
+    def init(self, seed):
+        """Parameters as Mlp(seed) starts, AdamW state cleared."""
+        capi.check(capi.lib().imx_mlp_init(self._h, int(seed)), "imx_mlp_init")
+
     def forward(self, X):
         X = np.ascontiguousarray(X, dtype=np.float32)
         Y = np.zeros((X.shape[0], OUT), dtype=np.float32)

@@ -247,7 +247,12 @@ typedef struct imx_mlp imx_mlp;
 int  imx_mlp_create(uint64_t seed, imx_mlp **out);
 int  imx_mlp_nparams(void);
 int  imx_mlp_get_params(imx_mlp *m, float *p);
+/* replaces the parameters; the AdamW moments are kept (a checkpoint
+   restore continues the same optimisation) */
 int  imx_mlp_set_params(imx_mlp *m, const float *p);
+/* draws the parameters imx_mlp_create(seed) starts from and clears the
+   AdamW state: a new fit without a new handle */
+int  imx_mlp_init(imx_mlp *m, uint64_t seed);
 /* host rows X[n][48] -> Y[n][2] */
 int  imx_mlp_forward(imx_mlp *m, int64_t n, const float *X, float *Y);
 /* L = sum_e w[e] |Y[e] - T[e]|^2: gradient into grad[nparams] (may be NULL),
@@ -272,9 +277,10 @@ void imx_mlp_destroy(imx_mlp *m);
    steps); the network's second output is an edge mass flux whose
    divergence is the h correction, so corrections conserve total mass.
    loss[2] receives the scaled u and h losses. imx_learn_step runs corrected
-   steps (one graph launch each): plain step + MLP correction, the plain
-   result kept when the correction's RMS (scaled units) exceeds `bound`;
-   *nfallback counts those steps. */
+   steps (one graph launch each): plain step + MLP correction; a
+   correction whose RMS (scaled units) exceeds `bound` is scaled down to
+   RMS `bound` (a non-finite one, or bound 0, keeps the plain step exactly);
+   *nlimited counts the scaled steps. */
 typedef struct imx_learn imx_learn;
 int  imx_learn_create(imx_swm *coarse, imx_mlp *m, const imx_mesh_desc *d, imx_learn **out);
 int  imx_learn_sample(imx_learn *l, float *raw);
@@ -292,7 +298,7 @@ int  imx_learn_reserve(imx_learn *l, int32_t cap);
 int  imx_learn_store(imx_learn *l, const double *du, const double *dh, int32_t *slot);
 int  imx_learn_fit(imx_learn *l, int32_t nsteps, float lr0, float lr1, float wd, uint64_t seed,
                    double loss[2]);
-int  imx_learn_step(imx_learn *l, int32_t nsteps, double bound, int32_t *nfallback);
+int  imx_learn_step(imx_learn *l, int32_t nsteps, double bound, int32_t *nlimited);
 int  imx_learn_last_rms(imx_learn *l, double *rms);
 void imx_learn_destroy(imx_learn *l);
 

@@ -106,10 +106,13 @@ def div_matrix(a):
 
 
 def correct(h, u, Y, D, tsu, tsh, bound):
-    """One correction: (h, u) + (D Y[:,1] / tsh, Y[:,0] / tsu) unless the
-    RMS sqrt(mean Y0^2 + mean (D Y1)^2) exceeds bound. Returns (h, u, rms)."""
+    """One correction: (h, u) + s (D Y[:,1] / tsh, Y[:,0] / tsu) with
+    s = 1 while the RMS sqrt(mean Y0^2 + mean (D Y1)^2) is within bound,
+    s = bound / RMS above it, and the plain (h, u) when s is 0 or the RMS
+    is not finite. Returns (h, u, rms)."""
     dhc = D @ Y[:, 1]
     rms = float(np.sqrt(np.mean(Y[:, 0] ** 2) + np.mean(dhc ** 2)))
-    if rms > bound:
+    s = 1.0 if rms <= bound else (bound / rms if np.isfinite(rms) else 0.0)
+    if not s > 0.0:
         return h, u, rms
-    return h + dhc / tsh, u + Y[:, 0] / tsu, rms
+    return h + s * (dhc / tsh), u + s * (Y[:, 0] / tsu), rms

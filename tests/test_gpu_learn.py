@@ -1,5 +1,5 @@
 """Learned coarse-grid correction on the GPU (imx_learn_*): features and the
-corrected step against the numpy restatement, the RMS fallback, mass
+corrected step against the numpy restatement, the RMS limiter, mass
 conservation of the flux-form h correction, and what a network trained
 with this solver's own finer run gains over the plain coarse step."""
 import numpy as np
@@ -68,8 +68,8 @@ def test_corrected_step_matches_oracle(g3):
     assert np.linalg.norm((hc - hp) - dh) < 2e-2 * np.linalg.norm(dh)
 
 
-def test_fallback_bound_zero_is_plain(g3):
-    say("learn fallback")
+def test_limiter_bound_zero_is_plain(g3):
+    say("learn limiter bound 0")
     s = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
     p = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
     L = Learn(s, Mlp(seed=2), g3)
@@ -82,6 +82,30 @@ def test_fallback_bound_zero_is_plain(g3):
     hp, up = p.get_state()
     assert np.array_equal(hs, hp) and np.array_equal(us, up)
     assert L.last_rms() > 0.0
+
+
+def test_limiter_scales_to_bound(g3):
+    """A bound of a quarter of the correction's RMS applies a quarter of it."""
+    say("learn limiter scale")
+    s = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
+    L = Learn(s, Mlp(seed=2), g3)
+    h0, u0 = g3.spectral(7)
+    s.set_state(h0, u0)
+    raw = L.sample(raw=True)
+    hp, up = s.get_state()
+    L.set_scaling(raw.mean(0), raw.std(0) + 1e-3, 50.0, 0.5)
+    s.set_state(h0, u0)
+    assert L.step(1, bound=1e30) == 0
+    r = L.last_rms()
+    h1, u1 = s.get_state()
+    s.set_state(h0, u0)
+    assert L.step(1, bound=0.25 * r) == 1
+    h2, u2 = s.get_state()
+    du1, du2 = u1.astype(np.float64) - up, u2.astype(np.float64) - up
+    dh1, dh2 = h1.astype(np.float64) - hp, h2.astype(np.float64) - hp
+    # FP32 state: the increments carry the rounding of u + du
+    assert np.linalg.norm(du2 - 0.25 * du1) < 1e-2 * np.linalg.norm(du1)
+    assert np.linalg.norm(dh2 - 0.25 * dh1) < 1e-2 * np.linalg.norm(dh1)
 
 
 def test_correction_conserves_mass(g3):
@@ -121,5 +145,5 @@ def test_learned_beats_plain():
     say(r)
     assert r["learned"]["h"] < 0.7 * r["plain"]["h"]
     assert r["learned"]["u"] < 0.8 * r["plain"]["u"]
-    assert 0 <= r["learned"]["fallbacks"] <= 48
+    assert 0 <= r["learned"]["limited"] <= 48
     assert abs(r["learned"]["mass_drift"]) < 1e-8

@@ -60,6 +60,38 @@ def test_training_reduces_loss():
     assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
 
 
+def test_init_restarts_optimisation():
+    """init(seed) gives Mlp(seed)'s parameters and a cleared AdamW state:
+    training from it repeats a fresh handle's training bit for bit."""
+    say("mlp init")
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((2048, 48)).astype(np.float32)
+    T = rng.standard_normal((2048, 2)).astype(np.float32)
+    a = Mlp(seed=4)
+    for _ in range(10):
+        a.train_step(X, T, lr=1e-3)
+    a.init(9)
+    b = Mlp(seed=9)
+    assert np.array_equal(a.params(), b.params())
+    la = [a.train_step(X, T, lr=1e-3)[0] for _ in range(5)]
+    lb = [b.train_step(X, T, lr=1e-3)[0] for _ in range(5)]
+    assert la == lb
+    assert np.array_equal(a.params(), b.params())
+
+
+def test_forward_large_grid_rows_unchanged():
+    """Above 32768 tiles the inference grid doubles; every row's result is
+    independent of the grid, so it equals the same rows evaluated alone."""
+    n = 32768 * 32 + 77
+    say("mlp forward large grid", n)
+    m = Mlp(seed=6)
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((n, 48)).astype(np.float32)
+    Y = m.forward(X)
+    for a in (0, n // 2 + 3, n - 1000):
+        np.testing.assert_array_equal(Y[a:a + 1000], m.forward(X[a:a + 1000]))
+
+
 def test_gradient_input_mask_matches_oracle():
     """gmask=2: T[:, 1] is dL/dy1 itself. The oracle gets the same gradient
     from the pseudo-target t1 = y1 - g / (2 w) of its squared loss."""

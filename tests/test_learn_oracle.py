@@ -102,7 +102,7 @@ def test_features_layout(g3):
     assert np.all(raw[:, :11][pad] == 0) and np.all(raw[:, 11:22][pad] == 0)
 
 
-def test_correct_fallback_keeps_plain():
+def test_correct_limiter():
     rng = np.random.default_rng(3)
     from scipy.sparse import identity
     h, u = rng.standard_normal(5), rng.standard_normal(7)
@@ -112,6 +112,15 @@ def test_correct_fallback_keeps_plain():
     assert h1 is h and u1 is u and r > 0
     h2, u2, _ = lo.correct(h, u, Y, D, 2.0, 4.0, 1e9)
     assert np.allclose(u2 - u, Y[:, 0] / 2.0) and np.allclose(h2 - h, Y[:5, 1] / 4.0)
+    # above the bound the correction is scaled to RMS = bound
+    h3, u3, _ = lo.correct(h, u, Y, D, 1.0, 1.0, 0.25 * r)
+    assert np.allclose(u3 - u, 0.25 * Y[:, 0]) and np.allclose(h3 - h, 0.25 * Y[:5, 1])
+    dh, du = h3 - h, u3 - u
+    assert np.isclose(np.sqrt(np.mean(du ** 2) + np.mean(dh ** 2)), 0.25 * r)
+    Yn = Y.copy()
+    Yn[0, 0] = np.nan
+    h4, u4, _ = lo.correct(h, u, Yn, D, 1.0, 1.0, 1.0)
+    assert h4 is h and u4 is u
 
 
 @pytest.fixture(scope="module")
