@@ -22,18 +22,26 @@ OMEGA = 7.292e-5          # constants.f90:67
 
 _F = ("h_ed", "uh", "eta", "h_tr", "q_tr", "q_ed", "eta_ed", "divuh", "ke_hx", "ghbk",
       "hbt", "grad_ghbk", "uhq_perp", "divu", "zeta", "v_hx", "massf", "momf", "h_new",
-      "u_new")
+      "u_new", "q_grad_ed", "q_grad_tr", "lapu", "lap_lapu", "cdiv", "cvort", "gdiv", "gvort",
+      "div_lap", "vort_lap")
 _F64P = ctypes.POINTER(ctypes.c_double)
+COEF_KIND = {"const": 0, "align": 1, "diam": 2}
+
+
+class _Scheme(ctypes.Structure):
+    _fields_ = [("apvm", ctypes.c_int), ("pvspar", ctypes.c_double), ("dt", ctypes.c_double),
+                ("diffusion", ctypes.c_int), ("hyperdiffusion", ctypes.c_int),
+                ("dif_hx", _F64P), ("dif_tr", _F64P)]
 
 
 class _Params(ctypes.Structure):
     _fields_ = [("erad", ctypes.c_double), ("grav", ctypes.c_double),
                 ("omega", ctypes.c_double), ("testcase", ctypes.c_int),
-                ("trhx_area", _F64P), ("trskw", _F64P)]
+                ("trhx_area", _F64P), ("trskw", _F64P), ("scheme", ctypes.POINTER(_Scheme))]
 
 
 class _Fields(ctypes.Structure):
-    _fields_ = [(n, _F64P) for n in _F]
+    _fields_ = [(n, _F64P) for n in _F] + [("perot", ctypes.c_int)]
 
 
 def build():
@@ -60,6 +68,8 @@ def lib():
                                  _F64P, _F64P, _F64P, ctypes.POINTER(_Fields)]
         L.oracle_diagnostics.argtypes = [ctypes.POINTER(capi.MeshDesc), ctypes.POINTER(_Params), _F64P,
                                          _F64P, _F64P, ctypes.POINTER(_Fields), _F64P]
+        L.oracle_diffusion_coef.argtypes = [ctypes.POINTER(capi.MeshDesc), _F64P, ctypes.c_double,
+                                            ctypes.c_int, ctypes.c_double, ctypes.c_int, _F64P, _F64P]
         _lib = L
     return _lib
 
@@ -68,10 +78,14 @@ class Oracle:
     """FP64 reference-order CPU model on one mesh (dict of IMXBIN arrays).
 
     The mesh dict must hold the gridprop arrays (tr_trhx_area, hx_trskw).
+    scheme: None (trsk10/HTC defaults) or a dict with any of
+      pv ("none" | "apvm"), pvspar, dt (the time step the APVM term reads),
+      k2, diffus, k4, hyperdiffus ("const" | "align" | "diam"),
+    the entries of par/swm.par.
     """
 
     def __init__(self, mesh, erad=ERAD_REF, grav=GRAV, omega=OMEGA, testcase=2,
-                 perot=False):
+                 perot=False, scheme=None):
         self.mesh = mesh
         self.desc, self._keep = capi.make_desc(mesh)
         self.nv, self.ne, self.nt = self.desc.nv, self.desc.ne, self.desc.nt
@@ -79,6 +93,20 @@ class Oracle:
         self._trskw = np.ascontiguousarray(mesh["hx_trskw"], dtype=np.float64)
         self.p = _Params(erad=erad, grav=grav, omega=omega, testcase=testcase,
                          trhx_area=capi.f64p(self._trhx), trskw=capi.f64p(self._trskw))
+        if scheme:
+            sc = dict(scheme)
+            k2, k4 = float(sc.get("k2", 0.0)), float(sc.get("k4", 0.0))
+            self.dif_hx = np.zeros(self.nv)
+            self.dif_tr = np.zeros(self.nt)
+            lib().oracle_diffusion_coef(ctypes.byref(self.desc), capi.f64p(self._trhx), k2,
+                                        COEF_KIND[sc.get("diffus", "const")], k4,
+                                        COEF_KIND[sc.get("hyperdiffus", "const")],
+                                        capi.f64p(self.dif_hx), capi.f64p(self.dif_tr))
+            self._sc = _Scheme(apvm=int(sc.get("pv", "none") == "apvm"),
+                               pvspar=float(sc.get("pvspar", 0.5)), dt=float(sc.get("dt", 0.0)),
+                               diffusion=int(k2 != 0.0), hyperdiffusion=int(k4 != 0.0),
+                               dif_hx=capi.f64p(self.dif_hx), dif_tr=capi.f64p(self.dif_tr))
+            self.p.scheme = ctypes.pointer(self._sc)
         self.f = _Fields()
         if lib().oracle_fields_alloc(ctypes.byref(self.desc), ctypes.byref(self.f), int(perot)) != 0:
             raise MemoryError("oracle_fields_alloc")
@@ -92,10 +120,14 @@ class Oracle:
 
     def field(self, name):
         n = {"v": self.nv, "e": self.ne, "t": self.nt}[capi.FIELD_POS.get(name, "v")]
-        if name in ("eta_ed",):
+        if name in ("eta_ed", "q_grad_ed", "lapu", "lap_lapu"):
             n = self.ne
         if name == "hbt":
             n = self.nv
+        if name == "v_hx":
+            n = 3 * self.nv
+        if name == "q_grad_tr":
+            n = 3 * self.nt
         return np.ctypeslib.as_array(getattr(self.f, name), shape=(n,)).copy()
 
     def tendency(self, h, u):

@@ -18,7 +18,12 @@
 ! Usage (run in a scratch directory; par/ grid/ data/ are created there):
 !   ref_harness build <glevel> <eqs|pol> <out.bin>
 !   ref_harness run   <mesh: build:G:pos | file.bin> <state: tc2 | file.bin>
-!                     <nsteps> <dt> <diag 0|1> <out.bin>
+!                     <nsteps> <dt> <diag 0|1> <out.bin> [scheme]
+! scheme: comma-separated key=value lines of par/swm.par, e.g.
+!   "stag=HTC,wrap=trsk10,k2=1e5,diffus=const,k4=0,hyperdiffus=const,pv=apvm,pvspar=0.5"
+! (defaults: HTC, trsk10, no diffusion, no PV stabilisation); the
+! diffusion coefficients are then set up as swm_tests does
+! (src/swm.f90:170-210).
 !=======================================================================
 module imxbin
   use constants, only: i4, r8
@@ -335,6 +340,7 @@ program ref_harness
   use omp_lib
   use interpack, only: plot_scalarfield
   use datastruct, only: scalar_field
+  use swm_operators, only: alignment_coef, diameter_coef, scalar_hx2ed, scalar_hx2tr
   implicit none
 
   character(len=512) :: cmd, a1, a2, a3, meshsrc, statesrc, outf
@@ -346,6 +352,10 @@ program ref_harness
   real(r8), allocatable :: series(:, :)
   real(r8), allocatable :: hst(:), ust(:)
   integer :: pu
+  character(len=512) :: scheme
+  character(len=16) :: sc_stag = 'HTC', sc_wrap = 'trsk10', sc_diffus = 'const', sc_hdiffus = 'const'
+  character(len=16) :: sc_pv = 'none'
+  real(r8) :: sc_k2 = 0._r8, sc_k4 = 0._r8, sc_pvspar = 0.5_r8
 
   call get_command_argument(1, cmd)
   call execute_command_line('mkdir -p par grid data')
@@ -401,6 +411,8 @@ program ref_harness
     call get_command_argument(5, a2); read(a2, *) dtin
     call get_command_argument(6, a3); read(a3, *) idiag
     call get_command_argument(7, outf)
+    call get_command_argument(8, scheme)
+    call parse_scheme(trim(scheme))
 
     if (meshsrc(1:6) == 'build:') then
       pc1 = 7
@@ -416,6 +428,7 @@ program ref_harness
     call allocate_globalswmvars()
     call initialize_gridprop()
     call initialize_fields()
+    call setup_diffusion()
     if (trim(statesrc) /= 'tc2') then
       call bin_read(trim(statesrc))
       hst = getr('h'); ust = getr('u')
@@ -437,6 +450,16 @@ program ref_harness
     call put_f8('ghbK', ghbK%f); call put_f8('grad_ghbK', grad_ghbK%f)
     call put_f8('uhq_perp', uhq_perp%f)
     call put_f8('divu', divu%f); call put_f8('zeta', zeta%f)
+    call put_f8('v_hx', (/ (v_hx%p(k)%v, k = 1, mesh%nv) /))
+    if (useAPVM) then
+      call put_f8('q_grad_ed', q_grad_ed%f)
+      call put_f8('q_grad_tr', (/ (q_grad_tr%p(k)%v, k = 1, mesh%nt) /))
+    end if
+    if (K2_max /= 0 .or. K4_max /= 0) then
+      call put_f8('dif_coef_hx', dif_coef_hx%f); call put_f8('dif_coef_tr', dif_coef_tr%f)
+      call put_f8('lapu', lapu%f)
+    end if
+    if (K4_max /= 0) call put_f8('lap_lapu', lap_lapu%f)
 
     !Initial diagnostics (swm.f90:224-227)
     mass0 = sumf_areas(h)
@@ -491,9 +514,72 @@ contains
     call meshbuild(mesh)
   end subroutine build_ref_mesh
 
+  !key=value list of the scheme options written into par/swm.par
+  subroutine parse_scheme(str)
+    character(len=*), intent(in) :: str
+    integer :: a, b, eq
+    character(len=64) :: item, key, val
+    a = 1
+    do while (a <= len(str))
+      b = index(str(a:), ',')
+      if (b == 0) then
+        b = len(str) + 1
+      else
+        b = a + b - 1
+      end if
+      item = str(a:b-1)
+      a = b + 1
+      eq = index(item, '=')
+      if (eq == 0) cycle
+      key = item(1:eq-1)
+      val = item(eq+1:)
+      select case (trim(key))
+      case ('stag'); sc_stag = val
+      case ('wrap'); sc_wrap = val
+      case ('k2'); read(val, *) sc_k2
+      case ('diffus'); sc_diffus = val
+      case ('k4'); read(val, *) sc_k4
+      case ('hyperdiffus'); sc_hdiffus = val
+      case ('pv'); sc_pv = val
+      case ('pvspar'); read(val, *) sc_pvspar
+      case default
+        print*, 'unknown scheme key ', trim(key)
+        stop 1
+      end select
+    end do
+  end subroutine parse_scheme
+
+  !Diffusion and hyperdiffusion coefficients as swm_tests sets them up
+  !after initialize_fields (src/swm.f90:170-210)
+  subroutine setup_diffusion()
+    if (K2_max > 0.d0) then
+      select case (diffus)
+      case ('const'); dif_coef_hx%f = K2_max
+      case ('align'); call alignment_coef(K2_max, dif_coef_hx, mesh)
+      case ('diam'); call diameter_coef(K2_max, dif_coef_hx, mesh)
+      case default; stop 'invalid diffusion coefficient function'
+      end select
+      call scalar_hx2ed(dif_coef_hx, dif_coef_ed, mesh)
+      call scalar_hx2tr(dif_coef_hx, dif_coef_tr, mesh)
+    end if
+    if (K4_max > 0.d0) then
+      select case (hyperdiffus)
+      case ('const'); hypdif_coef_hx%f = K4_max
+      case ('align'); call alignment_coef(K4_max, hypdif_coef_hx, mesh)
+      case ('diam'); call diameter_coef(K4_max, hypdif_coef_hx, mesh)
+      case default; stop 'invalid hyperdiffusion coefficient function'
+      end select
+      call scalar_hx2ed(hypdif_coef_hx, hypdif_coef_ed, mesh)
+      call scalar_hx2tr(hypdif_coef_hx, hypdif_coef_tr, mesh)
+      dif_coef_hx%f = dsqrt(hypdif_coef_hx%f)
+      dif_coef_ed%f = dsqrt(hypdif_coef_ed%f)
+      dif_coef_tr%f = dsqrt(hypdif_coef_tr%f)
+    end if
+  end subroutine setup_diffusion
+
   !Positional par/swm.par in the layout read by swmpars (swm_data.f90:389-424):
-  !TC2 (overridden by a state file when given), HTC + trsk10, no diffusion,
-  !no PV stabilisation, no plots.
+  !TC2 (overridden by a state file when given), the scheme options of
+  !parse_scheme, no plots.
   subroutine write_swm_par(dtv)
     real(r8), intent(in) :: dtv
     open(newunit=pu, file='par/swm.par', status='replace', iostat=ios)
@@ -505,9 +591,9 @@ contains
     write(pu, '(a)') '!c'
     write(pu, '(es25.17,a)') dtv, ' 0 0'
     write(pu, '(a)') '!c'
-    write(pu, '(a)') 'HTC'
+    write(pu, '(a)') trim(sc_stag)
     write(pu, '(a)') '!c'
-    write(pu, '(a)') 'trsk10'
+    write(pu, '(a)') trim(sc_wrap)
     write(pu, '(a)') '!c'
     write(pu, '(a)') 'trsk 1.0'
     write(pu, '(a)') '!c'
@@ -521,15 +607,15 @@ contains
     write(pu, '(a)') '!c'
     write(pu, '(a)') '0 1000000 0'
     write(pu, '(a)') '!c'
-    write(pu, '(a)') '0 const'
+    write(pu, '(es25.17,1x,a)') sc_k2, trim(sc_diffus)
     write(pu, '(a)') '!c'
-    write(pu, '(a)') '0 const'
+    write(pu, '(es25.17,1x,a)') sc_k4, trim(sc_hdiffus)
     write(pu, '(a)') '!c'
     write(pu, '(a)') '2.0'
     write(pu, '(a)') '!c'
     write(pu, '(a)') '0.0'
     write(pu, '(a)') '!c'
-    write(pu, '(a)') 'none 0.5'
+    write(pu, '(a,1x,es25.17)') trim(sc_pv), sc_pvspar
     write(pu, '(a)') '!c'
     write(pu, '(a)') '0'
     close(pu)

@@ -9,8 +9,9 @@
  * built with -ffp-contract=off on the same inputs (pinned against golden
  * vectors written by oracle/_ref/ref_harness, see tests/golden/).
  *
- * Scope: HTC staggering, trsk10 wrapper, geodesic areas, no diffusion, no
- * APVM (the reference defaults, par/swm.par), i.e. the branches
+ * Scope: HTC staggering, trsk10 wrapper, geodesic areas (the reference
+ * defaults, par/swm.par), optionally with APVM, diffusion and
+ * hyperdiffusion (oracle_scheme), i.e. the branches
  *   scalar_hx2ed        src/swm_operators.f90:88-98
  *   scalar_elem_product src/swm_operators.f90:1546-1577
  *   vort_tr             src/swm_operators.f90:551-611
@@ -23,6 +24,14 @@
  *   grad_ed             src/swm_operators.f90:511-548 (HTC :535-541)
  *   coriolis_ed         src/swm_operators.f90:1219-1424 (trsk :1283-1309)
  *   rel_vort_tr         src/swm_operators.f90:780-827
+ *   grad_tr2ed          src/swm_operators.f90:1427-1463     (APVM)
+ *   vector_edtr2tr_perot src/swm_operators.f90:408-452       (APVM)
+ *   apvm                src/swm_operators.f90:1465-1502
+ *   diffusion_ed        src/swm_operators.f90:855-895
+ *   hyperdiffusion_ed   src/swm_operators.f90:930-975
+ *   grad_ed_tg          src/swm_operators.f90:738-777 (HTC :760-769)
+ *   alignment_coef, diameter_coef  src/swm_operators.f90:977-1072
+ *   alignind            src/smeshpack.f90:6398-6490
  *   tendency            src/swm.f90:2416-2598
  *   ode_rk4             src/swm.f90:2600-2683
  *   sumf_areas, sumfsq_areas, calc_energies  src/swm.f90:3394-3490, :2686-2752
@@ -40,11 +49,79 @@
 /* epsilon(1.) of the reference (constants.f90:45) */
 static const double EPS_SINGLE = 1.1920928955078125e-07;
 
+/* dsign(1, x) */
+static double sgn1(double x) { return x < 0.0 || (x == 0.0 && signbit(x)) ? -1.0 : 1.0; }
+
+/* sign of dot_product(ed%nr, tr(sh2)%c - tr(sh1)%c), ed%nr = v1 x v2 / |v1 x v2|
+ * (src/smeshpack.f90:725-726): the orientation of the dual edge
+ * sh1 -> sh2 relative to the primal edge's normal */
+static double ed_nr_sign(const imx_mesh_desc *d, int l) {
+  const double *a = d->v_p + 3 * (d->ed_v[2 * l] - 1), *b = d->v_p + 3 * (d->ed_v[2 * l + 1] - 1);
+  double nr[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const double *t1 = d->tr_c + 3 * (d->ed_sh[2 * l] - 1), *t2 = d->tr_c + 3 * (d->ed_sh[2 * l + 1] - 1);
+  double dp = 0.0;
+  for (int q = 0; q < 3; ++q) dp = dp + nr[q] * (t2[q] - t1[q]);
+  return sgn1(dp);
+}
+
+/* proj_vec_sphere (src/smeshpack.f90:4419-4433) in place */
+static void proj_sphere(double *v, const double *pc) {
+  double vp = 0.0, pn = 0.0;
+  for (int q = 0; q < 3; ++q) vp = vp + v[q] * pc[q];
+  for (int q = 0; q < 3; ++q) pn = pn + pc[q] * pc[q];
+  pn = sqrt(pn);
+  for (int q = 0; q < 3; ++q) v[q] = v[q] - vp * pc[q] / pn;
+}
+
 /* getedindexonhx (src/smeshpack.f90:4099-4125): local index of edge ed in cell */
 static int edindex(const imx_mesh_desc *d, int ed1, int cell0) {
   for (int i = 0; i < d->v_nnb[cell0]; ++i)
     if (d->v_ed[MX * cell0 + i] == ed1) return i;
   return -1;
+}
+
+/* div_hx (HTC) of an edge field */
+static void div_hx(const imx_mesh_desc *d, double erad, const double *f, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < d->nv; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < d->v_nnb[i]; ++j) {
+      int l = d->v_ed[MX * i + j] - 1;
+      double signcor = (double)d->hx_ttgout[MX * i + j];
+      s = s + signcor * f[l] * d->edhx_leng[l];
+    }
+    out[i] = s / d->hx_areag[i] / erad;
+  }
+}
+
+/* rel_vort_tr (HTC) of an edge field */
+static void rel_vort_tr(const imx_mesh_desc *d, double erad, const double *f, double *out) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < d->nt; ++k) {
+    double z = 0.0;
+    for (int l = 0; l < 3; ++l) {
+      int ed = d->tr_ed[3 * k + l] - 1;
+      double signcor = (double)d->tr_tg[3 * k + l];
+      z = z + f[ed] * d->ed_leng[ed] * signcor;
+    }
+    out[k] = z / d->tr_areag[k] / erad;
+  }
+}
+
+/* diffusion_ed (HTC): grad_ed(K div) - grad_ed_tg(K vort) */
+static void diffusion(const imx_mesh_desc *d, double erad, const double *kh, const double *kt,
+                      const double *div, const double *vort, oracle_fields *w, double *out) {
+  for (int i = 0; i < d->nv; ++i) w->cdiv[i] = kh[i] * div[i];
+  for (int k = 0; k < d->nt; ++k) w->cvort[k] = kt[k] * vort[k];
+#pragma omp parallel for schedule(static)
+  for (int l = 0; l < d->ne; ++l) {
+    double g = w->cdiv[d->ed_v[2 * l + 1] - 1] - w->cdiv[d->ed_v[2 * l] - 1];
+    w->gdiv[l] = g / d->ed_leng[l] / erad;
+    const double f1 = w->cvort[d->ed_sh[2 * l] - 1], f2 = w->cvort[d->ed_sh[2 * l + 1] - 1];
+    double gt = ed_nr_sign(d, l) * (f2 - f1);
+    w->gvort[l] = gt / (d->edhx_leng[l] * erad);
+  }
+  for (int l = 0; l < d->ne; ++l) out[l] = w->gdiv[l] - w->gvort[l];
 }
 
 int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double *h,
@@ -117,8 +194,35 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
   for (int l = 0; l < ne; ++l)
     q_ed[l] = 0.5 * (q_tr[d->ed_sh[2 * l] - 1] + q_tr[d->ed_sh[2 * l + 1] - 1]);
 
-  /* vector_edhx2hx_perot(u): dead for trsk10 but part of the reference step */
-  if (vhx) {
+  const oracle_scheme *sc = p->scheme;
+  const int apvm = sc && sc->apvm;
+
+  /* grad_tr2ed(q_tr) and its Perot reconstruction at triangles (APVM) */
+  if (apvm) {
+#pragma omp parallel for schedule(static)
+    for (int l = 0; l < ne; ++l) {
+      const int k1 = d->ed_sh[2 * l] - 1, k2 = d->ed_sh[2 * l + 1] - 1;
+      w->q_grad_ed[l] = ed_nr_sign(d, l) * (q_tr[k2] - q_tr[k1]) / d->edhx_leng[l] / erad;
+    }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nt; ++k) {
+      double v[3] = {0.0, 0.0, 0.0};
+      const double *tc = d->tr_c + 3 * k;
+      for (int l = 0; l < 3; ++l) {
+        const int ed = d->tr_ed[3 * k + l] - 1;
+        const double signcor = sgn1(-(double)d->tr_tg[3 * k + l]); /* tr%nr = -tr%tg */
+        for (int q = 0; q < 3; ++q)
+          v[q] = v[q] + signcor * w->q_grad_ed[ed] * (d->ed_c[3 * ed + q] - tc[q]) * d->ed_leng[ed];
+      }
+      for (int q = 0; q < 3; ++q) v[q] = v[q] / d->tr_areag[k];
+      proj_sphere(v, tc);
+      for (int q = 0; q < 3; ++q) w->q_grad_tr[3 * k + q] = v[q];
+    }
+  }
+
+  /* vector_edhx2hx_perot(u): used by APVM, dead for plain trsk10 (computed on
+     request, as the reference does it every step) */
+  if (w->perot || apvm) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nv; ++i) {
       double v[3] = {0.0, 0.0, 0.0};
@@ -178,6 +282,19 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
   }
   for (int l = 0; l < ne; ++l) momeq[l] = -grad[l];
 
+  /* apvm: q_ed - pvspar dt (grad q at the edge's triangles) . (v at its cells) */
+  if (apvm) {
+    const double *qg = w->q_grad_tr;
+    for (int l = 0; l < ne; ++l) {
+      const int k1 = d->ed_sh[2 * l] - 1, k2 = d->ed_sh[2 * l + 1] - 1;
+      const int c1 = d->ed_v[2 * l] - 1, c2 = d->ed_v[2 * l + 1] - 1;
+      double dp = 0.0;
+      for (int q = 0; q < 3; ++q)
+        dp = dp + (qg[3 * k1 + q] + qg[3 * k2 + q]) * (vhx[3 * c1 + q] + vhx[3 * c2 + q]);
+      q_ed[l] = q_ed[l] - sc->pvspar * sc->dt * dp;
+    }
+  }
+
   /* coriolis_ed (trsk, PV form) */
 #pragma omp parallel for schedule(static)
   for (int l = 0; l < ne; ++l) {
@@ -219,6 +336,19 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
     zeta[k] = z / d->tr_areag[k] / erad;
   }
 
+  /* diffusion_ed / hyperdiffusion_ed: momeq + lapu, momeq - lap_lapu */
+  if (sc && (sc->diffusion || sc->hyperdiffusion)) {
+    diffusion(d, erad, sc->dif_hx, sc->dif_tr, divu, zeta, w, w->lapu);
+    if (sc->diffusion)
+      for (int l = 0; l < ne; ++l) momeq[l] = momeq[l] + w->lapu[l];
+    if (sc->hyperdiffusion) {
+      div_hx(d, erad, w->lapu, w->div_lap);
+      rel_vort_tr(d, erad, w->lapu, w->vort_lap);
+      diffusion(d, erad, sc->dif_hx, sc->dif_tr, w->div_lap, w->vort_lap, w, w->lap_lapu);
+      for (int l = 0; l < ne; ++l) momeq[l] = momeq[l] - w->lap_lapu[l];
+    }
+  }
+
   if (p->testcase <= 1)
     for (int l = 0; l < ne; ++l) momeq[l] = 0.0;
   return 0;
@@ -242,14 +372,27 @@ int oracle_fields_alloc(const imx_mesh_desc *d, oracle_fields *w, int with_perot
   w->uhq_perp = calloc(ne, 8);
   w->divu = calloc(nv, 8);
   w->zeta = calloc(nt, 8);
-  w->v_hx = with_perot ? calloc(3 * nv, 8) : NULL;
+  w->v_hx = calloc(3 * nv, 8);
+  w->perot = with_perot;
+  w->q_grad_ed = calloc(ne, 8);
+  w->q_grad_tr = calloc(3 * nt, 8);
+  w->lapu = calloc(ne, 8);
+  w->lap_lapu = calloc(ne, 8);
+  w->cdiv = calloc(nv, 8);
+  w->cvort = calloc(nt, 8);
+  w->gdiv = calloc(ne, 8);
+  w->gvort = calloc(ne, 8);
+  w->div_lap = calloc(nv, 8);
+  w->vort_lap = calloc(nt, 8);
   w->massf = calloc(4 * nv, 8);
   w->momf = calloc(4 * ne, 8);
   w->h_new = calloc(nv, 8);
   w->u_new = calloc(ne, 8);
   if (!w->h_ed || !w->uh || !w->eta || !w->h_tr || !w->q_tr || !w->q_ed || !w->eta_ed ||
       !w->divuh || !w->ke_hx || !w->ghbk || !w->hbt || !w->grad_ghbk || !w->uhq_perp || !w->divu ||
-      !w->zeta || (with_perot && !w->v_hx) || !w->massf || !w->momf || !w->h_new || !w->u_new)
+      !w->zeta || !w->v_hx || !w->massf || !w->momf || !w->h_new || !w->u_new || !w->q_grad_ed ||
+      !w->q_grad_tr || !w->lapu || !w->lap_lapu || !w->cdiv || !w->cvort || !w->gdiv || !w->gvort ||
+      !w->div_lap || !w->vort_lap)
     return -2;
   return 0;
 }
@@ -257,7 +400,9 @@ int oracle_fields_alloc(const imx_mesh_desc *d, oracle_fields *w, int with_perot
 void oracle_fields_free(oracle_fields *w) {
   double *ptrs[] = {w->h_ed, w->uh, w->eta, w->h_tr, w->q_tr, w->q_ed, w->eta_ed,
                     w->divuh, w->ke_hx, w->ghbk, w->hbt, w->grad_ghbk, w->uhq_perp, w->divu,
-                    w->zeta, w->v_hx, w->massf, w->momf, w->h_new, w->u_new};
+                    w->zeta, w->v_hx, w->massf, w->momf, w->h_new, w->u_new, w->q_grad_ed,
+                    w->q_grad_tr, w->lapu, w->lap_lapu, w->cdiv, w->cvort, w->gdiv, w->gvort,
+                    w->div_lap, w->vort_lap};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
   memset(w, 0, sizeof(*w));
 }
@@ -345,4 +490,107 @@ void oracle_diagnostics(const imx_mesh_desc *d, const oracle_params *p, const do
   out[5] = pz;
   out[6] = sqrt(s2 / sa);
   out[7] = badflag;
+}
+
+/* arclen (src/smeshpack.f90:4624-4657) */
+static double arclen(const double *p, const double *q) {
+  double d = 0.0;
+  for (int i = 0; i < 3; ++i) d = d + (p[i] + q[i]) * (p[i] + q[i]);
+  if (d == 0.0) return 3.141592653589793;
+  if (d >= 4.0) return 0.0;
+  return 2.0 * atan(sqrt((4.0 - d) / d));
+}
+
+/* alignind (src/smeshpack.f90:6398-6490), 0-based cell i */
+static double alignind(const imx_mesh_desc *d, int i) {
+  const int n = d->v_nnb[i];
+  const int *tr = d->v_tr + MX * i;
+  double tmp = 0.0, meanl = 0.0;
+#define TC(j) (d->tr_c + 3 * (tr[(j)-1] - 1))
+  if (n == 6 || n == 4) {
+    const int np = n == 6 ? 3 : 2;
+    for (int j = 1; j <= np; ++j) {
+      int j0 = n == 6 ? j + 4 : j + 3;
+      if (j0 > n) j0 -= n; /* modint */
+      const int i1 = j, i2 = j + 1, i3 = n == 6 ? j + 3 : j + 2;
+      double t1 = arclen(TC(i1), TC(i2)), t2 = arclen(TC(i3), TC(j0));
+      meanl = t1 + t2;
+      tmp = tmp + fabs(t1 - t2);
+      t1 = arclen(TC(i1), TC(j0));
+      t2 = arclen(TC(i2), TC(i3));
+      tmp = tmp + fabs(t1 - t2);
+    }
+    tmp = tmp / 6;
+    meanl = meanl / 6;
+    tmp = tmp / meanl;
+  } else {
+    tmp = 1;
+  }
+#undef TC
+  return tmp;
+}
+
+/* the coefficient at cells for one K_max and kind */
+static void coef_hx(const imx_mesh_desc *d, double kmax, int kind, double *c) {
+  const int nv = d->nv;
+  if (kind == 0) {
+    for (int i = 0; i < nv; ++i) c[i] = kmax;
+    return;
+  }
+  double *a = malloc((size_t)nv * 8);
+  if (kind == 1) { /* alignment_coef */
+    for (int i = 0; i < nv; ++i) a[i] = alignind(d, i);
+    for (int i = 0; i < nv; ++i) {
+      double sum = a[i];
+      for (int j = 0; j < d->v_nnb[i]; ++j) sum = sum + a[d->v_nb[MX * i + j] - 1];
+      sum = sum / (d->v_nnb[i] + 1.0);
+      c[i] = sum;
+    }
+    double mx = c[0];
+    for (int i = 1; i < nv; ++i) mx = c[i] > mx ? c[i] : mx;
+    for (int i = 0; i < nv; ++i) c[i] = c[i] / mx;
+    for (int i = 0; i < nv; ++i) c[i] = kmax * c[i];
+  } else { /* diameter_coef */
+    for (int i = 0; i < nv; ++i) {
+      double dm = 0.0;
+      for (int j = 0; j < d->v_nnb[i]; ++j)
+        for (int k = 0; k < d->v_nnb[i]; ++k) {
+          const double *pj = d->tr_c + 3 * (d->v_tr[MX * i + j] - 1);
+          const double *pk = d->tr_c + 3 * (d->v_tr[MX * i + k] - 1);
+          double al = arclen(pj, pk);
+          dm = dm > al ? dm : al;
+        }
+      c[i] = dm;
+    }
+    double mx = c[0];
+    for (int i = 1; i < nv; ++i) mx = c[i] > mx ? c[i] : mx;
+    for (int i = 0; i < nv; ++i) c[i] = c[i] / mx;
+    const double pw = log(10.0) / log(2.0);
+    for (int i = 0; i < nv; ++i) c[i] = kmax * pow(c[i], pw);
+  }
+  free(a);
+}
+
+/* scalar_hx2tr (trsk, src/swm_operators.f90:258-263) */
+static void hx2tr(const imx_mesh_desc *d, const double *trhx_area, const double *f, double *out) {
+  for (int k = 0; k < d->nt; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < 3; ++i) s = s + trhx_area[3 * k + i] * f[d->tr_v[3 * k + i] - 1];
+    out[k] = s / d->tr_areag[k];
+  }
+}
+
+int oracle_diffusion_coef(const imx_mesh_desc *d, const double *trhx_area, double k2, int k2kind,
+                          double k4, int k4kind, double *dif_hx, double *dif_tr) {
+  if (k2 > 0.0) {
+    coef_hx(d, k2, k2kind, dif_hx);
+    hx2tr(d, trhx_area, dif_hx, dif_tr);
+  }
+  if (k4 > 0.0) {
+    coef_hx(d, k4, k4kind, dif_hx);
+    hx2tr(d, trhx_area, dif_hx, dif_tr);
+    for (int i = 0; i < d->nv; ++i) dif_hx[i] = sqrt(dif_hx[i]);
+    for (int k = 0; k < d->nt; ++k) dif_tr[k] = sqrt(dif_tr[k]);
+  }
+  return 0;
 }

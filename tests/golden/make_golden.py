@@ -10,6 +10,10 @@ outputs as fixtures:
   ref_run_g3_tc2.bin        TC2: tendency intermediates, diag, 5 RK4 steps dt=900
   ref_run_g3_spec.bin       seeded spectral state (seed 7): same
   ref_run_g2_spec96.bin     seeded spectral state (seed 3): 96 steps dt=900
+  ref_run_g3_<scheme>.bin   seed 7, 3 steps dt=900 with the scheme options of
+                            SCHEMES (APVM, diffusion, hyperdiffusion with the
+                            const/align/diam coefficients), plus v_hx,
+                            q_grad_ed/tr, dif_coef_hx/tr, lapu, lap_lapu
 
 The seeded states are made by libimx (imx_ic_spectral) on the reference
 mesh and handed to the reference as input (the reference has no such case).
@@ -28,6 +32,14 @@ from imodel_amd import imxbin  # noqa: E402
 from imodel_amd.swm import Mesh  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+# par/swm.par options of the scheme fixtures (ref_harness run ... [scheme])
+SCHEMES = {
+    "apvm": "pv=apvm,pvspar=0.5",
+    "k2align": "k2=1e5,diffus=align",
+    "k4diam": "k4=1e15,hyperdiffus=diam",
+    "mix": "k2=2e5,diffus=const,k4=3e15,hyperdiffus=align,pv=apvm,pvspar=0.3",
+}
 
 
 def run(args, cwd):
@@ -53,6 +65,12 @@ def main():
             imxbin.write(os.path.join(td, "state.bin"), {"h": h, "u": u})
             run(["run", f"build:{g}:eqs", "state.bin", str(steps), "900", "1", "out.bin"], td)
             shutil.copy(os.path.join(td, "out.bin"), os.path.join(HERE, name))
+        mesh = Mesh(imxbin.read(os.path.join(HERE, "ref_mesh_g3.bin")))
+        h, u = mesh.spectral(7)
+        imxbin.write(os.path.join(td, "state.bin"), {"h": h, "u": u})
+        for name, spec in SCHEMES.items():
+            run(["run", "build:3:eqs", "state.bin", "3", "900", "0", "out.bin", spec], td)
+            shutil.copy(os.path.join(td, "out.bin"), os.path.join(HERE, f"ref_run_g3_{name}.bin"))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".bin"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

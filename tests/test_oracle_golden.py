@@ -62,3 +62,56 @@ def test_oracle_detects_planted_mistake(golden):
     o = Oracle(mesh)
     _, f = o.tendency(r["h0"], r["u0"])
     assert not np.array_equal(f, r["momeq"])
+
+
+# scheme options of the scheme fixtures (tests/golden/make_golden.py SCHEMES)
+SCHEMES = {
+    "apvm": {"pv": "apvm", "pvspar": 0.5},
+    "k2align": {"k2": 1e5, "diffus": "align"},
+    "k4diam": {"k4": 1e15, "hyperdiffus": "diam"},
+    "mix": {"k2": 2e5, "diffus": "const", "k4": 3e15, "hyperdiffus": "align", "pv": "apvm",
+            "pvspar": 0.3},
+}
+SCHEME_FIELDS = ["v_hx", "q_grad_ed", "q_grad_tr", "lapu", "lap_lapu"]
+
+
+@pytest.mark.parametrize("name", sorted(SCHEMES))
+def test_scheme_options_bitexact(golden, name):
+    """APVM (grad_tr2ed, Perot reconstructions, apvm), diffusion and
+    hyperdiffusion with their coefficient set-up: tendency, every
+    intermediate the reference exposes, and 3 RK4 steps, bit for bit."""
+    mesh = golden("ref_mesh_g3.bin")
+    r = golden(f"ref_run_g3_{name}.bin")
+    o = Oracle(mesh, perot=True, scheme=dict(SCHEMES[name], dt=900.0))
+    if "dif_coef_hx" in r:
+        assert np.array_equal(o.dif_hx, r["dif_coef_hx"]) and np.array_equal(o.dif_tr, r["dif_coef_tr"])
+    m, f = o.tendency(r["h0"], r["u0"])
+    assert np.array_equal(m, r["masseq"]) and np.array_equal(f, r["momeq"])
+    for k in SCHEME_FIELDS + ["q_ed", "uhq_perp"]:
+        if k in r:
+            assert np.array_equal(o.field(k), r[k]), k
+    h, u = o.step(r["h0"], r["u0"], 900.0, 3)
+    assert np.array_equal(h, r["h"]) and np.array_equal(u, r["u"])
+
+
+def test_perot_reconstruction_bitexact(golden):
+    """vector_edhx2hx_perot, computed by the reference every step (dead for
+    plain trsk10), against its v_hx."""
+    mesh = golden("ref_mesh_g3.bin")
+    r = golden("ref_run_g3_spec.bin")
+    o = Oracle(mesh, perot=True)
+    o.tendency(r["h0"], r["u0"])
+    assert np.array_equal(o.field("v_hx"), r["v_hx"])
+
+
+def test_scheme_planted_mistakes_fail(golden):
+    """The scheme fixtures catch a wrong APVM parameter or a wrong diffusion
+    coefficient in the oracle's inputs. (With K4 > 0 the reference runs both
+    Laplacian terms with the square roots of the hyperdiffusion coefficients,
+    src/swm.f90:206-208, so in "mix" the K2 kind and value do not enter.)"""
+    mesh = golden("ref_mesh_g3.bin")
+    r = golden("ref_run_g3_mix.bin")
+    for bad in ({"pvspar": 0.31}, {"k4": 3.1e15}, {"hyperdiffus": "const"}, {"k2": 0.0}):
+        o = Oracle(mesh, scheme=dict(SCHEMES["mix"], dt=900.0, **bad))
+        _, f = o.tendency(r["h0"], r["u0"])
+        assert not np.array_equal(f, r["momeq"]), bad
