@@ -19,14 +19,22 @@ NDIAG = 8
 DIAG_NAMES = ("mass", "penergy", "kenergy", "tenergy", "aenergy", "penstrophy",
               "rmsdiv", "badflag")
 FIELDS = ("h_ed", "uh", "eta", "h_tr", "q_tr", "q_ed", "divuh", "ke_hx", "ghbk",
-          "grad_ghbk", "uhq_perp", "divu", "zeta")
+          "grad_ghbk", "uhq_perp", "divu", "zeta", "q_grad_ed", "lapu", "lap_lapu")
 FIELD_POS = {"h_ed": "e", "uh": "e", "eta": "t", "h_tr": "t", "q_tr": "t",
              "q_ed": "e", "divuh": "v", "ke_hx": "v", "ghbk": "v",
-             "grad_ghbk": "e", "uhq_perp": "e", "divu": "v", "zeta": "t"}
+             "grad_ghbk": "e", "uhq_perp": "e", "divu": "v", "zeta": "t",
+             "q_grad_ed": "e", "lapu": "e", "lap_lapu": "e"}
+COEF = {"const": 0, "align": 1, "diam": 2}
 
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _F64P = ctypes.POINTER(ctypes.c_double)
 _F32P = ctypes.POINTER(ctypes.c_float)
+
+
+class Scheme(ctypes.Structure):
+    """imx_scheme: APVM, diffusion and hyperdiffusion options (par/swm.par)."""
+    _fields_ = [("apvm", ctypes.c_int32), ("pvspar", ctypes.c_double), ("k2", ctypes.c_double),
+                ("diffus", ctypes.c_int32), ("k4", ctypes.c_double), ("hyperdiffus", ctypes.c_int32)]
 
 
 class MeshDesc(ctypes.Structure):
@@ -122,6 +130,7 @@ def lib():
             "imx_swm_get_state": (ctypes.c_int, [vp, _F64P, _F64P]),
             "imx_swm_tendency": (ctypes.c_int, [vp, _F64P, _F64P, _F64P, _F64P]),
             "imx_swm_get_field": (ctypes.c_int, [vp, ctypes.c_int32, _F64P]),
+            "imx_swm_set_scheme": (ctypes.c_int, [vp, ctypes.POINTER(MeshDesc), ctypes.POINTER(Scheme)]),
             "imx_swm_step": (ctypes.c_int, [vp, ctypes.c_int32]),
             "imx_swm_step_timed": (ctypes.c_int, [vp, ctypes.c_int32, _F64P]),
             "imx_swm_profile": (ctypes.c_int, [vp, ctypes.c_int32, _F64P, _I32P]),

@@ -83,6 +83,14 @@ struct imx_swm {
          *f_q_tr = nullptr, *f_q_ed = nullptr, *f_divuh = nullptr, *f_ke = nullptr,
          *f_ghbk = nullptr, *f_grad = nullptr, *f_uhq = nullptr, *f_divu = nullptr,
          *f_zeta = nullptr;
+  // REF scheme options (imx_swm_set_scheme): APVM, diffusion, hyperdiffusion
+  imx_scheme sc{};
+  bool sc_apvm = false, sc_diff = false, sc_hyper = false;
+  int8_t *e_nrs = nullptr;           // sign of ed%nr . (tr(sh2)%c - tr(sh1)%c)
+  double *e_c = nullptr, *t_c = nullptr, *c_p = nullptr;  // [3][n] points
+  double *dif_hx = nullptr, *dif_tr = nullptr;            // coefficients
+  double *f_qg_ed = nullptr, *f_qg_tr = nullptr, *f_vhx = nullptr;  // APVM
+  double *f_lapu = nullptr, *f_lap2 = nullptr, *f_divl = nullptr, *f_vortl = nullptr;
 
   // FAST tables and state (typed by precision; void* holds T*)
   int32_t *x_c_nb = nullptr, *x_c_bitsA = nullptr, *x_c_bitsB = nullptr;

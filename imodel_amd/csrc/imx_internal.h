@@ -60,6 +60,15 @@ double det3(const double *p1, const double *p2, const double *p3);
 void cart2sph(double x, double y, double z, double *lon, double *lat);
 
 void gridprop(const imx_mesh_desc &d, double *trhx, double *hxtr, double *trskw);
+// Radius, in cell rings around a row's cells, of a stencil superset of the
+// handle's tendency: 1 for TRiSK, one more ring each for APVM and for
+// hyperdiffusion.
+int swm_stencil_rings(const imx_swm *s);
+// dif_coef_hx [nv] / dif_coef_tr [nt] as swm_tests sets them up
+// (src/swm.f90:170-210), caller numbering; kind IMX_COEF_*. With k4 > 0
+// both hold the square roots of the hyperdiffusion coefficients.
+void diffusion_coef(const imx_mesh_desc &d, const double *trhx, double k2, int k2kind, double k4,
+                    int k4kind, std::vector<double> &hx, std::vector<double> &tr);
 // Internal numbering: cells along a Hilbert curve (or caller order); edge of
 // owning cell c -> 3c+k, triangle of owning cell c -> 2c+k. Maps are caller
 // (0-based) -> internal; NE = 3 nv and NT = 2 nv slots (some are dummies).

@@ -16,6 +16,11 @@
 //   every edge whose two cells are in that set (the cells' own edges for
 //   K and the flux, and the edges between consecutive neighbours that
 //   close the triangles of the vorticity in q).
+// APVM (q at an edge reads grad q at its triangles and v at its cells) and
+// hyperdiffusion (the Laplacian of the Laplacian) each reach one cell ring
+// further; diffusion stays within the TRiSK set. The sets are therefore
+// the cells within R rings of the row's cells, R = 1, 2 or 3
+// (swm_stencil_rings), with the edges among them.
 // A few tens of colours replace nv + ne evaluations.
 #include <algorithm>
 #include <cstring>
@@ -30,7 +35,7 @@ namespace {
 
 // stencil superset of every row (0..nv-1 masseq, nv..nv+ne-1 momeq), as
 // column ids (0..nv-1 h, nv.. u)
-void stencils(const imx_mesh_desc &d, std::vector<std::vector<int32_t>> &S) {
+void stencils(const imx_mesh_desc &d, int rings, std::vector<std::vector<int32_t>> &S) {
   const int nv = d.nv, ne = d.ne, MX = IMX_MAXNB;
   S.assign((size_t)nv + ne, {});
   std::vector<int32_t> mark(nv, -1);
@@ -54,20 +59,29 @@ void stencils(const imx_mesh_desc &d, std::vector<std::vector<int32_t>> &S) {
       cells.push_back(c);
     }
   };
+  // the row's cells, then `rings` rings of neighbours around them
+  auto grow = [&](int row) {
+    size_t a = 0;
+    for (int r = 0; r < rings; ++r) {
+      const size_t b = cells.size();
+      for (size_t i = a; i < b; ++i) {
+        const int c = cells[i];
+        for (int j = 0; j < d.v_nnb[c]; ++j) add_cell(row, d.v_nb[MX * c + j] - 1);
+      }
+      a = b;
+    }
+  };
   for (int c = 0; c < nv; ++c) {
     cells.clear();
     add_cell(c, c);
-    for (int j = 0; j < d.v_nnb[c]; ++j) add_cell(c, d.v_nb[MX * c + j] - 1);
+    grow(c);
     close_over(c, cells);
   }
   for (int e = 0; e < ne; ++e) {
     const int row = nv + e;
     cells.clear();
-    for (int q = 0; q < 2; ++q) {
-      const int c = d.ed_v[2 * e + q] - 1;
-      add_cell(row, c);
-      for (int j = 0; j < d.v_nnb[c]; ++j) add_cell(row, d.v_nb[MX * c + j] - 1);
-    }
+    for (int q = 0; q < 2; ++q) add_cell(row, d.ed_v[2 * e + q] - 1);
+    grow(row);
     // mark is indexed by cell with the row id as stamp; rows >= nv reuse it
     close_over(row, cells);
   }
@@ -85,7 +99,7 @@ int imx_swm_jacobian(imx_swm *s, const imx_mesh_desc *d, const double *h, const 
     return imx::set_error(IMX_EINVAL, "imx_swm_jacobian: bad argument");
   const int nv = d->nv, ne = d->ne, n = nv + ne;
   std::vector<std::vector<int32_t>> S;
-  imx::stencils(*d, S);
+  imx::stencils(*d, imx::swm_stencil_rings(s), S);
   // column -> rows that may read it
   std::vector<int64_t> cnt((size_t)n + 1, 0);
   for (int r = 0; r < n; ++r)

@@ -773,3 +773,94 @@ int imx_mesh_write(const imx_mesh *m, const char *path) {
 void imx_mesh_destroy(imx_mesh *m) { delete m; }
 
 }  // extern "C"
+
+namespace imx {
+
+// alignind (src/smeshpack.f90:6398-6490): regularity of a hexagonal (or
+// square) cell from its dual edges; 1 for other cells. As in the reference,
+// the mean length keeps only the last pair's sum.
+static double alignind(const imx_mesh_desc &d, int i) {
+  const int n = d.v_nnb[i];
+  auto tc = [&](int j) { return d.tr_c + 3 * (d.v_tr[IMX_MAXNB * i + j - 1] - 1); };  // j 1-based
+  if (n != 6 && n != 4) return 1.0;
+  double tmp = 0.0, meanl = 0.0;
+  const int pairs = n == 6 ? 3 : 2;
+  for (int j = 1; j <= pairs; ++j) {
+    const int i1 = j, i2 = j + 1, i3 = n == 6 ? j + 3 : j + 2;
+    int i0 = n == 6 ? j + 4 : j + 3;
+    if (i0 > n) i0 -= n;  // modint
+    double t1 = arclen(tc(i1), tc(i2));
+    double t2 = arclen(tc(i3), tc(i0));
+    meanl = t1 + t2;
+    tmp = tmp + std::fabs(t1 - t2);
+    t1 = arclen(tc(i1), tc(i0));
+    t2 = arclen(tc(i2), tc(i3));
+    tmp = tmp + std::fabs(t1 - t2);
+  }
+  tmp = tmp / 6;
+  meanl = meanl / 6;
+  return tmp / meanl;
+}
+
+// alignment_coef / diameter_coef (src/swm_operators.f90:977-1072), or const
+static void coef_at_cells(const imx_mesh_desc &d, double kmax, int kind, std::vector<double> &c) {
+  const int nv = d.nv;
+  c.assign(nv, kmax);
+  if (kind == IMX_COEF_CONST) return;
+  if (kind == IMX_COEF_ALIGN) {
+    std::vector<double> a(nv);
+    for (int i = 0; i < nv; ++i) a[i] = alignind(d, i);
+    for (int i = 0; i < nv; ++i) {  // smoothing over the cell and its neighbours
+      double sum = a[i];
+      for (int j = 0; j < d.v_nnb[i]; ++j) sum = sum + a[d.v_nb[IMX_MAXNB * i + j] - 1];
+      c[i] = sum / (d.v_nnb[i] + 1.0);
+    }
+  } else {  // the largest circumcentre distance within the cell
+    for (int i = 0; i < nv; ++i) {
+      double dm = 0.0;
+      for (int j = 0; j < d.v_nnb[i]; ++j)
+        for (int k = 0; k < d.v_nnb[i]; ++k)
+          dm = std::max(dm, arclen(d.tr_c + 3 * (d.v_tr[IMX_MAXNB * i + j] - 1),
+                                   d.tr_c + 3 * (d.v_tr[IMX_MAXNB * i + k] - 1)));
+      c[i] = dm;
+    }
+  }
+  double mx = c[0];
+  for (int i = 1; i < nv; ++i) mx = std::max(mx, c[i]);
+  for (int i = 0; i < nv; ++i) c[i] = c[i] / mx;
+  if (kind == IMX_COEF_ALIGN) {
+    for (int i = 0; i < nv; ++i) c[i] = kmax * c[i];
+  } else {
+    const double p = std::log(10.0) / std::log(2.0);
+    for (int i = 0; i < nv; ++i) c[i] = kmax * std::pow(c[i], p);
+  }
+}
+
+// scalar_hx2tr, trsk branch (src/swm_operators.f90:258-263)
+static void hx2tr(const imx_mesh_desc &d, const double *trhx, const std::vector<double> &f,
+                  std::vector<double> &out) {
+  out.assign(d.nt, 0.0);
+  for (int k = 0; k < d.nt; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < 3; ++i) s = s + trhx[3 * k + i] * f[d.tr_v[3 * k + i] - 1];
+    out[k] = s / d.tr_areag[k];
+  }
+}
+
+void diffusion_coef(const imx_mesh_desc &d, const double *trhx, double k2, int k2kind, double k4,
+                    int k4kind, std::vector<double> &hx, std::vector<double> &tr) {
+  hx.assign(d.nv, 0.0);
+  tr.assign(d.nt, 0.0);
+  if (k2 > 0.0) {
+    coef_at_cells(d, k2, k2kind, hx);
+    hx2tr(d, trhx, hx, tr);
+  }
+  if (k4 > 0.0) {
+    coef_at_cells(d, k4, k4kind, hx);
+    hx2tr(d, trhx, hx, tr);
+    for (double &x : hx) x = std::sqrt(x);
+    for (double &x : tr) x = std::sqrt(x);
+  }
+}
+
+}  // namespace imx

@@ -204,6 +204,183 @@ __global__ __launch_bounds__(TPB) void ref_edge_mom(
   momeq[l] = zero_mom ? 0.0 : (-g) - p;
 }
 
+// ---- scheme options of the REF path (APVM, diffusion, hyperdiffusion) ----
+// Points are stored component-major: x of entity i at [i], y at [n + i],
+// z at [2n + i].
+
+// grad_tr2ed (swm_operators.f90:1427-1463): gradient of q across each edge,
+// oriented by the primal edge's normal
+__global__ __launch_bounds__(TPB) void ref_qgrad_ed(int ne, const int32_t *__restrict__ e_sh,
+                                                    const int8_t *__restrict__ e_nrs,
+                                                    const double *__restrict__ e_dleng,
+                                                    const double *__restrict__ q_tr, double erad,
+                                                    double *__restrict__ qg) {
+  const int l = xcd_index();
+  if (l >= ne || e_sh[l] < 0) return;
+  qg[l] = (double)e_nrs[l] * (q_tr[e_sh[ne + l]] - q_tr[e_sh[l]]) / e_dleng[l] / erad;
+}
+
+// proj_vec_sphere (smeshpack.f90:4419-4433)
+__device__ __forceinline__ void proj_sphere(double &v0, double &v1, double &v2, double p0, double p1,
+                                            double p2) {
+  double vp = 0.0;
+  vp = vp + v0 * p0;
+  vp = vp + v1 * p1;
+  vp = vp + v2 * p2;
+  double pn = 0.0;
+  pn = pn + p0 * p0;
+  pn = pn + p1 * p1;
+  pn = pn + p2 * p2;
+  pn = sqrt(pn);
+  v0 = v0 - vp * p0 / pn;
+  v1 = v1 - vp * p1 / pn;
+  v2 = v2 - vp * p2 / pn;
+}
+
+// vector_edtr2tr_perot (swm_operators.f90:408-452) of the edge field qg:
+// the vector at each triangle's circumcentre (tr%nr = -tr%tg)
+__global__ __launch_bounds__(TPB) void ref_perot_tr(int nt, int ne, const int32_t *__restrict__ t_ed,
+                                                    const int8_t *__restrict__ t_tg,
+                                                    const double *__restrict__ e_c,
+                                                    const double *__restrict__ t_c,
+                                                    const double *__restrict__ e_leng,
+                                                    const double *__restrict__ t_area,
+                                                    const double *__restrict__ qg,
+                                                    double *__restrict__ vt) {
+  const int k = xcd_index();
+  if (k >= nt || t_ed[k] < 0) return;
+  const double p0 = t_c[k], p1 = t_c[nt + k], p2 = t_c[2 * nt + k];
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const int ed = t_ed[l * nt + k];
+    const double a = -(double)t_tg[l * nt + k] * qg[ed];
+    const double len = e_leng[ed];
+    v0 = v0 + a * (e_c[ed] - p0) * len;
+    v1 = v1 + a * (e_c[ne + ed] - p1) * len;
+    v2 = v2 + a * (e_c[2 * ne + ed] - p2) * len;
+  }
+  const double ar = t_area[k];
+  v0 = v0 / ar;
+  v1 = v1 / ar;
+  v2 = v2 / ar;
+  proj_sphere(v0, v1, v2, p0, p1, p2);
+  vt[k] = v0;
+  vt[nt + k] = v1;
+  vt[2 * nt + k] = v2;
+}
+
+// vector_edhx2hx_perot (swm_operators.f90:454-509, HTC): velocity vector at
+// each cell node
+__global__ __launch_bounds__(TPB) void ref_perot_hx(int nv, int ne, const int32_t *__restrict__ c_nnb,
+                                                    const int32_t *__restrict__ c_ed,
+                                                    const int8_t *__restrict__ c_ttg,
+                                                    const double *__restrict__ e_c,
+                                                    const double *__restrict__ c_p,
+                                                    const double *__restrict__ e_dleng,
+                                                    const double *__restrict__ c_area,
+                                                    const double *__restrict__ u,
+                                                    double *__restrict__ vh) {
+  const int i = xcd_index();
+  if (i >= nv) return;
+  const double p0 = c_p[i], p1 = c_p[nv + i], p2 = c_p[2 * nv + i];
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  const int n = c_nnb[i];
+  for (int j = 0; j < n; ++j) {
+    const int ed = c_ed[j * nv + i];
+    const double a = (double)c_ttg[j * nv + i] * u[ed];
+    const double len = e_dleng[ed];
+    v0 = v0 + a * (e_c[ed] - p0) * len;
+    v1 = v1 + a * (e_c[ne + ed] - p1) * len;
+    v2 = v2 + a * (e_c[2 * ne + ed] - p2) * len;
+  }
+  const double ar = c_area[i];
+  v0 = v0 / ar;
+  v1 = v1 / ar;
+  v2 = v2 / ar;
+  proj_sphere(v0, v1, v2, p0, p1, p2);
+  vh[i] = v0;
+  vh[nv + i] = v1;
+  vh[2 * nv + i] = v2;
+}
+
+// apvm (swm_operators.f90:1497-1501): q_ed -= pvspar dt (qg_t1 + qg_t2) . (v_c1 + v_c2)
+__global__ __launch_bounds__(TPB) void ref_apvm_q(int ne, int nt, int nv, const int32_t *__restrict__ e_sh,
+                                                  const int32_t *__restrict__ e_v,
+                                                  const double *__restrict__ qgt,
+                                                  const double *__restrict__ vh, double pvspar,
+                                                  double dt, double *__restrict__ q_ed) {
+  const int l = xcd_index();
+  if (l >= ne || e_sh[l] < 0) return;
+  const int k1 = e_sh[l], k2 = e_sh[ne + l], c1 = e_v[l], c2 = e_v[ne + l];
+  double dp = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    dp = dp + (qgt[q * nt + k1] + qgt[q * nt + k2]) * (vh[q * nv + c1] + vh[q * nv + c2]);
+  q_ed[l] = q_ed[l] - pvspar * dt * dp;
+}
+
+// diffusion_ed (swm_operators.f90:855-895, HTC): grad_ed(K_hx div) -
+// grad_ed_tg(K_tr vort); mode 1 adds the result to momeq, mode 2 subtracts it
+__global__ __launch_bounds__(TPB) void ref_lap(int ne, const int32_t *__restrict__ e_v,
+                                               const int32_t *__restrict__ e_sh,
+                                               const int8_t *__restrict__ e_nrs,
+                                               const double *__restrict__ e_leng,
+                                               const double *__restrict__ e_dleng,
+                                               const double *__restrict__ kh,
+                                               const double *__restrict__ kt,
+                                               const double *__restrict__ div,
+                                               const double *__restrict__ vort, double erad, int mode,
+                                               double *__restrict__ out, double *__restrict__ momeq) {
+  const int l = xcd_index();
+  if (l >= ne || e_v[l] < 0) return;
+  const int c1 = e_v[l], c2 = e_v[ne + l], k1 = e_sh[l], k2 = e_sh[ne + l];
+  const double g = (kh[c2] * div[c2] - kh[c1] * div[c1]) / e_leng[l] / erad;
+  const double gt = (double)e_nrs[l] * (kt[k2] * vort[k2] - kt[k1] * vort[k1]);
+  const double gv = gt / (e_dleng[l] * erad);
+  const double lap = g - gv;
+  out[l] = lap;
+  if (mode == 1) momeq[l] = momeq[l] + lap;
+  else if (mode == 2) momeq[l] = momeq[l] - lap;
+}
+
+// div_hx (swm_operators.f90:613-661, HTC) of an edge field
+__global__ __launch_bounds__(TPB) void ref_div(int nv, const int32_t *__restrict__ c_nnb,
+                                               const int32_t *__restrict__ c_ed,
+                                               const int8_t *__restrict__ c_ttg,
+                                               const double *__restrict__ c_area,
+                                               const double *__restrict__ e_dleng,
+                                               const double *__restrict__ f, double erad,
+                                               double *__restrict__ out) {
+  const int i = xcd_index();
+  if (i >= nv) return;
+  const int n = c_nnb[i];
+  double s = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const int l = c_ed[j * nv + i];
+    s = s + (double)c_ttg[j * nv + i] * f[l] * e_dleng[l];
+  }
+  out[i] = s / c_area[i] / erad;
+}
+
+// rel_vort_tr (swm_operators.f90:780-827, HTC) of an edge field
+__global__ __launch_bounds__(TPB) void ref_rvort(int nt, const int32_t *__restrict__ t_ed,
+                                                 const int8_t *__restrict__ t_tg,
+                                                 const double *__restrict__ t_area,
+                                                 const double *__restrict__ e_leng,
+                                                 const double *__restrict__ f, double erad,
+                                                 double *__restrict__ out) {
+  const int k = xcd_index();
+  if (k >= nt || t_ed[k] < 0) return;
+  double z = 0.0;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const int ed = t_ed[l * nt + k];
+    z = z + f[ed] * e_leng[ed] * (double)t_tg[l * nt + k];
+  }
+  out[k] = z / t_area[k] / erad;
+}
+
 // ode_rk4 combinations (swm.f90:2647-2678)
 __global__ __launch_bounds__(TPB) void ref_rk_stage(int n, const double *__restrict__ y,
                                                     const double *__restrict__ k, double dt,
@@ -587,6 +764,18 @@ static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, do
                                         s->o.testcase >= 2, s->f_eta, s->f_zeta, s->f_h_tr,
                                         s->f_q_tr, s->flag);
   ref_edge_q<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_sh, s->f_q_tr, s->f_q_ed);
+  if (s->sc_apvm) {
+    // grad q to edges and triangles, v at cells, then the APVM PV at edges
+    // (tendency, swm.f90:2481-2492, :2516, :2556-2559)
+    ref_qgrad_ed<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_sh, s->e_nrs, s->e_dleng, s->f_q_tr, s->o.erad,
+                                               s->f_qg_ed);
+    ref_perot_tr<<<grid_for(NT), TPB, 0, st>>>(NT, NE, s->t_ed, s->t_tg, s->e_c, s->t_c, s->e_leng,
+                                               s->t_area, s->f_qg_ed, s->f_qg_tr);
+    ref_perot_hx<<<grid_for(nv), TPB, 0, st>>>(nv, NE, s->c_nnb, s->c_ed, s->c_ttg, s->e_c, s->c_p,
+                                               s->e_dleng, s->c_area, u, s->f_vhx);
+    ref_apvm_q<<<grid_for(NE), TPB, 0, st>>>(NE, NT, nv, s->e_sh, s->e_v, s->f_qg_tr, s->f_vhx,
+                                             s->sc.pvspar, s->o.dt, s->f_q_ed);
+  }
   ref_cell<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area, s->e_leng,
                                          s->e_dleng, u, s->f_uh, h, s->bt, s->o.erad, s->o.grav,
                                          s->f_divuh, masseq, s->f_divu, s->f_ke, s->f_ghbk);
@@ -594,6 +783,24 @@ static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, do
                                              s->c_nnb, s->c_ed, s->c_ttg, s->c_tnr, s->c_trskw,
                                              s->f_ghbk, s->f_uh, s->f_q_ed, s->o.erad,
                                              s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
+  if (s->sc_diff || s->sc_hyper) {
+    // diffusion_ed of u (momeq + lapu with K2), then with K4 diffusion_ed of
+    // lapu's divergence and vorticity (momeq - lap_lapu) (swm.f90:2575-2590);
+    // testcase <= 1 leaves momeq zero
+    const int live = s->o.testcase > 1;
+    ref_lap<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, s->e_sh, s->e_nrs, s->e_leng, s->e_dleng, s->dif_hx,
+                                          s->dif_tr, s->f_divu, s->f_zeta, s->o.erad,
+                                          (live && s->sc_diff) ? 1 : 0, s->f_lapu, momeq);
+    if (s->sc_hyper) {
+      ref_div<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area, s->e_dleng,
+                                            s->f_lapu, s->o.erad, s->f_divl);
+      ref_rvort<<<grid_for(NT), TPB, 0, st>>>(NT, s->t_ed, s->t_tg, s->t_area, s->e_leng, s->f_lapu,
+                                              s->o.erad, s->f_vortl);
+      ref_lap<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, s->e_sh, s->e_nrs, s->e_leng, s->e_dleng,
+                                            s->dif_hx, s->dif_tr, s->f_divl, s->f_vortl, s->o.erad,
+                                            live ? 2 : 0, s->f_lap2, momeq);
+    }
+  }
 }
 
 static void launch_ref_step(imx_swm *s) {
@@ -1077,6 +1284,8 @@ static void set_href(imx_swm *s, const double *h) {
   s->href_set = true;
 }
 
+int swm_stencil_rings(const imx_swm *s) { return 1 + (s->sc_apvm ? 1 : 0) + (s->sc_hyper ? 1 : 0); }
+
 static void drop_graph(imx_swm *s) {
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
@@ -1312,8 +1521,12 @@ int imx_swm_get_field(imx_swm *s, int32_t which, double *out) {
     case IMX_FIELD_UHQ_PERP: src = s->f_uhq; kind = 1; break;
     case IMX_FIELD_DIVU: src = s->f_divu; kind = 0; break;
     case IMX_FIELD_ZETA: src = s->f_zeta; kind = 2; break;
+    case IMX_FIELD_Q_GRAD_ED: src = s->sc_apvm ? s->f_qg_ed : nullptr; kind = 1; break;
+    case IMX_FIELD_LAPU: src = (s->sc_diff || s->sc_hyper) ? s->f_lapu : nullptr; kind = 1; break;
+    case IMX_FIELD_LAP_LAPU: src = s->sc_hyper ? s->f_lap2 : nullptr; kind = 1; break;
     default: return imx::set_error(IMX_EINVAL, "imx_swm_get_field: unknown field");
   }
+  if (!src) return imx::set_error(IMX_EINVAL, "imx_swm_get_field: the field's scheme option is off");
   const int n = kind == 0 ? s->nv : kind == 1 ? s->ne : s->nt;
   const int ni = kind == 0 ? s->nv : kind == 1 ? s->NE : s->NT;
   const std::vector<int32_t> &perm =
@@ -1321,6 +1534,83 @@ int imx_swm_get_field(imx_swm *s, int32_t which, double *out) {
   std::vector<double> b(ni);
   HIPTRY(hipMemcpy(b.data(), src, ni * sizeof(double), hipMemcpyDeviceToHost), "get_field");
   for (int i = 0; i < n; ++i) out[i] = b[perm[i]];
+  return IMX_OK;
+}
+
+int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc) {
+  if (!s || !d || !sc) return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: NULL argument");
+  if (s->fast)
+    return imx::set_error(IMX_ENOTSUP, "imx_swm_set_scheme: scheme options run on FP64_REF handles");
+  if (d->nv != s->nv || d->ne != s->ne || d->nt != s->nt)
+    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: not the handle's mesh");
+  if (!(sc->k2 >= 0.0) || !(sc->k4 >= 0.0) || !std::isfinite(sc->k2) || !std::isfinite(sc->k4) ||
+      !std::isfinite(sc->pvspar) || sc->diffus < IMX_COEF_CONST || sc->diffus > IMX_COEF_DIAM ||
+      sc->hyperdiffus < IMX_COEF_CONST || sc->hyperdiffus > IMX_COEF_DIAM)
+    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: bad option value");
+  if ((sc->k2 > 0.0 || sc->k4 > 0.0) && (!d->v_tr || !d->tr_c || !d->v_nb))
+    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: diffusion needs v_tr, v_nb and tr_c");
+  if (sc->apvm && (!d->ed_c || !d->tr_c || !d->v_p))
+    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: APVM needs ed_c, tr_c and v_p");
+  const int nv = s->nv, NE = s->NE, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  imx::Alloc &A = s->A;
+  if (!s->e_nrs && (sc->apvm || sc->k2 > 0.0 || sc->k4 > 0.0)) {
+    // geometry of the options, internal numbering, points component-major
+    std::vector<int8_t> nrs(NE, 1);
+    std::vector<double> ec((size_t)3 * NE, 0.0), tc((size_t)3 * NT, 0.0), cp((size_t)3 * nv, 0.0);
+    for (int eo = 0; eo < d->ne; ++eo) {
+      const int e = en[eo];
+      const double *a = d->v_p + 3 * (d->ed_v[2 * eo] - 1), *b = d->v_p + 3 * (d->ed_v[2 * eo + 1] - 1);
+      double nr[3];
+      imx::cross3(a, b, nr);  // ed%nr up to its positive norm (smeshpack.f90:725-726)
+      const double *t1 = d->tr_c + 3 * (d->ed_sh[2 * eo] - 1), *t2 = d->tr_c + 3 * (d->ed_sh[2 * eo + 1] - 1);
+      double dp = 0.0;
+      for (int q = 0; q < 3; ++q) dp = dp + nr[q] * (t2[q] - t1[q]);
+      nrs[e] = (int8_t)(std::signbit(dp) ? -1 : 1);
+      for (int q = 0; q < 3; ++q) ec[(size_t)q * NE + e] = d->ed_c[3 * eo + q];
+    }
+    for (int ko = 0; ko < d->nt; ++ko)
+      for (int q = 0; q < 3; ++q) tc[(size_t)q * NT + tn[ko]] = d->tr_c[3 * ko + q];
+    for (int co = 0; co < nv; ++co)
+      for (int q = 0; q < 3; ++q) cp[(size_t)q * nv + cn[co]] = d->v_p[3 * co + q];
+    s->e_nrs = A.put(nrs);
+    s->e_c = A.put(ec);
+    s->t_c = A.put(tc);
+    s->c_p = A.put(cp);
+    s->f_qg_ed = A.get<double>(NE);
+    s->f_qg_tr = A.get<double>((size_t)3 * NT);
+    s->f_vhx = A.get<double>((size_t)3 * nv);
+    s->f_lapu = A.get<double>(NE);
+    s->f_lap2 = A.get<double>(NE);
+    s->f_divl = A.get<double>(nv);
+    s->f_vortl = A.get<double>(NT);
+    s->dif_hx = A.get<double>(nv);
+    s->dif_tr = A.get<double>(NT);
+    if (A.err != hipSuccess) return imx::hip_fail(A.err, "imx_swm_set_scheme");
+  }
+  if (sc->k2 > 0.0 || sc->k4 > 0.0) {
+    std::vector<double> trhx_v, hxtr_v, trskw_v;
+    const double *trhx = d->tr_trhx_area;
+    if (!trhx) {
+      trhx_v.assign((size_t)3 * d->nt, 0.0);
+      hxtr_v.assign((size_t)MX * d->nv, 0.0);
+      trskw_v.assign((size_t)MX * MX * d->nv, 0.0);
+      imx::gridprop(*d, trhx_v.data(), hxtr_v.data(), trskw_v.data());
+      trhx = trhx_v.data();
+    }
+    std::vector<double> hx, tr;
+    imx::diffusion_coef(*d, trhx, sc->k2, sc->diffus, sc->k4, sc->hyperdiffus, hx, tr);
+    std::vector<double> hi(nv, 0.0), ti(NT, 0.0);
+    for (int co = 0; co < nv; ++co) hi[cn[co]] = hx[co];
+    for (int ko = 0; ko < d->nt; ++ko) ti[tn[ko]] = tr[ko];
+    HIPTRY(hipMemcpy(s->dif_hx, hi.data(), nv * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->dif_tr, ti.data(), NT * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+  }
+  s->sc = *sc;
+  s->sc_apvm = sc->apvm != 0;
+  s->sc_diff = sc->k2 != 0.0;
+  s->sc_hyper = sc->k4 != 0.0;
+  imx::drop_graph(s);  // the captured step holds the previous kernel sequence
   return IMX_OK;
 }
 

@@ -130,6 +130,31 @@ typedef struct imx_swm  imx_swm;   /* one SWM instance; owns device memory */
 #define IMX_FIELD_UHQ_PERP 10  /* [ne] */
 #define IMX_FIELD_DIVU     11  /* [nv] */
 #define IMX_FIELD_ZETA     12  /* [nt] */
+#define IMX_FIELD_Q_GRAD_ED 13 /* [ne]  grad_tr2ed(q_tr), with APVM            */
+#define IMX_FIELD_LAPU     14  /* [ne]  diffusion_ed of u, with K2 or K4       */
+#define IMX_FIELD_LAP_LAPU 15  /* [ne]  hyperdiffusion_ed of u, with K4        */
+
+/* Scheme options of par/swm.par beyond the trsk10/HTC defaults
+   (swmpars, src/swm_data.f90:413-424, :548-551), for FP64_REF handles:
+     apvm    pv_stab = apvm: q_ed -= pvspar dt (grad q) . v    (:1465-1502)
+     k2      K2_max, diffusion momeq += div(K grad) u           (:855-895)
+     k4      K4_max, hyperdiffusion momeq -= (div(K grad))^2 u  (:930-975)
+   with the coefficient function of each (const, align: smoothed alignment
+   index, diam: cell diameter; src/swm_operators.f90:977-1072) set up as
+   swm_tests does (src/swm.f90:170-210): with k4 > 0 both Laplacian terms
+   use the square roots of the hyperdiffusion coefficients. The APVM term
+   reads the handle's time step. All zero: the defaults. */
+#define IMX_COEF_CONST 0
+#define IMX_COEF_ALIGN 1
+#define IMX_COEF_DIAM  2
+typedef struct imx_scheme {
+  int32_t apvm;         /* 0/1                                                */
+  double  pvspar;       /* APVM parameter                                     */
+  double  k2;           /* K2_max, 0 = no diffusion                           */
+  int32_t diffus;       /* IMX_COEF_*                                         */
+  double  k4;           /* K4_max, 0 = no hyperdiffusion                      */
+  int32_t hyperdiffus;  /* IMX_COEF_*                                         */
+} imx_scheme;
 
 /* ---- errors -------------------------------------------------------------- */
 const char *imx_last_error(void);
@@ -182,6 +207,10 @@ int  imx_ic_spectral(const imx_mesh_desc *d, uint64_t seed, int32_t lmax,
 /* ---- the time step (module swm) ------------------------------------------- */
 void imx_swm_default_opts(imx_swm_opts *o);
 int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out);
+/* Scheme options (imx_scheme) of an FP64_REF handle; d is the mesh the
+   handle was created on (the coefficients and the APVM reconstruction
+   read its geometry). IMX_ENOTSUP on fused handles. */
+int  imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc);
 int  imx_swm_set_state(imx_swm *s, const double *h, const double *u);
 int  imx_swm_get_state(imx_swm *s, double *h, double *u);
 /* subroutine tendency(h, u, masseq, momeq)       src/swm.f90:2416-2598 */

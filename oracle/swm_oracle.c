@@ -496,7 +496,7 @@ void oracle_diagnostics(const imx_mesh_desc *d, const oracle_params *p, const do
 static double arclen(const double *p, const double *q) {
   double d = 0.0;
   for (int i = 0; i < 3; ++i) d = d + (p[i] + q[i]) * (p[i] + q[i]);
-  if (d == 0.0) return 3.141592653589793;
+  if (d == 0.0) return 3.1415927410125732; /* pi of constants.f90, single-rounded */
   if (d >= 4.0) return 0.0;
   return 2.0 * atan(sqrt((4.0 - d) / d));
 }

@@ -1,0 +1,102 @@
+"""Scheme options of the FP64_REF path on the GPU (imx_swm_set_scheme):
+APVM, diffusion and hyperdiffusion with their coefficient functions, bit for
+bit against the reference's own golden vectors (tests/golden/make_golden.py)
+and against the oracle on a larger grid; the normal-mode Jacobian with the
+widened stencil."""
+import numpy as np
+import pytest
+
+from imodel_amd import capi
+from imodel_amd.swm import Mesh, Swm
+from oracle.oracle import ERAD_REF, Oracle
+from tests.test_gpu_jacobian import per_column
+from tests.test_oracle_golden import SCHEMES
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+def set_scheme(s, sc):
+    s.set_scheme(**{k: v for k, v in sc.items() if k != "dt"})
+
+
+@pytest.mark.parametrize("sfc", [False, True])
+@pytest.mark.parametrize("name", sorted(SCHEMES))
+def test_scheme_tendency_and_steps_match_reference(golden, name, sfc):
+    say("scheme vs reference", name, "sfc", sfc)
+    mesh = Mesh(golden("ref_mesh_g3.bin"))
+    r = golden(f"ref_run_g3_{name}.bin")
+    with Swm(mesh, precision="fp64_ref", dt=900.0, use_sfc=sfc) as s:
+        set_scheme(s, SCHEMES[name])
+        m, f = s.tendency(r["h0"], r["u0"])
+        assert np.array_equal(m, r["masseq"]) and np.array_equal(f, r["momeq"])
+        for k in ("q_grad_ed", "lapu", "lap_lapu", "q_ed", "uhq_perp", "divu", "zeta"):
+            if k in r:
+                assert np.array_equal(s.field(k), r[k]), k
+        s.set_state(r["h0"], r["u0"])
+        s.step(3)
+        h, u = s.get_state()
+    assert np.array_equal(h, r["h"]) and np.array_equal(u, r["u"])
+
+
+# glevel-5 cases: with K4 > 0 the Laplacian term also runs with sqrt(K4)
+# (the reference's set-up), so K4 is kept small enough for dt = 450 s
+G5 = {"mix": {"k2": 1e5, "diffus": "diam", "k4": 1e14, "hyperdiffus": "align", "pv": "apvm", "pvspar": 0.5},
+      "k2align": {"k2": 1e5, "diffus": "align"}}
+
+
+@pytest.mark.parametrize("name", sorted(G5))
+def test_scheme_g5_matches_oracle(name):
+    """glevel 5 (10242 cells), spectral state, 12 steps with the graph."""
+    say("scheme g5 vs oracle", name)
+    m = Mesh.icos(5)
+    h, u = m.spectral(11)
+    sc = dict(G5[name], dt=450.0)
+    o = Oracle(m.arrays, scheme=sc)
+    ho, uo = o.step(h, u, 450.0, 12)
+    with Swm(m, precision="fp64_ref", dt=450.0) as s:
+        set_scheme(s, sc)
+        s.set_state(h, u)
+        s.step(12)
+        hg, ug = s.get_state()
+    assert np.all(np.isfinite(ho)) and np.all(np.isfinite(uo))
+    assert np.array_equal(hg, ho) and np.array_equal(ug, uo)
+
+
+@pytest.mark.parametrize("name", ["apvm", "mix"])
+def test_scheme_jacobian_equals_per_column(name):
+    say("scheme jacobian g2", name)
+    m = Mesh.icos(2)
+    h, u = m.spectral(5)
+    sc = dict(SCHEMES[name], dt=900.0)
+    s = Swm(m, precision="fp64_ref", dt=900.0, erad=ERAD_REF)
+    set_scheme(s, sc)
+    pert = float(np.mean(m.arrays["hx_areag"]))
+    J, nev = s.jacobian(h, u)
+    M = per_column(Oracle(m.arrays, scheme=sc), h, u, pert)
+    assert np.array_equal(J.toarray(), M)
+    assert nev < m.nv + m.ne
+
+
+def test_scheme_errors():
+    say("scheme errors")
+    m = Mesh.icos(2)
+    with Swm(m, precision="fp32", dt=900.0) as s:
+        with pytest.raises(capi.ImxError, match="FP64_REF"):
+            s.set_scheme(pv="apvm")
+    with Swm(m, precision="fp64_ref", dt=900.0) as s:
+        with pytest.raises(capi.ImxError, match="bad option"):
+            s.set_scheme(k2=-1.0)
+        s.set_scheme(k2=1e5)
+        h, u = m.spectral(3)
+        s.tendency(h, u)
+        with pytest.raises(capi.ImxError, match="option is off"):
+            s.field("q_grad_ed")
+        # back to the defaults: the plain trsk10 tendency
+        s.set_scheme()
+        f0 = s.tendency(h, u)[1]
+    with Swm(m, precision="fp64_ref", dt=900.0) as p:
+        assert np.array_equal(p.tendency(h, u)[1], f0)
