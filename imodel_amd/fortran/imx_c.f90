@@ -27,6 +27,16 @@ module imx_c
     real(c_double) :: erad, grav, omega, dt, h_ref
   end type imx_swm_opts
 
+  ! imx_scheme: APVM, diffusion, hyperdiffusion (IMX_COEF_*: const, align, diam)
+  integer(c_int32_t), parameter :: IMX_COEF_CONST = 0, IMX_COEF_ALIGN = 1, IMX_COEF_DIAM = 2
+  type, bind(C) :: imx_scheme
+    integer(c_int32_t) :: apvm
+    real(c_double) :: pvspar, k2
+    integer(c_int32_t) :: diffus
+    real(c_double) :: k4
+    integer(c_int32_t) :: hyperdiffus
+  end type imx_scheme
+
   interface
     function imx_last_error() bind(C, name='imx_last_error') result(p)
       import :: c_ptr
@@ -133,6 +143,13 @@ module imx_c
       type(imx_mesh_desc) :: d
       type(imx_swm_opts) :: o
       type(c_ptr) :: out
+      integer(c_int) :: rc
+    end function
+    function imx_swm_set_scheme(s, d, sc) bind(C, name='imx_swm_set_scheme') result(rc)
+      import :: c_int, c_ptr, imx_mesh_desc, imx_scheme
+      type(c_ptr), value :: s
+      type(imx_mesh_desc) :: d
+      type(imx_scheme) :: sc
       integer(c_int) :: rc
     end function
     function imx_swm_set_state(s, h, u) bind(C, name='imx_swm_set_state') result(rc)

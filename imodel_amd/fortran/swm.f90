@@ -32,6 +32,7 @@ contains
     type(c_ptr), intent(in) :: meshp
     integer, intent(in) :: glevel
     type(imx_swm_opts) :: o
+    type(imx_scheme) :: sc
     real(c_double) :: d0(IMX_NDIAG), d(IMX_NDIAG)
     real(r8), pointer :: leng(:)
     real(r8) :: maxvel, minvdist, time
@@ -59,6 +60,19 @@ contains
     swmname = "swm_tc"//trim(adjustl(atmp))
     write(atmp, '(i8)') nint(dt)
     swmname = trim(swmname)//"_dt"//trim(adjustl(atmp))//"_HTC_trsk10_areageo"
+    ! scheme-option suffixes of swmpars (src/swm_data.f90:658-683)
+    if (trim(pv_stab) /= 'none') then
+      write(atmp, '(f5.2)') real(pvspar)
+      swmname = trim(swmname)//"_pvs"//trim(adjustl(trim(pv_stab)))//trim(adjustl(trim(atmp)))
+    end if
+    if (k2_max > 0) then
+      write(atmp, '(f10.3)') real(dlog(k2_max)/dlog(10._r8))
+      swmname = trim(swmname)//"_"//trim(diffus)//"_diffusion_10to"//trim(adjustl(trim(atmp)))
+    end if
+    if (k4_max > 0) then
+      write(atmp, '(f10.3)') real(dlog(k4_max)/dlog(10._r8))
+      swmname = trim(swmname)//"_"//trim(hyperdiffus)//"_hyperdiffusion_10to"//trim(adjustl(trim(atmp)))
+    end if
     write(atmp, '(i8)') glevel
     meshname = trim(mkind)//"_"//trim(mpos)//"_"//trim(moptm)//"_"//trim(adjustl(atmp))
 
@@ -72,6 +86,19 @@ contains
     o%omega = omega
     o%dt = dt
     call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
+    if (scheme_options()) then
+      if (trim(precision) /= 'fp64_ref') then
+        print*, "ERROR: APVM / diffusion / hyperdiffusion run on the fp64_ref path (par/swm_gpu.par)"
+        stop 1
+      end if
+      sc%apvm = merge(1, 0, trim(pv_stab) == 'apvm')
+      sc%pvspar = pvspar
+      sc%k2 = max(k2_max, 0._r8)
+      sc%diffus = max(coef_code(diffus), 0)
+      sc%k4 = max(k4_max, 0._r8)
+      sc%hyperdiffus = max(coef_code(hyperdiffus), 0)
+      call imx_check(imx_swm_set_scheme(handle, mdesc, sc), 'imx_swm_set_scheme')
+    end if
     call imx_check(imx_swm_set_state(handle, h, u), 'imx_swm_set_state')
     if (normal_modes == 1) then
       call swm_normal_mode_analysis()

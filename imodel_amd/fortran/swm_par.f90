@@ -3,7 +3,8 @@
 !   par/mesh.par  as getparameters  (src/smeshpack.f90:7343-7416), argv
 !                 [meshpar] [nv] override as in the reference
 !   par/swm.par   as swmpars        (src/swm_data.f90:338-693), the
-!                 HTC/trsk10/geo/no-diffusion/no-APVM subset this build runs
+!                 HTC/trsk10/geo subset this build runs, with APVM,
+!                 diffusion and hyperdiffusion on the fp64_ref path
 !   par/swm_gpu.par (new, optional): precision, seed, SFC and graph switches
 !=======================================================================
 module swm_par
@@ -146,8 +147,13 @@ contains
           trim(gradmtd) /= 'trsk') call unsupported('method wrapper', mtdwrapper)
     end if
     if (areamtd(1:3) /= 'geo') call unsupported('area method', areamtd)
-    if (k2_max /= 0._r8 .or. k4_max /= 0._r8) call unsupported('diffusion', diffus)
-    if (trim(pv_stab) /= 'none') call unsupported('PV stabilisation', pv_stab)
+    if (k2_max > 0._r8) then
+      if (coef_code(diffus) < 0) call unsupported('diffusion coefficient function', diffus)
+    end if
+    if (k4_max > 0._r8) then
+      if (coef_code(hyperdiffus) < 0) call unsupported('hyperdiffusion coefficient function', hyperdiffus)
+    end if
+    if (trim(pv_stab) /= 'none' .and. trim(pv_stab) /= 'apvm') call unsupported('PV stabilisation', pv_stab)
     if (inopv /= 0) call unsupported('level model (noPV)', 'noPV=1')
     if (test_lterror /= 0) call unsupported('local truncation error test', '1')
     if (testcase /= 2 .and. testcase /= 100) call unsupported('test case (2 or 100)', 'other')
@@ -197,6 +203,22 @@ contains
       p = 2
     end select
   end function precision_code
+
+  ! diffusion coefficient function -> IMX_COEF_* (-1: unknown)
+  integer(c_int32_t) function coef_code(name) result(c)
+    character(len=*), intent(in) :: name
+    select case (trim(name))
+    case ('const'); c = 0
+    case ('align'); c = 1
+    case ('diam'); c = 2
+    case default; c = -1
+    end select
+  end function coef_code
+
+  ! APVM, diffusion or hyperdiffusion selected
+  logical function scheme_options() result(on)
+    on = trim(pv_stab) == 'apvm' .or. k2_max > 0._r8 .or. k4_max > 0._r8
+  end function scheme_options
 
   subroutine unsupported(what, val)
     character(len=*), intent(in) :: what, val

@@ -180,3 +180,44 @@ def test_fortran_driver_plot_files():
     h, u = o.step(h0, u0, 900.0, 96)
     assert np.array_equal(hp, m.remap_ll(h, 0))
     assert np.array_equal(qp, m.remap_ll(o.field("q_tr"), 1))
+
+
+def test_fortran_driver_scheme_options_bitexact():
+    """APVM, diffusion (align) and hyperdiffusion (diam) read from par/swm.par
+    as swmpars does: the file names carry the reference's suffixes and the
+    final state equals the oracle with the same options bit for bit."""
+    print("[gpu-case] fortran imodel g3 tc2 fp64_ref apvm+diffusion", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 48, "fp64_ref")
+        p = os.path.join(td, "par", "swm.par")
+        swm = open(p).read().splitlines()
+        swm[24] = "1e5 align"
+        swm[26] = "1e15 diam"
+        swm[32] = "apvm 0.5"
+        open(p, "w").write("\n".join(swm) + "\n")
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        base = ("swm_tc2_dt900_HTC_trsk10_areageo_pvsapvm0.50_align_diffusion_10to5.000"
+                "_diam_hyperdiffusion_10to15.000")
+        st = imxbin.read(os.path.join(td, "data", f"{base}_state_icos_eqs_nopt_3.bin"))
+        assert os.path.exists(os.path.join(td, "data", f"{base}_evolution_icos_eqs_nopt_3.txt"))
+    m = Mesh.icos(3)
+    h0, u0 = m.tc2()
+    o = Oracle(m.arrays, scheme={"pv": "apvm", "pvspar": 0.5, "dt": 900.0, "k2": 1e5, "diffus": "align",
+                                 "k4": 1e15, "hyperdiffus": "diam"})
+    h, u = o.step(h0, u0, 900.0, 96)
+    assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
+
+
+def test_fortran_driver_scheme_needs_fp64_ref():
+    print("[gpu-case] fortran imodel scheme on fp32 refused", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 48, "fp32")
+        p = os.path.join(td, "par", "swm.par")
+        swm = open(p).read().splitlines()
+        swm[32] = "apvm 0.5"
+        open(p, "w").write("\n".join(swm) + "\n")
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode != 0 and "fp64_ref" in r.stdout
