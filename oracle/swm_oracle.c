@@ -9,9 +9,9 @@
  * built with -ffp-contract=off on the same inputs (pinned against golden
  * vectors written by oracle/_ref/ref_harness, see tests/golden/).
  *
- * Scope: HTC staggering, trsk10 wrapper, geodesic areas (the reference
- * defaults, par/swm.par), optionally with APVM, diffusion and
- * hyperdiffusion (oracle_scheme), i.e. the branches
+ * Scope: HTC staggering, geodesic areas, the trsk10 wrapper (the reference
+ * defaults, par/swm.par) or the pxt16 / gass18 wrappers, optionally with
+ * APVM, diffusion and hyperdiffusion (oracle_scheme), i.e. the branches
  *   scalar_hx2ed        src/swm_operators.f90:88-98
  *   scalar_elem_product src/swm_operators.f90:1546-1577
  *   vort_tr             src/swm_operators.f90:551-611
@@ -31,6 +31,12 @@
  *   hyperdiffusion_ed   src/swm_operators.f90:930-975
  *   grad_ed_tg          src/swm_operators.f90:738-777 (HTC :760-769)
  *   alignment_coef, diameter_coef  src/swm_operators.f90:977-1072
+ * pxt16: scalar_hx2tr bary (:265-269, bar_coord_tr src/smeshpack.f90:5200-5245),
+ *   scalar_tr2ed distance-weighted (:163-168), kinetic_energy_hx perhx
+ *   (:1172-1173), coriolis_ed pered HTC (:1330-1368)
+ * gass18: scalar_edtr2trcc (:286-346), scalar_tr2ed rhombi (:155-162),
+ *   q_ed = eta_ed / h_rhb (src/swm.f90:2469-2473), coriolis_ed gass
+ *   (:1379-1416, gethxedgeconnection src/smeshpack.f90:4127-4244)
  *   alignind            src/smeshpack.f90:6398-6490
  *   tendency            src/swm.f90:2416-2598
  *   ode_rk4             src/swm.f90:2600-2683
@@ -48,6 +54,95 @@
 
 /* epsilon(1.) of the reference (constants.f90:45) */
 static const double EPS_SINGLE = 1.1920928955078125e-07;
+
+static double arclen(const double *p, const double *q);
+
+/* cross_product (src/smeshpack.f90:6560-6578) */
+static void cross(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static double dot(const double *a, const double *b) {
+  double s = 0.0;
+  for (int q = 0; q < 3; ++q) s = s + a[q] * b[q];
+  return s;
+}
+
+/* bar_coord_tr of triangle k's circumcentre (src/smeshpack.f90:5200-5245) */
+static void bary(const imx_mesh_desc *d, int k, double *b) {
+  const double *p = d->tr_c + 3 * k;
+  const double *p1 = d->v_p + 3 * (d->tr_v[3 * k] - 1), *p2 = d->v_p + 3 * (d->tr_v[3 * k + 1] - 1),
+               *p3 = d->v_p + 3 * (d->tr_v[3 * k + 2] - 1);
+  double c[3];
+  cross(p, p2, c);
+  b[0] = fabs(dot(c, p3));
+  cross(p, p3, c);
+  b[1] = fabs(dot(c, p1));
+  cross(p, p1, c);
+  b[2] = fabs(dot(c, p2));
+  const double area = b[0] + b[1] + b[2];
+  for (int i = 0; i < 3; ++i) b[i] = b[i] / area;
+}
+
+/* sphtriarea (src/smeshpack.f90:4710-4762) */
+static double sphtriarea(const double *p1, const double *p2, const double *p3) {
+  double t[3];
+  for (int q = 0; q < 3; ++q) t[q] = p1[q] - p2[q];
+  const double n12 = sqrt(dot(t, t));
+  for (int q = 0; q < 3; ++q) t[q] = p2[q] - p3[q];
+  const double n23 = sqrt(dot(t, t));
+  for (int q = 0; q < 3; ++q) t[q] = p1[q] - p3[q];
+  const double n13 = sqrt(dot(t, t));
+  const double eps100 = 1.1920928955078125e-07 / 100;
+  if (n12 < eps100 || n23 < eps100 || n13 < eps100) return 0.0;
+  const double a = arclen(p1, p2), b = arclen(p2, p3), c = arclen(p3, p1);
+  const double s = (a + b + c) / 2;
+  const double tmp = tan(s / 2) * tan((s - a) / 2) * tan((s - b) / 2) * tan((s - c) / 2);
+  if (tmp < 0) return 0.0;
+  return 4.0 * atan(sqrt(tmp));
+}
+
+/* ed%nr = v1 x v2 / |v1 x v2| (src/smeshpack.f90:725-726) */
+static void ed_nr(const imx_mesh_desc *d, int l, double *nr) {
+  const double *a = d->v_p + 3 * (d->ed_v[2 * l] - 1), *b = d->v_p + 3 * (d->ed_v[2 * l + 1] - 1);
+  cross(a, b, nr);
+  const double n = sqrt(dot(nr, nr));
+  for (int q = 0; q < 3; ++q) nr[q] = nr[q] / n;
+}
+
+/* gethxedgeconnection(ed1, ed2) (src/smeshpack.f90:4127-4244), 0-based ids:
+ * -2 when ed1 == ed2 (the reference's 0), the third edge of a triangle
+ * holding both, the edge linking them through a neighbouring triangle, or -1 */
+static int hxedgeconn(const imx_mesh_desc *d, int ed1, int ed2) {
+  if (ed1 == ed2) return -2;
+  const int t1 = d->ed_sh[2 * ed1] - 1, t2 = d->ed_sh[2 * ed1 + 1] - 1;
+  int tr = -1;
+  for (int i = 0; i < 3; ++i) {
+    if (ed2 == d->tr_ed[3 * t1 + i] - 1) { tr = t1; break; }
+    if (ed2 == d->tr_ed[3 * t2 + i] - 1) { tr = t2; break; }
+  }
+  if (tr >= 0) {
+    int e3 = -1;
+    for (int i = 0; i < 3; ++i) {
+      const int e = d->tr_ed[3 * tr + i] - 1;
+      if (e != ed1 && e != ed2) e3 = e;
+    }
+    if (e3 >= 0) return e3;
+  }
+  for (int j = 0; j < 2; ++j) {
+    const int trj = d->ed_sh[2 * ed2 + j] - 1;
+    for (int i = 0; i < 3; ++i) {
+      const int et = d->tr_ed[3 * trj + i] - 1;
+      if (et == ed2) continue;
+      const int tr2 = (d->ed_sh[2 * et] - 1 == trj) ? d->ed_sh[2 * et + 1] - 1 : d->ed_sh[2 * et] - 1;
+      for (int l = 0; l < 3; ++l)
+        if (ed1 == d->tr_ed[3 * tr2 + l] - 1) return et;
+    }
+  }
+  return -1;
+}
 
 /* dsign(1, x) */
 static double sgn1(double x) { return x < 0.0 || (x == 0.0 && signbit(x)) ? -1.0 : 1.0; }
@@ -161,18 +256,56 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
     eta[k] = e + 2.0 * p->omega * sin(d->tr_clat[k]);
   }
 
-  /* scalar_hx2tr (trsk) */
+  const int wrap = p->scheme ? p->scheme->wrap : 0;
+  if (wrap == 2) {
+    /* scalar_edtr2trcc(h_ed) (gass18) */
 #pragma omp parallel for schedule(static)
-  for (int k = 0; k < nt; ++k) {
-    double s = 0.0;
-    for (int i = 0; i < 3; ++i) s = s + p->trhx_area[3 * k + i] * h[d->tr_v[3 * k + i] - 1];
-    h_tr[k] = s / d->tr_areag[k];
+    for (int k = 0; k < nt; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < 3; ++i) {
+        const int ed = d->tr_ed[3 * k + i] - 1;
+        const double a = sphtriarea(d->tr_c + 3 * k, d->v_p + 3 * (d->ed_v[2 * ed] - 1),
+                                    d->v_p + 3 * (d->ed_v[2 * ed + 1] - 1));
+        s = s + a * h_ed[ed];
+      }
+      h_tr[k] = s / d->tr_areag[k];
+    }
+  } else if (wrap == 1) {
+    /* scalar_hx2tr (bary, pxt16) */
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nt; ++k) {
+      double b[3], s = 0.0;
+      bary(d, k, b);
+      for (int i = 0; i < 3; ++i) s = s + b[i] * h[d->tr_v[3 * k + i] - 1];
+      h_tr[k] = s;
+    }
+  } else {
+    /* scalar_hx2tr (trsk) */
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < nt; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < 3; ++i) s = s + p->trhx_area[3 * k + i] * h[d->tr_v[3 * k + i] - 1];
+      h_tr[k] = s / d->tr_areag[k];
+    }
   }
 
-  /* scalar_tr2ed(eta) */
-#pragma omp parallel for schedule(static)
-  for (int l = 0; l < ne; ++l)
-    eta_ed[l] = 0.5 * (eta[d->ed_sh[2 * l] - 1] + eta[d->ed_sh[2 * l + 1] - 1]);
+  /* scalar_tr2ed: simple average (trsk10), distance-weighted (pxt16 on HTC),
+     area-weighted rhombi (gass18) */
+#define TR2ED(f, out)                                                                  \
+  for (int l = 0; l < ne; ++l) {                                                       \
+    const int k1 = d->ed_sh[2 * l] - 1, k2 = d->ed_sh[2 * l + 1] - 1;                  \
+    if (wrap == 0) {                                                                   \
+      out[l] = 0.5 * (f[k1] + f[k2]);                                                  \
+    } else if (wrap == 2) {                                                            \
+      const double a1 = d->tr_areag[k1], a2 = d->tr_areag[k2];                          \
+      out[l] = (a1 * f[k1] + a2 * f[k2]) / (a1 + a2);                                  \
+    } else {                                                                           \
+      const double d1 = arclen(d->ed_c + 3 * l, d->tr_c + 3 * k1);                     \
+      const double d2 = arclen(d->ed_c + 3 * l, d->tr_c + 3 * k2);                     \
+      out[l] = (d2 * f[k1] + d1 * f[k2]) / (d1 + d2);                                  \
+    }                                                                                  \
+  }
+  TR2ED(eta, eta_ed)
 
   /* PV at triangles: scalar_elem_divide, with the reference's zero guard */
   if (p->testcase < 2) {
@@ -189,10 +322,17 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
   }
   if (bad) return -4;
 
-  /* scalar_tr2ed(q) */
-#pragma omp parallel for schedule(static)
-  for (int l = 0; l < ne; ++l)
-    q_ed[l] = 0.5 * (q_tr[d->ed_sh[2 * l] - 1] + q_tr[d->ed_sh[2 * l + 1] - 1]);
+  if (wrap == 2) {
+    /* PV on rhombi: eta_ed / h_rhb, h_rhb = scalar_tr2ed(h_tr) */
+    TR2ED(h_tr, w->h_rhb)
+    for (int l = 0; l < ne; ++l) {
+      if (fabs(w->h_rhb[l]) < EPS_SINGLE / 100000.0) return -4;
+      q_ed[l] = eta_ed[l] / w->h_rhb[l];
+    }
+  } else {
+    TR2ED(q_tr, q_ed)
+  }
+#undef TR2ED
 
   const oracle_scheme *sc = p->scheme;
   const int apvm = sc && sc->apvm;
@@ -222,7 +362,7 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
 
   /* vector_edhx2hx_perot(u): used by APVM, dead for plain trsk10 (computed on
      request, as the reference does it every step) */
-  if (w->perot || apvm) {
+  if (w->perot || apvm || wrap == 1) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nv; ++i) {
       double v[3] = {0.0, 0.0, 0.0};
@@ -259,9 +399,13 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
   }
   for (int i = 0; i < nv; ++i) masseq[i] = -divuh[i];
 
-  /* kinetic_energy_hx (trsk) and the Bernoulli potential g(h+b)+K */
+  /* kinetic_energy_hx (trsk, or perhx for pxt16) and the Bernoulli potential g(h+b)+K */
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < nv; ++i) {
+    if (wrap == 1) {
+      ke[i] = dot(vhx + 3 * i, vhx + 3 * i) / 2.0;
+      continue;
+    }
     double k = 0.0, cell_area = 0.0;
     for (int l = 0; l < d->v_nnb[i]; ++l) {
       int ed = d->v_ed[MX * i + l] - 1;
@@ -295,7 +439,36 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
     }
   }
 
-  /* coriolis_ed (trsk, PV form) */
+  /* coriolis_ed pered (pxt16, HTC): Perot reconstruction of uh q at the
+     edge's two cells, averaged, projected, minus its ed%nr component */
+  if (wrap == 1) {
+#pragma omp parallel for schedule(static)
+    for (int l = 0; l < ne; ++l) {
+      double vv[2][3];
+      for (int ic = 0; ic < 2; ++ic) {
+        const int node = d->ed_v[2 * l + ic] - 1;
+        const double *pc = d->v_p + 3 * node;
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int j = 0; j < d->v_nnb[node]; ++j) {
+          const int l2 = d->v_ed[MX * node + j] - 1;
+          const double qtmp = 0.5 * (q_ed[l] + q_ed[l2]);
+          const double signcor = (double)d->hx_ttgout[MX * node + j];
+          for (int q = 0; q < 3; ++q)
+            v[q] = v[q] + signcor * uh[l2] * qtmp * (d->ed_c[3 * l2 + q] - pc[q]) * d->edhx_leng[l2];
+        }
+        for (int q = 0; q < 3; ++q) v[q] = v[q] / d->hx_areag[node];
+        proj_sphere(v, pc);
+        for (int q = 0; q < 3; ++q) vv[ic][q] = v[q];
+      }
+      double v[3], nr[3];
+      for (int q = 0; q < 3; ++q) v[q] = 0.5 * vv[0][q] + 0.5 * vv[1][q];
+      proj_sphere(v, d->ed_c + 3 * l);
+      ed_nr(d, l, nr);
+      uhq[l] = -dot(v, nr);
+    }
+  } else {
+  /* coriolis_ed (trsk, PV form; gass18: q of the rhombus linking the two
+     edges where there is one) */
 #pragma omp parallel for schedule(static)
   for (int l = 0; l < ne; ++l) {
     double s = 0.0;
@@ -306,11 +479,15 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
         int k = d->v_ed[MX * node + j] - 1;
         double sg = 1.0 * (double)d->hx_ttgout[MX * node + j] * (double)d->hx_tnrccw[MX * node + ei];
         double signcor = sg < 0 ? -1.0 : 1.0;
-        double qtmp = 0.5 * (q_ed[l] + q_ed[k]);
+        double qtmp;
+        const int e3 = wrap == 2 ? hxedgeconn(d, l, k) : -1;
+        if (e3 >= 0) qtmp = q_ed[e3];
+        else qtmp = 0.5 * (q_ed[l] + q_ed[k]);
         s = s + uh[k] * qtmp * d->edhx_leng[k] * p->trskw[MX * MX * node + MX * ei + j] * signcor;
       }
     }
     uhq[l] = s / d->ed_leng[l];
+  }
   }
   for (int l = 0; l < ne; ++l) momeq[l] = momeq[l] - uhq[l];
 
@@ -384,6 +561,7 @@ int oracle_fields_alloc(const imx_mesh_desc *d, oracle_fields *w, int with_perot
   w->gvort = calloc(ne, 8);
   w->div_lap = calloc(nv, 8);
   w->vort_lap = calloc(nt, 8);
+  w->h_rhb = calloc(ne, 8);
   w->massf = calloc(4 * nv, 8);
   w->momf = calloc(4 * ne, 8);
   w->h_new = calloc(nv, 8);
@@ -392,7 +570,7 @@ int oracle_fields_alloc(const imx_mesh_desc *d, oracle_fields *w, int with_perot
       !w->divuh || !w->ke_hx || !w->ghbk || !w->hbt || !w->grad_ghbk || !w->uhq_perp || !w->divu ||
       !w->zeta || !w->v_hx || !w->massf || !w->momf || !w->h_new || !w->u_new || !w->q_grad_ed ||
       !w->q_grad_tr || !w->lapu || !w->lap_lapu || !w->cdiv || !w->cvort || !w->gdiv || !w->gvort ||
-      !w->div_lap || !w->vort_lap)
+      !w->div_lap || !w->vort_lap || !w->h_rhb)
     return -2;
   return 0;
 }
@@ -402,7 +580,7 @@ void oracle_fields_free(oracle_fields *w) {
                     w->divuh, w->ke_hx, w->ghbk, w->hbt, w->grad_ghbk, w->uhq_perp, w->divu,
                     w->zeta, w->v_hx, w->massf, w->momf, w->h_new, w->u_new, w->q_grad_ed,
                     w->q_grad_tr, w->lapu, w->lap_lapu, w->cdiv, w->cvort, w->gdiv, w->gvort,
-                    w->div_lap, w->vort_lap};
+                    w->div_lap, w->vort_lap, w->h_rhb};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
   memset(w, 0, sizeof(*w));
 }
@@ -571,24 +749,34 @@ static void coef_hx(const imx_mesh_desc *d, double kmax, int kind, double *c) {
   free(a);
 }
 
-/* scalar_hx2tr (trsk, src/swm_operators.f90:258-263) */
-static void hx2tr(const imx_mesh_desc *d, const double *trhx_area, const double *f, double *out) {
+/* scalar_hx2tr (src/swm_operators.f90:258-269): trsk (area-weighted), or
+ * barycentric for pxt16 */
+static void hx2tr(const imx_mesh_desc *d, const double *trhx_area, int wrap, const double *f, double *out) {
   for (int k = 0; k < d->nt; ++k) {
     double s = 0.0;
-    for (int i = 0; i < 3; ++i) s = s + trhx_area[3 * k + i] * f[d->tr_v[3 * k + i] - 1];
-    out[k] = s / d->tr_areag[k];
+    if (wrap == 1) {
+      double b[3];
+      bary(d, k, b);
+      for (int i = 0; i < 3; ++i) s = s + b[i] * f[d->tr_v[3 * k + i] - 1];
+      out[k] = s;
+    } else {
+      for (int i = 0; i < 3; ++i) s = s + trhx_area[3 * k + i] * f[d->tr_v[3 * k + i] - 1];
+      out[k] = s / d->tr_areag[k];
+    }
   }
 }
 
-int oracle_diffusion_coef(const imx_mesh_desc *d, const double *trhx_area, double k2, int k2kind,
-                          double k4, int k4kind, double *dif_hx, double *dif_tr) {
+int oracle_diffusion_coef(const imx_mesh_desc *d, const double *trhx_area, int wrap, double k2,
+                          int k2kind, double k4, int k4kind, double *dif_hx, double *dif_tr) {
+  /* gass18: scalar_hx2tr stops ("Gassmann's scheme needs edge values") */
+  if (wrap == 2 && (k2 > 0.0 || k4 > 0.0)) return -5;
   if (k2 > 0.0) {
     coef_hx(d, k2, k2kind, dif_hx);
-    hx2tr(d, trhx_area, dif_hx, dif_tr);
+    hx2tr(d, trhx_area, wrap, dif_hx, dif_tr);
   }
   if (k4 > 0.0) {
     coef_hx(d, k4, k4kind, dif_hx);
-    hx2tr(d, trhx_area, dif_hx, dif_tr);
+    hx2tr(d, trhx_area, wrap, dif_hx, dif_tr);
     for (int i = 0; i < d->nv; ++i) dif_hx[i] = sqrt(dif_hx[i]);
     for (int k = 0; k < d->nt; ++k) dif_tr[k] = sqrt(dif_tr[k]);
   }

@@ -11,6 +11,7 @@
 
 /* scheme options beyond the trsk10/HTC defaults (par/swm.par) */
 typedef struct oracle_scheme {
+  int wrap;              /* 0 trsk10, 1 pxt16, 2 gass18 (method wrapper)    */
   int apvm;              /* pv_stab = apvm                                  */
   double pvspar;         /* its parameter                                   */
   double dt;             /* the module time step the APVM term reads        */
@@ -37,6 +38,7 @@ typedef struct oracle_fields {
   double *lapu, *lap_lapu;              /* (hyper)diffusion: [ne], [ne]     */
   double *cdiv, *cvort, *gdiv, *gvort;  /* diffusion_ed scratch: [nv], [nt], [ne], [ne] */
   double *div_lap, *vort_lap;           /* hyperdiffusion scratch: [nv], [nt] */
+  double *h_rhb;                        /* gass18: h at the rhombi [ne]     */
   int perot;                            /* compute v_hx even when unused    */
 } oracle_fields;
 
@@ -49,9 +51,10 @@ int oracle_rk4(const imx_mesh_desc *d, const oracle_params *p, double dt, double
                double *u, const double *bt, oracle_fields *w);
 /* dif_coef_hx / dif_coef_tr as swm_tests sets them up (src/swm.f90:170-210);
  * kind: 0 const, 1 align (alignment_coef), 2 diam (diameter_coef). With
- * k4 > 0 both hold the square roots of the hyperdiffusion coefficients. */
-int oracle_diffusion_coef(const imx_mesh_desc *d, const double *trhx_area, double k2, int k2kind,
-                          double k4, int k4kind, double *dif_hx, double *dif_tr);
+ * k4 > 0 both hold the square roots of the hyperdiffusion coefficients.
+ * wrap as oracle_scheme; -5 for gass18 with diffusion (the reference stops). */
+int oracle_diffusion_coef(const imx_mesh_desc *d, const double *trhx_area, int wrap, double k2,
+                          int k2kind, double k4, int k4kind, double *dif_hx, double *dif_tr);
 void oracle_diagnostics(const imx_mesh_desc *d, const oracle_params *p, const double *h,
                         const double *u, const double *bt, const oracle_fields *w,
                         double out[8]);

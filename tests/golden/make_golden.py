@@ -12,7 +12,8 @@ outputs as fixtures:
   ref_run_g2_spec96.bin     seeded spectral state (seed 3): 96 steps dt=900
   ref_run_g3_<scheme>.bin   seed 7, 3 steps dt=900 with the scheme options of
                             SCHEMES (APVM, diffusion, hyperdiffusion with the
-                            const/align/diam coefficients), plus v_hx,
+                            const/align/diam coefficients; the pxt16 and
+                            gass18 wrappers), plus v_hx,
                             q_grad_ed/tr, dif_coef_hx/tr, lapu, lap_lapu
 
 The seeded states are made by libimx (imx_ic_spectral) on the reference
@@ -39,13 +40,22 @@ SCHEMES = {
     "k2align": "k2=1e5,diffus=align",
     "k4diam": "k4=1e15,hyperdiffus=diam",
     "mix": "k2=2e5,diffus=const,k4=3e15,hyperdiffus=align,pv=apvm,pvspar=0.3",
+    "pxt16": "wrap=pxt16",
+    "gass18": "wrap=gass18",
+    "pxmix": "wrap=pxt16,pv=apvm,pvspar=0.4,k4=1e15,hyperdiffus=const",
+    "gassapvm": "wrap=gass18,pv=apvm,pvspar=0.5",
 }
 
 
 def run(args, cwd):
+    # the reference's `stop` exits with status 0: a run counts only if it
+    # wrote its output file
+    out = os.path.join(cwd, args[-1] if args[0] == "build" else args[6])
+    if os.path.exists(out):
+        os.remove(out)
     r = subprocess.run([HARNESS, *args], cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                        text=True)
-    if r.returncode != 0:
+    if r.returncode != 0 or not os.path.exists(out):
         raise RuntimeError(" ".join(args) + "\n" + r.stdout[-2000:])
 
 

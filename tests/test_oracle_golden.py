@@ -71,6 +71,10 @@ SCHEMES = {
     "k4diam": {"k4": 1e15, "hyperdiffus": "diam"},
     "mix": {"k2": 2e5, "diffus": "const", "k4": 3e15, "hyperdiffus": "align", "pv": "apvm",
             "pvspar": 0.3},
+    "pxt16": {"wrap": "pxt16"},
+    "gass18": {"wrap": "gass18"},
+    "pxmix": {"wrap": "pxt16", "pv": "apvm", "pvspar": 0.4, "k4": 1e15, "hyperdiffus": "const"},
+    "gassapvm": {"wrap": "gass18", "pv": "apvm", "pvspar": 0.5},
 }
 SCHEME_FIELDS = ["v_hx", "q_grad_ed", "q_grad_tr", "lapu", "lap_lapu"]
 
@@ -87,7 +91,7 @@ def test_scheme_options_bitexact(golden, name):
         assert np.array_equal(o.dif_hx, r["dif_coef_hx"]) and np.array_equal(o.dif_tr, r["dif_coef_tr"])
     m, f = o.tendency(r["h0"], r["u0"])
     assert np.array_equal(m, r["masseq"]) and np.array_equal(f, r["momeq"])
-    for k in SCHEME_FIELDS + ["q_ed", "uhq_perp"]:
+    for k in SCHEME_FIELDS + ["h_tr", "q_tr", "q_ed", "ke_hx", "uhq_perp"]:
         if k in r:
             assert np.array_equal(o.field(k), r[k]), k
     h, u = o.step(r["h0"], r["u0"], 900.0, 3)
@@ -102,6 +106,23 @@ def test_perot_reconstruction_bitexact(golden):
     o = Oracle(mesh, perot=True)
     o.tendency(r["h0"], r["u0"])
     assert np.array_equal(o.field("v_hx"), r["v_hx"])
+
+
+def test_wrapper_planted_mistakes_fail(golden):
+    """Each wrapper's fixture rejects the oracle run with another wrapper."""
+    mesh = golden("ref_mesh_g3.bin")
+    for name, other in (("pxt16", "trsk10"), ("gass18", "trsk10"), ("pxt16", "gass18")):
+        r = golden(f"ref_run_g3_{name}.bin")
+        o = Oracle(mesh, scheme={"wrap": other, "dt": 900.0})
+        _, f = o.tendency(r["h0"], r["u0"])
+        assert not np.array_equal(f, r["momeq"]), (name, other)
+
+
+def test_gass18_with_diffusion_refused(golden):
+    """The reference stops there (scalar_hx2tr has no Gassmann branch for
+    the coefficients); so does the oracle."""
+    with pytest.raises(ValueError, match="gass18"):
+        Oracle(golden("ref_mesh_g3.bin"), scheme={"wrap": "gass18", "k2": 1e5})
 
 
 def test_scheme_planted_mistakes_fail(golden):
