@@ -25,6 +25,7 @@ FIELD_POS = {"h_ed": "e", "uh": "e", "eta": "t", "h_tr": "t", "q_tr": "t",
              "grad_ghbk": "e", "uhq_perp": "e", "divu": "v", "zeta": "t",
              "q_grad_ed": "e", "lapu": "e", "lap_lapu": "e"}
 COEF = {"const": 0, "align": 1, "diam": 2}
+WRAP = {"trsk10": 0, "pxt16": 1, "gass18": 2}
 
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _F64P = ctypes.POINTER(ctypes.c_double)
@@ -33,7 +34,8 @@ _F32P = ctypes.POINTER(ctypes.c_float)
 
 class Scheme(ctypes.Structure):
     """imx_scheme: APVM, diffusion and hyperdiffusion options (par/swm.par)."""
-    _fields_ = [("apvm", ctypes.c_int32), ("pvspar", ctypes.c_double), ("k2", ctypes.c_double),
+    _fields_ = [("wrap", ctypes.c_int32), ("apvm", ctypes.c_int32), ("pvspar", ctypes.c_double),
+                ("k2", ctypes.c_double),
                 ("diffus", ctypes.c_int32), ("k4", ctypes.c_double), ("hyperdiffus", ctypes.c_int32)]
 
 

@@ -86,6 +86,11 @@ struct imx_swm {
   // REF scheme options (imx_swm_set_scheme): APVM, diffusion, hyperdiffusion
   imx_scheme sc{};
   bool sc_apvm = false, sc_diff = false, sc_hyper = false;
+  int sc_wrap = 0;                   // IMX_WRAP_*
+  double *t_w = nullptr;             // [3][NT] pxt16: barycentric coords; gass18: edge areas
+  double *e_dw = nullptr;            // [2][NE] pxt16: distances edge -> its triangles
+  double *e_nr = nullptr;            // [3][NE] pxt16: unit normal ed%nr
+  int32_t *e_ed3 = nullptr;          // [2][MX][NE] gass18: linking edge per TRiSK term
   int8_t *e_nrs = nullptr;           // sign of ed%nr . (tr(sh2)%c - tr(sh1)%c)
   double *e_c = nullptr, *t_c = nullptr, *c_p = nullptr;  // [3][n] points
   double *dif_hx = nullptr, *dif_tr = nullptr;            // coefficients

@@ -61,14 +61,19 @@ void cart2sph(double x, double y, double z, double *lon, double *lat);
 
 void gridprop(const imx_mesh_desc &d, double *trhx, double *hxtr, double *trskw);
 // Radius, in cell rings around a row's cells, of a stencil superset of the
-// handle's tendency: 1 for TRiSK, one more ring each for APVM and for
-// hyperdiffusion.
+// handle's tendency: 1 for TRiSK, one more ring each for APVM, for
+// hyperdiffusion and for the gass18 Coriolis.
 int swm_stencil_rings(const imx_swm *s);
 // dif_coef_hx [nv] / dif_coef_tr [nt] as swm_tests sets them up
 // (src/swm.f90:170-210), caller numbering; kind IMX_COEF_*. With k4 > 0
 // both hold the square roots of the hyperdiffusion coefficients.
-void diffusion_coef(const imx_mesh_desc &d, const double *trhx, double k2, int k2kind, double k4,
-                    int k4kind, std::vector<double> &hx, std::vector<double> &tr);
+void diffusion_coef(const imx_mesh_desc &d, const double *trhx, int wrap, double k2, int k2kind,
+                    double k4, int k4kind, std::vector<double> &hx, std::vector<double> &tr);
+// bar_coord_tr of triangle k's circumcentre (src/smeshpack.f90:5200-5245)
+void bary_circumcentre(const imx_mesh_desc &d, int k, double b[3]);
+// gethxedgeconnection(ed1, ed2) (src/smeshpack.f90:4127-4244), 0-based:
+// -2 if ed1 == ed2, the linking edge, or -1 if the edges are far apart
+int hxedgeconn(const imx_mesh_desc &d, int ed1, int ed2);
 // Internal numbering: cells along a Hilbert curve (or caller order); edge of
 // owning cell c -> 3c+k, triangle of owning cell c -> 2c+k. Maps are caller
 // (0-based) -> internal; NE = 3 nv and NT = 2 nv slots (some are dummies).

@@ -836,28 +836,78 @@ static void coef_at_cells(const imx_mesh_desc &d, double kmax, int kind, std::ve
   }
 }
 
-// scalar_hx2tr, trsk branch (src/swm_operators.f90:258-263)
-static void hx2tr(const imx_mesh_desc &d, const double *trhx, const std::vector<double> &f,
+void bary_circumcentre(const imx_mesh_desc &d, int k, double b[3]) {
+  const double *p = d.tr_c + 3 * k;
+  const double *p1 = d.v_p + 3 * (d.tr_v[3 * k] - 1), *p2 = d.v_p + 3 * (d.tr_v[3 * k + 1] - 1),
+               *p3 = d.v_p + 3 * (d.tr_v[3 * k + 2] - 1);
+  b[0] = std::fabs(det3(p, p2, p3));
+  b[1] = std::fabs(det3(p, p3, p1));
+  b[2] = std::fabs(det3(p, p1, p2));
+  const double area = b[0] + b[1] + b[2];
+  for (int i = 0; i < 3; ++i) b[i] = b[i] / area;
+}
+
+int hxedgeconn(const imx_mesh_desc &d, int ed1, int ed2) {
+  if (ed1 == ed2) return -2;
+  // getedgeintriangle: the third edge of a triangle holding both
+  const int t1 = d.ed_sh[2 * ed1] - 1, t2 = d.ed_sh[2 * ed1 + 1] - 1;
+  int tr = -1;
+  for (int i = 0; i < 3 && tr < 0; ++i) {
+    if (ed2 == d.tr_ed[3 * t1 + i] - 1) tr = t1;
+    else if (ed2 == d.tr_ed[3 * t2 + i] - 1) tr = t2;
+  }
+  if (tr >= 0) {
+    int e3 = -1;
+    for (int i = 0; i < 3; ++i) {
+      const int e = d.tr_ed[3 * tr + i] - 1;
+      if (e != ed1 && e != ed2) e3 = e;
+    }
+    if (e3 >= 0) return e3;
+  }
+  // an edge of ed2's triangles whose other triangle holds ed1
+  for (int j = 0; j < 2; ++j) {
+    const int trj = d.ed_sh[2 * ed2 + j] - 1;
+    for (int i = 0; i < 3; ++i) {
+      const int et = d.tr_ed[3 * trj + i] - 1;
+      if (et == ed2) continue;
+      const int tr2 = (d.ed_sh[2 * et] - 1 == trj) ? d.ed_sh[2 * et + 1] - 1 : d.ed_sh[2 * et] - 1;
+      for (int l = 0; l < 3; ++l)
+        if (ed1 == d.tr_ed[3 * tr2 + l] - 1) return et;
+    }
+  }
+  return -1;
+}
+
+// scalar_hx2tr (src/swm_operators.f90:258-269): trsk area weights, or the
+// barycentric coordinates of the circumcentre for pxt16
+static void hx2tr(const imx_mesh_desc &d, const double *trhx, int wrap, const std::vector<double> &f,
                   std::vector<double> &out) {
   out.assign(d.nt, 0.0);
   for (int k = 0; k < d.nt; ++k) {
     double s = 0.0;
-    for (int i = 0; i < 3; ++i) s = s + trhx[3 * k + i] * f[d.tr_v[3 * k + i] - 1];
-    out[k] = s / d.tr_areag[k];
+    if (wrap == IMX_WRAP_PXT16) {
+      double b[3];
+      bary_circumcentre(d, k, b);
+      for (int i = 0; i < 3; ++i) s = s + b[i] * f[d.tr_v[3 * k + i] - 1];
+      out[k] = s;
+    } else {
+      for (int i = 0; i < 3; ++i) s = s + trhx[3 * k + i] * f[d.tr_v[3 * k + i] - 1];
+      out[k] = s / d.tr_areag[k];
+    }
   }
 }
 
-void diffusion_coef(const imx_mesh_desc &d, const double *trhx, double k2, int k2kind, double k4,
-                    int k4kind, std::vector<double> &hx, std::vector<double> &tr) {
+void diffusion_coef(const imx_mesh_desc &d, const double *trhx, int wrap, double k2, int k2kind,
+                    double k4, int k4kind, std::vector<double> &hx, std::vector<double> &tr) {
   hx.assign(d.nv, 0.0);
   tr.assign(d.nt, 0.0);
   if (k2 > 0.0) {
     coef_at_cells(d, k2, k2kind, hx);
-    hx2tr(d, trhx, hx, tr);
+    hx2tr(d, trhx, wrap, hx, tr);
   }
   if (k4 > 0.0) {
     coef_at_cells(d, k4, k4kind, hx);
-    hx2tr(d, trhx, hx, tr);
+    hx2tr(d, trhx, wrap, hx, tr);
     for (double &x : hx) x = std::sqrt(x);
     for (double &x : tr) x = std::sqrt(x);
   }

@@ -83,6 +83,9 @@ __global__ __launch_bounds__(TPB) void ref_edge_flux(int ne, const int32_t *__re
 
 // vort_tr + rel_vort_tr + scalar_hx2tr + scalar_elem_divide
 // (swm_operators.f90:551-611, :780-827, :258-263, :1579-1615)
+// h at triangles: trsk area weights (hmode 0), barycentric coordinates t_w
+// (1, pxt16), or edge values weighted by the edge's sub-triangle areas t_w
+// (2, gass18: scalar_edtr2trcc, swm_operators.f90:286-346)
 __global__ __launch_bounds__(TPB) void ref_tri(int nt, const int32_t *__restrict__ t_ed,
                                                const int8_t *__restrict__ t_tg,
                                                const int32_t *__restrict__ t_v,
@@ -91,7 +94,9 @@ __global__ __launch_bounds__(TPB) void ref_tri(int nt, const int32_t *__restrict
                                                const double *__restrict__ t_trhx,
                                                const double *__restrict__ e_leng,
                                                const double *__restrict__ u,
-                                               const double *__restrict__ h, double erad,
+                                               const double *__restrict__ h, int hmode,
+                                               const double *__restrict__ t_w,
+                                               const double *__restrict__ h_ed, double erad,
                                                int use_pv, double *__restrict__ eta,
                                                double *__restrict__ zeta,
                                                double *__restrict__ h_tr,
@@ -108,10 +113,20 @@ __global__ __launch_bounds__(TPB) void ref_tri(int nt, const int32_t *__restrict
   zeta[k] = z;
   const double et = z + t_f[k];
   eta[k] = et;
-  double s = 0.0;
+  double s = 0.0, ht;
+  if (hmode == 1) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i) s = s + t_trhx[i * nt + k] * h[t_v[i * nt + k]];
-  const double ht = s / t_area[k];
+    for (int i = 0; i < 3; ++i) s = s + t_w[i * nt + k] * h[t_v[i * nt + k]];
+    ht = s;
+  } else if (hmode == 2) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) s = s + t_w[i * nt + k] * h_ed[t_ed[i * nt + k]];
+    ht = s / t_area[k];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) s = s + t_trhx[i * nt + k] * h[t_v[i * nt + k]];
+    ht = s / t_area[k];
+  }
   h_tr[k] = ht;
   double q = 0.0;
   if (use_pv) {
@@ -121,13 +136,36 @@ __global__ __launch_bounds__(TPB) void ref_tri(int nt, const int32_t *__restrict
   q_tr[k] = q;
 }
 
-// scalar_tr2ed on q (swm_operators.f90:152-153)
-__global__ __launch_bounds__(TPB) void ref_edge_q(int ne, const int32_t *__restrict__ e_sh,
+// PV at edges: scalar_tr2ed of q, simple average (qmode 0, trsk10,
+// swm_operators.f90:152-153) or distance-weighted (1, pxt16 on HTC, :163-168);
+// or (2, gass18) eta and h averaged over the rhombus with the triangle areas
+// (:155-162) and divided (swm.f90:2469-2473, zero guard as the reference's)
+__global__ __launch_bounds__(TPB) void ref_edge_q(int ne, int qmode, const int32_t *__restrict__ e_sh,
+                                                  const double *__restrict__ e_dw,
+                                                  const double *__restrict__ t_area,
+                                                  const double *__restrict__ eta,
+                                                  const double *__restrict__ h_tr,
                                                   const double *__restrict__ q_tr,
-                                                  double *__restrict__ q_ed) {
+                                                  double *__restrict__ q_ed, int *__restrict__ flag) {
   const int l = xcd_index();
   if (l >= ne || e_sh[l] < 0) return;
-  q_ed[l] = 0.5 * (q_tr[e_sh[l]] + q_tr[e_sh[ne + l]]);
+  const int k1 = e_sh[l], k2 = e_sh[ne + l];
+  if (qmode == 1) {
+    const double d1 = e_dw[l], d2 = e_dw[ne + l];
+    q_ed[l] = (d2 * q_tr[k1] + d1 * q_tr[k2]) / (d1 + d2);
+  } else if (qmode == 2) {
+    const double a1 = t_area[k1], a2 = t_area[k2];
+    const double er = (a1 * eta[k1] + a2 * eta[k2]) / (a1 + a2);
+    const double hr = (a1 * h_tr[k1] + a2 * h_tr[k2]) / (a1 + a2);
+    if (fabs(hr) < HTR_GUARD) {
+      flag[0] = 1;
+      q_ed[l] = 0.0;
+    } else {
+      q_ed[l] = er / hr;
+    }
+  } else {
+    q_ed[l] = 0.5 * (q_tr[k1] + q_tr[k2]);
+  }
 }
 
 // div_hx(uh), div_hx(u), kinetic_energy_hx, g(h+b)+K
@@ -141,7 +179,8 @@ __global__ __launch_bounds__(TPB) void ref_cell(int nv, const int32_t *__restric
                                                 const double *__restrict__ u,
                                                 const double *__restrict__ uh,
                                                 const double *__restrict__ h,
-                                                const double *__restrict__ bt, double erad,
+                                                const double *__restrict__ bt,
+                                                const double *__restrict__ vhx, double erad,
                                                 double grav, double *__restrict__ divuh,
                                                 double *__restrict__ masseq,
                                                 double *__restrict__ divu,
@@ -164,7 +203,15 @@ __global__ __launch_bounds__(TPB) void ref_cell(int nv, const int32_t *__restric
   divuh[i] = dv;
   masseq[i] = -dv;
   divu[i] = sd / c_area[i] / erad;
-  const double kk = k / ca;
+  double kk = k / ca;
+  if (vhx) {  // kinetic_energy_hx perhx (pxt16): |v|^2 / 2 of the Perot vector
+    const double v0 = vhx[i], v1 = vhx[nv + i], v2 = vhx[2 * nv + i];
+    double dd = 0.0;
+    dd = dd + v0 * v0;
+    dd = dd + v1 * v1;
+    dd = dd + v2 * v2;
+    kk = dd / 2.0;
+  }
   ke[i] = kk;
   ghbk[i] = grav * (h[i] + bt[i]) + kk;
 }
@@ -177,7 +224,8 @@ __global__ __launch_bounds__(TPB) void ref_edge_mom(
     const int32_t *__restrict__ c_nnb, const int32_t *__restrict__ c_ed,
     const int8_t *__restrict__ c_ttg, const int8_t *__restrict__ c_tnr,
     const double *__restrict__ c_trskw, const double *__restrict__ ghbk,
-    const double *__restrict__ uh, const double *__restrict__ q_ed, double erad,
+    const double *__restrict__ uh, const double *__restrict__ q_ed,
+    const int32_t *__restrict__ e_ed3, double erad,
     int zero_mom, double *__restrict__ grad, double *__restrict__ uhq,
     double *__restrict__ momeq) {
   const int l = xcd_index();
@@ -195,7 +243,9 @@ __global__ __launch_bounds__(TPB) void ref_edge_mom(
       const int k = c_ed[j * nv + node];
       const double sg = 1.0 * (double)c_ttg[j * nv + node] * tn;
       const double signcor = sg < 0.0 ? -1.0 : 1.0;
-      const double qtmp = 0.5 * (ql + q_ed[k]);
+      // gass18: q of the rhombus linking edges l and k where there is one
+      const int e3 = e_ed3 ? e_ed3[(ic * MX + j) * ne + l] : -1;
+      const double qtmp = e3 >= 0 ? q_ed[e3] : 0.5 * (ql + q_ed[k]);
       s = s + uh[k] * qtmp * e_dleng[k] * c_trskw[(ei * MX + j) * nv + node] * signcor;
     }
   }
@@ -379,6 +429,60 @@ __global__ __launch_bounds__(TPB) void ref_rvort(int nt, const int32_t *__restri
     z = z + f[ed] * e_leng[ed] * (double)t_tg[l * nt + k];
   }
   out[k] = z / t_area[k] / erad;
+}
+
+// grad_ed (HTC) + coriolis_ed pered (pxt16, HTC: swm_operators.f90:1330-1368):
+// Perot reconstruction of uh q at the edge's two cells, averaged, projected
+// at the edge, minus its component along ed%nr -> momeq
+__global__ __launch_bounds__(TPB) void ref_edge_mom_pered(
+    int ne, int nv, const int32_t *__restrict__ e_v, const double *__restrict__ e_leng,
+    const double *__restrict__ e_dleng, const double *__restrict__ e_c,
+    const double *__restrict__ e_nr, const int32_t *__restrict__ c_nnb,
+    const int32_t *__restrict__ c_ed, const int8_t *__restrict__ c_ttg,
+    const double *__restrict__ c_p, const double *__restrict__ c_area,
+    const double *__restrict__ ghbk, const double *__restrict__ uh, const double *__restrict__ q_ed,
+    double erad, int zero_mom, double *__restrict__ grad, double *__restrict__ uhq,
+    double *__restrict__ momeq) {
+  const int l = xcd_index();
+  if (l >= ne || e_v[l] < 0) return;
+  const double g = (ghbk[e_v[ne + l]] - ghbk[e_v[l]]) / e_leng[l] / erad;
+  grad[l] = g;
+  const double ql = q_ed[l];
+  double w[2][3];
+  for (int ic = 0; ic < 2; ++ic) {
+    const int node = e_v[ic * ne + l];
+    const double p0 = c_p[node], p1 = c_p[nv + node], p2 = c_p[2 * nv + node];
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    const int n = c_nnb[node];
+    for (int j = 0; j < n; ++j) {
+      const int l2 = c_ed[j * nv + node];
+      const double qtmp = 0.5 * (ql + q_ed[l2]);
+      const double a = (double)c_ttg[j * nv + node] * uh[l2] * qtmp;
+      const double len = e_dleng[l2];
+      v0 = v0 + a * (e_c[l2] - p0) * len;
+      v1 = v1 + a * (e_c[ne + l2] - p1) * len;
+      v2 = v2 + a * (e_c[2 * ne + l2] - p2) * len;
+    }
+    const double ar = c_area[node];
+    v0 = v0 / ar;
+    v1 = v1 / ar;
+    v2 = v2 / ar;
+    proj_sphere(v0, v1, v2, p0, p1, p2);
+    w[ic][0] = v0;
+    w[ic][1] = v1;
+    w[ic][2] = v2;
+  }
+  double v0 = 0.5 * w[0][0] + 0.5 * w[1][0];
+  double v1 = 0.5 * w[0][1] + 0.5 * w[1][1];
+  double v2 = 0.5 * w[0][2] + 0.5 * w[1][2];
+  proj_sphere(v0, v1, v2, e_c[l], e_c[ne + l], e_c[2 * ne + l]);
+  double dp = 0.0;
+  dp = dp + v0 * e_nr[l];
+  dp = dp + v1 * e_nr[ne + l];
+  dp = dp + v2 * e_nr[2 * ne + l];
+  const double p = -dp;
+  uhq[l] = p;
+  momeq[l] = zero_mom ? 0.0 : (-g) - p;
 }
 
 // ode_rk4 combinations (swm.f90:2647-2678)
@@ -758,31 +862,44 @@ static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, do
                                 double *momeq) {
   const int nv = s->nv, NE = s->NE, NT = s->NT;
   hipStream_t st = s->stream;
+  const int wrap = s->sc_wrap;
   ref_edge_flux<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, h, u, s->f_h_ed, s->f_uh);
   ref_tri<<<grid_for(NT), TPB, 0, st>>>(NT, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f,
-                                        s->t_trhx, s->e_leng, u, h, s->o.erad,
-                                        s->o.testcase >= 2, s->f_eta, s->f_zeta, s->f_h_tr,
-                                        s->f_q_tr, s->flag);
-  ref_edge_q<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_sh, s->f_q_tr, s->f_q_ed);
+                                        s->t_trhx, s->e_leng, u, h, wrap, s->t_w, s->f_h_ed,
+                                        s->o.erad, s->o.testcase >= 2, s->f_eta, s->f_zeta,
+                                        s->f_h_tr, s->f_q_tr, s->flag);
+  ref_edge_q<<<grid_for(NE), TPB, 0, st>>>(NE, wrap, s->e_sh, s->e_dw, s->t_area, s->f_eta, s->f_h_tr,
+                                           s->f_q_tr, s->f_q_ed, s->flag);
+  // Perot velocity at cells: APVM and the pxt16 kinetic energy
+  // (vector_edhx2hx_perot, swm.f90:2516)
+  if (s->sc_apvm || wrap == IMX_WRAP_PXT16)
+    ref_perot_hx<<<grid_for(nv), TPB, 0, st>>>(nv, NE, s->c_nnb, s->c_ed, s->c_ttg, s->e_c, s->c_p,
+                                               s->e_dleng, s->c_area, u, s->f_vhx);
   if (s->sc_apvm) {
-    // grad q to edges and triangles, v at cells, then the APVM PV at edges
-    // (tendency, swm.f90:2481-2492, :2516, :2556-2559)
+    // grad q to edges and triangles, then the APVM PV at edges
+    // (tendency, swm.f90:2481-2492, :2556-2559)
     ref_qgrad_ed<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_sh, s->e_nrs, s->e_dleng, s->f_q_tr, s->o.erad,
                                                s->f_qg_ed);
     ref_perot_tr<<<grid_for(NT), TPB, 0, st>>>(NT, NE, s->t_ed, s->t_tg, s->e_c, s->t_c, s->e_leng,
                                                s->t_area, s->f_qg_ed, s->f_qg_tr);
-    ref_perot_hx<<<grid_for(nv), TPB, 0, st>>>(nv, NE, s->c_nnb, s->c_ed, s->c_ttg, s->e_c, s->c_p,
-                                               s->e_dleng, s->c_area, u, s->f_vhx);
     ref_apvm_q<<<grid_for(NE), TPB, 0, st>>>(NE, NT, nv, s->e_sh, s->e_v, s->f_qg_tr, s->f_vhx,
                                              s->sc.pvspar, s->o.dt, s->f_q_ed);
   }
   ref_cell<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area, s->e_leng,
-                                         s->e_dleng, u, s->f_uh, h, s->bt, s->o.erad, s->o.grav,
-                                         s->f_divuh, masseq, s->f_divu, s->f_ke, s->f_ghbk);
-  ref_edge_mom<<<grid_for(NE), TPB, 0, st>>>(NE, nv, s->e_v, s->e_loc, s->e_leng, s->e_dleng,
-                                             s->c_nnb, s->c_ed, s->c_ttg, s->c_tnr, s->c_trskw,
-                                             s->f_ghbk, s->f_uh, s->f_q_ed, s->o.erad,
-                                             s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
+                                         s->e_dleng, u, s->f_uh, h, s->bt,
+                                         wrap == IMX_WRAP_PXT16 ? s->f_vhx : nullptr, s->o.erad,
+                                         s->o.grav, s->f_divuh, masseq, s->f_divu, s->f_ke, s->f_ghbk);
+  if (wrap == IMX_WRAP_PXT16)
+    ref_edge_mom_pered<<<grid_for(NE), TPB, 0, st>>>(NE, nv, s->e_v, s->e_leng, s->e_dleng, s->e_c,
+                                                     s->e_nr, s->c_nnb, s->c_ed, s->c_ttg, s->c_p,
+                                                     s->c_area, s->f_ghbk, s->f_uh, s->f_q_ed, s->o.erad,
+                                                     s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
+  else
+    ref_edge_mom<<<grid_for(NE), TPB, 0, st>>>(NE, nv, s->e_v, s->e_loc, s->e_leng, s->e_dleng,
+                                               s->c_nnb, s->c_ed, s->c_ttg, s->c_tnr, s->c_trskw,
+                                               s->f_ghbk, s->f_uh, s->f_q_ed,
+                                               wrap == IMX_WRAP_GASS18 ? s->e_ed3 : nullptr, s->o.erad,
+                                               s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
   if (s->sc_diff || s->sc_hyper) {
     // diffusion_ed of u (momeq + lapu with K2), then with K4 diffusion_ed of
     // lapu's divergence and vorticity (momeq - lap_lapu) (swm.f90:2575-2590);
@@ -1284,7 +1401,9 @@ static void set_href(imx_swm *s, const double *h) {
   s->href_set = true;
 }
 
-int swm_stencil_rings(const imx_swm *s) { return 1 + (s->sc_apvm ? 1 : 0) + (s->sc_hyper ? 1 : 0); }
+int swm_stencil_rings(const imx_swm *s) {
+  return 1 + (s->sc_apvm ? 1 : 0) + (s->sc_hyper ? 1 : 0) + (s->sc_wrap == IMX_WRAP_GASS18 ? 1 : 0);
+}
 
 static void drop_graph(imx_swm *s) {
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
@@ -1545,8 +1664,15 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: not the handle's mesh");
   if (!(sc->k2 >= 0.0) || !(sc->k4 >= 0.0) || !std::isfinite(sc->k2) || !std::isfinite(sc->k4) ||
       !std::isfinite(sc->pvspar) || sc->diffus < IMX_COEF_CONST || sc->diffus > IMX_COEF_DIAM ||
-      sc->hyperdiffus < IMX_COEF_CONST || sc->hyperdiffus > IMX_COEF_DIAM)
+      sc->hyperdiffus < IMX_COEF_CONST || sc->hyperdiffus > IMX_COEF_DIAM ||
+      sc->wrap < IMX_WRAP_TRSK10 || sc->wrap > IMX_WRAP_GASS18)
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: bad option value");
+  if (sc->wrap == IMX_WRAP_GASS18 && (sc->k2 > 0.0 || sc->k4 > 0.0))
+    return imx::set_error(IMX_EINVAL,
+                          "imx_swm_set_scheme: gass18 with diffusion (the reference stops: "
+                          "scalar_hx2tr needs edge values for Gassmann's scheme)");
+  if (sc->wrap != IMX_WRAP_TRSK10 && (!d->ed_c || !d->tr_c || !d->v_p))
+    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: pxt16/gass18 need ed_c, tr_c and v_p");
   if ((sc->k2 > 0.0 || sc->k4 > 0.0) && (!d->v_tr || !d->tr_c || !d->v_nb))
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: diffusion needs v_tr, v_nb and tr_c");
   if (sc->apvm && (!d->ed_c || !d->tr_c || !d->v_p))
@@ -1554,7 +1680,7 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   const int nv = s->nv, NE = s->NE, NT = s->NT;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
   imx::Alloc &A = s->A;
-  if (!s->e_nrs && (sc->apvm || sc->k2 > 0.0 || sc->k4 > 0.0)) {
+  if (!s->e_nrs && (sc->apvm || sc->k2 > 0.0 || sc->k4 > 0.0 || sc->wrap != IMX_WRAP_TRSK10)) {
     // geometry of the options, internal numbering, points component-major
     std::vector<int8_t> nrs(NE, 1);
     std::vector<double> ec((size_t)3 * NE, 0.0), tc((size_t)3 * NT, 0.0), cp((size_t)3 * nv, 0.0);
@@ -1586,7 +1712,54 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     s->f_vortl = A.get<double>(NT);
     s->dif_hx = A.get<double>(nv);
     s->dif_tr = A.get<double>(NT);
+    s->t_w = A.get<double>((size_t)3 * NT);
+    s->e_dw = A.get<double>((size_t)2 * NE);
+    s->e_nr = A.get<double>((size_t)3 * NE);
+    s->e_ed3 = A.get<int32_t>((size_t)2 * MX * NE);
     if (A.err != hipSuccess) return imx::hip_fail(A.err, "imx_swm_set_scheme");
+  }
+  if (sc->wrap == IMX_WRAP_PXT16) {
+    // barycentric coordinates of the circumcentres (initialize_gridprop,
+    // swm.f90:1021-1026), edge-to-triangle distances, unit edge normals
+    std::vector<double> tw((size_t)3 * NT, 0.0), dw((size_t)2 * NE, 1.0), nr((size_t)3 * NE, 0.0);
+    for (int ko = 0; ko < d->nt; ++ko) {
+      double b[3];
+      imx::bary_circumcentre(*d, ko, b);
+      for (int i = 0; i < 3; ++i) tw[(size_t)i * NT + tn[ko]] = b[i];
+    }
+    for (int eo = 0; eo < d->ne; ++eo) {
+      const int e = en[eo];
+      for (int k = 0; k < 2; ++k)
+        dw[(size_t)k * NE + e] = imx::arclen(d->ed_c + 3 * eo, d->tr_c + 3 * (d->ed_sh[2 * eo + k] - 1));
+      double c[3];
+      imx::cross3(d->v_p + 3 * (d->ed_v[2 * eo] - 1), d->v_p + 3 * (d->ed_v[2 * eo + 1] - 1), c);
+      const double n = imx::norm3(c);
+      for (int q = 0; q < 3; ++q) nr[(size_t)q * NE + e] = c[q] / n;
+    }
+    HIPTRY(hipMemcpy(s->t_w, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->e_dw, dw.data(), dw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->e_nr, nr.data(), nr.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+  } else if (sc->wrap == IMX_WRAP_GASS18) {
+    // areas of the sub-triangles (circumcentre, edge) and the linking edge
+    // of each TRiSK term (gethxedgeconnection)
+    std::vector<double> tw((size_t)3 * NT, 0.0);
+    for (int ko = 0; ko < d->nt; ++ko)
+      for (int i = 0; i < 3; ++i) {
+        const int eo = d->tr_ed[3 * ko + i] - 1;
+        tw[(size_t)i * NT + tn[ko]] = imx::sphtriarea(d->tr_c + 3 * ko, d->v_p + 3 * (d->ed_v[2 * eo] - 1),
+                                                      d->v_p + 3 * (d->ed_v[2 * eo + 1] - 1));
+      }
+    std::vector<int32_t> e3((size_t)2 * MX * NE, -1);
+    for (int eo = 0; eo < d->ne; ++eo)
+      for (int ic = 0; ic < 2; ++ic) {
+        const int co = d->ed_v[2 * eo + ic] - 1;
+        for (int j = 0; j < d->v_nnb[co]; ++j) {
+          const int c3 = imx::hxedgeconn(*d, eo, d->v_ed[MX * co + j] - 1);
+          e3[((size_t)ic * MX + j) * NE + en[eo]] = c3 >= 0 ? en[c3] : -1;
+        }
+      }
+    HIPTRY(hipMemcpy(s->t_w, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->e_ed3, e3.data(), e3.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
   }
   if (sc->k2 > 0.0 || sc->k4 > 0.0) {
     std::vector<double> trhx_v, hxtr_v, trskw_v;
@@ -1599,7 +1772,7 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       trhx = trhx_v.data();
     }
     std::vector<double> hx, tr;
-    imx::diffusion_coef(*d, trhx, sc->k2, sc->diffus, sc->k4, sc->hyperdiffus, hx, tr);
+    imx::diffusion_coef(*d, trhx, sc->wrap, sc->k2, sc->diffus, sc->k4, sc->hyperdiffus, hx, tr);
     std::vector<double> hi(nv, 0.0), ti(NT, 0.0);
     for (int co = 0; co < nv; ++co) hi[cn[co]] = hx[co];
     for (int ko = 0; ko < d->nt; ++ko) ti[tn[ko]] = tr[ko];
@@ -1610,6 +1783,7 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   s->sc_apvm = sc->apvm != 0;
   s->sc_diff = sc->k2 != 0.0;
   s->sc_hyper = sc->k4 != 0.0;
+  s->sc_wrap = sc->wrap;
   imx::drop_graph(s);  // the captured step holds the previous kernel sequence
   return IMX_OK;
 }

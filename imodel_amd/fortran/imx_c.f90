@@ -29,8 +29,9 @@ module imx_c
 
   ! imx_scheme: APVM, diffusion, hyperdiffusion (IMX_COEF_*: const, align, diam)
   integer(c_int32_t), parameter :: IMX_COEF_CONST = 0, IMX_COEF_ALIGN = 1, IMX_COEF_DIAM = 2
+  integer(c_int32_t), parameter :: IMX_WRAP_TRSK10 = 0, IMX_WRAP_PXT16 = 1, IMX_WRAP_GASS18 = 2
   type, bind(C) :: imx_scheme
-    integer(c_int32_t) :: apvm
+    integer(c_int32_t) :: wrap, apvm
     real(c_double) :: pvspar, k2
     integer(c_int32_t) :: diffus
     real(c_double) :: k4

@@ -135,19 +135,32 @@ typedef struct imx_swm  imx_swm;   /* one SWM instance; owns device memory */
 #define IMX_FIELD_LAP_LAPU 15  /* [ne]  hyperdiffusion_ed of u, with K4        */
 
 /* Scheme options of par/swm.par beyond the trsk10/HTC defaults
-   (swmpars, src/swm_data.f90:413-424, :548-551), for FP64_REF handles:
+   (swmpars, src/swm_data.f90:413-424, :438-551), for FP64_REF handles:
+     wrap    method wrapper (HTC staggering):
+             IMX_WRAP_TRSK10  TRiSK (Ringler et al. 2010), the default
+             IMX_WRAP_PXT16   Peixoto 2016: barycentric h at triangles,
+                              distance-weighted q at edges, Perot kinetic
+                              energy, Perot (pered) Coriolis
+             IMX_WRAP_GASS18  Gassmann 2018: h at triangles from edges, PV
+                              on rhombi, Gassmann Coriolis
      apvm    pv_stab = apvm: q_ed -= pvspar dt (grad q) . v    (:1465-1502)
      k2      K2_max, diffusion momeq += div(K grad) u           (:855-895)
      k4      K4_max, hyperdiffusion momeq -= (div(K grad))^2 u  (:930-975)
    with the coefficient function of each (const, align: smoothed alignment
    index, diam: cell diameter; src/swm_operators.f90:977-1072) set up as
    swm_tests does (src/swm.f90:170-210): with k4 > 0 both Laplacian terms
-   use the square roots of the hyperdiffusion coefficients. The APVM term
-   reads the handle's time step. All zero: the defaults. */
+   use the square roots of the hyperdiffusion coefficients; gass18 with
+   diffusion is refused (the reference stops: scalar_hx2tr has no Gassmann
+   branch). The APVM term reads the handle's time step. All zero: the
+   defaults. */
 #define IMX_COEF_CONST 0
 #define IMX_COEF_ALIGN 1
 #define IMX_COEF_DIAM  2
+#define IMX_WRAP_TRSK10 0
+#define IMX_WRAP_PXT16  1
+#define IMX_WRAP_GASS18 2
 typedef struct imx_scheme {
+  int32_t wrap;         /* IMX_WRAP_*                                         */
   int32_t apvm;         /* 0/1                                                */
   double  pvspar;       /* APVM parameter                                     */
   double  k2;           /* K2_max, 0 = no diffusion                           */

@@ -1,6 +1,6 @@
 """Scheme options of the FP64_REF path on the GPU (imx_swm_set_scheme):
-APVM, diffusion and hyperdiffusion with their coefficient functions, bit for
-bit against the reference's own golden vectors (tests/golden/make_golden.py)
+the pxt16 and gass18 wrappers, APVM, diffusion and hyperdiffusion with their
+coefficient functions, bit for bit against the reference's own golden vectors (tests/golden/make_golden.py)
 and against the oracle on a larger grid; the normal-mode Jacobian with the
 widened stencil."""
 import numpy as np
@@ -45,7 +45,9 @@ def test_scheme_tendency_and_steps_match_reference(golden, name, sfc):
 # glevel-5 cases: with K4 > 0 the Laplacian term also runs with sqrt(K4)
 # (the reference's set-up), so K4 is kept small enough for dt = 450 s
 G5 = {"mix": {"k2": 1e5, "diffus": "diam", "k4": 1e14, "hyperdiffus": "align", "pv": "apvm", "pvspar": 0.5},
-      "k2align": {"k2": 1e5, "diffus": "align"}}
+      "k2align": {"k2": 1e5, "diffus": "align"},
+      "pxt16": {"wrap": "pxt16", "pv": "apvm", "pvspar": 0.5, "k2": 1e5, "diffus": "const"},
+      "gass18": {"wrap": "gass18", "pv": "apvm", "pvspar": 0.5}}
 
 
 @pytest.mark.parametrize("name", sorted(G5))
@@ -66,7 +68,7 @@ def test_scheme_g5_matches_oracle(name):
     assert np.array_equal(hg, ho) and np.array_equal(ug, uo)
 
 
-@pytest.mark.parametrize("name", ["apvm", "mix"])
+@pytest.mark.parametrize("name", ["apvm", "mix", "pxt16", "gass18", "gassapvm"])
 def test_scheme_jacobian_equals_per_column(name):
     say("scheme jacobian g2", name)
     m = Mesh.icos(2)
@@ -90,6 +92,8 @@ def test_scheme_errors():
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
+        with pytest.raises(capi.ImxError, match="gass18 with diffusion"):
+            s.set_scheme(wrap="gass18", k4=1e15)
         s.set_scheme(k2=1e5)
         h, u = m.spectral(3)
         s.tendency(h, u)
