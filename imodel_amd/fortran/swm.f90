@@ -59,7 +59,12 @@ contains
     write(atmp, '(i8)') testcase
     swmname = "swm_tc"//trim(adjustl(atmp))
     write(atmp, '(i8)') nint(dt)
-    swmname = trim(swmname)//"_dt"//trim(adjustl(atmp))//"_HTC_trsk10_areageo"
+    ! stag, wrapper (or the explicit method names), area method (swm_data.f90:640-656)
+    if (trim(mtdwrapper) /= 'none') then
+      swmname = trim(swmname)//"_dt"//trim(adjustl(atmp))//"_HTC_"//trim(adjustl(mtdwrapper))//"_areageo"
+    else
+      swmname = trim(swmname)//"_dt"//trim(adjustl(atmp))//"_HTC_vrectrsk_crectrsk_sinttrsk_grdtrsk_areageo"
+    end if
     ! scheme-option suffixes of swmpars (src/swm_data.f90:658-683)
     if (trim(pv_stab) /= 'none') then
       write(atmp, '(f5.2)') real(pvspar)
@@ -88,9 +93,11 @@ contains
     call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
     if (scheme_options()) then
       if (trim(precision) /= 'fp64_ref') then
-        print*, "ERROR: APVM / diffusion / hyperdiffusion run on the fp64_ref path (par/swm_gpu.par)"
+        print*, "ERROR: pxt16 / gass18 / APVM / diffusion / hyperdiffusion run on the fp64_ref path ", &
+          "(par/swm_gpu.par)"
         stop 1
       end if
+      sc%wrap = wrap_code()
       sc%apvm = merge(1, 0, trim(pv_stab) == 'apvm')
       sc%pvspar = pvspar
       sc%k2 = max(k2_max, 0._r8)

@@ -3,8 +3,9 @@
 !   par/mesh.par  as getparameters  (src/smeshpack.f90:7343-7416), argv
 !                 [meshpar] [nv] override as in the reference
 !   par/swm.par   as swmpars        (src/swm_data.f90:338-693), the
-!                 HTC/trsk10/geo subset this build runs, with APVM,
-!                 diffusion and hyperdiffusion on the fp64_ref path
+!                 HTC/geo subset this build runs: trsk10 on every path;
+!                 the pxt16 and gass18 wrappers, APVM, diffusion and
+!                 hyperdiffusion on the fp64_ref path
 !   par/swm_gpu.par (new, optional): precision, seed, SFC and graph switches
 !=======================================================================
 module swm_par
@@ -141,10 +142,11 @@ contains
 
     ! the scheme this build runs (the reference defaults, par/swm.par)
     if (trim(stag) /= 'HTC') call unsupported('staggering', stag)
-    if (trim(mtdwrapper) /= 'trsk10') then
-      if (trim(mtdwrapper) /= 'none' .or. trim(reconmtd) /= 'trsk' .or. &
-          trim(coriolis_reconmtd) /= 'trsk' .or. trim(sinterpol) /= 'trsk' .or. &
-          trim(gradmtd) /= 'trsk') call unsupported('method wrapper', mtdwrapper)
+    if (wrap_code() < 0) call unsupported('method wrapper', mtdwrapper)
+    if (wrap_code() == 2 .and. (k2_max > 0._r8 .or. k4_max > 0._r8)) then
+      ! as the reference's scalar_hx2tr (src/swm_operators.f90:271-273)
+      print*, "Error at scalar_hx2trcc: Gassmann's scheme needs edge values"
+      stop 1
     end if
     if (areamtd(1:3) /= 'geo') call unsupported('area method', areamtd)
     if (k2_max > 0._r8) then
@@ -215,9 +217,24 @@ contains
     end select
   end function coef_code
 
-  ! APVM, diffusion or hyperdiffusion selected
+  ! method wrapper -> IMX_WRAP_* (-1: not part of this build). The
+  ! explicit form "none" with trsk in the four method lines is trsk10.
+  integer(c_int32_t) function wrap_code() result(c)
+    select case (trim(mtdwrapper))
+    case ('trsk10'); c = 0
+    case ('pxt16'); c = 1
+    case ('gass18'); c = 2
+    case ('none')
+      c = -1
+      if (trim(reconmtd) == 'trsk' .and. trim(coriolis_reconmtd) == 'trsk' .and. &
+          trim(sinterpol) == 'trsk' .and. trim(gradmtd) == 'trsk') c = 0
+    case default; c = -1
+    end select
+  end function wrap_code
+
+  ! a wrapper other than trsk10, APVM, diffusion or hyperdiffusion selected
   logical function scheme_options() result(on)
-    on = trim(pv_stab) == 'apvm' .or. k2_max > 0._r8 .or. k4_max > 0._r8
+    on = wrap_code() /= 0 .or. trim(pv_stab) == 'apvm' .or. k2_max > 0._r8 .or. k4_max > 0._r8
   end function scheme_options
 
   subroutine unsupported(what, val)

@@ -221,3 +221,25 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)
         assert r.returncode != 0 and "fp64_ref" in r.stdout
+
+
+@pytest.mark.parametrize("wrap", ["pxt16", "gass18"])
+def test_fortran_driver_wrappers_bitexact(wrap):
+    """The pxt16 / gass18 wrapper line of par/swm.par: a day of TC2 on
+    glevel 3 equals the oracle with that wrapper bit for bit."""
+    print(f"[gpu-case] fortran imodel g3 tc2 fp64_ref {wrap}", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 48, "fp64_ref")
+        p = os.path.join(td, "par", "swm.par")
+        swm = open(p).read().splitlines()
+        swm[10] = wrap
+        open(p, "w").write("\n".join(swm) + "\n")
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        st = imxbin.read(os.path.join(td, "data", f"swm_tc2_dt900_HTC_{wrap}_areageo_state_icos_eqs_nopt_3.bin"))
+    m = Mesh.icos(3)
+    h0, u0 = m.tc2()
+    o = Oracle(m.arrays, scheme={"wrap": wrap, "dt": 900.0})
+    h, u = o.step(h0, u0, 900.0, 96)
+    assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
