@@ -104,7 +104,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=8)
     ap.add_argument("--copy-probe-mib", type=int, default=1024,
                     help="size of the streaming-copy bandwidth probe (0: skip)")
-    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "learned", "mlp"],
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "learned", "mlp", "scheme"],
                     help="single: one instance at --glevel; ensemble: --instances instances, "
                          "levels uniform in 4..7, CFL-0.8 time steps; learned: glevel-6 model with "
                          "the learned correction vs plain glevel 6 and 7 against restricted glevel 8; "
@@ -131,6 +131,8 @@ def main():
         return bench_learned(args)
     if args.workload == "mlp":
         return bench_mlp(args)
+    if args.workload == "scheme":
+        return bench_scheme(args)
 
     t = time.perf_counter()
     mesh = Mesh.icos(args.glevel)
@@ -265,6 +267,66 @@ def compare_coarser(args, mesh, hK, uK):
             "fine_gpu_ms_same_interval": None,
             "hprime_l2_rel_vs_fine": float(np.sqrt(np.sum(A * (hc - ht) ** 2) / np.sum(A * ha ** 2))),
             "u_l2_rel_vs_fine": float(np.linalg.norm(uc - ut) / np.linalg.norm(ut))}
+
+
+# option sets of the scheme workload (par/swm.par lines), FP64_REF path
+SCHEME_SETS = {
+    "trsk10": {},
+    "apvm": {"pv": "apvm", "pvspar": 0.5},
+    "k2const": {"k2": 1e5, "diffus": "const"},
+    "k4diam": {"k4": 1e14, "hyperdiffus": "diam"},
+    "pxt16": {"wrap": "pxt16"},
+    "gass18": {"wrap": "gass18"},
+    "pxmix": {"wrap": "pxt16", "pv": "apvm", "pvspar": 0.5, "k4": 1e14, "hyperdiffus": "const"},
+}
+
+
+def bench_scheme(args):
+    """The FP64 reference-order step (bit-identical to the reference) with
+    each option set of SCHEME_SETS at --glevel: ms per step and cell-steps/s,
+    against the plain trsk10 step of the same path. value = the pxmix set
+    (pxt16 + APVM + hyperdiffusion). The roofline uses the plain step's
+    CANON-1 bytes (FP64), so the options' extra passes show as a lower
+    fraction."""
+    from imodel_amd.swm import Mesh, Swm
+
+    mesh = Mesh.icos(args.glevel)
+    h0, u0 = mesh.spectral(args.seed)
+    nv, ne, nt = mesh.nv, mesh.ne, mesh.nt
+    steps = min(args.steps, 96)
+    res = {}
+    for name, sc in SCHEME_SETS.items():
+        with Swm(mesh, precision="fp64_ref", dt=args.dt) as s:
+            s.set_scheme(**sc)
+            s.set_state(h0, u0)
+            s.step(max(args.warmup, 1))
+            s.set_state(h0, u0)
+            ms = s.step_timed(steps)
+            d = s.diagnostics()
+        res[name] = {"ms_per_step": ms / steps, "cell_steps_per_s": nv * steps / (ms * 1e-3),
+                     "options": sc, "mass_end": float(d["mass"])}
+        log(f"{name}: {ms / steps:.3f} ms/step")
+    for name in res:
+        res[name]["vs_trsk10"] = res["trsk10"]["ms_per_step"] / res[name]["ms_per_step"]
+    top = res["pxmix"]
+    B = canon_step_bytes(nv, ne, nt, True)
+    achieved = B / (top["ms_per_step"] * 1e-3) / 1e9
+    out = {
+        "metric": "FP64 reference-order step with scheme options (cell-steps/s)",
+        "value": top["cell_steps_per_s"], "unit": "cell-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": max(args.warmup, 1), "ms_per_step": top["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded spectral state)",
+        "config": {"workload": f"pxt16 + APVM + hyperdiffusion, FP64 reference order, icos glevel {args.glevel}",
+                   "glevel": args.glevel, "cells": nv, "dt": args.dt},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "RK4 step, reference-order kernels (plain step's CANON-1 bytes)",
+                     "algorithmic_bytes_per_launch": B, "launch_ms": top["ms_per_step"]},
+        "cpu_baseline": None,
+        "schemes": res,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_ensemble(args):

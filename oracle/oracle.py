@@ -23,16 +23,17 @@ OMEGA = 7.292e-5          # constants.f90:67
 _F = ("h_ed", "uh", "eta", "h_tr", "q_tr", "q_ed", "eta_ed", "divuh", "ke_hx", "ghbk",
       "hbt", "grad_ghbk", "uhq_perp", "divu", "zeta", "v_hx", "massf", "momf", "h_new",
       "u_new", "q_grad_ed", "q_grad_tr", "lapu", "lap_lapu", "cdiv", "cvort", "gdiv", "gvort",
-      "div_lap", "vort_lap", "h_rhb")
+      "div_lap", "vort_lap", "h_rhb", "st_c", "st_t", "st_g", "st_tt", "st_tg")
 _F64P = ctypes.POINTER(ctypes.c_double)
 COEF_KIND = {"const": 0, "align": 1, "diam": 2}
 
 
 WRAP = {"trsk10": 0, "pxt16": 1, "gass18": 2}
+STAG = {"HTC": 0, "HC": 1}
 
 
 class _Scheme(ctypes.Structure):
-    _fields_ = [("wrap", ctypes.c_int), ("apvm", ctypes.c_int), ("pvspar", ctypes.c_double), ("dt", ctypes.c_double),
+    _fields_ = [("stag", ctypes.c_int), ("wrap", ctypes.c_int), ("apvm", ctypes.c_int), ("pvspar", ctypes.c_double), ("dt", ctypes.c_double),
                 ("diffusion", ctypes.c_int), ("hyperdiffusion", ctypes.c_int),
                 ("dif_hx", _F64P), ("dif_tr", _F64P)]
 
@@ -83,7 +84,7 @@ class Oracle:
 
     The mesh dict must hold the gridprop arrays (tr_trhx_area, hx_trskw).
     scheme: None (trsk10/HTC defaults) or a dict with any of
-      wrap ("trsk10" | "pxt16" | "gass18"),
+      stag ("HTC" | "HC"), wrap ("trsk10" | "pxt16" | "gass18"),
       pv ("none" | "apvm"), pvspar, dt (the time step the APVM term reads),
       k2, diffus, k4, hyperdiffus ("const" | "align" | "diam"),
     the entries of par/swm.par.
@@ -111,7 +112,7 @@ class Oracle:
             if rc:
                 raise ValueError("gass18 with diffusion: the reference stops (scalar_hx2tr needs "
                                  "edge values for Gassmann's scheme)")
-            self._sc = _Scheme(wrap=WRAP[sc.get("wrap", "trsk10")], apvm=int(sc.get("pv", "none") == "apvm"),
+            self._sc = _Scheme(stag=STAG[sc.get("stag", "HTC")], wrap=WRAP[sc.get("wrap", "trsk10")], apvm=int(sc.get("pv", "none") == "apvm"),
                                pvspar=float(sc.get("pvspar", 0.5)), dt=float(sc.get("dt", 0.0)),
                                diffusion=int(k2 != 0.0), hyperdiffusion=int(k4 != 0.0),
                                dif_hx=capi.f64p(self.dif_hx), dif_tr=capi.f64p(self.dif_tr))

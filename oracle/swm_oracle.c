@@ -9,9 +9,9 @@
  * built with -ffp-contract=off on the same inputs (pinned against golden
  * vectors written by oracle/_ref/ref_harness, see tests/golden/).
  *
- * Scope: HTC staggering, geodesic areas, the trsk10 wrapper (the reference
- * defaults, par/swm.par) or the pxt16 / gass18 wrappers, optionally with
- * APVM, diffusion and hyperdiffusion (oracle_scheme), i.e. the branches
+ * Scope: HTC (the reference default) or HC staggering, geodesic areas, the
+ * trsk10 wrapper (the default) or the pxt16 / gass18 wrappers, optionally
+ * with APVM, diffusion and hyperdiffusion (oracle_scheme), i.e. the branches
  *   scalar_hx2ed        src/swm_operators.f90:88-98
  *   scalar_elem_product src/swm_operators.f90:1546-1577
  *   vort_tr             src/swm_operators.f90:551-611
@@ -34,6 +34,12 @@
  * pxt16: scalar_hx2tr bary (:265-269, bar_coord_tr src/smeshpack.f90:5200-5245),
  *   scalar_tr2ed distance-weighted (:163-168), kinetic_energy_hx perhx
  *   (:1172-1173), coriolis_ed pered HTC (:1330-1368)
+ * HC: the useStagHC branches of scalar_hx2ed (:100-114, edhx%c%b of
+ *   initialize_gridprop src/swm.f90:1007-1018, insidetrmesh and robdet
+ *   src/smeshpack.f90:5329-5420, :6597-6629), vort_tr, div_hx, grad_ed,
+ *   grad_ed_tg, rel_vort_tr, vector_edhx2hx_perot and coriolis_ed; the signs
+ *   follow from hx%nr = hx%ttgout sign(ed%tg . edhx%nr)
+ *   (src/smeshpack.f90:1085-1116)
  * gass18: scalar_edtr2trcc (:286-346), scalar_tr2ed rhombi (:155-162),
  *   q_ed = eta_ed / h_rhb (src/swm.f90:2469-2473), coriolis_ed gass
  *   (:1379-1416, gethxedgeconnection src/smeshpack.f90:4127-4244)
@@ -70,9 +76,8 @@ static double dot(const double *a, const double *b) {
   return s;
 }
 
-/* bar_coord_tr of triangle k's circumcentre (src/smeshpack.f90:5200-5245) */
-static void bary(const imx_mesh_desc *d, int k, double *b) {
-  const double *p = d->tr_c + 3 * k;
+/* bar_coord_tr (src/smeshpack.f90:5200-5245) of point p in triangle k */
+static void bary_at(const imx_mesh_desc *d, const double *p, int k, double *b) {
   const double *p1 = d->v_p + 3 * (d->tr_v[3 * k] - 1), *p2 = d->v_p + 3 * (d->tr_v[3 * k + 1] - 1),
                *p3 = d->v_p + 3 * (d->tr_v[3 * k + 2] - 1);
   double c[3];
@@ -85,6 +90,7 @@ static void bary(const imx_mesh_desc *d, int k, double *b) {
   const double area = b[0] + b[1] + b[2];
   for (int i = 0; i < 3; ++i) b[i] = b[i] / area;
 }
+static void bary(const imx_mesh_desc *d, int k, double *b) { bary_at(d, d->tr_c + 3 * k, k, b); }
 
 /* sphtriarea (src/smeshpack.f90:4710-4762) */
 static double sphtriarea(const double *p1, const double *p2, const double *p3) {
@@ -110,6 +116,68 @@ static void ed_nr(const imx_mesh_desc *d, int l, double *nr) {
   cross(a, b, nr);
   const double n = sqrt(dot(nr, nr));
   for (int q = 0; q < 3; ++q) nr[q] = nr[q] / n;
+}
+
+/* edhx%nr = tr(sh1)%c x tr(sh2)%c normalised, edhx%tg = edhx%nr x edhx%c
+ * normalised (src/smeshpack.f90:1030-1034) */
+static void edhx_nr(const imx_mesh_desc *d, int l, double *nr) {
+  cross(d->tr_c + 3 * (d->ed_sh[2 * l] - 1), d->tr_c + 3 * (d->ed_sh[2 * l + 1] - 1), nr);
+  const double n = sqrt(dot(nr, nr));
+  for (int q = 0; q < 3; ++q) nr[q] = nr[q] / n;
+}
+static void edhx_tg(const imx_mesh_desc *d, int l, double *tg) {
+  double nr[3];
+  edhx_nr(d, l, nr);
+  cross(nr, d->edhx_c + 3 * l, tg);
+  const double n = sqrt(dot(tg, tg));
+  for (int q = 0; q < 3; ++q) tg[q] = tg[q] / n;
+}
+/* ed%tg = ed%nr x ed%c normalised (src/smeshpack.f90:727-729) */
+static void ed_tg(const imx_mesh_desc *d, int l, double *tg) {
+  double nr[3];
+  ed_nr(d, l, nr);
+  cross(nr, d->ed_c + 3 * l, tg);
+  const double n = sqrt(dot(tg, tg));
+  for (int q = 0; q < 3; ++q) tg[q] = tg[q] / n;
+}
+
+/* robdet (src/smeshpack.f90:6597-6629): the determinant in binary128 */
+static double robdet(const double *p1, const double *p2, const double *p3) {
+  const __float128 q1[3] = {p1[0], p1[1], p1[2]}, q2[3] = {p2[0], p2[1], p2[2]},
+                   q3[3] = {p3[0], p3[1], p3[2]};
+  __float128 a[6];
+  a[0] = q1[0] * q2[1] * q3[2];
+  a[1] = q2[0] * q3[1] * q1[2];
+  a[2] = q3[0] * q1[1] * q2[2];
+  a[3] = -q3[0] * q2[1] * q1[2];
+  a[4] = -q1[0] * q3[1] * q2[2];
+  a[5] = -q2[0] * q1[1] * q3[2];
+  const __float128 t = a[0] + a[1] + a[2] + a[3] + a[4] + a[5];
+  return (double)t;
+}
+
+/* insidetrmesh (src/smeshpack.f90:5329-5420), 0-based triangle */
+static int insidetrmesh(const imx_mesh_desc *d, const double *p, int tr) {
+  const double tol = 1.1920928955078125e-07 / 10000.0;
+  const double tol2 = 2.220446049250313e-16 * 100.0; /* epsilon(pi), pi double */
+  for (int l = 0; l < 3; ++l) {
+    const double *p1 = d->v_p + 3 * (d->tr_v[3 * tr + l] - 1);
+    const double *p2 = d->v_p + 3 * (d->tr_v[3 * tr + (l + 1) % 3] - 1);
+    double t[3];
+    for (int q = 0; q < 3; ++q) t[q] = p[q] - p1[q];
+    if (sqrt(dot(t, t)) < tol) return 1;
+    double vn[3];
+    cross(p1, p2, vn);
+    const double vnn = sqrt(dot(vn, vn));
+    for (int q = 0; q < 3; ++q) vn[q] = vn[q] / vnn;
+    const double determ = dot(vn, p);
+    if (determ < -tol) return 0;
+    if (determ > tol) continue;
+    const double determ2 = robdet(p, p1, p2);
+    if (fabs(determ2) < tol2 || determ * determ2 < 0) continue;
+    if (determ2 < -tol2) return 0;
+  }
+  return 1;
 }
 
 /* gethxedgeconnection(ed1, ed2) (src/smeshpack.f90:4127-4244), 0-based ids:
@@ -175,45 +243,47 @@ static int edindex(const imx_mesh_desc *d, int ed1, int cell0) {
   return -1;
 }
 
-/* div_hx (HTC) of an edge field */
-static void div_hx(const imx_mesh_desc *d, double erad, const double *f, double *out) {
+/* div_hx of an edge field (signs of the staggering in w->st_c) */
+static void div_hx(const imx_mesh_desc *d, double erad, const oracle_fields *w, const double *f,
+                   double *out) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < d->nv; ++i) {
     double s = 0.0;
     for (int j = 0; j < d->v_nnb[i]; ++j) {
       int l = d->v_ed[MX * i + j] - 1;
-      double signcor = (double)d->hx_ttgout[MX * i + j];
+      double signcor = w->st_c[MX * i + j];
       s = s + signcor * f[l] * d->edhx_leng[l];
     }
     out[i] = s / d->hx_areag[i] / erad;
   }
 }
 
-/* rel_vort_tr (HTC) of an edge field */
-static void rel_vort_tr(const imx_mesh_desc *d, double erad, const double *f, double *out) {
+/* rel_vort_tr of an edge field (signs in w->st_t) */
+static void rel_vort_tr(const imx_mesh_desc *d, double erad, const oracle_fields *w, const double *f,
+                        double *out) {
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < d->nt; ++k) {
     double z = 0.0;
     for (int l = 0; l < 3; ++l) {
       int ed = d->tr_ed[3 * k + l] - 1;
-      double signcor = (double)d->tr_tg[3 * k + l];
+      double signcor = w->st_t[3 * k + l];
       z = z + f[ed] * d->ed_leng[ed] * signcor;
     }
     out[k] = z / d->tr_areag[k] / erad;
   }
 }
 
-/* diffusion_ed (HTC): grad_ed(K div) - grad_ed_tg(K vort) */
+/* diffusion_ed: grad_ed(K div) - grad_ed_tg(K vort) (signs in w->st_g, st_tt) */
 static void diffusion(const imx_mesh_desc *d, double erad, const double *kh, const double *kt,
                       const double *div, const double *vort, oracle_fields *w, double *out) {
   for (int i = 0; i < d->nv; ++i) w->cdiv[i] = kh[i] * div[i];
   for (int k = 0; k < d->nt; ++k) w->cvort[k] = kt[k] * vort[k];
 #pragma omp parallel for schedule(static)
   for (int l = 0; l < d->ne; ++l) {
-    double g = w->cdiv[d->ed_v[2 * l + 1] - 1] - w->cdiv[d->ed_v[2 * l] - 1];
+    double g = w->st_g[l] * (w->cdiv[d->ed_v[2 * l + 1] - 1] - w->cdiv[d->ed_v[2 * l] - 1]);
     w->gdiv[l] = g / d->ed_leng[l] / erad;
     const double f1 = w->cvort[d->ed_sh[2 * l] - 1], f2 = w->cvort[d->ed_sh[2 * l + 1] - 1];
-    double gt = ed_nr_sign(d, l) * (f2 - f1);
+    double gt = w->st_tt[l] * (f2 - f1);
     w->gvort[l] = gt / (d->edhx_leng[l] * erad);
   }
   for (int l = 0; l < d->ne; ++l) out[l] = w->gdiv[l] - w->gvort[l];
@@ -230,15 +300,59 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
   double *eta_ed = w->eta_ed, *vhx = w->v_hx;
   int bad = 0;
 
+  const int hc = p->scheme && p->scheme->stag == 1;
+  /* signs and the Perot edge points of the staggering */
+  for (int l = 0; l < ne; ++l) {
+    double s_e = 1.0;
+    if (hc) {
+      double a[3], b[3];
+      ed_tg(d, l, a);
+      edhx_nr(d, l, b);
+      s_e = sgn1(dot(a, b)); /* ttgout = hx%nr sign(ed%tg . edhx%nr) */
+      double dv[3];
+      const double *v1 = d->v_p + 3 * (d->ed_v[2 * l] - 1), *v2 = d->v_p + 3 * (d->ed_v[2 * l + 1] - 1);
+      for (int q = 0; q < 3; ++q) dv[q] = v2[q] - v1[q];
+      w->st_g[l] = sgn1(dot(b, dv));
+      double tg[3], dt3[3];
+      edhx_tg(d, l, tg);
+      const double *t1 = d->tr_c + 3 * (d->ed_sh[2 * l] - 1), *t2 = d->tr_c + 3 * (d->ed_sh[2 * l + 1] - 1);
+      for (int q = 0; q < 3; ++q) dt3[q] = t2[q] - t1[q];
+      w->st_tt[l] = sgn1(dot(tg, dt3));
+      for (int q = 0; q < 3; ++q) w->st_tg[3 * l + q] = tg[q];
+    } else {
+      w->st_g[l] = 1.0;
+      w->st_tt[l] = ed_nr_sign(d, l);
+    }
+    w->h_rhb[l] = s_e; /* scratch: per-edge factor, consumed below */
+  }
+  for (int i = 0; i < nv; ++i)
+    for (int j = 0; j < d->v_nnb[i]; ++j)
+      w->st_c[MX * i + j] = (double)d->hx_ttgout[MX * i + j] * (hc ? w->h_rhb[d->v_ed[MX * i + j] - 1] : 1.0);
+  for (int k = 0; k < nt; ++k)
+    for (int l = 0; l < 3; ++l)
+      w->st_t[3 * k + l] = (double)d->tr_tg[3 * k + l] * (hc ? w->h_rhb[d->tr_ed[3 * k + l] - 1] : 1.0);
+  const double *epos = hc ? d->edhx_c : d->ed_c;
+
   /* zero_vector(momeq), zero_vector(masseq) */
   for (int l = 0; l < ne; ++l) momeq[l] = 0.0;
   for (int i = 0; i < nv; ++i) masseq[i] = 0.0;
 
-  /* scalar_hx2ed + scalar_elem_product */
+  /* scalar_hx2ed + scalar_elem_product: the average, or (HC with pxt16)
+     barycentric at the dual edge midpoint in its containing triangle */
+  const int wrap0 = p->scheme ? p->scheme->wrap : 0;
 #pragma omp parallel for schedule(static)
   for (int l = 0; l < ne; ++l) {
-    int c1 = d->ed_v[2 * l] - 1, c2 = d->ed_v[2 * l + 1] - 1;
-    h_ed[l] = (h[c1] + h[c2]) * 0.5;
+    if (hc && wrap0 == 1) {
+      const double *pc = d->edhx_c + 3 * l;
+      const int kt = insidetrmesh(d, pc, d->ed_sh[2 * l] - 1) ? d->ed_sh[2 * l] - 1 : d->ed_sh[2 * l + 1] - 1;
+      double b[3], f = 0.0;
+      bary_at(d, pc, kt, b);
+      for (int i = 0; i < 3; ++i) f = f + b[i] * h[d->tr_v[3 * kt + i] - 1];
+      h_ed[l] = f;
+    } else {
+      int c1 = d->ed_v[2 * l] - 1, c2 = d->ed_v[2 * l + 1] - 1;
+      h_ed[l] = (h[c1] + h[c2]) * 0.5;
+    }
   }
 #pragma omp parallel for schedule(static)
   for (int l = 0; l < ne; ++l) uh[l] = u[l] * h_ed[l];
@@ -249,7 +363,7 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
     double e = 0.0;
     for (int l = 0; l < 3; ++l) {
       int ed = d->tr_ed[3 * k + l] - 1;
-      double signcor = (double)d->tr_tg[3 * k + l];
+      double signcor = w->st_t[3 * k + l];
       e = e + u[ed] * d->ed_leng[ed] * signcor;
     }
     e = e / d->tr_areag[k] / erad;
@@ -294,7 +408,7 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
 #define TR2ED(f, out)                                                                  \
   for (int l = 0; l < ne; ++l) {                                                       \
     const int k1 = d->ed_sh[2 * l] - 1, k2 = d->ed_sh[2 * l + 1] - 1;                  \
-    if (wrap == 0) {                                                                   \
+    if (wrap == 0 || hc) {                                                             \
       out[l] = 0.5 * (f[k1] + f[k2]);                                                  \
     } else if (wrap == 2) {                                                            \
       const double a1 = d->tr_areag[k1], a2 = d->tr_areag[k2];                          \
@@ -369,9 +483,9 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
       const double *pc = d->v_p + 3 * i;
       for (int l = 0; l < d->v_nnb[i]; ++l) {
         int ed = d->v_ed[MX * i + l] - 1;
-        double signcor = (double)d->hx_ttgout[MX * i + l];
+        double signcor = w->st_c[MX * i + l];
         for (int q = 0; q < 3; ++q)
-          v[q] = v[q] + signcor * u[ed] * (d->ed_c[3 * ed + q] - pc[q]) * d->edhx_leng[ed];
+          v[q] = v[q] + signcor * u[ed] * (epos[3 * ed + q] - pc[q]) * d->edhx_leng[ed];
       }
       for (int q = 0; q < 3; ++q) v[q] = v[q] / d->hx_areag[i];
       /* proj_vec_sphere (src/smeshpack.f90:4419-4433) */
@@ -392,7 +506,7 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
     double s = 0.0;
     for (int j = 0; j < d->v_nnb[i]; ++j) {
       int l = d->v_ed[MX * i + j] - 1;
-      double signcor = (double)d->hx_ttgout[MX * i + j];
+      double signcor = w->st_c[MX * i + j];
       s = s + signcor * uh[l] * d->edhx_leng[l];
     }
     divuh[i] = s / d->hx_areag[i] / erad;
@@ -421,7 +535,7 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
   /* grad_ed (HTC) -> momeq = -grad */
 #pragma omp parallel for schedule(static)
   for (int l = 0; l < ne; ++l) {
-    double g = ghbk[d->ed_v[2 * l + 1] - 1] - ghbk[d->ed_v[2 * l] - 1];
+    double g = w->st_g[l] * (ghbk[d->ed_v[2 * l + 1] - 1] - ghbk[d->ed_v[2 * l] - 1]);
     grad[l] = g / d->ed_leng[l] / erad;
   }
   for (int l = 0; l < ne; ++l) momeq[l] = -grad[l];
@@ -452,9 +566,9 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
         for (int j = 0; j < d->v_nnb[node]; ++j) {
           const int l2 = d->v_ed[MX * node + j] - 1;
           const double qtmp = 0.5 * (q_ed[l] + q_ed[l2]);
-          const double signcor = (double)d->hx_ttgout[MX * node + j];
+          const double signcor = w->st_c[MX * node + j];
           for (int q = 0; q < 3; ++q)
-            v[q] = v[q] + signcor * uh[l2] * qtmp * (d->ed_c[3 * l2 + q] - pc[q]) * d->edhx_leng[l2];
+            v[q] = v[q] + signcor * uh[l2] * qtmp * (epos[3 * l2 + q] - pc[q]) * d->edhx_leng[l2];
         }
         for (int q = 0; q < 3; ++q) v[q] = v[q] / d->hx_areag[node];
         proj_sphere(v, pc);
@@ -462,9 +576,14 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
       }
       double v[3], nr[3];
       for (int q = 0; q < 3; ++q) v[q] = 0.5 * vv[0][q] + 0.5 * vv[1][q];
-      proj_sphere(v, d->ed_c + 3 * l);
-      ed_nr(d, l, nr);
-      uhq[l] = -dot(v, nr);
+      if (hc) {
+        proj_sphere(v, d->edhx_c + 3 * l);
+        uhq[l] = dot(v, w->st_tg + 3 * l);
+      } else {
+        proj_sphere(v, d->ed_c + 3 * l);
+        ed_nr(d, l, nr);
+        uhq[l] = -dot(v, nr);
+      }
     }
   } else {
   /* coriolis_ed (trsk, PV form; gass18: q of the rhombus linking the two
@@ -477,7 +596,8 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
       int ei = edindex(d, l + 1, node);
       for (int j = 0; j < d->v_nnb[node]; ++j) {
         int k = d->v_ed[MX * node + j] - 1;
-        double sg = 1.0 * (double)d->hx_ttgout[MX * node + j] * (double)d->hx_tnrccw[MX * node + ei];
+        double sg = hc ? w->st_c[MX * node + j] * w->st_c[MX * node + ei]
+                       : 1.0 * (double)d->hx_ttgout[MX * node + j] * (double)d->hx_tnrccw[MX * node + ei];
         double signcor = sg < 0 ? -1.0 : 1.0;
         double qtmp;
         const int e3 = wrap == 2 ? hxedgeconn(d, l, k) : -1;
@@ -497,7 +617,7 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
     double s = 0.0;
     for (int j = 0; j < d->v_nnb[i]; ++j) {
       int l = d->v_ed[MX * i + j] - 1;
-      double signcor = (double)d->hx_ttgout[MX * i + j];
+      double signcor = w->st_c[MX * i + j];
       s = s + signcor * u[l] * d->edhx_leng[l];
     }
     divu[i] = s / d->hx_areag[i] / erad;
@@ -507,7 +627,7 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
     double z = 0.0;
     for (int l = 0; l < 3; ++l) {
       int ed = d->tr_ed[3 * k + l] - 1;
-      double signcor = (double)d->tr_tg[3 * k + l];
+      double signcor = w->st_t[3 * k + l];
       z = z + u[ed] * d->ed_leng[ed] * signcor;
     }
     zeta[k] = z / d->tr_areag[k] / erad;
@@ -519,8 +639,8 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
     if (sc->diffusion)
       for (int l = 0; l < ne; ++l) momeq[l] = momeq[l] + w->lapu[l];
     if (sc->hyperdiffusion) {
-      div_hx(d, erad, w->lapu, w->div_lap);
-      rel_vort_tr(d, erad, w->lapu, w->vort_lap);
+      div_hx(d, erad, w, w->lapu, w->div_lap);
+      rel_vort_tr(d, erad, w, w->lapu, w->vort_lap);
       diffusion(d, erad, sc->dif_hx, sc->dif_tr, w->div_lap, w->vort_lap, w, w->lap_lapu);
       for (int l = 0; l < ne; ++l) momeq[l] = momeq[l] - w->lap_lapu[l];
     }
@@ -562,6 +682,11 @@ int oracle_fields_alloc(const imx_mesh_desc *d, oracle_fields *w, int with_perot
   w->div_lap = calloc(nv, 8);
   w->vort_lap = calloc(nt, 8);
   w->h_rhb = calloc(ne, 8);
+  w->st_c = calloc((size_t)MX * nv, 8);
+  w->st_t = calloc(3 * nt, 8);
+  w->st_g = calloc(ne, 8);
+  w->st_tt = calloc(ne, 8);
+  w->st_tg = calloc(3 * ne, 8);
   w->massf = calloc(4 * nv, 8);
   w->momf = calloc(4 * ne, 8);
   w->h_new = calloc(nv, 8);
@@ -570,7 +695,8 @@ int oracle_fields_alloc(const imx_mesh_desc *d, oracle_fields *w, int with_perot
       !w->divuh || !w->ke_hx || !w->ghbk || !w->hbt || !w->grad_ghbk || !w->uhq_perp || !w->divu ||
       !w->zeta || !w->v_hx || !w->massf || !w->momf || !w->h_new || !w->u_new || !w->q_grad_ed ||
       !w->q_grad_tr || !w->lapu || !w->lap_lapu || !w->cdiv || !w->cvort || !w->gdiv || !w->gvort ||
-      !w->div_lap || !w->vort_lap || !w->h_rhb)
+      !w->div_lap || !w->vort_lap || !w->h_rhb || !w->st_c || !w->st_t || !w->st_g || !w->st_tt ||
+      !w->st_tg)
     return -2;
   return 0;
 }
@@ -580,7 +706,8 @@ void oracle_fields_free(oracle_fields *w) {
                     w->divuh, w->ke_hx, w->ghbk, w->hbt, w->grad_ghbk, w->uhq_perp, w->divu,
                     w->zeta, w->v_hx, w->massf, w->momf, w->h_new, w->u_new, w->q_grad_ed,
                     w->q_grad_tr, w->lapu, w->lap_lapu, w->cdiv, w->cvort, w->gdiv, w->gvort,
-                    w->div_lap, w->vort_lap, w->h_rhb};
+                    w->div_lap, w->vort_lap, w->h_rhb, w->st_c, w->st_t, w->st_g, w->st_tt,
+                    w->st_tg};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
   memset(w, 0, sizeof(*w));
 }

@@ -11,6 +11,7 @@
 
 /* scheme options beyond the trsk10/HTC defaults (par/swm.par) */
 typedef struct oracle_scheme {
+  int stag;              /* 0 HTC, 1 HC (staggering)                        */
   int wrap;              /* 0 trsk10, 1 pxt16, 2 gass18 (method wrapper)    */
   int apvm;              /* pv_stab = apvm                                  */
   double pvspar;         /* its parameter                                   */
@@ -39,6 +40,11 @@ typedef struct oracle_fields {
   double *cdiv, *cvort, *gdiv, *gvort;  /* diffusion_ed scratch: [nv], [nt], [ne], [ne] */
   double *div_lap, *vort_lap;           /* hyperdiffusion scratch: [nv], [nt] */
   double *h_rhb;                        /* gass18: h at the rhombi [ne]     */
+  /* staggering-dependent signs (filled by each oracle_tendency call) */
+  double *st_c;                         /* [nv*6] div/Perot sign per cell slot */
+  double *st_t;                         /* [nt*3] vorticity sign per triangle slot */
+  double *st_g, *st_tt;                 /* [ne] grad_ed and grad_ed_tg signs */
+  double *st_tg;                        /* [ne*3] HC: edhx%tg */
   int perot;                            /* compute v_hx even when unused    */
 } oracle_fields;
 

@@ -13,7 +13,7 @@ outputs as fixtures:
   ref_run_g3_<scheme>.bin   seed 7, 3 steps dt=900 with the scheme options of
                             SCHEMES (APVM, diffusion, hyperdiffusion with the
                             const/align/diam coefficients; the pxt16 and
-                            gass18 wrappers), plus v_hx,
+                            gass18 wrappers; HC staggering), plus v_hx,
                             q_grad_ed/tr, dif_coef_hx/tr, lapu, lap_lapu
 
 The seeded states are made by libimx (imx_ic_spectral) on the reference
@@ -44,6 +44,10 @@ SCHEMES = {
     "gass18": "wrap=gass18",
     "pxmix": "wrap=pxt16,pv=apvm,pvspar=0.4,k4=1e15,hyperdiffus=const",
     "gassapvm": "wrap=gass18,pv=apvm,pvspar=0.5",
+    "hc": "stag=HC",
+    "hcpx": "stag=HC,wrap=pxt16,k2=1e5,diffus=align",
+    "hcmix": "stag=HC,wrap=gass18,pv=apvm,pvspar=0.5",
+    "hck4": "stag=HC,pv=apvm,pvspar=0.3,k4=1e15,hyperdiffus=diam",
 }
 
 

@@ -75,6 +75,10 @@ SCHEMES = {
     "gass18": {"wrap": "gass18"},
     "pxmix": {"wrap": "pxt16", "pv": "apvm", "pvspar": 0.4, "k4": 1e15, "hyperdiffus": "const"},
     "gassapvm": {"wrap": "gass18", "pv": "apvm", "pvspar": 0.5},
+    "hc": {"stag": "HC"},
+    "hcpx": {"stag": "HC", "wrap": "pxt16", "k2": 1e5, "diffus": "align"},
+    "hcmix": {"stag": "HC", "wrap": "gass18", "pv": "apvm", "pvspar": 0.5},
+    "hck4": {"stag": "HC", "pv": "apvm", "pvspar": 0.3, "k4": 1e15, "hyperdiffus": "diam"},
 }
 SCHEME_FIELDS = ["v_hx", "q_grad_ed", "q_grad_tr", "lapu", "lap_lapu"]
 
@@ -114,6 +118,12 @@ def test_wrapper_planted_mistakes_fail(golden):
     for name, other in (("pxt16", "trsk10"), ("gass18", "trsk10"), ("pxt16", "gass18")):
         r = golden(f"ref_run_g3_{name}.bin")
         o = Oracle(mesh, scheme={"wrap": other, "dt": 900.0})
+        _, f = o.tendency(r["h0"], r["u0"])
+        assert not np.array_equal(f, r["momeq"]), (name, other)
+    # and the HC fixture rejects HTC
+    for name in ("hc", "hcpx"):
+        r = golden(f"ref_run_g3_{name}.bin")
+        o = Oracle(mesh, scheme=dict(SCHEMES[name], stag="HTC", dt=900.0))
         _, f = o.tendency(r["h0"], r["u0"])
         assert not np.array_equal(f, r["momeq"]), (name, other)
 
