@@ -26,6 +26,7 @@ FIELD_POS = {"h_ed": "e", "uh": "e", "eta": "t", "h_tr": "t", "q_tr": "t",
              "q_grad_ed": "e", "lapu": "e", "lap_lapu": "e"}
 COEF = {"const": 0, "align": 1, "diam": 2}
 WRAP = {"trsk10": 0, "pxt16": 1, "gass18": 2}
+STAG = {"HTC": 0, "HC": 1}
 
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _F64P = ctypes.POINTER(ctypes.c_double)
@@ -34,7 +35,8 @@ _F32P = ctypes.POINTER(ctypes.c_float)
 
 class Scheme(ctypes.Structure):
     """imx_scheme: APVM, diffusion and hyperdiffusion options (par/swm.par)."""
-    _fields_ = [("wrap", ctypes.c_int32), ("apvm", ctypes.c_int32), ("pvspar", ctypes.c_double),
+    _fields_ = [("stag", ctypes.c_int32), ("wrap", ctypes.c_int32), ("apvm", ctypes.c_int32),
+                ("pvspar", ctypes.c_double),
                 ("k2", ctypes.c_double),
                 ("diffus", ctypes.c_int32), ("k4", ctypes.c_double), ("hyperdiffus", ctypes.c_int32)]
 

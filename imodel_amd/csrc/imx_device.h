@@ -87,6 +87,14 @@ struct imx_swm {
   imx_scheme sc{};
   bool sc_apvm = false, sc_diff = false, sc_hyper = false;
   int sc_wrap = 0;                   // IMX_WRAP_*
+  bool sc_hc = false;                // HC staggering
+  int8_t *hc_c = nullptr;            // [MX][nv] HC: hx%nr per cell slot (div, Perot, Coriolis)
+  int8_t *hc_t = nullptr;            // [3][NT] HC: vorticity signs
+  int8_t *hc_g = nullptr;            // [NE] HC: grad_ed sign
+  int8_t *hc_tt = nullptr;           // [NE] HC: grad_ed_tg sign
+  double *hc_pc = nullptr;           // [3][NE] HC: Voronoi edge midpoints
+  double *hc_hb = nullptr;           // [3][NE] HC + pxt16: barycentric weights of h at edges
+  int32_t *hc_hv = nullptr;          // [3][NE] HC + pxt16: their cells
   double *t_w = nullptr;             // [3][NT] pxt16: barycentric coords; gass18: edge areas
   double *e_dw = nullptr;            // [2][NE] pxt16: distances edge -> its triangles
   double *e_nr = nullptr;            // [3][NE] pxt16: unit normal ed%nr

@@ -69,8 +69,13 @@ int swm_stencil_rings(const imx_swm *s);
 // both hold the square roots of the hyperdiffusion coefficients.
 void diffusion_coef(const imx_mesh_desc &d, const double *trhx, int wrap, double k2, int k2kind,
                     double k4, int k4kind, std::vector<double> &hx, std::vector<double> &tr);
-// bar_coord_tr of triangle k's circumcentre (src/smeshpack.f90:5200-5245)
+// bar_coord_tr (src/smeshpack.f90:5200-5245) of point p in triangle k, and
+// of triangle k's circumcentre
+void bary_at(const imx_mesh_desc &d, const double *p, int k, double b[3]);
 void bary_circumcentre(const imx_mesh_desc &d, int k, double b[3]);
+// insidetrmesh (src/smeshpack.f90:5329-5420), with robdet's binary128
+// determinant near the edges (:6597-6629); 0-based triangle
+bool insidetrmesh(const imx_mesh_desc &d, const double *p, int tr);
 // gethxedgeconnection(ed1, ed2) (src/smeshpack.f90:4127-4244), 0-based:
 // -2 if ed1 == ed2, the linking edge, or -1 if the edges are far apart
 int hxedgeconn(const imx_mesh_desc &d, int ed1, int ed2);

@@ -836,8 +836,9 @@ static void coef_at_cells(const imx_mesh_desc &d, double kmax, int kind, std::ve
   }
 }
 
-void bary_circumcentre(const imx_mesh_desc &d, int k, double b[3]) {
-  const double *p = d.tr_c + 3 * k;
+void bary_circumcentre(const imx_mesh_desc &d, int k, double b[3]) { bary_at(d, d.tr_c + 3 * k, k, b); }
+
+void bary_at(const imx_mesh_desc &d, const double *p, int k, double b[3]) {
   const double *p1 = d.v_p + 3 * (d.tr_v[3 * k] - 1), *p2 = d.v_p + 3 * (d.tr_v[3 * k + 1] - 1),
                *p3 = d.v_p + 3 * (d.tr_v[3 * k + 2] - 1);
   b[0] = std::fabs(det3(p, p2, p3));
@@ -845,6 +846,43 @@ void bary_circumcentre(const imx_mesh_desc &d, int k, double b[3]) {
   b[2] = std::fabs(det3(p, p1, p2));
   const double area = b[0] + b[1] + b[2];
   for (int i = 0; i < 3; ++i) b[i] = b[i] / area;
+}
+
+static double robdet(const double *p1, const double *p2, const double *p3) {
+  const __float128 q1[3] = {p1[0], p1[1], p1[2]}, q2[3] = {p2[0], p2[1], p2[2]},
+                   q3[3] = {p3[0], p3[1], p3[2]};
+  __float128 a[6];
+  a[0] = q1[0] * q2[1] * q3[2];
+  a[1] = q2[0] * q3[1] * q1[2];
+  a[2] = q3[0] * q1[1] * q2[2];
+  a[3] = -q3[0] * q2[1] * q1[2];
+  a[4] = -q1[0] * q3[1] * q2[2];
+  a[5] = -q2[0] * q1[1] * q3[2];
+  const __float128 t = a[0] + a[1] + a[2] + a[3] + a[4] + a[5];
+  return (double)t;
+}
+
+bool insidetrmesh(const imx_mesh_desc &d, const double *p, int tr) {
+  const double tol = kEpsSingle / 10000.0;
+  const double tol2 = 2.220446049250313e-16 * 100.0;  // epsilon(pi) of the FP64 pi
+  for (int l = 0; l < 3; ++l) {
+    const double *p1 = d.v_p + 3 * (d.tr_v[3 * tr + l] - 1);
+    const double *p2 = d.v_p + 3 * (d.tr_v[3 * tr + (l + 1) % 3] - 1);
+    double t[3];
+    sub3(p, p1, t);
+    if (norm3(t) < tol) return true;
+    double vn[3];
+    cross3(p1, p2, vn);
+    const double vnn = norm3(vn);
+    for (int q = 0; q < 3; ++q) vn[q] = vn[q] / vnn;
+    const double determ = dot3(vn, p);
+    if (determ < -tol) return false;
+    if (determ > tol) continue;
+    const double determ2 = robdet(p, p1, p2);
+    if (std::fabs(determ2) < tol2 || determ * determ2 < 0) continue;
+    if (determ2 < -tol2) return false;
+  }
+  return true;
 }
 
 int hxedgeconn(const imx_mesh_desc &d, int ed1, int ed2) {

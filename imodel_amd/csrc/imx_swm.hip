@@ -68,15 +68,26 @@ static inline int grid_bs(int n, int bs) {
 #define HTR_GUARD (1.1920928955078125e-07 / 100000.0)
 
 // ============================================================ REF kernels ==
-// scalar_hx2ed + scalar_elem_product (swm_operators.f90:88-98, :1567-1573)
+// scalar_hx2ed + scalar_elem_product (swm_operators.f90:88-114, :1567-1573):
+// the average of the two cells, or (HC with pxt16) barycentric weights hb of
+// the cells hv of the triangle holding the dual edge midpoint
 __global__ __launch_bounds__(TPB) void ref_edge_flux(int ne, const int32_t *__restrict__ e_v,
+                                                     const double *__restrict__ hb,
+                                                     const int32_t *__restrict__ hv,
                                                      const double *__restrict__ h,
                                                      const double *__restrict__ u,
                                                      double *__restrict__ h_ed,
                                                      double *__restrict__ uh) {
   const int l = xcd_index();
   if (l >= ne || e_v[l] < 0) return;
-  const double he = (h[e_v[l]] + h[e_v[ne + l]]) * 0.5;
+  double he;
+  if (hb) {
+    he = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) he = he + hb[i * ne + l] * h[hv[i * ne + l]];
+  } else {
+    he = (h[e_v[l]] + h[e_v[ne + l]]) * 0.5;
+  }
   h_ed[l] = he;
   uh[l] = u[l] * he;
 }
@@ -153,10 +164,18 @@ __global__ __launch_bounds__(TPB) void ref_edge_q(int ne, int qmode, const int32
   if (qmode == 1) {
     const double d1 = e_dw[l], d2 = e_dw[ne + l];
     q_ed[l] = (d2 * q_tr[k1] + d1 * q_tr[k2]) / (d1 + d2);
-  } else if (qmode == 2) {
-    const double a1 = t_area[k1], a2 = t_area[k2];
-    const double er = (a1 * eta[k1] + a2 * eta[k2]) / (a1 + a2);
-    const double hr = (a1 * h_tr[k1] + a2 * h_tr[k2]) / (a1 + a2);
+  } else if (qmode >= 2) {
+    // gass18: rhombus averages with the triangle areas, or (HC, qmode 3)
+    // the simple averages scalar_tr2ed takes on HC
+    double er, hr;
+    if (qmode == 2) {
+      const double a1 = t_area[k1], a2 = t_area[k2];
+      er = (a1 * eta[k1] + a2 * eta[k2]) / (a1 + a2);
+      hr = (a1 * h_tr[k1] + a2 * h_tr[k2]) / (a1 + a2);
+    } else {
+      er = 0.5 * (eta[k1] + eta[k2]);
+      hr = 0.5 * (h_tr[k1] + h_tr[k2]);
+    }
     if (fabs(hr) < HTR_GUARD) {
       flag[0] = 1;
       q_ed[l] = 0.0;
@@ -225,12 +244,14 @@ __global__ __launch_bounds__(TPB) void ref_edge_mom(
     const int8_t *__restrict__ c_ttg, const int8_t *__restrict__ c_tnr,
     const double *__restrict__ c_trskw, const double *__restrict__ ghbk,
     const double *__restrict__ uh, const double *__restrict__ q_ed,
-    const int32_t *__restrict__ e_ed3, double erad,
+    const int32_t *__restrict__ e_ed3, const int8_t *__restrict__ e_gs, double erad,
     int zero_mom, double *__restrict__ grad, double *__restrict__ uhq,
     double *__restrict__ momeq) {
   const int l = xcd_index();
   if (l >= ne || e_v[l] < 0) return;
-  const double g = (ghbk[e_v[ne + l]] - ghbk[e_v[l]]) / e_leng[l] / erad;
+  // grad_ed: HC orients the difference by edhx%nr (e_gs)
+  const double gs = e_gs ? (double)e_gs[l] : 1.0;
+  const double g = gs * (ghbk[e_v[ne + l]] - ghbk[e_v[l]]) / e_leng[l] / erad;
   grad[l] = g;
   double s = 0.0;
   const double ql = q_ed[l];
@@ -375,6 +396,7 @@ __global__ __launch_bounds__(TPB) void ref_apvm_q(int ne, int nt, int nv, const 
 __global__ __launch_bounds__(TPB) void ref_lap(int ne, const int32_t *__restrict__ e_v,
                                                const int32_t *__restrict__ e_sh,
                                                const int8_t *__restrict__ e_nrs,
+                                               const int8_t *__restrict__ e_gs,
                                                const double *__restrict__ e_leng,
                                                const double *__restrict__ e_dleng,
                                                const double *__restrict__ kh,
@@ -385,7 +407,8 @@ __global__ __launch_bounds__(TPB) void ref_lap(int ne, const int32_t *__restrict
   const int l = xcd_index();
   if (l >= ne || e_v[l] < 0) return;
   const int c1 = e_v[l], c2 = e_v[ne + l], k1 = e_sh[l], k2 = e_sh[ne + l];
-  const double g = (kh[c2] * div[c2] - kh[c1] * div[c1]) / e_leng[l] / erad;
+  const double gs = e_gs ? (double)e_gs[l] : 1.0;
+  const double g = gs * (kh[c2] * div[c2] - kh[c1] * div[c1]) / e_leng[l] / erad;
   const double gt = (double)e_nrs[l] * (kt[k2] * vort[k2] - kt[k1] * vort[k1]);
   const double gv = gt / (e_dleng[l] * erad);
   const double lap = g - gv;
@@ -431,9 +454,10 @@ __global__ __launch_bounds__(TPB) void ref_rvort(int nt, const int32_t *__restri
   out[k] = z / t_area[k] / erad;
 }
 
-// grad_ed (HTC) + coriolis_ed pered (pxt16, HTC: swm_operators.f90:1330-1368):
-// Perot reconstruction of uh q at the edge's two cells, averaged, projected
-// at the edge, minus its component along ed%nr -> momeq
+// grad_ed + coriolis_ed pered (pxt16, swm_operators.f90:1330-1368): Perot
+// reconstruction of uh q at the edge's two cells, averaged, projected at the
+// edge point, then (HTC) minus its ed%nr component or (HC) its edhx%tg
+// component -> momeq
 __global__ __launch_bounds__(TPB) void ref_edge_mom_pered(
     int ne, int nv, const int32_t *__restrict__ e_v, const double *__restrict__ e_leng,
     const double *__restrict__ e_dleng, const double *__restrict__ e_c,
@@ -441,11 +465,12 @@ __global__ __launch_bounds__(TPB) void ref_edge_mom_pered(
     const int32_t *__restrict__ c_ed, const int8_t *__restrict__ c_ttg,
     const double *__restrict__ c_p, const double *__restrict__ c_area,
     const double *__restrict__ ghbk, const double *__restrict__ uh, const double *__restrict__ q_ed,
-    double erad, int zero_mom, double *__restrict__ grad, double *__restrict__ uhq,
-    double *__restrict__ momeq) {
+    const int8_t *__restrict__ e_gs, double psign, double erad, int zero_mom, double *__restrict__ grad,
+    double *__restrict__ uhq, double *__restrict__ momeq) {
   const int l = xcd_index();
   if (l >= ne || e_v[l] < 0) return;
-  const double g = (ghbk[e_v[ne + l]] - ghbk[e_v[l]]) / e_leng[l] / erad;
+  const double gs = e_gs ? (double)e_gs[l] : 1.0;
+  const double g = gs * (ghbk[e_v[ne + l]] - ghbk[e_v[l]]) / e_leng[l] / erad;
   grad[l] = g;
   const double ql = q_ed[l];
   double w[2][3];
@@ -480,7 +505,9 @@ __global__ __launch_bounds__(TPB) void ref_edge_mom_pered(
   dp = dp + v0 * e_nr[l];
   dp = dp + v1 * e_nr[ne + l];
   dp = dp + v2 * e_nr[2 * ne + l];
-  const double p = -dp;
+  // HTC: minus the ed%nr component; HC: the edhx%tg component (e_nr holds
+  // the vector, psign the sign)
+  const double p = psign < 0.0 ? -dp : dp;
   uhq[l] = p;
   momeq[l] = zero_mom ? 0.0 : (-g) - p;
 }
@@ -863,17 +890,26 @@ static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, do
   const int nv = s->nv, NE = s->NE, NT = s->NT;
   hipStream_t st = s->stream;
   const int wrap = s->sc_wrap;
-  ref_edge_flux<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, h, u, s->f_h_ed, s->f_uh);
-  ref_tri<<<grid_for(NT), TPB, 0, st>>>(NT, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f,
+  // the staggering's signs and Perot edge points (HC: hx%nr, edhx%c, ...)
+  const bool hc = s->sc_hc;
+  const int8_t *csg = hc ? s->hc_c : s->c_ttg, *ctn = hc ? s->hc_c : s->c_tnr;
+  const int8_t *tsg = hc ? s->hc_t : s->t_tg, *egs = hc ? s->hc_g : nullptr;
+  const int8_t *ett = hc ? s->hc_tt : s->e_nrs;
+  const double *epc = hc ? s->hc_pc : s->e_c;
+  const int qmode = wrap == IMX_WRAP_GASS18 ? (hc ? 3 : 2) : (!hc && wrap == IMX_WRAP_PXT16 ? 1 : 0);
+  const bool hbary = hc && wrap == IMX_WRAP_PXT16;
+  ref_edge_flux<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, hbary ? s->hc_hb : nullptr, s->hc_hv, h, u,
+                                              s->f_h_ed, s->f_uh);
+  ref_tri<<<grid_for(NT), TPB, 0, st>>>(NT, s->t_ed, tsg, s->t_v, s->t_area, s->t_f,
                                         s->t_trhx, s->e_leng, u, h, wrap, s->t_w, s->f_h_ed,
                                         s->o.erad, s->o.testcase >= 2, s->f_eta, s->f_zeta,
                                         s->f_h_tr, s->f_q_tr, s->flag);
-  ref_edge_q<<<grid_for(NE), TPB, 0, st>>>(NE, wrap, s->e_sh, s->e_dw, s->t_area, s->f_eta, s->f_h_tr,
+  ref_edge_q<<<grid_for(NE), TPB, 0, st>>>(NE, qmode, s->e_sh, s->e_dw, s->t_area, s->f_eta, s->f_h_tr,
                                            s->f_q_tr, s->f_q_ed, s->flag);
   // Perot velocity at cells: APVM and the pxt16 kinetic energy
   // (vector_edhx2hx_perot, swm.f90:2516)
   if (s->sc_apvm || wrap == IMX_WRAP_PXT16)
-    ref_perot_hx<<<grid_for(nv), TPB, 0, st>>>(nv, NE, s->c_nnb, s->c_ed, s->c_ttg, s->e_c, s->c_p,
+    ref_perot_hx<<<grid_for(nv), TPB, 0, st>>>(nv, NE, s->c_nnb, s->c_ed, csg, epc, s->c_p,
                                                s->e_dleng, s->c_area, u, s->f_vhx);
   if (s->sc_apvm) {
     // grad q to edges and triangles, then the APVM PV at edges
@@ -885,35 +921,36 @@ static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, do
     ref_apvm_q<<<grid_for(NE), TPB, 0, st>>>(NE, NT, nv, s->e_sh, s->e_v, s->f_qg_tr, s->f_vhx,
                                              s->sc.pvspar, s->o.dt, s->f_q_ed);
   }
-  ref_cell<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area, s->e_leng,
+  ref_cell<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->c_ed, csg, s->c_area, s->e_leng,
                                          s->e_dleng, u, s->f_uh, h, s->bt,
                                          wrap == IMX_WRAP_PXT16 ? s->f_vhx : nullptr, s->o.erad,
                                          s->o.grav, s->f_divuh, masseq, s->f_divu, s->f_ke, s->f_ghbk);
   if (wrap == IMX_WRAP_PXT16)
-    ref_edge_mom_pered<<<grid_for(NE), TPB, 0, st>>>(NE, nv, s->e_v, s->e_leng, s->e_dleng, s->e_c,
-                                                     s->e_nr, s->c_nnb, s->c_ed, s->c_ttg, s->c_p,
-                                                     s->c_area, s->f_ghbk, s->f_uh, s->f_q_ed, s->o.erad,
-                                                     s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
+    ref_edge_mom_pered<<<grid_for(NE), TPB, 0, st>>>(NE, nv, s->e_v, s->e_leng, s->e_dleng, epc,
+                                                     s->e_nr, s->c_nnb, s->c_ed, csg, s->c_p,
+                                                     s->c_area, s->f_ghbk, s->f_uh, s->f_q_ed, egs,
+                                                     hc ? 1.0 : -1.0, s->o.erad, s->o.testcase <= 1,
+                                                     s->f_grad, s->f_uhq, momeq);
   else
     ref_edge_mom<<<grid_for(NE), TPB, 0, st>>>(NE, nv, s->e_v, s->e_loc, s->e_leng, s->e_dleng,
-                                               s->c_nnb, s->c_ed, s->c_ttg, s->c_tnr, s->c_trskw,
+                                               s->c_nnb, s->c_ed, csg, ctn, s->c_trskw,
                                                s->f_ghbk, s->f_uh, s->f_q_ed,
-                                               wrap == IMX_WRAP_GASS18 ? s->e_ed3 : nullptr, s->o.erad,
+                                               wrap == IMX_WRAP_GASS18 ? s->e_ed3 : nullptr, egs, s->o.erad,
                                                s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
   if (s->sc_diff || s->sc_hyper) {
     // diffusion_ed of u (momeq + lapu with K2), then with K4 diffusion_ed of
     // lapu's divergence and vorticity (momeq - lap_lapu) (swm.f90:2575-2590);
     // testcase <= 1 leaves momeq zero
     const int live = s->o.testcase > 1;
-    ref_lap<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, s->e_sh, s->e_nrs, s->e_leng, s->e_dleng, s->dif_hx,
+    ref_lap<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, s->e_sh, ett, egs, s->e_leng, s->e_dleng, s->dif_hx,
                                           s->dif_tr, s->f_divu, s->f_zeta, s->o.erad,
                                           (live && s->sc_diff) ? 1 : 0, s->f_lapu, momeq);
     if (s->sc_hyper) {
-      ref_div<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area, s->e_dleng,
+      ref_div<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->c_ed, csg, s->c_area, s->e_dleng,
                                             s->f_lapu, s->o.erad, s->f_divl);
-      ref_rvort<<<grid_for(NT), TPB, 0, st>>>(NT, s->t_ed, s->t_tg, s->t_area, s->e_leng, s->f_lapu,
+      ref_rvort<<<grid_for(NT), TPB, 0, st>>>(NT, s->t_ed, tsg, s->t_area, s->e_leng, s->f_lapu,
                                               s->o.erad, s->f_vortl);
-      ref_lap<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, s->e_sh, s->e_nrs, s->e_leng, s->e_dleng,
+      ref_lap<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, s->e_sh, ett, egs, s->e_leng, s->e_dleng,
                                             s->dif_hx, s->dif_tr, s->f_divl, s->f_vortl, s->o.erad,
                                             live ? 2 : 0, s->f_lap2, momeq);
     }
@@ -1010,11 +1047,11 @@ static void launch_diag(imx_swm *s, const T *hs, const T *us, double href, int c
   hipStream_t st = s->stream;
   const int nc = c1 - c0, ne = 3 * nc, nt = 2 * nc;
   const int gv = (nc + TPB - 1) / TPB, ge = (ne + TPB - 1) / TPB, gt = (nt + TPB - 1) / TPB;
-  diag_cell<T><<<gv, TPB, 0, st>>>(c0, nc, s->nv, s->c_nnb, s->c_ed, s->c_ttg, s->c_area,
+  diag_cell<T><<<gv, TPB, 0, st>>>(c0, nc, s->nv, s->c_nnb, s->c_ed, s->sc_hc ? s->hc_c : s->c_ttg, s->c_area,
                                    s->e_dleng, hs, us, href, s->o.grav, s->o.erad, s->part);
   diag_edge<T><<<ge, TPB, 0, st>>>(3 * c0, ne, s->NE, s->e_v, s->e_leng, s->e_dleng, hs, us, href,
                                    s->part + (size_t)gv * NPART);
-  diag_tri<T><<<gt, TPB, 0, st>>>(2 * c0, nt, s->NT, s->t_ed, s->t_tg, s->t_v, s->t_area, s->t_f,
+  diag_tri<T><<<gt, TPB, 0, st>>>(2 * c0, nt, s->NT, s->t_ed, s->sc_hc ? s->hc_t : s->t_tg, s->t_v, s->t_area, s->t_f,
                                   s->t_trhx, s->e_leng, hs, us, href, s->o.erad,
                                   s->part + (size_t)(gv + ge) * NPART);
   diag_final<<<1, TPB, 0, st>>>(gv + ge + gt, s->part, s->diag);
@@ -1673,6 +1710,10 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
                           "scalar_hx2tr needs edge values for Gassmann's scheme)");
   if (sc->wrap != IMX_WRAP_TRSK10 && (!d->ed_c || !d->tr_c || !d->v_p))
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: pxt16/gass18 need ed_c, tr_c and v_p");
+  if (sc->stag != IMX_STAG_HTC && sc->stag != IMX_STAG_HC)
+    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: bad option value (stag)");
+  if (sc->stag == IMX_STAG_HC && (!d->ed_c || !d->tr_c || !d->v_p || !d->edhx_c))
+    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: HC needs ed_c, edhx_c, tr_c and v_p");
   if ((sc->k2 > 0.0 || sc->k4 > 0.0) && (!d->v_tr || !d->tr_c || !d->v_nb))
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: diffusion needs v_tr, v_nb and tr_c");
   if (sc->apvm && (!d->ed_c || !d->tr_c || !d->v_p))
@@ -1680,7 +1721,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   const int nv = s->nv, NE = s->NE, NT = s->NT;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
   imx::Alloc &A = s->A;
-  if (!s->e_nrs && (sc->apvm || sc->k2 > 0.0 || sc->k4 > 0.0 || sc->wrap != IMX_WRAP_TRSK10)) {
+  if (!s->e_nrs &&
+      (sc->apvm || sc->k2 > 0.0 || sc->k4 > 0.0 || sc->wrap != IMX_WRAP_TRSK10 || sc->stag != IMX_STAG_HTC)) {
     // geometry of the options, internal numbering, points component-major
     std::vector<int8_t> nrs(NE, 1);
     std::vector<double> ec((size_t)3 * NE, 0.0), tc((size_t)3 * NT, 0.0), cp((size_t)3 * nv, 0.0);
@@ -1716,7 +1758,74 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     s->e_dw = A.get<double>((size_t)2 * NE);
     s->e_nr = A.get<double>((size_t)3 * NE);
     s->e_ed3 = A.get<int32_t>((size_t)2 * MX * NE);
+    s->hc_c = A.get<int8_t>((size_t)MX * nv);
+    s->hc_t = A.get<int8_t>((size_t)3 * NT);
+    s->hc_g = A.get<int8_t>(NE);
+    s->hc_tt = A.get<int8_t>(NE);
+    s->hc_pc = A.get<double>((size_t)3 * NE);
+    s->hc_hb = A.get<double>((size_t)3 * NE);
+    s->hc_hv = A.get<int32_t>((size_t)3 * NE);
     if (A.err != hipSuccess) return imx::hip_fail(A.err, "imx_swm_set_scheme");
+  }
+  if (sc->stag == IMX_STAG_HC) {
+    // HC signs from the geometry: s_e = sign(ed%tg . edhx%nr) turns the
+    // HTC indicators into hx%nr = ttgout s_e and the vorticity signs
+    // tr%tg s_e (smeshpack.f90:1085-1116); grad_ed and grad_ed_tg orient
+    // by edhx%nr and edhx%tg (swm_operators.f90:531-534, :755-760)
+    std::vector<int8_t> sgn(d->ne), hcc((size_t)MX * nv, 1), hct((size_t)3 * NT, 1), hg(NE, 1), htt(NE, 1);
+    std::vector<double> pc((size_t)3 * NE, 0.0), tgv((size_t)3 * NE, 0.0), hb((size_t)3 * NE, 0.0);
+    std::vector<int32_t> hv((size_t)3 * NE, 0);
+    for (int eo = 0; eo < d->ne; ++eo) {
+      const int e = en[eo];
+      const double *v1 = d->v_p + 3 * (d->ed_v[2 * eo] - 1), *v2 = d->v_p + 3 * (d->ed_v[2 * eo + 1] - 1);
+      const double *t1 = d->tr_c + 3 * (d->ed_sh[2 * eo] - 1), *t2 = d->tr_c + 3 * (d->ed_sh[2 * eo + 1] - 1);
+      const double *hxc = d->edhx_c + 3 * eo;
+      double enr[3], etg[3], hnr[3], htg[3];
+      imx::cross3(v1, v2, enr);
+      double n = imx::norm3(enr);
+      for (int q = 0; q < 3; ++q) enr[q] = enr[q] / n;
+      imx::cross3(enr, d->ed_c + 3 * eo, etg);
+      imx::cross3(t1, t2, hnr);
+      n = imx::norm3(hnr);
+      for (int q = 0; q < 3; ++q) hnr[q] = hnr[q] / n;
+      imx::cross3(hnr, hxc, htg);
+      n = imx::norm3(htg);
+      for (int q = 0; q < 3; ++q) htg[q] = htg[q] / n;
+      sgn[eo] = std::signbit(imx::dot3(etg, hnr)) ? -1 : 1;
+      double dv[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+      double dt3[3] = {t2[0] - t1[0], t2[1] - t1[1], t2[2] - t1[2]};
+      hg[e] = std::signbit(imx::dot3(hnr, dv)) ? -1 : 1;
+      htt[e] = std::signbit(imx::dot3(htg, dt3)) ? -1 : 1;
+      for (int q = 0; q < 3; ++q) {
+        pc[(size_t)q * NE + e] = hxc[q];
+        tgv[(size_t)q * NE + e] = htg[q];
+      }
+      if (sc->wrap == IMX_WRAP_PXT16) {  // h at the dual edge midpoint (initialize_gridprop, swm.f90:1007-1018)
+        const int kt = imx::insidetrmesh(*d, hxc, d->ed_sh[2 * eo] - 1) ? d->ed_sh[2 * eo] - 1 : d->ed_sh[2 * eo + 1] - 1;
+        double b[3];
+        imx::bary_at(*d, hxc, kt, b);
+        for (int i = 0; i < 3; ++i) {
+          hb[(size_t)i * NE + e] = b[i];
+          hv[(size_t)i * NE + e] = cn[d->tr_v[3 * kt + i] - 1];
+        }
+      }
+    }
+    for (int co = 0; co < nv; ++co)
+      for (int j = 0; j < d->v_nnb[co]; ++j)
+        hcc[(size_t)j * nv + cn[co]] = (int8_t)((d->hx_ttgout[MX * co + j] < 0 ? -1 : 1) * sgn[d->v_ed[MX * co + j] - 1]);
+    for (int ko = 0; ko < d->nt; ++ko)
+      for (int l = 0; l < 3; ++l)
+        hct[(size_t)l * NT + tn[ko]] = (int8_t)((d->tr_tg[3 * ko + l] < 0 ? -1 : 1) * sgn[d->tr_ed[3 * ko + l] - 1]);
+    HIPTRY(hipMemcpy(s->hc_c, hcc.data(), hcc.size(), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->hc_t, hct.data(), hct.size(), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->hc_g, hg.data(), hg.size(), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->hc_tt, htt.data(), htt.size(), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->hc_pc, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->hc_hb, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->hc_hv, hv.data(), hv.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+    // pered on HC takes the edhx%tg component
+    if (sc->wrap == IMX_WRAP_PXT16)
+      HIPTRY(hipMemcpy(s->e_nr, tgv.data(), tgv.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
   }
   if (sc->wrap == IMX_WRAP_PXT16) {
     // barycentric coordinates of the circumcentres (initialize_gridprop,
@@ -1738,7 +1847,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     }
     HIPTRY(hipMemcpy(s->t_w, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
     HIPTRY(hipMemcpy(s->e_dw, dw.data(), dw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
-    HIPTRY(hipMemcpy(s->e_nr, nr.data(), nr.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+    if (sc->stag == IMX_STAG_HTC)  // HC holds edhx%tg there (set above)
+      HIPTRY(hipMemcpy(s->e_nr, nr.data(), nr.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
   } else if (sc->wrap == IMX_WRAP_GASS18) {
     // areas of the sub-triangles (circumcentre, edge) and the linking edge
     // of each TRiSK term (gethxedgeconnection)
@@ -1784,6 +1894,7 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   s->sc_diff = sc->k2 != 0.0;
   s->sc_hyper = sc->k4 != 0.0;
   s->sc_wrap = sc->wrap;
+  s->sc_hc = sc->stag == IMX_STAG_HC;
   imx::drop_graph(s);  // the captured step holds the previous kernel sequence
   return IMX_OK;
 }

@@ -31,7 +31,7 @@ module imx_c
   integer(c_int32_t), parameter :: IMX_COEF_CONST = 0, IMX_COEF_ALIGN = 1, IMX_COEF_DIAM = 2
   integer(c_int32_t), parameter :: IMX_WRAP_TRSK10 = 0, IMX_WRAP_PXT16 = 1, IMX_WRAP_GASS18 = 2
   type, bind(C) :: imx_scheme
-    integer(c_int32_t) :: wrap, apvm
+    integer(c_int32_t) :: stag, wrap, apvm
     real(c_double) :: pvspar, k2
     integer(c_int32_t) :: diffus
     real(c_double) :: k4

@@ -198,13 +198,14 @@ class Swm:
         capi.check(capi.lib().imx_swm_get_state(self._h, capi.f64p(h), capi.f64p(u)), "imx_swm_get_state")
         return h, u
 
-    def set_scheme(self, wrap="trsk10", pv="none", pvspar=0.5, k2=0.0, diffus="const", k4=0.0,
+    def set_scheme(self, stag="HTC", wrap="trsk10", pv="none", pvspar=0.5, k2=0.0, diffus="const", k4=0.0,
                    hyperdiffus="const"):
-        """Scheme options of par/swm.par on an FP64_REF handle: the method
-        wrapper ("trsk10" | "pxt16" | "gass18"), APVM (pv="apvm", parameter
+        """Scheme options of par/swm.par on an FP64_REF handle: the
+        staggering ("HTC" | "HC"), the method wrapper ("trsk10" | "pxt16" |
+        "gass18"), APVM (pv="apvm", parameter
         pvspar), diffusion K2_max with coefficient function diffus,
         hyperdiffusion K4_max with hyperdiffus ("const" | "align" | "diam")."""
-        sc = capi.Scheme(wrap=capi.WRAP[wrap], apvm=int(pv == "apvm"), pvspar=float(pvspar), k2=float(k2),
+        sc = capi.Scheme(stag=capi.STAG[stag], wrap=capi.WRAP[wrap], apvm=int(pv == "apvm"), pvspar=float(pvspar), k2=float(k2),
                          diffus=capi.COEF[diffus], k4=float(k4), hyperdiffus=capi.COEF[hyperdiffus])
         capi.check(capi.lib().imx_swm_set_scheme(self._h, ctypes.byref(self.mesh.desc()), ctypes.byref(sc)),
                    "imx_swm_set_scheme")

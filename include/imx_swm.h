@@ -135,8 +135,10 @@ typedef struct imx_swm  imx_swm;   /* one SWM instance; owns device memory */
 #define IMX_FIELD_LAP_LAPU 15  /* [ne]  hyperdiffusion_ed of u, with K4        */
 
 /* Scheme options of par/swm.par beyond the trsk10/HTC defaults
-   (swmpars, src/swm_data.f90:413-424, :438-551), for FP64_REF handles:
-     wrap    method wrapper (HTC staggering):
+   (swmpars, src/swm_data.f90:413-424, :430-551), for FP64_REF handles:
+     stag    IMX_STAG_HTC (the default) or IMX_STAG_HC: u at the Voronoi
+             edge midpoints instead of the primal/dual edge crossings
+     wrap    method wrapper:
              IMX_WRAP_TRSK10  TRiSK (Ringler et al. 2010), the default
              IMX_WRAP_PXT16   Peixoto 2016: barycentric h at triangles,
                               distance-weighted q at edges, Perot kinetic
@@ -156,10 +158,13 @@ typedef struct imx_swm  imx_swm;   /* one SWM instance; owns device memory */
 #define IMX_COEF_CONST 0
 #define IMX_COEF_ALIGN 1
 #define IMX_COEF_DIAM  2
+#define IMX_STAG_HTC 0
+#define IMX_STAG_HC  1
 #define IMX_WRAP_TRSK10 0
 #define IMX_WRAP_PXT16  1
 #define IMX_WRAP_GASS18 2
 typedef struct imx_scheme {
+  int32_t stag;         /* IMX_STAG_*                                         */
   int32_t wrap;         /* IMX_WRAP_*                                         */
   int32_t apvm;         /* 0/1                                                */
   double  pvspar;       /* APVM parameter                                     */

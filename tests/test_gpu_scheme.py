@@ -1,5 +1,5 @@
 """Scheme options of the FP64_REF path on the GPU (imx_swm_set_scheme):
-the pxt16 and gass18 wrappers, APVM, diffusion and hyperdiffusion with their
+HC staggering, the pxt16 and gass18 wrappers, APVM, diffusion and hyperdiffusion with their
 coefficient functions, bit for bit against the reference's own golden vectors (tests/golden/make_golden.py)
 and against the oracle on a larger grid; the normal-mode Jacobian with the
 widened stencil."""
@@ -47,7 +47,8 @@ def test_scheme_tendency_and_steps_match_reference(golden, name, sfc):
 G5 = {"mix": {"k2": 1e5, "diffus": "diam", "k4": 1e14, "hyperdiffus": "align", "pv": "apvm", "pvspar": 0.5},
       "k2align": {"k2": 1e5, "diffus": "align"},
       "pxt16": {"wrap": "pxt16", "pv": "apvm", "pvspar": 0.5, "k2": 1e5, "diffus": "const"},
-      "gass18": {"wrap": "gass18", "pv": "apvm", "pvspar": 0.5}}
+      "gass18": {"wrap": "gass18", "pv": "apvm", "pvspar": 0.5},
+      "hcpx": {"stag": "HC", "wrap": "pxt16", "pv": "apvm", "pvspar": 0.5, "k4": 1e14, "hyperdiffus": "align"}}
 
 
 @pytest.mark.parametrize("name", sorted(G5))
@@ -68,7 +69,7 @@ def test_scheme_g5_matches_oracle(name):
     assert np.array_equal(hg, ho) and np.array_equal(ug, uo)
 
 
-@pytest.mark.parametrize("name", ["apvm", "mix", "pxt16", "gass18", "gassapvm"])
+@pytest.mark.parametrize("name", ["apvm", "mix", "pxt16", "gass18", "gassapvm", "hc", "hcpx", "hcmix"])
 def test_scheme_jacobian_equals_per_column(name):
     say("scheme jacobian g2", name)
     m = Mesh.icos(2)
