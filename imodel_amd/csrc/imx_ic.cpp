@@ -110,6 +110,34 @@ int imx_ic_tc2(const imx_mesh *m, double erad, double *h, double *u) {
   return IMX_OK;
 }
 
+// initialize_fields, case(2) with HC staggering (src/swm.f90:1262-1270): u
+// is the component of the zonal flow along edhx%nr at the Voronoi edge
+// midpoint, whose latitude cart2sph gives (midpoint_edhx, smeshpack.f90:6311-6334).
+int imx_ic_tc2_hc(const imx_mesh *m, double erad, double *h, double *u) {
+  if (!m || !h || !u) return imx::set_error(IMX_EINVAL, "imx_ic_tc2_hc: NULL argument");
+  if (m->edhx_c.size() < (size_t)3 * m->ne || m->edhx_nr.size() < (size_t)3 * m->ne)
+    return imx::set_error(IMX_EINVAL, "imx_ic_tc2_hc: the mesh has no Voronoi edge vectors");
+  const double pi = 3.1415927410125732;  // constants.f90:31 (single-rounded)
+  const double pi2 = 2.0 * pi;
+  const double day2sec = 86400.0;
+  const double gravi = 1.0 / 9.80616, omega = 7.292e-5;
+  const double u0 = pi2 * erad / (12.0 * day2sec);
+  const double h0 = 2.94e4 * gravi;
+  const double h_ct = (erad * omega * u0 + (u0 * u0) / 2.0) * gravi;
+  for (int i = 0; i < m->nv; ++i) {
+    double s = std::sin(m->v_lat[i]);
+    h[i] = h0 - h_ct * (s * s);
+  }
+  for (int l = 0; l < m->ne; ++l) {
+    const double *c = &m->edhx_c[3 * l];
+    double lon, lat, v[3];
+    imx::cart2sph(c[0], c[1], c[2], &lon, &lat);
+    imx::vec_sph2cart(u0 * std::cos(lat), 0.0, c, v);
+    u[l] = imx::dot3(v, &m->edhx_nr[3 * l]);
+  }
+  return IMX_OK;
+}
+
 // TC2's exact potential vorticity at triangle circumcentres
 // (src/swm.f90:1293-1296): eta_ct sin(lat) / (h0 - h_ct sin(lat)^2).
 int imx_ic_tc2_pv(const imx_mesh *m, double erad, double *q) {

@@ -118,6 +118,13 @@ module imx_c
       real(c_double) :: out(*)
       integer(c_int) :: rc
     end function
+    function imx_ic_tc2_hc(m, erad, h, u) bind(C, name='imx_ic_tc2_hc') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: m
+      real(c_double), value :: erad
+      real(c_double) :: h(*), u(*)
+      integer(c_int) :: rc
+    end function
     function imx_ic_tc2(m, erad, h, u) bind(C, name='imx_ic_tc2') result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: m

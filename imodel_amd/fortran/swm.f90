@@ -43,7 +43,11 @@ contains
     call imx_check(imx_mesh_desc_of(meshp, mdesc), 'imx_mesh_desc_of')
     allocate(h(mdesc%nv), u(mdesc%ne))
     if (testcase == 2) then
-      call imx_check(imx_ic_tc2(meshp, erad, h, u), 'imx_ic_tc2')
+      if (trim(stag) == 'HC') then
+        call imx_check(imx_ic_tc2_hc(meshp, erad, h, u), 'imx_ic_tc2_hc')
+      else
+        call imx_check(imx_ic_tc2(meshp, erad, h, u), 'imx_ic_tc2')
+      end if
     else
       ! seeded spectral state (new test case 100): h' 5% of H0, 20 m/s rms wind
       call imx_check(imx_ic_spectral(mdesc, int(seed, c_int64_t), 16_c_int32_t, 2.94e4_r8/grav, &
@@ -61,9 +65,11 @@ contains
     write(atmp, '(i8)') nint(dt)
     ! stag, wrapper (or the explicit method names), area method (swm_data.f90:640-656)
     if (trim(mtdwrapper) /= 'none') then
-      swmname = trim(swmname)//"_dt"//trim(adjustl(atmp))//"_HTC_"//trim(adjustl(mtdwrapper))//"_areageo"
+      swmname = trim(swmname)//"_dt"//trim(adjustl(atmp))//"_"//trim(stag)//"_"//trim(adjustl(mtdwrapper))// &
+                "_areageo"
     else
-      swmname = trim(swmname)//"_dt"//trim(adjustl(atmp))//"_HTC_vrectrsk_crectrsk_sinttrsk_grdtrsk_areageo"
+      swmname = trim(swmname)//"_dt"//trim(adjustl(atmp))//"_"//trim(stag)// &
+                "_vrectrsk_crectrsk_sinttrsk_grdtrsk_areageo"
     end if
     ! scheme-option suffixes of swmpars (src/swm_data.f90:658-683)
     if (trim(pv_stab) /= 'none') then
@@ -93,10 +99,11 @@ contains
     call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
     if (scheme_options()) then
       if (trim(precision) /= 'fp64_ref') then
-        print*, "ERROR: pxt16 / gass18 / APVM / diffusion / hyperdiffusion run on the fp64_ref path ", &
-          "(par/swm_gpu.par)"
+        write(*, '(a)') " ERROR: HC, pxt16, gass18, APVM, diffusion and hyperdiffusion run on the fp64_ref path"
+        write(*, '(a)') "        (par/swm_gpu.par)"
         stop 1
       end if
+      sc%stag = merge(1, 0, trim(stag) == 'HC')
       sc%wrap = wrap_code()
       sc%apvm = merge(1, 0, trim(pv_stab) == 'apvm')
       sc%pvspar = pvspar

@@ -3,9 +3,9 @@
 !   par/mesh.par  as getparameters  (src/smeshpack.f90:7343-7416), argv
 !                 [meshpar] [nv] override as in the reference
 !   par/swm.par   as swmpars        (src/swm_data.f90:338-693), the
-!                 HTC/geo subset this build runs: trsk10 on every path;
-!                 the pxt16 and gass18 wrappers, APVM, diffusion and
-!                 hyperdiffusion on the fp64_ref path
+!                 geo-area subset this build runs: HTC trsk10 on every
+!                 path; HC staggering (TC2), the pxt16 and gass18 wrappers,
+!                 APVM, diffusion and hyperdiffusion on the fp64_ref path
 !   par/swm_gpu.par (new, optional): precision, seed, SFC and graph switches
 !=======================================================================
 module swm_par
@@ -141,7 +141,8 @@ contains
     close(fu)
 
     ! the scheme this build runs (the reference defaults, par/swm.par)
-    if (trim(stag) /= 'HTC') call unsupported('staggering', stag)
+    if (trim(stag) /= 'HTC' .and. trim(stag) /= 'HC') call unsupported('staggering', stag)
+    if (trim(stag) == 'HC' .and. testcase /= 2) call unsupported('HC staggering with test case (2 only)', 'other')
     if (wrap_code() < 0) call unsupported('method wrapper', mtdwrapper)
     if (wrap_code() == 2 .and. (k2_max > 0._r8 .or. k4_max > 0._r8)) then
       ! as the reference's scalar_hx2tr (src/swm_operators.f90:271-273)
@@ -234,7 +235,8 @@ contains
 
   ! a wrapper other than trsk10, APVM, diffusion or hyperdiffusion selected
   logical function scheme_options() result(on)
-    on = wrap_code() /= 0 .or. trim(pv_stab) == 'apvm' .or. k2_max > 0._r8 .or. k4_max > 0._r8
+    on = wrap_code() /= 0 .or. trim(pv_stab) == 'apvm' .or. k2_max > 0._r8 .or. k4_max > 0._r8 .or. &
+         trim(stag) == 'HC' 
   end function scheme_options
 
   subroutine unsupported(what, val)

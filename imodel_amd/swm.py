@@ -114,13 +114,16 @@ class Mesh:
                                            capi.f32p(out)), "imx_remap_ll")
         return out
 
-    def tc2(self, erad=ERAD_REF):
-        """Williamson TC2 state (initialize_fields case 2) on a library mesh."""
+    def tc2(self, erad=ERAD_REF, stag="HTC"):
+        """Williamson TC2 state (initialize_fields case 2) on a library mesh,
+        u for HTC (along ed%tg at the edge crossings) or HC (along edhx%nr
+        at the Voronoi edge midpoints)."""
         if self._handle is None:
             raise ValueError("tc2 needs a library-built mesh")
         h = np.zeros(self.nv)
         u = np.zeros(self.ne)
-        capi.check(capi.lib().imx_ic_tc2(self._handle, erad, capi.f64p(h), capi.f64p(u)), "imx_ic_tc2")
+        fn = capi.lib().imx_ic_tc2_hc if stag == "HC" else capi.lib().imx_ic_tc2
+        capi.check(fn(self._handle, erad, capi.f64p(h), capi.f64p(u)), "imx_ic_tc2")
         return h, u
 
     def spectral(self, seed, lmax=16, h0=H0_GEO, hfrac=0.05, urms=20.0, erad=ERAD_REF):

@@ -211,6 +211,9 @@ int  imx_gridprop(const imx_mesh_desc *d, double *tr_trhx_area,
 /* Williamson TC2 as initialize_fields case(2) with HTC (src/swm.f90:1205-1247)
    on a library-built mesh (needs ed%c%lat, ed%tg and v%lat). */
 int  imx_ic_tc2(const imx_mesh *m, double erad, double *h, double *u);
+/* the same with HC staggering: u along edhx%nr at the Voronoi edge
+   midpoints (src/swm.f90:1262-1270) */
+int  imx_ic_tc2_hc(const imx_mesh *m, double erad, double *h, double *u);
 /* TC2's exact potential vorticity q_tr_exact at the triangle circumcentres
    (src/swm.f90:1293-1296), for the error columns of write_evol_error. */
 int  imx_ic_tc2_pv(const imx_mesh *m, double erad, double *q_tr);

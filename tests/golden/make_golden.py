@@ -10,6 +10,7 @@ outputs as fixtures:
   ref_run_g3_tc2.bin        TC2: tendency intermediates, diag, 5 RK4 steps dt=900
   ref_run_g3_spec.bin       seeded spectral state (seed 7): same
   ref_run_g2_spec96.bin     seeded spectral state (seed 3): 96 steps dt=900
+  ref_run_g3_tc2hc.bin      TC2 with HC staggering, 1 step
   ref_run_g3_<scheme>.bin   seed 7, 3 steps dt=900 with the scheme options of
                             SCHEMES (APVM, diffusion, hyperdiffusion with the
                             const/align/diam coefficients; the pxt16 and
@@ -85,6 +86,9 @@ def main():
         for name, spec in SCHEMES.items():
             run(["run", "build:3:eqs", "state.bin", "3", "900", "0", "out.bin", spec], td)
             shutil.copy(os.path.join(td, "out.bin"), os.path.join(HERE, f"ref_run_g3_{name}.bin"))
+        # TC2 with HC staggering (initialize_fields' HC branch), one step
+        run(["run", "build:3:eqs", "tc2", "1", "900", "0", "out.bin", "stag=HC"], td)
+        shutil.copy(os.path.join(td, "out.bin"), os.path.join(HERE, "ref_run_g3_tc2hc.bin"))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".bin"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -243,3 +243,24 @@ def test_fortran_driver_wrappers_bitexact(wrap):
     o = Oracle(m.arrays, scheme={"wrap": wrap, "dt": 900.0})
     h, u = o.step(h0, u0, 900.0, 96)
     assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
+
+
+def test_fortran_driver_hc_tc2_bitexact():
+    """HC staggering from par/swm.par: TC2 initialised on the Voronoi edge
+    midpoints, a day on glevel 3 equal to the oracle with HC bit for bit."""
+    print("[gpu-case] fortran imodel g3 tc2 fp64_ref HC", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 48, "fp64_ref")
+        p = os.path.join(td, "par", "swm.par")
+        swm = open(p).read().splitlines()
+        swm[8] = "HC"
+        open(p, "w").write("\n".join(swm) + "\n")
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        st = imxbin.read(os.path.join(td, "data", "swm_tc2_dt900_HC_trsk10_areageo_state_icos_eqs_nopt_3.bin"))
+    m = Mesh.icos(3)
+    h0, u0 = m.tc2(stag="HC")
+    o = Oracle(m.arrays, scheme={"stag": "HC", "dt": 900.0})
+    h, u = o.step(h0, u0, 900.0, 96)
+    assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)

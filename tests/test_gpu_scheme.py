@@ -105,3 +105,20 @@ def test_scheme_errors():
         f0 = s.tendency(h, u)[1]
     with Swm(m, precision="fp64_ref", dt=900.0) as p:
         assert np.array_equal(p.tendency(h, u)[1], f0)
+
+
+@pytest.mark.parametrize("sfc", [False, True])
+def test_hc_tc2_matches_reference(golden, sfc):
+    """TC2 with HC staggering as the reference initialises it: tendency and
+    one step bit for bit."""
+    say("hc tc2 vs reference", sfc)
+    mesh = Mesh(golden("ref_mesh_g3.bin"))
+    r = golden("ref_run_g3_tc2hc.bin")
+    with Swm(mesh, precision="fp64_ref", dt=900.0, use_sfc=sfc) as s:
+        s.set_scheme(stag="HC")
+        m, f = s.tendency(r["h0"], r["u0"])
+        assert np.array_equal(m, r["masseq"]) and np.array_equal(f, r["momeq"])
+        s.set_state(r["h0"], r["u0"])
+        s.step(1)
+        h, u = s.get_state()
+    assert np.array_equal(h, r["h"]) and np.array_equal(u, r["u"])

@@ -93,3 +93,20 @@ def test_tc2_matches_reference(golden):
     nmap, emap, tmap, flip = match(m.arrays, ref_mesh)
     assert np.array_equal(h, r["h0"][nmap])
     assert np.array_equal(np.where(flip, -u, u), r["u0"][emap])
+
+
+def test_tc2_hc_matches_reference(golden):
+    """TC2 with HC staggering (u along edhx%nr at the Voronoi edge
+    midpoints) equals the reference's up to the orientation of renumbered
+    dual edges and the rounding of the circumcentres it is built from."""
+    ref_mesh = golden("ref_mesh_g3.bin")
+    r = golden("ref_run_g3_tc2hc.bin")
+    m = Mesh.icos(3)
+    h, u = m.tc2(stag="HC")
+    nmap, emap, tmap, flip = match(m.arrays, ref_mesh)
+    assert np.array_equal(h, r["h0"][nmap])
+    sh = m.arrays["ed_sh"].reshape(-1, 2) - 1
+    rsh = ref_mesh["ed_sh"].reshape(-1, 2) - 1
+    fsh = tmap[sh[:, 0]] != rsh[emap, 0]   # dual edge walked the other way
+    us = np.where(fsh, -u, u)
+    assert np.max(np.abs(us - r["u0"][emap])) < 1e-14 * np.max(np.abs(u))
