@@ -65,6 +65,12 @@ struct imx_swm {
   int32_t *c_nnb = nullptr, *c_ed = nullptr;
   int8_t *c_ttg = nullptr, *c_tnr = nullptr;
   double *c_area = nullptr, *c_trskw = nullptr;
+  // the TRiSK terms of each edge, edge-major [2 MX][NE] (term ic MX + j of
+  // cell ic of the edge): ring edge k (-1 past the cell's edges) and weight
+  int32_t *e_tk = nullptr;
+  double *e_tw = nullptr, *e_tdl = nullptr;  // weight, dual length of k
+  int8_t *e_ts = nullptr;                    // sign of the term (HTC)
+  int8_t *hc_ts = nullptr;                   // the same for HC
   int32_t *e_v = nullptr, *e_sh = nullptr;
   int8_t *e_loc = nullptr;
   double *e_leng = nullptr, *e_dleng = nullptr;

@@ -242,7 +242,8 @@ __global__ __launch_bounds__(TPB) void ref_edge_mom(
     const double *__restrict__ e_leng, const double *__restrict__ e_dleng,
     const int32_t *__restrict__ c_nnb, const int32_t *__restrict__ c_ed,
     const int8_t *__restrict__ c_ttg, const int8_t *__restrict__ c_tnr,
-    const double *__restrict__ c_trskw, const double *__restrict__ ghbk,
+    const int32_t *__restrict__ e_tk, const double *__restrict__ e_tw, const double *__restrict__ e_tdl,
+    const int8_t *__restrict__ e_ts, const double *__restrict__ ghbk,
     const double *__restrict__ uh, const double *__restrict__ q_ed,
     const int32_t *__restrict__ e_ed3, const int8_t *__restrict__ e_gs, double erad,
     int zero_mom, double *__restrict__ grad, double *__restrict__ uhq,
@@ -255,20 +256,20 @@ __global__ __launch_bounds__(TPB) void ref_edge_mom(
   grad[l] = g;
   double s = 0.0;
   const double ql = q_ed[l];
-  for (int ic = 0; ic < 2; ++ic) {
-    const int node = e_v[ic * ne + l];
-    const int ei = e_loc[ic * ne + l];
-    const int n = c_nnb[node];
-    const double tn = (double)c_tnr[ei * nv + node];
-    for (int j = 0; j < n; ++j) {
-      const int k = c_ed[j * nv + node];
-      const double sg = 1.0 * (double)c_ttg[j * nv + node] * tn;
-      const double signcor = sg < 0.0 ? -1.0 : 1.0;
-      // gass18: q of the rhombus linking edges l and k where there is one
-      const int e3 = e_ed3 ? e_ed3[(ic * MX + j) * ne + l] : -1;
-      const double qtmp = e3 >= 0 ? q_ed[e3] : 0.5 * (ql + q_ed[k]);
-      s = s + uh[k] * qtmp * e_dleng[k] * c_trskw[(ei * MX + j) * nv + node] * signcor;
-    }
+  // the terms in the reference's order (cell 1 then cell 2, each cell's
+  // edges in ring order), ring edges and weights from the edge-major term
+  // tables (coalesced) instead of the cell-major weight table
+  // the sign of a term is dsign(1, ttgout(j) tnrccw(ei)) (HTC) or
+  // dsign(1, hx%nr(j) hx%nr(ei)) (HC), tabulated per term
+#pragma unroll
+  for (int t = 0; t < 2 * MX; ++t) {
+    const int k = e_tk[t * ne + l];
+    if (k < 0) continue;
+    const double signcor = (double)e_ts[t * ne + l];
+    // gass18: q of the rhombus linking edges l and k where there is one
+    const int e3 = e_ed3 ? e_ed3[t * ne + l] : -1;
+    const double qtmp = e3 >= 0 ? q_ed[e3] : 0.5 * (ql + q_ed[k]);
+    s = s + uh[k] * qtmp * e_tdl[t * ne + l] * e_tw[t * ne + l] * signcor;
   }
   const double p = s / e_leng[l];
   uhq[l] = p;
@@ -933,7 +934,8 @@ static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, do
                                                      s->f_grad, s->f_uhq, momeq);
   else
     ref_edge_mom<<<grid_for(NE), TPB, 0, st>>>(NE, nv, s->e_v, s->e_loc, s->e_leng, s->e_dleng,
-                                               s->c_nnb, s->c_ed, csg, ctn, s->c_trskw,
+                                               s->c_nnb, s->c_ed, csg, ctn, s->e_tk, s->e_tw, s->e_tdl,
+                                               hc ? s->hc_ts : s->e_ts,
                                                s->f_ghbk, s->f_uh, s->f_q_ed,
                                                wrap == IMX_WRAP_GASS18 ? s->e_ed3 : nullptr, egs, s->o.erad,
                                                s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
@@ -1344,6 +1346,30 @@ static int create(const imx_mesh_desc &d, const imx_swm_opts &o, imx_swm *s,
 
   if (!s->fast) {
     s->c_trskw = A.put(c_trskw);
+    {  // TRiSK terms per edge, edge-major
+      std::vector<int32_t> tk((size_t)2 * MX * NE, -1);
+      std::vector<double> tw((size_t)2 * MX * NE, 0.0), tdl((size_t)2 * MX * NE, 0.0);
+      std::vector<int8_t> ts((size_t)2 * MX * NE, 1);
+      for (int e = 0; e < NE; ++e) {
+        if (e_v[e] < 0) continue;
+        for (int ic = 0; ic < 2; ++ic) {
+          const int node = e_v[(size_t)ic * NE + e], ei = e_loc[(size_t)ic * NE + e];
+          for (int j = 0; j < c_nnb[node]; ++j) {
+            const size_t t = ((size_t)ic * MX + j) * NE + e;
+            const int k = c_ed[(size_t)j * nv + node];
+            tk[t] = k;
+            tw[t] = c_trskw[((size_t)ei * MX + j) * nv + node];
+            tdl[t] = e_dleng[k];
+            ts[t] = (int8_t)(c_ttg[(size_t)j * nv + node] * c_tnr[(size_t)ei * nv + node]);
+          }
+        }
+      }
+      s->e_tk = A.put(tk);
+      s->e_tw = A.put(tw);
+      s->e_tdl = A.put(tdl);
+      s->e_ts = A.put(ts);
+      s->hc_ts = A.get<int8_t>((size_t)2 * MX * NE);
+    }
     s->rh = A.get<double>(nv);
     s->ru = A.get<double>(NE);
     s->rhn = A.get<double>(nv);
@@ -1816,6 +1842,18 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     for (int ko = 0; ko < d->nt; ++ko)
       for (int l = 0; l < 3; ++l)
         hct[(size_t)l * NT + tn[ko]] = (int8_t)((d->tr_tg[3 * ko + l] < 0 ? -1 : 1) * sgn[d->tr_ed[3 * ko + l] - 1]);
+    // the HC sign of each TRiSK term: hx%nr(j) hx%nr(ei) of its cell
+    std::vector<int8_t> hts((size_t)2 * MX * NE, 1);
+    for (int eo = 0; eo < d->ne; ++eo)
+      for (int ic = 0; ic < 2; ++ic) {
+        const int co = d->ed_v[2 * eo + ic] - 1, c = cn[co];
+        int ei = 0;
+        for (int j = 0; j < d->v_nnb[co]; ++j)
+          if (d->v_ed[MX * co + j] == eo + 1) ei = j;
+        for (int j = 0; j < d->v_nnb[co]; ++j)
+          hts[((size_t)ic * MX + j) * NE + en[eo]] = (int8_t)(hcc[(size_t)j * nv + c] * hcc[(size_t)ei * nv + c]);
+      }
+    HIPTRY(hipMemcpy(s->hc_ts, hts.data(), hts.size(), hipMemcpyHostToDevice), "set_scheme");
     HIPTRY(hipMemcpy(s->hc_c, hcc.data(), hcc.size(), hipMemcpyHostToDevice), "set_scheme");
     HIPTRY(hipMemcpy(s->hc_t, hct.data(), hct.size(), hipMemcpyHostToDevice), "set_scheme");
     HIPTRY(hipMemcpy(s->hc_g, hg.data(), hg.size(), hipMemcpyHostToDevice), "set_scheme");
