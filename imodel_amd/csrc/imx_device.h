@@ -71,6 +71,10 @@ struct imx_swm {
   double *e_tw = nullptr, *e_tdl = nullptr;  // weight, dual length of k
   int8_t *e_ts = nullptr;                    // sign of the term (HTC)
   int8_t *hc_ts = nullptr;                   // the same for HC
+  // fused handles with diffusion: K at cells and triangles and K zeta at
+  // triangles (per stage), typed T
+  bool f_diff = false;
+  void *x_kh = nullptr, *x_kt = nullptr, *x_cz = nullptr;
   int32_t *e_v = nullptr, *e_sh = nullptr;
   int8_t *e_loc = nullptr;
   double *e_leng = nullptr, *e_dleng = nullptr;

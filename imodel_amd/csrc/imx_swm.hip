@@ -537,6 +537,7 @@ __global__ __launch_bounds__(TPB) void ref_rk_final(int n, double *__restrict__ 
 // c is 2c+k (k < 3, k < 2). A cell rebuilds the ids of its ring edges and
 // triangles from the neighbour ids it holds plus per-slot bits:
 //   bitsA: nnb (0-2) | ttgout<0 (3-8) | tnrccw<0 (9-14) | cell owns edge j (15-20)
+//          | diffusion: sign of the K zeta jump of owned edge j is -1 (21-26)
 //   bitsB: k of edge j (2 bits at 2j) | owner of triangle j: 0 cell, 1 nb_j,
 //          2 nb_{j+1} (2 bits at 12+2j) | k of triangle j (bit 24+j)
 // Pentagons carry a padded 6th slot repeating slot 0, masked out of all sums,
@@ -545,6 +546,7 @@ __global__ __launch_bounds__(TPB) void ref_rk_final(int n, double *__restrict__ 
 #define BA_TTG 3
 #define BA_TNR 9
 #define BA_OWN 15
+#define BA_FZ 21
 #define BB_TSEL 12
 #define BB_KT 24
 #define OTH_MASK 0x3FFFFFFF
@@ -558,6 +560,7 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
                                                 const T *__restrict__ t_f,
                                                 const T *__restrict__ u,
                                                 const T *__restrict__ h, T href, int use_pv,
+                                                const T *__restrict__ kt, T *__restrict__ cz,
                                                 T *__restrict__ q, int *__restrict__ flag) {
   const int t = ent_index<BS>(remap);
   if (t >= NT) return;
@@ -574,6 +577,12 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
   eta += u[e0] * t_ceta[t];
   eta += u[e1] * t_ceta[NT + t];
   eta += u[e2] * t_ceta[2 * NT + t];
+  if (kt) {  // diffusion: K zeta (relative vorticity) at the triangle
+    T z = u[e0] * t_ceta[t];
+    z += u[e1] * t_ceta[NT + t];
+    z += u[e2] * t_ceta[2 * NT + t];
+    cz[t] = kt[t] * z;
+  }
   T hsum = t_kap[t] * h[o];
   hsum += t_kap[NT + t] * h[a];
   hsum += t_kap[2 * NT + t] * h[b];
@@ -600,6 +609,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     const int32_t *__restrict__ c_bitsB, const T *__restrict__ c_R,
     const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
     const double *__restrict__ c_dl, const T *__restrict__ c_kw, const T *__restrict__ q,
+    const T *__restrict__ kh, const T *__restrict__ cz,
     const T *__restrict__ hs, const T *__restrict__ us, double href, T grav, T inv_a,
     T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
@@ -651,7 +661,18 @@ __global__ __launch_bounds__(BS) void fast_cell(
   }
   const double dh = -(dv + dvc) * c_iaa[c];
   const T K = ke * c_ika[c];
-  phi[c] = (grav * hc + K) * inv_a;
+  T Kd = (T)0;
+  if (kh) {  // diffusion, normal part: grad(K div u) folds into the potential
+    double du = 0.0;
+#pragma unroll
+    for (int j = 0; j < MX; ++j) {
+      double g = (double)ue[j] * dl[j];
+      if ((A >> (BA_TTG + j)) & 1) g = -g;
+      if ((j < MX - 1) || hex) du += g;
+    }
+    Kd = kh[c] * (T)(du * c_iaa[c]);
+  }
+  phi[c] = (grav * hc + K - Kd) * inv_a;
 
   // PV at the cell's edges: edge j lies between ring triangles j-1 and j
   T qe[MX];
@@ -678,8 +699,20 @@ __global__ __launch_bounds__(BS) void fast_cell(
     T P = (T)0.5 * acc;
     if ((A >> (BA_TNR + i)) & 1) P = -P;
     if (i < MX - 1 || hex) {
-      if ((A >> (BA_OWN + i)) & 1) Pown[ed[i]] = P;
-      else Poth[ed[i]] = P;
+      if ((A >> (BA_OWN + i)) & 1) {
+        // diffusion, tangential part (owner only): the jump of K zeta
+        // between the edge's ring triangles i-1 and i, times |e| / (|e*| a)
+        // = 4 kw / (|e*|^2 a)
+        if (kh) {
+          const int ip = (i == 0) ? (hex ? MX - 1 : MX - 2) : i - 1;
+          T fz = (T)4 * kw[i] * (T)(1.0 / (dl[i] * dl[i])) * inv_a;
+          if ((A >> (BA_FZ + i)) & 1) fz = -fz;
+          P += fz * (cz[tr[i]] - cz[tr[ip]]);
+        }
+        Pown[ed[i]] = P;
+      } else {
+        Poth[ed[i]] = P;
+      }
     }
   }
 
@@ -999,14 +1032,16 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
 #define IMX_TRI(B)                                                                          \
   fast_tri<T, B><<<grid_bs(NT, B), B, 0, st>>>(                                            \
       s->kremap[0], NT, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta, (const T *)s->x_t_kap, \
-      (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, (T *)s->q, s->flag)
+      (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, s->f_diff ? (const T *)s->x_kt : nullptr, \
+      (T *)s->x_cz, (T *)s->q, s->flag)
   IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
 #undef IMX_TRI
   if (evs) (void)hipEventRecord(evs[1], st);
 #define IMX_CELL(B)                                                                           \
   fast_cell<T, B><<<grid_bs(nv, B), B, 0, st>>>(                                             \
       s->kremap[1], nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_R, s->x_c_iaa, \
-      (const T *)s->x_c_ika, s->x_c_dl, (const T *)s->x_c_kw, (const T *)s->q, hs, us, s->href,  \
+      (const T *)s->x_c_ika, s->x_c_dl, (const T *)s->x_c_kw, (const T *)s->q,                    \
+      s->f_diff ? (const T *)s->x_kh : nullptr, (const T *)s->x_cz, hs, us, s->href,             \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
   IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
@@ -1179,6 +1214,75 @@ static int build_fast(imx_swm *s, const imx_mesh_desc &d, const double *trhx, co
   s->dh_out = A.get<double>(nv);
   s->du_out = A.get<T>(NE);
   if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_create (fast tables)");
+  return IMX_OK;
+}
+
+// K2 diffusion on the fused kernels (Klemp form, HTC, TRiSK): the normal
+// part grad(K div u) enters the cell potential phi, and the tangential part
+// -grad_tg(K zeta) enters the owner's partial P of each edge as
+// fz (Kz[t_i] - Kz[t_{i-1}]) with fz = e (+-1) |e| / (|e*| a), where e is the
+// sign of ed%nr.(t2 - t1) and +-1 says whether ring triangle i is sh(2);
+// the sign goes into bits BA_FZ of the cell's bitsA.
+// fast_edge then forms -(dphi + P) / |e|, i.e.
+// + (Kd(v2) - Kd(v1)) / (|e| a) - e (Kz(sh2) - Kz(sh1)) / (|e*| a).
+template <typename T>
+static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
+  if (sc.k2 == 0.0) {
+    s->f_diff = false;
+    return IMX_OK;
+  }
+  const int nv = s->nv, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  std::vector<double> trhx_v, hxtr_v, trskw_v;
+  const double *trhx = d.tr_trhx_area;
+  if (!trhx) {
+    trhx_v.assign((size_t)3 * d.nt, 0.0);
+    hxtr_v.assign((size_t)MX * d.nv, 0.0);
+    trskw_v.assign((size_t)MX * MX * d.nv, 0.0);
+    gridprop(d, trhx_v.data(), hxtr_v.data(), trskw_v.data());
+    trhx = trhx_v.data();
+  }
+  std::vector<double> hx, tr;
+  diffusion_coef(d, trhx, sc.wrap, sc.k2, sc.diffus, 0.0, sc.hyperdiffus, hx, tr);
+  std::vector<double> kh(nv, 0.0), kt(NT, 0.0);
+  for (int co = 0; co < nv; ++co) kh[cn[co]] = hx[co];
+  for (int ko = 0; ko < d.nt; ++ko) kt[tn[ko]] = tr[ko];
+  std::vector<int32_t> bitsA(nv);
+  HIPTRY(hipMemcpy(bitsA.data(), s->x_c_bitsA, nv * sizeof(int32_t), hipMemcpyDeviceToHost), "set_scheme");
+  for (int co = 0; co < nv; ++co) {
+    const int c = cn[co], n = d.v_nnb[co];
+    bitsA[c] &= ~(((1 << MX) - 1) << BA_FZ);
+    for (int j = 0; j < n; ++j) {
+      const int eo = d.v_ed[MX * co + j] - 1;
+      if (en[eo] / 3 != c) continue;  // the owner cell writes the edge's P
+      const int ti = d.v_tr[MX * co + j] - 1, tp = d.v_tr[MX * co + (j + n - 1) % n] - 1;
+      const int s1 = d.ed_sh[2 * eo] - 1, s2 = d.ed_sh[2 * eo + 1] - 1;
+      double side;
+      if (ti == s2 && tp == s1) side = 1.0;
+      else if (ti == s1 && tp == s2) side = -1.0;
+      else return set_error(IMX_EINVAL, "imx_swm_set_scheme: cell ring triangles do not flank the edge");
+      const double *pa = d.v_p + 3 * (d.ed_v[2 * eo] - 1), *pb = d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1);
+      double nr[3];
+      cross3(pa, pb, nr);
+      const double *t1 = d.tr_c + 3 * s1, *t2 = d.tr_c + 3 * s2;
+      double dp = 0.0;
+      for (int q = 0; q < 3; ++q) dp = dp + nr[q] * (t2[q] - t1[q]);
+      const double e = std::signbit(dp) ? -1.0 : 1.0;
+      if (e * side < 0.0) bitsA[c] |= 1 << (BA_FZ + j);
+    }
+  }
+  Alloc &A = s->A;
+  if (!s->x_kh) {
+    s->x_kh = A.get<T>(nv);
+    s->x_kt = A.get<T>(NT);
+    s->x_cz = A.get<T>(NT);
+    if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
+  }
+  const std::vector<T> khT = cvt<T>(kh), ktT = cvt<T>(kt);
+  HIPTRY(hipMemcpy(s->x_kh, khT.data(), khT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_kt, ktT.data(), ktT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_c_bitsA, bitsA.data(), nv * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+  s->f_diff = true;
   return IMX_OK;
 }
 
@@ -1721,8 +1825,6 @@ int imx_swm_get_field(imx_swm *s, int32_t which, double *out) {
 
 int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc) {
   if (!s || !d || !sc) return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: NULL argument");
-  if (s->fast)
-    return imx::set_error(IMX_ENOTSUP, "imx_swm_set_scheme: scheme options run on FP64_REF handles");
   if (d->nv != s->nv || d->ne != s->ne || d->nt != s->nt)
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: not the handle's mesh");
   if (!(sc->k2 >= 0.0) || !(sc->k4 >= 0.0) || !std::isfinite(sc->k2) || !std::isfinite(sc->k4) ||
@@ -1744,6 +1846,18 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: diffusion needs v_tr, v_nb and tr_c");
   if (sc->apvm && (!d->ed_c || !d->tr_c || !d->v_p))
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: APVM needs ed_c, tr_c and v_p");
+  if (s->fast) {
+    // the fused kernels carry K2 diffusion of the HTC TRiSK scheme
+    if (sc->stag != IMX_STAG_HTC || sc->wrap != IMX_WRAP_TRSK10 || sc->apvm || sc->k4 != 0.0)
+      return imx::set_error(IMX_ENOTSUP,
+                            "imx_swm_set_scheme: fused handles take K2 diffusion only (HTC, trsk10, "
+                            "no APVM or K4); other options run on FP64_REF handles");
+    const int rc = s->single ? imx::fast_diffusion<float>(s, *d, *sc) : imx::fast_diffusion<double>(s, *d, *sc);
+    if (rc) return rc;
+    s->sc = *sc;
+    imx::drop_graph(s);
+    return IMX_OK;
+  }
   const int nv = s->nv, NE = s->NE, NT = s->NT;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
   imx::Alloc &A = s->A;

@@ -203,11 +203,12 @@ class Swm:
 
     def set_scheme(self, stag="HTC", wrap="trsk10", pv="none", pvspar=0.5, k2=0.0, diffus="const", k4=0.0,
                    hyperdiffus="const"):
-        """Scheme options of par/swm.par on an FP64_REF handle: the
-        staggering ("HTC" | "HC"), the method wrapper ("trsk10" | "pxt16" |
-        "gass18"), APVM (pv="apvm", parameter
-        pvspar), diffusion K2_max with coefficient function diffus,
-        hyperdiffusion K4_max with hyperdiffus ("const" | "align" | "diam")."""
+        """Scheme options of par/swm.par: the staggering ("HTC" | "HC"),
+        the method wrapper ("trsk10" | "pxt16" | "gass18"), APVM (pv="apvm",
+        parameter pvspar), diffusion K2_max with coefficient function diffus,
+        hyperdiffusion K4_max with hyperdiffus ("const" | "align" | "diam").
+        Every option runs on FP64_REF handles; fused (FP32/FP64) handles take
+        K2 diffusion of the HTC trsk10 scheme and refuse the rest."""
         sc = capi.Scheme(stag=capi.STAG[stag], wrap=capi.WRAP[wrap], apvm=int(pv == "apvm"), pvspar=float(pvspar), k2=float(k2),
                          diffus=capi.COEF[diffus], k4=float(k4), hyperdiffus=capi.COEF[hyperdiffus])
         capi.check(capi.lib().imx_swm_set_scheme(self._h, ctypes.byref(self.mesh.desc()), ctypes.byref(sc)),

@@ -223,6 +223,34 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         assert r.returncode != 0 and "fp64_ref" in r.stdout
 
 
+def test_fortran_driver_fp32_k2_diffusion():
+    """K2 diffusion runs on the fused FP32 path from par/swm.par: a day of
+    TC2 on glevel 3 against the oracle with the same option (FP32 storage
+    tolerances)."""
+    print("[gpu-case] fortran imodel g3 tc2 fp32 k2 align", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 48, "fp32")
+        p = os.path.join(td, "par", "swm.par")
+        swm = open(p).read().splitlines()
+        swm[24] = "1e5 align"
+        open(p, "w").write("\n".join(swm) + "\n")
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        base = "swm_tc2_dt900_HTC_trsk10_areageo_align_diffusion_10to5.000"
+        st = imxbin.read(os.path.join(td, "data", f"{base}_state_icos_eqs_nopt_3.bin"))
+    m = Mesh.icos(3)
+    h0, u0 = m.tc2()
+    h, u = Oracle(m.arrays, scheme={"k2": 1e5, "diffus": "align", "dt": 900.0}).step(h0, u0, 900.0, 96)
+    hp, up = Oracle(m.arrays).step(h0, u0, 900.0, 96)
+    A = m.arrays["hx_areag"]
+    eh = np.sqrt(np.sum(A * (st["h"] - h) ** 2) / np.sum(A * h ** 2))
+    eu = np.linalg.norm(st["u"] - u) / np.linalg.norm(u)
+    # the diffusion's effect over the day is far above the FP32 error
+    du = np.linalg.norm(up - u) / np.linalg.norm(u)
+    assert eh < 1e-7 and eu < 1e-5 and du > 20 * eu, (eh, eu, du)
+
+
 @pytest.mark.parametrize("wrap", ["pxt16", "gass18"])
 def test_fortran_driver_wrappers_bitexact(wrap):
     """The pxt16 / gass18 wrapper line of par/swm.par: a day of TC2 on

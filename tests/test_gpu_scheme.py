@@ -122,3 +122,70 @@ def test_hc_tc2_matches_reference(golden, sfc):
         s.step(1)
         h, u = s.get_state()
     assert np.array_equal(h, r["h"]) and np.array_equal(u, r["u"])
+
+
+# K2 diffusion on the fused kernels (HTC, trsk10): the tendency against the
+# oracle, the diffusion part on its own (tendency with minus without), and
+# steps with the graph
+FAST_K2 = {"const": {"k2": 1e5}, "align": {"k2": 1e5, "diffus": "align"},
+           "diam": {"k2": 2e5, "diffus": "diam"}}
+
+
+@pytest.mark.parametrize("name", sorted(FAST_K2))
+def test_fast_k2_tendency_vs_oracle(name):
+    from tests.conftest import rel_l2
+    say("fast k2 tendency", name)
+    m = Mesh.icos(4)
+    h, u = m.spectral(21)
+    sc = dict(FAST_K2[name], dt=900.0)
+    mo, fo = Oracle(m.arrays, scheme=sc).tendency(h, u)
+    _, fo0 = Oracle(m.arrays).tendency(h, u)
+    for prec, tol, tol_d in (("fp64", 1e-12, 1e-10), ("fp32", 5e-5, None)):
+        with Swm(m, precision=prec) as s:
+            _, f0 = s.tendency(h, u)
+            set_scheme(s, sc)
+            mg, fg = s.tendency(h, u)
+        assert rel_l2(mg, mo) < tol, (prec, rel_l2(mg, mo))
+        assert rel_l2(fg, fo) < tol, (prec, rel_l2(fg, fo))
+        if tol_d is not None:
+            # the diffusion term alone: both parts, with their signs
+            dg, do = fg - f0, fo - fo0
+            assert np.linalg.norm(do) > 1e-6 * np.linalg.norm(fo)
+            assert rel_l2(dg, do) < tol_d, (prec, rel_l2(dg, do))
+
+
+def test_fast_k2_steps_vs_oracle():
+    from tests.conftest import rel_l2
+    say("fast k2 steps fp64 / fp32")
+    m = Mesh.icos(4)
+    h, u = m.spectral(22)
+    sc = dict(FAST_K2["align"], dt=900.0)
+    ho, uo = Oracle(m.arrays, scheme=sc).step(h, u, 900.0, 10)
+    A = m.arrays["hx_areag"]
+    for prec, tol_h, tol_u in (("fp64", 1e-12, 1e-11), ("fp32", 1e-7, 1e-5)):
+        with Swm(m, precision=prec, dt=900.0) as s:
+            set_scheme(s, sc)
+            s.set_state(h, u)
+            s.step(10)
+            hg, ug = s.get_state()
+            # back to no diffusion: the plain fused step
+            s.set_scheme()
+            s.set_state(h, u)
+            s.step(10)
+            hp, up = s.get_state()
+        assert rel_l2(hg, ho, A) < tol_h, (prec, rel_l2(hg, ho, A))
+        assert rel_l2(ug, uo) < tol_u, (prec, rel_l2(ug, uo))
+        with Swm(m, precision=prec, dt=900.0) as p:
+            p.set_state(h, u)
+            p.step(10)
+            hq, uq = p.get_state()
+        assert np.array_equal(hp, hq) and np.array_equal(up, uq)
+
+
+def test_fast_scheme_refusals():
+    say("fast scheme refusals")
+    m = Mesh.icos(2)
+    with Swm(m, precision="fp64", dt=900.0) as s:
+        for kw in ({"pv": "apvm"}, {"k4": 1e14}, {"wrap": "pxt16", "k2": 1e5}, {"stag": "HC"}):
+            with pytest.raises(capi.ImxError, match="fused handles take K2 diffusion only"):
+                s.set_scheme(**kw)
