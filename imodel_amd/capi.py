@@ -19,11 +19,19 @@ NDIAG = 8
 DIAG_NAMES = ("mass", "penergy", "kenergy", "tenergy", "aenergy", "penstrophy",
               "rmsdiv", "badflag")
 FIELDS = ("h_ed", "uh", "eta", "h_tr", "q_tr", "q_ed", "divuh", "ke_hx", "ghbk",
-          "grad_ghbk", "uhq_perp", "divu", "zeta", "q_grad_ed", "lapu", "lap_lapu")
+          "grad_ghbk", "uhq_perp", "divu", "zeta", "q_grad_ed", "lapu", "lap_lapu",
+          "q_hx", "vhq_tr", "vh_hx", "vhq_hx", "q_grad_tr")
 FIELD_POS = {"h_ed": "e", "uh": "e", "eta": "t", "h_tr": "t", "q_tr": "t",
              "q_ed": "e", "divuh": "v", "ke_hx": "v", "ghbk": "v",
              "grad_ghbk": "e", "uhq_perp": "e", "divu": "v", "zeta": "t",
-             "q_grad_ed": "e", "lapu": "e", "lap_lapu": "e"}
+             "q_grad_ed": "e", "lapu": "e", "lap_lapu": "e",
+             "q_hx": "v", "vhq_tr": "t3", "vh_hx": "v3", "vhq_hx": "v3", "q_grad_tr": "t3"}
+# rows of imx_swm_lte (the table swm_horiz_loc_trunc_er prints) and its
+# mimetic sums
+LTE_ROWS = ("h_ed", "divuh", "kenergy", "abs_vort", "h_tr", "q_tr", "q_hx", "q_ed", "grad_ghbK",
+            "grad_ghbK_exact", "vhq_tr", "q_grad_tr", "vh_hx", "vhq_hx", "uhq_perp", "lapu", "masseq",
+            "momeq")
+LTE_MIM = ("hdivu", "prodrule", "divuh", "proddiff", "crossrule", "coriolis", "coriolis_rel")
 COEF = {"const": 0, "align": 1, "diam": 2}
 WRAP = {"trsk10": 0, "pxt16": 1, "gass18": 2}
 STAG = {"HTC": 0, "HC": 1}
@@ -112,6 +120,8 @@ def lib():
             "imx_mesh_desc_of": (ctypes.c_int, [vp, ctypes.POINTER(MeshDesc)]),
             "imx_mesh_gridprop": (ctypes.c_int, [vp]),
             "imx_mesh_write": (ctypes.c_int, [vp, ctypes.c_char_p]),
+            "imx_mesh_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(vp)]),
+            "imx_swm_lte": (ctypes.c_int, [vp, vp, _F64P, _F64P]),
             "imx_mesh_load_ref": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]),
             "imx_mesh_write_ref": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p]),
             "imx_make_dir": (ctypes.c_int, [ctypes.c_char_p]),

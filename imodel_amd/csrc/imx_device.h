@@ -71,6 +71,13 @@ struct imx_swm {
   double *e_tw = nullptr, *e_tdl = nullptr;  // weight, dual length of k
   int8_t *e_ts = nullptr;                    // sign of the term (HTC)
   int8_t *hc_ts = nullptr;                   // the same for HC
+  // analysis fields of the last LTE evaluation (imx_swm_lte): q_hx [nv],
+  // vhq_tr [3][NT], vh_hx and vhq_hx [3][nv]; the ring triangles and the
+  // tr -> hx weights of each cell [MX][nv]
+  double *f_lte_qhx = nullptr, *f_lte_vtr = nullptr, *f_lte_vhh = nullptr, *f_lte_vhq = nullptr;
+  int32_t *lte_ctr = nullptr;
+  double *lte_cw = nullptr;
+  bool lte_done = false;
   // fused handles with diffusion: K at cells and triangles and K zeta at
   // triangles (per stage), typed T
   bool f_diff = false;

@@ -9,7 +9,7 @@
 namespace imx {
 
 // convert_vec_sph2cart (src/smeshpack.f90:4379-4417)
-static void vec_sph2cart(double vlon, double vlat, const double *p, double *v) {
+void vec_sph2cart(double vlon, double vlat, const double *p, double *v) {
   double r2 = p[0] * p[0];
   r2 = r2 + p[1] * p[1];
   double rho = std::sqrt(r2);

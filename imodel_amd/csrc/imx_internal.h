@@ -60,6 +60,11 @@ double det3(const double *p1, const double *p2, const double *p3);
 void cart2sph(double x, double y, double z, double *lon, double *lat);
 
 void gridprop(const imx_mesh_desc &d, double *trhx, double *hxtr, double *trskw);
+// convert_vec_sph2cart (src/smeshpack.f90:4379-4417)
+void vec_sph2cart(double vlon, double vlat, const double *p, double *v);
+// Wachspress coordinates of cell node i (0-based) in its ring of
+// circumcentres, in v%tr order (wachspress_coords_hxv, interpack.f90:2310-2415)
+void wachspress_node(const imx_mesh_desc &d, int i, double *w);
 // Radius, in cell rings around a row's cells, of a stencil superset of the
 // handle's tendency: 1 for TRiSK, one more ring each for APVM, for
 // hyperdiffusion and for the gass18 Coriolis.
@@ -107,6 +112,18 @@ struct BinWriter {
   void put_i4(const char *name, const int32_t *a, int64_t n);
   void put_f8(const char *name, const double *a, int64_t n);
 };
+
+// Fields of one LTE evaluation in caller numbering (vectors x y z per entry;
+// q_grad_tr and lapu NULL when the options leave them uncomputed).
+struct LteFields {
+  const double *h, *u, *h_ed, *uh, *divuh, *ke_hx, *eta, *h_tr, *q_tr, *q_hx, *q_ed, *grad_ghbk,
+      *vhq_tr, *q_grad_tr, *vh_hx, *vhq_hx, *uhq_perp, *lapu, *masseq, *momeq;
+};
+// Closed forms of TC2, error_calc of each row and the mimetic sums of
+// swm_horiz_loc_trunc_er (imx_lte.cpp); perhx: Perot's cell reconstruction
+// (pxt16), which changes calc_energies' kinetic energy.
+int lte_analysis(const HostMesh &m, bool hc, bool perhx, double erad, double grav, const LteFields &f,
+                 double *err, double *mim);
 
 }  // namespace imx
 

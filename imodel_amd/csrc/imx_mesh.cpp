@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <new>
 #include <atomic>
 
@@ -767,6 +768,87 @@ int imx_mesh_write(const imx_mesh *m, const char *path) {
     w.put("hx_hxtr_area", m->hx_hxtr_area);
     w.put("hx_trskw", m->hx_trskw);
   }
+  return IMX_OK;
+}
+
+int imx_mesh_read(const char *path, imx_mesh **out) {
+  if (!path || !out) return imx::set_error(IMX_EINVAL, "imx_mesh_read: NULL argument");
+  *out = nullptr;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return imx::set_error(IMX_EIO, std::string("imx_mesh_read: cannot open ") + path);
+  char magic[8];
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, "IMXBIN01", 8) != 0) {
+    std::fclose(f);
+    return imx::set_error(IMX_EIO, std::string("imx_mesh_read: not an IMXBIN file: ") + path);
+  }
+  auto m = std::make_unique<imx_mesh>();
+  std::vector<int32_t> dims;
+  std::map<std::string, std::vector<int32_t> *> iv = {
+      {"v_nnb", &m->v_nnb}, {"v_nb", &m->v_nb}, {"v_ed", &m->v_ed}, {"v_tr", &m->v_tr},
+      {"hx_ttgout", &m->hx_ttgout}, {"hx_tnrccw", &m->hx_tnrccw}, {"hx_tg", &m->hx_tg},
+      {"hx_nr", &m->hx_nr}, {"ed_v", &m->ed_v}, {"ed_sh", &m->ed_sh}, {"tr_v", &m->tr_v},
+      {"tr_ed", &m->tr_ed}, {"tr_tg", &m->tr_tg}, {"tr_nr", &m->tr_nr}, {"dims", &dims}};
+  std::map<std::string, std::vector<double> *> rv = {
+      {"v_p", &m->v_p}, {"v_lat", &m->v_lat}, {"v_lon", &m->v_lon}, {"hx_areag", &m->hx_areag},
+      {"ed_leng", &m->ed_leng}, {"edhx_leng", &m->edhx_leng}, {"ed_c", &m->ed_c},
+      {"ed_clat", &m->ed_clat}, {"ed_clon", &m->ed_clon}, {"ed_tg", &m->ed_tg}, {"ed_nr", &m->ed_nr},
+      {"edhx_c", &m->edhx_c}, {"edhx_nr", &m->edhx_nr}, {"edhx_tg", &m->edhx_tg},
+      {"tr_areag", &m->tr_areag}, {"tr_c", &m->tr_c}, {"tr_clat", &m->tr_clat},
+      {"tr_clon", &m->tr_clon}, {"tr_trhx_area", &m->tr_trhx_area},
+      {"hx_hxtr_area", &m->hx_hxtr_area}, {"hx_trskw", &m->hx_trskw}};
+  std::string bad;
+  for (;;) {
+    char name[25] = {0}, dt[4];
+    int64_t n = 0;
+    if (std::fread(name, 1, 24, f) != 24) break;
+    if (std::fread(dt, 1, 4, f) != 4 || std::fread(&n, 8, 1, f) != 1 || n < 0 || n > (int64_t)1 << 31) {
+      bad = "truncated record header";
+      break;
+    }
+    std::string nm(name);
+    nm.erase(nm.find_last_not_of(' ') + 1);
+    const bool isf = std::memcmp(dt, "f8  ", 4) == 0, isi = std::memcmp(dt, "i4  ", 4) == 0;
+    if (!isf && !isi) {
+      bad = "unknown record type of " + nm;
+      break;
+    }
+    const size_t bytes = (size_t)n * (isf ? 8 : 4);
+    std::vector<char> buf(bytes);
+    if (bytes && std::fread(buf.data(), 1, bytes, f) != bytes) {
+      bad = "truncated record " + nm;
+      break;
+    }
+    if (isi && iv.count(nm)) {
+      iv[nm]->resize(n);
+      std::memcpy(iv[nm]->data(), buf.data(), bytes);
+    } else if (isf && rv.count(nm)) {
+      rv[nm]->resize(n);
+      std::memcpy(rv[nm]->data(), buf.data(), bytes);
+    }
+  }
+  std::fclose(f);
+  if (!bad.empty()) return imx::set_error(IMX_EIO, "imx_mesh_read: " + bad);
+  if (dims.size() < 3) return imx::set_error(IMX_EINVAL, "imx_mesh_read: no dims record");
+  m->nv = dims[0];
+  m->ne = dims[1];
+  m->nt = dims[2];
+  m->glevel = dims.size() > 3 ? dims[3] : -1;
+  const size_t nv = m->nv, ne = m->ne, nt = m->nt, X = IMX_MAXNB;
+  const bool ok = m->v_p.size() == 3 * nv && m->v_lat.size() == nv && m->hx_areag.size() == nv &&
+                  m->v_nnb.size() == nv && m->v_nb.size() == X * nv && m->v_ed.size() == X * nv &&
+                  m->v_tr.size() == X * nv && m->hx_ttgout.size() == X * nv &&
+                  m->hx_tnrccw.size() == X * nv && m->ed_v.size() == 2 * ne && m->ed_sh.size() == 2 * ne &&
+                  m->ed_leng.size() == ne && m->edhx_leng.size() == ne && m->ed_c.size() == 3 * ne &&
+                  m->edhx_c.size() == 3 * ne && m->tr_v.size() == 3 * nt && m->tr_ed.size() == 3 * nt &&
+                  m->tr_tg.size() == 3 * nt && m->tr_areag.size() == nt && m->tr_c.size() == 3 * nt &&
+                  m->tr_clat.size() == nt;
+  if (!ok) return imx::set_error(IMX_EINVAL, "imx_mesh_read: a required record is missing or has the wrong size");
+  imx_mesh_desc d;
+  imx::desc_of(*m, d);
+  if (!m->tr_trhx_area.empty() && (m->tr_trhx_area.size() != 3 * nt || m->hx_hxtr_area.size() != X * nv ||
+                                   m->hx_trskw.size() != X * X * nv))
+    return imx::set_error(IMX_EINVAL, "imx_mesh_read: gridprop records of the wrong size");
+  *out = m.release();
   return IMX_OK;
 }
 

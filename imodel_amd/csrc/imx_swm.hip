@@ -376,6 +376,70 @@ __global__ __launch_bounds__(TPB) void ref_perot_hx(int nv, int ne, const int32_
   vh[2 * nv + i] = v2;
 }
 
+// Analysis fields the tendency adds under the local truncation error
+// harness (test_lterror = 1, swm.f90:2500-2503): vhq_tr =
+// vector_edhx2tr_perot(uh) q_tr (swm_operators.f90:348-406, then
+// vector_elem_product :1617-1649). Signs tr%tg (HTC) or tr%tg s_e (HC), edge
+// points ed%c (HTC) or edhx%c (HC); the result is cross(tr%c, v) / area.
+__global__ __launch_bounds__(TPB) void ref_lte_tr(int nt, int ne, const int32_t *__restrict__ t_ed,
+                                                  const int8_t *__restrict__ tsg,
+                                                  const double *__restrict__ epc,
+                                                  const double *__restrict__ t_c,
+                                                  const double *__restrict__ e_leng,
+                                                  const double *__restrict__ t_area,
+                                                  const double *__restrict__ uh,
+                                                  const double *__restrict__ q_tr,
+                                                  double *__restrict__ vhq) {
+  const int k = xcd_index();
+  if (k >= nt || t_ed[k] < 0) return;
+  const double p0 = t_c[k], p1 = t_c[nt + k], p2 = t_c[2 * nt + k];
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const int ed = t_ed[l * nt + k];
+    const double a = (double)tsg[l * nt + k] * uh[ed];
+    const double len = e_leng[ed];
+    v0 = v0 + a * (epc[ed] - p0) * len;
+    v1 = v1 + a * (epc[ne + ed] - p1) * len;
+    v2 = v2 + a * (epc[2 * ne + ed] - p2) * len;
+  }
+  const double ar = t_area[k];
+  const double c0 = (p1 * v2 - p2 * v1) / ar;
+  const double c1 = (p2 * v0 - p0 * v2) / ar;
+  const double c2 = (p0 * v1 - p1 * v0) / ar;
+  const double q = q_tr[k];
+  vhq[k] = c0 * q;
+  vhq[nt + k] = c1 * q;
+  vhq[2 * nt + k] = c2 * q;
+}
+
+// ... and at cells (swm.f90:2546-2551): vh_hx = v_hx h, q_hx =
+// scalar_tr2hx(q_tr) (swm_operators.f90:178-227: weights w over the cell's
+// ring triangles, hxtr_area or Wachspress coordinates), vhq_hx = vh_hx q_hx
+__global__ __launch_bounds__(TPB) void ref_lte_cell(int nv, const int32_t *__restrict__ c_nnb,
+                                                    const int32_t *__restrict__ c_tr,
+                                                    const double *__restrict__ c_w,
+                                                    const double *__restrict__ vh,
+                                                    const double *__restrict__ h,
+                                                    const double *__restrict__ q_tr,
+                                                    double *__restrict__ vhh, double *__restrict__ qhx,
+                                                    double *__restrict__ vhq) {
+  const int i = xcd_index();
+  if (i >= nv) return;
+  const double hh = h[i];
+  const double a0 = vh[i] * hh, a1 = vh[nv + i] * hh, a2 = vh[2 * nv + i] * hh;
+  double q = 0.0;
+  const int n = c_nnb[i];
+  for (int j = 0; j < n; ++j) q = q + c_w[j * nv + i] * q_tr[c_tr[j * nv + i]];
+  vhh[i] = a0;
+  vhh[nv + i] = a1;
+  vhh[2 * nv + i] = a2;
+  qhx[i] = q;
+  vhq[i] = a0 * q;
+  vhq[nv + i] = a1 * q;
+  vhq[2 * nv + i] = a2 * q;
+}
+
 // apvm (swm_operators.f90:1497-1501): q_ed -= pvspar dt (qg_t1 + qg_t2) . (v_c1 + v_c2)
 __global__ __launch_bounds__(TPB) void ref_apvm_q(int ne, int nt, int nv, const int32_t *__restrict__ e_sh,
                                                   const int32_t *__restrict__ e_v,
@@ -1217,6 +1281,71 @@ static int build_fast(imx_swm *s, const imx_mesh_desc &d, const double *trhx, co
   return IMX_OK;
 }
 
+// Geometry and buffers of the scheme options and the LTE analysis, built
+// once per handle: internal numbering, points component-major.
+static int scheme_tables(imx_swm *s, const imx_mesh_desc *d) {
+  if (s->e_nrs) return IMX_OK;
+  const int nv = s->nv, NE = s->NE, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  Alloc &A = s->A;
+  // geometry of the options, internal numbering, points component-major
+  std::vector<int8_t> nrs(NE, 1);
+  std::vector<double> ec((size_t)3 * NE, 0.0), tc((size_t)3 * NT, 0.0), cp((size_t)3 * nv, 0.0);
+  for (int eo = 0; eo < d->ne; ++eo) {
+    const int e = en[eo];
+    const double *a = d->v_p + 3 * (d->ed_v[2 * eo] - 1), *b = d->v_p + 3 * (d->ed_v[2 * eo + 1] - 1);
+    double nr[3];
+    cross3(a, b, nr);  // ed%nr up to its positive norm (smeshpack.f90:725-726)
+    const double *t1 = d->tr_c + 3 * (d->ed_sh[2 * eo] - 1), *t2 = d->tr_c + 3 * (d->ed_sh[2 * eo + 1] - 1);
+    double dp = 0.0;
+    for (int q = 0; q < 3; ++q) dp = dp + nr[q] * (t2[q] - t1[q]);
+    nrs[e] = (int8_t)(std::signbit(dp) ? -1 : 1);
+    for (int q = 0; q < 3; ++q) ec[(size_t)q * NE + e] = d->ed_c[3 * eo + q];
+  }
+  for (int ko = 0; ko < d->nt; ++ko)
+    for (int q = 0; q < 3; ++q) tc[(size_t)q * NT + tn[ko]] = d->tr_c[3 * ko + q];
+  for (int co = 0; co < nv; ++co)
+    for (int q = 0; q < 3; ++q) cp[(size_t)q * nv + cn[co]] = d->v_p[3 * co + q];
+  s->e_nrs = A.put(nrs);
+  s->e_c = A.put(ec);
+  s->t_c = A.put(tc);
+  s->c_p = A.put(cp);
+  s->f_qg_ed = A.get<double>(NE);
+  s->f_qg_tr = A.get<double>((size_t)3 * NT);
+  s->f_vhx = A.get<double>((size_t)3 * nv);
+  s->f_lapu = A.get<double>(NE);
+  s->f_lap2 = A.get<double>(NE);
+  s->f_divl = A.get<double>(nv);
+  s->f_vortl = A.get<double>(NT);
+  s->dif_hx = A.get<double>(nv);
+  s->dif_tr = A.get<double>(NT);
+  s->t_w = A.get<double>((size_t)3 * NT);
+  s->e_dw = A.get<double>((size_t)2 * NE);
+  s->e_nr = A.get<double>((size_t)3 * NE);
+  s->e_ed3 = A.get<int32_t>((size_t)2 * MX * NE);
+  s->hc_c = A.get<int8_t>((size_t)MX * nv);
+  s->hc_t = A.get<int8_t>((size_t)3 * NT);
+  s->hc_g = A.get<int8_t>(NE);
+  s->hc_tt = A.get<int8_t>(NE);
+  s->hc_pc = A.get<double>((size_t)3 * NE);
+  s->hc_hb = A.get<double>((size_t)3 * NE);
+  s->hc_hv = A.get<int32_t>((size_t)3 * NE);
+  if (A.err != hipSuccess) return hip_fail(A.err, "scheme tables");
+  return IMX_OK;
+}
+
+// A component-major [3][N] device vector field (kind 0 cells, 2 triangles)
+// to caller order, x y z per entry.
+static int get_vector(imx_swm *s, const double *src, int kind, double *out) {
+  const int n = kind == 0 ? s->nv : s->nt, ni = kind == 0 ? s->nv : s->NT;
+  const std::vector<int32_t> &perm = kind == 0 ? s->lay.cell_new : s->lay.tri_new;
+  std::vector<double> b((size_t)3 * ni);
+  HIPTRY(hipMemcpy(b.data(), src, b.size() * sizeof(double), hipMemcpyDeviceToHost), "get_field");
+  for (int i = 0; i < n; ++i)
+    for (int q = 0; q < 3; ++q) out[3 * i + q] = b[(size_t)q * ni + perm[i]];
+  return IMX_OK;
+}
+
 // K2 diffusion on the fused kernels (Klemp form, HTC, TRiSK): the normal
 // part grad(K div u) enters the cell potential phi, and the tangential part
 // -grad_tg(K zeta) enters the owner's partial P of each edge as
@@ -1810,6 +1939,23 @@ int imx_swm_get_field(imx_swm *s, int32_t which, double *out) {
     case IMX_FIELD_Q_GRAD_ED: src = s->sc_apvm ? s->f_qg_ed : nullptr; kind = 1; break;
     case IMX_FIELD_LAPU: src = (s->sc_diff || s->sc_hyper) ? s->f_lapu : nullptr; kind = 1; break;
     case IMX_FIELD_LAP_LAPU: src = s->sc_hyper ? s->f_lap2 : nullptr; kind = 1; break;
+    case IMX_FIELD_Q_HX:
+    case IMX_FIELD_VHQ_TR:
+    case IMX_FIELD_VH_HX:
+    case IMX_FIELD_VHQ_HX: {
+      if (!s->lte_done)
+        return imx::set_error(IMX_EINVAL, "imx_swm_get_field: the analysis fields come from imx_swm_lte");
+      if (which == IMX_FIELD_Q_HX) {
+        src = s->f_lte_qhx;
+        kind = 0;
+        break;
+      }
+      const double *v = which == IMX_FIELD_VHQ_TR ? s->f_lte_vtr : which == IMX_FIELD_VH_HX ? s->f_lte_vhh : s->f_lte_vhq;
+      return imx::get_vector(s, v, which == IMX_FIELD_VHQ_TR ? 2 : 0, out);
+    }
+    case IMX_FIELD_Q_GRAD_TR:
+      if (!s->sc_apvm) return imx::set_error(IMX_EINVAL, "imx_swm_get_field: the field's scheme option is off");
+      return imx::get_vector(s, s->f_qg_tr, 2, out);
     default: return imx::set_error(IMX_EINVAL, "imx_swm_get_field: unknown field");
   }
   if (!src) return imx::set_error(IMX_EINVAL, "imx_swm_get_field: the field's scheme option is off");
@@ -1821,6 +1967,94 @@ int imx_swm_get_field(imx_swm *s, int32_t which, double *out) {
   HIPTRY(hipMemcpy(b.data(), src, ni * sizeof(double), hipMemcpyDeviceToHost), "get_field");
   for (int i = 0; i < n; ++i) out[i] = b[perm[i]];
   return IMX_OK;
+}
+
+int imx_swm_lte(imx_swm *s, const imx_mesh *m, double *err, double *mim) {
+  if (!s || !m || !err || !mim) return imx::set_error(IMX_EINVAL, "imx_swm_lte: NULL argument");
+  if (s->fast) return imx::set_error(IMX_ENOTSUP, "imx_swm_lte: the LTE analysis runs on FP64_REF handles");
+  if (s->o.testcase != 2)
+    return imx::set_error(IMX_ENOTSUP, "imx_swm_lte: the closed forms are those of TC2 (handle testcase 2)");
+  if (m->nv != s->nv || m->ne != s->ne || m->nt != s->nt)
+    return imx::set_error(IMX_EINVAL, "imx_swm_lte: not the handle's mesh");
+  imx_mesh_desc d;
+  int rc = imx_mesh_desc_of(m, &d);
+  if (rc) return rc;
+  if ((rc = imx::scheme_tables(s, &d))) return rc;
+  const int nv = s->nv, ne = s->ne, nt = s->nt, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &tn = s->lay.tri_new;
+  imx::Alloc &A = s->A;
+  if (!s->f_lte_qhx) {
+    s->f_lte_qhx = A.get<double>(nv);
+    s->f_lte_vtr = A.get<double>((size_t)3 * NT);
+    s->f_lte_vhh = A.get<double>((size_t)3 * nv);
+    s->f_lte_vhq = A.get<double>((size_t)3 * nv);
+    s->lte_ctr = A.get<int32_t>((size_t)MX * nv);
+    s->lte_cw = A.get<double>((size_t)MX * nv);
+    if (A.err != hipSuccess) return imx::hip_fail(A.err, "imx_swm_lte");
+  }
+  // scalar_tr2hx weights: Wachspress coordinates with pxt16's barycentric
+  // interpolation (initialize_gridprop, swm.f90:1001-1005), else hxtr_area
+  std::vector<int32_t> ctr((size_t)MX * nv, 0);
+  std::vector<double> cw((size_t)MX * nv, 0.0), hxtr_v, trhx_v, trskw_v;
+  const double *hxtr = d.hx_hxtr_area;
+  if (!hxtr && s->sc_wrap != IMX_WRAP_PXT16) {
+    trhx_v.assign((size_t)3 * nt, 0.0);
+    hxtr_v.assign((size_t)MX * nv, 0.0);
+    trskw_v.assign((size_t)MX * MX * nv, 0.0);
+    imx::gridprop(d, trhx_v.data(), hxtr_v.data(), trskw_v.data());
+    hxtr = hxtr_v.data();
+  }
+  for (int co = 0; co < nv; ++co) {
+    double w[MX];
+    if (s->sc_wrap == IMX_WRAP_PXT16) imx::wachspress_node(d, co, w);
+    for (int j = 0; j < d.v_nnb[co]; ++j) {
+      ctr[(size_t)j * nv + cn[co]] = tn[d.v_tr[MX * co + j] - 1];
+      cw[(size_t)j * nv + cn[co]] = s->sc_wrap == IMX_WRAP_PXT16 ? w[j] : hxtr[MX * co + j];
+    }
+  }
+  HIPTRY(hipMemcpy(s->lte_ctr, ctr.data(), ctr.size() * sizeof(int32_t), hipMemcpyHostToDevice), "imx_swm_lte");
+  HIPTRY(hipMemcpy(s->lte_cw, cw.data(), cw.size() * sizeof(double), hipMemcpyHostToDevice), "imx_swm_lte");
+  // TC2 as initialize_fields sets it, resident like the reference's globals
+  std::vector<double> h(nv), u(ne), mass(nv), mom(ne);
+  rc = s->sc_hc ? imx_ic_tc2_hc(m, s->o.erad, h.data(), u.data()) : imx_ic_tc2(m, s->o.erad, h.data(), u.data());
+  if (rc) return rc;
+  if ((rc = imx_swm_set_state(s, h.data(), u.data()))) return rc;
+  if ((rc = imx_swm_tendency(s, h.data(), u.data(), mass.data(), mom.data()))) return rc;
+  // the analysis fields, on the tendency's input (rhn, run) and fields
+  const bool hc = s->sc_hc;
+  hipStream_t st = s->stream;
+  ref_perot_hx<<<grid_for(nv), TPB, 0, st>>>(nv, s->NE, s->c_nnb, s->c_ed, hc ? s->hc_c : s->c_ttg,
+                                                       hc ? s->hc_pc : s->e_c, s->c_p, s->e_dleng, s->c_area,
+                                                       s->run, s->f_vhx);
+  ref_lte_tr<<<grid_for(NT), TPB, 0, st>>>(NT, s->NE, s->t_ed, hc ? s->hc_t : s->t_tg,
+                                                     hc ? s->hc_pc : s->e_c, s->t_c, s->e_leng, s->t_area,
+                                                     s->f_uh, s->f_q_tr, s->f_lte_vtr);
+  ref_lte_cell<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->lte_ctr, s->lte_cw, s->f_vhx, s->rhn,
+                                                       s->f_q_tr, s->f_lte_vhh, s->f_lte_qhx, s->f_lte_vhq);
+  HIPTRY(hipGetLastError(), "imx_swm_lte launch");
+  HIPTRY(hipStreamSynchronize(st), "imx_swm_lte");
+  s->lte_done = true;
+  // fields in caller numbering
+  auto fld = [&](int which, size_t n, std::vector<double> &v) {
+    v.assign(n, 0.0);
+    return imx_swm_get_field(s, which, v.data());
+  };
+  std::vector<double> h_ed, uh, divuh, ke, eta, h_tr, q_tr, q_hx, q_ed, grad, vhq_tr, qg_tr, vh_hx, vhq_hx, uhq, lapu;
+  if ((rc = fld(IMX_FIELD_H_ED, ne, h_ed)) || (rc = fld(IMX_FIELD_UH, ne, uh)) ||
+      (rc = fld(IMX_FIELD_DIVUH, nv, divuh)) || (rc = fld(IMX_FIELD_KE_HX, nv, ke)) ||
+      (rc = fld(IMX_FIELD_ETA, nt, eta)) || (rc = fld(IMX_FIELD_H_TR, nt, h_tr)) ||
+      (rc = fld(IMX_FIELD_Q_TR, nt, q_tr)) || (rc = fld(IMX_FIELD_Q_HX, nv, q_hx)) ||
+      (rc = fld(IMX_FIELD_Q_ED, ne, q_ed)) || (rc = fld(IMX_FIELD_GRAD_GHBK, ne, grad)) ||
+      (rc = fld(IMX_FIELD_VHQ_TR, (size_t)3 * nt, vhq_tr)) || (rc = fld(IMX_FIELD_VH_HX, (size_t)3 * nv, vh_hx)) ||
+      (rc = fld(IMX_FIELD_VHQ_HX, (size_t)3 * nv, vhq_hx)) || (rc = fld(IMX_FIELD_UHQ_PERP, ne, uhq)))
+    return rc;
+  if (s->sc_apvm && (rc = fld(IMX_FIELD_Q_GRAD_TR, (size_t)3 * nt, qg_tr))) return rc;
+  if ((s->sc_diff || s->sc_hyper) && (rc = fld(IMX_FIELD_LAPU, ne, lapu))) return rc;
+  imx::LteFields f{h.data(), u.data(), h_ed.data(), uh.data(), divuh.data(), ke.data(), eta.data(),
+                   h_tr.data(), q_tr.data(), q_hx.data(), q_ed.data(), grad.data(), vhq_tr.data(),
+                   qg_tr.empty() ? nullptr : qg_tr.data(), vh_hx.data(), vhq_hx.data(), uhq.data(),
+                   lapu.empty() ? nullptr : lapu.data(), mass.data(), mom.data()};
+  return imx::lte_analysis(*m, hc, s->sc_wrap == IMX_WRAP_PXT16, s->o.erad, s->o.grav, f, err, mim);
 }
 
 int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc) {
@@ -1861,51 +2095,9 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   const int nv = s->nv, NE = s->NE, NT = s->NT;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
   imx::Alloc &A = s->A;
-  if (!s->e_nrs &&
-      (sc->apvm || sc->k2 > 0.0 || sc->k4 > 0.0 || sc->wrap != IMX_WRAP_TRSK10 || sc->stag != IMX_STAG_HTC)) {
-    // geometry of the options, internal numbering, points component-major
-    std::vector<int8_t> nrs(NE, 1);
-    std::vector<double> ec((size_t)3 * NE, 0.0), tc((size_t)3 * NT, 0.0), cp((size_t)3 * nv, 0.0);
-    for (int eo = 0; eo < d->ne; ++eo) {
-      const int e = en[eo];
-      const double *a = d->v_p + 3 * (d->ed_v[2 * eo] - 1), *b = d->v_p + 3 * (d->ed_v[2 * eo + 1] - 1);
-      double nr[3];
-      imx::cross3(a, b, nr);  // ed%nr up to its positive norm (smeshpack.f90:725-726)
-      const double *t1 = d->tr_c + 3 * (d->ed_sh[2 * eo] - 1), *t2 = d->tr_c + 3 * (d->ed_sh[2 * eo + 1] - 1);
-      double dp = 0.0;
-      for (int q = 0; q < 3; ++q) dp = dp + nr[q] * (t2[q] - t1[q]);
-      nrs[e] = (int8_t)(std::signbit(dp) ? -1 : 1);
-      for (int q = 0; q < 3; ++q) ec[(size_t)q * NE + e] = d->ed_c[3 * eo + q];
-    }
-    for (int ko = 0; ko < d->nt; ++ko)
-      for (int q = 0; q < 3; ++q) tc[(size_t)q * NT + tn[ko]] = d->tr_c[3 * ko + q];
-    for (int co = 0; co < nv; ++co)
-      for (int q = 0; q < 3; ++q) cp[(size_t)q * nv + cn[co]] = d->v_p[3 * co + q];
-    s->e_nrs = A.put(nrs);
-    s->e_c = A.put(ec);
-    s->t_c = A.put(tc);
-    s->c_p = A.put(cp);
-    s->f_qg_ed = A.get<double>(NE);
-    s->f_qg_tr = A.get<double>((size_t)3 * NT);
-    s->f_vhx = A.get<double>((size_t)3 * nv);
-    s->f_lapu = A.get<double>(NE);
-    s->f_lap2 = A.get<double>(NE);
-    s->f_divl = A.get<double>(nv);
-    s->f_vortl = A.get<double>(NT);
-    s->dif_hx = A.get<double>(nv);
-    s->dif_tr = A.get<double>(NT);
-    s->t_w = A.get<double>((size_t)3 * NT);
-    s->e_dw = A.get<double>((size_t)2 * NE);
-    s->e_nr = A.get<double>((size_t)3 * NE);
-    s->e_ed3 = A.get<int32_t>((size_t)2 * MX * NE);
-    s->hc_c = A.get<int8_t>((size_t)MX * nv);
-    s->hc_t = A.get<int8_t>((size_t)3 * NT);
-    s->hc_g = A.get<int8_t>(NE);
-    s->hc_tt = A.get<int8_t>(NE);
-    s->hc_pc = A.get<double>((size_t)3 * NE);
-    s->hc_hb = A.get<double>((size_t)3 * NE);
-    s->hc_hv = A.get<int32_t>((size_t)3 * NE);
-    if (A.err != hipSuccess) return imx::hip_fail(A.err, "imx_swm_set_scheme");
+  if (sc->apvm || sc->k2 > 0.0 || sc->k4 > 0.0 || sc->wrap != IMX_WRAP_TRSK10 || sc->stag != IMX_STAG_HTC) {
+    const int rc = imx::scheme_tables(s, d);
+    if (rc) return rc;
   }
   if (sc->stag == IMX_STAG_HC) {
     // HC signs from the geometry: s_e = sign(ed%tg . edhx%nr) turns the

@@ -6,13 +6,14 @@
 ! reference built or optimised, e.g. SCVT, run unchanged); otherwise the
 ! icosahedral non-optimised grid is built with libimx and, with
 ! loadable = 1, stored in grid/ for later runs (meshstore). Then runs
-! simulcase 15, the shallow-water model (src/imodel.f90:152-153).
+! simulcase 15, the shallow-water model, or simulcase 14, its local
+! truncation error analysis (src/imodel.f90:149-153).
 !=======================================================================
 program imodel
   use iso_c_binding
   use imx_c
   use swm_par
-  use swm, only: swm_tests
+  use swm, only: swm_tests, swm_horiz_loc_trunc_er
   implicit none
   type(c_ptr) :: meshp = c_null_ptr
   integer :: g
@@ -44,10 +45,13 @@ program imodel
   call imx_check(imx_mesh_gridprop(meshp), 'initialize_gridprop')
 
   select case (simulcase)
+  case (14)
+    call swm_horiz_loc_trunc_er(meshp, g)
   case (15)
     call swm_tests(meshp, g)
   case default
-    print*, "simulcase", simulcase, "is not part of this build (15: shallow water model)"
+    print*, "simulcase", simulcase, "is not part of this build (14: local truncation error, ", &
+      "15: shallow water model)"
     stop 1
   end select
   call imx_mesh_destroy(meshp)

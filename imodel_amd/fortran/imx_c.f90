@@ -13,6 +13,7 @@ module imx_c
                                    IMX_FIELD_DIVUH = 6, IMX_FIELD_KE_HX = 7, IMX_FIELD_DIVU = 11
   integer(c_int32_t), parameter :: IMX_POS_EQS = 0, IMX_POS_POL = 1
   integer, parameter :: IMX_NDIAG = 8
+  integer, parameter :: IMX_LTE_NROW = 18, IMX_LTE_NMIM = 7
 
   type, bind(C) :: imx_mesh_desc
     integer(c_int32_t) :: nv, ne, nt, glevel
@@ -182,6 +183,12 @@ module imx_c
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: s
       integer(c_int32_t), value :: nsteps
+      integer(c_int) :: rc
+    end function
+    function imx_swm_lte(s, m, err, mim) bind(C, name='imx_swm_lte') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: s, m
+      real(c_double) :: err(*), mim(*)
       integer(c_int) :: rc
     end function
     function imx_swm_diagnostics(s, out) bind(C, name='imx_swm_diagnostics') result(rc)

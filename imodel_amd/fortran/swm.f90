@@ -1,6 +1,7 @@
 !=======================================================================
 ! swm: the shallow-water module of the imodel driver, on the GPU.
 ! Keeps the reference call surface (src/swm.f90): swm_tests (:102-380),
+! swm_horiz_loc_trunc_er (:382-734),
 ! tendency(h, u, masseq, momeq) (:2416-2598), ode_rk4 (:2600-2683); the
 ! state lives in HBM behind one libimx handle (the reference keeps it in
 ! the swm_data module globals, src/swm_data.f90:31-334).
@@ -31,13 +32,10 @@ contains
   subroutine swm_tests(meshp, glevel)
     type(c_ptr), intent(in) :: meshp
     integer, intent(in) :: glevel
-    type(imx_swm_opts) :: o
-    type(imx_scheme) :: sc
     real(c_double) :: d0(IMX_NDIAG), d(IMX_NDIAG)
     real(r8), pointer :: leng(:)
     real(r8) :: maxvel, minvdist, time
     integer :: k, kn
-    character(len=16) :: atmp
 
     call read_swm_par(glevel)
     call imx_check(imx_mesh_desc_of(meshp, mdesc), 'imx_mesh_desc_of')
@@ -60,61 +58,9 @@ contains
     cfl = abs(maxvel*dt/(minvdist*erad))   ! swm.f90:2383
     print*, "CFL:", cfl
 
-    write(atmp, '(i8)') testcase
-    swmname = "swm_tc"//trim(adjustl(atmp))
-    write(atmp, '(i8)') nint(dt)
-    ! stag, wrapper (or the explicit method names), area method (swm_data.f90:640-656)
-    if (trim(mtdwrapper) /= 'none') then
-      swmname = trim(swmname)//"_dt"//trim(adjustl(atmp))//"_"//trim(stag)//"_"//trim(adjustl(mtdwrapper))// &
-                "_areageo"
-    else
-      swmname = trim(swmname)//"_dt"//trim(adjustl(atmp))//"_"//trim(stag)// &
-                "_vrectrsk_crectrsk_sinttrsk_grdtrsk_areageo"
-    end if
-    ! scheme-option suffixes of swmpars (src/swm_data.f90:658-683)
-    if (trim(pv_stab) /= 'none') then
-      write(atmp, '(f5.2)') real(pvspar)
-      swmname = trim(swmname)//"_pvs"//trim(adjustl(trim(pv_stab)))//trim(adjustl(trim(atmp)))
-    end if
-    if (k2_max > 0) then
-      write(atmp, '(f10.3)') real(dlog(k2_max)/dlog(10._r8))
-      swmname = trim(swmname)//"_"//trim(diffus)//"_diffusion_10to"//trim(adjustl(trim(atmp)))
-    end if
-    if (k4_max > 0) then
-      write(atmp, '(f10.3)') real(dlog(k4_max)/dlog(10._r8))
-      swmname = trim(swmname)//"_"//trim(hyperdiffus)//"_hyperdiffusion_10to"//trim(adjustl(trim(atmp)))
-    end if
-    write(atmp, '(i8)') glevel
-    meshname = trim(mkind)//"_"//trim(mpos)//"_"//trim(moptm)//"_"//trim(adjustl(atmp))
+    call make_names(glevel, .true.)
 
-    call imx_swm_default_opts(o)
-    o%precision = precision_code()
-    o%testcase = merge(2, testcase, testcase == 100)
-    o%use_sfc = use_sfc
-    o%use_graph = use_graph
-    o%erad = erad
-    o%grav = grav
-    o%omega = omega
-    o%dt = dt
-    call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
-    if (scheme_options()) then
-      ! the fused fp32/fp64 kernels carry K2 diffusion of HTC trsk10 only
-      if (trim(precision) /= 'fp64_ref' .and. (wrap_code() /= 0 .or. trim(pv_stab) == 'apvm' .or. &
-          k4_max > 0._r8 .or. trim(stag) == 'HC')) then
-        write(*, '(a)') " ERROR: HC, pxt16, gass18, APVM and hyperdiffusion run on the fp64_ref path"
-        write(*, '(a)') "        (par/swm_gpu.par)"
-        stop 1
-      end if
-      sc%stag = merge(1, 0, trim(stag) == 'HC')
-      sc%wrap = wrap_code()
-      sc%apvm = merge(1, 0, trim(pv_stab) == 'apvm')
-      sc%pvspar = pvspar
-      sc%k2 = max(k2_max, 0._r8)
-      sc%diffus = max(coef_code(diffus), 0)
-      sc%k4 = max(k4_max, 0._r8)
-      sc%hyperdiffus = max(coef_code(hyperdiffus), 0)
-      call imx_check(imx_swm_set_scheme(handle, mdesc, sc), 'imx_swm_set_scheme')
-    end if
+    call open_handle()
     call imx_check(imx_swm_set_state(handle, h, u), 'imx_swm_set_state')
     if (normal_modes == 1) then
       call swm_normal_mode_analysis()
@@ -174,6 +120,149 @@ contains
     call imx_swm_destroy(handle)
     handle = c_null_ptr
   end subroutine swm_tests
+
+  ! swmname and meshname as swmpars builds them (src/swm_data.f90:640-683);
+  ! without the time step when swmpars runs with usetime = .false. (the LTE
+  ! analysis)
+  subroutine make_names(glevel, usetime)
+    integer, intent(in) :: glevel
+    logical, intent(in) :: usetime
+    character(len=16) :: atmp
+    character(len=32) :: dtpart
+    write(atmp, '(i8)') testcase
+    swmname = "swm_tc"//trim(adjustl(atmp))
+    write(atmp, '(i8)') nint(dt)
+    dtpart = ""
+    if (usetime) dtpart = "_dt"//trim(adjustl(atmp))
+    ! stag, wrapper (or the explicit method names), area method (swm_data.f90:640-656)
+    if (trim(mtdwrapper) /= 'none') then
+      swmname = trim(swmname)//trim(dtpart)//"_"//trim(stag)//"_"//trim(adjustl(mtdwrapper))// &
+                "_areageo"
+    else
+      swmname = trim(swmname)//trim(dtpart)//"_"//trim(stag)// &
+                "_vrectrsk_crectrsk_sinttrsk_grdtrsk_areageo"
+    end if
+    ! scheme-option suffixes of swmpars (src/swm_data.f90:658-683)
+    if (trim(pv_stab) /= 'none') then
+      write(atmp, '(f5.2)') real(pvspar)
+      swmname = trim(swmname)//"_pvs"//trim(adjustl(trim(pv_stab)))//trim(adjustl(trim(atmp)))
+    end if
+    if (k2_max > 0) then
+      write(atmp, '(f10.3)') real(dlog(k2_max)/dlog(10._r8))
+      swmname = trim(swmname)//"_"//trim(diffus)//"_diffusion_10to"//trim(adjustl(trim(atmp)))
+    end if
+    if (k4_max > 0) then
+      write(atmp, '(f10.3)') real(dlog(k4_max)/dlog(10._r8))
+      swmname = trim(swmname)//"_"//trim(hyperdiffus)//"_hyperdiffusion_10to"//trim(adjustl(trim(atmp)))
+    end if
+    write(atmp, '(i8)') glevel
+    meshname = trim(mkind)//"_"//trim(mpos)//"_"//trim(moptm)//"_"//trim(adjustl(atmp))
+
+  end subroutine make_names
+
+  ! One libimx handle for the run, with the scheme options of par/swm.par
+  subroutine open_handle()
+    type(imx_swm_opts) :: o
+    type(imx_scheme) :: sc
+    call imx_swm_default_opts(o)
+    o%precision = precision_code()
+    o%testcase = merge(2, testcase, testcase == 100)
+    o%use_sfc = use_sfc
+    o%use_graph = use_graph
+    o%erad = erad
+    o%grav = grav
+    o%omega = omega
+    o%dt = dt
+    call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
+    if (scheme_options()) then
+      ! the fused fp32/fp64 kernels carry K2 diffusion of HTC trsk10 only
+      if (trim(precision) /= 'fp64_ref' .and. (wrap_code() /= 0 .or. trim(pv_stab) == 'apvm' .or. &
+          k4_max > 0._r8 .or. trim(stag) == 'HC')) then
+        write(*, '(a)') " ERROR: HC, pxt16, gass18, APVM and hyperdiffusion run on the fp64_ref path"
+        write(*, '(a)') "        (par/swm_gpu.par)"
+        stop 1
+      end if
+      sc%stag = merge(1, 0, trim(stag) == 'HC')
+      sc%wrap = wrap_code()
+      sc%apvm = merge(1, 0, trim(pv_stab) == 'apvm')
+      sc%pvspar = pvspar
+      sc%k2 = max(k2_max, 0._r8)
+      sc%diffus = max(coef_code(diffus), 0)
+      sc%k4 = max(k4_max, 0._r8)
+      sc%hyperdiffus = max(coef_code(hyperdiffus), 0)
+      call imx_check(imx_swm_set_scheme(handle, mdesc, sc), 'imx_swm_set_scheme')
+    end if
+  end subroutine open_handle
+
+  ! swm_horiz_loc_trunc_er (src/swm.f90:382-734): TC2's closed forms against
+  ! one tendency evaluation (imx_swm_lte: the tendency and its analysis fields
+  ! on the GPU). Prints the reference's table and mimetic sums and appends
+  ! the table to data/swm_loc_trunc_errors.txt in its layout.
+  subroutine swm_horiz_loc_trunc_er(meshp, glevel)
+    type(c_ptr), intent(in) :: meshp
+    integer, intent(in) :: glevel
+    real(c_double) :: err(3*IMX_LTE_NROW), mim(IMX_LTE_NMIM)
+    character(len=16), parameter :: rows(IMX_LTE_NROW) = [character(len=16) :: &
+      " h_ed ", " divuh ", " kenergy ", " abs_vort ", " h_tr ", " q_tr ", " q_hx ", " q_ed ", &
+      " grad_ghbK ", " grad_ghbK_exact ", " vhq_tr ", " q_grad_tr ", " vh_hx ", " vhq_hx ", &
+      " uhq_perp ", " lapu ", " masseq ", " momeq "]
+    character(len=100), parameter :: fmt = "(a16, i12,  3e18.8, a80)"
+    integer :: fu, r
+    logical :: ex
+
+    call read_swm_par(glevel)
+    if (testcase /= 2) then
+      write(*, '(a)') " ERROR: the local truncation error analysis has the closed forms of test case 2"
+      stop 1
+    end if
+    if (trim(precision) /= 'fp64_ref') then
+      write(*, '(a)') " ERROR: the local truncation error analysis runs on the fp64_ref path (par/swm_gpu.par)"
+      stop 1
+    end if
+    call imx_check(imx_mesh_desc_of(meshp, mdesc), 'imx_mesh_desc_of')
+    call make_names(glevel, .false.)
+    call open_handle()
+    call imx_check(imx_swm_lte(handle, meshp, err, mim), 'swm_horiz_loc_trunc_er')
+    call imx_swm_destroy(handle)
+    handle = c_null_ptr
+
+    call imx_check(imx_make_dir('data'//c_null_char), 'swm_horiz_loc_trunc_er')
+    inquire(file="data/swm_loc_trunc_errors.txt", exist=ex)
+    if (ex) then
+      open(newunit=fu, file="data/swm_loc_trunc_errors.txt", status='old', position='append')
+    else
+      open(newunit=fu, file="data/swm_loc_trunc_errors.txt", status='replace')
+      write(fu, *) "          Field       Mesh       MaxErrorRel       "//&
+        " RMSError         MaxError                            "//&
+        "Methods                                     Grid"
+    end if
+    print*, " Field       Mesh       MaxErrorRel        RMSError         MaxError"
+    do r = 1, IMX_LTE_NROW
+      print '(a16, i12,  3e18.8)', rows(r), mdesc%nv, err(3*r-2), err(3*r-1), err(3*r)
+      write(fu, trim(fmt)) rows(r), mdesc%nv, err(3*r-2), err(3*r-1), err(3*r), &
+        trim(swmname)//" "//trim(meshname)
+    end do
+    close(fu)
+    print*
+    print*, "Verify mimetic properties (normalized)"
+    print*, "-------------------------"
+    print*
+    print*, "1) Mass Conservation"
+    print*, " To be analysed on the evolving system"
+    print*
+    print*, "2) Product rule - normalized fields"
+    print*, "Sum of h*Div(u): ", mim(1)
+    print*, "Sum of h*Div(u)+u*Grad(h): ", mim(2)
+    print*, "Sum of Div(hu): ", mim(3)
+    print*, "    Difference:", mim(4)
+    print*
+    print*, "3) Cross rule - normalized fields"
+    print*, "Sum of k.Rot(Grad(h)):", mim(5)
+    print*
+    print*, "4) Coriolis spurious energy budget - normalized fields "
+    print*, "Sum of Coriolis energy (Ae*q*h^2*u*u_perp):", mim(6)
+    print*, "Sum of Coriolis energy / Tenergy:", mim(7)
+  end subroutine swm_horiz_loc_trunc_er
 
   ! swm_normal_mode_analysis (src/swm.f90:738-976): the linearised operator
   ! of the initial state by one-sided differences with pert = mean cell

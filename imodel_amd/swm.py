@@ -49,6 +49,14 @@ class Mesh:
         return cls._from_handle(h, gridprop)
 
     @classmethod
+    def read(cls, path, gridprop=True):
+        """A mesh from an IMXBIN file in imx_mesh_write's layout (the
+        reference harness's mesh dumps share it)."""
+        h = ctypes.c_void_p()
+        capi.check(capi.lib().imx_mesh_read(str(path).encode(), ctypes.byref(h)), "imx_mesh_read")
+        return cls._from_handle(h, gridprop)
+
+    @classmethod
     def _from_handle(cls, h, gridprop):
         L = capi.lib()
         if gridprop:
@@ -250,11 +258,26 @@ class Swm:
         return csr_matrix((v, (r - 1, c - 1)), shape=(n, n)), int(ncol[0]) + 1
 
     def field(self, name):
-        n = {"v": self.nv, "e": self.ne, "t": self.nt}[capi.FIELD_POS[name]]
-        out = np.zeros(n)
+        pos = capi.FIELD_POS[name]
+        n = {"v": self.nv, "e": self.ne, "t": self.nt}[pos[0]]
+        out = np.zeros((n, 3) if pos.endswith("3") else n)
         capi.check(capi.lib().imx_swm_get_field(self._h, capi.FIELDS.index(name), capi.f64p(out)),
                    "imx_swm_get_field")
         return out
+
+    def lte(self):
+        """Local truncation error analysis on TC2 (swm_horiz_loc_trunc_er):
+        sets TC2 as the state, evaluates the tendency and returns
+        ({row: (max relative, relative L2, max)}, {mimetic sum: value}).
+        Needs an FP64_REF handle with testcase 2 on a library mesh."""
+        if self.mesh._handle is None:
+            raise ValueError("lte needs a library mesh (Mesh.icos, Mesh.read, Mesh.load_ref)")
+        err = np.zeros(3 * len(capi.LTE_ROWS))
+        mim = np.zeros(len(capi.LTE_MIM))
+        capi.check(capi.lib().imx_swm_lte(self._h, self.mesh._handle, capi.f64p(err), capi.f64p(mim)),
+                   "imx_swm_lte")
+        rows = {k: tuple(err[3 * i:3 * i + 3]) for i, k in enumerate(capi.LTE_ROWS)}
+        return rows, dict(zip(capi.LTE_MIM, mim))
 
     def step(self, nsteps=1):
         capi.check(capi.lib().imx_swm_step(self._h, int(nsteps)), "imx_swm_step")

@@ -133,6 +133,14 @@ typedef struct imx_swm  imx_swm;   /* one SWM instance; owns device memory */
 #define IMX_FIELD_Q_GRAD_ED 13 /* [ne]  grad_tr2ed(q_tr), with APVM            */
 #define IMX_FIELD_LAPU     14  /* [ne]  diffusion_ed of u, with K2 or K4       */
 #define IMX_FIELD_LAP_LAPU 15  /* [ne]  hyperdiffusion_ed of u, with K4        */
+/* Fields of the last imx_swm_lte evaluation (the tendency's analysis fields
+   under test_lterror = 1, src/swm.f90:2500-2503, :2546-2551); vectors are
+   Cartesian, x y z per entry. */
+#define IMX_FIELD_Q_HX     16  /* [nv]    scalar_tr2hx(q_tr)                   */
+#define IMX_FIELD_VHQ_TR   17  /* [nt*3]  vector_edhx2tr_perot(uh) q_tr        */
+#define IMX_FIELD_VH_HX    18  /* [nv*3]  v_hx h                               */
+#define IMX_FIELD_VHQ_HX   19  /* [nv*3]  v_hx h q_hx                          */
+#define IMX_FIELD_Q_GRAD_TR 20 /* [nt*3]  vector_edtr2tr_perot(q_grad_ed), APVM */
 
 /* Scheme options of par/swm.par beyond the trsk10/HTC defaults
    (swmpars, src/swm_data.f90:413-424, :430-551), for FP64_REF handles:
@@ -183,6 +191,10 @@ int  imx_mesh_build_icos(int32_t glevel, int32_t pos, imx_mesh **out);
 int  imx_mesh_desc_of(const imx_mesh *m, imx_mesh_desc *out);
 int  imx_mesh_gridprop(imx_mesh *m);             /* fills the optional arrays */
 int  imx_mesh_write(const imx_mesh *m, const char *path);  /* IMXBIN records  */
+/* Reads what imx_mesh_write writes (the records of grid_structure that
+   imx_mesh_desc and the closed-form test cases read): no counterpart in the
+   reference, which rebuilds or meshloads its grids. */
+int  imx_mesh_read(const char *path, imx_mesh **out);
 /* Grid files in the reference's binary layout (meshload / meshwrite,
    src/smeshpack.f90:7702-8221): <dir>/<name>_header.dat and the ten data
    files (Fortran unformatted sequential records). Loading fills every array
@@ -254,6 +266,46 @@ int  imx_swm_get_field(imx_swm *s, int32_t which, double *out);
 int  imx_swm_jacobian(imx_swm *s, const imx_mesh_desc *d, const double *h, const double *u,
                       double pert, int64_t cap, int32_t *row, int32_t *col, double *val,
                       int64_t *nnz, int32_t *ncolors);
+/* Local truncation error analysis, swm_horiz_loc_trunc_er
+   (src/swm.f90:382-734), for TC2 on an FP64_REF handle created with
+   testcase 2 on mesh m, with the handle's scheme options. Sets the TC2 state
+   (initialize_fields, src/swm.f90:1205-1326) as the resident state, evaluates
+   the tendency with the analysis fields (IMX_FIELD_Q_HX..), and compares 18
+   fields with TC2's closed forms.
+   err[3 r + 0..2] = error_calc's (max relative, relative L2, max) of row r,
+   rows in the order the reference prints them (IMX_LTE_*). Fields the
+   tendency does not compute under the handle's options are zero, as in the
+   reference: q_grad_tr without APVM, lapu without K2 or K4.
+   mim[IMX_LTE_NMIM]: the mimetic sums it prints (src/swm.f90:631-718),
+   see IMX_MIM_*. */
+#define IMX_LTE_H_ED            0
+#define IMX_LTE_DIVUH           1
+#define IMX_LTE_KENERGY         2
+#define IMX_LTE_ABS_VORT        3
+#define IMX_LTE_H_TR            4
+#define IMX_LTE_Q_TR            5
+#define IMX_LTE_Q_HX            6
+#define IMX_LTE_Q_ED            7
+#define IMX_LTE_GRAD_GHBK       8
+#define IMX_LTE_GRAD_GHBK_EXACT 9  /* grad_ed of the exact g h + K           */
+#define IMX_LTE_VHQ_TR         10
+#define IMX_LTE_Q_GRAD_TR      11
+#define IMX_LTE_VH_HX          12
+#define IMX_LTE_VHQ_HX         13
+#define IMX_LTE_UHQ_PERP       14
+#define IMX_LTE_LAPU           15
+#define IMX_LTE_MASSEQ         16
+#define IMX_LTE_MOMEQ          17
+#define IMX_LTE_NROW           18
+#define IMX_MIM_HDIVU          0  /* sum of h div u                          */
+#define IMX_MIM_PRODRULE       1  /* sum of h div u + u grad h               */
+#define IMX_MIM_DIVUH          2  /* sum of div(hu)                          */
+#define IMX_MIM_PRODDIFF       3  /* |[2] - [1]|                             */
+#define IMX_MIM_CROSSRULE      4  /* sum of k . rot(grad h)                  */
+#define IMX_MIM_CORIOLIS       5  /* sum of Coriolis energy                  */
+#define IMX_MIM_CORIOLIS_REL   6  /* [5] / total energy                      */
+#define IMX_LTE_NMIM           7
+int  imx_swm_lte(imx_swm *s, const imx_mesh *m, double *err, double *mim);
 /* nsteps x ode_rk4 on the resident state            src/swm.f90:2600-2683 */
 int  imx_swm_step(imx_swm *s, int32_t nsteps);
 /* as imx_swm_step, returns device time (HIP events on the handle's stream)  */

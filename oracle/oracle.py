@@ -129,7 +129,7 @@ class Oracle:
             pass
 
     def field(self, name):
-        n = {"v": self.nv, "e": self.ne, "t": self.nt}[capi.FIELD_POS.get(name, "v")]
+        n = {"v": self.nv, "e": self.ne, "t": self.nt}[capi.FIELD_POS.get(name, "v")[0]]
         if name in ("eta_ed", "q_grad_ed", "lapu", "lap_lapu"):
             n = self.ne
         if name == "hbt":

@@ -19,6 +19,11 @@
 !   ref_harness build <glevel> <eqs|pol> <out.bin>
 !   ref_harness run   <mesh: build:G:pos | file.bin> <state: tc2 | file.bin>
 !                     <nsteps> <dt> <diag 0|1> <out.bin> [scheme]
+!   ref_harness lte   <mesh: build:G:pos | file.bin> <out.bin> [scheme]
+!                     swm_horiz_loc_trunc_er (src/swm.f90:382-734) on TC2:
+!                     its table and mimetic sums go to stdout; out.bin gets
+!                     error_calc / error_calc_vec of every row it prints
+!                     (lte_<row> = maxrel, rel L2, max) and exact fields
 ! scheme: comma-separated key=value lines of par/swm.par, e.g.
 !   "stag=HTC,wrap=trsk10,k2=1e5,diffus=const,k4=0,hyperdiffus=const,pv=apvm,pvspar=0.5"
 ! (defaults: HTC, trsk10, no diffusion, no PV stabilisation); the
@@ -330,16 +335,17 @@ contains
 end module harness_mesh
 
 program ref_harness
-  use constants, only: i4, r8
+  use constants, only: i4, r8, grav, erad
   use imxbin
   use harness_mesh
   use smeshpack, only: meshbuild, meshload, showonscreen
   use swm_data
   use swm, only: tendency, ode_rk4, initialize_gridprop, initialize_fields, &
-       calc_energies, sumf_areas, sumfsq_areas
+       calc_energies, sumf_areas, sumfsq_areas, swm_horiz_loc_trunc_er, error_calc, error_calc_vec
   use omp_lib
   use interpack, only: plot_scalarfield
-  use datastruct, only: scalar_field
+  use diffoperpack, only: grad_edge_Cgrid
+  use datastruct, only: scalar_field, vector_field_cart, grid_structure
   use swm_operators, only: alignment_coef, diameter_coef, scalar_hx2ed, scalar_hx2tr
   implicit none
 
@@ -351,7 +357,8 @@ program ref_harness
   real(r8) :: pe0, ke0, te0, ae0, pe, ke, te, ae, mass0
   real(r8), allocatable :: series(:, :)
   real(r8), allocatable :: hst(:), ust(:)
-  integer :: pu
+  integer :: pu, l
+  type(grid_structure) :: meshcp
   character(len=512) :: scheme
   character(len=16) :: sc_stag = 'HTC', sc_wrap = 'trsk10', sc_diffus = 'const', sc_hdiffus = 'const'
   character(len=16) :: sc_pv = 'none'
@@ -491,6 +498,64 @@ program ref_harness
     call put_f8('wall', (/ wall1 - wall0, real(omp_get_max_threads(), r8) /))
     call bin_close()
 
+  case ('lte')
+    call get_command_argument(2, meshsrc)
+    call get_command_argument(3, outf)
+    call get_command_argument(4, scheme)
+    call parse_scheme(trim(scheme))
+    if (meshsrc(1:6) == 'build:') then
+      pc1 = 7
+      pc2 = index(meshsrc(pc1:), ':') + pc1 - 1
+      read(meshsrc(pc1:pc2-1), *) glevel
+      call build_ref_mesh(glevel, trim(meshsrc(pc2+1:)))
+    else
+      call load_mesh(trim(meshsrc), mesh)
+    end if
+    call write_swm_par(900._r8)
+    !the routine assigns its argument to the module mesh: hand it a copy
+    meshcp = mesh
+    call swm_horiz_loc_trunc_er(meshcp)
+    !it leaves grad_ghbK holding the gradient of the exact potential: the
+    !same tendency again restores the fields of the rows it printed
+    call tendency(h, u, masseq%f, momeq%f)
+    call bin_open_w(trim(outf))
+    call put_f8('h0', h%f); call put_f8('u0', u%f)
+    call lte_s('lte_h_ed', h_ed, h_ed_exact, h_ed_error)
+    call lte_s('lte_divuh', divuh, divuh_exact, divuh_error)
+    call lte_s('lte_kenergy', ke_hx, ke_hx_exact, ke_hx_error)
+    call lte_s('lte_abs_vort', eta, eta_exact, eta_error)
+    call lte_s('lte_h_tr', h_tr, h_tr_exact, h_tr_error)
+    call lte_s('lte_q_tr', q_tr, q_tr_exact, q_tr_error)
+    call lte_s('lte_q_hx', q_hx, q_hx_exact, q_hx_error)
+    call lte_s('lte_q_ed', q_ed, q_ed_exact, q_ed_error)
+    call lte_s('lte_grad_ghbK', grad_ghbK, grad_ghbK_exact, grad_ghbK_error)
+    call put_f8('x_grad_ghbK', grad_ghbK_exact%f)
+    !the row of the gradient of the exact potential (src/swm.f90:548-558)
+    ghbK%f(1:ghbK%n) = grav*(h_exact%f(1:ghbK%n) + bt%f(1:ghbK%n)) + ke_hx_exact%f(1:ghbK%n)
+    do l = 1, mesh%ne
+      grad_ghbK%f(l) = grad_edge_Cgrid(l, ghbk, u%pos, gradmtd, mesh)/erad
+    end do
+    call lte_s('lte_grad_ghbK_exact', grad_ghbK, grad_ghbK_exact, grad_ghbK_error)
+    call lte_v('lte_vhq_tr', vhq_tr, vhq_tr_exact, vhq_tr_error)
+    if (useAPVM) call lte_v('lte_q_grad_tr', q_grad_tr, q_grad_tr_exact, q_grad_tr_error)
+    call lte_v('lte_vh_hx', vh_hx, vh_hx_exact, vh_hx_error)
+    call lte_v('lte_vhq_hx', vhq_hx, vhq_hx_exact, vhq_hx_error)
+    call lte_s('lte_uhq_perp', uhq_perp, uhq_perp_exact, uhq_perp_error)
+    call lte_s('lte_masseq', masseq, masseq_exact, masseq_error)
+    call lte_s('lte_momeq', momeq, momeq_exact, momeq_error)
+    !fields of the tendency and the exact fields, for pinning each part
+    call put_f8('vhq_tr', (/ (vhq_tr%p(k)%v, k = 1, mesh%nt) /))
+    call put_f8('vh_hx', (/ (vh_hx%p(k)%v, k = 1, mesh%nv) /))
+    call put_f8('q_hx', q_hx%f)
+    call put_f8('x_h_ed', h_ed_exact%f); call put_f8('x_ke_hx', ke_hx_exact%f)
+    call put_f8('x_eta', eta_exact%f); call put_f8('x_q_ed', q_ed_exact%f)
+    call put_f8('x_uhq_perp', uhq_perp_exact%f); call put_f8('x_lapu', lapu_exact%f)
+    call put_f8('x_q_hx', q_hx_exact%f)
+    call put_f8('x_vhq_tr', (/ (vhq_tr_exact%p(k)%v, k = 1, mesh%nt) /))
+    call put_f8('x_q_grad_tr', (/ (q_grad_tr_exact%p(k)%v, k = 1, mesh%nt) /))
+    call put_f8('x_vhq_hx', (/ (vhq_hx_exact%p(k)%v, k = 1, mesh%nv) /))
+    call bin_close()
+
   case default
     print*, 'usage: ref_harness build <glevel> <eqs|pol> <out.bin>'
     print*, '       ref_harness run <build:G:pos|mesh.bin> <tc2|state.bin> <nsteps> <dt> <diag> <out.bin>'
@@ -500,6 +565,25 @@ program ref_harness
   end select
 
 contains
+
+  !error_calc of one scalar row of the LTE table (maxrel, rel L2, max)
+  subroutine lte_s(name, f, fref, fdif)
+    character(len=*), intent(in) :: name
+    type(scalar_field), intent(in) :: f, fref
+    type(scalar_field), intent(inout) :: fdif
+    real(r8) :: a, b, c
+    call error_calc(f, fref, fdif, a, b, c)
+    call put_f8(name, (/ a, b, c /))
+  end subroutine lte_s
+
+  subroutine lte_v(name, v, vref, vdif)
+    character(len=*), intent(in) :: name
+    type(vector_field_cart), intent(in) :: v, vref
+    type(vector_field_cart), intent(inout) :: vdif
+    real(r8) :: a, b, c
+    call error_calc_vec(v, vref, vdif, a, b, c)
+    call put_f8(name, (/ a, b, c /))
+  end subroutine lte_v
 
   subroutine build_ref_mesh(g, pos)
     integer, intent(in) :: g

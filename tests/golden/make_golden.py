@@ -16,6 +16,12 @@ outputs as fixtures:
                             const/align/diam coefficients; the pxt16 and
                             gass18 wrappers; HC staggering), plus v_hx,
                             q_grad_ed/tr, dif_coef_hx/tr, lapu, lap_lapu
+  ref_lte_g3_<scheme>.bin   swm_horiz_loc_trunc_er on TC2 with the options of
+                            LTE_SCHEMES: lte_<row> = error_calc triple of each
+                            row of its table, mim = the mimetic sums it prints
+                            (parsed from its output), and the analysis and
+                            exact fields (``python make_golden.py lte`` makes
+                            only these)
 
 The seeded states are made by libimx (imx_ic_spectral) on the reference
 mesh and handed to the reference as input (the reference has no such case).
@@ -52,6 +58,46 @@ SCHEMES = {
 }
 
 
+# par/swm.par options of the LTE fixtures (ref_harness lte ... [scheme])
+LTE_SCHEMES = {
+    "trsk10": "",
+    "pxt16": "wrap=pxt16",
+    "gass18": "wrap=gass18",
+    "apvm": "pv=apvm,pvspar=0.5",
+    "hc": "stag=HC",
+    "hcpx": "stag=HC,wrap=pxt16",
+    "hcgass": "stag=HC,wrap=gass18,pv=apvm,pvspar=0.5",
+}
+# the mimetic lines of swm_horiz_loc_trunc_er (src/swm.f90:641-718), in the
+# order of imx_swm_lte's mim vector
+MIM_LINES = ("Sum of h*Div(u):", "Sum of h*Div(u)+u*Grad(h):", "Sum of Div(hu):", "Difference:",
+             "Sum of k.Rot(Grad(h)):", "Sum of Coriolis energy (Ae*q*h^2*u*u_perp):",
+             "Sum of Coriolis energy / Tenergy:")
+
+
+def run_lte(spec, cwd, dest):
+    out = os.path.join(cwd, "out.bin")
+    if os.path.exists(out):
+        os.remove(out)
+    r = subprocess.run([HARNESS, "lte", "build:3:eqs", "out.bin", spec], cwd=cwd, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0 or not os.path.exists(out):
+        raise RuntimeError("lte " + spec + "\n" + r.stdout[-2000:])
+    mim = {}
+    for line in r.stdout.splitlines():
+        t = line.strip()
+        for k in MIM_LINES:
+            if t.startswith(k):
+                mim[k] = float(t[len(k):])
+    if len(mim) != len(MIM_LINES):
+        raise RuntimeError("lte " + spec + ": mimetic lines missing\n" + r.stdout[-2000:])
+    rec = imxbin.read(out)
+    rec["mim"] = [mim[k] for k in MIM_LINES]
+    if spec not in ("", "stag=HC"):  # the exact fields depend on the staggering only
+        rec = {k: v for k, v in rec.items() if not k.startswith("x_")}
+    imxbin.write(dest, rec)
+
+
 def run(args, cwd):
     # the reference's `stop` exits with status 0: a run counts only if it
     # wrote its output file
@@ -67,6 +113,11 @@ def run(args, cwd):
 def main():
     if not os.access(HARNESS, os.X_OK):
         sys.exit("build the reference harness first: make -f oracle/Makefile.ref")
+    with tempfile.TemporaryDirectory() as td:
+        for name, spec in LTE_SCHEMES.items():
+            run_lte(spec, td, os.path.join(HERE, f"ref_lte_g3_{name}.bin"))
+    if sys.argv[1:] == ["lte"]:
+        return
     with tempfile.TemporaryDirectory() as td:
         for g, pos, name in ((2, "eqs", "ref_mesh_g2.bin"), (3, "eqs", "ref_mesh_g3.bin"),
                              (2, "pol", "ref_mesh_g2_pol.bin")):

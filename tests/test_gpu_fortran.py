@@ -292,3 +292,50 @@ def test_fortran_driver_hc_tc2_bitexact():
     o = Oracle(m.arrays, scheme={"stag": "HC", "dt": 900.0})
     h, u = o.step(h0, u0, 900.0, 96)
     assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
+
+
+@pytest.mark.parametrize("stag", ["HTC", "HC"])
+def test_fortran_driver_lte(golden, stag):
+    """simulcase 14 (swm_horiz_loc_trunc_er) from par/mesh.par: the printed
+    table equals the reference's error triples to its 8 printed digits, the
+    mimetic sums (printed with 17) bit for bit, and
+    data/swm_loc_trunc_errors.txt has the reference's header and row layout."""
+    print(f"[gpu-case] fortran imodel g3 lte {stag}", flush=True)
+    from imodel_amd.capi import LTE_ROWS
+    r = golden("ref_lte_g3_trsk10.bin" if stag == "HTC" else "ref_lte_g3_hc.bin")
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 48, "fp64_ref")
+        p = os.path.join(td, "par", "mesh.par")
+        mesh = open(p).read().splitlines()
+        mesh[14] = "14"
+        open(p, "w").write("\n".join(mesh) + "\n")
+        p = os.path.join(td, "par", "swm.par")
+        swm = open(p).read().splitlines()
+        swm[8] = stag
+        open(p, "w").write("\n".join(swm) + "\n")
+        res = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                             text=True, timeout=120)
+        assert res.returncode == 0, res.stdout[-2000:]
+        lines = open(os.path.join(td, "data", "swm_loc_trunc_errors.txt")).read().splitlines()
+    out = res.stdout.splitlines()
+    table = {}
+    for ln in out:
+        f = ln.split()
+        if len(f) == 5 and f[1] == "642":
+            table[f[0]] = np.array([float(x) for x in f[2:]])
+    for row in LTE_ROWS:
+        if f"lte_{row}" in r:
+            assert np.allclose(table[row], r[f"lte_{row}"], rtol=6e-8, atol=0), (row, table[row])
+    mim = {}
+    for ln in out:
+        t = ln.strip()
+        for i, k in enumerate(("Sum of h*Div(u):", "Sum of h*Div(u)+u*Grad(h):", "Sum of Div(hu):",
+                               "Difference:", "Sum of k.Rot(Grad(h)):",
+                               "Sum of Coriolis energy (Ae*q*h^2*u*u_perp):",
+                               "Sum of Coriolis energy / Tenergy:")):
+            if t.startswith(k):
+                mim[i] = float(t[len(k):])
+    assert [mim[i] for i in range(7)] == list(r["mim"])
+    assert lines[0].split() == ["Field", "Mesh", "MaxErrorRel", "RMSError", "MaxError", "Methods", "Grid"]
+    assert len(lines) == 1 + len(LTE_ROWS)
+    assert lines[1].split()[-2:] == [f"swm_tc2_{stag}_trsk10_areageo", "icos_eqs_nopt_3"]

@@ -5,6 +5,8 @@ of re-running a Delaunay insertion, so numbering differs from the reference;
 the mesh itself must be the same: bit-identical node coordinates, identical
 edge/triangle sets and CCW neighbour rings, geometry equal to rounding.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -110,3 +112,25 @@ def test_tc2_hc_matches_reference(golden):
     fsh = tmap[sh[:, 0]] != rsh[emap, 0]   # dual edge walked the other way
     us = np.where(fsh, -u, u)
     assert np.max(np.abs(us - r["u0"][emap])) < 1e-14 * np.max(np.abs(u))
+
+
+def test_mesh_read_round_trip(golden, tmp_path):
+    """imx_mesh_read takes the reference harness's mesh dump and what
+    imx_mesh_write writes, record for record."""
+    from imodel_amd.swm import Mesh
+    from tests.conftest import GOLDEN
+    r = golden("ref_mesh_g3.bin")
+    m = Mesh.read(os.path.join(GOLDEN, "ref_mesh_g3.bin"), gridprop=False)
+    for k, v in m.arrays.items():
+        if k in r:
+            assert np.array_equal(np.asarray(v).ravel(), np.asarray(r[k]).ravel()), k
+    a = Mesh.icos(2)
+    p = str(tmp_path / "m.bin")
+    a.write(p)
+    b = Mesh.read(p, gridprop=False)
+    for k, v in a.arrays.items():
+        assert np.array_equal(np.asarray(v), np.asarray(b.arrays[k])), k
+    with pytest.raises(Exception, match="IMXBIN"):
+        bad = tmp_path / "bad.bin"
+        bad.write_bytes(b"not a mesh")
+        Mesh.read(str(bad))
