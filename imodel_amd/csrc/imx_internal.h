@@ -97,6 +97,10 @@ struct Layout {
 bool build_layout(const imx_mesh_desc &d, bool use_sfc, Layout &L, std::string &err,
                   const int32_t *group = nullptr);
 void desc_of(const HostMesh &m, imx_mesh_desc &d);
+// meshstruct on nodes p [3 nv] and their CCW neighbour rings (0-based,
+// IMX_MAXNB slots per node): edges, triangles, geometry and sign tables.
+bool build_from_rings(HostMesh &m, int nv, const std::vector<double> &p, const std::vector<int32_t> &nnb,
+                      const std::vector<int32_t> &nb0, std::string &err);
 // Grid files in the reference's meshwrite/meshload layout (imx_gridio.cpp).
 bool load_ref(const std::string &dir, const std::string &name, HostMesh &m, std::string &err);
 bool write_ref(const HostMesh &m, const std::string &dir, const std::string &name, std::string &err);

@@ -366,6 +366,9 @@ static bool rings(int nv, const std::vector<Tri> &tris, std::vector<int32_t> &nn
   return true;
 }
 
+static bool structure(HostMesh &m, int nv, const std::vector<double> &p, const std::vector<int32_t> &nnb,
+                      const std::vector<int32_t> &nb0, std::string &err);
+
 static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
   const int MX = IMX_MAXNB;
   std::vector<double> p(36);
@@ -437,17 +440,26 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
     nv = nn;
   }
 
-  // ---- meshstruct-equivalent structuring (src/smeshpack.f90:370-1195) ----
   std::vector<int32_t> nnb, nb0;
   if (!rings(nv, tris, nnb, nb0)) {
     err = "bad neighbour ring";
     return false;
   }
+  m.glevel = glevel;
+  m.pos = (pos == IMX_POS_POL) ? "pol" : "eqs";
+  p.resize((size_t)3 * nv);
+  return structure(m, nv, p, nnb, nb0, err);
+}
+
+// ---- meshstruct-equivalent structuring (src/smeshpack.f90:370-1195) ----
+// From nodes and their CCW neighbour rings (0-based, IMX_MAXNB slots per
+// node, in the order meshstruct takes them).
+static bool structure(HostMesh &m, int nv, const std::vector<double> &p, const std::vector<int32_t> &nnb,
+                      const std::vector<int32_t> &nb0, std::string &err) {
+  const int MX = IMX_MAXNB;
   m.nv = nv;
   m.nt = 2 * nv - 4;
   m.ne = 3 * nv - 6;
-  m.glevel = glevel;
-  m.pos = (pos == IMX_POS_POL) ? "pol" : "eqs";
   const int ne = m.ne, nt = m.nt;
   m.v_p = p;
   m.v_p.resize((size_t)3 * nv);
@@ -672,6 +684,11 @@ static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
   return true;
 }
 
+bool build_from_rings(HostMesh &m, int nv, const std::vector<double> &p, const std::vector<int32_t> &nnb,
+                      const std::vector<int32_t> &nb0, std::string &err) {
+  return structure(m, nv, p, nnb, nb0, err);
+}
+
 }  // namespace imx
 
 // ------------------------------------------------------------------ C ABI --
@@ -699,6 +716,79 @@ int imx_mesh_build_icos(int32_t glevel, int32_t pos, imx_mesh **out) {
     return imx::set_error(IMX_ENOMEM, "imx_mesh_build_icos: out of memory");
   }
   *out = m;
+  return IMX_OK;
+}
+
+int imx_mesh_build_read(const char *path, imx_mesh **out) {
+  if (!path || !out) return imx::set_error(IMX_EINVAL, "imx_mesh_build_read: NULL argument");
+  *out = nullptr;
+  const std::string fn(path);
+  const std::string ext = fn.size() > 4 ? fn.substr(fn.size() - 3) : "";
+  if (ext != "xyz" && ext != "gmt")
+    return imx::set_error(IMX_EINVAL, "imx_mesh_build_read: unknown kind of file format. Extension: " + ext);
+  FILE *f = std::fopen(path, "r");
+  if (!f) return imx::set_error(IMX_EIO, "imx_mesh_build_read: " + fn + " not found");
+  std::vector<double> p;
+  int n = 0;
+  bool ok = true;
+  if (ext == "xyz") {
+    // meshread (src/smeshpack.f90:7549-7700): the node count, then x y z
+    ok = std::fscanf(f, "%d", &n) == 1 && n >= 4;
+    p.resize((size_t)3 * std::max(n, 0));
+    for (int i = 0; ok && i < n; ++i)
+      ok = std::fscanf(f, "%lf%*[ ,\t]%lf%*[ ,\t]%lf", &p[3 * i], &p[3 * i + 1], &p[3 * i + 2]) == 3;
+  } else {
+    // lon lat in degrees per line, to Cartesian with sph2cart (:4251-4277)
+    const double deg2rad = 3.1415927410125732 / 180.0;  // constants.f90:38 (single-rounded pi)
+    double lon, lat;
+    char line[512];
+    while (std::fgets(line, sizeof line, f)) {
+      if (std::sscanf(line, "%lf%*[ ,\t]%lf", &lon, &lat) != 2) continue;
+      lon = lon * deg2rad;
+      lat = lat * deg2rad;
+      const double c = std::cos(lat);
+      p.push_back(c * std::cos(lon));
+      p.push_back(c * std::sin(lon));
+      p.push_back(std::sin(lat));
+    }
+    n = (int)(p.size() / 3);
+    ok = n >= 4;
+  }
+  std::fclose(f);
+  if (!ok) return imx::set_error(IMX_EIO, "imx_mesh_build_read: cannot read the nodes of " + fn);
+  // the Delaunay structure: one line of 6 neighbour indices per node, 0 for
+  // none, in ring order
+  const std::string ngb = fn.substr(0, fn.size() - 4) + ".ngb";
+  f = std::fopen(ngb.c_str(), "r");
+  if (!f)
+    return imx::set_error(IMX_EIO, "imx_mesh_build_read: " + ngb +
+                                       " not found (the Delaunay triangulation of bare nodes, STRIPACK, is "
+                                       "not part of this build)");
+  const int MX = IMX_MAXNB;
+  std::vector<int32_t> nnb(n, 0), nb((size_t)MX * n, 0);
+  for (int i = 0; ok && i < n; ++i) {
+    int v[MX];
+    for (int j = 0; ok && j < MX; ++j) ok = std::fscanf(f, "%d", &v[j]) == 1;
+    for (int j = 0; ok && j < MX; ++j)
+      if (v[j] > 0) {
+        if (v[j] > n) ok = false;
+        else nb[(size_t)MX * i + nnb[i]++] = v[j] - 1;
+      }
+    ok = ok && nnb[i] >= 3;
+  }
+  std::fclose(f);
+  if (!ok) return imx::set_error(IMX_EIO, "imx_mesh_build_read: bad neighbour list in " + ngb);
+  auto m = std::make_unique<imx_mesh>();
+  m->kind = "read";
+  m->glevel = -1;
+  std::string err;
+  try {
+    if (!imx::build_from_rings(*m, n, p, nnb, nb, err))
+      return imx::set_error(IMX_EINVAL, "imx_mesh_build_read: " + err);
+  } catch (const std::bad_alloc &) {
+    return imx::set_error(IMX_ENOMEM, "imx_mesh_build_read: out of memory");
+  }
+  *out = m.release();
   return IMX_OK;
 }
 

@@ -24,10 +24,24 @@ program imodel
   call read_mesh_par()
   g = glevel_for(nv_req)
   gname = grid_name(g)
+  if (trim(mkind) == 'read') then
+    ! namegrid for read grids: the node file's name without its extension
+    gname = mname(1:len_trim(mname)-4)
+    if (trim(moptm) /= 'nopt') gname = trim(gname)//'_'//trim(moptm)
+    g = -1
+  end if
+  gridname = gname
   inquire(file='grid/'//trim(gname)//'_header.dat', exist=have)
   if (loadable == 1 .and. have) then
     print*, "Mesh: loading grid/", trim(gname)
     call imx_check(imx_mesh_load_ref('grid/'//c_null_char, trim(gname)//c_null_char, meshp), 'meshload')
+  else if (trim(mkind) == 'read') then
+    if (trim(moptm) /= 'nopt') call unsupported('mesh optimisation (without its grid files)', moptm)
+    print*, "Mesh: reading grid/", trim(mname)
+    call imx_check(imx_mesh_build_read('grid/'//trim(mname)//c_null_char, meshp), 'meshread')
+    if (loadable == 1) then
+      call imx_check(imx_mesh_write_ref(meshp, 'grid/'//c_null_char, trim(gname)//c_null_char), 'meshstore')
+    end if
   else
     if (trim(mkind) /= 'icos') call unsupported('mesh kind', mkind)
     if (trim(moptm) /= 'nopt') call unsupported('mesh optimisation (without its grid files)', moptm)

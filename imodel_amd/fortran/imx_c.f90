@@ -50,6 +50,12 @@ module imx_c
       type(c_ptr) :: out
       integer(c_int) :: rc
     end function
+    function imx_mesh_build_read(path, out) bind(C, name='imx_mesh_build_read') result(rc)
+      import :: c_int, c_char, c_ptr
+      character(kind=c_char) :: path(*)
+      type(c_ptr) :: out
+      integer(c_int) :: rc
+    end function
     function imx_mesh_gridprop(m) bind(C, name='imx_mesh_gridprop') result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: m

@@ -157,6 +157,7 @@ contains
     end if
     write(atmp, '(i8)') glevel
     meshname = trim(mkind)//"_"//trim(mpos)//"_"//trim(moptm)//"_"//trim(adjustl(atmp))
+    if (len_trim(gridname) > 0) meshname = gridname
 
   end subroutine make_names
 

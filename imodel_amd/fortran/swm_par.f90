@@ -19,6 +19,9 @@ module swm_par
   integer :: nv_req = 100, loadable = 0, showonscreen = 0, simulcase = 15, hrchy = 1
   character(len=16) :: mkind = 'icos', mpos = 'eqs', moptm = 'nopt'
   character(len=128) :: mname = ''
+  ! mesh%name of the grid the driver built or loaded (namegrid); the output
+  ! file names carry it
+  character(len=128) :: gridname = ''
   ! swm.par
   integer :: testcase = 2, test_lterror = 0, ntime = 0, adjustntime = 0
   integer :: normal_modes = 0

@@ -49,6 +49,14 @@ class Mesh:
         return cls._from_handle(h, gridprop)
 
     @classmethod
+    def build_read(cls, path, gridprop=True):
+        """meshbuild for mesh kind "read": nodes from a .xyz (or .gmt lon/lat)
+        file and the neighbour rings from the .ngb file beside it."""
+        h = ctypes.c_void_p()
+        capi.check(capi.lib().imx_mesh_build_read(str(path).encode(), ctypes.byref(h)), "imx_mesh_build_read")
+        return cls._from_handle(h, gridprop)
+
+    @classmethod
     def read(cls, path, gridprop=True):
         """A mesh from an IMXBIN file in imx_mesh_write's layout (the
         reference harness's mesh dumps share it)."""

@@ -188,6 +188,12 @@ const char *imx_last_error(void);
 /* ---- mesh (replaces meshbuild for icos/nopt grids, src/smeshpack.f90:77-368,
         and initialize_gridprop, src/swm.f90:979-1041) ------------------------ */
 int  imx_mesh_build_icos(int32_t glevel, int32_t pos, imx_mesh **out);
+/* meshbuild for mesh%kind = "read" (meshread, src/smeshpack.f90:7549-7700,
+   then meshstruct): nodes from path (.xyz: count then x y z per line; .gmt:
+   lon lat in degrees per line) and the neighbour rings from the .ngb file of
+   the same name (6 indices per node, 0 for none). Without the .ngb file the
+   reference triangulates with STRIPACK, which is not part of this build. */
+int  imx_mesh_build_read(const char *path, imx_mesh **out);
 int  imx_mesh_desc_of(const imx_mesh *m, imx_mesh_desc *out);
 int  imx_mesh_gridprop(imx_mesh *m);             /* fills the optional arrays */
 int  imx_mesh_write(const imx_mesh *m, const char *path);  /* IMXBIN records  */

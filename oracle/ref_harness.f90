@@ -19,6 +19,9 @@
 !   ref_harness build <glevel> <eqs|pol> <out.bin>
 !   ref_harness run   <mesh: build:G:pos | file.bin> <state: tc2 | file.bin>
 !                     <nsteps> <dt> <diag 0|1> <out.bin> [scheme]
+!   ref_harness buildread <name.xyz|name.gmt> <nv> <out.bin>
+!                     meshbuild for mesh%kind = 'read' from grid/<name> and
+!                     grid/<name without extension>.ngb, then as build
 !   ref_harness lte   <mesh: build:G:pos | file.bin> <out.bin> [scheme]
 !                     swm_horiz_loc_trunc_er (src/swm.f90:382-734) on TC2:
 !                     its table and mimetic sums go to stdout; out.bin gets
@@ -374,6 +377,26 @@ program ref_harness
     call get_command_argument(3, a2)
     call get_command_argument(4, outf)
     call build_ref_mesh(glevel, trim(a2))
+    call write_swm_par(900._r8)
+    call swmpars(usetime=.true.)
+    call allocate_globalswmvars()
+    call initialize_gridprop()
+    call bin_open_w(trim(outf))
+    call dump_mesh(mesh, .true.)
+    call bin_close()
+
+  case ('buildread')
+    call get_command_argument(2, a1)
+    call get_command_argument(3, a2); read(a2, *) glevel
+    call get_command_argument(4, outf)
+    mesh%kind = 'read'
+    mesh%pos = 'eqs'
+    mesh%optm = 'nopt'
+    mesh%loadable = 0
+    mesh%hrchy = 0
+    mesh%nv = glevel
+    mesh%name = trim(a1)
+    call meshbuild(mesh)
     call write_swm_par(900._r8)
     call swmpars(usetime=.true.)
     call allocate_globalswmvars()

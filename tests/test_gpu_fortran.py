@@ -348,3 +348,30 @@ def test_fortran_driver_lte(golden, stag):
                                            "Grid"]
     assert len(lines) == 2 + len(LTE_ROWS)
     assert lines[2].split()[-2:] == [f"swm_tc2_{stag}_trsk10_areageo", "icos_eqs_nopt_3"]
+
+
+def test_fortran_driver_read_grid():
+    """mesh kind "read" in par/mesh.par: the driver reads grid/<name>.xyz and
+    its .ngb (a glevel-3 grid with shuffled nodes and ring starts), stores
+    the grid files with loadable = 1, and its one-day FP64_REF TC2 run equals
+    the oracle on the same mesh bit for bit; the output files carry the grid
+    name."""
+    print("[gpu-case] fortran imodel read-kind grid g3", flush=True)
+    from tests.test_meshread import _write_grid
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 48, "fp64_ref", loadable=1)
+        _write_grid(td, "shuf3", 11, "xyz", glevel=3)
+        p = os.path.join(td, "par", "mesh.par")
+        mesh = open(p).read().splitlines()
+        mesh[4] = "read"
+        mesh[16] = "shuf3.xyz"
+        open(p, "w").write("\n".join(mesh) + "\n")
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        assert os.path.exists(os.path.join(td, "grid", "shuf3_header.dat"))
+        st = imxbin.read(os.path.join(td, "data", "swm_tc2_dt900_HTC_trsk10_areageo_state_shuf3.bin"))
+        m = Mesh.build_read(os.path.join(td, "grid", "shuf3.xyz"))
+    h0, u0 = m.tc2()
+    h, u = Oracle(m.arrays).step(h0, u0, 900.0, 96)
+    assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
