@@ -42,13 +42,13 @@ def impl_kernel_bytes(nv, ne, nt):
     # triangle: 2 vertex ids, edge bits, 3 eta coefficients, 3 kite weights,
     #           f, q out; gathers u (edges) and h (cells) once
     # cell:     6 neighbour ids, 2 bit words, 6 kite areas, 1/(A a) (FP64),
-    #           K normaliser, h/h0/hout/acc/phi, 6 FP64 dual lengths and 6 K
+    #           K normaliser, h/h0/hout/acc/phi, 6 dual lengths and 6 K
     #           weights (per slot); per edge u and the Coriolis partial out;
     #           q per triangle
     # edge:     other-cell id, 1/l, 2 partials, u0, acc (r+w), u out; phi
     return {
         "fast_tri": nt * (8 + 4 + 12 + 12 + 4 + 4) + ne * 4 + nv * 4,
-        "fast_cell": nv * (24 + 4 + 4 + 24 + 8 + 4 + 4 + 4 + 8 + 4 + 4 + 48 + 24) + ne * (4 + 8) + nt * 4,
+        "fast_cell": nv * (24 + 4 + 4 + 24 + 8 + 4 + 4 + 4 + 8 + 4 + 4 + 24 + 24) + ne * (4 + 8) + nt * 4,
         "fast_edge": ne * (4 + 4 + 4 + 4 + 4 + 8 + 4) + nv * 4,
     }
 

@@ -126,7 +126,10 @@ struct imx_swm {
   int32_t *x_c_nb = nullptr, *x_c_bitsA = nullptr, *x_c_bitsB = nullptr;
   void *x_c_R = nullptr, *x_c_ika = nullptr;
   double *x_c_iaa = nullptr;
-  double *x_c_dl = nullptr;  // [MX][nv] dual length of each cell's edge slots
+  // [MX][nv] dual length of each cell's edge slots, typed T: both cells of
+  // an edge read the same rounded value, so the mass fluxes stay
+  // antisymmetric and the total mass exact in FP32 as well
+  void *x_c_dl = nullptr;
   void *x_c_kw = nullptr, *x_e_il = nullptr;
   int32_t *x_e_oth = nullptr;
   int32_t *x_t_ab = nullptr, *x_t_eb = nullptr;

@@ -672,7 +672,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     int remap, int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
     const int32_t *__restrict__ c_bitsB, const T *__restrict__ c_R,
     const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
-    const double *__restrict__ c_dl, const T *__restrict__ c_kw, const T *__restrict__ q,
+    const T *__restrict__ c_dl, const T *__restrict__ c_kw, const T *__restrict__ q,
     const T *__restrict__ kh, const T *__restrict__ cz,
     const T *__restrict__ hs, const T *__restrict__ us, double href, T grav, T inv_a,
     T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     ue[j] = us[ed[j]];
     hn[j] = hs[nb[j]];
     qt[j] = q[tr[j]];
-    dl[j] = c_dl[j * nv + c];
+    dl[j] = (double)c_dl[j * nv + c];
     kw[j] = c_kw[j * nv + c];
   }
   double dv = 0.0, dvc = 0.0;
@@ -1104,7 +1104,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
 #define IMX_CELL(B)                                                                           \
   fast_cell<T, B><<<grid_bs(nv, B), B, 0, st>>>(                                             \
       s->kremap[1], nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_R, s->x_c_iaa, \
-      (const T *)s->x_c_ika, s->x_c_dl, (const T *)s->x_c_kw, (const T *)s->q,                    \
+      (const T *)s->x_c_ika, (const T *)s->x_c_dl, (const T *)s->x_c_kw, (const T *)s->q,       \
       s->f_diff ? (const T *)s->x_kh : nullptr, (const T *)s->x_cz, hs, us, s->href,             \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
@@ -1255,7 +1255,7 @@ static int build_fast(imx_swm *s, const imx_mesh_desc &d, const double *trhx, co
   s->x_c_iaa = A.put(c_iaa);
   s->x_c_ika = A.put(cvt<T>(c_ika));
   s->x_e_oth = A.put(e_oth);
-  s->x_c_dl = A.put(c_dl);
+  s->x_c_dl = A.put(cvt<T>(c_dl));
   s->x_c_kw = A.put(cvt<T>(c_kw));
   s->x_e_il = A.put(cvt<T>(e_il));
   s->x_t_ab = A.put(t_ab);
