@@ -120,7 +120,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is a design (DESIGN.md), not implemented", file=sys.stderr)
+              "sharding one instance across GPUs is designed but not implemented", file=sys.stderr)
         sys.exit(2)
 
     from imodel_amd.swm import Mesh, Swm
