@@ -198,3 +198,40 @@ def test_mass_conservation_fp32_day_g6():
     assert d1["badflag"] == 0
     assert abs(d1["mass"] - d0["mass"]) / d0["mass"] < 1e-7
     assert abs(d1["tenergy"] - d0["tenergy"]) / d0["tenergy"] < 1e-4
+
+
+@pytest.fixture(scope="module")
+def g8_case():
+    m = Mesh.icos(8)
+    h, u = m.spectral(5)
+    return m, h, u
+
+
+def test_ref_g8_bitexact_vs_oracle(g8_case):
+    """The configuration the throughput is quoted on (glevel 8, 655,362
+    cells, dt = 112.5 s): FP64_REF with SFC ordering and the graph equals the
+    oracle bit for bit over 2 RK4 steps."""
+    say("fp64_ref g8 2 steps vs oracle")
+    m, h, u = g8_case
+    ho, uo = Oracle(m.arrays).step(h, u, 112.5, 2)
+    with Swm(m, precision="fp64_ref", dt=112.5) as s:
+        s.set_state(h, u)
+        s.step(2)
+        hg, ug = s.get_state()
+    assert np.array_equal(hg, ho) and np.array_equal(ug, uo)
+
+
+def test_fast_g8_vs_oracle(g8_case):
+    """Glevel 8, 16 steps: the fused FP64 path to rounding and the FP32 path
+    to its storage tolerances (h - h_ref and u in FP32)."""
+    say("fast g8 16 steps vs oracle")
+    m, h, u = g8_case
+    ho, uo = Oracle(m.arrays).step(h, u, 112.5, 16)
+    A = m.arrays["hx_areag"]
+    for prec, tol_h, tol_u in (("fp64", 1e-12, 1e-11), ("fp32", 1e-7, 1e-5)):
+        with Swm(m, precision=prec, dt=112.5) as s:
+            s.set_state(h, u)
+            s.step(16)
+            hg, ug = s.get_state()
+        assert rel_l2(hg, ho, A) < tol_h, (prec, rel_l2(hg, ho, A))
+        assert rel_l2(ug, uo) < tol_u, (prec, rel_l2(ug, uo))
