@@ -308,11 +308,13 @@ def bench_scheme(args):
         log(f"{name}: {ms / steps:.3f} ms/step")
     for name in res:
         res[name]["vs_trsk10"] = res["trsk10"]["ms_per_step"] / res[name]["ms_per_step"]
-    # K2 diffusion on the fused kernels, FP32 and FP64, against their plain step
+    # APVM and K2 diffusion on the fused kernels, FP32 and FP64, against
+    # their plain step
     fused = {}
     for prec in ("fp32", "fp64"):
         for name, sc in (("trsk10", {}), ("k2const", SCHEME_SETS["k2const"]),
-                         ("k2diam", {"k2": 2e5, "diffus": "diam"})):
+                         ("k2diam", {"k2": 2e5, "diffus": "diam"}), ("apvm", SCHEME_SETS["apvm"]),
+                         ("apvm_k2", dict(SCHEME_SETS["apvm"], **SCHEME_SETS["k2const"]))):
             with Swm(mesh, precision=prec, dt=args.dt) as s:
                 s.set_scheme(**sc)
                 s.set_state(h0, u0)
@@ -323,7 +325,7 @@ def bench_scheme(args):
             fused[key] = {"ms_per_step": ms / steps, "cell_steps_per_s": nv * steps / (ms * 1e-3),
                           "options": sc}
             log(f"fused {key}: {ms / steps:.4f} ms/step")
-        for name in ("k2const", "k2diam"):
+        for name in ("k2const", "k2diam", "apvm", "apvm_k2"):
             fused[f"{prec}_{name}"]["vs_plain"] = (fused[f"{prec}_trsk10"]["ms_per_step"]
                                                    / fused[f"{prec}_{name}"]["ms_per_step"])
     top = res["pxmix"]
@@ -343,7 +345,7 @@ def bench_scheme(args):
                      "algorithmic_bytes_per_launch": B, "launch_ms": top["ms_per_step"]},
         "cpu_baseline": None,
         "schemes": res,
-        "fused_diffusion": fused,
+        "fused_options": fused,
     }
     print(json.dumps(out), flush=True)
 

@@ -667,13 +667,89 @@ __device__ __forceinline__ void kadd(double &s, double &c, double x) {
   s = t;
 }
 
+// Points and vectors of the fused APVM pass, one 16 B (FP32) or 32 B (FP64)
+// record each, so a gather is one vector load instead of three.
+template <typename T>
+struct alignas(4 * sizeof(T)) V4 {
+  T x, y, z, w;
+};
+
+// APVM on the fused path, one thread per cell after fast_tri: the Perot
+// velocity at the cell (vector_edhx2hx_perot) and, for the cell's owned
+// triangles, the Perot reconstruction of grad q (grad_tr2ed then
+// vector_edtr2tr_perot), each projected on the tangent plane. tcf folds
+// -tr%tg |e| / A_t and ed%nr-sign / (|e*| a) into one factor per slot, with
+// the sign of q(across) - q(t) against q(sh2) - q(sh1).
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void fast_apvm_pre(
+    int remap, int nv, int NE, int NT, const int32_t *__restrict__ c_nb,
+    const int32_t *__restrict__ c_bitsA, const int32_t *__restrict__ c_bitsB,
+    const T *__restrict__ c_dl, const double *__restrict__ c_iaa, double a,
+    const int32_t *__restrict__ t_ab, const int32_t *__restrict__ t_eb, const int32_t *__restrict__ tnb,
+    const T *__restrict__ tcf, const V4<T> *__restrict__ ec, const V4<T> *__restrict__ cp,
+    const V4<T> *__restrict__ tc, const T *__restrict__ us, const T *__restrict__ q, V4<T> *__restrict__ vc,
+    V4<T> *__restrict__ qg) {
+  const int c = ent_index<BS>(remap);
+  if (c >= nv) return;
+  const int A = c_bitsA[c];
+  const int B = c_bitsB[c];
+  const bool hex = ((A & 7) == MX);
+  const V4<T> pc = cp[c];
+  const T p0 = pc.x, p1 = pc.y, p2 = pc.z;
+  T v0 = (T)0, v1 = (T)0, v2 = (T)0;
+#pragma unroll
+  for (int j = 0; j < MX; ++j) {
+    if (j == MX - 1 && !hex) break;
+    const int nbj = c_nb[j * nv + c];
+    const int eo = ((A >> (BA_OWN + j)) & 1) ? c : nbj;
+    const int e = 3 * eo + ((B >> (2 * j)) & 3);
+    T w = us[e] * c_dl[j * nv + c];
+    if ((A >> (BA_TTG + j)) & 1) w = -w;
+    const V4<T> pe = ec[e];
+    v0 += w * (pe.x - p0);
+    v1 += w * (pe.y - p1);
+    v2 += w * (pe.z - p2);
+  }
+  const T sc = (T)(c_iaa[c] * a);
+  v0 *= sc;
+  v1 *= sc;
+  v2 *= sc;
+  T vp = v0 * p0 + v1 * p1 + v2 * p2;
+  vc[c] = V4<T>{v0 - vp * p0, v1 - vp * p1, v2 - vp * p2, (T)0};
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int t = 2 * c + k;
+    const int ta = t_ab[t];
+    if (ta < 0) continue;
+    const int tb = t_ab[NT + t];
+    const int eb = t_eb[t];
+    const int ed[3] = {3 * ((eb & 1) ? c : ta) + ((eb >> 1) & 3), 3 * (((eb >> 3) & 1) ? ta : tb) + ((eb >> 4) & 3),
+                       3 * (((eb >> 6) & 1) ? tb : c) + ((eb >> 7) & 3)};
+    const V4<T> pt = tc[t];
+    const T r0 = pt.x, r1 = pt.y, r2 = pt.z;
+    const T qt = q[t];
+    T g0 = (T)0, g1 = (T)0, g2 = (T)0;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const T w = tcf[l * NT + t] * (q[tnb[l * NT + t]] - qt);
+      const V4<T> pe = ec[ed[l]];
+      g0 += w * (pe.x - r0);
+      g1 += w * (pe.y - r1);
+      g2 += w * (pe.z - r2);
+    }
+    const T gp = g0 * r0 + g1 * r1 + g2 * r2;
+    qg[t] = V4<T>{g0 - gp * r0, g1 - gp * r1, g2 - gp * r2, (T)0};
+  }
+}
+
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void fast_cell(
     int remap, int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
     const int32_t *__restrict__ c_bitsB, const T *__restrict__ c_R,
     const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
     const T *__restrict__ c_dl, const T *__restrict__ c_kw, const T *__restrict__ q,
-    const T *__restrict__ kh, const T *__restrict__ cz,
+    const T *__restrict__ kh, const T *__restrict__ cz, const V4<T> *__restrict__ apv,
+    const V4<T> *__restrict__ apg, T pvsdt,
     const T *__restrict__ hs, const T *__restrict__ us, double href, T grav, T inv_a,
     T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
@@ -743,6 +819,21 @@ __global__ __launch_bounds__(BS) void fast_cell(
   qe[0] = (T)0.5 * ((hex ? qt[MX - 1] : qt[MX - 2]) + qt[0]);
 #pragma unroll
   for (int j = 1; j < MX; ++j) qe[j] = (T)0.5 * (qt[j - 1] + qt[j]);
+  if (apv) {
+    // APVM: q_e -= pvspar dt (grad q(t_{j-1}) + grad q(t_j)) . (v(c) + v(nb_j))
+    const V4<T> vs = apv[c];
+    V4<T> g[MX];
+#pragma unroll
+    for (int j = 0; j < MX; ++j) g[j] = apg[tr[j]];
+#pragma unroll
+    for (int j = 0; j < MX; ++j) {
+      const int jp = (j == 0) ? (hex ? MX - 1 : MX - 2) : j - 1;
+      const V4<T> vn = apv[nb[j]];
+      const T d = (g[jp].x + g[j].x) * (vs.x + vn.x) + (g[jp].y + g[j].y) * (vs.y + vn.y) +
+                  (g[jp].z + g[j].z) * (vs.z + vn.z);
+      qe[j] -= pvsdt * d;
+    }
+  }
   // TRiSK weights from prefix sums of the kite areas:
   // w(i,j) = S_j - S_i - 1/2 (j>i), tot - S_i + S_j - 1/2 (j<i), w(i,i) = 0
   T S[MX + 1];
@@ -1100,12 +1191,20 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       (T *)s->x_cz, (T *)s->q, s->flag)
   IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
 #undef IMX_TRI
+  if (s->f_apvm)
+    fast_apvm_pre<T, 256><<<grid_bs(nv, 256), 256, 0, st>>>(
+        s->kremap[1], nv, NE, NT, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_dl, s->x_c_iaa,
+        s->o.erad, s->x_t_ab, s->x_t_eb, s->x_ap_tnb, (const T *)s->x_ap_tcf, (const V4<T> *)s->x_ap_ec,
+        (const V4<T> *)s->x_ap_cp, (const V4<T> *)s->x_ap_tc, us, (const T *)s->q, (V4<T> *)s->x_ap_v,
+        (V4<T> *)s->x_ap_g);
   if (evs) (void)hipEventRecord(evs[1], st);
 #define IMX_CELL(B)                                                                           \
   fast_cell<T, B><<<grid_bs(nv, B), B, 0, st>>>(                                             \
       s->kremap[1], nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_R, s->x_c_iaa, \
       (const T *)s->x_c_ika, (const T *)s->x_c_dl, (const T *)s->x_c_kw, (const T *)s->q,       \
-      s->f_diff ? (const T *)s->x_kh : nullptr, (const T *)s->x_cz, hs, us, s->href,             \
+      s->f_diff ? (const T *)s->x_kh : nullptr, (const T *)s->x_cz,                              \
+      s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
+      hs, us, s->href,                                                                            \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
   IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
@@ -1412,6 +1511,70 @@ static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &
   HIPTRY(hipMemcpy(s->x_kt, ktT.data(), ktT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
   HIPTRY(hipMemcpy(s->x_c_bitsA, bitsA.data(), nv * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
   s->f_diff = true;
+  return IMX_OK;
+}
+
+// APVM on the fused kernels: the geometry fast_apvm_pre reads (see there),
+// in the ownership numbering, typed T.
+template <typename T>
+static int fast_apvm(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
+  if (!sc.apvm) {
+    s->f_apvm = false;
+    return IMX_OK;
+  }
+  const int nv = s->nv, NE = s->NE, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  const double a = s->o.erad;
+  // points as (x, y, z, 0) records
+  std::vector<double> ec((size_t)4 * NE, 0.0), cp((size_t)4 * nv, 0.0), tc((size_t)4 * NT, 0.0),
+      tcf((size_t)3 * NT, 0.0);
+  std::vector<int32_t> tnb((size_t)3 * NT, 0);
+  for (int eo = 0; eo < d.ne; ++eo)
+    for (int q = 0; q < 3; ++q) ec[(size_t)4 * en[eo] + q] = d.ed_c[3 * eo + q];
+  for (int co = 0; co < nv; ++co)
+    for (int q = 0; q < 3; ++q) cp[(size_t)4 * cn[co] + q] = d.v_p[3 * co + q];
+  for (int ko = 0; ko < d.nt; ++ko) {
+    const int t = tn[ko];
+    for (int q = 0; q < 3; ++q) tc[(size_t)4 * t + q] = d.tr_c[3 * ko + q];
+    int r = -1;  // slot m of the fused kernels is reference edge (r + m) % 3
+    for (int l = 0; l < 3; ++l)
+      if (cn[d.tr_v[3 * ko + l] - 1] == t / 2) r = l;
+    if (r < 0) return set_error(IMX_EINVAL, "imx_swm_set_scheme: triangle owner not found");
+    for (int m = 0; m < 3; ++m) {
+      const int lr = (r + m) % 3;
+      const int eo = d.tr_ed[3 * ko + lr] - 1;
+      const int s1 = d.ed_sh[2 * eo] - 1, s2 = d.ed_sh[2 * eo + 1] - 1;
+      if (s1 != ko && s2 != ko) return set_error(IMX_EINVAL, "imx_swm_set_scheme: triangle not on its edge");
+      const double *pa = d.v_p + 3 * (d.ed_v[2 * eo] - 1), *pb = d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1);
+      double nr[3];
+      cross3(pa, pb, nr);
+      double dp = 0.0;
+      for (int q = 0; q < 3; ++q) dp = dp + nr[q] * (d.tr_c[3 * s2 + q] - d.tr_c[3 * s1 + q]);
+      const double nrs = std::signbit(dp) ? -1.0 : 1.0;
+      const double side = (ko == s1) ? 1.0 : -1.0;  // q(across) - q(t) = +-(q(sh2) - q(sh1))
+      tnb[(size_t)m * NT + t] = tn[ko == s1 ? s2 : s1];
+      tcf[(size_t)m * NT + t] = -(double)d.tr_tg[3 * ko + lr] * d.ed_leng[eo] / d.tr_areag[ko] * nrs /
+                                (d.edhx_leng[eo] * a) * side;
+    }
+  }
+  Alloc &A = s->A;
+  if (!s->x_ap_tnb) {
+    s->x_ap_tnb = A.get<int32_t>((size_t)3 * NT);
+    s->x_ap_tcf = A.get<T>((size_t)3 * NT);
+    s->x_ap_ec = A.get<T>((size_t)4 * NE);
+    s->x_ap_cp = A.get<T>((size_t)4 * nv);
+    s->x_ap_tc = A.get<T>((size_t)4 * NT);
+    s->x_ap_v = A.get<T>((size_t)4 * nv);
+    s->x_ap_g = A.get<T>((size_t)4 * NT);
+    if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
+  }
+  const std::vector<T> tcfT = cvt<T>(tcf), ecT = cvt<T>(ec), cpT = cvt<T>(cp), tcT = cvt<T>(tc);
+  HIPTRY(hipMemcpy(s->x_ap_tnb, tnb.data(), tnb.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_ap_tcf, tcfT.data(), tcfT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_ap_ec, ecT.data(), ecT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_ap_cp, cpT.data(), cpT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_ap_tc, tcT.data(), tcT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  s->f_apvm = true;
   return IMX_OK;
 }
 
@@ -2081,12 +2244,14 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   if (sc->apvm && (!d->ed_c || !d->tr_c || !d->v_p))
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: APVM needs ed_c, tr_c and v_p");
   if (s->fast) {
-    // the fused kernels carry K2 diffusion of the HTC TRiSK scheme
-    if (sc->stag != IMX_STAG_HTC || sc->wrap != IMX_WRAP_TRSK10 || sc->apvm || sc->k4 != 0.0)
+    // the fused kernels carry APVM and K2 diffusion of the HTC TRiSK scheme
+    if (sc->stag != IMX_STAG_HTC || sc->wrap != IMX_WRAP_TRSK10 || sc->k4 != 0.0)
       return imx::set_error(IMX_ENOTSUP,
-                            "imx_swm_set_scheme: fused handles take K2 diffusion only (HTC, trsk10, "
-                            "no APVM or K4); other options run on FP64_REF handles");
-    const int rc = s->single ? imx::fast_diffusion<float>(s, *d, *sc) : imx::fast_diffusion<double>(s, *d, *sc);
+                            "imx_swm_set_scheme: fused handles take APVM and K2 diffusion only (HTC, "
+                            "trsk10, no K4); other options run on FP64_REF handles");
+    int rc = s->single ? imx::fast_diffusion<float>(s, *d, *sc) : imx::fast_diffusion<double>(s, *d, *sc);
+    if (rc) return rc;
+    rc = s->single ? imx::fast_apvm<float>(s, *d, *sc) : imx::fast_apvm<double>(s, *d, *sc);
     if (rc) return rc;
     s->sc = *sc;
     imx::drop_graph(s);

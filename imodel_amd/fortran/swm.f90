@@ -176,10 +176,10 @@ contains
     o%dt = dt
     call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
     if (scheme_options()) then
-      ! the fused fp32/fp64 kernels carry K2 diffusion of HTC trsk10 only
-      if (trim(precision) /= 'fp64_ref' .and. (wrap_code() /= 0 .or. trim(pv_stab) == 'apvm' .or. &
-          k4_max > 0._r8 .or. trim(stag) == 'HC')) then
-        write(*, '(a)') " ERROR: HC, pxt16, gass18, APVM and hyperdiffusion run on the fp64_ref path"
+      ! the fused fp32/fp64 kernels carry APVM and K2 diffusion of HTC trsk10
+      if (trim(precision) /= 'fp64_ref' .and. (wrap_code() /= 0 .or. k4_max > 0._r8 .or. &
+          trim(stag) == 'HC')) then
+        write(*, '(a)') " ERROR: HC, pxt16, gass18 and hyperdiffusion run on the fp64_ref path"
         write(*, '(a)') "        (par/swm_gpu.par)"
         stop 1
       end if

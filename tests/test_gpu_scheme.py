@@ -89,7 +89,7 @@ def test_scheme_errors():
     m = Mesh.icos(2)
     with Swm(m, precision="fp32", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="FP64_REF"):
-            s.set_scheme(pv="apvm")
+            s.set_scheme(k4=1e14)
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
@@ -124,11 +124,12 @@ def test_hc_tc2_matches_reference(golden, sfc):
     assert np.array_equal(h, r["h"]) and np.array_equal(u, r["u"])
 
 
-# K2 diffusion on the fused kernels (HTC, trsk10): the tendency against the
-# oracle, the diffusion part on its own (tendency with minus without), and
-# steps with the graph
+# APVM and K2 diffusion on the fused kernels (HTC, trsk10): the tendency
+# against the oracle, the option's part on its own (tendency with minus
+# without), and steps with the graph
 FAST_K2 = {"const": {"k2": 1e5}, "align": {"k2": 1e5, "diffus": "align"},
-           "diam": {"k2": 2e5, "diffus": "diam"}}
+           "diam": {"k2": 2e5, "diffus": "diam"}, "apvm": {"pv": "apvm", "pvspar": 0.5},
+           "apvm_k2": {"pv": "apvm", "pvspar": 0.3, "k2": 1e5, "diffus": "align"}}
 
 
 @pytest.mark.parametrize("name", sorted(FAST_K2))
@@ -154,12 +155,13 @@ def test_fast_k2_tendency_vs_oracle(name):
             assert rel_l2(dg, do) < tol_d, (prec, rel_l2(dg, do))
 
 
-def test_fast_k2_steps_vs_oracle():
+@pytest.mark.parametrize("name", ["align", "apvm_k2"])
+def test_fast_k2_steps_vs_oracle(name):
     from tests.conftest import rel_l2
-    say("fast k2 steps fp64 / fp32")
+    say("fast scheme steps fp64 / fp32", name)
     m = Mesh.icos(4)
     h, u = m.spectral(22)
-    sc = dict(FAST_K2["align"], dt=900.0)
+    sc = dict(FAST_K2[name], dt=900.0)
     ho, uo = Oracle(m.arrays, scheme=sc).step(h, u, 900.0, 10)
     A = m.arrays["hx_areag"]
     for prec, tol_h, tol_u in (("fp64", 1e-12, 1e-11), ("fp32", 1e-7, 1e-5)):
@@ -186,6 +188,6 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"pv": "apvm"}, {"k4": 1e14}, {"wrap": "pxt16", "k2": 1e5}, {"stag": "HC"}):
-            with pytest.raises(capi.ImxError, match="fused handles take K2 diffusion only"):
+        for kw in ({"k4": 1e14}, {"wrap": "pxt16", "k2": 1e5}, {"stag": "HC"}, {"wrap": "gass18", "pv": "apvm"}):
+            with pytest.raises(capi.ImxError, match="fused handles take APVM and K2 diffusion only"):
                 s.set_scheme(**kw)
