@@ -82,13 +82,14 @@ struct imx_swm {
   // triangles (per stage), typed T
   bool f_diff = false;
   void *x_kh = nullptr, *x_kt = nullptr, *x_cz = nullptr;
-  // fused handles with APVM: edge midpoints, cell points and circumcentres
-  // as (x, y, z, 0) records [N][4], per triangle slot the triangle across
-  // the edge and the grad q factor [3][NT], and per stage the Perot velocity
-  // at cells and grad q at triangles [N][4] (typed T)
+  // fused handles with APVM: per cell slot and per triangle slot the Perot
+  // weight with the projection folded in, as (x, y, z, 0) records ([MX][nv],
+  // [3][NT]), per triangle slot the triangle across the edge (-1 padding),
+  // and per stage the velocity at cells and grad q at triangles [N][4]
+  // (typed T)
   bool f_apvm = false;
   int32_t *x_ap_tnb = nullptr;
-  void *x_ap_tcf = nullptr, *x_ap_ec = nullptr, *x_ap_cp = nullptr, *x_ap_tc = nullptr;
+  void *x_ap_wc = nullptr, *x_ap_wt = nullptr;
   void *x_ap_v = nullptr, *x_ap_g = nullptr;
   int32_t *e_v = nullptr, *e_sh = nullptr;
   int8_t *e_loc = nullptr;

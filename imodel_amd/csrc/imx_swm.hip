@@ -677,68 +677,43 @@ struct alignas(4 * sizeof(T)) V4 {
 // APVM on the fused path, one thread per cell after fast_tri: the Perot
 // velocity at the cell (vector_edhx2hx_perot) and, for the cell's owned
 // triangles, the Perot reconstruction of grad q (grad_tr2ed then
-// vector_edtr2tr_perot), each projected on the tangent plane. tcf folds
-// -tr%tg |e| / A_t and ed%nr-sign / (|e*| a) into one factor per slot, with
-// the sign of q(across) - q(t) against q(sh2) - q(sh1).
+// vector_edtr2tr_perot). Both are linear in u and in q differences, and the
+// tangent-plane projection is linear too, so the host folds geometry, signs,
+// lengths, areas and the projection into one vector weight per cell slot
+// (wc) and per triangle slot (wt): v = sum_j wc_j u_j and
+// grad q = sum_l wt_l (q(across l) - q(t)). tnb < 0 marks padding triangles.
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void fast_apvm_pre(
-    int remap, int nv, int NE, int NT, const int32_t *__restrict__ c_nb,
-    const int32_t *__restrict__ c_bitsA, const int32_t *__restrict__ c_bitsB,
-    const T *__restrict__ c_dl, const double *__restrict__ c_iaa, double a,
-    const int32_t *__restrict__ t_ab, const int32_t *__restrict__ t_eb, const int32_t *__restrict__ tnb,
-    const T *__restrict__ tcf, const V4<T> *__restrict__ ec, const V4<T> *__restrict__ cp,
-    const V4<T> *__restrict__ tc, const T *__restrict__ us, const T *__restrict__ q, V4<T> *__restrict__ vc,
+    int remap, int nv, int NT, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
+    const int32_t *__restrict__ c_bitsB, const V4<T> *__restrict__ wc, const int32_t *__restrict__ tnb,
+    const V4<T> *__restrict__ wt, const T *__restrict__ us, const T *__restrict__ q, V4<T> *__restrict__ vc,
     V4<T> *__restrict__ qg) {
   const int c = ent_index<BS>(remap);
   if (c >= nv) return;
   const int A = c_bitsA[c];
   const int B = c_bitsB[c];
-  const bool hex = ((A & 7) == MX);
-  const V4<T> pc = cp[c];
-  const T p0 = pc.x, p1 = pc.y, p2 = pc.z;
   T v0 = (T)0, v1 = (T)0, v2 = (T)0;
 #pragma unroll
-  for (int j = 0; j < MX; ++j) {
-    if (j == MX - 1 && !hex) break;
+  for (int j = 0; j < MX; ++j) {  // a pentagon's padded slot has a zero weight
     const int nbj = c_nb[j * nv + c];
     const int eo = ((A >> (BA_OWN + j)) & 1) ? c : nbj;
-    const int e = 3 * eo + ((B >> (2 * j)) & 3);
-    T w = us[e] * c_dl[j * nv + c];
-    if ((A >> (BA_TTG + j)) & 1) w = -w;
-    const V4<T> pe = ec[e];
-    v0 += w * (pe.x - p0);
-    v1 += w * (pe.y - p1);
-    v2 += w * (pe.z - p2);
+    const T uj = us[3 * eo + ((B >> (2 * j)) & 3)];
+    const V4<T> w = wc[j * nv + c];
+    v0 += w.x * uj;
+    v1 += w.y * uj;
+    v2 += w.z * uj;
   }
-  const T sc = (T)(c_iaa[c] * a);
-  v0 *= sc;
-  v1 *= sc;
-  v2 *= sc;
-  T vp = v0 * p0 + v1 * p1 + v2 * p2;
-  vc[c] = V4<T>{v0 - vp * p0, v1 - vp * p1, v2 - vp * p2, (T)0};
+  vc[c] = V4<T>{v0, v1, v2, (T)0};
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int t = 2 * c + k;
-    const int ta = t_ab[t];
-    if (ta < 0) continue;
-    const int tb = t_ab[NT + t];
-    const int eb = t_eb[t];
-    const int ed[3] = {3 * ((eb & 1) ? c : ta) + ((eb >> 1) & 3), 3 * (((eb >> 3) & 1) ? ta : tb) + ((eb >> 4) & 3),
-                       3 * (((eb >> 6) & 1) ? tb : c) + ((eb >> 7) & 3)};
-    const V4<T> pt = tc[t];
-    const T r0 = pt.x, r1 = pt.y, r2 = pt.z;
+    const int n0 = tnb[t];
+    if (n0 < 0) continue;
     const T qt = q[t];
-    T g0 = (T)0, g1 = (T)0, g2 = (T)0;
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      const T w = tcf[l * NT + t] * (q[tnb[l * NT + t]] - qt);
-      const V4<T> pe = ec[ed[l]];
-      g0 += w * (pe.x - r0);
-      g1 += w * (pe.y - r1);
-      g2 += w * (pe.z - r2);
-    }
-    const T gp = g0 * r0 + g1 * r1 + g2 * r2;
-    qg[t] = V4<T>{g0 - gp * r0, g1 - gp * r1, g2 - gp * r2, (T)0};
+    const T d0 = q[n0] - qt, d1 = q[tnb[NT + t]] - qt, d2 = q[tnb[2 * NT + t]] - qt;
+    const V4<T> a0 = wt[t], a1 = wt[NT + t], a2 = wt[2 * NT + t];
+    qg[t] = V4<T>{a0.x * d0 + a1.x * d1 + a2.x * d2, a0.y * d0 + a1.y * d1 + a2.y * d2,
+                  a0.z * d0 + a1.z * d1 + a2.z * d2, (T)0};
   }
 }
 
@@ -1193,10 +1168,8 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
 #undef IMX_TRI
   if (s->f_apvm)
     fast_apvm_pre<T, 256><<<grid_bs(nv, 256), 256, 0, st>>>(
-        s->kremap[1], nv, NE, NT, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_dl, s->x_c_iaa,
-        s->o.erad, s->x_t_ab, s->x_t_eb, s->x_ap_tnb, (const T *)s->x_ap_tcf, (const V4<T> *)s->x_ap_ec,
-        (const V4<T> *)s->x_ap_cp, (const V4<T> *)s->x_ap_tc, us, (const T *)s->q, (V4<T> *)s->x_ap_v,
-        (V4<T> *)s->x_ap_g);
+        s->kremap[1], nv, NT, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const V4<T> *)s->x_ap_wc, s->x_ap_tnb,
+        (const V4<T> *)s->x_ap_wt, us, (const T *)s->q, (V4<T> *)s->x_ap_v, (V4<T> *)s->x_ap_g);
   if (evs) (void)hipEventRecord(evs[1], st);
 #define IMX_CELL(B)                                                                           \
   fast_cell<T, B><<<grid_bs(nv, B), B, 0, st>>>(                                             \
@@ -1522,20 +1495,30 @@ static int fast_apvm(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
     s->f_apvm = false;
     return IMX_OK;
   }
-  const int nv = s->nv, NE = s->NE, NT = s->NT;
-  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  const int nv = s->nv, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &tn = s->lay.tri_new;
   const double a = s->o.erad;
-  // points as (x, y, z, 0) records
-  std::vector<double> ec((size_t)4 * NE, 0.0), cp((size_t)4 * nv, 0.0), tc((size_t)4 * NT, 0.0),
-      tcf((size_t)3 * NT, 0.0);
-  std::vector<int32_t> tnb((size_t)3 * NT, 0);
-  for (int eo = 0; eo < d.ne; ++eo)
-    for (int q = 0; q < 3; ++q) ec[(size_t)4 * en[eo] + q] = d.ed_c[3 * eo + q];
-  for (int co = 0; co < nv; ++co)
-    for (int q = 0; q < 3; ++q) cp[(size_t)4 * cn[co] + q] = d.v_p[3 * co + q];
+  // weights as (x, y, z, 0) records: wc [MX][nv], wt [3][NT]
+  std::vector<double> wc((size_t)4 * MX * nv, 0.0), wt((size_t)4 * 3 * NT, 0.0);
+  std::vector<int32_t> tnb((size_t)3 * NT, -1);
+  auto put_projected = [](double *dst, double w[3], const double *p) {
+    const double wp = w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
+    for (int q = 0; q < 3; ++q) dst[q] = w[q] - wp * p[q];
+  };
+  for (int co = 0; co < nv; ++co) {
+    const double *p = d.v_p + 3 * co;
+    for (int j = 0; j < d.v_nnb[co]; ++j) {
+      const int eo = d.v_ed[MX * co + j] - 1;
+      // ttgout u (ed%c - v%p) edhx%leng / hx%areag (swm_operators.f90:474-505)
+      const double f = (double)d.hx_ttgout[MX * co + j] * d.edhx_leng[eo] / d.hx_areag[co];
+      double w[3];
+      for (int q = 0; q < 3; ++q) w[q] = f * (d.ed_c[3 * eo + q] - p[q]);
+      put_projected(&wc[4 * ((size_t)j * nv + cn[co])], w, p);
+    }
+  }
   for (int ko = 0; ko < d.nt; ++ko) {
     const int t = tn[ko];
-    for (int q = 0; q < 3; ++q) tc[(size_t)4 * t + q] = d.tr_c[3 * ko + q];
+    const double *p = d.tr_c + 3 * ko;
     int r = -1;  // slot m of the fused kernels is reference edge (r + m) % 3
     for (int l = 0; l < 3; ++l)
       if (cn[d.tr_v[3 * ko + l] - 1] == t / 2) r = l;
@@ -1553,27 +1536,28 @@ static int fast_apvm(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
       const double nrs = std::signbit(dp) ? -1.0 : 1.0;
       const double side = (ko == s1) ? 1.0 : -1.0;  // q(across) - q(t) = +-(q(sh2) - q(sh1))
       tnb[(size_t)m * NT + t] = tn[ko == s1 ? s2 : s1];
-      tcf[(size_t)m * NT + t] = -(double)d.tr_tg[3 * ko + lr] * d.ed_leng[eo] / d.tr_areag[ko] * nrs /
-                                (d.edhx_leng[eo] * a) * side;
+      // grad q at the edge: ed%nr-sign (q(sh2) - q(sh1)) / (|e*| a); its Perot
+      // term: -tr%tg |e| (ed%c - tr%c) / tr%areag (swm_operators.f90:1427-1462)
+      const double f = -(double)d.tr_tg[3 * ko + lr] * d.ed_leng[eo] / d.tr_areag[ko] * nrs /
+                       (d.edhx_leng[eo] * a) * side;
+      double w[3];
+      for (int q = 0; q < 3; ++q) w[q] = f * (d.ed_c[3 * eo + q] - p[q]);
+      put_projected(&wt[4 * ((size_t)m * NT + t)], w, p);
     }
   }
   Alloc &A = s->A;
   if (!s->x_ap_tnb) {
     s->x_ap_tnb = A.get<int32_t>((size_t)3 * NT);
-    s->x_ap_tcf = A.get<T>((size_t)3 * NT);
-    s->x_ap_ec = A.get<T>((size_t)4 * NE);
-    s->x_ap_cp = A.get<T>((size_t)4 * nv);
-    s->x_ap_tc = A.get<T>((size_t)4 * NT);
+    s->x_ap_wc = A.get<T>((size_t)4 * MX * nv);
+    s->x_ap_wt = A.get<T>((size_t)4 * 3 * NT);
     s->x_ap_v = A.get<T>((size_t)4 * nv);
     s->x_ap_g = A.get<T>((size_t)4 * NT);
     if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
   }
-  const std::vector<T> tcfT = cvt<T>(tcf), ecT = cvt<T>(ec), cpT = cvt<T>(cp), tcT = cvt<T>(tc);
+  const std::vector<T> wcT = cvt<T>(wc), wtT = cvt<T>(wt);
   HIPTRY(hipMemcpy(s->x_ap_tnb, tnb.data(), tnb.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
-  HIPTRY(hipMemcpy(s->x_ap_tcf, tcfT.data(), tcfT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
-  HIPTRY(hipMemcpy(s->x_ap_ec, ecT.data(), ecT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
-  HIPTRY(hipMemcpy(s->x_ap_cp, cpT.data(), cpT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
-  HIPTRY(hipMemcpy(s->x_ap_tc, tcT.data(), tcT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_ap_wc, wcT.data(), wcT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_ap_wt, wtT.data(), wtT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
   s->f_apvm = true;
   return IMX_OK;
 }
