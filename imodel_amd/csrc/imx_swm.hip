@@ -1487,6 +1487,145 @@ static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &
   return IMX_OK;
 }
 
+// HC sign tables and Voronoi-midpoint points (the REF kernels' HC branches
+// and, for fused handles, the diagnostics), built from the geometry.
+static int hc_tables(imx_swm *s, const imx_mesh_desc *d, int wrap) {
+  const int nv = s->nv, NE = s->NE, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  // HC signs from the geometry: s_e = sign(ed%tg . edhx%nr) turns the
+  // HTC indicators into hx%nr = ttgout s_e and the vorticity signs
+  // tr%tg s_e (smeshpack.f90:1085-1116); grad_ed and grad_ed_tg orient
+  // by edhx%nr and edhx%tg (swm_operators.f90:531-534, :755-760)
+  std::vector<int8_t> sgn(d->ne), hcc((size_t)MX * nv, 1), hct((size_t)3 * NT, 1), hg(NE, 1), htt(NE, 1);
+  std::vector<double> pc((size_t)3 * NE, 0.0), tgv((size_t)3 * NE, 0.0), hb((size_t)3 * NE, 0.0);
+  std::vector<int32_t> hv((size_t)3 * NE, 0);
+  for (int eo = 0; eo < d->ne; ++eo) {
+    const int e = en[eo];
+    const double *v1 = d->v_p + 3 * (d->ed_v[2 * eo] - 1), *v2 = d->v_p + 3 * (d->ed_v[2 * eo + 1] - 1);
+    const double *t1 = d->tr_c + 3 * (d->ed_sh[2 * eo] - 1), *t2 = d->tr_c + 3 * (d->ed_sh[2 * eo + 1] - 1);
+    const double *hxc = d->edhx_c + 3 * eo;
+    double enr[3], etg[3], hnr[3], htg[3];
+    cross3(v1, v2, enr);
+    double n = norm3(enr);
+    for (int q = 0; q < 3; ++q) enr[q] = enr[q] / n;
+    cross3(enr, d->ed_c + 3 * eo, etg);
+    cross3(t1, t2, hnr);
+    n = norm3(hnr);
+    for (int q = 0; q < 3; ++q) hnr[q] = hnr[q] / n;
+    cross3(hnr, hxc, htg);
+    n = norm3(htg);
+    for (int q = 0; q < 3; ++q) htg[q] = htg[q] / n;
+    sgn[eo] = std::signbit(dot3(etg, hnr)) ? -1 : 1;
+    double dv[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+    double dt3[3] = {t2[0] - t1[0], t2[1] - t1[1], t2[2] - t1[2]};
+    hg[e] = std::signbit(dot3(hnr, dv)) ? -1 : 1;
+    htt[e] = std::signbit(dot3(htg, dt3)) ? -1 : 1;
+    for (int q = 0; q < 3; ++q) {
+      pc[(size_t)q * NE + e] = hxc[q];
+      tgv[(size_t)q * NE + e] = htg[q];
+    }
+    if (wrap == IMX_WRAP_PXT16) {  // h at the dual edge midpoint (initialize_gridprop, swm.f90:1007-1018)
+      const int kt = insidetrmesh(*d, hxc, d->ed_sh[2 * eo] - 1) ? d->ed_sh[2 * eo] - 1 : d->ed_sh[2 * eo + 1] - 1;
+      double b[3];
+      bary_at(*d, hxc, kt, b);
+      for (int i = 0; i < 3; ++i) {
+        hb[(size_t)i * NE + e] = b[i];
+        hv[(size_t)i * NE + e] = cn[d->tr_v[3 * kt + i] - 1];
+      }
+    }
+  }
+  for (int co = 0; co < nv; ++co)
+    for (int j = 0; j < d->v_nnb[co]; ++j)
+      hcc[(size_t)j * nv + cn[co]] = (int8_t)((d->hx_ttgout[MX * co + j] < 0 ? -1 : 1) * sgn[d->v_ed[MX * co + j] - 1]);
+  for (int ko = 0; ko < d->nt; ++ko)
+    for (int l = 0; l < 3; ++l)
+      hct[(size_t)l * NT + tn[ko]] = (int8_t)((d->tr_tg[3 * ko + l] < 0 ? -1 : 1) * sgn[d->tr_ed[3 * ko + l] - 1]);
+  // the HC sign of each TRiSK term: hx%nr(j) hx%nr(ei) of its cell
+  std::vector<int8_t> hts((size_t)2 * MX * NE, 1);
+  for (int eo = 0; eo < d->ne; ++eo)
+    for (int ic = 0; ic < 2; ++ic) {
+      const int co = d->ed_v[2 * eo + ic] - 1, c = cn[co];
+      int ei = 0;
+      for (int j = 0; j < d->v_nnb[co]; ++j)
+        if (d->v_ed[MX * co + j] == eo + 1) ei = j;
+      for (int j = 0; j < d->v_nnb[co]; ++j)
+        hts[((size_t)ic * MX + j) * NE + en[eo]] = (int8_t)(hcc[(size_t)j * nv + c] * hcc[(size_t)ei * nv + c]);
+    }
+  if (s->hc_ts)  // the REF TRiSK term table (fused handles have none)
+    HIPTRY(hipMemcpy(s->hc_ts, hts.data(), hts.size(), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->hc_c, hcc.data(), hcc.size(), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->hc_t, hct.data(), hct.size(), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->hc_g, hg.data(), hg.size(), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->hc_tt, htt.data(), htt.size(), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->hc_pc, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->hc_hb, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->hc_hv, hv.data(), hv.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+  // pered on HC takes the edhx%tg component
+  if (wrap == IMX_WRAP_PXT16)
+    HIPTRY(hipMemcpy(s->e_nr, tgv.data(), tgv.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+  return IMX_OK;
+}
+
+// The staggering's signs in the fused tables: the flux and TRiSK sign bits
+// of each cell slot (hx%ttgout / hx%tnrccw with HTC; hx%nr = ttgout s_e for
+// both with HC, whose TRiSK term sign is hx%nr(j) hx%nr(i)), the vorticity
+// coefficients (tr%tg, or tr%tg s_e), and the orientation of each edge's
+// potential difference (HC: sign of edhx%nr . (v2 - v1), applied by swapping
+// the edge's end cells). HC reads the tables hc_tables built.
+template <typename T>
+static int fast_stagger(imx_swm *s, const imx_mesh_desc &d, bool hc) {
+  const int nv = s->nv, NE = s->NE, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  const double a = s->o.erad;
+  std::vector<int8_t> hcc, hct, hg;
+  if (hc) {
+    hcc.resize((size_t)MX * nv);
+    hct.resize((size_t)3 * NT);
+    hg.resize(NE);
+    HIPTRY(hipMemcpy(hcc.data(), s->hc_c, hcc.size(), hipMemcpyDeviceToHost), "set_scheme");
+    HIPTRY(hipMemcpy(hct.data(), s->hc_t, hct.size(), hipMemcpyDeviceToHost), "set_scheme");
+    HIPTRY(hipMemcpy(hg.data(), s->hc_g, hg.size(), hipMemcpyDeviceToHost), "set_scheme");
+  }
+  std::vector<int32_t> bitsA(nv), e_oth(NE, -1);
+  HIPTRY(hipMemcpy(bitsA.data(), s->x_c_bitsA, nv * sizeof(int32_t), hipMemcpyDeviceToHost), "set_scheme");
+  for (int co = 0; co < nv; ++co) {
+    const int c = cn[co];
+    bitsA[c] &= ~((((1 << MX) - 1) << BA_TTG) | (((1 << MX) - 1) << BA_TNR));
+    for (int j = 0; j < d.v_nnb[co]; ++j) {
+      const int sf = hc ? hcc[(size_t)j * nv + c] : d.hx_ttgout[MX * co + j];
+      const int sn = hc ? hcc[(size_t)j * nv + c] : d.hx_tnrccw[MX * co + j];
+      if (sf < 0) bitsA[c] |= 1 << (BA_TTG + j);
+      if (sn < 0) bitsA[c] |= 1 << (BA_TNR + j);
+    }
+  }
+  for (int eo = 0; eo < d.ne; ++eo) {
+    const int e = en[eo];
+    const int o = e / 3;
+    int v1 = cn[d.ed_v[2 * eo] - 1], v2 = cn[d.ed_v[2 * eo + 1] - 1];
+    if (hc && hg[e] < 0) std::swap(v1, v2);
+    e_oth[e] = (o == v1) ? v2 : (v1 | (1 << OTH_V2));
+  }
+  std::vector<double> t_ceta((size_t)3 * NT, 0.0);
+  for (int ko = 0; ko < d.nt; ++ko) {
+    const int t = tn[ko];
+    int r = -1;
+    for (int l = 0; l < 3; ++l)
+      if (cn[d.tr_v[3 * ko + l] - 1] == t / 2) r = l;
+    if (r < 0) return set_error(IMX_EINVAL, "imx_swm_set_scheme: triangle owner not found");
+    for (int m = 0; m < 3; ++m) {
+      const int lr = (r + m) % 3;
+      const int eo = d.tr_ed[3 * ko + lr] - 1;
+      const double sg = hc ? (double)hct[(size_t)lr * NT + t] : (double)d.tr_tg[3 * ko + lr];
+      t_ceta[(size_t)m * NT + t] = d.ed_leng[eo] * sg / (d.tr_areag[ko] * a);
+    }
+  }
+  const std::vector<T> tcT = cvt<T>(t_ceta);
+  HIPTRY(hipMemcpy(s->x_c_bitsA, bitsA.data(), nv * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_e_oth, e_oth.data(), NE * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_t_ceta, tcT.data(), tcT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  return IMX_OK;
+}
+
 // APVM on the fused kernels: the geometry fast_apvm_pre reads (see there),
 // in the ownership numbering, typed T.
 template <typename T>
@@ -2228,12 +2367,19 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   if (sc->apvm && (!d->ed_c || !d->tr_c || !d->v_p))
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: APVM needs ed_c, tr_c and v_p");
   if (s->fast) {
-    // the fused kernels carry APVM and K2 diffusion of the HTC TRiSK scheme
-    if (sc->stag != IMX_STAG_HTC || sc->wrap != IMX_WRAP_TRSK10 || sc->k4 != 0.0)
+    // the fused kernels carry the trsk10 scheme with HTC (with APVM and K2
+    // diffusion) or HC staggering
+    const bool hc = sc->stag == IMX_STAG_HC;
+    if (sc->wrap != IMX_WRAP_TRSK10 || sc->k4 != 0.0 || (hc && (sc->apvm || sc->k2 != 0.0)))
       return imx::set_error(IMX_ENOTSUP,
-                            "imx_swm_set_scheme: fused handles take APVM and K2 diffusion only (HTC, "
-                            "trsk10, no K4); other options run on FP64_REF handles");
-    int rc = s->single ? imx::fast_diffusion<float>(s, *d, *sc) : imx::fast_diffusion<double>(s, *d, *sc);
+                            "imx_swm_set_scheme: fused handles take trsk10 with HTC (APVM and K2 diffusion "
+                            "allowed) or HC (no options); other options run on FP64_REF handles");
+    int rc;
+    if (hc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, sc->wrap)))) return rc;
+    rc = s->single ? imx::fast_stagger<float>(s, *d, hc) : imx::fast_stagger<double>(s, *d, hc);
+    if (rc) return rc;
+    s->sc_hc = hc;
+    rc = s->single ? imx::fast_diffusion<float>(s, *d, *sc) : imx::fast_diffusion<double>(s, *d, *sc);
     if (rc) return rc;
     rc = s->single ? imx::fast_apvm<float>(s, *d, *sc) : imx::fast_apvm<double>(s, *d, *sc);
     if (rc) return rc;
@@ -2249,76 +2395,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     if (rc) return rc;
   }
   if (sc->stag == IMX_STAG_HC) {
-    // HC signs from the geometry: s_e = sign(ed%tg . edhx%nr) turns the
-    // HTC indicators into hx%nr = ttgout s_e and the vorticity signs
-    // tr%tg s_e (smeshpack.f90:1085-1116); grad_ed and grad_ed_tg orient
-    // by edhx%nr and edhx%tg (swm_operators.f90:531-534, :755-760)
-    std::vector<int8_t> sgn(d->ne), hcc((size_t)MX * nv, 1), hct((size_t)3 * NT, 1), hg(NE, 1), htt(NE, 1);
-    std::vector<double> pc((size_t)3 * NE, 0.0), tgv((size_t)3 * NE, 0.0), hb((size_t)3 * NE, 0.0);
-    std::vector<int32_t> hv((size_t)3 * NE, 0);
-    for (int eo = 0; eo < d->ne; ++eo) {
-      const int e = en[eo];
-      const double *v1 = d->v_p + 3 * (d->ed_v[2 * eo] - 1), *v2 = d->v_p + 3 * (d->ed_v[2 * eo + 1] - 1);
-      const double *t1 = d->tr_c + 3 * (d->ed_sh[2 * eo] - 1), *t2 = d->tr_c + 3 * (d->ed_sh[2 * eo + 1] - 1);
-      const double *hxc = d->edhx_c + 3 * eo;
-      double enr[3], etg[3], hnr[3], htg[3];
-      imx::cross3(v1, v2, enr);
-      double n = imx::norm3(enr);
-      for (int q = 0; q < 3; ++q) enr[q] = enr[q] / n;
-      imx::cross3(enr, d->ed_c + 3 * eo, etg);
-      imx::cross3(t1, t2, hnr);
-      n = imx::norm3(hnr);
-      for (int q = 0; q < 3; ++q) hnr[q] = hnr[q] / n;
-      imx::cross3(hnr, hxc, htg);
-      n = imx::norm3(htg);
-      for (int q = 0; q < 3; ++q) htg[q] = htg[q] / n;
-      sgn[eo] = std::signbit(imx::dot3(etg, hnr)) ? -1 : 1;
-      double dv[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
-      double dt3[3] = {t2[0] - t1[0], t2[1] - t1[1], t2[2] - t1[2]};
-      hg[e] = std::signbit(imx::dot3(hnr, dv)) ? -1 : 1;
-      htt[e] = std::signbit(imx::dot3(htg, dt3)) ? -1 : 1;
-      for (int q = 0; q < 3; ++q) {
-        pc[(size_t)q * NE + e] = hxc[q];
-        tgv[(size_t)q * NE + e] = htg[q];
-      }
-      if (sc->wrap == IMX_WRAP_PXT16) {  // h at the dual edge midpoint (initialize_gridprop, swm.f90:1007-1018)
-        const int kt = imx::insidetrmesh(*d, hxc, d->ed_sh[2 * eo] - 1) ? d->ed_sh[2 * eo] - 1 : d->ed_sh[2 * eo + 1] - 1;
-        double b[3];
-        imx::bary_at(*d, hxc, kt, b);
-        for (int i = 0; i < 3; ++i) {
-          hb[(size_t)i * NE + e] = b[i];
-          hv[(size_t)i * NE + e] = cn[d->tr_v[3 * kt + i] - 1];
-        }
-      }
-    }
-    for (int co = 0; co < nv; ++co)
-      for (int j = 0; j < d->v_nnb[co]; ++j)
-        hcc[(size_t)j * nv + cn[co]] = (int8_t)((d->hx_ttgout[MX * co + j] < 0 ? -1 : 1) * sgn[d->v_ed[MX * co + j] - 1]);
-    for (int ko = 0; ko < d->nt; ++ko)
-      for (int l = 0; l < 3; ++l)
-        hct[(size_t)l * NT + tn[ko]] = (int8_t)((d->tr_tg[3 * ko + l] < 0 ? -1 : 1) * sgn[d->tr_ed[3 * ko + l] - 1]);
-    // the HC sign of each TRiSK term: hx%nr(j) hx%nr(ei) of its cell
-    std::vector<int8_t> hts((size_t)2 * MX * NE, 1);
-    for (int eo = 0; eo < d->ne; ++eo)
-      for (int ic = 0; ic < 2; ++ic) {
-        const int co = d->ed_v[2 * eo + ic] - 1, c = cn[co];
-        int ei = 0;
-        for (int j = 0; j < d->v_nnb[co]; ++j)
-          if (d->v_ed[MX * co + j] == eo + 1) ei = j;
-        for (int j = 0; j < d->v_nnb[co]; ++j)
-          hts[((size_t)ic * MX + j) * NE + en[eo]] = (int8_t)(hcc[(size_t)j * nv + c] * hcc[(size_t)ei * nv + c]);
-      }
-    HIPTRY(hipMemcpy(s->hc_ts, hts.data(), hts.size(), hipMemcpyHostToDevice), "set_scheme");
-    HIPTRY(hipMemcpy(s->hc_c, hcc.data(), hcc.size(), hipMemcpyHostToDevice), "set_scheme");
-    HIPTRY(hipMemcpy(s->hc_t, hct.data(), hct.size(), hipMemcpyHostToDevice), "set_scheme");
-    HIPTRY(hipMemcpy(s->hc_g, hg.data(), hg.size(), hipMemcpyHostToDevice), "set_scheme");
-    HIPTRY(hipMemcpy(s->hc_tt, htt.data(), htt.size(), hipMemcpyHostToDevice), "set_scheme");
-    HIPTRY(hipMemcpy(s->hc_pc, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
-    HIPTRY(hipMemcpy(s->hc_hb, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
-    HIPTRY(hipMemcpy(s->hc_hv, hv.data(), hv.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
-    // pered on HC takes the edhx%tg component
-    if (sc->wrap == IMX_WRAP_PXT16)
-      HIPTRY(hipMemcpy(s->e_nr, tgv.data(), tgv.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+    const int rc = imx::hc_tables(s, d, sc->wrap);
+    if (rc) return rc;
   }
   if (sc->wrap == IMX_WRAP_PXT16) {
     // barycentric coordinates of the circumcentres (initialize_gridprop,

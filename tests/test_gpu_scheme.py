@@ -188,6 +188,52 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"k4": 1e14}, {"wrap": "pxt16", "k2": 1e5}, {"stag": "HC"}, {"wrap": "gass18", "pv": "apvm"}):
-            with pytest.raises(capi.ImxError, match="fused handles take APVM and K2 diffusion only"):
+        for kw in ({"k4": 1e14}, {"wrap": "pxt16", "k2": 1e5}, {"stag": "HC", "pv": "apvm"},
+                   {"stag": "HC", "k2": 1e5}, {"wrap": "gass18", "pv": "apvm"}):
+            with pytest.raises(capi.ImxError, match="fused handles take trsk10"):
                 s.set_scheme(**kw)
+
+
+def test_fast_hc_vs_oracle():
+    """HC staggering on the fused kernels (trsk10): the tendency against the
+    oracle's HC tendency, 10 steps of TC2 initialised along edhx%nr, the
+    diagnostics against an FP64_REF HC handle on the same state, and the
+    switch back to HTC restoring the plain step bit for bit."""
+    from tests.conftest import rel_l2
+    say("fast hc")
+    m = Mesh.icos(4)
+    h, u = m.spectral(23)
+    sc = {"stag": "HC", "dt": 900.0}
+    mo, fo = Oracle(m.arrays, scheme=sc).tendency(h, u)
+    for prec, tol in (("fp64", 1e-12), ("fp32", 5e-5)):
+        with Swm(m, precision=prec) as s:
+            s.set_scheme(stag="HC")
+            mg, fg = s.tendency(h, u)
+        assert rel_l2(mg, mo) < tol, (prec, rel_l2(mg, mo))
+        assert rel_l2(fg, fo) < tol, (prec, rel_l2(fg, fo))
+    h2, u2 = m.tc2(stag="HC")
+    ho, uo = Oracle(m.arrays, scheme=sc).step(h2, u2, 900.0, 10)
+    A = m.arrays["hx_areag"]
+    with Swm(m, precision="fp64", dt=900.0) as s:
+        s.set_scheme(stag="HC")
+        s.set_state(h2, u2)
+        s.step(10)
+        hg, ug = s.get_state()
+        d = s.diagnostics()
+        s.set_scheme()
+        s.set_state(h, u)
+        s.step(3)
+        hp, up = s.get_state()
+    assert rel_l2(hg, ho, A) < 1e-12 and rel_l2(ug, uo) < 1e-11, (rel_l2(hg, ho, A), rel_l2(ug, uo))
+    with Swm(m, precision="fp64_ref", dt=900.0) as r:
+        r.set_scheme(stag="HC")
+        r.set_state(hg, ug)
+        dr = r.diagnostics()
+    for k in ("mass", "tenergy", "rmsdiv", "penstrophy"):
+        assert abs(d[k] - dr[k]) <= 1e-12 * abs(dr[k]), (k, d[k], dr[k])
+    with Swm(m, precision="fp64", dt=900.0) as p:
+        p.set_state(h2, u2)  # the same h_ref (the area mean of the first state set)
+        p.set_state(h, u)
+        p.step(3)
+        hq, uq = p.get_state()
+    assert np.array_equal(hp, hq) and np.array_equal(up, uq)
