@@ -1434,6 +1434,12 @@ static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &
   }
   const int nv = s->nv, NT = s->NT;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  // HC orients grad_ed_tg by edhx%tg (the hc_tt table) instead of ed%nr
+  std::vector<int8_t> htt;
+  if (s->sc_hc) {
+    htt.resize(s->NE);
+    HIPTRY(hipMemcpy(htt.data(), s->hc_tt, htt.size(), hipMemcpyDeviceToHost), "set_scheme");
+  }
   std::vector<double> trhx_v, hxtr_v, trskw_v;
   const double *trhx = d.tr_trhx_area;
   if (!trhx) {
@@ -1468,7 +1474,7 @@ static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &
       const double *t1 = d.tr_c + 3 * s1, *t2 = d.tr_c + 3 * s2;
       double dp = 0.0;
       for (int q = 0; q < 3; ++q) dp = dp + nr[q] * (t2[q] - t1[q]);
-      const double e = std::signbit(dp) ? -1.0 : 1.0;
+      const double e = s->sc_hc ? (double)htt[en[eo]] : (std::signbit(dp) ? -1.0 : 1.0);
       if (e * side < 0.0) bitsA[c] |= 1 << (BA_FZ + j);
     }
   }
@@ -1644,14 +1650,22 @@ static int fast_apvm(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
     const double wp = w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
     for (int q = 0; q < 3; ++q) dst[q] = w[q] - wp * p[q];
   };
+  // HC: hx%nr signs and the Voronoi edge midpoints
+  std::vector<int8_t> hcc;
+  if (s->sc_hc) {
+    hcc.resize((size_t)MX * nv);
+    HIPTRY(hipMemcpy(hcc.data(), s->hc_c, hcc.size(), hipMemcpyDeviceToHost), "set_scheme");
+  }
   for (int co = 0; co < nv; ++co) {
     const double *p = d.v_p + 3 * co;
     for (int j = 0; j < d.v_nnb[co]; ++j) {
       const int eo = d.v_ed[MX * co + j] - 1;
       // ttgout u (ed%c - v%p) edhx%leng / hx%areag (swm_operators.f90:474-505)
-      const double f = (double)d.hx_ttgout[MX * co + j] * d.edhx_leng[eo] / d.hx_areag[co];
+      const double sg = s->sc_hc ? (double)hcc[(size_t)j * nv + cn[co]] : (double)d.hx_ttgout[MX * co + j];
+      const double *pe = s->sc_hc ? d.edhx_c + 3 * eo : d.ed_c + 3 * eo;
+      const double f = sg * d.edhx_leng[eo] / d.hx_areag[co];
       double w[3];
-      for (int q = 0; q < 3; ++q) w[q] = f * (d.ed_c[3 * eo + q] - p[q]);
+      for (int q = 0; q < 3; ++q) w[q] = f * (pe[q] - p[q]);
       put_projected(&wc[4 * ((size_t)j * nv + cn[co])], w, p);
     }
   }
@@ -2370,10 +2384,10 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     // the fused kernels carry the trsk10 scheme with HTC (with APVM and K2
     // diffusion) or HC staggering
     const bool hc = sc->stag == IMX_STAG_HC;
-    if (sc->wrap != IMX_WRAP_TRSK10 || sc->k4 != 0.0 || (hc && (sc->apvm || sc->k2 != 0.0)))
+    if (sc->wrap != IMX_WRAP_TRSK10 || sc->k4 != 0.0)
       return imx::set_error(IMX_ENOTSUP,
-                            "imx_swm_set_scheme: fused handles take trsk10 with HTC (APVM and K2 diffusion "
-                            "allowed) or HC (no options); other options run on FP64_REF handles");
+                            "imx_swm_set_scheme: fused handles take trsk10 with HTC or HC, APVM and K2 "
+                            "diffusion; other options run on FP64_REF handles");
     int rc;
     if (hc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, sc->wrap)))) return rc;
     rc = s->single ? imx::fast_stagger<float>(s, *d, hc) : imx::fast_stagger<double>(s, *d, hc);

@@ -176,11 +176,10 @@ contains
     o%dt = dt
     call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
     if (scheme_options()) then
-      ! the fused fp32/fp64 kernels carry trsk10 with HTC (APVM and K2
-      ! diffusion allowed) or HC (no options)
-      if (trim(precision) /= 'fp64_ref' .and. (wrap_code() /= 0 .or. k4_max > 0._r8 .or. &
-          (trim(stag) == 'HC' .and. (trim(pv_stab) == 'apvm' .or. k2_max > 0._r8)))) then
-        write(*, '(a)') " ERROR: pxt16, gass18, hyperdiffusion and HC with options run on the fp64_ref path"
+      ! the fused fp32/fp64 kernels carry trsk10 with HTC or HC, APVM and K2
+      ! diffusion
+      if (trim(precision) /= 'fp64_ref' .and. (wrap_code() /= 0 .or. k4_max > 0._r8)) then
+        write(*, '(a)') " ERROR: pxt16, gass18 and hyperdiffusion run on the fp64_ref path"
         write(*, '(a)') "        (par/swm_gpu.par)"
         stop 1
       end if

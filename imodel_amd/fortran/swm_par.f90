@@ -3,10 +3,9 @@
 !   par/mesh.par  as getparameters  (src/smeshpack.f90:7343-7416), argv
 !                 [meshpar] [nv] override as in the reference
 !   par/swm.par   as swmpars        (src/swm_data.f90:338-693), the
-!                 geo-area subset this build runs: trsk10 with HTC (APVM
-!                 and K2 diffusion allowed) or HC (TC2) on every path; HC
-!                 with options, the pxt16 and gass18 wrappers and
-!                 hyperdiffusion on the fp64_ref path
+!                 geo-area subset this build runs: trsk10 with HTC or HC
+!                 (TC2), APVM and K2 diffusion on every path; the pxt16 and
+!                 gass18 wrappers and hyperdiffusion on the fp64_ref path
 !   par/swm_gpu.par (new, optional): precision, seed, SFC and graph switches
 !=======================================================================
 module swm_par

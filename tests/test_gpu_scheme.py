@@ -129,7 +129,8 @@ def test_hc_tc2_matches_reference(golden, sfc):
 # without), and steps with the graph
 FAST_K2 = {"const": {"k2": 1e5}, "align": {"k2": 1e5, "diffus": "align"},
            "diam": {"k2": 2e5, "diffus": "diam"}, "apvm": {"pv": "apvm", "pvspar": 0.5},
-           "apvm_k2": {"pv": "apvm", "pvspar": 0.3, "k2": 1e5, "diffus": "align"}}
+           "apvm_k2": {"pv": "apvm", "pvspar": 0.3, "k2": 1e5, "diffus": "align"},
+           "hc_apvm_k2": {"stag": "HC", "pv": "apvm", "pvspar": 0.4, "k2": 1e5, "diffus": "diam"}}
 
 
 @pytest.mark.parametrize("name", sorted(FAST_K2))
@@ -155,7 +156,7 @@ def test_fast_k2_tendency_vs_oracle(name):
             assert rel_l2(dg, do) < tol_d, (prec, rel_l2(dg, do))
 
 
-@pytest.mark.parametrize("name", ["align", "apvm_k2"])
+@pytest.mark.parametrize("name", ["align", "apvm_k2", "hc_apvm_k2"])
 def test_fast_k2_steps_vs_oracle(name):
     from tests.conftest import rel_l2
     say("fast scheme steps fp64 / fp32", name)
@@ -188,8 +189,8 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"k4": 1e14}, {"wrap": "pxt16", "k2": 1e5}, {"stag": "HC", "pv": "apvm"},
-                   {"stag": "HC", "k2": 1e5}, {"wrap": "gass18", "pv": "apvm"}):
+        for kw in ({"k4": 1e14}, {"wrap": "pxt16", "k2": 1e5}, {"stag": "HC", "k4": 1e14},
+                   {"wrap": "gass18", "pv": "apvm"}):
             with pytest.raises(capi.ImxError, match="fused handles take trsk10"):
                 s.set_scheme(**kw)
 
