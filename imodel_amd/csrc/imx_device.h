@@ -82,6 +82,13 @@ struct imx_swm {
   // triangles (per stage), typed T
   bool f_diff = false;
   void *x_kh = nullptr, *x_kt = nullptr, *x_cz = nullptr;
+  // fused handles with hyperdiffusion: per edge its triangles [2][NE] and the
+  // factors 1/(|e| a) and s/(|e*| a); per stage K div u, lapu, K div lapu and
+  // K curl lapu (typed T)
+  bool f_hyper = false;
+  int32_t *x_hy_esh = nullptr;
+  void *x_hy_ea = nullptr, *x_hy_eb = nullptr, *x_hy_kd = nullptr, *x_hy_lap = nullptr, *x_hy_d = nullptr,
+       *x_hy_z = nullptr;
   // fused handles with APVM: per cell slot and per triangle slot the Perot
   // weight with the projection folded in, as (x, y, z, 0) records ([MX][nv],
   // [3][NT]), per triangle slot the triangle across the edge (-1 padding),

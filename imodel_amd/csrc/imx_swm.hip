@@ -717,13 +717,103 @@ __global__ __launch_bounds__(BS) void fast_apvm_pre(
   }
 }
 
+// Hyperdiffusion on the fused path: lap_lapu = diffusion_ed of the
+// divergence and vorticity of lapu = diffusion_ed(div u, zeta)
+// (swm_operators.f90:897-975), in three passes after fast_tri (which holds
+// K zeta in cz). hyper_a: Kd = K div u at cells.
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void fast_hyper_a(int remap, int nv, const int32_t *__restrict__ c_nb,
+                                                    const int32_t *__restrict__ c_bitsA,
+                                                    const int32_t *__restrict__ c_bitsB,
+                                                    const T *__restrict__ c_dl, const double *__restrict__ c_iaa,
+                                                    const T *__restrict__ kh, const T *__restrict__ f,
+                                                    T *__restrict__ out) {
+  const int c = ent_index<BS>(remap);
+  if (c >= nv) return;
+  const int A = c_bitsA[c];
+  const int B = c_bitsB[c];
+  const bool hex = ((A & 7) == MX);
+  double du = 0.0;
+#pragma unroll
+  for (int j = 0; j < MX; ++j) {
+    const int eo = ((A >> (BA_OWN + j)) & 1) ? c : c_nb[j * nv + c];
+    double g = (double)f[3 * eo + ((B >> (2 * j)) & 3)] * (double)c_dl[j * nv + c];
+    if ((A >> (BA_TTG + j)) & 1) g = -g;
+    if ((j < MX - 1) || hex) du += g;
+  }
+  out[c] = kh[c] * (T)(du * c_iaa[c]);
+}
+
+// hyper_b: lapu at each edge, grad(Kd) - grad_tg(K zeta):
+// (Kd(v2) - Kd(v1)) / (|e| a) - s (Kz(sh2) - Kz(sh1)) / (|e*| a), with the
+// end cells in the edge's gradient orientation (e_oth) and s folded into eb.
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void fast_hyper_b(int remap, int NE, const int32_t *__restrict__ e_oth,
+                                                    const int32_t *__restrict__ e_sh, const T *__restrict__ ea,
+                                                    const T *__restrict__ eb, const T *__restrict__ kd,
+                                                    const T *__restrict__ cz, T *__restrict__ lap) {
+  const int e = ent_index<BS>(remap);
+  if (e >= NE) return;
+  const int oth = e_oth[e];
+  if (oth < 0) return;
+  const int o = e / 3;
+  const int c2 = oth & OTH_MASK;
+  const bool own_v2 = (oth >> OTH_V2) & 1;
+  const int v1 = own_v2 ? c2 : o, v2 = own_v2 ? o : c2;
+  lap[e] = (kd[v2] - kd[v1]) * ea[e] - (cz[e_sh[NE + e]] - cz[e_sh[e]]) * eb[e];
+}
+
+// hyper_c: K div(lapu) at cells and, for the cell's owned triangles,
+// K curl(lapu) (the vorticity coefficients of fast_tri)
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void fast_hyper_c(int remap, int nv, int NT, const int32_t *__restrict__ c_nb,
+                                                    const int32_t *__restrict__ c_bitsA,
+                                                    const int32_t *__restrict__ c_bitsB,
+                                                    const T *__restrict__ c_dl, const double *__restrict__ c_iaa,
+                                                    const T *__restrict__ kh, const T *__restrict__ kt,
+                                                    const int32_t *__restrict__ t_ab,
+                                                    const int32_t *__restrict__ t_eb,
+                                                    const T *__restrict__ t_ceta, const T *__restrict__ lap,
+                                                    T *__restrict__ hd, T *__restrict__ hz) {
+  const int c = ent_index<BS>(remap);
+  if (c >= nv) return;
+  const int A = c_bitsA[c];
+  const int B = c_bitsB[c];
+  const bool hex = ((A & 7) == MX);
+  double du = 0.0;
+#pragma unroll
+  for (int j = 0; j < MX; ++j) {
+    const int eo = ((A >> (BA_OWN + j)) & 1) ? c : c_nb[j * nv + c];
+    double g = (double)lap[3 * eo + ((B >> (2 * j)) & 3)] * (double)c_dl[j * nv + c];
+    if ((A >> (BA_TTG + j)) & 1) g = -g;
+    if ((j < MX - 1) || hex) du += g;
+  }
+  hd[c] = kh[c] * (T)(du * c_iaa[c]);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int t = 2 * c + k;
+    const int ta = t_ab[t];
+    if (ta < 0) continue;
+    const int tb = t_ab[NT + t];
+    const int eb = t_eb[t];
+    const int e0 = 3 * ((eb & 1) ? c : ta) + ((eb >> 1) & 3);
+    const int e1 = 3 * (((eb >> 3) & 1) ? ta : tb) + ((eb >> 4) & 3);
+    const int e2 = 3 * (((eb >> 6) & 1) ? tb : c) + ((eb >> 7) & 3);
+    T z = lap[e0] * t_ceta[t];
+    z += lap[e1] * t_ceta[NT + t];
+    z += lap[e2] * t_ceta[2 * NT + t];
+    hz[t] = kt[t] * z;
+  }
+}
+
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void fast_cell(
     int remap, int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
     const int32_t *__restrict__ c_bitsB, const T *__restrict__ c_R,
     const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
     const T *__restrict__ c_dl, const T *__restrict__ c_kw, const T *__restrict__ q,
-    const T *__restrict__ kh, const T *__restrict__ cz, const V4<T> *__restrict__ apv,
+    const T *__restrict__ kh, const T *__restrict__ cz, const T *__restrict__ hd,
+    const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ apv,
     const V4<T> *__restrict__ apg, T pvsdt,
     const T *__restrict__ hs, const T *__restrict__ us, double href, T grav, T inv_a,
     T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
@@ -787,6 +877,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     }
     Kd = kh[c] * (T)(du * c_iaa[c]);
   }
+  if (hd) Kd -= hd[c];  // hyperdiffusion: - grad(K div lapu)
   phi[c] = (grav * hc + K - Kd) * inv_a;
 
   // PV at the cell's edges: edge j lies between ring triangles j-1 and j
@@ -833,11 +924,14 @@ __global__ __launch_bounds__(BS) void fast_cell(
         // diffusion, tangential part (owner only): the jump of K zeta
         // between the edge's ring triangles i-1 and i, times |e| / (|e*| a)
         // = 4 kw / (|e*|^2 a)
-        if (kh) {
+        if (tang) {  // non-null with diffusion or hyperdiffusion
           const int ip = (i == 0) ? (hex ? MX - 1 : MX - 2) : i - 1;
           T fz = (T)4 * kw[i] * (T)(1.0 / (dl[i] * dl[i])) * inv_a;
           if ((A >> (BA_FZ + i)) & 1) fz = -fz;
-          P += fz * (cz[tr[i]] - cz[tr[ip]]);
+          T jump = (T)0;
+          if (kh) jump = cz[tr[i]] - cz[tr[ip]];
+          if (hz) jump -= hz[tr[i]] - hz[tr[ip]];  // hyperdiffusion: + grad_tg(K curl lapu)
+          P += fz * jump;
         }
         Pown[ed[i]] = P;
       } else {
@@ -1162,10 +1256,24 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
 #define IMX_TRI(B)                                                                          \
   fast_tri<T, B><<<grid_bs(NT, B), B, 0, st>>>(                                            \
       s->kremap[0], NT, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta, (const T *)s->x_t_kap, \
-      (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, s->f_diff ? (const T *)s->x_kt : nullptr, \
+      (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2,                                      \
+      (s->f_diff || s->f_hyper) ? (const T *)s->x_kt : nullptr,                                    \
       (T *)s->x_cz, (T *)s->q, s->flag)
   IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
 #undef IMX_TRI
+  if (s->f_hyper) {
+    fast_hyper_a<T, 256><<<grid_bs(nv, 256), 256, 0, st>>>(s->kremap[1], nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB,
+                                                           (const T *)s->x_c_dl, s->x_c_iaa, (const T *)s->x_kh,
+                                                           us, (T *)s->x_hy_kd);
+    fast_hyper_b<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(s->kremap[2], NE, s->x_e_oth, s->x_hy_esh,
+                                                           (const T *)s->x_hy_ea, (const T *)s->x_hy_eb,
+                                                           (const T *)s->x_hy_kd, (const T *)s->x_cz,
+                                                           (T *)s->x_hy_lap);
+    fast_hyper_c<T, 256><<<grid_bs(nv, 256), 256, 0, st>>>(
+        s->kremap[1], nv, NT, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_dl, s->x_c_iaa,
+        (const T *)s->x_kh, (const T *)s->x_kt, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta,
+        (const T *)s->x_hy_lap, (T *)s->x_hy_d, (T *)s->x_hy_z);
+  }
   if (s->f_apvm)
     fast_apvm_pre<T, 256><<<grid_bs(nv, 256), 256, 0, st>>>(
         s->kremap[1], nv, NT, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const V4<T> *)s->x_ap_wc, s->x_ap_tnb,
@@ -1176,6 +1284,8 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->kremap[1], nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_R, s->x_c_iaa, \
       (const T *)s->x_c_ika, (const T *)s->x_c_dl, (const T *)s->x_c_kw, (const T *)s->q,       \
       s->f_diff ? (const T *)s->x_kh : nullptr, (const T *)s->x_cz,                              \
+      s->f_hyper ? (const T *)s->x_hy_d : nullptr, s->f_hyper ? (const T *)s->x_hy_z : nullptr,    \
+      (s->f_diff || s->f_hyper) ? s->x_kh : nullptr,                                               \
       s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
       hs, us, s->href,                                                                            \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
@@ -1428,10 +1538,9 @@ static int get_vector(imx_swm *s, const double *src, int kind, double *out) {
 // + (Kd(v2) - Kd(v1)) / (|e| a) - e (Kz(sh2) - Kz(sh1)) / (|e*| a).
 template <typename T>
 static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
-  if (sc.k2 == 0.0) {
-    s->f_diff = false;
-    return IMX_OK;
-  }
+  s->f_diff = false;
+  s->f_hyper = false;
+  if (sc.k2 == 0.0 && sc.k4 == 0.0) return IMX_OK;
   const int nv = s->nv, NT = s->NT;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
   // HC orients grad_ed_tg by edhx%tg (the hc_tt table) instead of ed%nr
@@ -1450,7 +1559,9 @@ static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &
     trhx = trhx_v.data();
   }
   std::vector<double> hx, tr;
-  diffusion_coef(d, trhx, sc.wrap, sc.k2, sc.diffus, 0.0, sc.hyperdiffus, hx, tr);
+  // with K4 > 0 both terms take the square roots of the hyperdiffusion
+  // coefficients, as the reference sets them up
+  diffusion_coef(d, trhx, sc.wrap, sc.k2, sc.diffus, sc.k4, sc.hyperdiffus, hx, tr);
   std::vector<double> kh(nv, 0.0), kt(NT, 0.0);
   for (int co = 0; co < nv; ++co) kh[cn[co]] = hx[co];
   for (int ko = 0; ko < d.nt; ++ko) kt[tn[ko]] = tr[ko];
@@ -1489,7 +1600,48 @@ static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &
   HIPTRY(hipMemcpy(s->x_kh, khT.data(), khT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
   HIPTRY(hipMemcpy(s->x_kt, ktT.data(), ktT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
   HIPTRY(hipMemcpy(s->x_c_bitsA, bitsA.data(), nv * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
-  s->f_diff = true;
+  if (sc.k4 > 0.0) {
+    // lapu at edges: the triangles either side and the two factors
+    const int NE = s->NE;
+    const double a = s->o.erad;
+    std::vector<int32_t> esh((size_t)2 * NE, 0);
+    std::vector<double> ea(NE, 0.0), eb(NE, 0.0);
+    for (int eo = 0; eo < d.ne; ++eo) {
+      const int e = en[eo];
+      const int s1 = d.ed_sh[2 * eo] - 1, s2 = d.ed_sh[2 * eo + 1] - 1;
+      esh[e] = tn[s1];
+      esh[(size_t)NE + e] = tn[s2];
+      double sg;
+      if (s->sc_hc) {
+        sg = (double)htt[e];
+      } else {
+        const double *pa = d.v_p + 3 * (d.ed_v[2 * eo] - 1), *pb = d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1);
+        double nr[3];
+        cross3(pa, pb, nr);
+        double dp = 0.0;
+        for (int q = 0; q < 3; ++q) dp = dp + nr[q] * (d.tr_c[3 * s2 + q] - d.tr_c[3 * s1 + q]);
+        sg = std::signbit(dp) ? -1.0 : 1.0;
+      }
+      ea[e] = 1.0 / (d.ed_leng[eo] * a);
+      eb[e] = sg / (d.edhx_leng[eo] * a);
+    }
+    if (!s->x_hy_esh) {
+      s->x_hy_esh = A.get<int32_t>((size_t)2 * NE);
+      s->x_hy_ea = A.get<T>(NE);
+      s->x_hy_eb = A.get<T>(NE);
+      s->x_hy_kd = A.get<T>(nv);
+      s->x_hy_lap = A.get<T>(NE);
+      s->x_hy_d = A.get<T>(nv);
+      s->x_hy_z = A.get<T>(NT);
+      if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
+    }
+    const std::vector<T> eaT = cvt<T>(ea), ebT = cvt<T>(eb);
+    HIPTRY(hipMemcpy(s->x_hy_esh, esh.data(), esh.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->x_hy_ea, eaT.data(), eaT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->x_hy_eb, ebT.data(), ebT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  }
+  s->f_diff = sc.k2 > 0.0;
+  s->f_hyper = sc.k4 > 0.0;
   return IMX_OK;
 }
 
@@ -2384,10 +2536,10 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     // the fused kernels carry the trsk10 scheme with HTC (with APVM and K2
     // diffusion) or HC staggering
     const bool hc = sc->stag == IMX_STAG_HC;
-    if (sc->wrap != IMX_WRAP_TRSK10 || sc->k4 != 0.0)
+    if (sc->wrap != IMX_WRAP_TRSK10)
       return imx::set_error(IMX_ENOTSUP,
-                            "imx_swm_set_scheme: fused handles take trsk10 with HTC or HC, APVM and K2 "
-                            "diffusion; other options run on FP64_REF handles");
+                            "imx_swm_set_scheme: fused handles take trsk10 with HTC or HC, APVM, diffusion "
+                            "and hyperdiffusion; the pxt16 and gass18 wrappers run on FP64_REF handles");
     int rc;
     if (hc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, sc->wrap)))) return rc;
     rc = s->single ? imx::fast_stagger<float>(s, *d, hc) : imx::fast_stagger<double>(s, *d, hc);
@@ -2397,6 +2549,10 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     if (rc) return rc;
     rc = s->single ? imx::fast_apvm<float>(s, *d, *sc) : imx::fast_apvm<double>(s, *d, *sc);
     if (rc) return rc;
+    // the option flags size the Jacobian's stencil superset
+    s->sc_apvm = sc->apvm != 0;
+    s->sc_diff = sc->k2 != 0.0;
+    s->sc_hyper = sc->k4 != 0.0;
     s->sc = *sc;
     imx::drop_graph(s);
     return IMX_OK;

@@ -176,10 +176,9 @@ contains
     o%dt = dt
     call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
     if (scheme_options()) then
-      ! the fused fp32/fp64 kernels carry trsk10 with HTC or HC, APVM and K2
-      ! diffusion
-      if (trim(precision) /= 'fp64_ref' .and. (wrap_code() /= 0 .or. k4_max > 0._r8)) then
-        write(*, '(a)') " ERROR: pxt16, gass18 and hyperdiffusion run on the fp64_ref path"
+      ! the fused fp32/fp64 kernels carry trsk10 with every option
+      if (trim(precision) /= 'fp64_ref' .and. wrap_code() /= 0) then
+        write(*, '(a)') " ERROR: the pxt16 and gass18 wrappers run on the fp64_ref path"
         write(*, '(a)') "        (par/swm_gpu.par)"
         stop 1
       end if

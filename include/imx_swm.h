@@ -249,8 +249,8 @@ int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out
 /* Scheme options (imx_scheme); d is the mesh the handle was created on
    (the coefficients and the APVM reconstruction read its geometry).
    FP64_REF handles take every option. Fused (FP32/FP64) handles take the
-   trsk10 scheme with HTC or HC, APVM and K2 diffusion, and return
-   IMX_ENOTSUP for pxt16/gass18 and K4. */
+   trsk10 scheme with every other option (HTC or HC, APVM, diffusion,
+   hyperdiffusion) and return IMX_ENOTSUP for pxt16 and gass18. */
 int  imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc);
 int  imx_swm_set_state(imx_swm *s, const double *h, const double *u);
 int  imx_swm_get_state(imx_swm *s, double *h, double *u);

@@ -89,7 +89,7 @@ def test_scheme_errors():
     m = Mesh.icos(2)
     with Swm(m, precision="fp32", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="FP64_REF"):
-            s.set_scheme(k4=1e14)
+            s.set_scheme(wrap="pxt16")
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
@@ -130,7 +130,10 @@ def test_hc_tc2_matches_reference(golden, sfc):
 FAST_K2 = {"const": {"k2": 1e5}, "align": {"k2": 1e5, "diffus": "align"},
            "diam": {"k2": 2e5, "diffus": "diam"}, "apvm": {"pv": "apvm", "pvspar": 0.5},
            "apvm_k2": {"pv": "apvm", "pvspar": 0.3, "k2": 1e5, "diffus": "align"},
-           "hc_apvm_k2": {"stag": "HC", "pv": "apvm", "pvspar": 0.4, "k2": 1e5, "diffus": "diam"}}
+           "hc_apvm_k2": {"stag": "HC", "pv": "apvm", "pvspar": 0.4, "k2": 1e5, "diffus": "diam"},
+           "k4diam": {"k4": 1e14, "hyperdiffus": "diam"},
+           "mix": {"k2": 1e5, "diffus": "diam", "k4": 1e14, "hyperdiffus": "align", "pv": "apvm", "pvspar": 0.5},
+           "hck4": {"stag": "HC", "pv": "apvm", "pvspar": 0.3, "k4": 1e14, "hyperdiffus": "const"}}
 
 
 @pytest.mark.parametrize("name", sorted(FAST_K2))
@@ -156,7 +159,7 @@ def test_fast_k2_tendency_vs_oracle(name):
             assert rel_l2(dg, do) < tol_d, (prec, rel_l2(dg, do))
 
 
-@pytest.mark.parametrize("name", ["align", "apvm_k2", "hc_apvm_k2"])
+@pytest.mark.parametrize("name", ["align", "apvm_k2", "hc_apvm_k2", "mix", "hck4"])
 def test_fast_k2_steps_vs_oracle(name):
     from tests.conftest import rel_l2
     say("fast scheme steps fp64 / fp32", name)
@@ -189,8 +192,7 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"k4": 1e14}, {"wrap": "pxt16", "k2": 1e5}, {"stag": "HC", "k4": 1e14},
-                   {"wrap": "gass18", "pv": "apvm"}):
+        for kw in ({"wrap": "pxt16"}, {"wrap": "pxt16", "k2": 1e5}, {"wrap": "gass18", "pv": "apvm"}):
             with pytest.raises(capi.ImxError, match="fused handles take trsk10"):
                 s.set_scheme(**kw)
 
@@ -238,3 +240,26 @@ def test_fast_hc_vs_oracle():
         p.step(3)
         hq, uq = p.get_state()
     assert np.array_equal(hp, hq) and np.array_equal(up, uq)
+
+
+def test_fast_scheme_jacobian_equals_per_column():
+    """The normal-mode Jacobian of a fused FP64 handle with APVM and
+    hyperdiffusion: its stencil superset widens with the options, so the
+    coloured columns equal the one-column-at-a-time differences of the same
+    handle's tendency."""
+    say("fast scheme jacobian g2")
+    m = Mesh.icos(2)
+    h, u = m.spectral(5)
+    with Swm(m, precision="fp64", dt=900.0, erad=ERAD_REF) as s:
+        s.set_scheme(pv="apvm", pvspar=0.5, k4=1e14, hyperdiffus="const")
+        pert = float(np.mean(m.arrays["hx_areag"]))
+        J, _ = s.jacobian(h, u, pert)
+        x = np.concatenate([h, u])
+        m0, f0 = s.tendency(h, u)
+        base = np.concatenate([m0, f0])
+        for col in (0, 7, m.nv + 3, m.nv + 400):
+            xp = x.copy()
+            xp[col] += pert
+            mp, fp = s.tendency(xp[:m.nv], xp[m.nv:])
+            ref = (np.concatenate([mp, fp]) - base) / pert
+            assert np.array_equal(J[:, col].toarray().ravel(), ref), col
