@@ -223,30 +223,41 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         assert r.returncode != 0 and "fp64_ref" in r.stdout
 
 
-def test_fortran_driver_fp32_k2_diffusion():
-    """K2 diffusion runs on the fused FP32 path from par/swm.par: a day of
-    TC2 on glevel 3 against the oracle with the same option (FP32 storage
-    tolerances)."""
-    print("[gpu-case] fortran imodel g3 tc2 fp32 k2 align", flush=True)
+FUSED_OPTIONS = {
+    "k2": ({24: "1e5 align"}, {"k2": 1e5, "diffus": "align"}, "_align_diffusion_10to5.000"),
+    "mix": ({24: "1e5 align", 26: "1e14 const", 32: "apvm 0.5"},
+            {"k2": 1e5, "diffus": "align", "k4": 1e14, "hyperdiffus": "const", "pv": "apvm", "pvspar": 0.5},
+            "_pvsapvm0.50_align_diffusion_10to5.000_const_hyperdiffusion_10to14.000"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FUSED_OPTIONS))
+def test_fortran_driver_fp32_options(name):
+    """Diffusion (and, in "mix", hyperdiffusion and APVM) run on the fused
+    FP32 path from par/swm.par: a day of TC2 on glevel 3 against the oracle
+    with the same options (FP32 storage tolerances), and the options' effect
+    over the day is far above the FP32 error."""
+    print(f"[gpu-case] fortran imodel g3 tc2 fp32 {name}", flush=True)
+    lines, sc, suffix = FUSED_OPTIONS[name]
     with tempfile.TemporaryDirectory() as td:
         _par(td, 642, 2, 900, 1, 48, "fp32")
         p = os.path.join(td, "par", "swm.par")
         swm = open(p).read().splitlines()
-        swm[24] = "1e5 align"
+        for k, v in lines.items():
+            swm[k] = v
         open(p, "w").write("\n".join(swm) + "\n")
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)
         assert r.returncode == 0, r.stdout[-2000:]
-        base = "swm_tc2_dt900_HTC_trsk10_areageo_align_diffusion_10to5.000"
+        base = "swm_tc2_dt900_HTC_trsk10_areageo" + suffix
         st = imxbin.read(os.path.join(td, "data", f"{base}_state_icos_eqs_nopt_3.bin"))
     m = Mesh.icos(3)
     h0, u0 = m.tc2()
-    h, u = Oracle(m.arrays, scheme={"k2": 1e5, "diffus": "align", "dt": 900.0}).step(h0, u0, 900.0, 96)
+    h, u = Oracle(m.arrays, scheme=dict(sc, dt=900.0)).step(h0, u0, 900.0, 96)
     hp, up = Oracle(m.arrays).step(h0, u0, 900.0, 96)
     A = m.arrays["hx_areag"]
     eh = np.sqrt(np.sum(A * (st["h"] - h) ** 2) / np.sum(A * h ** 2))
     eu = np.linalg.norm(st["u"] - u) / np.linalg.norm(u)
-    # the diffusion's effect over the day is far above the FP32 error
     du = np.linalg.norm(up - u) / np.linalg.norm(u)
     assert eh < 1e-7 and eu < 1e-5 and du > 20 * eu, (eh, eu, du)
 
