@@ -315,7 +315,8 @@ def bench_scheme(args):
         for name, sc in (("trsk10", {}), ("k2const", SCHEME_SETS["k2const"]),
                          ("k2diam", {"k2": 2e5, "diffus": "diam"}), ("apvm", SCHEME_SETS["apvm"]),
                          ("apvm_k2", dict(SCHEME_SETS["apvm"], **SCHEME_SETS["k2const"])),
-                         ("hc", {"stag": "HC"}), ("k4diam", SCHEME_SETS["k4diam"])):
+                         ("hc", {"stag": "HC"}), ("k4diam", SCHEME_SETS["k4diam"]),
+                         ("pxt16", SCHEME_SETS["pxt16"]), ("pxmix", SCHEME_SETS["pxmix"])):
             with Swm(mesh, precision=prec, dt=args.dt) as s:
                 s.set_scheme(**sc)
                 s.set_state(h0, u0)
@@ -326,7 +327,7 @@ def bench_scheme(args):
             fused[key] = {"ms_per_step": ms / steps, "cell_steps_per_s": nv * steps / (ms * 1e-3),
                           "options": sc}
             log(f"fused {key}: {ms / steps:.4f} ms/step")
-        for name in ("k2const", "k2diam", "apvm", "apvm_k2", "hc", "k4diam"):
+        for name in ("k2const", "k2diam", "apvm", "apvm_k2", "hc", "k4diam", "pxt16", "pxmix"):
             fused[f"{prec}_{name}"]["vs_plain"] = (fused[f"{prec}_trsk10"]["ms_per_step"]
                                                    / fused[f"{prec}_{name}"]["ms_per_step"])
     top = res["pxmix"]

@@ -95,6 +95,12 @@ struct imx_swm {
   // and per stage the velocity at cells and grad q at triangles [N][4]
   // (typed T)
   bool f_apvm = false;
+  // fused handles with pxt16: per cell slot the q_e weights of ring
+  // triangles j-1 and j [2][MX][nv], per edge the folded normal of the
+  // pered term [NE][4]; per stage A = P(uh) and B = P(uh q) at cells
+  // [2][nv][4] and q_e at edges (typed T). The cell Perot weights are x_ap_wc.
+  bool f_px = false;
+  void *x_px_wq = nullptr, *x_px_n = nullptr, *x_px_ab = nullptr, *x_px_qe = nullptr;
   int32_t *x_ap_tnb = nullptr;
   void *x_ap_wc = nullptr, *x_ap_wt = nullptr;
   void *x_ap_v = nullptr, *x_ap_g = nullptr;

@@ -813,7 +813,8 @@ __global__ __launch_bounds__(BS) void fast_cell(
     const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
     const T *__restrict__ c_dl, const T *__restrict__ c_kw, const T *__restrict__ q,
     const T *__restrict__ kh, const T *__restrict__ cz, const T *__restrict__ hd,
-    const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ apv,
+    const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ pxw,
+    const T *__restrict__ pxq, V4<T> *__restrict__ pxab, T *__restrict__ pxqe, const V4<T> *__restrict__ apv,
     const V4<T> *__restrict__ apg, T pvsdt,
     const T *__restrict__ hs, const T *__restrict__ us, double href, T grav, T inv_a,
     T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
@@ -865,7 +866,18 @@ __global__ __launch_bounds__(BS) void fast_cell(
     ke += valid ? kw[j] * ue[j] * ue[j] : (T)0;
   }
   const double dh = -(dv + dvc) * c_iaa[c];
-  const T K = ke * c_ika[c];
+  T K = ke * c_ika[c];
+  if (pxw) {  // pxt16: |v|^2 / 2 of the cell's Perot velocity (perhx)
+    T v0 = (T)0, v1 = (T)0, v2 = (T)0;
+#pragma unroll
+    for (int j = 0; j < MX; ++j) {  // a pentagon's padded slot has a zero weight
+      const V4<T> w = pxw[j * nv + c];
+      v0 += w.x * ue[j];
+      v1 += w.y * ue[j];
+      v2 += w.z * ue[j];
+    }
+    K = (v0 * v0 + v1 * v1 + v2 * v2) / (T)2;
+  }
   T Kd = (T)0;
   if (kh) {  // diffusion, normal part: grad(K div u) folds into the potential
     double du = 0.0;
@@ -882,9 +894,17 @@ __global__ __launch_bounds__(BS) void fast_cell(
 
   // PV at the cell's edges: edge j lies between ring triangles j-1 and j
   T qe[MX];
-  qe[0] = (T)0.5 * ((hex ? qt[MX - 1] : qt[MX - 2]) + qt[0]);
+  if (pxq) {  // pxt16: weighted by the distances to the circumcentres
 #pragma unroll
-  for (int j = 1; j < MX; ++j) qe[j] = (T)0.5 * (qt[j - 1] + qt[j]);
+    for (int j = 0; j < MX; ++j) {
+      const int jp = (j == 0) ? (hex ? MX - 1 : MX - 2) : j - 1;
+      qe[j] = pxq[j * nv + c] * qt[jp] + pxq[(MX + j) * nv + c] * qt[j];
+    }
+  } else {
+    qe[0] = (T)0.5 * ((hex ? qt[MX - 1] : qt[MX - 2]) + qt[0]);
+#pragma unroll
+    for (int j = 1; j < MX; ++j) qe[j] = (T)0.5 * (qt[j - 1] + qt[j]);
+  }
   if (apv) {
     // APVM: q_e -= pvspar dt (grad q(t_{j-1}) + grad q(t_j)) . (v(c) + v(nb_j))
     const V4<T> vs = apv[c];
@@ -900,6 +920,43 @@ __global__ __launch_bounds__(BS) void fast_cell(
       qe[j] -= pvsdt * d;
     }
   }
+  if (pxw) {
+    // pxt16's pered Coriolis term: the Perot reconstruction at this cell of
+    // uh (q_e + q_l) / 2 for an edge l is (q_l A + B) / 2 with A = P(uh) and
+    // B = P(uh q); fast_edge combines both cells' A and B with q_l
+    T a0 = (T)0, a1 = (T)0, a2 = (T)0, b0 = (T)0, b1 = (T)0, b2 = (T)0;
+#pragma unroll
+    for (int j = 0; j < MX; ++j) {
+      const V4<T> w = pxw[j * nv + c];
+      const T uh = ue[j] * (T)(href + 0.5 * ((double)hc + (double)hn[j]));
+      const T uq = uh * qe[j];
+      a0 += w.x * uh;
+      a1 += w.y * uh;
+      a2 += w.z * uh;
+      b0 += w.x * uq;
+      b1 += w.y * uq;
+      b2 += w.z * uq;
+    }
+    pxab[c] = V4<T>{a0, a1, a2, (T)0};
+    pxab[nv + c] = V4<T>{b0, b1, b2, (T)0};
+#pragma unroll
+    for (int i = 0; i < MX; ++i) {
+      if ((i < MX - 1 || hex) && ((A >> (BA_OWN + i)) & 1)) {
+        T P = (T)0;
+        if (tang) {  // diffusion's tangential part, as below
+          const int ip = (i == 0) ? (hex ? MX - 1 : MX - 2) : i - 1;
+          T fz = (T)4 * kw[i] * (T)(1.0 / (dl[i] * dl[i])) * inv_a;
+          if ((A >> (BA_FZ + i)) & 1) fz = -fz;
+          T jump = (T)0;
+          if (kh) jump = cz[tr[i]] - cz[tr[ip]];
+          if (hz) jump -= hz[tr[i]] - hz[tr[ip]];
+          P = fz * jump;
+        }
+        Pown[ed[i]] = P;
+        pxqe[ed[i]] = qe[i];
+      }
+    }
+  } else {
   // TRiSK weights from prefix sums of the kite areas:
   // w(i,j) = S_j - S_i - 1/2 (j>i), tot - S_i + S_j - 1/2 (j<i), w(i,i) = 0
   T S[MX + 1];
@@ -939,6 +996,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
       }
     }
   }
+  }
 
   const double dt = dtc ? dtc[c] : dt_all;  // ensembles: per-instance step
   if (stage < 0) {
@@ -959,7 +1017,10 @@ __global__ __launch_bounds__(BS) void fast_edge(int remap, int NE, const int32_t
                                                  const T *__restrict__ e_il,
                                                  const T *__restrict__ phi,
                                                  const T *__restrict__ Pown,
-                                                 const T *__restrict__ Poth, int zero_mom,
+                                                 const T *__restrict__ Poth,
+                                                 const V4<T> *__restrict__ pxab,
+                                                 const T *__restrict__ pxqe,
+                                                 const V4<T> *__restrict__ pxn, int zero_mom,
                                                  int stage, T dt_all, const T *__restrict__ dte,
                                                  const T *u0, T *uout, T *__restrict__ accu,
                                                  T *__restrict__ du_out) {
@@ -971,7 +1032,20 @@ __global__ __launch_bounds__(BS) void fast_edge(int remap, int NE, const int32_t
   const int c2 = oth & OTH_MASK;
   const bool own_v2 = (oth >> OTH_V2) & 1;
   const int v1 = own_v2 ? c2 : o, v2 = own_v2 ? o : c2;
-  T du = -((phi[v2] - phi[v1]) + (Pown[e] + Poth[e])) * e_il[e];
+  T du;
+  if (pxab) {
+    // pered (pxt16): the tangent-plane average of both cells' Perot vectors
+    // of uh (q_e + q_l) / 2, dotted with the edge normal (minus the ed%nr
+    // component; the projection and the sign are folded into pxn)
+    const int nv = NE / 3;
+    const T ql = pxqe[e];
+    const V4<T> A1 = pxab[v1], A2 = pxab[v2], B1 = pxab[nv + v1], B2 = pxab[nv + v2], n = pxn[e];
+    const T uhq = (T)0.25 * (n.x * (ql * (A1.x + A2.x) + (B1.x + B2.x)) + n.y * (ql * (A1.y + A2.y) + (B1.y + B2.y)) +
+                             n.z * (ql * (A1.z + A2.z) + (B1.z + B2.z)));
+    du = -((phi[v2] - phi[v1]) + Pown[e]) * e_il[e] - uhq;
+  } else {
+    du = -((phi[v2] - phi[v1]) + (Pown[e] + Poth[e])) * e_il[e];
+  }
   if (zero_mom) du = (T)0;
   const T dt = dte ? dte[e] : dt_all;
   if (stage < 0) {
@@ -1286,6 +1360,8 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->f_diff ? (const T *)s->x_kh : nullptr, (const T *)s->x_cz,                              \
       s->f_hyper ? (const T *)s->x_hy_d : nullptr, s->f_hyper ? (const T *)s->x_hy_z : nullptr,    \
       (s->f_diff || s->f_hyper) ? s->x_kh : nullptr,                                               \
+      s->f_px ? (const V4<T> *)s->x_ap_wc : nullptr, s->f_px ? (const T *)s->x_px_wq : nullptr,    \
+      (V4<T> *)s->x_px_ab, (T *)s->x_px_qe,                                                        \
       s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
       hs, us, s->href,                                                                            \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
@@ -1296,7 +1372,8 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
 #define IMX_EDGE(B)                                                                              \
   fast_edge<T, B><<<grid_bs(NE, B), B, 0, st>>>(                                                \
       s->kremap[2], NE, s->x_e_oth, (const T *)s->x_e_il, (const T *)s->phi, (const T *)s->Pown, \
-      (const T *)s->Poth, s->o.testcase <= 1, stage, (T)s->o.dt, (const T *)s->dt_e,             \
+      (const T *)s->Poth, s->f_px ? (const V4<T> *)s->x_px_ab : nullptr, (const T *)s->x_px_qe,  \
+      (const V4<T> *)s->x_px_n, s->o.testcase <= 1, stage, (T)s->o.dt, (const T *)s->dt_e,      \
       (const T *)s->u0, uout, (T *)s->accu, (T *)s->du_out)
   IMX_BS_SWITCH(s->kbs[2], IMX_EDGE)
 #undef IMX_EDGE
@@ -1784,24 +1861,20 @@ static int fast_stagger(imx_swm *s, const imx_mesh_desc &d, bool hc) {
   return IMX_OK;
 }
 
-// APVM on the fused kernels: the geometry fast_apvm_pre reads (see there),
-// in the ownership numbering, typed T.
+static void put_projected(double *dst, const double w[3], const double *p) {
+  const double wp = w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
+  for (int q = 0; q < 3; ++q) dst[q] = w[q] - wp * p[q];
+}
+
+// The cell Perot weights of the fused kernels (x_ap_wc, [MX][nv] records):
+// v = sum_j wc_j u_j is vector_edhx2hx_perot of u at the cell node,
+// projected, with hx%nr and the Voronoi midpoints under HC. APVM and pxt16
+// both read them.
 template <typename T>
-static int fast_apvm(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
-  if (!sc.apvm) {
-    s->f_apvm = false;
-    return IMX_OK;
-  }
-  const int nv = s->nv, NT = s->NT;
-  const std::vector<int32_t> &cn = s->lay.cell_new, &tn = s->lay.tri_new;
-  const double a = s->o.erad;
-  // weights as (x, y, z, 0) records: wc [MX][nv], wt [3][NT]
-  std::vector<double> wc((size_t)4 * MX * nv, 0.0), wt((size_t)4 * 3 * NT, 0.0);
-  std::vector<int32_t> tnb((size_t)3 * NT, -1);
-  auto put_projected = [](double *dst, double w[3], const double *p) {
-    const double wp = w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
-    for (int q = 0; q < 3; ++q) dst[q] = w[q] - wp * p[q];
-  };
+static int fast_cell_weights(imx_swm *s, const imx_mesh_desc &d) {
+  const int nv = s->nv;
+  const std::vector<int32_t> &cn = s->lay.cell_new;
+  std::vector<double> wc((size_t)4 * MX * nv, 0.0);
   // HC: hx%nr signs and the Voronoi edge midpoints
   std::vector<int8_t> hcc;
   if (s->sc_hc) {
@@ -1821,6 +1894,106 @@ static int fast_apvm(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
       put_projected(&wc[4 * ((size_t)j * nv + cn[co])], w, p);
     }
   }
+  if (!s->x_ap_wc) {
+    s->x_ap_wc = s->A.get<T>((size_t)4 * MX * nv);
+    if (s->A.err != hipSuccess) return hip_fail(s->A.err, "imx_swm_set_scheme");
+  }
+  const std::vector<T> wcT = cvt<T>(wc);
+  HIPTRY(hipMemcpy(s->x_ap_wc, wcT.data(), wcT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  return IMX_OK;
+}
+
+// pxt16 on the fused kernels (HTC): h at triangles from the barycentric
+// coordinates of the circumcentre (t_kap), q at edges weighted by the
+// distances to the two circumcentres, and the tables of the pered term.
+// Switching back restores the kite-area t_kap.
+template <typename T>
+static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
+  const bool px = sc.wrap == IMX_WRAP_PXT16;
+  if (!px && !s->f_px) return IMX_OK;
+  const int nv = s->nv, NE = s->NE, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  std::vector<double> trhx_v, hxtr_v, trskw_v;
+  const double *trhx = d.tr_trhx_area;
+  if (!px && !trhx) {
+    trhx_v.assign((size_t)3 * d.nt, 0.0);
+    hxtr_v.assign((size_t)MX * d.nv, 0.0);
+    trskw_v.assign((size_t)MX * MX * d.nv, 0.0);
+    gridprop(d, trhx_v.data(), hxtr_v.data(), trskw_v.data());
+    trhx = trhx_v.data();
+  }
+  std::vector<double> tk((size_t)3 * NT, 0.0);
+  for (int ko = 0; ko < d.nt; ++ko) {
+    const int t = tn[ko];
+    int r = -1;
+    for (int l = 0; l < 3; ++l)
+      if (cn[d.tr_v[3 * ko + l] - 1] == t / 2) r = l;
+    if (r < 0) return set_error(IMX_EINVAL, "imx_swm_set_scheme: triangle owner not found");
+    double b[3];
+    if (px) bary_circumcentre(d, ko, b);
+    for (int m = 0; m < 3; ++m)
+      tk[(size_t)m * NT + t] = px ? b[(r + m) % 3] : trhx[3 * ko + (r + m) % 3] / d.tr_areag[ko];
+  }
+  const std::vector<T> tkT = cvt<T>(tk);
+  HIPTRY(hipMemcpy(s->x_t_kap, tkT.data(), tkT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  s->f_px = px;
+  if (!px) return IMX_OK;
+  int rc = fast_cell_weights<T>(s, d);
+  if (rc) return rc;
+  // q_e = (d2 q(sh1) + d1 q(sh2)) / (d1 + d2), d_k the arc from ed%c to the
+  // circumcentre of sh(k), as weights of ring triangles j-1 and j
+  std::vector<double> wq((size_t)2 * MX * nv, 0.0), pn((size_t)4 * NE, 0.0);
+  for (int co = 0; co < nv; ++co) {
+    const int c = cn[co], n = d.v_nnb[co];
+    for (int j = 0; j < n; ++j) {
+      const int eo = d.v_ed[MX * co + j] - 1;
+      const int tp = d.v_tr[MX * co + (j + n - 1) % n] - 1;
+      const int s1 = d.ed_sh[2 * eo] - 1;
+      const double d1 = arclen(d.ed_c + 3 * eo, d.tr_c + 3 * s1);
+      const double d2 = arclen(d.ed_c + 3 * eo, d.tr_c + 3 * (d.ed_sh[2 * eo + 1] - 1));
+      const double w1 = d2 / (d1 + d2), w2 = d1 / (d1 + d2);  // of sh1, sh2
+      wq[(size_t)j * nv + c] = (tp == s1) ? w1 : w2;
+      wq[(size_t)(MX + j) * nv + c] = (tp == s1) ? w2 : w1;
+    }
+  }
+  // pered: minus the ed%nr component of the tangent-plane average at ed%c
+  for (int eo = 0; eo < d.ne; ++eo) {
+    double nr[3];
+    cross3(d.v_p + 3 * (d.ed_v[2 * eo] - 1), d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1), nr);
+    const double nn = norm3(nr);
+    for (int q = 0; q < 3; ++q) nr[q] = -nr[q] / nn;
+    put_projected(&pn[(size_t)4 * en[eo]], nr, d.ed_c + 3 * eo);
+  }
+  Alloc &A = s->A;
+  if (!s->x_px_wq) {
+    s->x_px_wq = A.get<T>((size_t)2 * MX * nv);
+    s->x_px_n = A.get<T>((size_t)4 * NE);
+    s->x_px_ab = A.get<T>((size_t)4 * 2 * nv);
+    s->x_px_qe = A.get<T>(NE);
+    if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
+  }
+  const std::vector<T> wqT = cvt<T>(wq), pnT = cvt<T>(pn);
+  HIPTRY(hipMemcpy(s->x_px_wq, wqT.data(), wqT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_px_n, pnT.data(), pnT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  return IMX_OK;
+}
+
+// APVM on the fused kernels: the geometry fast_apvm_pre reads (see there),
+// in the ownership numbering, typed T.
+template <typename T>
+static int fast_apvm(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
+  if (!sc.apvm) {
+    s->f_apvm = false;
+    return IMX_OK;
+  }
+  const int nv = s->nv, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &tn = s->lay.tri_new;
+  const double a = s->o.erad;
+  // weights as (x, y, z, 0) records: wt [3][NT]
+  std::vector<double> wt((size_t)4 * 3 * NT, 0.0);
+  std::vector<int32_t> tnb((size_t)3 * NT, -1);
+  int rc = fast_cell_weights<T>(s, d);
+  if (rc) return rc;
   for (int ko = 0; ko < d.nt; ++ko) {
     const int t = tn[ko];
     const double *p = d.tr_c + 3 * ko;
@@ -1853,15 +2026,13 @@ static int fast_apvm(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
   Alloc &A = s->A;
   if (!s->x_ap_tnb) {
     s->x_ap_tnb = A.get<int32_t>((size_t)3 * NT);
-    s->x_ap_wc = A.get<T>((size_t)4 * MX * nv);
     s->x_ap_wt = A.get<T>((size_t)4 * 3 * NT);
     s->x_ap_v = A.get<T>((size_t)4 * nv);
     s->x_ap_g = A.get<T>((size_t)4 * NT);
     if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
   }
-  const std::vector<T> wcT = cvt<T>(wc), wtT = cvt<T>(wt);
+  const std::vector<T> wtT = cvt<T>(wt);
   HIPTRY(hipMemcpy(s->x_ap_tnb, tnb.data(), tnb.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
-  HIPTRY(hipMemcpy(s->x_ap_wc, wcT.data(), wcT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
   HIPTRY(hipMemcpy(s->x_ap_wt, wtT.data(), wtT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
   s->f_apvm = true;
   return IMX_OK;
@@ -2536,10 +2707,10 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     // the fused kernels carry the trsk10 scheme with HTC (with APVM and K2
     // diffusion) or HC staggering
     const bool hc = sc->stag == IMX_STAG_HC;
-    if (sc->wrap != IMX_WRAP_TRSK10)
+    if (sc->wrap == IMX_WRAP_GASS18 || (hc && sc->wrap == IMX_WRAP_PXT16))
       return imx::set_error(IMX_ENOTSUP,
-                            "imx_swm_set_scheme: fused handles take trsk10 with HTC or HC, APVM, diffusion "
-                            "and hyperdiffusion; the pxt16 and gass18 wrappers run on FP64_REF handles");
+                            "imx_swm_set_scheme: fused handles take trsk10 (HTC or HC) and pxt16 (HTC) with "
+                            "APVM, diffusion and hyperdiffusion; gass18 and HC pxt16 run on FP64_REF handles");
     int rc;
     if (hc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, sc->wrap)))) return rc;
     rc = s->single ? imx::fast_stagger<float>(s, *d, hc) : imx::fast_stagger<double>(s, *d, hc);
@@ -2549,6 +2720,9 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     if (rc) return rc;
     rc = s->single ? imx::fast_apvm<float>(s, *d, *sc) : imx::fast_apvm<double>(s, *d, *sc);
     if (rc) return rc;
+    rc = s->single ? imx::fast_pxt16<float>(s, *d, *sc) : imx::fast_pxt16<double>(s, *d, *sc);
+    if (rc) return rc;
+    s->sc_wrap = sc->wrap;
     // the option flags size the Jacobian's stencil superset
     s->sc_apvm = sc->apvm != 0;
     s->sc_diff = sc->k2 != 0.0;

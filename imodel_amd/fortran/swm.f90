@@ -176,9 +176,11 @@ contains
     o%dt = dt
     call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
     if (scheme_options()) then
-      ! the fused fp32/fp64 kernels carry trsk10 with every option
-      if (trim(precision) /= 'fp64_ref' .and. wrap_code() /= 0) then
-        write(*, '(a)') " ERROR: the pxt16 and gass18 wrappers run on the fp64_ref path"
+      ! the fused fp32/fp64 kernels carry trsk10 and (with HTC) pxt16, with
+      ! every other option
+      if (trim(precision) /= 'fp64_ref' .and. (wrap_code() == 2 .or. &
+          (wrap_code() == 1 .and. trim(stag) == 'HC'))) then
+        write(*, '(a)') " ERROR: gass18 and HC with pxt16 run on the fp64_ref path"
         write(*, '(a)') "        (par/swm_gpu.par)"
         stop 1
       end if

@@ -4,8 +4,9 @@
 !                 [meshpar] [nv] override as in the reference
 !   par/swm.par   as swmpars        (src/swm_data.f90:338-693), the
 !                 geo-area subset this build runs: trsk10 with HTC or HC
-!                 (TC2), APVM, diffusion and hyperdiffusion on every path;
-!                 the pxt16 and gass18 wrappers on the fp64_ref path
+!                 (TC2) and pxt16 with HTC, APVM, diffusion and
+!                 hyperdiffusion on every path; gass18 and HC pxt16 on the
+!                 fp64_ref path
 !   par/swm_gpu.par (new, optional): precision, seed, SFC and graph switches
 !=======================================================================
 module swm_par

@@ -224,7 +224,8 @@ class Swm:
         parameter pvspar), diffusion K2_max with coefficient function diffus,
         hyperdiffusion K4_max with hyperdiffus ("const" | "align" | "diam").
         Every option runs on FP64_REF handles; fused (FP32/FP64) handles take
-        trsk10 with every other option and refuse pxt16 and gass18."""
+        trsk10 (HTC or HC) and pxt16 (HTC) with APVM, diffusion and
+        hyperdiffusion, and refuse gass18 and HC pxt16."""
         sc = capi.Scheme(stag=capi.STAG[stag], wrap=capi.WRAP[wrap], apvm=int(pv == "apvm"), pvspar=float(pvspar), k2=float(k2),
                          diffus=capi.COEF[diffus], k4=float(k4), hyperdiffus=capi.COEF[hyperdiffus])
         capi.check(capi.lib().imx_swm_set_scheme(self._h, ctypes.byref(self.mesh.desc()), ctypes.byref(sc)),

@@ -248,9 +248,9 @@ void imx_swm_default_opts(imx_swm_opts *o);
 int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out);
 /* Scheme options (imx_scheme); d is the mesh the handle was created on
    (the coefficients and the APVM reconstruction read its geometry).
-   FP64_REF handles take every option. Fused (FP32/FP64) handles take the
-   trsk10 scheme with every other option (HTC or HC, APVM, diffusion,
-   hyperdiffusion) and return IMX_ENOTSUP for pxt16 and gass18. */
+   FP64_REF handles take every option. Fused (FP32/FP64) handles take
+   trsk10 (HTC or HC) and pxt16 (HTC) with APVM, diffusion and
+   hyperdiffusion, and return IMX_ENOTSUP for gass18 and HC pxt16. */
 int  imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc);
 int  imx_swm_set_state(imx_swm *s, const double *h, const double *u);
 int  imx_swm_get_state(imx_swm *s, double *h, double *u);

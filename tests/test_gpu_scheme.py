@@ -89,7 +89,7 @@ def test_scheme_errors():
     m = Mesh.icos(2)
     with Swm(m, precision="fp32", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="FP64_REF"):
-            s.set_scheme(wrap="pxt16")
+            s.set_scheme(wrap="gass18")
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
@@ -133,7 +133,10 @@ FAST_K2 = {"const": {"k2": 1e5}, "align": {"k2": 1e5, "diffus": "align"},
            "hc_apvm_k2": {"stag": "HC", "pv": "apvm", "pvspar": 0.4, "k2": 1e5, "diffus": "diam"},
            "k4diam": {"k4": 1e14, "hyperdiffus": "diam"},
            "mix": {"k2": 1e5, "diffus": "diam", "k4": 1e14, "hyperdiffus": "align", "pv": "apvm", "pvspar": 0.5},
-           "hck4": {"stag": "HC", "pv": "apvm", "pvspar": 0.3, "k4": 1e14, "hyperdiffus": "const"}}
+           "hck4": {"stag": "HC", "pv": "apvm", "pvspar": 0.3, "k4": 1e14, "hyperdiffus": "const"},
+           "pxt16": {"wrap": "pxt16"},
+           "pxk2": {"wrap": "pxt16", "k2": 1e5, "diffus": "align"},
+           "pxmix": {"wrap": "pxt16", "pv": "apvm", "pvspar": 0.5, "k4": 1e14, "hyperdiffus": "const"}}
 
 
 @pytest.mark.parametrize("name", sorted(FAST_K2))
@@ -159,7 +162,7 @@ def test_fast_k2_tendency_vs_oracle(name):
             assert rel_l2(dg, do) < tol_d, (prec, rel_l2(dg, do))
 
 
-@pytest.mark.parametrize("name", ["align", "apvm_k2", "hc_apvm_k2", "mix", "hck4"])
+@pytest.mark.parametrize("name", ["align", "apvm_k2", "hc_apvm_k2", "mix", "hck4", "pxt16", "pxmix"])
 def test_fast_k2_steps_vs_oracle(name):
     from tests.conftest import rel_l2
     say("fast scheme steps fp64 / fp32", name)
@@ -192,7 +195,7 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"wrap": "pxt16"}, {"wrap": "pxt16", "k2": 1e5}, {"wrap": "gass18", "pv": "apvm"}):
+        for kw in ({"wrap": "gass18"}, {"stag": "HC", "wrap": "pxt16"}, {"wrap": "gass18", "pv": "apvm"}):
             with pytest.raises(capi.ImxError, match="fused handles take trsk10"):
                 s.set_scheme(**kw)
 
