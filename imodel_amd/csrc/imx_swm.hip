@@ -806,7 +806,9 @@ __global__ __launch_bounds__(BS) void fast_hyper_c(int remap, int nv, int NT, co
   }
 }
 
-template <typename T, int BS>
+// OPT = false compiles the plain scheme only (no option code and its
+// registers); the option pointers are then ignored.
+template <typename T, int BS, bool OPT>
 __global__ __launch_bounds__(BS) void fast_cell(
     int remap, int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
     const int32_t *__restrict__ c_bitsB, const T *__restrict__ c_R,
@@ -867,7 +869,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
   }
   const double dh = -(dv + dvc) * c_iaa[c];
   T K = ke * c_ika[c];
-  if (pxw) {  // pxt16: |v|^2 / 2 of the cell's Perot velocity (perhx)
+  if (OPT && pxw) {  // pxt16: |v|^2 / 2 of the cell's Perot velocity (perhx)
     T v0 = (T)0, v1 = (T)0, v2 = (T)0;
 #pragma unroll
     for (int j = 0; j < MX; ++j) {  // a pentagon's padded slot has a zero weight
@@ -879,7 +881,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     K = (v0 * v0 + v1 * v1 + v2 * v2) / (T)2;
   }
   T Kd = (T)0;
-  if (kh) {  // diffusion, normal part: grad(K div u) folds into the potential
+  if (OPT && kh) {  // diffusion, normal part: grad(K div u) folds into the potential
     double du = 0.0;
 #pragma unroll
     for (int j = 0; j < MX; ++j) {
@@ -889,12 +891,12 @@ __global__ __launch_bounds__(BS) void fast_cell(
     }
     Kd = kh[c] * (T)(du * c_iaa[c]);
   }
-  if (hd) Kd -= hd[c];  // hyperdiffusion: - grad(K div lapu)
+  if (OPT && hd) Kd -= hd[c];  // hyperdiffusion: - grad(K div lapu)
   phi[c] = (grav * hc + K - Kd) * inv_a;
 
   // PV at the cell's edges: edge j lies between ring triangles j-1 and j
   T qe[MX];
-  if (pxq) {  // pxt16: weighted by the distances to the circumcentres
+  if (OPT && pxq) {  // pxt16: weighted by the distances to the circumcentres
 #pragma unroll
     for (int j = 0; j < MX; ++j) {
       const int jp = (j == 0) ? (hex ? MX - 1 : MX - 2) : j - 1;
@@ -905,7 +907,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
 #pragma unroll
     for (int j = 1; j < MX; ++j) qe[j] = (T)0.5 * (qt[j - 1] + qt[j]);
   }
-  if (apv) {
+  if (OPT && apv) {
     // APVM: q_e -= pvspar dt (grad q(t_{j-1}) + grad q(t_j)) . (v(c) + v(nb_j))
     const V4<T> vs = apv[c];
     V4<T> g[MX];
@@ -920,7 +922,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
       qe[j] -= pvsdt * d;
     }
   }
-  if (pxw) {
+  if (OPT && pxw) {
     // pxt16's pered Coriolis term: the Perot reconstruction at this cell of
     // uh (q_e + q_l) / 2 for an edge l is (q_l A + B) / 2 with A = P(uh) and
     // B = P(uh q); fast_edge combines both cells' A and B with q_l
@@ -981,7 +983,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
         // diffusion, tangential part (owner only): the jump of K zeta
         // between the edge's ring triangles i-1 and i, times |e| / (|e*| a)
         // = 4 kw / (|e*|^2 a)
-        if (tang) {  // non-null with diffusion or hyperdiffusion
+        if (OPT && tang) {  // non-null with diffusion or hyperdiffusion
           const int ip = (i == 0) ? (hex ? MX - 1 : MX - 2) : i - 1;
           T fz = (T)4 * kw[i] * (T)(1.0 / (dl[i] * dl[i])) * inv_a;
           if ((A >> (BA_FZ + i)) & 1) fz = -fz;
@@ -1354,7 +1356,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
         (const V4<T> *)s->x_ap_wt, us, (const T *)s->q, (V4<T> *)s->x_ap_v, (V4<T> *)s->x_ap_g);
   if (evs) (void)hipEventRecord(evs[1], st);
 #define IMX_CELL(B)                                                                           \
-  fast_cell<T, B><<<grid_bs(nv, B), B, 0, st>>>(                                             \
+  fast_cell<T, B, OPT><<<grid_bs(nv, B), B, 0, st>>>(                                        \
       s->kremap[1], nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const T *)s->x_c_R, s->x_c_iaa, \
       (const T *)s->x_c_ika, (const T *)s->x_c_dl, (const T *)s->x_c_kw, (const T *)s->q,       \
       s->f_diff ? (const T *)s->x_kh : nullptr, (const T *)s->x_cz,                              \
@@ -1366,7 +1368,13 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       hs, us, s->href,                                                                            \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
-  IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
+  if (s->f_diff || s->f_hyper || s->f_apvm || s->f_px) {
+    constexpr bool OPT = true;
+    IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
+  } else {
+    constexpr bool OPT = false;
+    IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
+  }
 #undef IMX_CELL
   if (evs) (void)hipEventRecord(evs[2], st);
 #define IMX_EDGE(B)                                                                              \
