@@ -806,9 +806,10 @@ __global__ __launch_bounds__(BS) void fast_hyper_c(int remap, int nv, int NT, co
   }
 }
 
-// OPT = false compiles the plain scheme only (no option code and its
-// registers); the option pointers are then ignored.
-template <typename T, int BS, bool OPT>
+// OPT selects the code compiled in, and with it the register footprint:
+// 0 the plain scheme, 1 with diffusion and hyperdiffusion, 2 with every
+// option (APVM and pxt16 too). Pointers of options left out are ignored.
+template <typename T, int BS, int OPT>
 __global__ __launch_bounds__(BS) void fast_cell(
     int remap, int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
     const int32_t *__restrict__ c_bitsB, const T *__restrict__ c_R,
@@ -869,7 +870,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
   }
   const double dh = -(dv + dvc) * c_iaa[c];
   T K = ke * c_ika[c];
-  if (OPT && pxw) {  // pxt16: |v|^2 / 2 of the cell's Perot velocity (perhx)
+  if (OPT >= 2 && pxw) {  // pxt16: |v|^2 / 2 of the cell's Perot velocity (perhx)
     T v0 = (T)0, v1 = (T)0, v2 = (T)0;
 #pragma unroll
     for (int j = 0; j < MX; ++j) {  // a pentagon's padded slot has a zero weight
@@ -881,7 +882,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     K = (v0 * v0 + v1 * v1 + v2 * v2) / (T)2;
   }
   T Kd = (T)0;
-  if (OPT && kh) {  // diffusion, normal part: grad(K div u) folds into the potential
+  if (OPT >= 1 && kh) {  // diffusion, normal part: grad(K div u) folds into the potential
     double du = 0.0;
 #pragma unroll
     for (int j = 0; j < MX; ++j) {
@@ -891,12 +892,12 @@ __global__ __launch_bounds__(BS) void fast_cell(
     }
     Kd = kh[c] * (T)(du * c_iaa[c]);
   }
-  if (OPT && hd) Kd -= hd[c];  // hyperdiffusion: - grad(K div lapu)
+  if (OPT >= 1 && hd) Kd -= hd[c];  // hyperdiffusion: - grad(K div lapu)
   phi[c] = (grav * hc + K - Kd) * inv_a;
 
   // PV at the cell's edges: edge j lies between ring triangles j-1 and j
   T qe[MX];
-  if (OPT && pxq) {  // pxt16: weighted by the distances to the circumcentres
+  if (OPT >= 2 && pxq) {  // pxt16: weighted by the distances to the circumcentres
 #pragma unroll
     for (int j = 0; j < MX; ++j) {
       const int jp = (j == 0) ? (hex ? MX - 1 : MX - 2) : j - 1;
@@ -907,7 +908,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
 #pragma unroll
     for (int j = 1; j < MX; ++j) qe[j] = (T)0.5 * (qt[j - 1] + qt[j]);
   }
-  if (OPT && apv) {
+  if (OPT >= 2 && apv) {
     // APVM: q_e -= pvspar dt (grad q(t_{j-1}) + grad q(t_j)) . (v(c) + v(nb_j))
     const V4<T> vs = apv[c];
     V4<T> g[MX];
@@ -922,7 +923,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
       qe[j] -= pvsdt * d;
     }
   }
-  if (OPT && pxw) {
+  if (OPT >= 2 && pxw) {
     // pxt16's pered Coriolis term: the Perot reconstruction at this cell of
     // uh (q_e + q_l) / 2 for an edge l is (q_l A + B) / 2 with A = P(uh) and
     // B = P(uh q); fast_edge combines both cells' A and B with q_l
@@ -945,7 +946,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     for (int i = 0; i < MX; ++i) {
       if ((i < MX - 1 || hex) && ((A >> (BA_OWN + i)) & 1)) {
         T P = (T)0;
-        if (tang) {  // diffusion's tangential part, as below
+        if (OPT >= 1 && tang) {  // diffusion's tangential part, as below
           const int ip = (i == 0) ? (hex ? MX - 1 : MX - 2) : i - 1;
           T fz = (T)4 * kw[i] * (T)(1.0 / (dl[i] * dl[i])) * inv_a;
           if ((A >> (BA_FZ + i)) & 1) fz = -fz;
@@ -983,7 +984,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
         // diffusion, tangential part (owner only): the jump of K zeta
         // between the edge's ring triangles i-1 and i, times |e| / (|e*| a)
         // = 4 kw / (|e*|^2 a)
-        if (OPT && tang) {  // non-null with diffusion or hyperdiffusion
+        if (OPT >= 1 && tang) {  // non-null with diffusion or hyperdiffusion
           const int ip = (i == 0) ? (hex ? MX - 1 : MX - 2) : i - 1;
           T fz = (T)4 * kw[i] * (T)(1.0 / (dl[i] * dl[i])) * inv_a;
           if ((A >> (BA_FZ + i)) & 1) fz = -fz;
@@ -1368,11 +1369,14 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       hs, us, s->href,                                                                            \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
-  if (s->f_diff || s->f_hyper || s->f_apvm || s->f_px) {
-    constexpr bool OPT = true;
+  if (s->f_apvm || s->f_px) {
+    constexpr int OPT = 2;
+    IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
+  } else if (s->f_diff || s->f_hyper) {
+    constexpr int OPT = 1;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
   } else {
-    constexpr bool OPT = false;
+    constexpr int OPT = 0;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
   }
 #undef IMX_CELL
