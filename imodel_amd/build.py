@@ -27,7 +27,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch
              "-Wno-unused-function"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
-HOST_SRCS = ["imx_mesh.cpp", "imx_ic.cpp", "imx_sfc.cpp", "imx_gridio.cpp", "imx_plot.cpp", "imx_jacobian.cpp",
+HOST_SRCS = ["imx_mesh.cpp", "imx_trmesh.cpp", "imx_ic.cpp", "imx_sfc.cpp", "imx_gridio.cpp", "imx_plot.cpp", "imx_jacobian.cpp",
              "imx_lte.cpp"]
 HIP_SRCS = ["imx_swm.hip", "imx_mlp.hip", "imx_learn.hip"]
 

@@ -46,7 +46,16 @@ struct HostMesh {
   std::vector<int32_t> tr_v, tr_ed, tr_tg, tr_nr;
   std::vector<double> tr_areag, tr_c, tr_clat, tr_clon;
   std::vector<double> tr_trhx_area, hx_hxtr_area, hx_trskw;  // gridprop
+  int64_t trmesh_restarts = 0;  // located-walk restarts of the triangulation
 };
+
+// Delaunay triangulation of n unit vectors in STRIPACK TRMESH's insertion
+// order (imx_trmesh.cpp); returns each node's CCW neighbour ring (0-based,
+// maxnb slots per node) starting where meshstruct starts reading it, and
+// the number of walk restarts (non-zero: the numbering may differ from the
+// reference's, whose restarts are clock-seeded).
+bool trmesh_rings(int n, const double *x, const double *y, const double *z, std::vector<int32_t> &nnb,
+                  std::vector<int32_t> &nb, int maxnb, int64_t *restarts, std::string &err);
 
 // Spherical geometry with the reference's expression order
 // (src/smeshpack.f90: arclen :4624, sphtriarea :4710, cross_product :6560,

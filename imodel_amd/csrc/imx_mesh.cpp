@@ -14,10 +14,10 @@
 //  * the geometry formulas (arclen, sphtriarea, circumcentre, sphhxarea,
 //    midpoints) and initialize_gridprop (kite areas, TRiSK weights of
 //    Thuburn et al. 2009 eq. 33): they define the discretisation.
-// The triangulation itself is NOT computed with a Delaunay insertion as in
-// the reference (STRIPACK trmesh): the icosahedral subdivision is carried as
-// explicit triangles, so neighbour rotations start at the lowest-numbered
-// neighbour. The resulting mesh equals the reference's up to a renumbering.
+// The triangulation is computed as the reference computes it, with STRIPACK's
+// incremental Delaunay insertion restated in imx_trmesh.cpp, for the
+// refinement levels (midpoint order) and for the final node set (neighbour
+// rings), so icosahedral grids carry the reference's numbering.
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -299,156 +299,62 @@ static void icos_base(int pos, double x[12], double y[12], double z[12]) {
   }
 }
 
-using Tri = std::array<int32_t, 3>;  // 0-based node ids, CCW seen from outside
-
-// The 20 faces of the convex hull of the 12 base nodes, oriented outward.
-static std::vector<Tri> icos_faces(const std::vector<double> &p) {
-  std::vector<Tri> f;
-  for (int i = 0; i < 12; ++i)
-    for (int j = i + 1; j < 12; ++j)
-      for (int k = j + 1; k < 12; ++k) {
-        double e1[3], e2[3], n[3];
-        sub3(&p[3 * j], &p[3 * i], e1);
-        sub3(&p[3 * k], &p[3 * i], e2);
-        cross3(e1, e2, n);
-        int pos = 0, neg = 0;
-        for (int q = 0; q < 12; ++q) {
-          if (q == i || q == j || q == k) continue;
-          double w[3];
-          sub3(&p[3 * q], &p[3 * i], w);
-          double s = dot3(n, w);
-          if (s > 1e-12) ++pos;
-          else if (s < -1e-12) ++neg;
-        }
-        if (pos == 0 || neg == 0) {
-          // outward when n . p_i > 0
-          if (dot3(n, &p[3 * i]) > 0) f.push_back({i, j, k});
-          else f.push_back({i, k, j});
-        }
-      }
-  return f;
-}
-
-// CCW neighbour rings from CCW triangles, each ring started at the
-// lowest-numbered neighbour.
-static bool rings(int nv, const std::vector<Tri> &tris, std::vector<int32_t> &nnb,
-                  std::vector<int32_t> &nb) {
-  const int MX = IMX_MAXNB;
-  std::vector<int32_t> cnt(nv, 0);
-  std::vector<int32_t> succ_from((size_t)nv * MX, -1), succ_to((size_t)nv * MX, -1);
-  for (const Tri &t : tris) {
-    for (int e = 0; e < 3; ++e) {
-      int a = t[e], b = t[(e + 1) % 3], c = t[(e + 2) % 3];
-      if (cnt[a] >= MX) return false;
-      succ_from[(size_t)a * MX + cnt[a]] = b;
-      succ_to[(size_t)a * MX + cnt[a]] = c;
-      ++cnt[a];
-    }
-  }
-  nnb.assign(nv, 0);
-  nb.assign((size_t)nv * MX, 0);
-  for (int a = 0; a < nv; ++a) {
-    int n = cnt[a];
-    int start = INT32_MAX;
-    for (int s = 0; s < n; ++s) start = std::min(start, succ_from[(size_t)a * MX + s]);
-    int cur = start;
-    for (int s = 0; s < n; ++s) {
-      nb[(size_t)a * MX + s] = cur;
-      int nxt = -1;
-      for (int q = 0; q < n; ++q)
-        if (succ_from[(size_t)a * MX + q] == cur) nxt = succ_to[(size_t)a * MX + q];
-      if (nxt < 0) return false;
-      cur = nxt;
-    }
-    if (cur != start) return false;
-    nnb[a] = n;
-  }
-  return true;
-}
-
 static bool structure(HostMesh &m, int nv, const std::vector<double> &p, const std::vector<int32_t> &nnb,
                       const std::vector<int32_t> &nb0, std::string &err);
 
+// meshbuild for icos/nopt grids (src/smeshpack.f90:77-368 with icosanodes
+// :1297-1392): the 12 base nodes, then glevel refinements that each
+// triangulate the current nodes (trmesh) and append the normalised midpoint
+// of every arc i-j with j > i, node by node in ring order (meshrefin
+// :1587-1675); finally the mesh is structured from the triangulation of all
+// nodes (meshstruct). The hierarchical construction (hrchy > 0) repeats the
+// same refinements level by level and yields the same mesh.
 static bool build_icos(int glevel, int pos, HostMesh &m, std::string &err) {
   const int MX = IMX_MAXNB;
-  std::vector<double> p(36);
-  {
-    double x[12], y[12], z[12];
-    icos_base(pos, x, y, z);
-    for (int i = 0; i < 12; ++i) {
-      p[3 * i] = x[i];
-      p[3 * i + 1] = y[i];
-      p[3 * i + 2] = z[i];
-    }
-  }
-  std::vector<Tri> tris = icos_faces(p);
-  if (tris.size() != 20) {
-    err = "icosahedron hull has " + std::to_string(tris.size()) + " faces";
-    return false;
-  }
+  const size_t nfin = (size_t)10 * ((size_t)1 << (2 * glevel)) + 2;
+  std::vector<double> x(nfin), y(nfin), z(nfin);
+  icos_base(pos, x.data(), y.data(), z.data());
   int nv = 12;
+  std::vector<int32_t> nnb, nb;
+  int64_t restarts = 0;
   for (int lev = 1; lev <= glevel; ++lev) {
-    // Append midpoints node by node, neighbours in ring order (higher ids only)
-    std::vector<int32_t> nnb, nb;
-    if (!rings(nv, tris, nnb, nb)) {
-      err = "bad neighbour ring at level " + std::to_string(lev);
-      return false;
-    }
-    // midpoint id of edge (i, nb(j)) kept at the lower node's ring slot
-    std::vector<int32_t> mid((size_t)nv * MX, -1);
-    auto mid_of = [&](int a, int b) -> int32_t {
-      const int lo = std::min(a, b), hi = std::max(a, b);
-      for (int j = 0; j < nnb[lo]; ++j)
-        if (nb[(size_t)lo * MX + j] == hi) return mid[(size_t)lo * MX + j];
-      return -1;
-    };
-    int nn = nv;
-    p.resize((size_t)3 * (4 * (size_t)nv - 6));
+    int64_t r = 0;
+    if (!trmesh_rings(nv, x.data(), y.data(), z.data(), nnb, nb, MX, &r, err)) return false;
+    restarts += r;
+    int k = nv;
     for (int i = 0; i < nv; ++i)
       for (int j = 0; j < nnb[i]; ++j) {
-        int k = nb[(size_t)i * MX + j];
-        if (k <= i) continue;
-        double q[3];
-        q[0] = 0.5 * (p[3 * i] + p[3 * k]);
-        q[1] = 0.5 * (p[3 * i + 1] + p[3 * k + 1]);
-        q[2] = 0.5 * (p[3 * i + 2] + p[3 * k + 2]);
-        double r = q[0] * q[0];
-        r = r + q[1] * q[1];
-        r = r + q[2] * q[2];
-        r = std::sqrt(r);
-        p[3 * nn] = q[0] / r;
-        p[3 * nn + 1] = q[1] / r;
-        p[3 * nn + 2] = q[2] / r;
-        mid[(size_t)i * MX + j] = nn;
-        ++nn;
+        const int o = nb[(size_t)i * MX + j];
+        if (o <= i) continue;
+        double qx = 0.5 * (x[i] + x[o]), qy = 0.5 * (y[i] + y[o]), qz = 0.5 * (z[i] + z[o]);
+        double r2 = qx * qx;
+        r2 = r2 + qy * qy;
+        r2 = r2 + qz * qz;
+        const double rr = std::sqrt(r2);
+        x[k] = qx / rr;
+        y[k] = qy / rr;
+        z[k] = qz / rr;
+        ++k;
       }
-    std::vector<Tri> nt;
-    nt.reserve(tris.size() * 4);
-    for (const Tri &t : tris) {
-      int a = t[0], b = t[1], c = t[2];
-      int ab = mid_of(a, b), bc = mid_of(b, c), ca = mid_of(c, a);
-      if (ab < 0 || bc < 0 || ca < 0) {
-        err = "midpoint lookup failed";
-        return false;
-      }
-      nt.push_back({a, ab, ca});
-      nt.push_back({ab, b, bc});
-      nt.push_back({ca, bc, c});
-      nt.push_back({ab, bc, ca});
+    if ((size_t)k != (size_t)10 * ((size_t)1 << (2 * lev)) + 2) {
+      err = "refinement to level " + std::to_string(lev) + " gave " + std::to_string(k) + " nodes";
+      return false;
     }
-    tris.swap(nt);
-    nv = nn;
+    nv = k;
   }
-
-  std::vector<int32_t> nnb, nb0;
-  if (!rings(nv, tris, nnb, nb0)) {
-    err = "bad neighbour ring";
-    return false;
+  int64_t r = 0;
+  if (!trmesh_rings(nv, x.data(), y.data(), z.data(), nnb, nb, MX, &r, err)) return false;
+  restarts += r;
+  std::vector<double> p((size_t)3 * nv);
+  for (int i = 0; i < nv; ++i) {
+    p[3 * i] = x[i];
+    p[3 * i + 1] = y[i];
+    p[3 * i + 2] = z[i];
   }
   m.glevel = glevel;
   m.pos = (pos == IMX_POS_POL) ? "pol" : "eqs";
-  p.resize((size_t)3 * nv);
-  return structure(m, nv, p, nnb, nb0, err);
+  m.trmesh_restarts = restarts;
+  return structure(m, nv, p, nnb, nb, err);
 }
 
 // ---- meshstruct-equivalent structuring (src/smeshpack.f90:370-1195) ----
@@ -757,27 +663,41 @@ int imx_mesh_build_read(const char *path, imx_mesh **out) {
   std::fclose(f);
   if (!ok) return imx::set_error(IMX_EIO, "imx_mesh_build_read: cannot read the nodes of " + fn);
   // the Delaunay structure: one line of 6 neighbour indices per node, 0 for
-  // none, in ring order
+  // none, in ring order; without the file meshstruct triangulates the nodes
+  // (deltriread = 0, src/smeshpack.f90:497-520)
   const std::string ngb = fn.substr(0, fn.size() - 4) + ".ngb";
-  f = std::fopen(ngb.c_str(), "r");
-  if (!f)
-    return imx::set_error(IMX_EIO, "imx_mesh_build_read: " + ngb +
-                                       " not found (the Delaunay triangulation of bare nodes, STRIPACK, is "
-                                       "not part of this build)");
   const int MX = IMX_MAXNB;
   std::vector<int32_t> nnb(n, 0), nb((size_t)MX * n, 0);
-  for (int i = 0; ok && i < n; ++i) {
-    int v[MX];
-    for (int j = 0; ok && j < MX; ++j) ok = std::fscanf(f, "%d", &v[j]) == 1;
-    for (int j = 0; ok && j < MX; ++j)
-      if (v[j] > 0) {
-        if (v[j] > n) ok = false;
-        else nb[(size_t)MX * i + nnb[i]++] = v[j] - 1;
-      }
-    ok = ok && nnb[i] >= 3;
+  f = std::fopen(ngb.c_str(), "r");
+  if (f) {
+    for (int i = 0; ok && i < n; ++i) {
+      int v[MX];
+      for (int j = 0; ok && j < MX; ++j) ok = std::fscanf(f, "%d", &v[j]) == 1;
+      for (int j = 0; ok && j < MX; ++j)
+        if (v[j] > 0) {
+          if (v[j] > n) ok = false;
+          else nb[(size_t)MX * i + nnb[i]++] = v[j] - 1;
+        }
+      ok = ok && nnb[i] >= 3;
+    }
+    std::fclose(f);
+    if (!ok) return imx::set_error(IMX_EIO, "imx_mesh_build_read: bad neighbour list in " + ngb);
+  } else {
+    std::vector<double> x(n), y(n), z(n);
+    for (int i = 0; i < n; ++i) {
+      x[i] = p[3 * i];
+      y[i] = p[3 * i + 1];
+      z[i] = p[3 * i + 2];
+    }
+    std::string terr;
+    int64_t restarts = 0;
+    try {
+      if (!imx::trmesh_rings(n, x.data(), y.data(), z.data(), nnb, nb, MX, &restarts, terr))
+        return imx::set_error(IMX_EINVAL, "imx_mesh_build_read: " + terr);
+    } catch (const std::bad_alloc &) {
+      return imx::set_error(IMX_ENOMEM, "imx_mesh_build_read: out of memory");
+    }
   }
-  std::fclose(f);
-  if (!ok) return imx::set_error(IMX_EIO, "imx_mesh_build_read: bad neighbour list in " + ngb);
   auto m = std::make_unique<imx_mesh>();
   m->kind = "read";
   m->glevel = -1;

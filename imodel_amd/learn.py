@@ -283,6 +283,7 @@ class Trainer:
         self.tsu = 1.0 / float(np.sqrt(np.mean(np.concatenate(dus) ** 2)))
         self.tsh = 1.0 / float(np.sqrt(np.mean(np.concatenate(dhs) ** 2)))
         self.learn.set_scaling(mu, sig, self.tsu, self.tsh)
+        self.learn_scaling = (mu.astype(np.float64), sig.astype(np.float64))
         # both scaled targets have unit mean square: RMS of the pair is sqrt(2)
         self.train_rms = float(np.sqrt(2.0))
         return mu, sig

@@ -192,7 +192,8 @@ int  imx_mesh_build_icos(int32_t glevel, int32_t pos, imx_mesh **out);
    then meshstruct): nodes from path (.xyz: count then x y z per line; .gmt:
    lon lat in degrees per line) and the neighbour rings from the .ngb file of
    the same name (6 indices per node, 0 for none). Without the .ngb file the
-   reference triangulates with STRIPACK, which is not part of this build. */
+   nodes are triangulated as the reference's STRIPACK trmesh does
+   (src/stripackm.f90:6843-7314); at most IMX_MAXNB neighbours per node. */
 int  imx_mesh_build_read(const char *path, imx_mesh **out);
 int  imx_mesh_desc_of(const imx_mesh *m, imx_mesh_desc *out);
 int  imx_mesh_gridprop(imx_mesh *m);             /* fills the optional arrays */
