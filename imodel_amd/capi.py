@@ -136,6 +136,9 @@ def lib():
             "imx_ic_tc2": (ctypes.c_int, [vp, ctypes.c_double, _F64P, _F64P]),
             "imx_ic_tc2_hc": (ctypes.c_int, [vp, ctypes.c_double, _F64P, _F64P]),
             "imx_ic_tc2_pv": (ctypes.c_int, [vp, ctypes.c_double, _F64P]),
+            "imx_ic_testcase": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                               ctypes.c_double, _F64P, _F64P, _F64P]),
+            "imx_swm_set_topography": (ctypes.c_int, [vp, _F64P]),
             "imx_ic_spectral": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_uint64,
                                                ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_double, _F64P, _F64P]),

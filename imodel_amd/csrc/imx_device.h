@@ -54,6 +54,10 @@ struct imx_swm {
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
+  // bumped whenever the step's kernel sequence or arguments change (scheme,
+  // tuning, topography, h_ref): graphs captured over the step by other
+  // objects (imx_learn) compare it with the value they captured at
+  uint64_t generation = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   imx::Alloc A;
   bool fast = false;
@@ -110,7 +114,11 @@ struct imx_swm {
   int32_t *t_ed = nullptr, *t_v = nullptr;
   int8_t *t_tg = nullptr;
   double *t_area = nullptr, *t_f = nullptr, *t_trhx = nullptr;
-  double *bt = nullptr;
+  double *bt = nullptr;  // [nv] bottom topography (internal order, FP64; zeros when none)
+  void *x_gb = nullptr;   // g b typed T for the fused kernels: x_gb_buf, or nullptr when b = 0
+  void *x_gb_buf = nullptr;
+  bool has_topo = false;
+  double topo_sb = 0.0, topo_sb2 = 0.0;  // sum A b, sum A b^2 (unit-sphere areas)
   int *flag = nullptr;
   double *part = nullptr, *diag = nullptr;
   int nparts = 0;

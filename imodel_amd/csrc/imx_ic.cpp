@@ -214,3 +214,290 @@ int imx_ic_spectral(const imx_mesh_desc *d, uint64_t seed, int32_t lmax, double 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------
+// The test cases of initialize_fields (src/swm.f90:1043-2414) that define a
+// state analytically or from a tabulated balance: 0, 1, 2, 5, 51, 6, 21,
+// 22, 23, 52, 53. Every expression keeps the reference's operation order,
+// its single-precision literals (0.4, 2., 7.0, epsilon(1.)) and the
+// single-rounded pi, so the states equal the reference's bit for bit.
+namespace imx {
+namespace {
+
+constexpr double kPi = 3.1415927410125732;  // constants.f90:31 (single-rounded)
+constexpr double kPio2 = kPi / 2.0;
+constexpr double kPiby2 = kPi * 0.5;
+constexpr double kDeg2rad = kPi / 180.0;
+constexpr double kEpsSingle = 1.1920928955078125e-07;  // epsilon(1.)
+constexpr double kGrav = 9.80616, kGravi = 1.0 / 9.80616, kOmega = 7.292e-5, kDay2sec = 86400.0;
+
+// real(8)**integer as the Fortran runtime evaluates it: binary powering
+// from 1 (a product by 1 is exact, so x**2 = x*x and x**3 = x*(x*x))
+double ipow(double b, int n) {
+  double r = 1.0;
+  for (;;) {
+    if (n & 1) r *= b;
+    n >>= 1;
+    if (n == 0) break;
+    b *= b;
+  }
+  return r;
+}
+
+// position, latitude, longitude and the direction u is measured along, at
+// edge l: ed%c and ed%tg (HTC) or edhx%c and edhx%nr (HC)
+struct EdgePoint {
+  const double *p, *dir;
+  double lat, lon;
+};
+
+EdgePoint edge_point(const imx_mesh &m, int l, bool hc) {
+  EdgePoint e;
+  if (hc) {
+    e.p = &m.edhx_c[3 * l];
+    e.dir = &m.edhx_nr[3 * l];
+    cart2sph(e.p[0], e.p[1], e.p[2], &e.lon, &e.lat);
+  } else {
+    e.p = &m.ed_c[3 * l];
+    e.dir = &m.ed_tg[3 * l];
+    e.lat = m.ed_clat[l];
+    e.lon = m.ed_clon[l];
+  }
+  return e;
+}
+
+double along(const EdgePoint &e, double vlon, double vlat) {
+  double v[3];
+  vec_sph2cart(vlon, vlat, e.p, v);
+  return dot3(v, e.dir);
+}
+
+// case(1): cosine bell of 1000 m radius erad/3 at (0, 0) in a solid-body
+// wind of one revolution per 12 days
+void tc1(const imx_mesh &m, bool hc, double erad, double *h, double *u) {
+  const double u0 = 2.0 * kPi * erad / (12.0 * kDay2sec);
+  const double h0 = 1000.0, rmax = erad / 3.0;
+  const double p0[3] = {std::cos(0.0) * std::cos(0.0), std::cos(0.0) * std::sin(0.0), std::sin(0.0)};
+  for (int i = 0; i < m.nv; ++i) {
+    const double r = erad * arclen(&m.v_p[3 * i], p0);
+    h[i] = (r < rmax) ? (h0 * 0.5) * (1.0 + std::cos(kPi * r / rmax)) : 0.0;
+  }
+  for (int l = 0; l < m.ne; ++l) {
+    const EdgePoint e = edge_point(m, l, hc);
+    u[l] = along(e, u0 * std::cos(e.lat), 0.0);
+  }
+}
+
+// case(2): steady zonal geostrophic flow
+void tc2(const imx_mesh &m, bool hc, double erad, double *h, double *u) {
+  const double u0 = 2.0 * kPi * erad / (12.0 * kDay2sec);
+  const double h0 = 2.94e4 * kGravi;
+  const double h_ct = (erad * kOmega * u0 + (u0 * u0) / 2.0) * kGravi;
+  for (int i = 0; i < m.nv; ++i) {
+    const double s = std::sin(m.v_lat[i]);
+    h[i] = h0 - h_ct * (s * s);
+  }
+  for (int l = 0; l < m.ne; ++l) {
+    const EdgePoint e = edge_point(m, l, hc);
+    u[l] = along(e, u0 * std::cos(e.lat), 0.0);
+  }
+}
+
+// case(5) / case(51): zonal flow over a conical (5) or Gaussian (51) mountain
+void tc5(const imx_mesh &m, bool hc, double erad, double hollgw, bool gauss, double *h, double *u, double *bt) {
+  const double u0 = 20.0, h0 = 5960.0;
+  const double h_ct = (erad * kOmega * u0 + (u0 * u0) / 2.0) * kGravi;
+  const double lon0 = -kPi * 0.5, lat0 = kPi / 6.0, rmax = kPi / 9.0;
+  if (!gauss) {
+    for (int i = 0; i < m.nv; ++i) {
+      const double lon = m.v_lon[i], lat = m.v_lat[i];
+      const double r = std::sqrt((lon - lon0) * (lon - lon0) + (lat - lat0) * (lat - lat0));
+      const double s = std::sin(lat);
+      const double hh = h0 - h_ct * (s * s);
+      bt[i] = (r < rmax) ? 2000.0 * (1.0 - r / rmax) : 0.0;
+      h[i] = hh - bt[i];
+    }
+  } else {
+    const double cl = std::cos(lat0);
+    const double p0[3] = {cl * std::cos(lon0), cl * std::sin(lon0), std::sin(lat0)};
+    const double c04 = (double)0.4f;  // default-real literal
+    for (int i = 0; i < m.nv; ++i) {
+      const double s = std::sin(m.v_lat[i]);
+      double b = h0 - h_ct * (s * s);
+      double d[3];
+      for (int q = 0; q < 3; ++q) d[q] = m.v_p[3 * i + q] - p0[q];
+      const double r = std::sqrt(dot3(d, d));
+      const double rmaxc = 2.0 * (1.0 - std::cos(rmax));
+      const double g = std::exp(-(c04 * (r * r) / (rmaxc * rmaxc)));
+      bt[i] = b + 20.0 * g;
+      h[i] = hollgw - 20.0 * g;
+    }
+  }
+  for (int l = 0; l < m.ne; ++l) {
+    const EdgePoint e = edge_point(m, l, hc);
+    u[l] = along(e, u0 * std::cos(e.lat), 0.0);
+  }
+}
+
+// case(6): Rossby-Haurwitz wave of wavenumber 4
+void tc6(const imx_mesh &m, bool hc, double erad, double *h, double *u) {
+  const double K_RH = 7.848e-6, w = 7.848e-6, h0 = 8.0e3;
+  const int M = 4;
+  auto h_tc6 = [&](double lon, double lat) {
+    const double c = std::cos(lat);
+    double atmp;
+    if (std::fabs(lat) > kPio2 - kEpsSingle) {
+      atmp = 0.0;
+    } else {
+      atmp = 0.5 * w * (2.0 * kOmega + w) * (c * c) +
+             0.25 * (K_RH * K_RH) * ipow(c, M * 2) *
+                 ((double)(M + 1) * (c * c) + (double)(2.0f * (float)(M * M) - (float)M - 2.0f) -
+                  (double)(2.0f * (float)(M * M)) / (c * c));
+    }
+    const double btmp = 2.0 * ((kOmega + w) * K_RH / (double)((M + 1) * (M + 2))) * ipow(c, M) *
+                        ((double)((float)(M * M) + 2.0f * (float)M + 2.0f) - (double)((M + 1) * (M + 1)) * (c * c));
+    const double ctmp = 0.25 * (K_RH * K_RH) * ipow(c, 2 * M) * ((double)(M + 1) * (c * c) - (double)(M + 2));
+    return h0 + kGravi * (erad * erad) *
+                    (atmp + btmp * std::cos((double)M * lon) + ctmp * std::cos((double)(2.0f * (float)M) * lon));
+  };
+  for (int l = 0; l < m.ne; ++l) {
+    const EdgePoint e = edge_point(m, l, hc);
+    const double c = std::cos(e.lat), s = std::sin(e.lat);
+    const double ut = erad * w * c + erad * K_RH * ipow(c, M - 1) * ((double)M * (s * s) - (c * c)) *
+                                         std::cos((double)M * e.lon);
+    const double vt = -(erad * K_RH * (double)M * ipow(c, M - 1) * s * std::sin((double)M * e.lon));
+    u[l] = along(e, ut, vt);
+  }
+  for (int i = 0; i < m.nv; ++i) h[i] = h_tc6(m.v_lon[i], m.v_lat[i]);
+}
+
+// case(21, 22, 23, 52, 53): Galewsky et al. barotropic jet, h in gradient
+// balance tabulated by trapezoidal integration in latitude and interpolated
+// (cubic, linear next to the poles); 21, 23 and 53 add the bump
+bool galewsky(const imx_mesh &m, bool hc, int tc, double erad, double *h, double *u, std::string &err) {
+  double u00, lat0, lat1;
+  if (tc == 23) {
+    u00 = 200.0;
+    lat0 = kPi / 7.0;
+    lat1 = kPi / 2.0 - lat0;
+  } else if (tc == 52 || tc == 53) {
+    u00 = 80.0;
+    lat0 = -(5.0 * kDeg2rad);
+    lat1 = -(45.0 * kDeg2rad);
+  } else {
+    u00 = 80.0;
+    lat0 = kPi / 7.0;
+    lat1 = kPi / 2.0 - lat0;
+  }
+  const double en = std::exp(-4.0 / ((lat1 - lat0) * (lat1 - lat0)));
+  const double umen = u00 / en;
+  const double rearth = erad, rotatn = kOmega;
+  const int nygg = 4 * (int)std::floor(std::sqrt((float)m.nv));
+  std::vector<double> hgg((size_t)nygg + 2, 0.0);  // 1-based, hgg[1..nygg+1]
+  const double dygg = kPi / (double)nygg;
+  double totvol = 0.0, totarea = 0.0;
+  hgg[1] = 0.0;
+  auto jet = [&](double l) {
+    const double den = (l - lat0) * (l - lat1);
+    return (den < 0.0) ? umen * std::exp(1.0 / den) : 0.0;
+  };
+  for (int j = 2; j <= nygg; ++j) {
+    const double l1 = (double)(j - 2) * dygg - kPiby2;
+    const double l2 = (double)(j - 1) * dygg - kPiby2;
+    double u1 = jet(l1), u2 = jet(l2);
+    u1 = u1 * (2.0 * rotatn * std::sin(l1) + std::tan(l1) * u1 / rearth);
+    u2 = u2 * (2.0 * rotatn * std::sin(l2) + std::tan(l2) * u2 / rearth);
+    hgg[j] = hgg[j - 1] - rearth * 0.5 * (u1 + u2) * dygg;
+    totarea = totarea + std::cos(l2) * dygg;
+    totvol = totvol + hgg[j] * std::cos(l2) * dygg;
+  }
+  hgg[nygg + 1] = hgg[nygg];
+  totvol = totvol / (totarea * kGrav);
+  for (int j = 1; j <= nygg + 1; ++j) hgg[j] = hgg[j] + (1.0e4 - totvol) * kGrav;
+  for (int j = 1; j <= nygg + 1; ++j) hgg[j] = hgg[j] * kGravi;
+  for (int i = 0; i < m.nv; ++i) {
+    const double l1 = m.v_lat[i] + kPiby2;
+    const int jy = (int)std::floor(l1 / dygg) + 1;
+    const double beta = (l1 - (double)(jy - 1) * dygg) / dygg;
+    if (jy < 1 || jy > nygg) {
+      // a node exactly at the north pole: the reference's cubic stencil
+      // reads past its table there
+      err = "Galewsky state: a node lies on the pole (latitude table index " + std::to_string(jy) + ")";
+      return false;
+    }
+    if (jy == 1 || jy == nygg) {
+      const double cc2 = 1.0 - beta, cc3 = beta;
+      h[i] = cc2 * hgg[jy] + cc3 * hgg[jy + 1];
+    } else {
+      const double cc1 = -(beta * (beta - 1.0) * (beta - 2.0) / 6.0);
+      const double cc2 = 0.5 * (beta + 1.0) * (beta - 1.0) * (beta - 2.0);
+      const double cc3 = -(0.5 * (beta + 1.0) * beta * (beta - 2.0));
+      const double cc4 = (beta + 1.0) * beta * (beta - 1.0) / 6.0;
+      h[i] = cc1 * hgg[jy - 1] + cc2 * hgg[jy] + cc3 * hgg[jy + 1] + cc4 * hgg[jy + 2];
+    }
+  }
+  for (int l = 0; l < m.ne; ++l) {
+    const EdgePoint e = edge_point(m, l, hc);
+    u[l] = along(e, jet(e.lat), 0.0);
+  }
+  if (tc == 21 || tc == 23 || tc == 53) {
+    const double hpert = 120.0, alpha = 1.0 / 3.0, beta = 1.0 / 15.0;
+    const double lat2 = (tc == 53) ? -(25.0 * kDeg2rad) : 0.5 * kPiby2;
+    for (int i = 0; i < m.nv; ++i) {
+      const double l2 = m.v_lat[i];
+      double l1 = m.v_lon[i];
+      const double clat = std::cos(l2);
+      if (tc == 53) l1 = l1 + 120.0 * kDeg2rad;
+      const double a = l1 / alpha, b = (lat2 - l2) / beta;
+      const double e1 = std::exp(-(a * a));
+      const double e2 = std::exp(-(b * b));
+      h[i] = h[i] + hpert * clat * e1 * e2;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+}  // namespace imx
+
+extern "C" {
+
+int imx_ic_testcase(const imx_mesh *m, int32_t tc, int32_t stag, double erad, double hollgw, double *h, double *u,
+                    double *bt) {
+  if (!m || !h || !u || !bt) return imx::set_error(IMX_EINVAL, "imx_ic_testcase: NULL argument");
+  if (stag != IMX_STAG_HTC && stag != IMX_STAG_HC) return imx::set_error(IMX_EINVAL, "imx_ic_testcase: bad stag");
+  const bool hc = stag == IMX_STAG_HC;
+  if (hc && (m->edhx_c.size() < (size_t)3 * m->ne || m->edhx_nr.size() < (size_t)3 * m->ne))
+    return imx::set_error(IMX_EINVAL, "imx_ic_testcase: the mesh has no Voronoi edge vectors");
+  if (m->v_lat.size() < (size_t)m->nv || m->ed_clat.size() < (size_t)m->ne)
+    return imx::set_error(IMX_EINVAL, "imx_ic_testcase: the mesh has no latitudes");
+  std::fill(bt, bt + m->nv, 0.0);
+  std::string err;
+  switch (tc) {
+    case 0:  // time discretisation check: h = cos(lon) cos(lat)^2, u = 0
+      for (int i = 0; i < m->nv; ++i) {
+        const double c = std::cos(m->v_lat[i]);
+        h[i] = std::cos(m->v_lon[i]) * (c * c);
+      }
+      std::fill(u, u + m->ne, 0.0);
+      break;
+    case 1: imx::tc1(*m, hc, erad, h, u); break;
+    case 2: imx::tc2(*m, hc, erad, h, u); break;
+    case 5: imx::tc5(*m, hc, erad, hollgw, false, h, u, bt); break;
+    case 51: imx::tc5(*m, hc, erad, hollgw, true, h, u, bt); break;
+    case 6: imx::tc6(*m, hc, erad, h, u); break;
+    case 21:
+    case 22:
+    case 23:
+    case 52:
+    case 53:
+      if (!imx::galewsky(*m, hc, tc, erad, h, u, err)) return imx::set_error(IMX_EINVAL, "imx_ic_testcase: " + err);
+      break;
+    default:
+      return imx::set_error(IMX_ENOTSUP, "imx_ic_testcase: test case " + std::to_string(tc) +
+                                             " is not part of this build (0, 1, 2, 5, 6, 21, 22, 23, 51, 52, 53 are)");
+  }
+  return IMX_OK;
+}
+
+}  // extern "C"

@@ -64,6 +64,7 @@ struct imx_learn {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   const short *gX = nullptr;  // MLP input buffer the graph was captured with
+  uint64_t gen = 0;           // the step handle's generation at capture
   // device sample store: normalised features, scaled targets
   int cap = 0, count = 0;
   int64_t xstride = 0;      // shorts per stored feature matrix
@@ -359,7 +360,9 @@ static int learn_sample(imx_learn *L) {
 static int learn_ensure_graph(imx_learn *L) {
   int rc = mlp_reserve(L->m, L->NE);
   if (rc) return rc;
-  if (L->gexec && L->gX == L->m->X) return IMX_OK;
+  // recapture when the MLP scratch moved or the handle's step changed
+  // (scheme, tuning, topography, h_ref)
+  if (L->gexec && L->gX == L->m->X && L->gen == L->s->generation) return IMX_OK;
   learn_drop_graph(L);
   hipStream_t st = L->s->stream;
   LTRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "learn graph capture");
@@ -380,6 +383,7 @@ static int learn_ensure_graph(imx_learn *L) {
   L->graph = gr;
   LTRY(hipGraphInstantiate(&L->gexec, L->graph, nullptr, nullptr, 0), "learn graph instantiate");
   L->gX = L->m->X;
+  L->gen = L->s->generation;
   return IMX_OK;
 }
 

@@ -819,8 +819,8 @@ __global__ __launch_bounds__(BS) void fast_cell(
     const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ pxw,
     const T *__restrict__ pxq, V4<T> *__restrict__ pxab, T *__restrict__ pxqe, const V4<T> *__restrict__ apv,
     const V4<T> *__restrict__ apg, T pvsdt,
-    const T *__restrict__ hs, const T *__restrict__ us, double href, T grav, T inv_a,
-    T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
+    const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
+    T inv_a, T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
     double *__restrict__ dh_out) {
   const int c = ent_index<BS>(remap);
@@ -893,7 +893,9 @@ __global__ __launch_bounds__(BS) void fast_cell(
     Kd = kh[c] * (T)(du * c_iaa[c]);
   }
   if (OPT >= 1 && hd) Kd -= hd[c];  // hyperdiffusion: - grad(K div lapu)
-  phi[c] = (grav * hc + K - Kd) * inv_a;
+  T pot = grav * hc;
+  if (gb) pot += gb[c];  // bottom topography: g (h + b)
+  phi[c] = (pot + K - Kd) * inv_a;
 
   // PV at the cell's edges: edge j lies between ring triangles j-1 and j
   T qe[MX];
@@ -1090,7 +1092,7 @@ __device__ void block_store(double (&v)[NPART], double *part) {
   }
 }
 
-// slot 0: sum A h, 1: sum A, 2: sum A g h (h/2), 3: sum A divu^2, 7: bad
+// slot 0: sum A h, 1: sum A, 2: sum A g h (h/2 + b), 3: sum A divu^2, 7: bad
 template <typename T>
 __global__ __launch_bounds__(TPB) void diag_cell(int off, int cnt, int nv, const int32_t *__restrict__ c_nnb,
                                                  const int32_t *__restrict__ c_ed,
@@ -1099,7 +1101,7 @@ __global__ __launch_bounds__(TPB) void diag_cell(int off, int cnt, int nv, const
                                                  const double *__restrict__ e_dleng,
                                                  const T *__restrict__ hs,
                                                  const T *__restrict__ us, double href,
-                                                 double grav, double erad,
+                                                 const double *__restrict__ bt, double grav, double erad,
                                                  double *__restrict__ part) {
   double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int i = off + blockIdx.x * TPB + threadIdx.x;
@@ -1114,7 +1116,7 @@ __global__ __launch_bounds__(TPB) void diag_cell(int off, int cnt, int nv, const
     const double dvu = s / A / erad;
     v[0] = A * h;
     v[1] = A;
-    v[2] = A * grav * h * (h * 0.5);
+    v[2] = A * grav * h * (h * 0.5 + bt[i]);
     v[3] = A * (dvu * dvu);
     v[7] = isfinite(h) ? 0.0 : 1.0;
   }
@@ -1366,7 +1368,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->f_px ? (const V4<T> *)s->x_ap_wc : nullptr, s->f_px ? (const T *)s->x_px_wq : nullptr,    \
       (V4<T> *)s->x_px_ab, (T *)s->x_px_qe,                                                        \
       s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
-      hs, us, s->href,                                                                            \
+      hs, us, s->href, (const T *)s->x_gb,                                                        \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
   if (s->f_apvm || s->f_px) {
@@ -1420,7 +1422,7 @@ static void launch_diag(imx_swm *s, const T *hs, const T *us, double href, int c
   const int nc = c1 - c0, ne = 3 * nc, nt = 2 * nc;
   const int gv = (nc + TPB - 1) / TPB, ge = (ne + TPB - 1) / TPB, gt = (nt + TPB - 1) / TPB;
   diag_cell<T><<<gv, TPB, 0, st>>>(c0, nc, s->nv, s->c_nnb, s->c_ed, s->sc_hc ? s->hc_c : s->c_ttg, s->c_area,
-                                   s->e_dleng, hs, us, href, s->o.grav, s->o.erad, s->part);
+                                   s->e_dleng, hs, us, href, s->bt, s->o.grav, s->o.erad, s->part);
   diag_edge<T><<<ge, TPB, 0, st>>>(3 * c0, ne, s->NE, s->e_v, s->e_leng, s->e_dleng, hs, us, href,
                                    s->part + (size_t)gv * NPART);
   diag_tri<T><<<gt, TPB, 0, st>>>(2 * c0, nt, s->NT, s->t_ed, s->sc_hc ? s->hc_t : s->t_tg, s->t_v, s->t_area, s->t_f,
@@ -2317,6 +2319,8 @@ static int download_state(imx_swm *s, double *h, double *u, const T *hd, const T
   return IMX_OK;
 }
 
+static void drop_graph(imx_swm *s);
+
 static void set_href(imx_swm *s, const double *h) {
   if (s->href_set || !s->fast) return;
   double sa = 0.0, sh = 0.0;
@@ -2330,6 +2334,7 @@ static void set_href(imx_swm *s, const double *h) {
   }
   s->href = sh / sa;
   s->href_set = true;
+  drop_graph(s);  // a step captured before holds href = 0
 }
 
 int swm_stencil_rings(const imx_swm *s) {
@@ -2337,6 +2342,7 @@ int swm_stencil_rings(const imx_swm *s) {
 }
 
 static void drop_graph(imx_swm *s) {
+  ++s->generation;  // graphs captured elsewhere over this handle's step are stale too
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   s->gexec = nullptr;
@@ -2375,8 +2381,16 @@ static int diag_range(imx_swm *s, int c0, int c1, double *out) {
   const double mass = v[0] / v[1];
   const double pe = v[2];
   const double ke = v[4];
-  // unavailable potential energy of the flat state at the mean height (b = 0)
-  const double upe = v[1] * s->o.grav * mass * (mass * 0.5);
+  // unavailable potential energy: the fluid at rest at the mean height m of
+  // h + b (calc_energies, src/swm.f90:2706-2715):
+  // sum A g (m - b)((m - b)/2 + b) = g/2 (m^2 sum A - sum A b^2)
+  double upe;
+  if (s->has_topo) {
+    const double m = (v[0] + s->topo_sb) / v[1];
+    upe = 0.5 * s->o.grav * (m * m * v[1] - s->topo_sb2);
+  } else {
+    upe = v[1] * s->o.grav * mass * (mass * 0.5);
+  }
   out[IMX_DIAG_MASS] = mass;
   out[IMX_DIAG_PENERGY] = pe;
   out[IMX_DIAG_KENERGY] = ke;
@@ -2492,6 +2506,57 @@ int imx_swm_set_state(imx_swm *s, const double *h, const double *u) {
   imx::set_href(s, h);
   if (s->single) return imx::upload_state<float>(s, h, u, (float *)s->h0, (float *)s->u0);
   return imx::upload_state<double>(s, h, u, (double *)s->h0, (double *)s->u0);
+}
+
+int imx_swm_set_topography(imx_swm *s, const double *bt) {
+  if (!s) return imx::set_error(IMX_EINVAL, "imx_swm_set_topography: NULL handle");
+  std::vector<double> b(s->nv, 0.0);
+  double sb = 0.0, sb2 = 0.0;
+  bool any = false;
+  if (bt) {
+    std::vector<double> area(s->nv);
+    HIPTRY(hipMemcpy(area.data(), s->c_area, s->nv * sizeof(double), hipMemcpyDeviceToHost), "topography");
+    for (int i = 0; i < s->nv; ++i) {
+      if (!std::isfinite(bt[i])) return imx::set_error(IMX_EINVAL, "imx_swm_set_topography: non-finite b");
+      const int k = s->lay.cell_new[i];
+      b[k] = bt[i];
+      any = any || bt[i] != 0.0;
+    }
+    for (int k = 0; k < s->nv; ++k) {  // internal order: the same sums for every caller numbering
+      sb += area[k] * b[k];
+      sb2 += area[k] * (b[k] * b[k]);
+    }
+  }
+  HIPTRY(hipMemcpyAsync(s->bt, b.data(), s->nv * sizeof(double), hipMemcpyHostToDevice, s->stream), "topography");
+  if (s->fast) {
+    if (any && !s->x_gb_buf) {
+      s->x_gb_buf = s->single ? (void *)s->A.get<float>(s->nv) : (void *)s->A.get<double>(s->nv);
+      if (s->A.err != hipSuccess) return imx::set_error(IMX_ENOMEM, "imx_swm_set_topography: out of device memory");
+    }
+    if (any) {
+      const double g = s->o.grav;
+      if (s->single) {
+        std::vector<float> gb(s->nv);
+        for (int k = 0; k < s->nv; ++k) gb[k] = (float)(g * b[k]);
+        HIPTRY(hipMemcpyAsync(s->x_gb_buf, gb.data(), gb.size() * sizeof(float), hipMemcpyHostToDevice, s->stream),
+               "topography");
+        HIPTRY(hipStreamSynchronize(s->stream), "topography");
+      } else {
+        std::vector<double> gb(s->nv);
+        for (int k = 0; k < s->nv; ++k) gb[k] = g * b[k];
+        HIPTRY(hipMemcpyAsync(s->x_gb_buf, gb.data(), gb.size() * sizeof(double), hipMemcpyHostToDevice, s->stream),
+               "topography");
+        HIPTRY(hipStreamSynchronize(s->stream), "topography");
+      }
+    }
+  }
+  HIPTRY(hipStreamSynchronize(s->stream), "topography");
+  s->has_topo = any;
+  s->topo_sb = sb;
+  s->topo_sb2 = sb2;
+  s->x_gb = (s->fast && any) ? s->x_gb_buf : nullptr;
+  imx::drop_graph(s);  // the captured step holds the previous topography pointer
+  return IMX_OK;
 }
 
 int imx_swm_get_state(imx_swm *s, double *h, double *u) {

@@ -142,6 +142,17 @@ class Mesh:
         capi.check(fn(self._handle, erad, capi.f64p(h), capi.f64p(u)), "imx_ic_tc2")
         return h, u
 
+    def testcase(self, tc, stag="HTC", erad=ERAD_REF, hollgw=2.0):
+        """initialize_fields of test case tc (0, 1, 2, 5, 6, 21, 22, 23, 51,
+        52, 53) on a library mesh: (h, u, bt), u for HTC or HC staggering;
+        hollgw is par/swm.par's Hollingsworth depth (test case 51)."""
+        if self._handle is None:
+            raise ValueError("testcase needs a library-built mesh")
+        h, u, bt = np.zeros(self.nv), np.zeros(self.ne), np.zeros(self.nv)
+        capi.check(capi.lib().imx_ic_testcase(self._handle, int(tc), capi.STAG[stag], float(erad), float(hollgw),
+                                              capi.f64p(h), capi.f64p(u), capi.f64p(bt)), "imx_ic_testcase")
+        return h, u, bt
+
     def spectral(self, seed, lmax=16, h0=H0_GEO, hfrac=0.05, urms=20.0, erad=ERAD_REF):
         """Seeded spectral state (h, u)."""
         h = np.zeros(self.nv)
@@ -210,6 +221,14 @@ class Swm:
         h = np.ascontiguousarray(h, dtype=np.float64)
         u = np.ascontiguousarray(u, dtype=np.float64)
         capi.check(capi.lib().imx_swm_set_state(self._h, capi.f64p(h), capi.f64p(u)), "imx_swm_set_state")
+
+    def set_topography(self, bt):
+        """Bottom topography b [nv] (None: b = 0); h stays the depth."""
+        if bt is None:
+            capi.check(capi.lib().imx_swm_set_topography(self._h, None), "imx_swm_set_topography")
+            return
+        bt = np.ascontiguousarray(bt, dtype=np.float64)
+        capi.check(capi.lib().imx_swm_set_topography(self._h, capi.f64p(bt)), "imx_swm_set_topography")
 
     def get_state(self):
         h = np.zeros(self.nv)

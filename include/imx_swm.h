@@ -255,6 +255,13 @@ int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out
 int  imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc);
 int  imx_swm_set_state(imx_swm *s, const double *h, const double *u);
 int  imx_swm_get_state(imx_swm *s, double *h, double *u);
+/* Bottom topography bt%f [nv] in caller order (module global bt of swm_data,
+   set by initialize_fields, e.g. the mountain of test case 5,
+   src/swm.f90:1327-1377): the Bernoulli potential becomes g (h + b) + K
+   (src/swm.f90:2531-2536) and the potential and available energies take b
+   (calc_energies, :2686-2752). NULL restores b = 0. h stays the fluid
+   depth. */
+int  imx_swm_set_topography(imx_swm *s, const double *bt);
 /* subroutine tendency(h, u, masseq, momeq)       src/swm.f90:2416-2598 */
 int  imx_swm_tendency(imx_swm *s, const double *h, const double *u,
                       double *masseq, double *momeq);

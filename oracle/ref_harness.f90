@@ -366,6 +366,7 @@ program ref_harness
   character(len=16) :: sc_stag = 'HTC', sc_wrap = 'trsk10', sc_diffus = 'const', sc_hdiffus = 'const'
   character(len=16) :: sc_pv = 'none'
   real(r8) :: sc_k2 = 0._r8, sc_k4 = 0._r8, sc_pvspar = 0.5_r8
+  integer :: sc_tc = 2
 
   call get_command_argument(1, cmd)
   call execute_command_line('mkdir -p par grid data')
@@ -459,7 +460,7 @@ program ref_harness
     call initialize_gridprop()
     call initialize_fields()
     call setup_diffusion()
-    if (trim(statesrc) /= 'tc2') then
+    if (trim(statesrc) /= 'tc2' .and. trim(statesrc) /= 'tc') then
       call bin_read(trim(statesrc))
       hst = getr('h'); ust = getr('u')
       if (size(hst) /= mesh%nv .or. size(ust) /= mesh%ne) stop 'run: state size mismatch'
@@ -468,7 +469,7 @@ program ref_harness
     end if
 
     call bin_open_w(trim(outf))
-    call put_f8('h0', h%f); call put_f8('u0', u%f)
+    call put_f8('h0', h%f); call put_f8('u0', u%f); call put_f8('bt', bt%f)
 
     !One tendency at the initial state (swm.f90:214) and its intermediates
     call tendency(h, u, masseq%f, momeq%f)
@@ -581,7 +582,7 @@ program ref_harness
 
   case default
     print*, 'usage: ref_harness build <glevel> <eqs|pol> <out.bin>'
-    print*, '       ref_harness run <build:G:pos|mesh.bin> <tc2|state.bin> <nsteps> <dt> <diag> <out.bin>'
+    print*, '       ref_harness run <build:G:pos|mesh.bin> <tc|tc2|state.bin> <nsteps> <dt> <diag> <out.bin> [tc=N,...]'
     print*, '       ref_harness loadref <name> <out.bin>   (reads grid/<name>_*.dat)'
     print*, '       ref_harness plot <glevel> <field.bin> <pos>   (data/plottest_<grid>.dat)'
     stop 1
@@ -649,6 +650,7 @@ contains
       case ('hyperdiffus'); sc_hdiffus = val
       case ('pv'); sc_pv = val
       case ('pvspar'); read(val, *) sc_pvspar
+      case ('tc'); read(val, *) sc_tc
       case default
         print*, 'unknown scheme key ', trim(key)
         stop 1
@@ -692,7 +694,7 @@ contains
     open(newunit=pu, file='par/swm.par', status='replace', iostat=ios)
     write(pu, '(a)') '!c'
     write(pu, '(a)') '!c'
-    write(pu, '(a)') '2 0'
+    write(pu, '(i0,a)') sc_tc, ' 0'
     write(pu, '(a)') '!c'
     write(pu, '(a)') '1 1'
     write(pu, '(a)') '!c'

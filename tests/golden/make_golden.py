@@ -16,6 +16,10 @@ outputs as fixtures:
                             const/align/diam coefficients; the pxt16 and
                             gass18 wrappers; HC staggering), plus v_hx,
                             q_grad_ed/tr, dif_coef_hx/tr, lapu, lap_lapu
+  ref_run_g3_tc<N>.bin      initialize_fields' test case N (with its bottom
+                            topography bt): tendency intermediates, diag,
+                            3 RK4 steps dt=900 with diagnostics
+                            (``python make_golden.py tc`` makes only these)
   ref_lte_g3_<scheme>.bin   swm_horiz_loc_trunc_er on TC2 with the options of
                             LTE_SCHEMES: lte_<row> = error_calc triple of each
                             row of its table, mim = the mimetic sums it prints
@@ -110,9 +114,27 @@ def run(args, cwd):
         raise RuntimeError(" ".join(args) + "\n" + r.stdout[-2000:])
 
 
+# test-case fixtures: name -> par/swm.par options (test case, staggering)
+TESTCASES = {
+    "tc1": "tc=1",
+    "tc5": "tc=5",
+    "tc5hc": "tc=5,stag=HC",
+    "tc51": "tc=51",
+    "tc6": "tc=6",
+    "tc21": "tc=21",
+    "tc53": "tc=53",
+}
+
+
 def main():
     if not os.access(HARNESS, os.X_OK):
         sys.exit("build the reference harness first: make -f oracle/Makefile.ref")
+    with tempfile.TemporaryDirectory() as td:
+        for name, spec in TESTCASES.items():
+            run(["run", "build:3:eqs", "tc", "3", "900", "1", "out.bin", spec], td)
+            shutil.copy(os.path.join(td, "out.bin"), os.path.join(HERE, f"ref_run_g3_{name}.bin"))
+    if sys.argv[1:] == ["tc"]:
+        return
     with tempfile.TemporaryDirectory() as td:
         for name, spec in LTE_SCHEMES.items():
             run_lte(spec, td, os.path.join(HERE, f"ref_lte_g3_{name}.bin"))

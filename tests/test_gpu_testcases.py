@@ -1,0 +1,126 @@
+"""initialize_fields' test cases with bottom topography on the GPU.
+
+* FP64_REF: the tendency, its intermediates and 3 RK4 steps equal the
+  reference's fixtures bit for bit for test cases 1, 5 (HTC and HC), 51, 6,
+  21 and 53, with the state and topography from imx_ic_testcase.
+* the fused FP64 and FP32 paths against the oracle: the tendency, the
+  diagnostics with b, and one simulated day of test case 5 at glevel 4.
+"""
+import numpy as np
+import pytest
+
+from imodel_amd.swm import Mesh, Swm
+from oracle.oracle import Oracle
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CASES = {"tc1": (1, "HTC"), "tc5": (5, "HTC"), "tc5hc": (5, "HC"), "tc51": (51, "HTC"), "tc6": (6, "HTC"),
+         "tc21": (21, "HTC"), "tc53": (53, "HTC")}
+FIELDS = ["h_ed", "uh", "eta", "h_tr", "q_tr", "q_ed", "divuh", "ke_hx", "ghbK", "grad_ghbK", "uhq_perp"]
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def g3():
+    return Mesh.icos(3)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_ref_testcase_bitexact(g3, golden, name):
+    say("fp64_ref test case", name)
+    tc, stag = CASES[name]
+    r = golden(f"ref_run_g3_{name}.bin")
+    h, u, bt = g3.testcase(tc, stag)
+    with Swm(g3, precision="fp64_ref", dt=900.0, testcase=tc) as s:
+        if stag == "HC":
+            s.set_scheme(stag="HC")
+        s.set_topography(bt)
+        m, f = s.tendency(h, u)
+        assert np.array_equal(m, r["masseq"]) and np.array_equal(f, r["momeq"])
+        for k in FIELDS:
+            assert np.array_equal(s.field(k.lower()), r[k]), k
+        s.set_state(h, u)
+        s.step(3)
+        hg, ug = s.get_state()
+    assert np.array_equal(hg, r["h"]) and np.array_equal(ug, r["u"])
+
+
+@pytest.mark.parametrize("name", ["tc5", "tc5hc", "tc6", "tc21"])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 5e-5)])
+def test_fast_testcase_tendency_vs_oracle(name, prec, tol):
+    say("fast tendency", name, prec)
+    tc, stag = CASES[name]
+    m = Mesh.icos(4)
+    h, u, bt = m.testcase(tc, stag)
+    o = Oracle(m.arrays, testcase=tc, scheme={"stag": stag} if stag == "HC" else None)
+    o.bt = bt
+    mo, fo = o.tendency(h, u)
+    with Swm(m, precision=prec, testcase=tc) as s:
+        if stag == "HC":
+            s.set_scheme(stag="HC")
+        s.set_topography(bt)
+        mg, fg = s.tendency(h, u)
+    assert rel_l2(mg, mo) < tol and rel_l2(fg, fo) < tol, (rel_l2(mg, mo), rel_l2(fg, fo))
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64_ref", 1e-12), ("fp64", 1e-12), ("fp32", 1e-6)])
+def test_diagnostics_with_topography(prec, tol):
+    say("diagnostics with topography", prec)
+    m = Mesh.icos(4)
+    h, u, bt = m.testcase(5)
+    o = Oracle(m.arrays, testcase=5)
+    o.bt = bt
+    d_o = o.diagnostics(h, u, refresh=True)
+    with Swm(m, precision=prec, testcase=5) as s:
+        s.set_topography(bt)
+        s.set_state(h, u)
+        d = s.diagnostics()
+        s.set_topography(None)
+        d_flat = s.diagnostics()
+    for k, name in ((0, "mass"), (1, "penergy"), (2, "kenergy"), (3, "tenergy")):
+        assert abs(d[name] - d_o[k]) <= tol * abs(d_o[k]), (name, d[name], d_o[k])
+    assert abs(d["aenergy"] - d_o[4]) <= tol * abs(d_o[3]), (d["aenergy"], d_o[4])
+    assert d_flat["penergy"] < d["penergy"]  # b > 0 raises the potential energy
+
+
+@pytest.mark.parametrize("prec,tol_h,tol_u", [("fp64", 1e-11, 1e-10), ("fp32", 1e-6, 1e-4)])
+def test_fast_tc5_one_day_vs_oracle(prec, tol_h, tol_u):
+    """Flow over the mountain, glevel 4, 96 steps of 900 s."""
+    say("fast tc5 day", prec)
+    m = Mesh.icos(4)
+    h, u, bt = m.testcase(5)
+    o = Oracle(m.arrays, testcase=5)
+    o.bt = bt
+    ho, uo = o.step(h, u, 900.0, 96)
+    with Swm(m, precision=prec, dt=900.0, testcase=5) as s:
+        s.set_topography(bt)
+        s.set_state(h, u)
+        s.step(96)
+        hg, ug = s.get_state()
+        dg = s.diagnostics()
+    A = m.arrays["hx_areag"]
+    say(prec, "h", rel_l2(hg, ho, A), "u", rel_l2(ug, uo))
+    assert rel_l2(hg, ho, A) < tol_h and rel_l2(ug, uo) < tol_u
+    assert dg["badflag"] == 0
+
+
+def test_topography_off_restores_plain_step():
+    say("topography off")
+    m = Mesh.icos(3)
+    h, u = m.spectral(3)
+    out = []
+    for topo in (False, True):
+        with Swm(m, precision="fp32", dt=900.0) as s:
+            s.set_state(h, u)
+            s.step(1)  # graph captured without topography
+            if topo:
+                s.set_topography(np.full(m.nv, 100.0))
+                s.set_topography(None)
+            s.set_state(h, u)
+            s.step(3)
+            out.append(s.get_state())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
