@@ -15,6 +15,9 @@ FSRC = os.path.join(HERE, "fortran")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libimx.so")
 IMODEL = os.path.join(FSRC, "imodel")
+# a test program against the swm call surface (tests/fortran), built beside it
+CALLER_SRC = os.path.join(os.path.dirname(HERE), "tests", "fortran", "swm_caller.f90")
+CALLER = os.path.join(os.path.dirname(HERE), "tests", "fortran", "swm_caller")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("FC_IMX", "/opt/rocm/bin/amdflang")
@@ -30,6 +33,8 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 HOST_SRCS = ["imx_mesh.cpp", "imx_trmesh.cpp", "imx_ic.cpp", "imx_sfc.cpp", "imx_gridio.cpp", "imx_plot.cpp", "imx_jacobian.cpp",
              "imx_lte.cpp"]
 HIP_SRCS = ["imx_swm.hip", "imx_mlp.hip", "imx_learn.hip"]
+# the Fortran host's modules, in dependency order
+FORTRAN_SRCS = ("datastruct.f90", "imx_c.f90", "swm_par.f90", "smeshpack.f90", "swm_data.f90", "swm.f90")
 
 
 def _run(cmd):
@@ -60,16 +65,26 @@ def build_lib():
 
 
 def build_fortran():
-    """The Fortran host (imodel driver + swm module) over the C ABI."""
+    """The Fortran host (imodel driver + swm module over the C ABI), and the
+    test program written against the swm call surface."""
     if not os.path.isdir(FSRC):
         return None
-    srcs = [os.path.join(FSRC, f) for f in ("imx_c.f90", "swm_par.f90", "swm.f90", "imodel.f90")]
+    srcs = [os.path.join(FSRC, f) for f in FORTRAN_SRCS]
     if not all(os.path.exists(s) for s in srcs):
         return None
     mod = os.path.join(BUILD, "fmod")
     os.makedirs(mod, exist_ok=True)
-    _run([FC, "-O2", "-module-dir", mod, "-I", mod, *srcs, "-o", IMODEL, f"-L{HERE}", "-limx",
-          f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."])
+    objs = []
+    for src in srcs:  # in dependency order: each module's .mod before its users
+        o = os.path.join(BUILD, os.path.basename(src).replace(".f90", ".fo"))
+        _run([FC, "-O2", "-module-dir", mod, "-I", mod, "-c", src, "-o", o])
+        objs.append(o)
+    link = [f"-L{HERE}", "-limx", f"-Wl,-rpath,{HERE}"]
+    _run([FC, "-O2", "-I", mod, os.path.join(FSRC, "imodel.f90"), *objs, "-o", IMODEL, *link,
+          "-Wl,-rpath,$ORIGIN/.."])
+    if os.path.exists(CALLER_SRC):
+        _run([FC, "-O2", "-I", mod, CALLER_SRC, *objs, "-o", CALLER, *link,
+              "-Wl,-rpath,$ORIGIN/../../imodel_amd"])
     return IMODEL
 
 

@@ -9,7 +9,7 @@ module imx_c
   implicit none
 
   integer(c_int32_t), parameter :: IMX_PREC_FP64_REF = 0, IMX_PREC_FP64 = 1, IMX_PREC_FP32 = 2
-  integer(c_int32_t), parameter :: IMX_FIELD_ETA = 2, IMX_FIELD_Q_TR = 4, IMX_FIELD_Q_ED = 5, &
+  integer(c_int32_t), parameter :: IMX_FIELD_H_ED = 0, IMX_FIELD_ETA = 2, IMX_FIELD_Q_TR = 4, IMX_FIELD_Q_ED = 5, &
                                    IMX_FIELD_DIVUH = 6, IMX_FIELD_KE_HX = 7, IMX_FIELD_DIVU = 11
   integer(c_int32_t), parameter :: IMX_POS_EQS = 0, IMX_POS_POL = 1
   integer, parameter :: IMX_NDIAG = 8
@@ -147,6 +147,20 @@ module imx_c
       integer(c_int32_t), value :: lmax
       real(c_double), value :: h0, hfrac, urms, erad
       real(c_double) :: h(*), u(*)
+      integer(c_int) :: rc
+    end function
+    function imx_ic_testcase(m, tc, stag, erad, hollgw, h, u, bt) bind(C, name='imx_ic_testcase') result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: m
+      integer(c_int32_t), value :: tc, stag
+      real(c_double), value :: erad, hollgw
+      real(c_double) :: h(*), u(*), bt(*)
+      integer(c_int) :: rc
+    end function
+    function imx_swm_set_topography(s, bt) bind(C, name='imx_swm_set_topography') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: s
+      real(c_double) :: bt(*)
       integer(c_int) :: rc
     end function
     subroutine imx_swm_default_opts(o) bind(C, name='imx_swm_default_opts')

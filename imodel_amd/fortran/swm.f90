@@ -1,80 +1,86 @@
 !=======================================================================
-! swm: the shallow-water module of the imodel driver, on the GPU.
-! Keeps the reference call surface (src/swm.f90): swm_tests (:102-380),
-! swm_horiz_loc_trunc_er (:382-734),
-! tendency(h, u, masseq, momeq) (:2416-2598), ode_rk4 (:2600-2683); the
-! state lives in HBM behind one libimx handle (the reference keeps it in
-! the swm_data module globals, src/swm_data.f90:31-334).
+! swm: the shallow-water module of the imodel driver, on the GPU, with the
+! reference's call surface (src/swm.f90):
+!   swm_tests(meshtmp)                        :102-380
+!   swm_horiz_loc_trunc_er(meshtmp)           :382-734
+!   initialize_gridprop()                     :979-1041
+!   initialize_fields()                       :1043-2414
+!   tendency(h, u, masseq, momeq)             :2416-2598
+!   ode_rk4(t, h, u, h_new, u_new, dt)        :2600-2683
+!   calc_energies(P, K, T, A)                 :2686-2752
+!   sumf_areas(f), sumfsq_areas(f)            :3394-3490
+!   error_calc(f, fref, fdif, emr, e2, em)    :3492-3543
+! with the types of module datastruct and the module globals of swm_data
+! (mesh, h, u, bt, masseq, momeq, ...), so a caller written against the
+! reference (e.g. moist_swm's `use swm, only: initialize_gridprop,
+! sumf_areas, calc_energies, tendency, error_calc`, src/moist_swm.f90:61-66)
+! compiles against it. The state lives in HBM behind one libimx handle,
+! opened by initialize_gridprop.
 !=======================================================================
 module swm
   use iso_c_binding
   use imx_c
-  use swm_par
+  use datastruct
+  use swm_data
   implicit none
+  private
+  public :: swm_tests, swm_horiz_loc_trunc_er, initialize_gridprop, initialize_fields, tendency, ode_rk4, &
+            calc_energies, sumf_areas, sumfsq_areas, error_calc
   real(r8), parameter :: erad = 6.37122e6_r8   ! constants.f90:55
   real(r8), parameter :: grav = 9.80616_r8     ! constants.f90:61
   real(r8), parameter :: omega = 7.292e-5_r8   ! constants.f90:67
   real(r8), parameter :: rad2deg = 57.295779513082323_r8
 
   real(r8), parameter :: sec2day = 1._r8/86400._r8
-  type(c_ptr) :: handle = c_null_ptr
   type(imx_mesh_desc) :: mdesc
-  real(r8), allocatable :: h(:), u(:)
-  ! analytic reference solution (RefSolAnal, src/swm_data.f90:685): TC2
+  ! an FP64 reference-order handle on the same mesh and scheme, for the
+  ! reference's intermediate fields (div u, K, h_ed) of a state evolved on a
+  ! fused handle
+  type(c_ptr) :: diag_handle = c_null_ptr
+  ! analytic reference solution (RefSolAnal, src/swm_data.f90:685)
   logical :: refsol = .false.
   integer :: blowup = 0, plotsteps = 1
-  real(r8), allocatable :: h_exact(:), u_exact(:), q_exact(:), qtr(:)
+  real(r8), allocatable :: q_exact(:), qtr(:)
   character(len=256) :: swmname = '', meshname = ''
   real(r8) :: cfl = 0._r8, meanvdist = 0._r8
 
 contains
 
-  subroutine swm_tests(meshp, glevel)
-    type(c_ptr), intent(in) :: meshp
-    integer, intent(in) :: glevel
+  subroutine swm_tests(meshtmp)
+    type(grid_structure), intent(in) :: meshtmp
     real(c_double) :: d0(IMX_NDIAG), d(IMX_NDIAG)
     real(r8), pointer :: leng(:)
     real(r8) :: maxvel, minvdist, time
     integer :: k, kn
 
-    call read_swm_par(glevel)
-    call imx_check(imx_mesh_desc_of(meshp, mdesc), 'imx_mesh_desc_of')
-    allocate(h(mdesc%nv), u(mdesc%ne))
-    if (testcase == 2) then
-      if (trim(stag) == 'HC') then
-        call imx_check(imx_ic_tc2_hc(meshp, erad, h, u), 'imx_ic_tc2_hc')
-      else
-        call imx_check(imx_ic_tc2(meshp, erad, h, u), 'imx_ic_tc2')
-      end if
-    else
-      ! seeded spectral state (new test case 100): h' 5% of H0, 20 m/s rms wind
-      call imx_check(imx_ic_spectral(mdesc, int(seed, c_int64_t), 16_c_int32_t, 2.94e4_r8/grav, &
-                                     0.05_r8, 20._r8, erad, h, u), 'imx_ic_spectral')
-    end if
-    maxvel = maxval(abs(u))
+    mesh = meshtmp
+    call swmpars(usetime=.true.)
+    call allocate_globalswmvars()
+    call initialize_gridprop()
+    call initialize_fields()
+    maxvel = maxval(abs(u%f))
     call c_f_pointer(mdesc%ed_leng, leng, [mdesc%ne])
     minvdist = minval(leng)
     meanvdist = sum(leng)/mdesc%ne
     cfl = abs(maxvel*dt/(minvdist*erad))   ! swm.f90:2383
     print*, "CFL:", cfl
 
-    call make_names(glevel, .true.)
+    call make_names(mesh%glevel, .true.)
 
-    call open_handle()
-    call imx_check(imx_swm_set_state(handle, h, u), 'imx_swm_set_state')
     if (normal_modes == 1) then
       call swm_normal_mode_analysis()
-      call imx_swm_destroy(handle)
-      handle = c_null_ptr
+      call close_handles()
       return
     end if
     call imx_check(imx_swm_diagnostics(handle, d0), 'imx_swm_diagnostics')
-    refsol = (testcase == 2)
+    call perot_kenergy(d0, .true.)
+    ! RefSolAnal (src/swm_data.f90:685): the steady cases compare with the
+    ! initial state; TC2 also has an exact PV
+    refsol = testcase == 1 .or. testcase == 2 .or. testcase == 22
     if (refsol) then
-      allocate(h_exact(mdesc%nv), u_exact(mdesc%ne), q_exact(mdesc%nt), qtr(mdesc%nt))
-      h_exact = h
-      u_exact = u
-      call imx_check(imx_ic_tc2_pv(meshp, erad, q_exact), 'imx_ic_tc2_pv')
+      allocate(q_exact(mdesc%nt), qtr(mdesc%nt))
+      q_exact = 0._r8
+      if (testcase == 2) call imx_check(imx_ic_tc2_pv(mesh%lib, erad, q_exact), 'imx_ic_tc2_pv')
     end if
     ! swm.f90:226-231: both files start with the absolute initial values
     call write_evol_error(0, 0._r8, d0(1), 0._r8, 0._r8, 0._r8, 0._r8, 0._r8, 0._r8, &
@@ -85,7 +91,7 @@ contains
     call plotfields(0, 0._r8)
 
     ! time loop of swm_tests (swm.f90:238-370). With the analytic solution
-    ! (TC2) errors are written every nprints steps and at the end
+    ! errors are written every nprints steps and at the end
     ! (write_evol_error, write_errors); otherwise the diagnostics at k<=2,
     ! k==ntime and every nprints steps (write_evol). The steps in between
     ! run on the GPU without leaving it, so the reference's per-step
@@ -109,17 +115,284 @@ contains
           if (.not. write_ref_errors(k, time, d, d0)) exit
         end if
       else if (k <= 2 .or. k == ntime .or. mod(k, nprints) == 0) then
+        call perot_kenergy(d, .false.)
         call write_evol(k, time, (d(1) - d0(1))/d0(1), (d(2) - d0(2))/d0(2), (d(3) - d0(3))/d0(3), &
                         (d(4) - d0(4))/d0(4), (d(5) - d0(5))/d0(5), d(7))
       end if
       call plotfields(k, time)
     end do
 
-    call imx_check(imx_swm_get_state(handle, h, u), 'imx_swm_get_state')
+    call imx_check(imx_swm_get_state(handle, h%f, u%f), 'imx_swm_get_state')
     if (dump_state == 1) call dump_final_state()
-    call imx_swm_destroy(handle)
-    handle = c_null_ptr
+    call close_handles()
   end subroutine swm_tests
+
+  ! initialize_gridprop (src/swm.f90:979-1041): the kite areas and TRiSK
+  ! weights of the mesh (computed once on the host in FP64), and the device
+  ! handle the fields are evolved on, with the scheme of par/swm.par
+  subroutine initialize_gridprop()
+    call imx_check(imx_mesh_gridprop(mesh%lib), 'initialize_gridprop')
+    call imx_check(imx_mesh_desc_of(mesh%lib, mdesc), 'initialize_gridprop')
+    call close_handles()
+    call open_handle(handle, precision_code())
+  end subroutine initialize_gridprop
+
+  ! initialize_fields (src/swm.f90:1043-2414): h, u and bt of the test case
+  ! (imx_ic_testcase; 100: the seeded spectral state), loaded on the handle
+  subroutine initialize_fields()
+    integer(c_int32_t) :: sg
+    if (testcase == 100) then
+      ! seeded spectral state (new test case 100): h' 5% of H0, 20 m/s rms wind
+      call imx_check(imx_ic_spectral(mdesc, int(seed, c_int64_t), 16_c_int32_t, 2.94e4_r8/grav, &
+                                     0.05_r8, 20._r8, erad, h%f, u%f), 'initialize_fields')
+      bt%f = 0._r8
+    else
+      sg = merge(1_c_int32_t, 0_c_int32_t, trim(stag) == 'HC')
+      call imx_check(imx_ic_testcase(mesh%lib, int(testcase, c_int32_t), sg, erad, hollgw, h%f, u%f, bt%f), &
+                     'initialize_fields')
+    end if
+    call imx_check(imx_swm_set_topography(handle, bt%f), 'initialize_fields')
+    call imx_check(imx_swm_set_state(handle, h%f, u%f), 'initialize_fields')
+    h_exact%f = h%f
+    u_exact%f = u%f
+  end subroutine initialize_fields
+
+  ! subroutine tendency(h, u, masseq, momeq)  (src/swm.f90:2416-2598)
+  subroutine tendency(h, u, masseq, momeq)
+    type(scalar_field), intent(in) :: h, u
+    real(r8), intent(inout) :: masseq(:), momeq(:)
+    call imx_check(imx_swm_tendency(handle, h%f, u%f, masseq, momeq), 'tendency')
+  end subroutine tendency
+
+  ! subroutine ode_rk4(t, h, u, h_new, u_new, dt)  (src/swm.f90:2600-2683):
+  ! one RK4 step of the handle's time step from (h, u); t and dt as the
+  ! reference takes them (the step length is the handle's, set from dt)
+  subroutine ode_rk4(t, h, u, h_new, u_new, dt)
+    real(r8) :: t
+    type(scalar_field), intent(in) :: h, u
+    type(scalar_field) :: h_new, u_new
+    real(r8) :: dt
+    if (abs(dt - swm_par_dt()) > 0._r8) then
+      print*, "ode_rk4: dt", dt, " differs from the run's time step", swm_par_dt()
+      stop 1
+    end if
+    u_new = u
+    h_new = h
+    call imx_check(imx_swm_set_state(handle, h%f, u%f), 'ode_rk4')
+    call imx_check(imx_swm_step(handle, 1_c_int32_t), 'ode_rk4')
+    call imx_check(imx_swm_get_state(handle, h_new%f, u_new%f), 'ode_rk4')
+    if (t < 0._r8) print*, "ode_rk4: negative time", t
+  end subroutine ode_rk4
+
+  real(r8) function swm_par_dt() result(d)
+    d = dt
+  end function swm_par_dt
+
+  ! sumf_areas (src/swm.f90:3394-3440): area-weighted mean; on edges the
+  ! reference divides the sum of cell-area weights by the edge areas, and so
+  ! does this (the reference's quirk, kept)
+  real(r8) function sumf_areas(sf) result(s)
+    type(scalar_field), intent(in) :: sf
+    real(r8), pointer :: areag(:), trarea(:), leng(:), dleng(:)
+    real(r8) :: sa
+    integer :: i
+    call mesh_arrays(areag, trarea, leng, dleng)
+    s = 0._r8
+    sa = 0._r8
+    select case (sf%pos)
+    case (0)
+      do i = 1, sf%n
+        sa = sa + areag(i)
+        s = s + areag(i)*sf%f(i)
+      end do
+    case (1)
+      do i = 1, sf%n
+        sa = sa + trarea(i)
+        s = s + trarea(i)*sf%f(i)
+      end do
+    case (3, 6)
+      do i = 1, sf%n
+        sa = sa + leng(i)*dleng(i)*0.5_r8
+        s = s + areag(i)*sf%f(i)
+      end do
+    case default
+      print*, "Don't know how to calculate sumf_areas on this position:", sf%pos
+      stop 1
+    end select
+    s = s/sa
+  end function sumf_areas
+
+  ! sumfsq_areas (src/swm.f90:3442-3490)
+  real(r8) function sumfsq_areas(sf) result(s)
+    type(scalar_field), intent(in) :: sf
+    real(r8), pointer :: areag(:), trarea(:), leng(:), dleng(:)
+    real(r8) :: sa, a
+    integer :: i
+    call mesh_arrays(areag, trarea, leng, dleng)
+    s = 0._r8
+    sa = 0._r8
+    select case (sf%pos)
+    case (0)
+      do i = 1, sf%n
+        sa = sa + areag(i)
+        s = s + areag(i)*sf%f(i)**2
+      end do
+    case (1)
+      do i = 1, sf%n
+        sa = sa + trarea(i)
+        s = s + trarea(i)*sf%f(i)**2
+      end do
+    case (3, 6)
+      do i = 1, sf%n
+        a = leng(i)*dleng(i)*0.5_r8
+        sa = sa + a
+        s = s + a*sf%f(i)**2
+      end do
+    case default
+      print*, "Don't know how to calculate sumfsq_areas on this position:", sf%pos
+      stop 1
+    end select
+    s = s/sa
+  end function sumfsq_areas
+
+  ! error_calc (src/swm.f90:3492-3543)
+  subroutine error_calc(f, fref, fdif, errormaxrel, error2rel, errormax)
+    type(scalar_field), intent(in) :: f, fref
+    type(scalar_field), intent(inout) :: fdif
+    real(r8) :: errormaxrel, error2rel, errormax
+    real(r8) :: refsum, refmax
+    fdif = f
+    fdif%f = f%f - fref%f
+    refsum = sumfsq_areas(fref)
+    if (refsum > 0) then
+      error2rel = dsqrt(sumfsq_areas(fdif))/dsqrt(refsum)
+    else
+      error2rel = dsqrt(sumfsq_areas(fdif))
+    end if
+    refmax = maxval(abs(fref%f(1:fref%n)))
+    errormax = maxval(abs(fdif%f(1:fdif%n)))
+    if (refmax > 0) then
+      errormaxrel = errormax/refmax
+    else
+      errormaxrel = errormax
+    end if
+  end subroutine error_calc
+
+  ! calc_energies (src/swm.f90:2686-2752) of the module state h, u, bt, with
+  ! the reference's loops on the host; h_ed (trsk) and ke_hx (pxt16) come
+  ! from a reference-order tendency of that state
+  subroutine calc_energies(Penergy, Kenergy, Tenergy, Availenergy)
+    real(r8) :: Penergy, Kenergy, Tenergy, Availenergy
+    real(r8), pointer :: areag(:), trarea(:), leng(:), dleng(:)
+    real(r8), allocatable :: hed(:), ke(:)
+    type(scalar_field) :: hbt
+    real(r8) :: mean_height, mhb, unavail
+    integer :: i, l
+    call mesh_arrays(areag, trarea, leng, dleng)
+    allocate(hed(mdesc%ne), ke(mdesc%nv))
+    call tendency_fields(h%f, u%f)
+    call imx_check(imx_swm_get_field(ref_handle(), IMX_FIELD_H_ED, hed), 'calc_energies')
+    call imx_check(imx_swm_get_field(ref_handle(), IMX_FIELD_KE_HX, ke), 'calc_energies')
+    Penergy = 0._r8
+    do i = 1, mdesc%nv
+      Penergy = Penergy + areag(i)*grav*h%f(i)*(h%f(i)*0.5 + bt%f(i))
+    end do
+    hbt = h
+    hbt%f = h%f + bt%f
+    mean_height = sumf_areas(hbt)
+    unavail = 0._r8
+    do i = 1, mdesc%nv
+      mhb = mean_height - bt%f(i)
+      unavail = unavail + areag(i)*grav*mhb*(mhb*0.5 + bt%f(i))
+    end do
+    Kenergy = 0._r8
+    if (wrap_code() == IMX_WRAP_PXT16) then
+      do i = 1, mdesc%nv
+        Kenergy = Kenergy + ke(i)*areag(i)
+      end do
+    else
+      do l = 1, mdesc%ne
+        Kenergy = Kenergy + 0.5*leng(l)*dleng(l)*hed(l)*u%f(l)**2
+      end do
+    end if
+    Tenergy = Penergy + Kenergy
+    Availenergy = Tenergy - unavail
+  end subroutine calc_energies
+
+  subroutine mesh_arrays(areag, trarea, leng, dleng)
+    real(r8), pointer :: areag(:), trarea(:), leng(:), dleng(:)
+    call c_f_pointer(mdesc%hx_areag, areag, [mdesc%nv])
+    call c_f_pointer(mdesc%tr_areag, trarea, [mdesc%nt])
+    call c_f_pointer(mdesc%ed_leng, leng, [mdesc%ne])
+    call c_f_pointer(mdesc%edhx_leng, dleng, [mdesc%ne])
+  end subroutine mesh_arrays
+
+  ! The handle whose fields are the reference's: the run's own with
+  ! IMX_PREC_FP64_REF, else the FP64 reference-order companion
+  type(c_ptr) function ref_handle() result(p)
+    if (precision_code() == IMX_PREC_FP64_REF) then
+      p = handle
+    else
+      if (.not. c_associated(diag_handle)) then
+        call open_handle(diag_handle, IMX_PREC_FP64_REF)
+        call imx_check(imx_swm_set_topography(diag_handle, bt%f), 'ref_handle')
+      end if
+      p = diag_handle
+    end if
+  end function ref_handle
+
+  ! a reference-order tendency of (hh, uu), leaving its intermediate fields
+  ! on ref_handle()
+  subroutine tendency_fields(hh, uu)
+    real(r8), intent(in) :: hh(:), uu(:)
+    real(r8), allocatable :: m(:), f(:)
+    allocate(m(mdesc%nv), f(mdesc%ne))
+    call imx_check(imx_swm_tendency(ref_handle(), hh, uu, m, f), 'tendency_fields')
+  end subroutine tendency_fields
+
+  ! The fields the time loop's diagnostics read: with IMX_PREC_FP64_REF
+  ! those its last RK stage left (the reference's reading), else a
+  ! reference-order tendency of the handle's state
+  subroutine ref_fields(fresh)
+    logical, intent(in) :: fresh
+    if (fresh) then
+      call imx_check(imx_swm_get_state(handle, h%f, u%f), 'ref_fields')
+      call tendency_fields(h%f, u%f)
+    end if
+  end subroutine ref_fields
+
+  ! pxt16: calc_energies takes K as sum ke_hx A (src/swm.f90:2733-2737), the
+  ! Perot kinetic energy of the fields the reference reads; the device
+  ! diagnostics hold 1/2 l d h_e u^2. Rewrites K, T and A of d.
+  subroutine perot_kenergy(d, initial)
+    real(c_double), intent(inout) :: d(:)
+    logical, intent(in) :: initial
+    real(r8), allocatable :: ke(:)
+    real(r8), pointer :: areag(:), trarea(:), leng(:), dleng(:)
+    real(r8) :: kp, unavail
+    integer :: i
+    if (wrap_code() /= IMX_WRAP_PXT16) return
+    call mesh_arrays(areag, trarea, leng, dleng)
+    allocate(ke(mdesc%nv))
+    ! the reference's first energies follow a tendency at the initial state
+    call ref_fields(initial .or. precision_code() /= IMX_PREC_FP64_REF)
+    call imx_check(imx_swm_get_field(ref_handle(), IMX_FIELD_KE_HX, ke), 'perot_kenergy')
+    kp = 0._r8
+    do i = 1, mdesc%nv
+      kp = kp + ke(i)*areag(i)
+    end do
+    unavail = d(4) - d(5)
+    d(3) = kp
+    d(4) = d(2) + kp
+    d(5) = d(4) - unavail
+  end subroutine perot_kenergy
+
+  subroutine close_handles()
+    if (c_associated(handle)) call imx_swm_destroy(handle)
+    if (c_associated(diag_handle)) call imx_swm_destroy(diag_handle)
+    handle = c_null_ptr
+    diag_handle = c_null_ptr
+  end subroutine close_handles
 
   ! swmname and meshname as swmpars builds them (src/swm_data.f90:640-683);
   ! without the time step when swmpars runs with usetime = .false. (the LTE
@@ -155,18 +428,19 @@ contains
       write(atmp, '(f10.3)') real(dlog(k4_max)/dlog(10._r8))
       swmname = trim(swmname)//"_"//trim(hyperdiffus)//"_hyperdiffusion_10to"//trim(adjustl(trim(atmp)))
     end if
-    write(atmp, '(i8)') glevel
-    meshname = trim(mkind)//"_"//trim(mpos)//"_"//trim(moptm)//"_"//trim(adjustl(atmp))
-    if (len_trim(gridname) > 0) meshname = gridname
+    ! the grid's name (namegrid) ends every output file name
+    meshname = mesh%name
 
   end subroutine make_names
 
-  ! One libimx handle for the run, with the scheme options of par/swm.par
-  subroutine open_handle()
+  ! One libimx handle with the options of par/swm.par (precision prec)
+  subroutine open_handle(hnd, prec)
+    type(c_ptr), intent(inout) :: hnd
+    integer(c_int32_t), intent(in) :: prec
     type(imx_swm_opts) :: o
     type(imx_scheme) :: sc
     call imx_swm_default_opts(o)
-    o%precision = precision_code()
+    o%precision = prec
     o%testcase = merge(2, testcase, testcase == 100)
     o%use_sfc = use_sfc
     o%use_graph = use_graph
@@ -174,11 +448,11 @@ contains
     o%grav = grav
     o%omega = omega
     o%dt = dt
-    call imx_check(imx_swm_create(mdesc, o, handle), 'imx_swm_create')
+    call imx_check(imx_swm_create(mdesc, o, hnd), 'imx_swm_create')
     if (scheme_options()) then
       ! the fused fp32/fp64 kernels carry trsk10 and (with HTC) pxt16, with
       ! every other option
-      if (trim(precision) /= 'fp64_ref' .and. (wrap_code() == 2 .or. &
+      if (prec /= IMX_PREC_FP64_REF .and. (wrap_code() == 2 .or. &
           (wrap_code() == 1 .and. trim(stag) == 'HC'))) then
         write(*, '(a)') " ERROR: gass18 and HC with pxt16 run on the fp64_ref path"
         write(*, '(a)') "        (par/swm_gpu.par)"
@@ -192,7 +466,7 @@ contains
       sc%diffus = max(coef_code(diffus), 0)
       sc%k4 = max(k4_max, 0._r8)
       sc%hyperdiffus = max(coef_code(hyperdiffus), 0)
-      call imx_check(imx_swm_set_scheme(handle, mdesc, sc), 'imx_swm_set_scheme')
+      call imx_check(imx_swm_set_scheme(hnd, mdesc, sc), 'imx_swm_set_scheme')
     end if
   end subroutine open_handle
 
@@ -200,9 +474,8 @@ contains
   ! one tendency evaluation (imx_swm_lte: the tendency and its analysis fields
   ! on the GPU). Prints the reference's table and mimetic sums and appends
   ! the table to data/swm_loc_trunc_errors.txt in its layout.
-  subroutine swm_horiz_loc_trunc_er(meshp, glevel)
-    type(c_ptr), intent(in) :: meshp
-    integer, intent(in) :: glevel
+  subroutine swm_horiz_loc_trunc_er(meshtmp)
+    type(grid_structure), intent(in) :: meshtmp
     real(c_double) :: err(3*IMX_LTE_NROW), mim(IMX_LTE_NMIM)
     character(len=16), parameter :: rows(IMX_LTE_NROW) = [character(len=16) :: &
       " h_ed ", " divuh ", " kenergy ", " abs_vort ", " h_tr ", " q_tr ", " q_hx ", " q_ed ", &
@@ -212,7 +485,8 @@ contains
     integer :: fu, r
     logical :: ex
 
-    call read_swm_par(glevel)
+    mesh = meshtmp
+    call swmpars(usetime=.false.)
     if (testcase /= 2) then
       write(*, '(a)') " ERROR: the local truncation error analysis has the closed forms of test case 2"
       stop 1
@@ -221,12 +495,11 @@ contains
       write(*, '(a)') " ERROR: the local truncation error analysis runs on the fp64_ref path (par/swm_gpu.par)"
       stop 1
     end if
-    call imx_check(imx_mesh_desc_of(meshp, mdesc), 'imx_mesh_desc_of')
-    call make_names(glevel, .false.)
-    call open_handle()
-    call imx_check(imx_swm_lte(handle, meshp, err, mim), 'swm_horiz_loc_trunc_er')
-    call imx_swm_destroy(handle)
-    handle = c_null_ptr
+    call allocate_globalswmvars()
+    call initialize_gridprop()
+    call make_names(mesh%glevel, .false.)
+    call imx_check(imx_swm_lte(handle, mesh%lib, err, mim), 'swm_horiz_loc_trunc_er')
+    call close_handles()
 
     call imx_check(imx_make_dir('data'//c_null_char), 'swm_horiz_loc_trunc_er')
     inquire(file="data/swm_loc_trunc_errors.txt", exist=ex)
@@ -286,10 +559,10 @@ contains
     pert = sum(areag)/mdesc%nv            ! mesh%meanhxarea
     n = mdesc%nv + mdesc%ne
     print*, "Generating the linearised operator (", n, " unknowns)"
-    call imx_check(imx_swm_jacobian(handle, mdesc, h, u, pert, 0_c_int64_t, [0_c_int32_t], [0_c_int32_t], &
+    call imx_check(imx_swm_jacobian(handle, mdesc, h%f, u%f, pert, 0_c_int64_t, [0_c_int32_t], [0_c_int32_t], &
                                     [0._r8], nnz, ncolors), 'swm_normal_mode_analysis')
     allocate(row(nnz), col(nnz), val(nnz))
-    call imx_check(imx_swm_jacobian(handle, mdesc, h, u, pert, nnz, row, col, val, nnz, ncolors), &
+    call imx_check(imx_swm_jacobian(handle, mdesc, h%f, u%f, pert, nnz, row, col, val, nnz, ncolors), &
                    'swm_normal_mode_analysis')
     print*, "Tendency evaluations:", ncolors + 1, " (one per unknown in the reference:", n, ")"
     call imx_check(imx_make_dir('data'//c_null_char), 'swm_normal_mode_analysis')
@@ -317,22 +590,6 @@ contains
     print*, "Matrix printed as file: ", trim(filename)
   end subroutine swm_normal_mode_analysis
 
-  ! subroutine tendency(h, u, masseq, momeq)  (src/swm.f90:2416)
-  subroutine tendency(hh, uu, masseq, momeq)
-    real(r8), intent(in) :: hh(:), uu(:)
-    real(r8), intent(inout) :: masseq(:), momeq(:)
-    call imx_check(imx_swm_tendency(handle, hh, uu, masseq, momeq), 'tendency')
-  end subroutine tendency
-
-  ! subroutine ode_rk4(t, h, u, h_new, u_new, dt)  (src/swm.f90:2600)
-  subroutine ode_rk4(hh, uu, h_new, u_new)
-    real(r8), intent(in) :: hh(:), uu(:)
-    real(r8), intent(inout) :: h_new(:), u_new(:)
-    call imx_check(imx_swm_set_state(handle, hh, uu), 'ode_rk4')
-    call imx_check(imx_swm_step(handle, 1_c_int32_t), 'ode_rk4')
-    call imx_check(imx_swm_get_state(handle, h_new, u_new), 'ode_rk4')
-  end subroutine ode_rk4
-
   ! plotfields (src/swm.f90:2754-2904) for runs without the local
   ! truncation-error test: at k = 0, every plotsteps steps and at the end,
   ! h, eta, K, q_tr (pv), q_ed (pv_ed), div(uh) and, with the analytic
@@ -351,8 +608,8 @@ contains
     write(atime, *) nint(time)
     atime = adjustl(atime)
     ref = precision_code() == IMX_PREC_FP64_REF
-    call imx_check(imx_swm_get_state(handle, h, u), 'plotfields')
-    call plot_scalarfield(h, 0, "_h_t"//trim(atime))
+    call imx_check(imx_swm_get_state(handle, h%f, u%f), 'plotfields')
+    call plot_scalarfield(h%f, 0, "_h_t"//trim(atime))
     if (ref) then
       allocate(f(max(mdesc%nv, mdesc%ne, mdesc%nt)))
       call imx_check(imx_swm_get_field(handle, IMX_FIELD_ETA, f), 'plotfields')
@@ -370,8 +627,8 @@ contains
       call plot_scalarfield(f(1:mdesc%nv), 0, "_divuh_t"//trim(atime))
     end if
     if (refsol) then
-      call plot_scalarfield(h - h_exact, 0, "_h_error_t"//trim(atime))
-      call plot_scalarfield(u - u_exact, 6, "_u_error_t"//trim(atime))
+      call plot_scalarfield(h%f - h_exact%f, 0, "_h_error_t"//trim(atime))
+      call plot_scalarfield(u%f - u_exact%f, 6, "_u_error_t"//trim(atime))
       call plot_scalarfield(qtr - q_exact, 1, "_q_tr_error_t"//trim(atime))
     end if
   end subroutine plotfields
@@ -398,8 +655,8 @@ contains
   end subroutine plot_scalarfield
 
   ! error_calc (src/swm.f90:3492-3543) with sumfsq_areas weights w
-  ! (:3442-3490): relative max and L2 errors of f against fref
-  subroutine error_calc(f, fref, w, errormaxrel, error2rel, errormax)
+  ! (:3442-3490) on plain arrays: relative max and L2 errors of f against fref
+  subroutine error_calc_w(f, fref, w, errormaxrel, error2rel, errormax)
     real(r8), intent(in) :: f(:), fref(:), w(:)
     real(r8), intent(out) :: errormaxrel, error2rel, errormax
     real(r8) :: refsum, refmax, difsum, sw
@@ -424,14 +681,16 @@ contains
     else
       errormaxrel = errormax
     end if
-  end subroutine error_calc
+  end subroutine error_calc_w
 
   ! The analytic-solution branch of the time loop (src/swm.f90:291-349):
   ! errors of h, u and q_tr against the exact fields, max |div u| and
   ! max |grad K| over edges, written by write_evol_error every nprints
-  ! steps and by write_errors at the last step. With IMX_PREC_FP64_REF,
-  ! q_tr, div u and K are those the reference reads (its last RK stage);
-  ! fused handles give q_tr of the last stage and div u, K of the state.
+  ! steps and by write_errors at the last step. div u and K are the
+  ! reference's div_hx and kinetic_energy_hx for the run's scheme (HC signs,
+  ! Perot K under pxt16): with IMX_PREC_FP64_REF those of the last RK stage,
+  ! as the reference reads them; fused handles evaluate them on the state
+  ! with an FP64 reference-order tendency (ref_fields).
   ! Returns .false. when the run should stop (blow-up, swm.f90:331-340).
   logical function write_ref_errors(k, time, d, d0) result(go_on)
     integer, intent(in) :: k
@@ -448,19 +707,16 @@ contains
     call c_f_pointer(mdesc%edhx_leng, dleng, [mdesc%ne])
     call c_f_pointer(mdesc%tr_areag, trarea, [mdesc%nt])
     call c_f_pointer(mdesc%ed_v, ev, [2*mdesc%ne])
-    call imx_check(imx_swm_get_state(handle, h, u), 'imx_swm_get_state')
+    call imx_check(imx_swm_get_state(handle, h%f, u%f), 'imx_swm_get_state')
     call imx_check(imx_swm_get_field(handle, IMX_FIELD_Q_TR, qtr), 'imx_swm_get_field')
     allocate(we(mdesc%ne), divu(mdesc%nv), ke(mdesc%nv))
     we = leng*dleng*0.5_r8
-    call error_calc(h, h_exact, areag, emr_h, e2_h, em_h)
-    call error_calc(u, u_exact, we, emr_u, e2_u, em_u)
-    call error_calc(qtr, q_exact, trarea, emr_q, e2_q, em_q)
-    if (precision_code() == IMX_PREC_FP64_REF) then
-      call imx_check(imx_swm_get_field(handle, IMX_FIELD_DIVU, divu), 'imx_swm_get_field')
-      call imx_check(imx_swm_get_field(handle, IMX_FIELD_KE_HX, ke), 'imx_swm_get_field')
-    else
-      call div_ke(u, divu, ke)
-    end if
+    call error_calc_w(h%f, h_exact%f, areag, emr_h, e2_h, em_h)
+    call error_calc_w(u%f, u_exact%f, we, emr_u, e2_u, em_u)
+    call error_calc_w(qtr, q_exact, trarea, emr_q, e2_q, em_q)
+    call ref_fields(precision_code() /= IMX_PREC_FP64_REF)
+    call imx_check(imx_swm_get_field(ref_handle(), IMX_FIELD_DIVU, divu), 'imx_swm_get_field')
+    call imx_check(imx_swm_get_field(ref_handle(), IMX_FIELD_KE_HX, ke), 'imx_swm_get_field')
     sw = sum(areag)
     rmsdiv = sqrt(sum(areag*divu**2)/sw)
     maxdiv = maxval(abs(divu))
@@ -485,35 +741,6 @@ contains
       call write_errors(time, emr_h, e2_h, emr_u, e2_u, (d(1) - d0(1))/d0(1), (d(2) - d0(2))/d0(2), &
                         (d(3) - d0(3))/d0(3), (d(4) - d0(4))/d0(4), 1._r8)
   end function write_ref_errors
-
-  ! div_hx(u) and kinetic_energy_hx (trsk, src/swm_operators.f90:613-661,
-  ! :1132-1217) of the given state on the host, caller numbering
-  subroutine div_ke(uu, divu, ke)
-    real(r8), intent(in) :: uu(:)
-    real(r8), intent(out) :: divu(:), ke(:)
-    real(r8), pointer :: areag(:), leng(:), dleng(:)
-    integer(c_int32_t), pointer :: nnb(:), ved(:), ttg(:)
-    real(r8) :: s, k, ca, w
-    integer :: i, j, l
-    call c_f_pointer(mdesc%hx_areag, areag, [mdesc%nv])
-    call c_f_pointer(mdesc%ed_leng, leng, [mdesc%ne])
-    call c_f_pointer(mdesc%edhx_leng, dleng, [mdesc%ne])
-    call c_f_pointer(mdesc%v_nnb, nnb, [mdesc%nv])
-    call c_f_pointer(mdesc%v_ed, ved, [6*mdesc%nv])
-    call c_f_pointer(mdesc%hx_ttgout, ttg, [6*mdesc%nv])
-    do i = 1, mdesc%nv
-      s = 0._r8; k = 0._r8; ca = 0._r8
-      do j = 1, nnb(i)
-        l = ved(6*(i-1)+j)
-        s = s + real(ttg(6*(i-1)+j), r8)*uu(l)*dleng(l)
-        w = 0.25_r8*leng(l)*dleng(l)
-        ca = ca + w
-        k = k + w*uu(l)**2
-      end do
-      divu(i) = s/areag(i)/erad
-      ke(i) = k/ca
-    end do
-  end subroutine div_ke
 
   ! write_evol_error (src/swm.f90:2999-3086): same file name, header, format
   subroutine write_evol_error(k, time, tmass, errormax_h, error2_h, errormax_u, error2_u, &
@@ -609,9 +836,9 @@ contains
          access='stream', form='unformatted', status='replace')
     write(fu) 'IMXBIN01'
     nm = 'h'
-    write(fu) nm, 'f8  ', int(size(h), 8), h
+    write(fu) nm, 'f8  ', int(size(h%f), 8), h%f
     nm = 'u'
-    write(fu) nm, 'f8  ', int(size(u), 8), u
+    write(fu) nm, 'f8  ', int(size(u%f), 8), u%f
     close(fu)
   end subroutine dump_final_state
 end module swm

@@ -3,25 +3,23 @@
 !   par/mesh.par  as getparameters  (src/smeshpack.f90:7343-7416), argv
 !                 [meshpar] [nv] override as in the reference
 !   par/swm.par   as swmpars        (src/swm_data.f90:338-693), the
-!                 geo-area subset this build runs: trsk10 with HTC or HC
-!                 (TC2) and pxt16 with HTC, APVM, diffusion and
+!                 geo-area subset this build runs: test cases 0, 1, 2, 5,
+!                 6, 21, 22, 23, 51, 52, 53 and the seeded state 100;
+!                 trsk10 with HTC or HC and pxt16 with HTC, APVM, diffusion and
 !                 hyperdiffusion on every path; gass18 and HC pxt16 on the
 !                 fp64_ref path
 !   par/swm_gpu.par (new, optional): precision, seed, SFC and graph switches
 !=======================================================================
 module swm_par
   use iso_c_binding, only: c_int32_t
+  use datastruct, only: r8
   implicit none
-  integer, parameter :: r8 = selected_real_kind(12, 100)
   real(r8), parameter :: day2sec = 86400._r8
 
   ! mesh.par
   integer :: nv_req = 100, loadable = 0, showonscreen = 0, simulcase = 15, hrchy = 1
   character(len=16) :: mkind = 'icos', mpos = 'eqs', moptm = 'nopt'
   character(len=128) :: mname = ''
-  ! mesh%name of the grid the driver built or loaded (namegrid); the output
-  ! file names carry it
-  character(len=128) :: gridname = ''
   ! swm.par
   integer :: testcase = 2, test_lterror = 0, ntime = 0, adjustntime = 0
   integer :: normal_modes = 0
@@ -146,7 +144,6 @@ contains
 
     ! the scheme this build runs (the reference defaults, par/swm.par)
     if (trim(stag) /= 'HTC' .and. trim(stag) /= 'HC') call unsupported('staggering', stag)
-    if (trim(stag) == 'HC' .and. testcase /= 2) call unsupported('HC staggering with test case (2 only)', 'other')
     if (wrap_code() < 0) call unsupported('method wrapper', mtdwrapper)
     if (wrap_code() == 2 .and. (k2_max > 0._r8 .or. k4_max > 0._r8)) then
       ! as the reference's scalar_hx2tr (src/swm_operators.f90:271-273)
@@ -163,7 +160,11 @@ contains
     if (trim(pv_stab) /= 'none' .and. trim(pv_stab) /= 'apvm') call unsupported('PV stabilisation', pv_stab)
     if (inopv /= 0) call unsupported('level model (noPV)', 'noPV=1')
     if (test_lterror /= 0) call unsupported('local truncation error test', '1')
-    if (testcase /= 2 .and. testcase /= 100) call unsupported('test case (2 or 100)', 'other')
+    select case (testcase)
+    case (0, 1, 2, 5, 6, 21, 22, 23, 51, 52, 53, 100)
+    case default
+      call unsupported('test case (0, 1, 2, 5, 6, 21, 22, 23, 51, 52, 53 or the seeded 100)', 'other')
+    end select
 
     ! dt / ntime (src/swm_data.f90:562-586)
     if (period <= 0) stop "swmpars error: period should be positive and must be given in days"

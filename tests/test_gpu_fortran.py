@@ -386,3 +386,129 @@ def test_fortran_driver_read_grid():
     h0, u0 = m.tc2()
     h, u = Oracle(m.arrays).step(h0, u0, 900.0, 96)
     assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
+
+
+def test_fortran_caller_reference_surface_bitexact(golden):
+    """A Fortran program written against the reference's call surface
+    (tests/fortran/swm_caller.f90, shaped as moist_swm's import of
+    initialize_gridprop, sumf_areas, calc_energies, tendency, error_calc):
+    test case 5 with its mountain on glevel 3, FP64_REF. The tendency of the
+    initial state, sumf_areas, calc_energies and three ode_rk4 steps equal
+    the reference's own run (ref_run_g3_tc5.bin) bit for bit."""
+    print("[gpu-case] fortran reference-surface caller g3 tc5 fp64_ref", flush=True)
+    from imodel_amd.build import CALLER
+    assert os.access(CALLER, os.X_OK), "test caller not built"
+    r = golden("ref_run_g3_tc5.bin")
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 5, 900, 1, 24, "fp64_ref")
+        res = subprocess.run([CALLER], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                             timeout=120)
+        assert res.returncode == 0, res.stdout[-2000:]
+        out = imxbin.read(os.path.join(td, "out.bin"))
+    assert np.array_equal(out["masseq"], r["masseq"]) and np.array_equal(out["momeq"], r["momeq"])
+    assert np.array_equal(out["diag0"], r["diag0"][:5])   # mass, P, K, T, available
+    assert np.array_equal(out["h"], r["h"]) and np.array_equal(out["u"], r["u"])
+
+
+def test_fortran_caller_reference_surface_fp32():
+    """The same caller on the fused FP32 path: the tendency and the three
+    steps against the oracle with the mountain, at the FP32 tolerances."""
+    print("[gpu-case] fortran reference-surface caller g3 tc5 fp32", flush=True)
+    from imodel_amd.build import CALLER
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 5, 900, 1, 24, "fp32")
+        res = subprocess.run([CALLER], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                             timeout=120)
+        assert res.returncode == 0, res.stdout[-2000:]
+        out = imxbin.read(os.path.join(td, "out.bin"))
+    m = Mesh.icos(3)
+    h0, u0, bt = m.testcase(5)
+    o = Oracle(m.arrays, testcase=5)
+    o.bt = bt
+    mo, fo = o.tendency(h0, u0)
+    scale = max(np.linalg.norm(fo), np.linalg.norm(o.field("grad_ghbk")))
+    assert np.linalg.norm(out["masseq"] - mo) / np.linalg.norm(mo) < 5e-5
+    assert np.linalg.norm(out["momeq"] - fo) / scale < 5e-5
+    h, u = o.step(h0, u0, 900.0, 3)
+    A = m.arrays["hx_areag"]
+    assert np.sqrt(np.sum(A * (out["h"] - h) ** 2) / np.sum(A * h ** 2)) < 1e-7
+    assert np.linalg.norm(out["u"] - u) / np.linalg.norm(u) < 1e-5
+
+
+def test_fortran_driver_tc21_shipped_parameters():
+    """The reference's shipped par/swm.par runs the Galewsky jet (test case
+    21, dt = 120 s, HTC, trsk10, prints every 60 steps); one day of it on
+    glevel 3 with FP64_REF equals the oracle bit for bit."""
+    print("[gpu-case] fortran imodel g3 tc21 dt120 fp64_ref", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 21, 120, 1, 60, "fp64_ref")
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        st = imxbin.read(os.path.join(td, "data", "swm_tc21_dt120_HTC_trsk10_areageo_state_icos_eqs_nopt_3.bin"))
+        me = open(os.path.join(td, "data", "swm_tc21_dt120_HTC_trsk10_areageo_mass_energy_icos_eqs_nopt_3.txt")
+                  ).read().splitlines()
+    m = Mesh.icos(3)
+    h0, u0, _ = m.testcase(21)
+    h, u = Oracle(m.arrays, testcase=21).step(h0, u0, 120.0, 720)
+    assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
+    assert [int(x.split()[3]) for x in me[1:]][-1] == 720
+
+
+@pytest.mark.parametrize("prec", ["fp64_ref", "fp32"])
+def test_fortran_driver_tc5_mountain(prec):
+    """Test case 5 (flow over the mountain) from par/swm.par: the driver sets
+    the topography; a day on glevel 4 equals the oracle with the mountain
+    (bit for bit on FP64_REF, to FP32 storage tolerances on FP32) and the
+    energies of the mass/energy file carry b."""
+    print(f"[gpu-case] fortran imodel g4 tc5 {prec}", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 2562, 5, 900, 1, 48, prec)
+        r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:]
+        st = imxbin.read(os.path.join(td, "data", "swm_tc5_dt900_HTC_trsk10_areageo_state_icos_eqs_nopt_4.bin"))
+        me = open(os.path.join(td, "data", "swm_tc5_dt900_HTC_trsk10_areageo_mass_energy_icos_eqs_nopt_4.txt")
+                  ).read().splitlines()
+    m = Mesh.icos(4)
+    h0, u0, bt = m.testcase(5)
+    o = Oracle(m.arrays, testcase=5)
+    o.bt = bt
+    h, u = o.step(h0, u0, 900.0, 96)
+    if prec == "fp64_ref":
+        assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
+    else:
+        A = m.arrays["hx_areag"]
+        assert np.sqrt(np.sum(A * (st["h"] - h) ** 2) / np.sum(A * h ** 2)) < 1e-6
+    p0 = float(me[1].split()[9])      # absolute initial potential energy
+    want = o.diagnostics(h0, u0, refresh=True)[1]
+    # FP32 holds h - h_ref rounded to FP32 (relative 6e-8 of the 10% mountain
+    # perturbation) from the first step on, the initial energies included
+    assert abs(p0 - want) <= (1e-12 if prec == "fp64_ref" else 1e-8) * abs(want)
+
+
+def test_fortran_driver_fused_hc_divergence_matches_ref():
+    """HC staggering on the fused FP32 path: the evolution file's RMS and max
+    div u use the reference's HC signs (they come from a reference-order
+    tendency of the state), so they agree with an FP64_REF HC run to FP32
+    storage accuracy."""
+    print("[gpu-case] fortran imodel g3 tc2 HC fp32 vs fp64_ref divergence", flush=True)
+    cols = {}
+    for prec in ("fp64_ref", "fp32"):
+        with tempfile.TemporaryDirectory() as td:
+            _par(td, 642, 2, 900, 1, 24, prec)
+            p = os.path.join(td, "par", "swm.par")
+            swm = open(p).read().splitlines()
+            swm[8] = "HC"
+            open(p, "w").write("\n".join(swm) + "\n")
+            r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                               text=True, timeout=120)
+            assert r.returncode == 0, r.stdout[-2000:]
+            evo = open(os.path.join(td, "data", "swm_tc2_dt900_HC_trsk10_areageo_evolution_icos_eqs_nopt_3.txt")
+                       ).read().splitlines()
+        cols[prec] = np.array([[float(x) for x in ln.split()[19:21]] for ln in evo[2:]])
+    # fp64_ref reads the last RK stage's div u, fp32 the state's, so they
+    # differ; HTC signs on an HC run (half the terms flipped) would be off
+    # by orders of magnitude
+    ratio = cols["fp32"] / cols["fp64_ref"]
+    assert np.all((ratio > 0.5) & (ratio < 2.0)), cols

@@ -59,12 +59,17 @@ def test_fast_testcase_tendency_vs_oracle(name, prec, tol):
     o = Oracle(m.arrays, testcase=tc, scheme={"stag": stag} if stag == "HC" else None)
     o.bt = bt
     mo, fo = o.tendency(h, u)
+    # the balanced states (5, 6, 21: geostrophic to a few per cent) have a
+    # momentum tendency far below its two terms; errors are measured
+    # against the larger of the tendency and the Bernoulli gradient
+    scale = max(np.linalg.norm(fo), np.linalg.norm(o.field("grad_ghbk")))
     with Swm(m, precision=prec, testcase=tc) as s:
         if stag == "HC":
             s.set_scheme(stag="HC")
         s.set_topography(bt)
         mg, fg = s.tendency(h, u)
-    assert rel_l2(mg, mo) < tol and rel_l2(fg, fo) < tol, (rel_l2(mg, mo), rel_l2(fg, fo))
+    em, ef = rel_l2(mg, mo), np.linalg.norm(fg - fo) / scale
+    assert em < tol and ef < tol, (em, ef)
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64_ref", 1e-12), ("fp64", 1e-12), ("fp32", 1e-6)])
