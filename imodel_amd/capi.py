@@ -190,6 +190,7 @@ def lib():
                                      ctypes.c_uint64, _F64P]),
     "imx_learn_step": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_double, _I32P]),
             "imx_learn_last_rms": (ctypes.c_int, [vp, _F64P]),
+            "imx_learn_set_limiter": (ctypes.c_int, [vp, ctypes.c_int32]),
             "imx_learn_destroy": (None, [vp]),
         }
         for name, (res, args) in sig.items():

@@ -58,7 +58,7 @@ struct imx_learn {
   float *th = nullptr;      // [nv] scaled dh target
   double *part = nullptr;   // [LPART] partial sums of the correction norm
   double *rms = nullptr;    // [1] correction RMS of the last step
-  double *bound = nullptr;  // [1]
+  double *bound = nullptr;  // [2] RMS bound, limiter mode (IMX_LIMIT_*)
   double *lh = nullptr;     // [1] h loss of the last training epoch
   int *nfb = nullptr;       // [1] limited steps since the counter was cleared
   hipGraph_t graph = nullptr;
@@ -266,11 +266,14 @@ __global__ __launch_bounds__(LTPB) void learn_apply(int nv, int NE, const float 
     if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  // limiter: a correction whose RMS exceeds the bound is scaled down to
-  // it; a non-finite one (or bound 0) leaves the plain step untouched
+  // limiter (bound[0] the RMS bound, bound[1] the mode): a correction whose
+  // RMS exceeds the bound is dropped and the plain step kept (mode 0), or
+  // scaled down to the bound (mode 1); a non-finite one (or bound 0) always
+  // leaves the plain step untouched
   const double r = sqrt(red[0]);
   const bool ok = r <= bound[0];
-  const float sc = ok ? 1.f : (r < 1e300 ? (float)(bound[0] / r) : 0.f);
+  const bool scale = bound[1] > 0.5;
+  const float sc = ok ? 1.f : ((scale && r < 1e300) ? (float)(bound[0] / r) : 0.f);
   const int i = blockIdx.x * LTPB + threadIdx.x;
   if (i == 0) {
     rms[0] = r;
@@ -503,7 +506,7 @@ int imx_learn_create(imx_swm *s, imx_mlp *m, const imx_mesh_desc *d, imx_learn *
   L->th = A.get<float>(nv);
   L->part = A.get<double>(LPART);
   L->rms = A.get<double>(1);
-  L->bound = A.put(std::vector<double>(1, 1e300));
+  L->bound = A.put(std::vector<double>{1e300, (double)IMX_LIMIT_FALLBACK});
   L->lh = A.get<double>(1);
   L->nfb = A.get<int>(1);
   if (A.err != hipSuccess) {
@@ -596,6 +599,15 @@ int imx_learn_train(imx_learn *L, const double *du, const double *dh, int32_t ep
 // nsteps corrected steps: plain RK4, then the MLP correction, scaled down
 // to RMS `bound` (scaled units) where it exceeds it; *nlimited counts the
 // scaled steps.
+int imx_learn_set_limiter(imx_learn *L, int32_t mode) {
+  if (!L || (mode != IMX_LIMIT_FALLBACK && mode != IMX_LIMIT_SCALE))
+    return imx::set_error(IMX_EINVAL, "imx_learn_set_limiter: bad argument");
+  const double m = (double)mode;
+  LTRY(hipMemcpyAsync(L->bound + 1, &m, 8, hipMemcpyHostToDevice, L->s->stream), "imx_learn_set_limiter");
+  LTRY(hipStreamSynchronize(L->s->stream), "imx_learn_set_limiter");
+  return IMX_OK;
+}
+
 int imx_learn_step(imx_learn *L, int32_t nsteps, double bound, int32_t *nlimited) {
   if (!L || nsteps < 0) return imx::set_error(IMX_EINVAL, "imx_learn_step: bad argument");
   imx_swm *s = L->s;

@@ -1,5 +1,5 @@
 // libimx learned correction: per-edge MLP 48 -> 128 -> 128 -> 128 -> 2 with
-// hard-sigmoid GELU, BF16 weights and activations on MFMA, FP32 accumulation;
+// GELU (tanh form), BF16 weights and activations on MFMA, FP32 accumulation;
 // hand-written inference, training (forward + backward), weight-gradient and
 // AdamW kernels.
 //

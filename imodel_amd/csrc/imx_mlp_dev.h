@@ -53,22 +53,23 @@ __device__ __forceinline__ float bf2f(short s) {
   return __bfloat162float(b);
 }
 
-// Activation: GELU with its sigmoid replaced by the hard (piecewise
-// linear) sigmoid matching sigma(1.702 x) at 0,
-//   g(x) = x s(x),  s(x) = clamp(fma(0.4255, x, 0.5), 0, 1),
-// two full-rate vector instructions (an FMA with output clamp, a
-// multiply). Smooth GELU forms need exp and a reciprocal, transcendental
-// instructions that issue at a quarter of the vector rate and bound the
-// network, not the MFMAs.
-// g'(x) = s + 0.4255 x on 0 < s < 1, else s.
-#define GELU_K 0.4255f
+// Activation: GELU in its tanh form (Hendrycks & Gimpel 2016),
+//   g(x) = x/2 (1 + tanh(c (x + a x^3))) = x s(x),
+//   s(x) = sigma(2 c (x + a x^3)) = 1 / (1 + exp(-2 c (x + a x^3))),
+// c = sqrt(2/pi), a = 0.044715: one exponential and one reciprocal
+// (v_exp_f32, v_rcp_f32) per activation. Large |x| saturates s to 0 or 1
+// (exp overflows to inf, whose reciprocal is 0).
+// g'(x) = s + x s (1 - s) 2 c (1 + 3 a x^2).
+#define GELU_C2 1.5957691216057308f  // 2 sqrt(2/pi)
+#define GELU_A 0.044715f
 __device__ __forceinline__ float gelu_s(float x) {
-  return __builtin_amdgcn_fmed3f(__builtin_fmaf(GELU_K, x, 0.5f), 0.0f, 1.0f);
+  const float t = GELU_C2 * __builtin_fmaf(GELU_A * x, x * x, x);
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-t));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
 __device__ __forceinline__ float gelu_d(float x) {
   const float s = gelu_s(x);
-  return (s > 0.0f && s < 1.0f) ? s + GELU_K * x : s;
+  return s + x * s * (1.0f - s) * GELU_C2 * __builtin_fmaf(3.0f * GELU_A, x * x, 1.0f);
 }
 
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {

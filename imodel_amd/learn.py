@@ -170,9 +170,16 @@ class Learn:
                                             capi.f64p(loss)), "imx_learn_fit")
         return float(loss[0]), float(loss[1])
 
+    def set_limiter(self, mode):
+        """What a correction above the RMS bound does: "fallback" (the
+        default) keeps the plain step, "scale" scales it down to the bound."""
+        capi.check(capi.lib().imx_learn_set_limiter(self._h, {"fallback": 0, "scale": 1}[mode]),
+                   "imx_learn_set_limiter")
+
     def step(self, nsteps, bound=1e300):
-        """nsteps corrected steps; corrections above RMS `bound` (scaled
-        units) are scaled down to it. Returns the number of scaled steps."""
+        """nsteps corrected steps; a correction above RMS `bound` (scaled
+        units) is dropped for the plain step (or scaled down to the bound,
+        set_limiter("scale")). Returns the number of limited steps."""
         nlim = np.zeros(1, dtype=np.int32)
         capi.check(capi.lib().imx_learn_step(self._h, int(nsteps), float(bound), capi.i32p(nlim)),
                    "imx_learn_step")

@@ -396,9 +396,13 @@ void imx_mlp_destroy(imx_mlp *m);
    divergence is the h correction, so corrections conserve total mass.
    loss[2] receives the scaled u and h losses. imx_learn_step runs corrected
    steps (one graph launch each): plain step + MLP correction; a
-   correction whose RMS (scaled units) exceeds `bound` is scaled down to
-   RMS `bound` (a non-finite one, or bound 0, keeps the plain step exactly);
-   *nlimited counts the scaled steps. */
+   correction whose RMS (scaled units) exceeds `bound` is dropped and the
+   plain step kept (IMX_LIMIT_FALLBACK, the default), or scaled down to RMS
+   `bound` (IMX_LIMIT_SCALE, imx_learn_set_limiter); a non-finite one, or
+   bound 0, keeps the plain step exactly in both modes; *nlimited counts
+   the limited steps. */
+#define IMX_LIMIT_FALLBACK 0
+#define IMX_LIMIT_SCALE    1
 typedef struct imx_learn imx_learn;
 int  imx_learn_create(imx_swm *coarse, imx_mlp *m, const imx_mesh_desc *d, imx_learn **out);
 int  imx_learn_sample(imx_learn *l, float *raw);
@@ -417,6 +421,7 @@ int  imx_learn_store(imx_learn *l, const double *du, const double *dh, int32_t *
 int  imx_learn_fit(imx_learn *l, int32_t nsteps, float lr0, float lr1, float wd, uint64_t seed,
                    double loss[2]);
 int  imx_learn_step(imx_learn *l, int32_t nsteps, double bound, int32_t *nlimited);
+int  imx_learn_set_limiter(imx_learn *l, int32_t mode);
 int  imx_learn_last_rms(imx_learn *l, double *rms);
 void imx_learn_destroy(imx_learn *l);
 

@@ -1,6 +1,6 @@
 """TEST INFRASTRUCTURE ONLY: FP64 numpy reference of the learned-correction
-MLP (48 -> 128 -> 128 -> 128 -> 2, activation g(x) = x clamp(0.5 + 0.4255 x, 0, 1),
-the hard-sigmoid form of GELU), its loss
+MLP (48 -> 128 -> 128 -> 128 -> 2, activation GELU in its tanh form,
+g(x) = x sigma(2 c (x + 0.044715 x^3)), c = sqrt(2/pi)), its loss
 L = sum_e w_e |y_e - t_e|^2 and its parameter gradients, with optional
 emulation of the BF16 storage the GPU kernels use (weights, layer inputs and
 pre-activation gradients rounded to BF16, round-to-nearest-even).
@@ -35,11 +35,15 @@ def bf16(x):
     return b.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-K_GELU = 0.4255
+# GELU, tanh form (Hendrycks & Gimpel 2016): x/2 (1 + tanh(c (x + a x^3)))
+# = x sigma(2 c (x + a x^3)), c = sqrt(2/pi), a = 0.044715
+GELU_C2 = 2.0 * np.sqrt(2.0 / np.pi)
+GELU_A = 0.044715
 
 
 def _s(x):
-    return np.clip(0.5 + K_GELU * x, 0.0, 1.0)
+    t = GELU_C2 * (x + GELU_A * x ** 3)
+    return 0.5 * (1.0 + np.tanh(0.5 * t))  # sigma(t), overflow-free
 
 
 def gelu(x):
@@ -48,7 +52,7 @@ def gelu(x):
 
 def gelu_d(x):
     s = _s(x)
-    return np.where((s > 0.0) & (s < 1.0), s + K_GELU * x, s)
+    return s + x * s * (1.0 - s) * GELU_C2 * (1.0 + 3.0 * GELU_A * x * x)
 
 
 def forward(p, X, emulate_bf16=False):

@@ -84,11 +84,39 @@ def test_limiter_bound_zero_is_plain(g3):
     assert L.last_rms() > 0.0
 
 
+def test_limiter_falls_back_to_plain_step(g3):
+    """A correction above the bound (the default limiter) is dropped: the
+    corrected step equals the plain step bit for bit and counts as limited;
+    below the bound the correction is applied."""
+    say("learn limiter fallback")
+    s = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
+    p = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
+    L = Learn(s, Mlp(seed=2), g3)
+    h0, u0 = g3.spectral(7)
+    s.set_state(h0, u0)
+    raw = L.sample(raw=True)
+    L.set_scaling(raw.mean(0), raw.std(0) + 1e-3, 50.0, 0.5)
+    s.set_state(h0, u0)
+    assert L.step(1, bound=1e30) == 0
+    r = L.last_rms()
+    hc, uc = s.get_state()
+    s.set_state(h0, u0)
+    assert L.step(1, bound=0.99 * r) == 1
+    hs, us = s.get_state()
+    p.set_state(h0, u0)
+    p.step(1)
+    hp, up = p.get_state()
+    assert np.array_equal(hs, hp) and np.array_equal(us, up)
+    assert not np.array_equal(uc, up)
+
+
 def test_limiter_scales_to_bound(g3):
-    """A bound of a quarter of the correction's RMS applies a quarter of it."""
+    """With the scaling limiter, a bound of a quarter of the correction's
+    RMS applies a quarter of it."""
     say("learn limiter scale")
     s = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
     L = Learn(s, Mlp(seed=2), g3)
+    L.set_limiter("scale")
     h0, u0 = g3.spectral(7)
     s.set_state(h0, u0)
     raw = L.sample(raw=True)
