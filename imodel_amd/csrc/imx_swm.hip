@@ -1193,12 +1193,23 @@ __global__ __launch_bounds__(TPB) void diag_final(int nparts, const double *__re
 }
 
 // ======================================================= bandwidth probe ==
-// Plain streaming copy, used to measure the attainable HBM bandwidth and to
-// scale the memory counters of a profiler (bytes moved are known exactly).
-__global__ __launch_bounds__(TPB) void copy_probe(const float4 *__restrict__ a,
-                                                  float4 *__restrict__ b, size_t n) {
+// Streaming copy, used to measure the attainable HBM bandwidth and to scale
+// the memory counters of a profiler (bytes moved are known exactly). Each
+// lane issues UNR independent 16-byte loads before its stores, so every
+// wave keeps UNR x 1 KiB of reads in flight.
+template <int UNR>
+__global__ __launch_bounds__(TPB) void copy_probe(const float4 *__restrict__ a, float4 *__restrict__ b,
+                                                  size_t n) {
   const size_t stride = (size_t)gridDim.x * TPB;
-  for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += stride) b[i] = a[i];
+  size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+  for (; i + (UNR - 1) * stride < n; i += UNR * stride) {
+    float4 x[UNR];
+#pragma unroll
+    for (int k = 0; k < UNR; ++k) x[k] = a[i + k * stride];
+#pragma unroll
+    for (int k = 0; k < UNR; ++k) b[i + k * stride] = x[k];
+  }
+  for (; i < n; i += stride) b[i] = a[i];
 }
 
 // =============================================================== handle ====
@@ -1417,18 +1428,19 @@ void swm_launch_step(imx_swm *s) { launch_step(s); }
 
 // Diagnostics of the cells [c0, c1) and the edge/triangle slots they own.
 template <typename T>
-static void launch_diag(imx_swm *s, const T *hs, const T *us, double href, int c0, int c1) {
+static void launch_diag(imx_swm *s, const T *hs, const T *us, double href, int c0, int c1, double *part,
+                        double *diag) {
   hipStream_t st = s->stream;
   const int nc = c1 - c0, ne = 3 * nc, nt = 2 * nc;
   const int gv = (nc + TPB - 1) / TPB, ge = (ne + TPB - 1) / TPB, gt = (nt + TPB - 1) / TPB;
   diag_cell<T><<<gv, TPB, 0, st>>>(c0, nc, s->nv, s->c_nnb, s->c_ed, s->sc_hc ? s->hc_c : s->c_ttg, s->c_area,
-                                   s->e_dleng, hs, us, href, s->bt, s->o.grav, s->o.erad, s->part);
+                                   s->e_dleng, hs, us, href, s->bt, s->o.grav, s->o.erad, part);
   diag_edge<T><<<ge, TPB, 0, st>>>(3 * c0, ne, s->NE, s->e_v, s->e_leng, s->e_dleng, hs, us, href,
-                                   s->part + (size_t)gv * NPART);
+                                   part + (size_t)gv * NPART);
   diag_tri<T><<<gt, TPB, 0, st>>>(2 * c0, nt, s->NT, s->t_ed, s->sc_hc ? s->hc_t : s->t_tg, s->t_v, s->t_area, s->t_f,
                                   s->t_trhx, s->e_leng, hs, us, href, s->o.erad,
-                                  s->part + (size_t)(gv + ge) * NPART);
-  diag_final<<<1, TPB, 0, st>>>(gv + ge + gt, s->part, s->diag);
+                                  part + (size_t)(gv + ge) * NPART);
+  diag_final<<<1, TPB, 0, st>>>(gv + ge + gt, part, diag);
 }
 
 // ------------------------------------------------------------ create -------
@@ -2371,10 +2383,15 @@ static int run_steps(imx_swm *s, int nsteps) {
 }
 
 // Diagnostics vector of the cells [c0, c1) (IMX_DIAG_* order, FP64).
+static void launch_diag_state(imx_swm *s, int c0, int c1, double *part, double *diag) {
+  if (!s->fast) launch_diag<double>(s, s->rh, s->ru, 0.0, c0, c1, part, diag);
+  else if (s->single)
+    launch_diag<float>(s, (const float *)s->h0, (const float *)s->u0, s->href, c0, c1, part, diag);
+  else launch_diag<double>(s, (const double *)s->h0, (const double *)s->u0, s->href, c0, c1, part, diag);
+}
+
 static int diag_range(imx_swm *s, int c0, int c1, double *out) {
-  if (!s->fast) launch_diag<double>(s, s->rh, s->ru, 0.0, c0, c1);
-  else if (s->single) launch_diag<float>(s, (const float *)s->h0, (const float *)s->u0, s->href, c0, c1);
-  else launch_diag<double>(s, (const double *)s->h0, (const double *)s->u0, s->href, c0, c1);
+  launch_diag_state(s, c0, c1, s->part, s->diag);
   double v[NPART];
   HIPTRY(hipMemcpyAsync(v, s->diag, sizeof(v), hipMemcpyDeviceToHost, s->stream), "diagnostics");
   HIPTRY(hipStreamSynchronize(s->stream), "diagnostics");
@@ -2979,15 +2996,25 @@ int imx_copy_probe(int64_t bytes, int32_t reps, double *gbs) {
              hipEventCreate(&e1) != hipSuccess) {
     rc = imx::set_error(IMX_EDEVICE, "imx_copy_probe: setup failed");
   } else {
-    const int blocks = 4096;
-    copy_probe<<<blocks, TPB>>>(a, b, n);
-    (void)hipEventRecord(e0, nullptr);
-    for (int r = 0; r < reps; ++r) copy_probe<<<blocks, TPB>>>(a, b, n);
-    (void)hipEventRecord(e1, nullptr);
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-      rc = imx::set_error(IMX_EDEVICE, "imx_copy_probe: kernel failed");
-    else
-      *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+    // the best of a few launch shapes: 4 or 8 loads in flight per lane,
+    // 2048 or 8192 blocks (8 or 32 per CU)
+    double best = 0.0;
+    for (int shape = 0; shape < 4 && rc == IMX_OK; ++shape) {
+      const int blocks = (shape & 1) ? 8192 : 2048;
+      auto launch = [&]() {
+        if (shape < 2) copy_probe<4><<<blocks, TPB>>>(a, b, n);
+        else copy_probe<8><<<blocks, TPB>>>(a, b, n);
+      };
+      launch();
+      (void)hipEventRecord(e0, nullptr);
+      for (int r = 0; r < reps; ++r) launch();
+      (void)hipEventRecord(e1, nullptr);
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+        rc = imx::set_error(IMX_EDEVICE, "imx_copy_probe: kernel failed");
+      else
+        best = std::max(best, 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9);
+    }
+    *gbs = best;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
