@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -2968,6 +2969,77 @@ const char *imx_swm_kernel_name(imx_swm *s, int32_t k) {
 int imx_swm_diagnostics(imx_swm *s, double out[IMX_NDIAG]) {
   if (!s || !out) return imx::set_error(IMX_EINVAL, "imx_swm_diagnostics: NULL argument");
   return imx::diag_range(s, 0, s->nv, out);
+}
+
+// ---- diagnostics without allocation -------------------------------------
+// The block partials of the diagnostics reductions live in a file-scope
+// device array (one per GPU, held until the process ends) instead of the
+// handle's buffers; its device address is looked up once per device with
+// hipGetSymbolAddress. Calls that overlap on one GPU share the array and race.
+#define IMX_DIAG_SCRATCH_DOUBLES (1 << 19)  // 4 MiB: partials of up to 65,535 blocks
+static __device__ __align__(256) double scratch[IMX_DIAG_SCRATCH_DOUBLES];
+
+namespace imx {
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[64] = {};
+
+static int scratch_address(double **out) {
+  int dev = 0;
+  HIPTRY(hipGetDevice(&dev), "diagnostics scratch");
+  if (dev < 0 || dev >= 64) return set_error(IMX_EDEVICE, "diagnostics scratch: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    HIPTRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "diagnostics scratch");
+    g_scratch_dev[dev] = (double *)p;
+  }
+  *out = g_scratch_dev[dev];
+  return IMX_OK;
+}
+}  // namespace imx
+
+// The host finish of diag_range, on the device: mass, energies (with the
+// topography's sums), potential enstrophy, RMS divergence, bad-state flag.
+__global__ void diag_finish(const double *__restrict__ v, double grav, int has_topo, double sb, double sb2,
+                            double *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double mass = v[0] / v[1];
+  const double pe = v[2];
+  const double ke = v[4];
+  double upe;
+  if (has_topo) {
+    const double m = (v[0] + sb) / v[1];
+    upe = 0.5 * grav * (m * m * v[1] - sb2);
+  } else {
+    upe = v[1] * grav * mass * (mass * 0.5);
+  }
+  out[IMX_DIAG_MASS] = mass;
+  out[IMX_DIAG_PENERGY] = pe;
+  out[IMX_DIAG_KENERGY] = ke;
+  out[IMX_DIAG_TENERGY] = pe + ke;
+  out[IMX_DIAG_AENERGY] = (pe + ke) - upe;
+  out[IMX_DIAG_PENSTROPHY] = v[5];
+  out[IMX_DIAG_RMSDIV] = sqrt(v[3] / v[1]);
+  out[IMX_DIAG_BADFLAG] = v[7] > 0 ? 1.0 : 0.0;
+}
+
+int imx_swm_diagnostics_dev(imx_swm *s, double *out_dev) {
+  if (!s || !out_dev) return imx::set_error(IMX_EINVAL, "imx_swm_diagnostics_dev: NULL argument");
+  const size_t need = (size_t)(s->nparts + 1) * NPART;
+  if (need > (size_t)IMX_DIAG_SCRATCH_DOUBLES)
+    return imx::set_error(IMX_EINVAL, "imx_swm_diagnostics_dev: the mesh needs " + std::to_string(need) +
+                                          " scratch doubles, more than the " +
+                                          std::to_string(IMX_DIAG_SCRATCH_DOUBLES) + " it holds (about " +
+                                          std::to_string(IMX_DIAG_SCRATCH_DOUBLES / 8 * 256 / 6) + " cells)");
+  double *sc = nullptr;
+  int rc = imx::scratch_address(&sc);
+  if (rc) return rc;
+  // every block writes all of its partials, so nothing needs clearing
+  double *part = sc, *diag = sc + (size_t)s->nparts * NPART;
+  imx::launch_diag_state(s, 0, s->nv, part, diag);
+  diag_finish<<<1, 64, 0, s->stream>>>(diag, s->o.grav, s->has_topo ? 1 : 0, s->topo_sb, s->topo_sb2, out_dev);
+  HIPTRY(hipGetLastError(), "imx_swm_diagnostics_dev");
+  return IMX_OK;
 }
 
 int imx_swm_info(imx_swm *s, int64_t out[8]) {

@@ -329,6 +329,15 @@ int  imx_swm_step_timed(imx_swm *s, int32_t nsteps, double *ms);
 int  imx_swm_profile(imx_swm *s, int32_t nsteps, double *ms, int32_t *nk);
 const char *imx_swm_kernel_name(imx_swm *s, int32_t k);
 int  imx_swm_diagnostics(imx_swm *s, double out[IMX_NDIAG]);
+/* No counterpart in the reference: for callers that cannot allocate (code
+   under HIP graph capture, tight serving loops). The diagnostics of
+   imx_swm_diagnostics written into the caller's device buffer
+   out_dev[IMX_NDIAG], enqueued on the handle's stream without waiting and
+   without allocating: the reduction partials live in a static device array
+   of the library (4 MiB per GPU, held until the process ends), enough for
+   meshes up to about 2.79 million cells (larger ones return IMX_EINVAL).
+   The array is shared by every call on a GPU: calls that overlap race. */
+int  imx_swm_diagnostics_dev(imx_swm *s, double *out_dev);
 int  imx_swm_info(imx_swm *s, int64_t out[8]);   /* nv ne nt bytes_dev ... */
 void imx_swm_destroy(imx_swm *s);
 
