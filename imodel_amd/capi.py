@@ -33,7 +33,10 @@ LTE_ROWS = ("h_ed", "divuh", "kenergy", "abs_vort", "h_tr", "q_tr", "q_hx", "q_e
             "momeq")
 LTE_MIM = ("hdivu", "prodrule", "divuh", "proddiff", "crossrule", "coriolis", "coriolis_rel")
 COEF = {"const": 0, "align": 1, "diam": 2}
-WRAP = {"trsk10": 0, "pxt16": 1, "gass18": 2}
+WRAP = {"trsk10": 0, "pxt16": 1, "gass18": 2, "none": 3}
+SINT = {"trsk": 0, "bary": 1, "gass": 2}
+COR = {"trsk": 0, "pered": 1, "gass": 2, "dtred": 3}
+REC = {"trsk": 0, "perhx": 1, "gass": 2, "melv": 3}
 STAG = {"HTC": 0, "HC": 1}
 
 _I32P = ctypes.POINTER(ctypes.c_int32)
@@ -46,7 +49,9 @@ class Scheme(ctypes.Structure):
     _fields_ = [("stag", ctypes.c_int32), ("wrap", ctypes.c_int32), ("apvm", ctypes.c_int32),
                 ("pvspar", ctypes.c_double),
                 ("k2", ctypes.c_double),
-                ("diffus", ctypes.c_int32), ("k4", ctypes.c_double), ("hyperdiffus", ctypes.c_int32)]
+                ("diffus", ctypes.c_int32), ("k4", ctypes.c_double), ("hyperdiffus", ctypes.c_int32),
+                ("sinterp", ctypes.c_int32), ("coriolis", ctypes.c_int32), ("recon", ctypes.c_int32),
+                ("gasscoef", ctypes.c_double), ("nopv", ctypes.c_int32)]
 
 
 class MeshDesc(ctypes.Structure):

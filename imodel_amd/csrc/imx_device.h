@@ -134,6 +134,14 @@ struct imx_swm {
   imx_scheme sc{};
   bool sc_apvm = false, sc_diff = false, sc_hyper = false;
   int sc_wrap = 0;                   // IMX_WRAP_*
+  // the methods the wrapper (or, with IMX_WRAP_NONE, the caller) selects
+  int sc_sint = IMX_SINT_TRSK, sc_cor = IMX_COR_TRSK, sc_rec = IMX_REC_TRSK;
+  double sc_gcoef = 1.0;
+  bool sc_nopv = false;
+  double *t_ka = nullptr;            // [3][NT] Gassmann K: areas of (circumcentre, edge ends)
+  int32_t *c_tr = nullptr;           // [MX][nv] ring triangles of each cell
+  double *c_hxtr = nullptr;          // [MX][nv] hxtr_area of each ring triangle
+  double *f_ketr = nullptr, *f_eta_ed = nullptr, *f_vhq = nullptr, *f_vuh = nullptr;
   bool sc_hc = false;                // HC staggering
   int8_t *hc_c = nullptr;            // [MX][nv] HC: hx%nr per cell slot (div, Perot, Coriolis)
   int8_t *hc_t = nullptr;            // [3][NT] HC: vorticity signs

@@ -81,7 +81,7 @@ int swm_stencil_rings(const imx_swm *s);
 // dif_coef_hx [nv] / dif_coef_tr [nt] as swm_tests sets them up
 // (src/swm.f90:170-210), caller numbering; kind IMX_COEF_*. With k4 > 0
 // both hold the square roots of the hyperdiffusion coefficients.
-void diffusion_coef(const imx_mesh_desc &d, const double *trhx, int wrap, double k2, int k2kind,
+void diffusion_coef(const imx_mesh_desc &d, const double *trhx, bool bary, double k2, int k2kind,
                     double k4, int k4kind, std::vector<double> &hx, std::vector<double> &tr);
 // bar_coord_tr (src/smeshpack.f90:5200-5245) of point p in triangle k, and
 // of triangle k's circumcentre

@@ -1009,13 +1009,13 @@ int hxedgeconn(const imx_mesh_desc &d, int ed1, int ed2) {
 }
 
 // scalar_hx2tr (src/swm_operators.f90:258-269): trsk area weights, or the
-// barycentric coordinates of the circumcentre for pxt16
-static void hx2tr(const imx_mesh_desc &d, const double *trhx, int wrap, const std::vector<double> &f,
+// barycentric coordinates of the circumcentre (bary interpolation)
+static void hx2tr(const imx_mesh_desc &d, const double *trhx, bool bary, const std::vector<double> &f,
                   std::vector<double> &out) {
   out.assign(d.nt, 0.0);
   for (int k = 0; k < d.nt; ++k) {
     double s = 0.0;
-    if (wrap == IMX_WRAP_PXT16) {
+    if (bary) {
       double b[3];
       bary_circumcentre(d, k, b);
       for (int i = 0; i < 3; ++i) s = s + b[i] * f[d.tr_v[3 * k + i] - 1];
@@ -1027,17 +1027,17 @@ static void hx2tr(const imx_mesh_desc &d, const double *trhx, int wrap, const st
   }
 }
 
-void diffusion_coef(const imx_mesh_desc &d, const double *trhx, int wrap, double k2, int k2kind,
+void diffusion_coef(const imx_mesh_desc &d, const double *trhx, bool bary, double k2, int k2kind,
                     double k4, int k4kind, std::vector<double> &hx, std::vector<double> &tr) {
   hx.assign(d.nv, 0.0);
   tr.assign(d.nt, 0.0);
   if (k2 > 0.0) {
     coef_at_cells(d, k2, k2kind, hx);
-    hx2tr(d, trhx, wrap, hx, tr);
+    hx2tr(d, trhx, bary, hx, tr);
   }
   if (k4 > 0.0) {
     coef_at_cells(d, k4, k4kind, hx);
-    hx2tr(d, trhx, wrap, hx, tr);
+    hx2tr(d, trhx, bary, hx, tr);
     for (double &x : hx) x = std::sqrt(x);
     for (double &x : tr) x = std::sqrt(x);
   }

@@ -152,19 +152,24 @@ __global__ __launch_bounds__(TPB) void ref_tri(int nt, const int32_t *__restrict
 // swm_operators.f90:152-153) or distance-weighted (1, pxt16 on HTC, :163-168);
 // or (2, gass18) eta and h averaged over the rhombus with the triangle areas
 // (:155-162) and divided (swm.f90:2469-2473, zero guard as the reference's)
+// The vorticity at edges (eta_ed, scalar_tr2ed of eta with the same
+// interpolation, swm.f90:2465) is written when eta_ed is not NULL (the level
+// model's Coriolis term reads it).
 __global__ __launch_bounds__(TPB) void ref_edge_q(int ne, int qmode, const int32_t *__restrict__ e_sh,
                                                   const double *__restrict__ e_dw,
                                                   const double *__restrict__ t_area,
                                                   const double *__restrict__ eta,
                                                   const double *__restrict__ h_tr,
                                                   const double *__restrict__ q_tr,
-                                                  double *__restrict__ q_ed, int *__restrict__ flag) {
+                                                  double *__restrict__ q_ed, double *__restrict__ eta_ed,
+                                                  int *__restrict__ flag) {
   const int l = xcd_index();
   if (l >= ne || e_sh[l] < 0) return;
   const int k1 = e_sh[l], k2 = e_sh[ne + l];
   if (qmode == 1) {
     const double d1 = e_dw[l], d2 = e_dw[ne + l];
     q_ed[l] = (d2 * q_tr[k1] + d1 * q_tr[k2]) / (d1 + d2);
+    if (eta_ed) eta_ed[l] = (d2 * eta[k1] + d1 * eta[k2]) / (d1 + d2);
   } else if (qmode >= 2) {
     // gass18: rhombus averages with the triangle areas, or (HC, qmode 3)
     // the simple averages scalar_tr2ed takes on HC
@@ -177,6 +182,7 @@ __global__ __launch_bounds__(TPB) void ref_edge_q(int ne, int qmode, const int32
       er = 0.5 * (eta[k1] + eta[k2]);
       hr = 0.5 * (h_tr[k1] + h_tr[k2]);
     }
+    if (eta_ed) eta_ed[l] = er;
     if (fabs(hr) < HTR_GUARD) {
       flag[0] = 1;
       q_ed[l] = 0.0;
@@ -185,7 +191,26 @@ __global__ __launch_bounds__(TPB) void ref_edge_q(int ne, int qmode, const int32
     }
   } else {
     q_ed[l] = 0.5 * (q_tr[k1] + q_tr[k2]);
+    if (eta_ed) eta_ed[l] = 0.5 * (eta[k1] + eta[k2]);
   }
+}
+
+// kinetic_energy_tr (swm_operators.f90:1074-1130, geodesic areas): K at
+// triangles, sum over the edges of the area of (circumcentre, edge ends)
+// times u^2, over the triangle's area (Gassmann's kinetic energy)
+__global__ __launch_bounds__(TPB) void ref_ketr(int nt, const int32_t *__restrict__ t_ed,
+                                                const double *__restrict__ t_ka,
+                                                const double *__restrict__ t_area,
+                                                const double *__restrict__ u, double *__restrict__ ketr) {
+  const int k = xcd_index();
+  if (k >= nt || t_ed[k] < 0) return;
+  double s = 0.0;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const double uu = u[t_ed[l * nt + k]];
+    s = s + t_ka[l * nt + k] * (uu * uu);
+  }
+  ketr[k] = s / t_area[k];
 }
 
 // div_hx(uh), div_hx(u), kinetic_energy_hx, g(h+b)+K
@@ -199,9 +224,13 @@ __global__ __launch_bounds__(TPB) void ref_cell(int nv, const int32_t *__restric
                                                 const double *__restrict__ u,
                                                 const double *__restrict__ uh,
                                                 const double *__restrict__ h,
-                                                const double *__restrict__ bt,
-                                                const double *__restrict__ vhx, double erad,
-                                                double grav, double *__restrict__ divuh,
+                                                const double *__restrict__ bt, int kmode,
+                                                const double *__restrict__ vhx,
+                                                const double *__restrict__ vuh,
+                                                const double *__restrict__ ketr,
+                                                const int32_t *__restrict__ c_tr,
+                                                const double *__restrict__ c_hxtr, double gcoef,
+                                                double erad, double grav, double *__restrict__ divuh,
                                                 double *__restrict__ masseq,
                                                 double *__restrict__ divu,
                                                 double *__restrict__ ke,
@@ -217,20 +246,35 @@ __global__ __launch_bounds__(TPB) void ref_cell(int nv, const int32_t *__restric
     sd = sd + sg * u[l] * e_dleng[l];
     const double ed_area = 0.25 * e_leng[l] * e_dleng[l];
     ca = ca + ed_area;
-    k = k + ed_area * (u[l] * u[l]);
+    // melv (HTC): sum of ed_area uh u
+    k = (kmode == IMX_REC_MELV) ? k + ed_area * uh[l] * u[l] : k + ed_area * (u[l] * u[l]);
   }
   const double dv = s / c_area[i] / erad;
   divuh[i] = dv;
   masseq[i] = -dv;
   divu[i] = sd / c_area[i] / erad;
   double kk = k / ca;
-  if (vhx) {  // kinetic_energy_hx perhx (pxt16): |v|^2 / 2 of the Perot vector
+  if (kmode == IMX_REC_PERHX) {  // perhx (pxt16): |v|^2 / 2 of the Perot vector
     const double v0 = vhx[i], v1 = vhx[nv + i], v2 = vhx[2 * nv + i];
     double dd = 0.0;
     dd = dd + v0 * v0;
     dd = dd + v1 * v1;
     dd = dd + v2 * v2;
     kk = dd / 2.0;
+  } else if (kmode == IMX_REC_MELV) {
+    if (vuh) {  // HC: Perot vector of uh dotted with that of u, over 2 h
+      double dd = 0.0;
+      dd = dd + vuh[i] * vhx[i];
+      dd = dd + vuh[nv + i] * vhx[nv + i];
+      dd = dd + vuh[2 * nv + i] * vhx[2 * nv + i];
+      kk = dd / h[i] / 2.0;
+    } else {
+      kk = k / h[i] / ca;
+    }
+  } else if (kmode == IMX_REC_GASS) {
+    // Gassmann: the trsk K blended with the ring triangles' K
+    kk = kk * gcoef;
+    for (int j = 0; j < n; ++j) kk = kk + (1.0 - gcoef) * c_hxtr[j * nv + i] * ketr[c_tr[j * nv + i]];
   }
   ke[i] = kk;
   ghbk[i] = grav * (h[i] + bt[i]) + kk;
@@ -524,6 +568,48 @@ __global__ __launch_bounds__(TPB) void ref_rvort(int nt, const int32_t *__restri
 // reconstruction of uh q at the edge's two cells, averaged, projected at the
 // edge point, then (HTC) minus its ed%nr component or (HC) its edhx%tg
 // component -> momeq
+// coriolis_ed dtred (swm_operators.f90:1310-1329): the Perot vector of
+// uh q at the edge's two triangles (vhq_tr) interpolated to the edge, with
+// the distances to the circumcentres and projected on the tangent plane
+// (HTC, -component along ed%nr), or averaged (HC, component along edhx%tg)
+__global__ __launch_bounds__(TPB) void ref_edge_mom_dtred(
+    int ne, int nt, const int32_t *__restrict__ e_v, const int32_t *__restrict__ e_sh,
+    const double *__restrict__ e_leng, const double *__restrict__ e_dw, const double *__restrict__ e_c,
+    const double *__restrict__ e_nr, const double *__restrict__ vhq, const double *__restrict__ ghbk,
+    const int8_t *__restrict__ e_gs, int hc, double erad, int zero_mom, double *__restrict__ grad,
+    double *__restrict__ uhq, double *__restrict__ momeq) {
+  const int l = xcd_index();
+  if (l >= ne || e_v[l] < 0) return;
+  const double gs = e_gs ? (double)e_gs[l] : 1.0;
+  const double g = gs * (ghbk[e_v[ne + l]] - ghbk[e_v[l]]) / e_leng[l] / erad;
+  grad[l] = g;
+  const int k1 = e_sh[l], k2 = e_sh[ne + l];
+  double v0, v1, v2, p;
+  if (!hc) {
+    const double d1 = e_dw[l], d2 = e_dw[ne + l];
+    v0 = (d2 * vhq[k1] + d1 * vhq[k2]) / (d1 + d2);
+    v1 = (d2 * vhq[nt + k1] + d1 * vhq[nt + k2]) / (d1 + d2);
+    v2 = (d2 * vhq[2 * nt + k1] + d1 * vhq[2 * nt + k2]) / (d1 + d2);
+    proj_sphere(v0, v1, v2, e_c[l], e_c[ne + l], e_c[2 * ne + l]);
+    double d = 0.0;
+    d = d + v0 * e_nr[l];
+    d = d + v1 * e_nr[ne + l];
+    d = d + v2 * e_nr[2 * ne + l];
+    p = -d;
+  } else {
+    v0 = 0.5 * vhq[k1] + 0.5 * vhq[k2];
+    v1 = 0.5 * vhq[nt + k1] + 0.5 * vhq[nt + k2];
+    v2 = 0.5 * vhq[2 * nt + k1] + 0.5 * vhq[2 * nt + k2];
+    double d = 0.0;
+    d = d + v0 * e_nr[l];
+    d = d + v1 * e_nr[ne + l];
+    d = d + v2 * e_nr[2 * ne + l];
+    p = d;
+  }
+  uhq[l] = p;
+  momeq[l] = zero_mom ? 0.0 : (-g) - p;
+}
+
 __global__ __launch_bounds__(TPB) void ref_edge_mom_pered(
     int ne, int nv, const int32_t *__restrict__ e_v, const double *__restrict__ e_leng,
     const double *__restrict__ e_dleng, const double *__restrict__ e_c,
@@ -1238,28 +1324,38 @@ static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, do
                                 double *momeq) {
   const int nv = s->nv, NE = s->NE, NT = s->NT;
   hipStream_t st = s->stream;
-  const int wrap = s->sc_wrap;
+  // the method triple: scalar interpolation, Coriolis, kinetic energy
+  const int sint = s->sc_sint, cor = s->sc_cor, rec = s->sc_rec;
   // the staggering's signs and Perot edge points (HC: hx%nr, edhx%c, ...)
   const bool hc = s->sc_hc;
   const int8_t *csg = hc ? s->hc_c : s->c_ttg, *ctn = hc ? s->hc_c : s->c_tnr;
   const int8_t *tsg = hc ? s->hc_t : s->t_tg, *egs = hc ? s->hc_g : nullptr;
   const int8_t *ett = hc ? s->hc_tt : s->e_nrs;
   const double *epc = hc ? s->hc_pc : s->e_c;
-  const int qmode = wrap == IMX_WRAP_GASS18 ? (hc ? 3 : 2) : (!hc && wrap == IMX_WRAP_PXT16 ? 1 : 0);
-  const bool hbary = hc && wrap == IMX_WRAP_PXT16;
+  const int qmode = sint == IMX_SINT_GASS ? (hc ? 3 : 2) : (!hc && sint == IMX_SINT_BARY ? 1 : 0);
+  const bool hbary = hc && sint == IMX_SINT_BARY;
   ref_edge_flux<<<grid_for(NE), TPB, 0, st>>>(NE, s->e_v, hbary ? s->hc_hb : nullptr, s->hc_hv, h, u,
                                               s->f_h_ed, s->f_uh);
   ref_tri<<<grid_for(NT), TPB, 0, st>>>(NT, s->t_ed, tsg, s->t_v, s->t_area, s->t_f,
-                                        s->t_trhx, s->e_leng, u, h, wrap, s->t_w, s->f_h_ed,
+                                        s->t_trhx, s->e_leng, u, h, sint, s->t_w, s->f_h_ed,
                                         s->o.erad, s->o.testcase >= 2, s->f_eta, s->f_zeta,
                                         s->f_h_tr, s->f_q_tr, s->flag);
   ref_edge_q<<<grid_for(NE), TPB, 0, st>>>(NE, qmode, s->e_sh, s->e_dw, s->t_area, s->f_eta, s->f_h_tr,
-                                           s->f_q_tr, s->f_q_ed, s->flag);
-  // Perot velocity at cells: APVM and the pxt16 kinetic energy
+                                           s->f_q_tr, s->f_q_ed, s->sc_nopv ? s->f_eta_ed : nullptr,
+                                           s->flag);
+  // Perot velocity at cells: APVM, the pxt16 and Melvin kinetic energies
   // (vector_edhx2hx_perot, swm.f90:2516)
-  if (s->sc_apvm || wrap == IMX_WRAP_PXT16)
+  if (s->sc_apvm || rec == IMX_REC_PERHX || (hc && rec == IMX_REC_MELV))
     ref_perot_hx<<<grid_for(nv), TPB, 0, st>>>(nv, NE, s->c_nnb, s->c_ed, csg, epc, s->c_p,
                                                s->e_dleng, s->c_area, u, s->f_vhx);
+  if (hc && rec == IMX_REC_MELV)  // ... and of uh
+    ref_perot_hx<<<grid_for(nv), TPB, 0, st>>>(nv, NE, s->c_nnb, s->c_ed, csg, epc, s->c_p,
+                                               s->e_dleng, s->c_area, s->f_uh, s->f_vuh);
+  if (rec == IMX_REC_GASS)  // kinetic_energy_tr (swm.f90:2521-2523)
+    ref_ketr<<<grid_for(NT), TPB, 0, st>>>(NT, s->t_ed, s->t_ka, s->t_area, u, s->f_ketr);
+  if (cor == IMX_COR_DTRED)  // vhq_tr (swm.f90:2501-2503)
+    ref_lte_tr<<<grid_for(NT), TPB, 0, st>>>(NT, NE, s->t_ed, tsg, epc, s->t_c, s->e_leng, s->t_area, s->f_uh,
+                                             s->f_q_tr, s->f_vhq);
   if (s->sc_apvm) {
     // grad q to edges and triangles, then the APVM PV at edges
     // (tendency, swm.f90:2481-2492, :2556-2559)
@@ -1271,10 +1367,15 @@ static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, do
                                              s->sc.pvspar, s->o.dt, s->f_q_ed);
   }
   ref_cell<<<grid_for(nv), TPB, 0, st>>>(nv, s->c_nnb, s->c_ed, csg, s->c_area, s->e_leng,
-                                         s->e_dleng, u, s->f_uh, h, s->bt,
-                                         wrap == IMX_WRAP_PXT16 ? s->f_vhx : nullptr, s->o.erad,
-                                         s->o.grav, s->f_divuh, masseq, s->f_divu, s->f_ke, s->f_ghbk);
-  if (wrap == IMX_WRAP_PXT16)
+                                         s->e_dleng, u, s->f_uh, h, s->bt, rec, s->f_vhx,
+                                         (hc && rec == IMX_REC_MELV) ? s->f_vuh : nullptr, s->f_ketr, s->c_tr,
+                                         s->c_hxtr, s->sc_gcoef, s->o.erad, s->o.grav, s->f_divuh, masseq,
+                                         s->f_divu, s->f_ke, s->f_ghbk);
+  if (cor == IMX_COR_DTRED)
+    ref_edge_mom_dtred<<<grid_for(NE), TPB, 0, st>>>(NE, NT, s->e_v, s->e_sh, s->e_leng, s->e_dw, s->e_c, s->e_nr,
+                                                     s->f_vhq, s->f_ghbk, egs, hc ? 1 : 0, s->o.erad,
+                                                     s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
+  else if (cor == IMX_COR_PERED)
     ref_edge_mom_pered<<<grid_for(NE), TPB, 0, st>>>(NE, nv, s->e_v, s->e_leng, s->e_dleng, epc,
                                                      s->e_nr, s->c_nnb, s->c_ed, csg, s->c_p,
                                                      s->c_area, s->f_ghbk, s->f_uh, s->f_q_ed, egs,
@@ -1283,9 +1384,10 @@ static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, do
   else
     ref_edge_mom<<<grid_for(NE), TPB, 0, st>>>(NE, nv, s->e_v, s->e_loc, s->e_leng, s->e_dleng,
                                                s->c_nnb, s->c_ed, csg, ctn, s->e_tk, s->e_tw, s->e_tdl,
-                                               hc ? s->hc_ts : s->e_ts,
-                                               s->f_ghbk, s->f_uh, s->f_q_ed,
-                                               wrap == IMX_WRAP_GASS18 ? s->e_ed3 : nullptr, egs, s->o.erad,
+                                               hc ? s->hc_ts : s->e_ts, s->f_ghbk,
+                                               s->sc_nopv ? u : s->f_uh,             // level model:
+                                               s->sc_nopv ? s->f_eta_ed : s->f_q_ed,  // u and eta
+                                               cor == IMX_COR_GASS ? s->e_ed3 : nullptr, egs, s->o.erad,
                                                s->o.testcase <= 1, s->f_grad, s->f_uhq, momeq);
   if (s->sc_diff || s->sc_hyper) {
     // diffusion_ed of u (momeq + lapu with K2), then with K4 diffusion_ed of
@@ -1616,6 +1718,13 @@ static int scheme_tables(imx_swm *s, const imx_mesh_desc *d) {
   s->hc_pc = A.get<double>((size_t)3 * NE);
   s->hc_hb = A.get<double>((size_t)3 * NE);
   s->hc_hv = A.get<int32_t>((size_t)3 * NE);
+  s->t_ka = A.get<double>((size_t)3 * NT);
+  s->c_tr = A.get<int32_t>((size_t)MX * nv);
+  s->c_hxtr = A.get<double>((size_t)MX * nv);
+  s->f_ketr = A.get<double>(NT);
+  s->f_eta_ed = A.get<double>(NE);
+  s->f_vhq = A.get<double>((size_t)3 * NT);
+  s->f_vuh = A.get<double>((size_t)3 * nv);
   if (A.err != hipSuccess) return hip_fail(A.err, "scheme tables");
   return IMX_OK;
 }
@@ -1665,7 +1774,7 @@ static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &
   std::vector<double> hx, tr;
   // with K4 > 0 both terms take the square roots of the hyperdiffusion
   // coefficients, as the reference sets them up
-  diffusion_coef(d, trhx, sc.wrap, sc.k2, sc.diffus, sc.k4, sc.hyperdiffus, hx, tr);
+  diffusion_coef(d, trhx, sc.wrap == IMX_WRAP_PXT16, sc.k2, sc.diffus, sc.k4, sc.hyperdiffus, hx, tr);
   std::vector<double> kh(nv, 0.0), kt(NT, 0.0);
   for (int co = 0; co < nv; ++co) kh[cn[co]] = hx[co];
   for (int ko = 0; ko < d.nt; ++ko) kt[tn[ko]] = tr[ko];
@@ -1751,7 +1860,9 @@ static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &
 
 // HC sign tables and Voronoi-midpoint points (the REF kernels' HC branches
 // and, for fused handles, the diagnostics), built from the geometry.
-static int hc_tables(imx_swm *s, const imx_mesh_desc *d, int wrap) {
+// hbary: barycentric h at the dual edge midpoints (bary interpolation);
+// tg: e_nr takes edhx%tg (pered and dtred Coriolis)
+static int hc_tables(imx_swm *s, const imx_mesh_desc *d, bool hbary, bool tg) {
   const int nv = s->nv, NE = s->NE, NT = s->NT;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
   // HC signs from the geometry: s_e = sign(ed%tg . edhx%nr) turns the
@@ -1786,7 +1897,7 @@ static int hc_tables(imx_swm *s, const imx_mesh_desc *d, int wrap) {
       pc[(size_t)q * NE + e] = hxc[q];
       tgv[(size_t)q * NE + e] = htg[q];
     }
-    if (wrap == IMX_WRAP_PXT16) {  // h at the dual edge midpoint (initialize_gridprop, swm.f90:1007-1018)
+    if (hbary) {  // h at the dual edge midpoint (initialize_gridprop, swm.f90:1007-1018)
       const int kt = insidetrmesh(*d, hxc, d->ed_sh[2 * eo] - 1) ? d->ed_sh[2 * eo] - 1 : d->ed_sh[2 * eo + 1] - 1;
       double b[3];
       bary_at(*d, hxc, kt, b);
@@ -1822,8 +1933,8 @@ static int hc_tables(imx_swm *s, const imx_mesh_desc *d, int wrap) {
   HIPTRY(hipMemcpy(s->hc_pc, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
   HIPTRY(hipMemcpy(s->hc_hb, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
   HIPTRY(hipMemcpy(s->hc_hv, hv.data(), hv.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
-  // pered on HC takes the edhx%tg component
-  if (wrap == IMX_WRAP_PXT16)
+  // pered and dtred on HC take the edhx%tg component
+  if (tg)
     HIPTRY(hipMemcpy(s->e_nr, tgv.data(), tgv.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
   return IMX_OK;
 }
@@ -2351,7 +2462,8 @@ static void set_href(imx_swm *s, const double *h) {
 }
 
 int swm_stencil_rings(const imx_swm *s) {
-  return 1 + (s->sc_apvm ? 1 : 0) + (s->sc_hyper ? 1 : 0) + (s->sc_wrap == IMX_WRAP_GASS18 ? 1 : 0);
+  return 1 + (s->sc_apvm ? 1 : 0) + (s->sc_hyper ? 1 : 0) + (s->sc_cor == IMX_COR_GASS ? 1 : 0) +
+         (s->sc_rec == IMX_REC_GASS ? 1 : 0);
 }
 
 static void drop_graph(imx_swm *s) {
@@ -2715,7 +2827,7 @@ int imx_swm_lte(imx_swm *s, const imx_mesh *m, double *err, double *mim) {
   std::vector<int32_t> ctr((size_t)MX * nv, 0);
   std::vector<double> cw((size_t)MX * nv, 0.0), hxtr_v, trhx_v, trskw_v;
   const double *hxtr = d.hx_hxtr_area;
-  if (!hxtr && s->sc_wrap != IMX_WRAP_PXT16) {
+  if (!hxtr && s->sc_sint != IMX_SINT_BARY) {
     trhx_v.assign((size_t)3 * nt, 0.0);
     hxtr_v.assign((size_t)MX * nv, 0.0);
     trskw_v.assign((size_t)MX * MX * nv, 0.0);
@@ -2724,10 +2836,10 @@ int imx_swm_lte(imx_swm *s, const imx_mesh *m, double *err, double *mim) {
   }
   for (int co = 0; co < nv; ++co) {
     double w[MX];
-    if (s->sc_wrap == IMX_WRAP_PXT16) imx::wachspress_node(d, co, w);
+    if (s->sc_sint == IMX_SINT_BARY) imx::wachspress_node(d, co, w);
     for (int j = 0; j < d.v_nnb[co]; ++j) {
       ctr[(size_t)j * nv + cn[co]] = tn[d.v_tr[MX * co + j] - 1];
-      cw[(size_t)j * nv + cn[co]] = s->sc_wrap == IMX_WRAP_PXT16 ? w[j] : hxtr[MX * co + j];
+      cw[(size_t)j * nv + cn[co]] = s->sc_sint == IMX_SINT_BARY ? w[j] : hxtr[MX * co + j];
     }
   }
   HIPTRY(hipMemcpy(s->lte_ctr, ctr.data(), ctr.size() * sizeof(int32_t), hipMemcpyHostToDevice), "imx_swm_lte");
@@ -2772,8 +2884,26 @@ int imx_swm_lte(imx_swm *s, const imx_mesh *m, double *err, double *mim) {
                    h_tr.data(), q_tr.data(), q_hx.data(), q_ed.data(), grad.data(), vhq_tr.data(),
                    qg_tr.empty() ? nullptr : qg_tr.data(), vh_hx.data(), vhq_hx.data(), uhq.data(),
                    lapu.empty() ? nullptr : lapu.data(), mass.data(), mom.data()};
-  return imx::lte_analysis(*m, hc, s->sc_wrap == IMX_WRAP_PXT16, s->o.erad, s->o.grav, f, err, mim);
+  return imx::lte_analysis(*m, hc, s->sc_sint == IMX_SINT_BARY, s->o.erad, s->o.grav, f, err, mim);
 }
+
+namespace imx {
+// The method triple each wrapper stands for (swm_data.f90:443-517): trsk10 =
+// (trsk, trsk, trsk), pxt16 = (bary, pered, perhx), gass18 = (gass, gass,
+// trsk) for (scalar interpolation, Coriolis, kinetic energy)
+static void wrapper_methods(int wrap, int &sint, int &cor, int &rec) {
+  sint = IMX_SINT_TRSK, cor = IMX_COR_TRSK, rec = IMX_REC_TRSK;
+  if (wrap == IMX_WRAP_PXT16) sint = IMX_SINT_BARY, cor = IMX_COR_PERED, rec = IMX_REC_PERHX;
+  if (wrap == IMX_WRAP_GASS18) sint = IMX_SINT_GASS, cor = IMX_COR_GASS;
+}
+
+// area of the triangle (circumcentre of k, ends of its edge i): the
+// Gassmann h_tr weights and kinetic_energy_tr's areas (swm_operators.f90:1112-1117)
+static double gass_subarea(const imx_mesh_desc &d, int ko, int i) {
+  const int eo = d.tr_ed[3 * ko + i] - 1;
+  return sphtriarea(d.tr_c + 3 * ko, d.v_p + 3 * (d.ed_v[2 * eo] - 1), d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1));
+}
+}  // namespace imx
 
 int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc) {
   if (!s || !d || !sc) return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: NULL argument");
@@ -2782,70 +2912,121 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   if (!(sc->k2 >= 0.0) || !(sc->k4 >= 0.0) || !std::isfinite(sc->k2) || !std::isfinite(sc->k4) ||
       !std::isfinite(sc->pvspar) || sc->diffus < IMX_COEF_CONST || sc->diffus > IMX_COEF_DIAM ||
       sc->hyperdiffus < IMX_COEF_CONST || sc->hyperdiffus > IMX_COEF_DIAM ||
-      sc->wrap < IMX_WRAP_TRSK10 || sc->wrap > IMX_WRAP_GASS18)
+      sc->wrap < IMX_WRAP_TRSK10 || sc->wrap > IMX_WRAP_NONE || (sc->nopv != 0 && sc->nopv != 1))
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: bad option value");
-  if (sc->wrap == IMX_WRAP_GASS18 && (sc->k2 > 0.0 || sc->k4 > 0.0))
+  int sint, cor, rec;
+  double gcoef = 1.0;
+  if (sc->wrap == IMX_WRAP_NONE) {
+    if (sc->sinterp < IMX_SINT_TRSK || sc->sinterp > IMX_SINT_GASS || sc->coriolis < IMX_COR_TRSK ||
+        sc->coriolis > IMX_COR_DTRED || sc->recon < IMX_REC_TRSK || sc->recon > IMX_REC_MELV ||
+        (sc->recon == IMX_REC_GASS && !std::isfinite(sc->gasscoef)))
+      return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: bad method value (sinterp, coriolis, recon, gasscoef)");
+    sint = sc->sinterp, cor = sc->coriolis, rec = sc->recon;
+    if (rec == IMX_REC_GASS) gcoef = sc->gasscoef;
+  } else {
+    imx::wrapper_methods(sc->wrap, sint, cor, rec);
+  }
+  if (sc->nopv && (cor == IMX_COR_PERED || cor == IMX_COR_DTRED))
     return imx::set_error(IMX_EINVAL,
-                          "imx_swm_set_scheme: gass18 with diffusion (the reference stops: "
+                          "imx_swm_set_scheme: nopv with pered or dtred Coriolis (the reference stops: these "
+                          "reconstructions need the PV)");
+  if (sint == IMX_SINT_GASS && (sc->k2 > 0.0 || sc->k4 > 0.0))
+    return imx::set_error(IMX_EINVAL,
+                          "imx_swm_set_scheme: gass interpolation with diffusion (the reference stops: "
                           "scalar_hx2tr needs edge values for Gassmann's scheme)");
-  if (sc->wrap != IMX_WRAP_TRSK10 && (!d->ed_c || !d->tr_c || !d->v_p))
-    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: pxt16/gass18 need ed_c, tr_c and v_p");
+  const bool plain = sint == IMX_SINT_TRSK && cor == IMX_COR_TRSK && rec == IMX_REC_TRSK;
+  if (!plain && (!d->ed_c || !d->tr_c || !d->v_p))
+    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: methods other than trsk need ed_c, tr_c and v_p");
   if (sc->stag != IMX_STAG_HTC && sc->stag != IMX_STAG_HC)
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: bad option value (stag)");
   if (sc->stag == IMX_STAG_HC && (!d->ed_c || !d->tr_c || !d->v_p || !d->edhx_c))
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: HC needs ed_c, edhx_c, tr_c and v_p");
   if ((sc->k2 > 0.0 || sc->k4 > 0.0) && (!d->v_tr || !d->tr_c || !d->v_nb))
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: diffusion needs v_tr, v_nb and tr_c");
-  if (sc->apvm && (!d->ed_c || !d->tr_c || !d->v_p))
-    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: APVM needs ed_c, tr_c and v_p");
+  if ((sc->apvm || rec == IMX_REC_GASS) && (!d->ed_c || !d->tr_c || !d->v_p || !d->v_tr))
+    return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: APVM and gass K need ed_c, tr_c, v_p and v_tr");
+  const bool hc = sc->stag == IMX_STAG_HC;
   if (s->fast) {
     // the fused kernels carry the trsk10 scheme with HTC (with APVM and K2
-    // diffusion) or HC staggering
-    const bool hc = sc->stag == IMX_STAG_HC;
-    if (sc->wrap == IMX_WRAP_GASS18 || (hc && sc->wrap == IMX_WRAP_PXT16))
+    // diffusion) or HC staggering, and pxt16 with HTC
+    int fw = -1;
+    for (int w : {IMX_WRAP_TRSK10, IMX_WRAP_PXT16}) {
+      int a, b, c;
+      imx::wrapper_methods(w, a, b, c);
+      if (a == sint && b == cor && c == rec) fw = w;
+    }
+    if (fw < 0 || sc->nopv || (hc && fw == IMX_WRAP_PXT16))
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take trsk10 (HTC or HC) and pxt16 (HTC) with "
-                            "APVM, diffusion and hyperdiffusion; gass18 and HC pxt16 run on FP64_REF handles");
+                            "APVM, diffusion and hyperdiffusion; gass18, HC pxt16, the level model and other "
+                            "method triples run on FP64_REF handles");
+    imx_scheme r = *sc;
+    r.wrap = fw;
     int rc;
-    if (hc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, sc->wrap)))) return rc;
+    if (hc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, false, false)))) return rc;
     rc = s->single ? imx::fast_stagger<float>(s, *d, hc) : imx::fast_stagger<double>(s, *d, hc);
     if (rc) return rc;
     s->sc_hc = hc;
-    rc = s->single ? imx::fast_diffusion<float>(s, *d, *sc) : imx::fast_diffusion<double>(s, *d, *sc);
+    rc = s->single ? imx::fast_diffusion<float>(s, *d, r) : imx::fast_diffusion<double>(s, *d, r);
     if (rc) return rc;
-    rc = s->single ? imx::fast_apvm<float>(s, *d, *sc) : imx::fast_apvm<double>(s, *d, *sc);
+    rc = s->single ? imx::fast_apvm<float>(s, *d, r) : imx::fast_apvm<double>(s, *d, r);
     if (rc) return rc;
-    rc = s->single ? imx::fast_pxt16<float>(s, *d, *sc) : imx::fast_pxt16<double>(s, *d, *sc);
+    rc = s->single ? imx::fast_pxt16<float>(s, *d, r) : imx::fast_pxt16<double>(s, *d, r);
     if (rc) return rc;
-    s->sc_wrap = sc->wrap;
+    s->sc_wrap = fw;
+    s->sc_sint = sint, s->sc_cor = cor, s->sc_rec = rec, s->sc_gcoef = 1.0, s->sc_nopv = false;
     // the option flags size the Jacobian's stencil superset
     s->sc_apvm = sc->apvm != 0;
     s->sc_diff = sc->k2 != 0.0;
     s->sc_hyper = sc->k4 != 0.0;
-    s->sc = *sc;
+    s->sc = r;
     imx::drop_graph(s);
     return IMX_OK;
   }
   const int nv = s->nv, NE = s->NE, NT = s->NT;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
-  imx::Alloc &A = s->A;
-  if (sc->apvm || sc->k2 > 0.0 || sc->k4 > 0.0 || sc->wrap != IMX_WRAP_TRSK10 || sc->stag != IMX_STAG_HTC) {
+  if (sc->apvm || sc->k2 > 0.0 || sc->k4 > 0.0 || !plain || sc->nopv || hc) {
     const int rc = imx::scheme_tables(s, d);
     if (rc) return rc;
   }
-  if (sc->stag == IMX_STAG_HC) {
-    const int rc = imx::hc_tables(s, d, sc->wrap);
+  const bool tg = cor == IMX_COR_PERED || cor == IMX_COR_DTRED;
+  if (hc) {
+    const int rc = imx::hc_tables(s, d, sint == IMX_SINT_BARY, tg);
     if (rc) return rc;
   }
-  if (sc->wrap == IMX_WRAP_PXT16) {
+  // intersection areas from the mesh, or computed here when it has none
+  std::vector<double> trhx_v, hxtr_v, trskw_v;
+  const double *trhx = d->tr_trhx_area, *hxtr = d->hx_hxtr_area;
+  auto areas = [&]() {
+    if (trhx && hxtr) return;
+    trhx_v.assign((size_t)3 * d->nt, 0.0);
+    hxtr_v.assign((size_t)MX * d->nv, 0.0);
+    trskw_v.assign((size_t)MX * MX * d->nv, 0.0);
+    imx::gridprop(*d, trhx_v.data(), hxtr_v.data(), trskw_v.data());
+    trhx = trhx_v.data();
+    hxtr = hxtr_v.data();
+  };
+  if (sint == IMX_SINT_BARY) {
     // barycentric coordinates of the circumcentres (initialize_gridprop,
-    // swm.f90:1021-1026), edge-to-triangle distances, unit edge normals
-    std::vector<double> tw((size_t)3 * NT, 0.0), dw((size_t)2 * NE, 1.0), nr((size_t)3 * NE, 0.0);
+    // swm.f90:1021-1026)
+    std::vector<double> tw((size_t)3 * NT, 0.0);
     for (int ko = 0; ko < d->nt; ++ko) {
       double b[3];
       imx::bary_circumcentre(*d, ko, b);
       for (int i = 0; i < 3; ++i) tw[(size_t)i * NT + tn[ko]] = b[i];
     }
+    HIPTRY(hipMemcpy(s->t_w, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+  } else if (sint == IMX_SINT_GASS) {
+    // areas of the sub-triangles (circumcentre, edge)
+    std::vector<double> tw((size_t)3 * NT, 0.0);
+    for (int ko = 0; ko < d->nt; ++ko)
+      for (int i = 0; i < 3; ++i) tw[(size_t)i * NT + tn[ko]] = imx::gass_subarea(*d, ko, i);
+    HIPTRY(hipMemcpy(s->t_w, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+  }
+  if (sint == IMX_SINT_BARY || cor == IMX_COR_DTRED || (!hc && tg)) {
+    // edge-to-circumcentre distances (bary q at edges, dtred) and unit edge
+    // normals ed%nr (pered and dtred, HTC)
+    std::vector<double> dw((size_t)2 * NE, 1.0), nr((size_t)3 * NE, 0.0);
     for (int eo = 0; eo < d->ne; ++eo) {
       const int e = en[eo];
       for (int k = 0; k < 2; ++k)
@@ -2855,20 +3036,12 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       const double n = imx::norm3(c);
       for (int q = 0; q < 3; ++q) nr[(size_t)q * NE + e] = c[q] / n;
     }
-    HIPTRY(hipMemcpy(s->t_w, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
     HIPTRY(hipMemcpy(s->e_dw, dw.data(), dw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
-    if (sc->stag == IMX_STAG_HTC)  // HC holds edhx%tg there (set above)
+    if (!hc && tg)  // HC holds edhx%tg there (set above)
       HIPTRY(hipMemcpy(s->e_nr, nr.data(), nr.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
-  } else if (sc->wrap == IMX_WRAP_GASS18) {
-    // areas of the sub-triangles (circumcentre, edge) and the linking edge
-    // of each TRiSK term (gethxedgeconnection)
-    std::vector<double> tw((size_t)3 * NT, 0.0);
-    for (int ko = 0; ko < d->nt; ++ko)
-      for (int i = 0; i < 3; ++i) {
-        const int eo = d->tr_ed[3 * ko + i] - 1;
-        tw[(size_t)i * NT + tn[ko]] = imx::sphtriarea(d->tr_c + 3 * ko, d->v_p + 3 * (d->ed_v[2 * eo] - 1),
-                                                      d->v_p + 3 * (d->ed_v[2 * eo + 1] - 1));
-      }
+  }
+  if (cor == IMX_COR_GASS) {
+    // the linking edge of each TRiSK term (gethxedgeconnection)
     std::vector<int32_t> e3((size_t)2 * MX * NE, -1);
     for (int eo = 0; eo < d->ne; ++eo)
       for (int ic = 0; ic < 2; ++ic) {
@@ -2878,21 +3051,29 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
           e3[((size_t)ic * MX + j) * NE + en[eo]] = c3 >= 0 ? en[c3] : -1;
         }
       }
-    HIPTRY(hipMemcpy(s->t_w, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
     HIPTRY(hipMemcpy(s->e_ed3, e3.data(), e3.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
   }
+  if (rec == IMX_REC_GASS) {
+    // kinetic_energy_tr's areas and each cell's ring triangles with their
+    // hxtr_area (kinetic_energy_hx, swm_operators.f90:1198-1207)
+    areas();
+    std::vector<double> ka((size_t)3 * NT, 0.0), hw((size_t)MX * nv, 0.0);
+    std::vector<int32_t> ct((size_t)MX * nv, 0);
+    for (int ko = 0; ko < d->nt; ++ko)
+      for (int i = 0; i < 3; ++i) ka[(size_t)i * NT + tn[ko]] = imx::gass_subarea(*d, ko, i);
+    for (int co = 0; co < nv; ++co)
+      for (int j = 0; j < d->v_nnb[co]; ++j) {
+        ct[(size_t)j * nv + cn[co]] = tn[d->v_tr[MX * co + j] - 1];
+        hw[(size_t)j * nv + cn[co]] = hxtr[MX * co + j];
+      }
+    HIPTRY(hipMemcpy(s->t_ka, ka.data(), ka.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->c_tr, ct.data(), ct.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->c_hxtr, hw.data(), hw.size() * sizeof(double), hipMemcpyHostToDevice), "set_scheme");
+  }
   if (sc->k2 > 0.0 || sc->k4 > 0.0) {
-    std::vector<double> trhx_v, hxtr_v, trskw_v;
-    const double *trhx = d->tr_trhx_area;
-    if (!trhx) {
-      trhx_v.assign((size_t)3 * d->nt, 0.0);
-      hxtr_v.assign((size_t)MX * d->nv, 0.0);
-      trskw_v.assign((size_t)MX * MX * d->nv, 0.0);
-      imx::gridprop(*d, trhx_v.data(), hxtr_v.data(), trskw_v.data());
-      trhx = trhx_v.data();
-    }
+    areas();
     std::vector<double> hx, tr;
-    imx::diffusion_coef(*d, trhx, sc->wrap, sc->k2, sc->diffus, sc->k4, sc->hyperdiffus, hx, tr);
+    imx::diffusion_coef(*d, trhx, sint == IMX_SINT_BARY, sc->k2, sc->diffus, sc->k4, sc->hyperdiffus, hx, tr);
     std::vector<double> hi(nv, 0.0), ti(NT, 0.0);
     for (int co = 0; co < nv; ++co) hi[cn[co]] = hx[co];
     for (int ko = 0; ko < d->nt; ++ko) ti[tn[ko]] = tr[ko];
@@ -2904,7 +3085,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   s->sc_diff = sc->k2 != 0.0;
   s->sc_hyper = sc->k4 != 0.0;
   s->sc_wrap = sc->wrap;
-  s->sc_hc = sc->stag == IMX_STAG_HC;
+  s->sc_sint = sint, s->sc_cor = cor, s->sc_rec = rec, s->sc_gcoef = gcoef, s->sc_nopv = sc->nopv != 0;
+  s->sc_hc = hc;
   imx::drop_graph(s);  // the captured step holds the previous kernel sequence
   return IMX_OK;
 }

@@ -30,13 +30,22 @@ module imx_c
 
   ! imx_scheme: APVM, diffusion, hyperdiffusion (IMX_COEF_*: const, align, diam)
   integer(c_int32_t), parameter :: IMX_COEF_CONST = 0, IMX_COEF_ALIGN = 1, IMX_COEF_DIAM = 2
-  integer(c_int32_t), parameter :: IMX_WRAP_TRSK10 = 0, IMX_WRAP_PXT16 = 1, IMX_WRAP_GASS18 = 2
+  integer(c_int32_t), parameter :: IMX_WRAP_TRSK10 = 0, IMX_WRAP_PXT16 = 1, IMX_WRAP_GASS18 = 2, &
+                                   IMX_WRAP_NONE = 3
+  ! the individual methods of wrapper none: scalar interpolation, Coriolis,
+  ! kinetic energy
+  integer(c_int32_t), parameter :: IMX_SINT_TRSK = 0, IMX_SINT_BARY = 1, IMX_SINT_GASS = 2
+  integer(c_int32_t), parameter :: IMX_COR_TRSK = 0, IMX_COR_PERED = 1, IMX_COR_GASS = 2, IMX_COR_DTRED = 3
+  integer(c_int32_t), parameter :: IMX_REC_TRSK = 0, IMX_REC_PERHX = 1, IMX_REC_GASS = 2, IMX_REC_MELV = 3
   type, bind(C) :: imx_scheme
     integer(c_int32_t) :: stag, wrap, apvm
     real(c_double) :: pvspar, k2
     integer(c_int32_t) :: diffus
     real(c_double) :: k4
     integer(c_int32_t) :: hyperdiffus
+    integer(c_int32_t) :: sinterp = 0, coriolis = 0, recon = 0
+    real(c_double) :: gasscoef = 1._c_double
+    integer(c_int32_t) :: nopv = 0
   end type imx_scheme
 
   interface

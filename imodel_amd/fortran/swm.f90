@@ -306,7 +306,7 @@ contains
       unavail = unavail + areag(i)*grav*mhb*(mhb*0.5 + bt%f(i))
     end do
     Kenergy = 0._r8
-    if (wrap_code() == IMX_WRAP_PXT16) then
+    if (rec_code() == IMX_REC_PERHX) then
       do i = 1, mdesc%nv
         Kenergy = Kenergy + ke(i)*areag(i)
       end do
@@ -371,7 +371,7 @@ contains
     real(r8), pointer :: areag(:), trarea(:), leng(:), dleng(:)
     real(r8) :: kp, unavail
     integer :: i
-    if (wrap_code() /= IMX_WRAP_PXT16) return
+    if (rec_code() /= IMX_REC_PERHX) return
     call mesh_arrays(areag, trarea, leng, dleng)
     allocate(ke(mdesc%nv))
     ! the reference's first energies follow a tendency at the initial state
@@ -412,14 +412,20 @@ contains
       swmname = trim(swmname)//trim(dtpart)//"_"//trim(stag)//"_"//trim(adjustl(mtdwrapper))// &
                 "_areageo"
     else
-      swmname = trim(swmname)//trim(dtpart)//"_"//trim(stag)// &
-                "_vrectrsk_crectrsk_sinttrsk_grdtrsk_areageo"
+      swmname = trim(swmname)//trim(dtpart)//"_"//trim(stag)//"_vrec"//trim(adjustl(reconmtd))
+      if (rec_code() == IMX_REC_GASS) then
+        write(atmp, '(f4.2)') real(gass_coef())
+        swmname = trim(swmname)//trim(adjustl(trim(atmp)))
+      end if
+      swmname = trim(swmname)//"_crec"//trim(adjustl(coriolis_reconmtd))//"_sint"// &
+                trim(adjustl(sinterpol))//"_grd"//trim(adjustl(gradmtd))//"_areageo"
     end if
     ! scheme-option suffixes of swmpars (src/swm_data.f90:658-683)
     if (trim(pv_stab) /= 'none') then
       write(atmp, '(f5.2)') real(pvspar)
       swmname = trim(swmname)//"_pvs"//trim(adjustl(trim(pv_stab)))//trim(adjustl(trim(atmp)))
     end if
+    if (inopv > 0) swmname = trim(swmname)//"_nopv"
     if (k2_max > 0) then
       write(atmp, '(f10.3)') real(dlog(k2_max)/dlog(10._r8))
       swmname = trim(swmname)//"_"//trim(diffus)//"_diffusion_10to"//trim(adjustl(trim(atmp)))
@@ -451,10 +457,10 @@ contains
     call imx_check(imx_swm_create(mdesc, o, hnd), 'imx_swm_create')
     if (scheme_options()) then
       ! the fused fp32/fp64 kernels carry trsk10 and (with HTC) pxt16, with
-      ! every other option
-      if (prec /= IMX_PREC_FP64_REF .and. (wrap_code() == 2 .or. &
+      ! APVM, diffusion and hyperdiffusion
+      if (prec /= IMX_PREC_FP64_REF .and. (wrap_code() >= 2 .or. inopv > 0 .or. &
           (wrap_code() == 1 .and. trim(stag) == 'HC'))) then
-        write(*, '(a)') " ERROR: gass18 and HC with pxt16 run on the fp64_ref path"
+        write(*, '(a)') " ERROR: gass18, HC with pxt16, wrapper none and noPV run on the fp64_ref path"
         write(*, '(a)') "        (par/swm_gpu.par)"
         stop 1
       end if
@@ -466,6 +472,11 @@ contains
       sc%diffus = max(coef_code(diffus), 0)
       sc%k4 = max(k4_max, 0._r8)
       sc%hyperdiffus = max(coef_code(hyperdiffus), 0)
+      sc%sinterp = sint_code()
+      sc%coriolis = cor_code()
+      sc%recon = rec_code()
+      sc%gasscoef = gass_coef()
+      sc%nopv = merge(1, 0, inopv > 0)
       call imx_check(imx_swm_set_scheme(hnd, mdesc, sc), 'imx_swm_set_scheme')
     end if
   end subroutine open_handle

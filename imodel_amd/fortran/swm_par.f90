@@ -6,8 +6,10 @@
 !                 geo-area subset this build runs: test cases 0, 1, 2, 5,
 !                 6, 21, 22, 23, 51, 52, 53 and the seeded state 100;
 !                 trsk10 with HTC or HC and pxt16 with HTC, APVM, diffusion and
-!                 hyperdiffusion on every path; gass18 and HC pxt16 on the
-!                 fp64_ref path
+!                 hyperdiffusion on every path; gass18, HC pxt16, wrapper
+!                 none (sinterpol trsk/bary/gass, Coriolis trsk/pered/gass/
+!                 dtred, reconmtd trsk/perhx/gass/dubos/melv, gradmtd trsk)
+!                 and the level model (noPV) on the fp64_ref path
 !   par/swm_gpu.par (new, optional): precision, seed, SFC and graph switches
 !=======================================================================
 module swm_par
@@ -144,8 +146,19 @@ contains
 
     ! the scheme this build runs (the reference defaults, par/swm.par)
     if (trim(stag) /= 'HTC' .and. trim(stag) /= 'HC') call unsupported('staggering', stag)
-    if (wrap_code() < 0) call unsupported('method wrapper', mtdwrapper)
-    if (wrap_code() == 2 .and. (k2_max > 0._r8 .or. k4_max > 0._r8)) then
+    if (wrap_code() < 0) then
+      if (sint_code() < 0) call unsupported('scalar interpolation', sinterpol)
+      if (cor_code() < 0) call unsupported('Coriolis reconstruction', coriolis_reconmtd)
+      if (rec_code() < 0) call unsupported('vector reconstruction', reconmtd)
+      call unsupported('gradient method', gradmtd)
+    end if
+    if (inopv > 0 .and. (cor_code() == 1 .or. cor_code() == 3)) then
+      ! as swmpars (src/swm_data.f90:490-496)
+      print*, "Cannot use this Coriolis vector reconstruction method with level model (noPV)", &
+        coriolis_reconmtd, .true.
+      stop 1
+    end if
+    if (sint_code() == 2 .and. (k2_max > 0._r8 .or. k4_max > 0._r8)) then
       ! as the reference's scalar_hx2tr (src/swm_operators.f90:271-273)
       print*, "Error at scalar_hx2trcc: Gassmann's scheme needs edge values"
       stop 1
@@ -158,7 +171,6 @@ contains
       if (coef_code(hyperdiffus) < 0) call unsupported('hyperdiffusion coefficient function', hyperdiffus)
     end if
     if (trim(pv_stab) /= 'none' .and. trim(pv_stab) /= 'apvm') call unsupported('PV stabilisation', pv_stab)
-    if (inopv /= 0) call unsupported('level model (noPV)', 'noPV=1')
     if (test_lterror /= 0) call unsupported('local truncation error test', '1')
     select case (testcase)
     case (0, 1, 2, 5, 6, 21, 22, 23, 51, 52, 53, 100)
@@ -224,7 +236,8 @@ contains
   end function coef_code
 
   ! method wrapper -> IMX_WRAP_* (-1: not part of this build). The
-  ! explicit form "none" with trsk in the four method lines is trsk10.
+  ! explicit form "none" with trsk in the four method lines is trsk10; any
+  ! other known triple with the trsk gradient is IMX_WRAP_NONE.
   integer(c_int32_t) function wrap_code() result(c)
     select case (trim(mtdwrapper))
     case ('trsk10'); c = 0
@@ -232,16 +245,73 @@ contains
     case ('gass18'); c = 2
     case ('none')
       c = -1
-      if (trim(reconmtd) == 'trsk' .and. trim(coriolis_reconmtd) == 'trsk' .and. &
-          trim(sinterpol) == 'trsk' .and. trim(gradmtd) == 'trsk') c = 0
+      if (sint_code() >= 0 .and. cor_code() >= 0 .and. rec_code() >= 0 .and. trim(gradmtd) == 'trsk') c = 3
+      if (c == 3 .and. sint_code() == 0 .and. cor_code() == 0 .and. rec_code() == 0) c = 0
     case default; c = -1
     end select
   end function wrap_code
 
-  ! a wrapper other than trsk10, APVM, diffusion or hyperdiffusion selected
+  ! the scalar interpolation, Coriolis and kinetic-energy methods the
+  ! wrapper stands for, or the named ones with wrapper none
+  ! (src/swm_data.f90:445-522) -> IMX_SINT_*, IMX_COR_*, IMX_REC_* (-1:
+  ! not part of this build: hyb and exact Coriolis)
+  integer(c_int32_t) function sint_code() result(c)
+    select case (trim(mtdwrapper))
+    case ('trsk10'); c = 0
+    case ('pxt16'); c = 1
+    case ('gass18'); c = 2
+    case default
+      select case (trim(sinterpol))
+      case ('trsk'); c = 0
+      case ('bary'); c = 1
+      case ('gass'); c = 2
+      case default; c = -1
+      end select
+    end select
+  end function sint_code
+
+  integer(c_int32_t) function cor_code() result(c)
+    select case (trim(mtdwrapper))
+    case ('trsk10'); c = 0
+    case ('pxt16'); c = 1
+    case ('gass18'); c = 2
+    case default
+      select case (trim(coriolis_reconmtd))
+      case ('trsk'); c = 0
+      case ('pered'); c = 1
+      case ('gass'); c = 2
+      case ('dtred'); c = 3
+      case default; c = -1
+      end select
+    end select
+  end function cor_code
+
+  integer(c_int32_t) function rec_code() result(c)
+    select case (trim(mtdwrapper))
+    case ('trsk10', 'gass18'); c = 0
+    case ('pxt16'); c = 1
+    case default
+      select case (trim(reconmtd))
+      case ('trsk'); c = 0
+      case ('perhx'); c = 1
+      case ('gass', 'dubos'); c = 2
+      case ('melv'); c = 3
+      case default; c = -1
+      end select
+    end select
+  end function rec_code
+
+  ! gasscoef of the gass kinetic energy; dubos is gass with 0
+  real(r8) function gass_coef() result(g)
+    g = gasscoef
+    if (trim(mtdwrapper) == 'none' .and. trim(reconmtd) == 'dubos') g = 0._r8
+  end function gass_coef
+
+  ! a wrapper other than trsk10, APVM, diffusion, hyperdiffusion, HC or the
+  ! level model selected
   logical function scheme_options() result(on)
     on = wrap_code() /= 0 .or. trim(pv_stab) == 'apvm' .or. k2_max > 0._r8 .or. k4_max > 0._r8 .or. &
-         trim(stag) == 'HC' 
+         trim(stag) == 'HC' .or. inopv > 0
   end function scheme_options
 
   subroutine unsupported(what, val)

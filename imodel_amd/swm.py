@@ -237,16 +237,24 @@ class Swm:
         return h, u
 
     def set_scheme(self, stag="HTC", wrap="trsk10", pv="none", pvspar=0.5, k2=0.0, diffus="const", k4=0.0,
-                   hyperdiffus="const"):
+                   hyperdiffus="const", sint="trsk", cor="trsk", rec="trsk", gasscoef=1.0, nopv=False):
         """Scheme options of par/swm.par: the staggering ("HTC" | "HC"),
-        the method wrapper ("trsk10" | "pxt16" | "gass18"), APVM (pv="apvm",
-        parameter pvspar), diffusion K2_max with coefficient function diffus,
-        hyperdiffusion K4_max with hyperdiffus ("const" | "align" | "diam").
+        the method wrapper ("trsk10" | "pxt16" | "gass18" | "none"), APVM
+        (pv="apvm", parameter pvspar), diffusion K2_max with coefficient
+        function diffus, hyperdiffusion K4_max with hyperdiffus ("const" |
+        "align" | "diam"). With wrap="none" the individual methods: scalar
+        interpolation sint ("trsk" | "bary" | "gass"), Coriolis cor ("trsk" |
+        "pered" | "gass" | "dtred"), kinetic energy rec ("trsk" | "perhx" |
+        "gass" with gasscoef | "dubos" | "melv") and the level model nopv.
         Every option runs on FP64_REF handles; fused (FP32/FP64) handles take
         trsk10 (HTC or HC) and pxt16 (HTC) with APVM, diffusion and
-        hyperdiffusion, and refuse gass18 and HC pxt16."""
+        hyperdiffusion, and refuse the others."""
+        if rec == "dubos":  # Gassmann's K with no trsk part (swmpars, src/swm_data.f90:513-516)
+            rec, gasscoef = "gass", 0.0
         sc = capi.Scheme(stag=capi.STAG[stag], wrap=capi.WRAP[wrap], apvm=int(pv == "apvm"), pvspar=float(pvspar), k2=float(k2),
-                         diffus=capi.COEF[diffus], k4=float(k4), hyperdiffus=capi.COEF[hyperdiffus])
+                         diffus=capi.COEF[diffus], k4=float(k4), hyperdiffus=capi.COEF[hyperdiffus],
+                         sinterp=capi.SINT[sint], coriolis=capi.COR[cor], recon=capi.REC[rec],
+                         gasscoef=float(gasscoef), nopv=int(bool(nopv)))
         capi.check(capi.lib().imx_swm_set_scheme(self._h, ctypes.byref(self.mesh.desc()), ctypes.byref(sc)),
                    "imx_swm_set_scheme")
 

@@ -171,6 +171,37 @@ typedef struct imx_swm  imx_swm;   /* one SWM instance; owns device memory */
 #define IMX_WRAP_TRSK10 0
 #define IMX_WRAP_PXT16  1
 #define IMX_WRAP_GASS18 2
+#define IMX_WRAP_NONE   3  /* the individual method choices below            */
+/* With wrap IMX_WRAP_NONE (par/swm.par wrapper line "none",
+   src/swm_data.f90:438-551), FP64_REF handles:
+     sinterp   scalar interpolation: trsk (kite areas, simple averages),
+               bary (pxt16's barycentric h and distance-weighted q), gass
+               (gass18's edge-based h and rhombus PV)
+     coriolis  trsk, pered (pxt16's), gass (gass18's linking-edge PV) or
+               dtred (dual-triangle reconstruction: Perot vector of uh q at
+               the two triangles, swm_operators.f90:1310-1329)
+     recon     kinetic energy: trsk, perhx (pxt16's), gass (trsk K blended
+               with the triangle K of kinetic_energy_tr by gasscoef; dubos is
+               gass with gasscoef 0) or melv (uh u / h, :1174-1198)
+     nopv      level model (inoPV = 1): the TRiSK and Gassmann Coriolis
+               terms take the vorticity at edges and u instead of q and uh
+   The wrappers are the triples (trsk, trsk, trsk), (bary, pered, perhx)
+   and (gass, gass, trsk). As in the reference: nopv is refused with pered
+   and dtred, gass interpolation with diffusion. The reference's hyb
+   Coriolis (which rewrites shared method flags inside its parallel edge
+   loop), its exact Coriolis (test-case closed forms) and the CLUST PV
+   correction (which stops in the reference) are not offered. */
+#define IMX_SINT_TRSK 0
+#define IMX_SINT_BARY 1
+#define IMX_SINT_GASS 2
+#define IMX_COR_TRSK  0
+#define IMX_COR_PERED 1
+#define IMX_COR_GASS  2
+#define IMX_COR_DTRED 3
+#define IMX_REC_TRSK  0
+#define IMX_REC_PERHX 1
+#define IMX_REC_GASS  2
+#define IMX_REC_MELV  3
 typedef struct imx_scheme {
   int32_t stag;         /* IMX_STAG_*                                         */
   int32_t wrap;         /* IMX_WRAP_*                                         */
@@ -180,6 +211,11 @@ typedef struct imx_scheme {
   int32_t diffus;       /* IMX_COEF_*                                         */
   double  k4;           /* K4_max, 0 = no hyperdiffusion                      */
   int32_t hyperdiffus;  /* IMX_COEF_*                                         */
+  int32_t sinterp;      /* IMX_SINT_*, with IMX_WRAP_NONE                      */
+  int32_t coriolis;     /* IMX_COR_*,  with IMX_WRAP_NONE                      */
+  int32_t recon;        /* IMX_REC_*,  with IMX_WRAP_NONE                      */
+  double  gasscoef;     /* recon IMX_REC_GASS: weight of the trsk K            */
+  int32_t nopv;         /* 0/1                                                */
 } imx_scheme;
 
 /* ---- errors -------------------------------------------------------------- */
@@ -236,6 +272,16 @@ int  imx_ic_tc2_hc(const imx_mesh *m, double erad, double *h, double *u);
 /* TC2's exact potential vorticity q_tr_exact at the triangle circumcentres
    (src/swm.f90:1293-1296), for the error columns of write_evol_error. */
 int  imx_ic_tc2_pv(const imx_mesh *m, double erad, double *q_tr);
+/* initialize_fields (src/swm.f90:1044-1700) for test case tc, staggering
+   stag (IMX_STAG_*): h [nv], u [ne] and the bottom topography bt [nv]
+   (zero where the case has none), bit-identical to the reference on
+   library-built meshes. Cases 0 (cos lon cos^2 lat at rest), 1, 2, 5
+   (conical mountain), 51 (Gaussian mountain, fluid depth hollgw minus the
+   bump: the Hollingsworth line of par/swm.par), 6 (Rossby-Haurwitz), 21,
+   22, 23, 52, 53 (Galewsky jets with and without the perturbation).
+   IMX_ENOTSUP for other cases. */
+int  imx_ic_testcase(const imx_mesh *m, int32_t tc, int32_t stag, double erad, double hollgw,
+                     double *h, double *u, double *bt);
 /* Seeded spectral state: real spherical harmonics l=1..lmax, coefficients
    N(0,1)*l^-2 from SplitMix64 + Box-Muller; h = h0 + h', h' area-rms =
    hfrac*h0; u = -(psi(t2)-psi(t1))/(l_e a) * sign, scaled to area-rms
@@ -250,8 +296,9 @@ int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out
 /* Scheme options (imx_scheme); d is the mesh the handle was created on
    (the coefficients and the APVM reconstruction read its geometry).
    FP64_REF handles take every option. Fused (FP32/FP64) handles take
-   trsk10 (HTC or HC) and pxt16 (HTC) with APVM, diffusion and
-   hyperdiffusion, and return IMX_ENOTSUP for gass18 and HC pxt16. */
+   trsk10 (HTC or HC) and pxt16 (HTC), also spelled as IMX_WRAP_NONE
+   triples, with APVM, diffusion and hyperdiffusion, and return IMX_ENOTSUP
+   for gass18, HC pxt16, other method triples and nopv. */
 int  imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc);
 int  imx_swm_set_state(imx_swm *s, const double *h, const double *u);
 int  imx_swm_get_state(imx_swm *s, double *h, double *u);

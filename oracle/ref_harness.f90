@@ -366,7 +366,9 @@ program ref_harness
   character(len=16) :: sc_stag = 'HTC', sc_wrap = 'trsk10', sc_diffus = 'const', sc_hdiffus = 'const'
   character(len=16) :: sc_pv = 'none'
   real(r8) :: sc_k2 = 0._r8, sc_k4 = 0._r8, sc_pvspar = 0.5_r8
-  integer :: sc_tc = 2
+  integer :: sc_tc = 2, sc_nopv = 0
+  character(len=16) :: sc_rec = 'trsk', sc_cor = 'trsk', sc_sint = 'trsk'
+  real(r8) :: sc_gcoef = 1.0_r8
 
   call get_command_argument(1, cmd)
   call execute_command_line('mkdir -p par grid data')
@@ -651,6 +653,11 @@ contains
       case ('pv'); sc_pv = val
       case ('pvspar'); read(val, *) sc_pvspar
       case ('tc'); read(val, *) sc_tc
+      case ('rec'); sc_rec = val
+      case ('cor'); sc_cor = val
+      case ('sint'); sc_sint = val
+      case ('gasscoef'); read(val, *) sc_gcoef
+      case ('nopv'); read(val, *) sc_nopv
       case default
         print*, 'unknown scheme key ', trim(key)
         stop 1
@@ -704,11 +711,11 @@ contains
     write(pu, '(a)') '!c'
     write(pu, '(a)') trim(sc_wrap)
     write(pu, '(a)') '!c'
-    write(pu, '(a)') 'trsk 1.0'
+    write(pu, '(a,1x,es25.17)') trim(sc_rec), sc_gcoef
     write(pu, '(a)') '!c'
-    write(pu, '(a)') 'trsk'
+    write(pu, '(a)') trim(sc_cor)
     write(pu, '(a)') '!c'
-    write(pu, '(a)') 'trsk'
+    write(pu, '(a)') trim(sc_sint)
     write(pu, '(a)') '!c'
     write(pu, '(a)') 'trsk'
     write(pu, '(a)') '!c'
@@ -726,7 +733,7 @@ contains
     write(pu, '(a)') '!c'
     write(pu, '(a,1x,es25.17)') trim(sc_pv), sc_pvspar
     write(pu, '(a)') '!c'
-    write(pu, '(a)') '0'
+    write(pu, '(i0)') sc_nopv
     close(pu)
   end subroutine write_swm_par
 end program ref_harness

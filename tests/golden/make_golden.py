@@ -20,6 +20,10 @@ outputs as fixtures:
                             topography bt): tendency intermediates, diag,
                             3 RK4 steps dt=900 with diagnostics
                             (``python make_golden.py tc`` makes only these)
+  ref_run_g3_m<name>.bin    seed 7, 3 steps dt=900 with the individual method
+                            choices of METHOD_SCHEMES (wrapper "none")
+                            (``python make_golden.py opts`` makes the test
+                            cases and these)
   ref_lte_g3_<scheme>.bin   swm_horiz_loc_trunc_er on TC2 with the options of
                             LTE_SCHEMES: lte_<row> = error_calc triple of each
                             row of its table, mim = the mimetic sums it prints
@@ -59,6 +63,23 @@ SCHEMES = {
     "hcpx": "stag=HC,wrap=pxt16,k2=1e5,diffus=align",
     "hcmix": "stag=HC,wrap=gass18,pv=apvm,pvspar=0.5",
     "hck4": "stag=HC,pv=apvm,pvspar=0.3,k4=1e15,hyperdiffus=diam",
+}
+
+
+# individual method choices of par/swm.par with the wrapper line "none"
+# (src/swm_data.f90:438-551): kinetic energy (reconmtd), Coriolis
+# reconstruction, scalar interpolation, level model (noPV); seed 7, 3 steps
+METHOD_SCHEMES = {
+    "rgass": "wrap=none,rec=gass,gasscoef=0.75",
+    "rdubos": "wrap=none,rec=dubos",
+    "rmelv": "wrap=none,rec=melv",
+    "rmelvhc": "stag=HC,wrap=none,rec=melv",
+    "cdtred": "wrap=none,cor=dtred",
+    "cdtredhc": "stag=HC,wrap=none,cor=dtred",
+    "nopv": "wrap=none,nopv=1",
+    "gassnopv": "wrap=none,sint=gass,cor=gass,nopv=1",
+    "baryperhx": "wrap=none,sint=bary,rec=perhx,cor=trsk",
+    "mixnone": "wrap=none,sint=gass,cor=dtred,rec=gass,gasscoef=0.5,pv=apvm,pvspar=0.4",
 }
 
 
@@ -134,6 +155,15 @@ def main():
             run(["run", "build:3:eqs", "tc", "3", "900", "1", "out.bin", spec], td)
             shutil.copy(os.path.join(td, "out.bin"), os.path.join(HERE, f"ref_run_g3_{name}.bin"))
     if sys.argv[1:] == ["tc"]:
+        return
+    with tempfile.TemporaryDirectory() as td:
+        mesh = Mesh(imxbin.read(os.path.join(HERE, "ref_mesh_g3.bin")))
+        h, u = mesh.spectral(7)
+        imxbin.write(os.path.join(td, "state.bin"), {"h": h, "u": u})
+        for name, spec in METHOD_SCHEMES.items():
+            run(["run", "build:3:eqs", "state.bin", "3", "900", "0", "out.bin", spec], td)
+            shutil.copy(os.path.join(td, "out.bin"), os.path.join(HERE, f"ref_run_g3_m{name}.bin"))
+    if sys.argv[1:] == ["opts"]:
         return
     with tempfile.TemporaryDirectory() as td:
         for name, spec in LTE_SCHEMES.items():

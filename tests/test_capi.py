@@ -27,6 +27,23 @@ def test_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_every_export_is_declared():
+    """The other direction: no imx_ entry point is exported without a
+    declaration in include/."""
+    import shutil
+    import subprocess
+    nm = shutil.which("nm")
+    if nm is None:
+        import pytest
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-D", "--defined-only", capi.LIB_PATH], check=True, stdout=subprocess.PIPE,
+                         text=True).stdout
+    exported = sorted({ln.split()[-1] for ln in out.splitlines() if ln.split() and
+                       ln.split()[-1].startswith("imx_")})
+    undeclared = sorted(set(exported) - set(declared()))
+    assert not undeclared, undeclared
+
+
 def test_error_paths_without_gpu():
     L = capi.lib()
     h = ctypes.c_void_p()

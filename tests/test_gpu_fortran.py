@@ -284,6 +284,58 @@ def test_fortran_driver_wrappers_bitexact(wrap):
     assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
 
 
+# wrapper "none" in par/swm.par (stag, reconmtd + gasscoef, Coriolis,
+# sinterpol, PV stabilisation, level model lines) against the reference's
+# own 3-step runs from the seeded state of seed 7 (ref_run_g3_m<name>.bin)
+DRIVER_METHODS = {
+    "mixnone": ({10: "none", 12: "gass 0.5", 14: "dtred", 16: "gass", 32: "apvm 0.4"},
+                "HTC_vrecgass0.50_crecdtred_sintgass_grdtrsk_areageo_pvsapvm0.40"),
+    "rmelvhc": ({8: "HC", 10: "none", 12: "melv 1.0"}, "HC_vrecmelv_crectrsk_sinttrsk_grdtrsk_areageo"),
+    "gassnopv": ({10: "none", 14: "gass", 16: "gass", 34: "1"},
+                 "HTC_vrectrsk_crecgass_sintgass_grdtrsk_areageo_nopv"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DRIVER_METHODS))
+def test_fortran_driver_methods_bitexact(golden, name):
+    """The individual method lines read as swmpars reads them: the output
+    names carry the reference's method suffixes and 3 steps equal the
+    reference's bit for bit."""
+    print(f"[gpu-case] fortran imodel g3 seeded fp64_ref methods {name}", flush=True)
+    lines, tag = DRIVER_METHODS[name]
+    r = golden(f"ref_run_g3_m{name}.bin")
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 100, 900, 0.03125, 3, "fp64_ref")
+        p = os.path.join(td, "par", "swm.par")
+        swm = open(p).read().splitlines()
+        for k, v in lines.items():
+            swm[k] = v
+        open(p, "w").write("\n".join(swm) + "\n")
+        p = os.path.join(td, "par", "swm_gpu.par")
+        gpu = open(p).read().splitlines()
+        gpu[3] = "7"
+        open(p, "w").write("\n".join(gpu) + "\n")
+        res = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                             text=True, timeout=120)
+        assert res.returncode == 0, res.stdout[-2000:]
+        st = imxbin.read(os.path.join(td, "data", f"swm_tc100_dt900_{tag}_state_icos_eqs_nopt_3.bin"))
+    assert np.array_equal(st["h"], r["h"]) and np.array_equal(st["u"], r["u"])
+
+
+def test_fortran_driver_nopv_pered_refused():
+    """noPV with the pered Coriolis term stops as swmpars does."""
+    print("[gpu-case] fortran imodel nopv + pered refused", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, 2, 900, 1, 48, "fp64_ref")
+        p = os.path.join(td, "par", "swm.par")
+        swm = open(p).read().splitlines()
+        swm[10], swm[14], swm[34] = "none", "pered", "1"
+        open(p, "w").write("\n".join(swm) + "\n")
+        res = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                             text=True, timeout=120)
+        assert res.returncode != 0 and "level model (noPV)" in res.stdout
+
+
 def test_fortran_driver_hc_tc2_bitexact():
     """HC staggering from par/swm.par: TC2 initialised on the Voronoi edge
     midpoints, a day on glevel 3 equal to the oracle with HC bit for bit."""

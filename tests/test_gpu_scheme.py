@@ -42,6 +42,67 @@ def test_scheme_tendency_and_steps_match_reference(golden, name, sfc):
     assert np.array_equal(h, r["h"]) and np.array_equal(u, r["u"])
 
 
+# wrapper "none": the individual method choices of the reference's
+# ref_run_g3_m<name>.bin fixtures (tests/golden/make_golden.py)
+METHODS = {
+    "rgass": {"wrap": "none", "rec": "gass", "gasscoef": 0.75},
+    "rdubos": {"wrap": "none", "rec": "dubos"},
+    "rmelv": {"wrap": "none", "rec": "melv"},
+    "rmelvhc": {"stag": "HC", "wrap": "none", "rec": "melv"},
+    "cdtred": {"wrap": "none", "cor": "dtred"},
+    "cdtredhc": {"stag": "HC", "wrap": "none", "cor": "dtred"},
+    "nopv": {"wrap": "none", "nopv": True},
+    "gassnopv": {"wrap": "none", "sint": "gass", "cor": "gass", "nopv": True},
+    "baryperhx": {"wrap": "none", "sint": "bary", "rec": "perhx"},
+    "mixnone": {"wrap": "none", "sint": "gass", "cor": "dtred", "rec": "gass", "gasscoef": 0.5, "pv": "apvm",
+                "pvspar": 0.4},
+}
+METHOD_FIELDS = {"h_ed": "h_ed", "uh": "uh", "eta": "eta", "h_tr": "h_tr", "q_tr": "q_tr", "q_ed": "q_ed",
+                 "divuh": "divuh", "ke_hx": "ke_hx", "ghbk": "ghbK", "grad_ghbk": "grad_ghbK",
+                 "uhq_perp": "uhq_perp", "divu": "divu", "zeta": "zeta", "q_grad_ed": "q_grad_ed"}
+
+
+@pytest.mark.parametrize("sfc", [False, True])
+@pytest.mark.parametrize("name", sorted(METHODS))
+def test_methods_match_reference(golden, name, sfc):
+    """Each method choice: the tendency, its intermediates and 3 RK4
+    steps, bit for bit."""
+    say("methods vs reference", name, "sfc", sfc)
+    mesh = Mesh(golden("ref_mesh_g3.bin"))
+    r = golden(f"ref_run_g3_m{name}.bin")
+    with Swm(mesh, precision="fp64_ref", dt=900.0, use_sfc=sfc) as s:
+        s.set_scheme(**METHODS[name])
+        m, f = s.tendency(r["h0"], r["u0"])
+        bad = [k for k, g in METHOD_FIELDS.items() if g in r and not np.array_equal(s.field(k), r[g])]
+        assert not bad, bad
+        assert np.array_equal(m, r["masseq"]) and np.array_equal(f, r["momeq"])
+        s.set_state(r["h0"], r["u0"])
+        s.step(3)
+        h, u = s.get_state()
+    assert np.array_equal(h, r["h"]) and np.array_equal(u, r["u"])
+
+
+def test_method_refusals():
+    """nopv with pered or dtred, gass interpolation with diffusion, bad
+    method values; fused handles refuse wrapper-none triples other than
+    trsk10's and pxt16's."""
+    m = Mesh.icos(3)
+    with Swm(m, precision="fp64_ref", dt=900.0) as s:
+        for kw in ({"wrap": "none", "cor": "pered", "nopv": True}, {"wrap": "none", "cor": "dtred", "nopv": True},
+                   {"wrap": "pxt16", "nopv": True}, {"wrap": "none", "sint": "gass", "k2": 1e5}):
+            with pytest.raises(capi.ImxError):
+                s.set_scheme(**kw)
+        s.set_scheme(wrap="none", cor="gass", nopv=True)
+    with Swm(m, precision="fp32", dt=900.0) as s:
+        for kw in ({"wrap": "none", "rec": "melv"}, {"wrap": "none", "nopv": True},
+                   {"wrap": "none", "cor": "dtred"}):
+            with pytest.raises(capi.ImxError):
+                s.set_scheme(**kw)
+        # the trsk10 and pxt16 triples spelled out run on the fused kernels
+        s.set_scheme(wrap="none", sint="bary", cor="pered", rec="perhx")
+        s.set_scheme(wrap="none")
+
+
 # glevel-5 cases: with K4 > 0 the Laplacian term also runs with sqrt(K4)
 # (the reference's set-up), so K4 is kept small enough for dt = 450 s
 G5 = {"mix": {"k2": 1e5, "diffus": "diam", "k4": 1e14, "hyperdiffus": "align", "pv": "apvm", "pvspar": 0.5},
@@ -93,7 +154,7 @@ def test_scheme_errors():
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
-        with pytest.raises(capi.ImxError, match="gass18 with diffusion"):
+        with pytest.raises(capi.ImxError, match="gass interpolation with diffusion"):
             s.set_scheme(wrap="gass18", k4=1e15)
         s.set_scheme(k2=1e5)
         h, u = m.spectral(3)
