@@ -243,6 +243,7 @@ def main():
             "h_linf_rel": float(np.max(np.abs(hK - hR)) / np.max(np.abs(hR))),
             "hprime_l2_rel": float(np.sqrt(np.sum(A * (hK - hR) ** 2) / np.sum(A * (hR - hR.mean()) ** 2))),
             "u_l2_rel": float(np.sqrt(np.sum(L * (uK - uR) ** 2) / np.sum(L * uR ** 2))),
+            "u_linf_rel": float(np.max(np.abs(uK - uR)) / np.max(np.abs(uR))),
             "mass_drift": float((np.sum(A * hK) - m0) / m0),
             "mass_drift_fp64": float((np.sum(A * hR) - m0) / m0),
             "simulated_s": args.steps * args.dt,

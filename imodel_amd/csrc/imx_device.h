@@ -23,7 +23,12 @@ struct Alloc {
       err = e;
       return nullptr;
     }
-    e = hipMemset(p, 0, n * sizeof(X));  // dummy slots read as zeros
+    // dummy slots read as zeros. The fill runs on the null stream, which the
+    // handles' non-blocking streams do not wait for: it must be complete
+    // before a copy or kernel on such a stream writes the buffer, or it can
+    // land afterwards and wipe what they wrote.
+    e = hipMemsetAsync(p, 0, n * sizeof(X), nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) err = e;
     ptrs.push_back(p);
     bytes += n * sizeof(X);

@@ -474,9 +474,11 @@ int imx_mlp_create(uint64_t seed, imx_mlp **out) {
     rc = imx::set_error(IMX_EDEVICE, "imx_mlp_create: device allocation failed");
   } else {
     (void)hipMemcpy(m->P, P.data(), P_N * 4, hipMemcpyHostToDevice);
-    (void)hipMemset(m->M, 0, P_N * 4);
-    (void)hipMemset(m->V, 0, P_N * 4);
-    (void)hipMemset(m->G, 0, P_N * 4);
+    // the fills run on the null stream, which m->stream does not wait for
+    (void)hipMemsetAsync(m->M, 0, P_N * 4, nullptr);
+    (void)hipMemsetAsync(m->V, 0, P_N * 4, nullptr);
+    (void)hipMemsetAsync(m->G, 0, P_N * 4, nullptr);
+    (void)hipStreamSynchronize(nullptr);
     rc = imx::mlp_pack_launch(m);
     if (!rc && hipStreamSynchronize(m->stream) != hipSuccess)
       rc = imx::set_error(IMX_EDEVICE, "imx_mlp_create: pack failed");
@@ -514,8 +516,9 @@ int imx_mlp_init(imx_mlp *m, uint64_t seed) {
   imx::mlp_initial_params(seed, P);
   if (hipStreamSynchronize(m->stream) != hipSuccess ||
       hipMemcpy(m->P, P.data(), P_N * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(m->M, 0, P_N * 4) != hipSuccess || hipMemset(m->V, 0, P_N * 4) != hipSuccess ||
-      hipMemset(m->G, 0, P_N * 4) != hipSuccess)
+      hipMemsetAsync(m->M, 0, P_N * 4, nullptr) != hipSuccess ||
+      hipMemsetAsync(m->V, 0, P_N * 4, nullptr) != hipSuccess ||
+      hipMemsetAsync(m->G, 0, P_N * 4, nullptr) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess)
     return imx::set_error(IMX_EDEVICE, "imx_mlp_init: device copy failed");
   m->t = 0;
   int rc = imx::mlp_pack_launch(m);

@@ -284,6 +284,25 @@ def test_fortran_driver_wrappers_bitexact(wrap):
     assert np.array_equal(st["h"], h) and np.array_equal(st["u"], u)
 
 
+@pytest.mark.skipif(not ref_harness_available(), reason="reference harness not built (oracle/Makefile.ref)")
+def test_fortran_driver_g5_tc5_day_vs_reference_end_to_end():
+    """Both sides end to end from the same parameters: the reference builds
+    its own glevel-5 mesh (meshbuild + trmesh), initialises test case 5 with
+    its mountain and runs 96 steps of 900 s; the Fortran driver does the same
+    from par/ files on the GPU (FP64_REF). The final states are identical."""
+    print("[gpu-case] fortran imodel g5 tc5 day vs reference meshbuild end to end", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        subprocess.run([REF_HARNESS, "run", "build:5:eqs", "tc", "96", "900", "0", "ref.bin", "tc=5"], cwd=td,
+                       check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
+        r = imxbin.read(os.path.join(td, "ref.bin"))
+        _par(td, 10242, 5, 900, 1, 96, "fp64_ref")
+        res = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                             timeout=300)
+        assert res.returncode == 0, res.stdout[-2000:]
+        st = imxbin.read(os.path.join(td, "data", "swm_tc5_dt900_HTC_trsk10_areageo_state_icos_eqs_nopt_5.bin"))
+    assert np.array_equal(st["h"], r["h"]) and np.array_equal(st["u"], r["u"])
+
+
 # wrapper "none" in par/swm.par (stag, reconmtd + gasscoef, Coriolis,
 # sinterpol, PV stabilisation, level model lines) against the reference's
 # own 3-step runs from the seeded state of seed 7 (ref_run_g3_m<name>.bin)
