@@ -151,7 +151,11 @@ def main():
     ap.add_argument("--train-seeds", type=int, default=24)
     ap.add_argument("--train-samples", type=int, default=48)
     ap.add_argument("--fit-steps", type=int, default=6000)
-    ap.add_argument("--eval-steps", type=int, default=192)
+    ap.add_argument("--eval-steps", type=int, default=0, help="learned: coarse steps evaluated (0: one day)")
+    ap.add_argument("--coarse-glevel", type=int, default=6,
+                    help="learned: the corrected glevel gc (6: config 4; 8: config 5's leg), trained on "
+                         "glevel gc+2 truth, compared with plain gc+1 at half the step; dt = 450 s at glevel 6, "
+                         "halved per level")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -497,18 +501,22 @@ def bench_mlp(args):
 
 
 def bench_learned(args):
-    """Learned correction (glevel 6, dt 450 s) against the plain glevel-6 and
-    glevel-7 (dt 225 s) models, all started from the FP64 glevel-8 run
-    restricted to their grids and compared on glevel 6 after --eval-steps
-    coarse steps (192 = 1 day). Training: --train-seeds runs x
+    """Learned correction at glevel gc = --coarse-glevel (dt 450 s at glevel
+    6, halved per finer level) against the plain glevel-gc and glevel-gc+1
+    (half the step) models, all started from the FP64 glevel-gc+2 run
+    restricted to their grids and compared on glevel gc after --eval-steps
+    coarse steps (default one day). Training: --train-seeds runs x
     --train-samples one-step samples produced by this solver, stored on the
     device, --fit-steps shuffled AdamW steps. Reports errors, wall times per
     simulated interval, limited steps and the MLP kernel's MFMA throughput."""
     from imodel_amd.learn import Trainer, evaluate, restriction
     from imodel_amd.swm import H0_GEO, Mesh, Swm
 
+    gc = args.coarse_glevel
+    dt_c = 450.0 / 2 ** (gc - 6)
+    nsteps = args.eval_steps or int(round(86400.0 / dt_c))
     t0 = time.perf_counter()
-    tr = Trainer(gc=6, gf=8, dt_c=450.0, log=log)
+    tr = Trainer(gc=gc, gf=gc + 2, dt_c=dt_c, log=log)
     tr.calibrate([1000, 1001], nsamples=8, spinup=4)
     tc = time.perf_counter()
     tr.collect(list(range(1, args.train_seeds + 1)), args.train_samples, spinup=4)
@@ -516,10 +524,10 @@ def bench_learned(args):
     tc = time.perf_counter()
     fit = tr.fit(args.fit_steps, 1e-3, 5e-5, 1e-4)
     t_fit = time.perf_counter() - tc
-    m7 = Mesh.icos(7)
-    mid = (Swm(m7, precision="fp32", dt=225.0, h_ref=H0_GEO),) + restriction(tr.fine_mesh, m7) + \
-        restriction(m7, tr.coarse_mesh)
-    res = [evaluate(tr, sd, args.eval_steps, spinup=4, mid=mid) for sd in (9999, 8888)]
+    mm = Mesh.icos(gc + 1)
+    mid = (Swm(mm, precision="fp32", dt=dt_c / 2, h_ref=H0_GEO),) + restriction(tr.fine_mesh, mm) + \
+        restriction(mm, tr.coarse_mesh)
+    res = [evaluate(tr, sd, nsteps, spinup=4, mid=mid) for sd in (9999, 8888)]
     for r in res:
         log(json.dumps(r))
     ne6 = 3 * tr.coarse_mesh.nv
@@ -531,12 +539,14 @@ def bench_learned(args):
 
     speed = mean("mid", "wall_s") / mean("learned", "wall_s")
     out = {
-        "metric": "learned glevel-6 vs plain glevel-7: error ratio and wall-time speedup (1 day)",
-        "value": speed, "unit": "x (plain glevel-7 wall time / learned glevel-6 wall time)",
-        "n_gpus": 1, "steps": args.eval_steps, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+        "metric": f"learned glevel-{gc} vs plain glevel-{gc + 1}: error ratio and wall-time speedup "
+                  f"({nsteps * dt_c / 86400:g} day)",
+        "value": speed, "unit": f"x (plain glevel-{gc + 1} wall time / learned glevel-{gc} wall time)",
+        "n_gpus": 1, "steps": nsteps, "warmup": 1, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 state, bf16 MFMA network",
-        "data": "synthetic (seeded spectral states; truth = FP64 glevel-8 run of this solver)",
-        "config": {"workload": "learned correction, glevel 6 dt 450 s vs glevel 7 dt 225 s vs glevel 8 truth",
+        "data": f"synthetic (seeded spectral states; truth = FP64 glevel-{gc + 2} run of this solver)",
+        "config": {"workload": f"learned correction, glevel {gc} dt {dt_c:g} s vs glevel {gc + 1} dt "
+                               f"{dt_c / 2:g} s vs glevel {gc + 2} truth",
                    "train_samples": args.train_seeds * args.train_samples, "fit_steps": args.fit_steps,
                    "collect_s": t_collect, "fit_s": t_fit, "setup_s": time.perf_counter() - t0},
         "errors": {k: {"h": mean(k, "h"), "u": mean(k, "u"), "wall_s": mean(k, "wall_s")}

@@ -38,6 +38,7 @@ SINT = {"trsk": 0, "bary": 1, "gass": 2}
 COR = {"trsk": 0, "pered": 1, "gass": 2, "dtred": 3}
 REC = {"trsk": 0, "perhx": 1, "gass": 2, "melv": 3}
 STAG = {"HTC": 0, "HC": 1}
+FSPHERE = {"variable": 0, "linear": 1, "constant": 2}
 
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _F64P = ctypes.POINTER(ctypes.c_double)
@@ -144,6 +145,8 @@ def lib():
             "imx_ic_testcase": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                                ctypes.c_double, _F64P, _F64P, _F64P]),
             "imx_swm_set_topography": (ctypes.c_int, [vp, _F64P]),
+            "imx_swm_set_coriolis": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_double]),
+            "imx_ic_testcase_coriolis": (ctypes.c_int, [ctypes.c_int32, _I32P, _F64P]),
             "imx_ic_spectral": (ctypes.c_int, [ctypes.POINTER(MeshDesc), ctypes.c_uint64,
                                                ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_double, _F64P, _F64P]),

@@ -143,6 +143,10 @@ struct imx_swm {
   int sc_sint = IMX_SINT_TRSK, sc_cor = IMX_COR_TRSK, sc_rec = IMX_REC_TRSK;
   double sc_gcoef = 1.0;
   bool sc_nopv = false;
+  // Coriolis parameter (imx_swm_set_coriolis): f = 2 Omega sin(lat) per
+  // triangle slot, kept on the host to restore it; f_lin: eta = f alone
+  std::vector<double> f_var;
+  bool f_lin = false;
   double *t_ka = nullptr;            // [3][NT] Gassmann K: areas of (circumcentre, edge ends)
   int32_t *c_tr = nullptr;           // [MX][nv] ring triangles of each cell
   double *c_hxtr = nullptr;          // [MX][nv] hxtr_area of each ring triangle

@@ -217,8 +217,8 @@ int imx_ic_spectral(const imx_mesh_desc *d, uint64_t seed, int32_t lmax, double 
 
 // ------------------------------------------------------------------------
 // The test cases of initialize_fields (src/swm.f90:1043-2414) that define a
-// state analytically or from a tabulated balance: 0, 1, 2, 5, 51, 6, 21,
-// 22, 23, 52, 53. Every expression keeps the reference's operation order,
+// state analytically or from a tabulated balance: 0, 1, 2, 5, 51, 6, 11, 12,
+// 21, 22, 23, 52, 53, 32-35 and 40-42. Every expression keeps the reference's operation order,
 // its single-precision literals (0.4, 2., 7.0, epsilon(1.)) and the
 // single-rounded pi, so the states equal the reference's bit for bit.
 namespace imx {
@@ -457,10 +457,132 @@ bool galewsky(const imx_mesh &m, bool hc, int tc, double erad, double *h, double
   return true;
 }
 
+// case(11, 12): the linearised equations at rest, for normal modes
+void tc11(const imx_mesh &m, double *h, double *u) {
+  for (int i = 0; i < m.nv; ++i) h[i] = 1e5 * kGravi;
+  std::fill(u, u + m.ne, 0.0);
+}
+
+// case(32, 33, 34, 35): Hollingsworth's test, a layer of depth hollgw over
+// a bottom in balance with the zonal flow of case 2; 33 and 35 on a
+// non-rotating sphere (f-sphere with f = 0, the balance without Coriolis)
+void hollingsworth(const imx_mesh &m, bool hc, int tc, double erad, double hollgw, double *h, double *u,
+                   double *bt) {
+  const double u0 = 2.0 * kPi * erad / (12.0 * kDay2sec);
+  const double h0 = 2.94e4 * kGravi;
+  const bool fs = (tc == 33 || tc == 35);
+  const double fcte = 0.0;
+  const double h_ct = fs ? ((u0 * u0) / 2.0) * kGravi : (erad * kOmega * u0 + (u0 * u0) / 2.0) * kGravi;
+  for (int i = 0; i < m.nv; ++i) {
+    h[i] = hollgw;
+    const double s = std::sin(m.v_lat[i]);
+    double b = h0 - h_ct * (s * s);
+    if (fs) b = b - (erad / kGrav) * fcte * s;
+    bt[i] = b;
+  }
+  for (int l = 0; l < m.ne; ++l) {
+    const EdgePoint e = edge_point(m, l, hc);
+    u[l] = along(e, u0 * std::cos(e.lat), 0.0);
+  }
+}
+
+// the rotation of cases 41 and 42 that moves (lon0, lat0) to the pole:
+// r = R p, summed in column order as matmul does
+struct Rotation {
+  double R[3][3];
+  Rotation(double lon0, double lat0) {
+    R[0][0] = std::sin(lat0) * std::cos(lon0);
+    R[0][1] = std::sin(lat0) * std::sin(lon0);
+    R[0][2] = -std::cos(lat0);
+    R[1][0] = -std::sin(lon0);
+    R[1][1] = std::cos(lon0);
+    R[1][2] = 0.0;
+    R[2][0] = std::cos(lat0) * std::cos(lon0);
+    R[2][1] = std::cos(lat0) * std::sin(lon0);
+    R[2][2] = std::sin(lat0);
+  }
+  void apply(const double *p, double *r) const {
+    for (int i = 0; i < 3; ++i) {
+      double a = 0.0;
+      for (int j = 0; j < 3; ++j) a = a + R[i][j] * p[j];
+      r[i] = a;
+    }
+  }
+  void apply_t(const double *p, double *r) const {
+    for (int i = 0; i < 3; ++i) {
+      double a = 0.0;
+      for (int j = 0; j < 3; ++j) a = a + R[j][i] * p[j];
+      r[i] = a;
+    }
+  }
+};
+
+// case(40, 41, 42): steady localised flows, h0 (2 - sin(lat)^n) about the
+// pole (40) or about (1 deg, 3 deg) (41, 42, through the rotation). 40 and
+// 41: u = u0 sin^k cos with f = 0; 42: on the f-sphere f = fcte, u from the
+// gradient-wind balance, at rest south of the rotated equator
+void localised(const imx_mesh &m, bool hc, int tc, double erad, double *h, double *u, double *bt) {
+  const double h0 = (tc == 40 ? 10000.0 : 100000.0) * kGravi;
+  const int k = (tc == 42) ? 160 : 120;
+  const int n = k * 2 + 2;
+  const double u0 = std::sqrt((double)n * h0 * kGrav);
+  const double fcte = (tc == 42) ? 1.4584e-4 : 0.0;
+  const Rotation rot(1.0 * kDeg2rad, 3.0 * kDeg2rad);
+  for (int i = 0; i < m.nv; ++i) {
+    double lat = m.v_lat[i];
+    if (tc != 40) {
+      double p[3], lon;
+      rot.apply(&m.v_p[3 * i], p);
+      cart2sph(p[0], p[1], p[2], &lon, &lat);
+    }
+    if (tc == 42 && !(lat > 0.0)) h[i] = 2.0 * h0;
+    else h[i] = h0 * (2.0 - ipow(std::sin(lat), n));
+    bt[i] = 0.0;
+  }
+  for (int l = 0; l < m.ne; ++l) {
+    const EdgePoint e = edge_point(m, l, hc);
+    if (tc == 40) {
+      u[l] = along(e, u0 * ipow(std::sin(e.lat), k) * std::cos(e.lat), 0.0);
+      continue;
+    }
+    double p[3], lon, lat;
+    rot.apply(e.p, p);
+    cart2sph(p[0], p[1], p[2], &lon, &lat);
+    double ut;
+    if (tc == 41) {
+      ut = u0 * ipow(std::sin(lat), k) * std::cos(lat);
+    } else {
+      // the HC branch tests lat > eps, the HTC one |lat| > eps
+      const bool away = hc ? lat > kEpsSingle : std::fabs(lat) > kEpsSingle;
+      const double F = away ? erad * fcte * std::cos(lat) / std::sin(lat) : 0.0;
+      const double c = std::cos(lat);
+      const double C = -(kGrav * h0 * (double)n * ipow(std::sin(lat), n - 2) * (c * c));
+      ut = (lat > 0.0) ? (-F + std::sqrt(F * F - 4.0 * C)) / 2.0 : 0.0;
+    }
+    double v[3], w[3];
+    vec_sph2cart(ut, 0.0, p, v);
+    rot.apply_t(v, w);
+    u[l] = dot3(w, e.dir);
+  }
+}
+
 }  // namespace
 }  // namespace imx
 
 extern "C" {
+
+int imx_ic_testcase_coriolis(int32_t tc, int32_t *fsphere, double *fcte) {
+  if (!fsphere || !fcte) return imx::set_error(IMX_EINVAL, "imx_ic_testcase_coriolis: NULL argument");
+  *fsphere = IMX_F_VARIABLE;
+  *fcte = 0.0;
+  switch (tc) {
+    case 11: *fsphere = IMX_F_LINEAR; *fcte = 1.4584e-4; break;
+    case 33: case 35: case 40: case 41: *fsphere = IMX_F_CONSTANT; break;
+    case 42: *fsphere = IMX_F_CONSTANT; *fcte = 1.4584e-4; break;
+    default: break;
+  }
+  return IMX_OK;
+}
 
 int imx_ic_testcase(const imx_mesh *m, int32_t tc, int32_t stag, double erad, double hollgw, double *h, double *u,
                     double *bt) {
@@ -486,6 +608,20 @@ int imx_ic_testcase(const imx_mesh *m, int32_t tc, int32_t stag, double erad, do
     case 5: imx::tc5(*m, hc, erad, hollgw, false, h, u, bt); break;
     case 51: imx::tc5(*m, hc, erad, hollgw, true, h, u, bt); break;
     case 6: imx::tc6(*m, hc, erad, h, u); break;
+    case 11:
+    case 12: imx::tc11(*m, h, u); break;
+    case 32:
+    case 33:
+    case 34:
+    case 35:
+      if (hollgw < imx::kEpsSingle)
+        return imx::set_error(IMX_EINVAL, "imx_ic_testcase: the Hollingsworth test cases need a positive layer "
+                                          "thickness (hollgw)");
+      imx::hollingsworth(*m, hc, tc, erad, hollgw, h, u, bt);
+      break;
+    case 40:
+    case 41:
+    case 42: imx::localised(*m, hc, tc, erad, h, u, bt); break;
     case 21:
     case 22:
     case 23:
@@ -495,7 +631,8 @@ int imx_ic_testcase(const imx_mesh *m, int32_t tc, int32_t stag, double erad, do
       break;
     default:
       return imx::set_error(IMX_ENOTSUP, "imx_ic_testcase: test case " + std::to_string(tc) +
-                                             " is not part of this build (0, 1, 2, 5, 6, 21, 22, 23, 51, 52, 53 are)");
+                                             " is not part of this build (0, 1, 2, 5, 6, 11, 12, 21, 22, 23, 32, 33, 34, 35, 40, 41, "
+                                             "42, 51, 52, 53 are)");
   }
   return IMX_OK;
 }

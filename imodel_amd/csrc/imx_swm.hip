@@ -109,7 +109,7 @@ __global__ __launch_bounds__(TPB) void ref_tri(int nt, const int32_t *__restrict
                                                const double *__restrict__ h, int hmode,
                                                const double *__restrict__ t_w,
                                                const double *__restrict__ h_ed, double erad,
-                                               int use_pv, double *__restrict__ eta,
+                                               int use_pv, int flin, double *__restrict__ eta,
                                                double *__restrict__ zeta,
                                                double *__restrict__ h_tr,
                                                double *__restrict__ q_tr, int *__restrict__ flag) {
@@ -123,7 +123,8 @@ __global__ __launch_bounds__(TPB) void ref_tri(int nt, const int32_t *__restrict
   }
   const double z = e / t_area[k] / erad;
   zeta[k] = z;
-  const double et = z + t_f[k];
+  // absolute vorticity; the linearised f-sphere keeps f alone
+  const double et = flin ? t_f[k] : z + t_f[k];
   eta[k] = et;
   double s = 0.0, ht;
   if (hmode == 1) {
@@ -710,7 +711,7 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
                                                 const T *__restrict__ t_kap,
                                                 const T *__restrict__ t_f,
                                                 const T *__restrict__ u,
-                                                const T *__restrict__ h, T href, int use_pv,
+                                                const T *__restrict__ h, T href, int use_pv, int flin,
                                                 const T *__restrict__ kt, T *__restrict__ cz,
                                                 T *__restrict__ q, int *__restrict__ flag) {
   const int t = ent_index<BS>(remap);
@@ -725,9 +726,11 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
   const int e1 = 3 * (((eb >> 3) & 1) ? a : b) + ((eb >> 4) & 3);
   const int e2 = 3 * (((eb >> 6) & 1) ? b : o) + ((eb >> 7) & 3);
   T eta = t_f[t];
-  eta += u[e0] * t_ceta[t];
-  eta += u[e1] * t_ceta[NT + t];
-  eta += u[e2] * t_ceta[2 * NT + t];
+  if (!flin) {  // the linearised f-sphere keeps f alone
+    eta += u[e0] * t_ceta[t];
+    eta += u[e1] * t_ceta[NT + t];
+    eta += u[e2] * t_ceta[2 * NT + t];
+  }
   if (kt) {  // diffusion: K zeta (relative vorticity) at the triangle
     T z = u[e0] * t_ceta[t];
     z += u[e1] * t_ceta[NT + t];
@@ -1240,7 +1243,7 @@ __global__ __launch_bounds__(TPB) void diag_tri(int off, int cnt, int nt, const 
                                                 const double *__restrict__ e_leng,
                                                 const T *__restrict__ hs,
                                                 const T *__restrict__ us, double href,
-                                                double erad, double *__restrict__ part) {
+                                                double erad, int flin, double *__restrict__ part) {
   double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int k = off + blockIdx.x * TPB + threadIdx.x;
   if (k < off + cnt && t_ed[k] >= 0) {
@@ -1250,7 +1253,7 @@ __global__ __launch_bounds__(TPB) void diag_tri(int off, int cnt, int nt, const 
       e += (double)us[ed] * e_leng[ed] * (double)t_tg[l * nt + k];
       s += t_trhx[l * nt + k] * (href + (double)hs[t_v[l * nt + k]]);
     }
-    const double eta = e / t_area[k] / erad + t_f[k];
+    const double eta = flin ? t_f[k] : e / t_area[k] / erad + t_f[k];
     const double ht = s / t_area[k];
     const double qq = eta / ht;
     v[5] = 0.5 * t_area[k] * erad * erad * ht * qq * qq;
@@ -1338,7 +1341,7 @@ static void launch_ref_tendency(imx_swm *s, const double *h, const double *u, do
                                               s->f_h_ed, s->f_uh);
   ref_tri<<<grid_for(NT), TPB, 0, st>>>(NT, s->t_ed, tsg, s->t_v, s->t_area, s->t_f,
                                         s->t_trhx, s->e_leng, u, h, sint, s->t_w, s->f_h_ed,
-                                        s->o.erad, s->o.testcase >= 2, s->f_eta, s->f_zeta,
+                                        s->o.erad, s->o.testcase >= 2, s->f_lin ? 1 : 0, s->f_eta, s->f_zeta,
                                         s->f_h_tr, s->f_q_tr, s->flag);
   ref_edge_q<<<grid_for(NE), TPB, 0, st>>>(NE, qmode, s->e_sh, s->e_dw, s->t_area, s->f_eta, s->f_h_tr,
                                            s->f_q_tr, s->f_q_ed, s->sc_nopv ? s->f_eta_ed : nullptr,
@@ -1449,7 +1452,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
 #define IMX_TRI(B)                                                                          \
   fast_tri<T, B><<<grid_bs(NT, B), B, 0, st>>>(                                            \
       s->kremap[0], NT, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta, (const T *)s->x_t_kap, \
-      (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2,                                      \
+      (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, s->f_lin ? 1 : 0,                    \
       (s->f_diff || s->f_hyper) ? (const T *)s->x_kt : nullptr,                                    \
       (T *)s->x_cz, (T *)s->q, s->flag)
   IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
@@ -1541,7 +1544,7 @@ static void launch_diag(imx_swm *s, const T *hs, const T *us, double href, int c
   diag_edge<T><<<ge, TPB, 0, st>>>(3 * c0, ne, s->NE, s->e_v, s->e_leng, s->e_dleng, hs, us, href,
                                    part + (size_t)gv * NPART);
   diag_tri<T><<<gt, TPB, 0, st>>>(2 * c0, nt, s->NT, s->t_ed, s->sc_hc ? s->hc_t : s->t_tg, s->t_v, s->t_area, s->t_f,
-                                  s->t_trhx, s->e_leng, hs, us, href, s->o.erad,
+                                  s->t_trhx, s->e_leng, hs, us, href, s->o.erad, s->f_lin ? 1 : 0,
                                   part + (size_t)(gv + ge) * NPART);
   diag_final<<<1, TPB, 0, st>>>(gv + ge + gt, part, diag);
 }
@@ -2330,6 +2333,7 @@ static int create(const imx_mesh_desc &d, const imx_swm_opts &o, imx_swm *s,
   s->t_tg = A.put(t_tg);
   s->t_area = A.put(t_area);
   s->t_f = A.put(t_f);
+  s->f_var = t_f;
   s->t_trhx = A.put(t_trhx);
   s->bt = A.put(std::vector<double>(nv, 0.0));
   s->flag = A.put(std::vector<int>(1, 0));
@@ -2686,6 +2690,26 @@ int imx_swm_set_topography(imx_swm *s, const double *bt) {
   s->topo_sb2 = sb2;
   s->x_gb = (s->fast && any) ? s->x_gb_buf : nullptr;
   imx::drop_graph(s);  // the captured step holds the previous topography pointer
+  return IMX_OK;
+}
+
+int imx_swm_set_coriolis(imx_swm *s, int32_t fsphere, double fcte) {
+  if (!s) return imx::set_error(IMX_EINVAL, "imx_swm_set_coriolis: NULL handle");
+  if (fsphere < IMX_F_VARIABLE || fsphere > IMX_F_CONSTANT || !std::isfinite(fcte))
+    return imx::set_error(IMX_EINVAL, "imx_swm_set_coriolis: bad fsphere or fcte");
+  std::vector<double> f(s->f_var);
+  if (fsphere != IMX_F_VARIABLE) std::fill(f.begin(), f.end(), fcte);
+  HIPTRY(hipMemcpy(s->t_f, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice), "imx_swm_set_coriolis");
+  if (s->fast) {
+    if (s->single) {
+      std::vector<float> g(f.begin(), f.end());
+      HIPTRY(hipMemcpy(s->x_t_f, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice), "imx_swm_set_coriolis");
+    } else {
+      HIPTRY(hipMemcpy(s->x_t_f, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice), "imx_swm_set_coriolis");
+    }
+  }
+  s->f_lin = fsphere == IMX_F_LINEAR;
+  imx::drop_graph(s);  // the captured step holds the previous flag
   return IMX_OK;
 }
 

@@ -166,6 +166,20 @@ module imx_c
       real(c_double) :: h(*), u(*), bt(*)
       integer(c_int) :: rc
     end function
+    function imx_ic_testcase_coriolis(tc, fsphere, fcte) bind(C, name='imx_ic_testcase_coriolis') result(rc)
+      import :: c_int, c_int32_t, c_double
+      integer(c_int32_t), value :: tc
+      integer(c_int32_t) :: fsphere
+      real(c_double) :: fcte
+      integer(c_int) :: rc
+    end function
+    function imx_swm_set_coriolis(s, fsphere, fcte) bind(C, name='imx_swm_set_coriolis') result(rc)
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: s
+      integer(c_int32_t), value :: fsphere
+      real(c_double), value :: fcte
+      integer(c_int) :: rc
+    end function
     function imx_swm_set_topography(s, bt) bind(C, name='imx_swm_set_topography') result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: s

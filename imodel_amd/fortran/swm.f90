@@ -76,7 +76,12 @@ contains
     call perot_kenergy(d0, .true.)
     ! RefSolAnal (src/swm_data.f90:685): the steady cases compare with the
     ! initial state; TC2 also has an exact PV
-    refsol = testcase == 1 .or. testcase == 2 .or. testcase == 22
+    select case (testcase)
+    case (1, 2, 22, 32, 33, 34, 35, 40, 41, 42)
+      refsol = .true.
+    case default
+      refsol = .false.
+    end select
     if (refsol) then
       allocate(q_exact(mdesc%nt), qtr(mdesc%nt))
       q_exact = 0._r8
@@ -426,6 +431,10 @@ contains
       swmname = trim(swmname)//"_pvs"//trim(adjustl(trim(pv_stab)))//trim(adjustl(trim(atmp)))
     end if
     if (inopv > 0) swmname = trim(swmname)//"_nopv"
+    if (hollgw > 0 .and. testcase >= 32 .and. testcase <= 35) then
+      write(atmp, '(f6.2)') real(hollgw)
+      swmname = trim(swmname)//"_hol"//trim(adjustl(trim(atmp)))
+    end if
     if (k2_max > 0) then
       write(atmp, '(f10.3)') real(dlog(k2_max)/dlog(10._r8))
       swmname = trim(swmname)//"_"//trim(diffus)//"_diffusion_10to"//trim(adjustl(trim(atmp)))
@@ -455,6 +464,15 @@ contains
     o%omega = omega
     o%dt = dt
     call imx_check(imx_swm_create(mdesc, o, hnd), 'imx_swm_create')
+    ! the f-sphere of the test case, as initialize_fields sets it
+    block
+      integer(c_int32_t) :: fs
+      real(c_double) :: fc
+      call imx_check(imx_ic_testcase_coriolis(int(testcase, c_int32_t), fs, fc), 'open_handle')
+      fsphere = fs
+      fcte = fc
+      if (fs /= 0) call imx_check(imx_swm_set_coriolis(hnd, fs, fc), 'imx_swm_set_coriolis')
+    end block
     if (scheme_options()) then
       ! the fused fp32/fp64 kernels carry trsk10 and (with HTC) pxt16, with
       ! APVM, diffusion and hyperdiffusion

@@ -19,6 +19,10 @@ module swm_data
   ! initialize_gridprop
   type(c_ptr) :: handle = c_null_ptr
   logical :: usetime_names = .true.
+  ! the Coriolis term of the test case (src/swm_data.f90:42-43): 0 variable
+  ! f, 1 linearised (eta = fcte), 2 f-sphere (f = fcte)
+  integer :: fsphere = 0
+  real(r8) :: fcte = 0._r8
 
 contains
 

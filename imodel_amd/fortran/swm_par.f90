@@ -4,7 +4,8 @@
 !                 [meshpar] [nv] override as in the reference
 !   par/swm.par   as swmpars        (src/swm_data.f90:338-693), the
 !                 geo-area subset this build runs: test cases 0, 1, 2, 5,
-!                 6, 21, 22, 23, 51, 52, 53 and the seeded state 100;
+!                 6, 11, 12, 21, 22, 23, 32-35, 40-42, 51, 52, 53 and the
+!                 seeded state 100;
 !                 trsk10 with HTC or HC and pxt16 with HTC, APVM, diffusion and
 !                 hyperdiffusion on every path; gass18, HC pxt16, wrapper
 !                 none (sinterpol trsk/bary/gass, Coriolis trsk/pered/gass/
@@ -173,9 +174,10 @@ contains
     if (trim(pv_stab) /= 'none' .and. trim(pv_stab) /= 'apvm') call unsupported('PV stabilisation', pv_stab)
     if (test_lterror /= 0) call unsupported('local truncation error test', '1')
     select case (testcase)
-    case (0, 1, 2, 5, 6, 21, 22, 23, 51, 52, 53, 100)
+    case (0, 1, 2, 5, 6, 11, 12, 21, 22, 23, 32, 33, 34, 35, 40, 41, 42, 51, 52, 53, 100)
     case default
-      call unsupported('test case (0, 1, 2, 5, 6, 21, 22, 23, 51, 52, 53 or the seeded 100)', 'other')
+      call unsupported('test case (0, 1, 2, 5, 6, 11, 12, 21, 22, 23, 32-35, 40-42, 51, 52, 53 or the seeded 100)', &
+                       'other')
     end select
 
     ! dt / ntime (src/swm_data.f90:562-586)

@@ -143,15 +143,26 @@ class Mesh:
         return h, u
 
     def testcase(self, tc, stag="HTC", erad=ERAD_REF, hollgw=2.0):
-        """initialize_fields of test case tc (0, 1, 2, 5, 6, 21, 22, 23, 51,
-        52, 53) on a library mesh: (h, u, bt), u for HTC or HC staggering;
-        hollgw is par/swm.par's Hollingsworth depth (test case 51)."""
+        """initialize_fields of test case tc (0, 1, 2, 5, 6, 11, 12, 21, 22,
+        23, 32-35, 40-42, 51, 52, 53) on a library mesh: (h, u, bt), u for
+        HTC or HC staggering; hollgw is par/swm.par's Hollingsworth depth
+        (test cases 32-35 and 51). The Coriolis parameter a case runs with
+        is testcase_coriolis(tc)."""
         if self._handle is None:
             raise ValueError("testcase needs a library-built mesh")
         h, u, bt = np.zeros(self.nv), np.zeros(self.ne), np.zeros(self.nv)
         capi.check(capi.lib().imx_ic_testcase(self._handle, int(tc), capi.STAG[stag], float(erad), float(hollgw),
                                               capi.f64p(h), capi.f64p(u), capi.f64p(bt)), "imx_ic_testcase")
         return h, u, bt
+
+    @staticmethod
+    def testcase_coriolis(tc):
+        """(fsphere, fcte) of test case tc, for Swm.set_coriolis."""
+        fs = np.zeros(1, dtype=np.int32)
+        fc = np.zeros(1)
+        capi.check(capi.lib().imx_ic_testcase_coriolis(int(tc), capi.i32p(fs), capi.f64p(fc)),
+                   "imx_ic_testcase_coriolis")
+        return int(fs[0]), float(fc[0])
 
     def spectral(self, seed, lmax=16, h0=H0_GEO, hfrac=0.05, urms=20.0, erad=ERAD_REF):
         """Seeded spectral state (h, u)."""
@@ -229,6 +240,12 @@ class Swm:
             return
         bt = np.ascontiguousarray(bt, dtype=np.float64)
         capi.check(capi.lib().imx_swm_set_topography(self._h, capi.f64p(bt)), "imx_swm_set_topography")
+
+    def set_coriolis(self, fsphere=0, fcte=0.0):
+        """The Coriolis term: fsphere 0 (f = 2 Omega sin(lat)), 1 (eta = fcte
+        alone, the linearised equations) or 2 (f = fcte), as vort_tr."""
+        fsphere = capi.FSPHERE.get(fsphere, fsphere)
+        capi.check(capi.lib().imx_swm_set_coriolis(self._h, int(fsphere), float(fcte)), "imx_swm_set_coriolis")
 
     def get_state(self):
         h = np.zeros(self.nv)

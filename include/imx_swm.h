@@ -282,6 +282,16 @@ int  imx_ic_tc2_pv(const imx_mesh *m, double erad, double *q_tr);
    IMX_ENOTSUP for other cases. */
 int  imx_ic_testcase(const imx_mesh *m, int32_t tc, int32_t stag, double erad, double hollgw,
                      double *h, double *u, double *bt);
+/* The Coriolis parameter a test case runs with (the module globals fsphere
+   and fcte that initialize_fields sets, src/swm_data.f90:42-43):
+   IMX_F_VARIABLE f = 2 Omega sin(lat) (the default), IMX_F_LINEAR the
+   absolute vorticity is the constant fcte alone (11: the linearised
+   equations), IMX_F_CONSTANT f = fcte (33, 35, 40, 41 with fcte = 0; 42 with
+   1.4584e-4). Apply it with imx_swm_set_coriolis. */
+#define IMX_F_VARIABLE 0
+#define IMX_F_LINEAR   1
+#define IMX_F_CONSTANT 2
+int  imx_ic_testcase_coriolis(int32_t tc, int32_t *fsphere, double *fcte);
 /* Seeded spectral state: real spherical harmonics l=1..lmax, coefficients
    N(0,1)*l^-2 from SplitMix64 + Box-Muller; h = h0 + h', h' area-rms =
    hfrac*h0; u = -(psi(t2)-psi(t1))/(l_e a) * sign, scaled to area-rms
@@ -309,6 +319,12 @@ int  imx_swm_get_state(imx_swm *s, double *h, double *u);
    (calc_energies, :2686-2752). NULL restores b = 0. h stays the fluid
    depth. */
 int  imx_swm_set_topography(imx_swm *s, const double *bt);
+/* The Coriolis term of vort_tr (src/swm_operators.f90:596-605): fsphere
+   IMX_F_VARIABLE (f = 2 Omega sin(lat) at the circumcentres, the default),
+   IMX_F_LINEAR (eta = fcte, the relative vorticity left out) or
+   IMX_F_CONSTANT (eta = zeta + fcte). Every path; the potential enstrophy
+   takes the same eta. */
+int  imx_swm_set_coriolis(imx_swm *s, int32_t fsphere, double fcte);
 /* subroutine tendency(h, u, masseq, momeq)       src/swm.f90:2416-2598 */
 int  imx_swm_tendency(imx_swm *s, const double *h, const double *u,
                       double *masseq, double *momeq);

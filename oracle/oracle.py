@@ -41,7 +41,8 @@ class _Scheme(ctypes.Structure):
 class _Params(ctypes.Structure):
     _fields_ = [("erad", ctypes.c_double), ("grav", ctypes.c_double),
                 ("omega", ctypes.c_double), ("testcase", ctypes.c_int),
-                ("trhx_area", _F64P), ("trskw", _F64P), ("scheme", ctypes.POINTER(_Scheme))]
+                ("trhx_area", _F64P), ("trskw", _F64P), ("scheme", ctypes.POINTER(_Scheme)),
+                ("fsphere", ctypes.c_int), ("fcte", ctypes.c_double)]
 
 
 class _Fields(ctypes.Structure):
@@ -87,18 +88,20 @@ class Oracle:
       stag ("HTC" | "HC"), wrap ("trsk10" | "pxt16" | "gass18"),
       pv ("none" | "apvm"), pvspar, dt (the time step the APVM term reads),
       k2, diffus, k4, hyperdiffus ("const" | "align" | "diam"),
-    the entries of par/swm.par.
+    the entries of par/swm.par. fsphere/fcte: the Coriolis term a test case
+    sets (0: f = 2 Omega sin(lat), 1: eta = fcte, 2: f = fcte).
     """
 
     def __init__(self, mesh, erad=ERAD_REF, grav=GRAV, omega=OMEGA, testcase=2,
-                 perot=False, scheme=None):
+                 perot=False, scheme=None, fsphere=0, fcte=0.0):
         self.mesh = mesh
         self.desc, self._keep = capi.make_desc(mesh)
         self.nv, self.ne, self.nt = self.desc.nv, self.desc.ne, self.desc.nt
         self._trhx = np.ascontiguousarray(mesh["tr_trhx_area"], dtype=np.float64)
         self._trskw = np.ascontiguousarray(mesh["hx_trskw"], dtype=np.float64)
         self.p = _Params(erad=erad, grav=grav, omega=omega, testcase=testcase,
-                         trhx_area=capi.f64p(self._trhx), trskw=capi.f64p(self._trskw))
+                         trhx_area=capi.f64p(self._trhx), trskw=capi.f64p(self._trskw),
+                         fsphere=int(fsphere), fcte=float(fcte))
         if scheme:
             sc = dict(scheme)
             k2, k4 = float(sc.get("k2", 0.0)), float(sc.get("k4", 0.0))

@@ -367,7 +367,10 @@ int oracle_tendency(const imx_mesh_desc *d, const oracle_params *p, const double
       e = e + u[ed] * d->ed_leng[ed] * signcor;
     }
     e = e / d->tr_areag[k] / erad;
-    eta[k] = e + 2.0 * p->omega * sin(d->tr_clat[k]);
+    /* the Coriolis term by fsphere (src/swm_operators.f90:596-605) */
+    if (p->fsphere == 1) eta[k] = p->fcte;
+    else if (p->fsphere == 2) eta[k] = e + p->fcte;
+    else eta[k] = e + 2.0 * p->omega * sin(d->tr_clat[k]);
   }
 
   const int wrap = p->scheme ? p->scheme->wrap : 0;

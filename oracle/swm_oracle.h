@@ -28,6 +28,8 @@ typedef struct oracle_params {
   const double *trhx_area; /* [nt*3] */
   const double *trskw;     /* [nv*6*6] */
   const oracle_scheme *scheme; /* NULL: the defaults */
+  int fsphere;             /* 0: f = 2 Omega sin(lat); 1: eta = fcte; 2: f = fcte */
+  double fcte;
 } oracle_params;
 
 /* module-global fields of swm_data written by tendency */

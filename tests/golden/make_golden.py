@@ -144,6 +144,14 @@ TESTCASES = {
     "tc6": "tc=6",
     "tc21": "tc=21",
     "tc53": "tc=53",
+    "tc11": "tc=11",
+    "tc32": "tc=32",
+    "tc33": "tc=33",
+    "tc34hc": "tc=34,stag=HC",
+    "tc40": "tc=40",
+    "tc41": "tc=41",
+    "tc42": "tc=42",
+    "tc42hc": "tc=42,stag=HC",
 }
 
 

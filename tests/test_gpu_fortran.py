@@ -303,6 +303,22 @@ def test_fortran_driver_g5_tc5_day_vs_reference_end_to_end():
     assert np.array_equal(st["h"], r["h"]) and np.array_equal(st["u"], r["u"])
 
 
+@pytest.mark.parametrize("tc,tag", [(33, "_hol2.00"), (42, ""), (11, "")])
+def test_fortran_driver_fsphere_cases_bitexact(golden, tc, tag):
+    """Test cases that set the Coriolis term (33: f = 0 under the
+    Hollingsworth layer, 42: the f-sphere, 11: the linearised equations):
+    3 steps from par/ files equal the reference's run of the same case."""
+    print(f"[gpu-case] fortran imodel g3 tc{tc} fp64_ref f-sphere", flush=True)
+    r = golden(f"ref_run_g3_tc{tc}.bin")
+    with tempfile.TemporaryDirectory() as td:
+        _par(td, 642, tc, 900, 0.03125, 3, "fp64_ref")
+        res = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                             timeout=120)
+        assert res.returncode == 0, res.stdout[-2000:]
+        st = imxbin.read(os.path.join(td, "data", f"swm_tc{tc}_dt900_HTC_trsk10_areageo{tag}_state_icos_eqs_nopt_3.bin"))
+    assert np.array_equal(st["h"], r["h"]) and np.array_equal(st["u"], r["u"])
+
+
 # wrapper "none" in par/swm.par (stag, reconmtd + gasscoef, Coriolis,
 # sinterpol, PV stabilisation, level model lines) against the reference's
 # own 3-step runs from the seeded state of seed 7 (ref_run_g3_m<name>.bin)

@@ -3,13 +3,14 @@ topography they carry, against fixtures the reference wrote
 (tests/golden/ref_run_g3_tc*.bin, make_golden.py tc).
 
 * imx_ic_testcase reproduces the reference's h, u and bt bit for bit
-  (test cases 1, 5 HTC and HC, 51, 6, 21, 53 at glevel 3; the others of
-  the build's list are checked against the reference harness when it is
-  built).
-* the oracle, given bt, reproduces the reference's tendency (with g (h + b)
-  in the Bernoulli potential), its intermediates, the diagnostics (potential
-  and available energy with b) and 3 RK4 steps bit for bit; dropping bt
-  fails the check.
+  (test cases 1, 5 HTC and HC, 51, 6, 21, 53, 11, 32, 33, 34 HC, 40, 41, 42
+  HTC and HC at glevel 3; the others of the build's list are checked
+  against the reference harness when it is built).
+* the oracle, given bt and the case's Coriolis setting (f-sphere), reproduces
+  the reference's tendency (with g (h + b) in the Bernoulli potential), its
+  intermediates, the diagnostics (potential and available energy with b)
+  and 3 RK4 steps bit for bit; dropping bt, or the f-sphere, fails the
+  check.
 """
 import os
 import subprocess
@@ -22,7 +23,9 @@ from imodel_amd.swm import Mesh
 from oracle.oracle import REF_HARNESS, Oracle, ref_harness_available
 
 CASES = {"tc1": (1, "HTC"), "tc5": (5, "HTC"), "tc5hc": (5, "HC"), "tc51": (51, "HTC"), "tc6": (6, "HTC"),
-         "tc21": (21, "HTC"), "tc53": (53, "HTC")}
+         "tc21": (21, "HTC"), "tc53": (53, "HTC"), "tc11": (11, "HTC"), "tc32": (32, "HTC"),
+         "tc33": (33, "HTC"), "tc34hc": (34, "HC"), "tc40": (40, "HTC"), "tc41": (41, "HTC"),
+         "tc42": (42, "HTC"), "tc42hc": (42, "HC")}
 FIELDS = ["h_ed", "uh", "eta", "h_tr", "q_tr", "q_ed", "divuh", "ke_hx", "ghbK", "grad_ghbK", "uhq_perp"]
 
 
@@ -35,7 +38,7 @@ def test_testcase_state_bitexact(golden, name):
 
 
 @pytest.mark.skipif(not ref_harness_available(), reason="reference harness not built (oracle/Makefile.ref)")
-@pytest.mark.parametrize("tc", [0, 2, 22, 23, 52])
+@pytest.mark.parametrize("tc", [0, 2, 12, 22, 23, 34, 35, 52])
 @pytest.mark.parametrize("stag", ["HTC", "HC"])
 def test_testcase_state_vs_harness(tmp_path, tc, stag):
     spec = f"tc={tc}" + (",stag=HC" if stag == "HC" else "")
@@ -68,7 +71,9 @@ def test_testcase_state_pol_vs_harness(tmp_path, tc):
 
 def _oracle(golden, name):
     tc, stag = CASES[name]
-    return Oracle(golden("ref_mesh_g3.bin"), testcase=tc, scheme={"stag": stag} if stag == "HC" else None)
+    fsphere, fcte = Mesh.testcase_coriolis(tc)
+    return Oracle(golden("ref_mesh_g3.bin"), testcase=tc, scheme={"stag": stag} if stag == "HC" else None,
+                  fsphere=fsphere, fcte=fcte)
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -89,6 +94,27 @@ def test_oracle_with_topography_bitexact(golden, name):
         d = o.diagnostics(h, u, refresh=False)
         assert np.array_equal(d[[0, 1, 2, 3, 4, 6]], ser[k, 1:]), k
     assert np.array_equal(h, r["h"]) and np.array_equal(u, r["u"])
+
+
+def test_coriolis_of_test_cases():
+    """The f-sphere settings initialize_fields makes: f = 0 for the
+    non-rotating Hollingsworth and localised cases, 1.4584e-4 on the
+    f-sphere of 42 and the linearised 11."""
+    assert Mesh.testcase_coriolis(2) == (0, 0.0)
+    assert Mesh.testcase_coriolis(32) == (0, 0.0) and Mesh.testcase_coriolis(34) == (0, 0.0)
+    for tc in (33, 35, 40, 41):
+        assert Mesh.testcase_coriolis(tc) == (2, 0.0)
+    assert Mesh.testcase_coriolis(42) == (2, 1.4584e-4)
+    assert Mesh.testcase_coriolis(11) == (1, 1.4584e-4)
+
+
+def test_wrong_coriolis_is_caught(golden):
+    """Test case 42 on the f-sphere: the oracle with the variable f
+    disagrees with the reference's tendency."""
+    r = golden("ref_run_g3_tc42.bin")
+    o = Oracle(golden("ref_mesh_g3.bin"), testcase=42)
+    m, f = o.tendency(r["h0"], r["u0"])
+    assert not np.array_equal(f, r["momeq"])
 
 
 def test_dropped_topography_is_caught(golden):
