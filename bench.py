@@ -152,6 +152,8 @@ def main():
     ap.add_argument("--train-samples", type=int, default=48)
     ap.add_argument("--fit-steps", type=int, default=6000)
     ap.add_argument("--eval-steps", type=int, default=0, help="learned: coarse steps evaluated (0: one day)")
+    ap.add_argument("--limiter", default="fallback", choices=["fallback", "scale"],
+                    help="learned: a correction above the bound keeps the plain step, or is scaled to it")
     ap.add_argument("--coarse-glevel", type=int, default=6,
                     help="learned: the corrected glevel gc (6: config 4; 8: config 5's leg), trained on "
                          "glevel gc+2 truth, compared with plain gc+1 at half the step; dt = 450 s at glevel 6, "
@@ -517,6 +519,7 @@ def bench_learned(args):
     nsteps = args.eval_steps or int(round(86400.0 / dt_c))
     t0 = time.perf_counter()
     tr = Trainer(gc=gc, gf=gc + 2, dt_c=dt_c, log=log)
+    tr.learn.set_limiter(args.limiter)
     tr.calibrate([1000, 1001], nsamples=8, spinup=4)
     tc = time.perf_counter()
     tr.collect(list(range(1, args.train_seeds + 1)), args.train_samples, spinup=4)
@@ -548,6 +551,7 @@ def bench_learned(args):
         "config": {"workload": f"learned correction, glevel {gc} dt {dt_c:g} s vs glevel {gc + 1} dt "
                                f"{dt_c / 2:g} s vs glevel {gc + 2} truth",
                    "train_samples": args.train_seeds * args.train_samples, "fit_steps": args.fit_steps,
+                   "limiter": args.limiter,
                    "collect_s": t_collect, "fit_s": t_fit, "setup_s": time.perf_counter() - t0},
         "errors": {k: {"h": mean(k, "h"), "u": mean(k, "u"), "wall_s": mean(k, "wall_s")}
                    for k in ("plain", "learned", "mid")},

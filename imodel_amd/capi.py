@@ -129,6 +129,7 @@ def lib():
             "imx_mesh_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(vp)]),
             "imx_mesh_build_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(vp)]),
             "imx_swm_lte": (ctypes.c_int, [vp, vp, _F64P, _F64P]),
+            "imx_swm_lte_hol": (ctypes.c_int, [vp, vp, ctypes.c_double, _F64P, _F64P]),
             "imx_mesh_load_ref": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]),
             "imx_mesh_write_ref": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_char_p]),
             "imx_make_dir": (ctypes.c_int, [ctypes.c_char_p]),

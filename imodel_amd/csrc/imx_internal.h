@@ -130,13 +130,14 @@ struct BinWriter {
 // q_grad_tr and lapu NULL when the options leave them uncomputed).
 struct LteFields {
   const double *h, *u, *h_ed, *uh, *divuh, *ke_hx, *eta, *h_tr, *q_tr, *q_hx, *q_ed, *grad_ghbk,
-      *vhq_tr, *q_grad_tr, *vh_hx, *vhq_hx, *uhq_perp, *lapu, *masseq, *momeq;
+      *vhq_tr, *q_grad_tr, *vh_hx, *vhq_hx, *uhq_perp, *lapu, *masseq, *momeq, *bt;
 };
-// Closed forms of TC2, error_calc of each row and the mimetic sums of
+// Closed forms of test case tc (2, or the Hollingsworth cases 32-35 with
+// layer depth hollgw), error_calc of each row and the mimetic sums of
 // swm_horiz_loc_trunc_er (imx_lte.cpp); perhx: Perot's cell reconstruction
-// (pxt16), which changes calc_energies' kinetic energy.
-int lte_analysis(const HostMesh &m, bool hc, bool perhx, double erad, double grav, const LteFields &f,
-                 double *err, double *mim);
+// of the kinetic energy, which changes calc_energies' kinetic energy.
+int lte_analysis(const HostMesh &m, int tc, double hollgw, bool hc, bool perhx, double erad, double grav,
+                 const LteFields &f, double *err, double *mim);
 
 }  // namespace imx
 

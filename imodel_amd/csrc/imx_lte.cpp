@@ -109,20 +109,23 @@ void wachspress_node(const imx_mesh_desc &d, int i, double *w) {
   for (int j = 0; j < n; ++j) w[j] = w[j] / sw;
 }
 
-int lte_analysis(const HostMesh &m, bool hc, bool perhx, double erad, double grav, const LteFields &f,
-                 double *err, double *mim) {
+int lte_analysis(const HostMesh &m, int tc, double hollgw, bool hc, bool perhx, double erad, double grav,
+                 const LteFields &f, double *err, double *mim) {
   const int nv = m.nv, ne = m.ne, nt = m.nt;
   if (hc && ((int)m.edhx_nr.size() < 3 * ne || (int)m.edhx_tg.size() < 3 * ne))
     return set_error(IMX_EINVAL, "imx_swm_lte: the mesh has no Voronoi edge vectors");
   if ((int)m.ed_tg.size() < 3 * ne || (int)m.ed_nr.size() < 3 * ne || (int)m.hx_nr.size() < IMX_MAXNB * nv)
     return set_error(IMX_EINVAL, "imx_swm_lte: the mesh has no edge vectors");
-  // TC2 constants (src/swm.f90:1206-1209, constants.f90)
+  // constants of TC2 (src/swm.f90:1206-1209) and of the Hollingsworth
+  // cases (:1918-1952): the same zonal flow, over a bottom in balance with
+  // it under a layer of constant depth hollgw; 33 and 35 without rotation
   const double pi = 3.1415927410125732;  // constants.f90:31 (single-rounded)
   const double pi2 = 2.0 * pi, day2sec = 86400.0, gravi = 1.0 / 9.80616, omega = 7.292e-5;
+  const bool hol = tc >= 32 && tc <= 35, fs = tc == 33 || tc == 35;
   const double u0 = pi2 * erad / (12.0 * day2sec);
   const double h0 = 2.94e4 * gravi;
-  const double h_ct = (erad * omega * u0 + (u0 * u0) / 2.0) * gravi;
-  const double eta_ct = (2. * u0 / erad + 2. * omega);
+  const double h_ct = fs ? ((u0 * u0) / 2.0) * gravi : (erad * omega * u0 + (u0 * u0) / 2.0) * gravi;
+  const double eta_ct = fs ? (2. * u0 / erad) : (2. * u0 / erad + 2. * omega);
   const double grad_ct = u0 * eta_ct;
 
   // exact fields at cells
@@ -154,7 +157,7 @@ int lte_analysis(const HostMesh &m, bool hc, bool perhx, double erad, double gra
     const double *prp = hc ? &m.edhx_tg[3 * l] : &m.ed_nr[3 * l];   // uhq_perp's
     double ved[3], g[3], lp[3];
     vec_sph2cart(u0 * c, 0.0, p, ved);
-    h_ed_x[l] = h0 - h_ct * (s * s);
+    h_ed_x[l] = hol ? hollgw : h0 - h_ct * (s * s);
     vec_sph2cart(0.0, -grad_ct * s * c, p, g);
     grad_x[l] = dot3(g, dir);
     q_ed_x[l] = eta_ct * s / h_ed_x[l];
@@ -167,20 +170,22 @@ int lte_analysis(const HostMesh &m, bool hc, bool perhx, double erad, double gra
   for (int k = 0; k < nt; ++k) {
     const double lat = m.tr_clat[k];
     const double s = std::sin(lat), c = std::cos(lat);
-    const double dd = h0 - h_ct * (s * s);
-    const double vt = eta_ct * c * (h0 + h_ct * (s * s)) / (erad * (dd * dd));
-    vec_sph2cart(0., vt, &m.tr_c[3 * k], &qg_x[3 * k]);
+    if (!hol) {  // TC2's grad q at the circumcentre (the others leave it zero)
+      const double dd = h0 - h_ct * (s * s);
+      const double vt = eta_ct * c * (h0 + h_ct * (s * s)) / (erad * (dd * dd));
+      vec_sph2cart(0., vt, &m.tr_c[3 * k], &qg_x[3 * k]);
+    }
     eta_x[k] = eta_ct * s;
-    h_tr_x[k] = h0 - h_ct * (s * s);
+    h_tr_x[k] = hol ? hollgw : h0 - h_ct * (s * s);
     q_tr_x[k] = eta_x[k] / h_tr_x[k];
     double v[3];
     vec_sph2cart(u0 * c, 0.0, &m.tr_c[3 * k], v);
     for (int q = 0; q < 3; ++q) vhq_tr_x[3 * k + q] = v[q] * eta_x[k];
   }
-  // grad_ed of the exact potential g (h + b) + K (b = 0), grad_edge_Cgrid
+  // grad_ed of the exact potential g (h + b) + K, grad_edge_Cgrid
   // (src/diffoperpack.f90:226-330, trsk) over erad
   std::vector<double> ghbk_x(nv), grad_gx(ne);
-  for (int i = 0; i < nv; ++i) ghbk_x[i] = grav * (f.h[i] + 0.0) + ke_x[i];
+  for (int i = 0; i < nv; ++i) ghbk_x[i] = grav * (f.h[i] + f.bt[i]) + ke_x[i];
   auto grad_cells = [&](const double *hf, int l) {
     const int v1 = m.ed_v[2 * l] - 1, v2 = m.ed_v[2 * l + 1] - 1;
     double g = hf[v2] - hf[v1];
@@ -272,7 +277,7 @@ int lte_analysis(const HostMesh &m, bool hc, bool perhx, double erad, double gra
   // (src/swm.f90:2686-2752): the kinetic part from the tendency's h_ed, or
   // with Perot's cell reconstruction (pxt16) the sum of ke_hx over the areas
   double pe = 0.0, ke = 0.0;
-  for (int i = 0; i < nv; ++i) pe = pe + m.hx_areag[i] * grav * f.h[i] * (f.h[i] * 0.5 + 0.0);
+  for (int i = 0; i < nv; ++i) pe = pe + m.hx_areag[i] * grav * f.h[i] * (f.h[i] * 0.5 + f.bt[i]);
   if (perhx) {
     for (int i = 0; i < nv; ++i) ke = ke + f.ke_hx[i] * m.hx_areag[i];
   } else {

@@ -2824,10 +2824,16 @@ int imx_swm_get_field(imx_swm *s, int32_t which, double *out) {
 }
 
 int imx_swm_lte(imx_swm *s, const imx_mesh *m, double *err, double *mim) {
+  return imx_swm_lte_hol(s, m, 0.0, err, mim);
+}
+
+int imx_swm_lte_hol(imx_swm *s, const imx_mesh *m, double hollgw, double *err, double *mim) {
   if (!s || !m || !err || !mim) return imx::set_error(IMX_EINVAL, "imx_swm_lte: NULL argument");
   if (s->fast) return imx::set_error(IMX_ENOTSUP, "imx_swm_lte: the LTE analysis runs on FP64_REF handles");
-  if (s->o.testcase != 2)
-    return imx::set_error(IMX_ENOTSUP, "imx_swm_lte: the closed forms are those of TC2 (handle testcase 2)");
+  const int tc = s->o.testcase;
+  if (tc != 2 && !(tc >= 32 && tc <= 35))
+    return imx::set_error(IMX_ENOTSUP, "imx_swm_lte: the closed forms are those of test cases 2 and 32-35 "
+                                       "(the handle's testcase)");
   if (m->nv != s->nv || m->ne != s->ne || m->nt != s->nt)
     return imx::set_error(IMX_EINVAL, "imx_swm_lte: not the handle's mesh");
   imx_mesh_desc d;
@@ -2868,10 +2874,17 @@ int imx_swm_lte(imx_swm *s, const imx_mesh *m, double *err, double *mim) {
   }
   HIPTRY(hipMemcpy(s->lte_ctr, ctr.data(), ctr.size() * sizeof(int32_t), hipMemcpyHostToDevice), "imx_swm_lte");
   HIPTRY(hipMemcpy(s->lte_cw, cw.data(), cw.size() * sizeof(double), hipMemcpyHostToDevice), "imx_swm_lte");
-  // TC2 as initialize_fields sets it, resident like the reference's globals
-  std::vector<double> h(nv), u(ne), mass(nv), mom(ne);
-  rc = s->sc_hc ? imx_ic_tc2_hc(m, s->o.erad, h.data(), u.data()) : imx_ic_tc2(m, s->o.erad, h.data(), u.data());
+  // the test case as initialize_fields sets it (state, bottom and Coriolis
+  // term), resident like the reference's globals
+  std::vector<double> h(nv), u(ne), bt(nv), mass(nv), mom(ne);
+  rc = imx_ic_testcase(m, tc, s->sc_hc ? IMX_STAG_HC : IMX_STAG_HTC, s->o.erad, hollgw, h.data(), u.data(),
+                       bt.data());
   if (rc) return rc;
+  int32_t fsphere;
+  double fcte;
+  if ((rc = imx_ic_testcase_coriolis(tc, &fsphere, &fcte)) || (rc = imx_swm_set_coriolis(s, fsphere, fcte)) ||
+      (rc = imx_swm_set_topography(s, bt.data())))
+    return rc;
   if ((rc = imx_swm_set_state(s, h.data(), u.data()))) return rc;
   if ((rc = imx_swm_tendency(s, h.data(), u.data(), mass.data(), mom.data()))) return rc;
   // the analysis fields, on the tendency's input (rhn, run) and fields
@@ -2907,8 +2920,8 @@ int imx_swm_lte(imx_swm *s, const imx_mesh *m, double *err, double *mim) {
   imx::LteFields f{h.data(), u.data(), h_ed.data(), uh.data(), divuh.data(), ke.data(), eta.data(),
                    h_tr.data(), q_tr.data(), q_hx.data(), q_ed.data(), grad.data(), vhq_tr.data(),
                    qg_tr.empty() ? nullptr : qg_tr.data(), vh_hx.data(), vhq_hx.data(), uhq.data(),
-                   lapu.empty() ? nullptr : lapu.data(), mass.data(), mom.data()};
-  return imx::lte_analysis(*m, hc, s->sc_sint == IMX_SINT_BARY, s->o.erad, s->o.grav, f, err, mim);
+                   lapu.empty() ? nullptr : lapu.data(), mass.data(), mom.data(), bt.data()};
+  return imx::lte_analysis(*m, tc, hollgw, hc, s->sc_rec == IMX_REC_PERHX, s->o.erad, s->o.grav, f, err, mim);
 }
 
 namespace imx {

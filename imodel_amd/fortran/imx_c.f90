@@ -234,6 +234,13 @@ module imx_c
       real(c_double) :: err(*), mim(*)
       integer(c_int) :: rc
     end function
+    function imx_swm_lte_hol(s, m, hollgw, err, mim) bind(C, name='imx_swm_lte_hol') result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: s, m
+      real(c_double), value :: hollgw
+      real(c_double) :: err(*), mim(*)
+      integer(c_int) :: rc
+    end function
     function imx_swm_diagnostics(s, out) bind(C, name='imx_swm_diagnostics') result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: s

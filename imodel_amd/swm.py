@@ -318,17 +318,18 @@ class Swm:
                    "imx_swm_get_field")
         return out
 
-    def lte(self):
-        """Local truncation error analysis on TC2 (swm_horiz_loc_trunc_er):
-        sets TC2 as the state, evaluates the tendency and returns
+    def lte(self, hollgw=2.0):
+        """Local truncation error analysis (swm_horiz_loc_trunc_er) of the
+        handle's test case, 2 or 32-35 (layer depth hollgw): sets the case as
+        the state, evaluates the tendency and returns
         ({row: (max relative, relative L2, max)}, {mimetic sum: value}).
-        Needs an FP64_REF handle with testcase 2 on a library mesh."""
+        Needs an FP64_REF handle on a library mesh."""
         if self.mesh._handle is None:
             raise ValueError("lte needs a library mesh (Mesh.icos, Mesh.read, Mesh.load_ref)")
         err = np.zeros(3 * len(capi.LTE_ROWS))
         mim = np.zeros(len(capi.LTE_MIM))
-        capi.check(capi.lib().imx_swm_lte(self._h, self.mesh._handle, capi.f64p(err), capi.f64p(mim)),
-                   "imx_swm_lte")
+        capi.check(capi.lib().imx_swm_lte_hol(self._h, self.mesh._handle, float(hollgw), capi.f64p(err),
+                                              capi.f64p(mim)), "imx_swm_lte_hol")
         rows = {k: tuple(err[3 * i:3 * i + 3]) for i, k in enumerate(capi.LTE_ROWS)}
         return rows, dict(zip(capi.LTE_MIM, mim))
 

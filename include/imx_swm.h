@@ -344,11 +344,14 @@ int  imx_swm_jacobian(imx_swm *s, const imx_mesh_desc *d, const double *h, const
                       double pert, int64_t cap, int32_t *row, int32_t *col, double *val,
                       int64_t *nnz, int32_t *ncolors);
 /* Local truncation error analysis, swm_horiz_loc_trunc_er
-   (src/swm.f90:382-734), for TC2 on an FP64_REF handle created with
-   testcase 2 on mesh m, with the handle's scheme options. Sets the TC2 state
-   (initialize_fields, src/swm.f90:1205-1326) as the resident state, evaluates
-   the tendency with the analysis fields (IMX_FIELD_Q_HX..), and compares 18
-   fields with TC2's closed forms.
+   (src/swm.f90:382-734), on an FP64_REF handle created with testcase 2 or
+   one of the Hollingsworth cases 32-35 on mesh m, with the handle's scheme
+   options. Sets the test case's state, bottom and Coriolis term
+   (initialize_fields, src/swm.f90:1205-1326, :1918-2068) as the resident
+   state, evaluates the tendency with the analysis fields (IMX_FIELD_Q_HX..),
+   and compares 18 fields with the case's closed forms. imx_swm_lte_hol takes
+   the layer depth hollgw of cases 32-35 (the Hollingsworth line of
+   par/swm.par); imx_swm_lte passes 0, which those cases refuse.
    err[3 r + 0..2] = error_calc's (max relative, relative L2, max) of row r,
    rows in the order the reference prints them (IMX_LTE_*). Fields the
    tendency does not compute under the handle's options are zero, as in the
@@ -383,6 +386,7 @@ int  imx_swm_jacobian(imx_swm *s, const imx_mesh_desc *d, const double *h, const
 #define IMX_MIM_CORIOLIS_REL   6  /* [5] / total energy                      */
 #define IMX_LTE_NMIM           7
 int  imx_swm_lte(imx_swm *s, const imx_mesh *m, double *err, double *mim);
+int  imx_swm_lte_hol(imx_swm *s, const imx_mesh *m, double hollgw, double *err, double *mim);
 /* nsteps x ode_rk4 on the resident state            src/swm.f90:2600-2683 */
 int  imx_swm_step(imx_swm *s, int32_t nsteps);
 /* as imx_swm_step, returns device time (HIP events on the handle's stream)  */

@@ -92,6 +92,9 @@ LTE_SCHEMES = {
     "hc": "stag=HC",
     "hcpx": "stag=HC,wrap=pxt16",
     "hcgass": "stag=HC,wrap=gass18,pv=apvm,pvspar=0.5",
+    "tc32": "tc=32",
+    "tc33": "tc=33",
+    "tc34hc": "tc=34,stag=HC",
 }
 # the mimetic lines of swm_horiz_loc_trunc_er (src/swm.f90:641-718), in the
 # order of imx_swm_lte's mim vector
@@ -118,7 +121,7 @@ def run_lte(spec, cwd, dest):
         raise RuntimeError("lte " + spec + ": mimetic lines missing\n" + r.stdout[-2000:])
     rec = imxbin.read(out)
     rec["mim"] = [mim[k] for k in MIM_LINES]
-    if spec not in ("", "stag=HC"):  # the exact fields depend on the staggering only
+    if spec not in ("", "stag=HC") and not spec.startswith("tc="):  # exact fields: staggering and case only
         rec = {k: v for k, v in rec.items() if not k.startswith("x_")}
     imxbin.write(dest, rec)
 

@@ -1,10 +1,11 @@
 """Local truncation error analysis (imx_swm_lte, swm_horiz_loc_trunc_er) on
 the GPU.
 
-Against the reference's own harness on its glevel-3 mesh: the TC2 state, the
-analysis fields of the tendency (vhq_tr, vh_hx, q_hx), every error triple of
-the table and the mimetic sums, bit for bit, for the staggerings, wrappers and
-APVM of LTE_SCHEMES (tests/golden/make_golden.py). Rows the reference
+Against the reference's own harness on its glevel-3 mesh: the TC2 (or
+Hollingsworth 32-35) state, the analysis fields of the tendency (vhq_tr,
+vh_hx, q_hx), every error triple of the table and the mimetic sums, bit for
+bit, for the staggerings, wrappers, APVM and test cases of LTE_SCHEMES
+(tests/golden/make_golden.py). Rows the reference
 leaves undefined (q_grad_tr without APVM and lapu, whose field its harness
 never computes) are not compared.
 
@@ -31,6 +32,11 @@ LTE_SCHEMES = {
     "hc": {"stag": "HC"},
     "hcpx": {"stag": "HC", "wrap": "pxt16"},
     "hcgass": {"stag": "HC", "wrap": "gass18", "pv": "apvm", "pvspar": 0.5},
+    # the Hollingsworth cases: a layer of 2 m over the balanced bottom,
+    # 33 on a non-rotating sphere
+    "tc32": {"tc": 32},
+    "tc33": {"tc": 33},
+    "tc34hc": {"tc": 34, "stag": "HC"},
 }
 
 
@@ -48,10 +54,12 @@ def ref_mesh():
 def test_lte_matches_reference(golden, ref_mesh, name, sfc):
     say("lte vs reference", name, "sfc", sfc)
     r = golden(f"ref_lte_g3_{name}.bin")
-    with Swm(ref_mesh, precision="fp64_ref", dt=900.0, testcase=2, use_sfc=sfc) as s:
-        if LTE_SCHEMES[name]:
-            s.set_scheme(**LTE_SCHEMES[name])
-        rows, mim = s.lte()
+    sc = dict(LTE_SCHEMES[name])
+    tc = sc.pop("tc", 2)
+    with Swm(ref_mesh, precision="fp64_ref", dt=900.0, testcase=tc, use_sfc=sfc) as s:
+        if sc:
+            s.set_scheme(**sc)
+        rows, mim = s.lte(hollgw=2.0)
         h, u = s.get_state()
         assert np.array_equal(h, r["h0"]) and np.array_equal(u, r["u0"])
         assert np.array_equal(s.field("vhq_tr").ravel(), r["vhq_tr"])
@@ -74,8 +82,11 @@ def test_lte_errors(ref_mesh):
         with pytest.raises(capi.ImxError, match="FP64_REF"):
             s.lte()
     with Swm(ref_mesh, precision="fp64_ref", dt=900.0, testcase=5) as s:
-        with pytest.raises(capi.ImxError, match="TC2"):
+        with pytest.raises(capi.ImxError, match="test cases 2 and 32-35"):
             s.lte()
+    with Swm(ref_mesh, precision="fp64_ref", dt=900.0, testcase=32) as s:
+        with pytest.raises(capi.ImxError, match="layer thickness"):
+            s.lte(hollgw=0.0)
     with Swm(Mesh.icos(2), precision="fp64_ref", dt=900.0) as s:
         s.mesh = ref_mesh  # a mesh of another size
         with pytest.raises(capi.ImxError, match="not the handle's mesh"):
