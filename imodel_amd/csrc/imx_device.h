@@ -109,6 +109,14 @@ struct imx_swm {
   // pered term [NE][4]; per stage A = P(uh) and B = P(uh q) at cells
   // [2][nv][4] and q_e at edges (typed T). The cell Perot weights are x_ap_wc.
   bool f_px = false;
+  // fused handles with gass18 (1 HTC, 2 HC): per triangle slot the areas of
+  // (circumcentre, edge m) and their sum [4][NT] and the area [NT]; per
+  // edge slot its two triangles [2][NE]; per cell slot the outer edge of
+  // ring triangle j [MX][nv]; per stage E and H at triangles and q at edges
+  // (typed T)
+  int f_gass = 0;
+  void *x_g_w = nullptr, *x_g_a = nullptr, *x_g_E = nullptr, *x_g_H = nullptr, *x_g_q = nullptr;
+  int32_t *x_g_et = nullptr, *x_g_out = nullptr;
   void *x_px_wq = nullptr, *x_px_n = nullptr, *x_px_ab = nullptr, *x_px_qe = nullptr;
   int32_t *x_ap_tnb = nullptr;
   void *x_ap_wc = nullptr, *x_ap_wt = nullptr;

@@ -81,6 +81,15 @@ void error_calc_vec(const HostMesh &m, Pos pos, int n, const double *v, const do
   out[2] = emax;
 }
 
+// cos(x) as a call of its own. The compilers fuse sin(x) and cos(x) of one
+// argument into one sincos call, whose cos differs from cos at some
+// arguments (glibc); the reference's closed forms at edges take the
+// separate cos, its initial states (imx_ic.cpp) the fused one.
+double cos_alone(double x) {
+  volatile double v = x;
+  return std::cos(v);
+}
+
 }  // namespace
 
 // wachspress_coords_hxv(v(i)%p, mesh, i) (src/interpack.f90:2310-2415): the
@@ -152,7 +161,7 @@ int lte_analysis(const HostMesh &m, int tc, double hollgw, bool hc, bool perhx, 
       double lon;
       cart2sph(p[0], p[1], p[2], &lon, &lat);
     }
-    const double s = std::sin(lat), c = std::cos(lat);
+    const double s = std::sin(lat), c = cos_alone(lat);
     const double *dir = hc ? &m.edhx_nr[3 * l] : &m.ed_tg[3 * l];   // u's direction
     const double *prp = hc ? &m.edhx_tg[3 * l] : &m.ed_nr[3 * l];   // uhq_perp's
     double ved[3], g[3], lp[3];

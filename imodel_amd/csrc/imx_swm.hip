@@ -713,7 +713,10 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
                                                 const T *__restrict__ u,
                                                 const T *__restrict__ h, T href, int use_pv, int flin,
                                                 const T *__restrict__ kt, T *__restrict__ cz,
-                                                T *__restrict__ q, int *__restrict__ flag) {
+                                                int gmode, const T *__restrict__ g_w,
+                                                const T *__restrict__ g_a, T *__restrict__ gE,
+                                                T *__restrict__ gH, T *__restrict__ q,
+                                                int *__restrict__ flag) {
   const int t = ent_index<BS>(remap);
   if (t >= NT) return;
   const int a = t_ab[t];
@@ -737,6 +740,27 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
     z += u[e2] * t_ceta[2 * NT + t];
     cz[t] = kt[t] * z;
   }
+  if (gmode) {
+    // gass18: h at each edge from its two vertices, h_t = sum w_m h_e / A_t
+    // with w_m the area of (circumcentre, edge m) (scalar_edtr2trcc); the
+    // rhombus PV at edges takes E = alpha eta and H = alpha h_t of both
+    // triangles, alpha = A_t with HTC and 1 with HC (scalar_tr2ed)
+    const T ho = h[o], ha = h[a], hb = h[b];
+    T hs = g_w[t] * ((T)0.5 * (ho + ha));
+    hs += g_w[NT + t] * ((T)0.5 * (ha + hb));
+    hs += g_w[2 * NT + t] * ((T)0.5 * (hb + ho));
+    const T Hs = href * g_w[3 * NT + t] + hs;  // h = href + h'
+    const T At = g_a[t];
+    if (use_pv && fabs((double)Hs / (double)At) < HTR_GUARD) flag[0] = 1;
+    if (gmode == 1) {
+      gE[t] = At * eta;
+      gH[t] = Hs;
+    } else {
+      gE[t] = eta;
+      gH[t] = Hs / At;
+    }
+    return;
+  }
   T hsum = t_kap[t] * h[o];
   hsum += t_kap[NT + t] * h[a];
   hsum += t_kap[2 * NT + t] * h[b];
@@ -747,6 +771,21 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
     else qq = eta / ht;
   }
   q[t] = qq;
+}
+
+// gass18's PV at every edge slot from the rhombus of its two triangles:
+// q = (E1 + E2) / (H1 + H2)
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void fast_gass_q(int remap, int NE, const int32_t *__restrict__ e_t,
+                                                   const T *__restrict__ gE, const T *__restrict__ gH,
+                                                   T *__restrict__ gq) {
+  const int l = ent_index<BS>(remap);
+  if (l >= NE) return;
+  const int t1 = e_t[l];
+  if (t1 < 0) return;
+  const int t2 = e_t[NE + l];
+  const T H = gH[t1] + gH[t2];
+  gq[l] = (H != (T)0) ? (gE[t1] + gE[t2]) / H : (T)0;
 }
 
 // Neumaier's compensated summation step.
@@ -908,11 +947,12 @@ __global__ __launch_bounds__(BS) void fast_cell(
     const T *__restrict__ kh, const T *__restrict__ cz, const T *__restrict__ hd,
     const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ pxw,
     const T *__restrict__ pxq, V4<T> *__restrict__ pxab, T *__restrict__ pxqe, const V4<T> *__restrict__ apv,
-    const V4<T> *__restrict__ apg, T pvsdt,
+    const V4<T> *__restrict__ apg, T pvsdt, const T *__restrict__ gq, const int32_t *__restrict__ gout,
     const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
     T inv_a, T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
     double *__restrict__ dh_out) {
+  constexpr bool kDiff = OPT == 1 || OPT == 2, kAll = OPT == 2, kGass = OPT == 3;
   const int c = ent_index<BS>(remap);
   if (c >= nv) return;
   const int A = c_bitsA[c];
@@ -940,7 +980,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
   for (int j = 0; j < MX; ++j) {
     ue[j] = us[ed[j]];
     hn[j] = hs[nb[j]];
-    qt[j] = q[tr[j]];
+    qt[j] = kGass ? (T)0 : q[tr[j]];
     dl[j] = (double)c_dl[j * nv + c];
     kw[j] = c_kw[j * nv + c];
   }
@@ -960,7 +1000,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
   }
   const double dh = -(dv + dvc) * c_iaa[c];
   T K = ke * c_ika[c];
-  if (OPT >= 2 && pxw) {  // pxt16: |v|^2 / 2 of the cell's Perot velocity (perhx)
+  if (kAll && pxw) {  // pxt16: |v|^2 / 2 of the cell's Perot velocity (perhx)
     T v0 = (T)0, v1 = (T)0, v2 = (T)0;
 #pragma unroll
     for (int j = 0; j < MX; ++j) {  // a pentagon's padded slot has a zero weight
@@ -972,7 +1012,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     K = (v0 * v0 + v1 * v1 + v2 * v2) / (T)2;
   }
   T Kd = (T)0;
-  if (OPT >= 1 && kh) {  // diffusion, normal part: grad(K div u) folds into the potential
+  if (kDiff && kh) {  // diffusion, normal part: grad(K div u) folds into the potential
     double du = 0.0;
 #pragma unroll
     for (int j = 0; j < MX; ++j) {
@@ -982,14 +1022,22 @@ __global__ __launch_bounds__(BS) void fast_cell(
     }
     Kd = kh[c] * (T)(du * c_iaa[c]);
   }
-  if (OPT >= 1 && hd) Kd -= hd[c];  // hyperdiffusion: - grad(K div lapu)
+  if (kDiff && hd) Kd -= hd[c];  // hyperdiffusion: - grad(K div lapu)
   T pot = grav * hc;
   if (gb) pot += gb[c];  // bottom topography: g (h + b)
   phi[c] = (pot + K - Kd) * inv_a;
 
   // PV at the cell's edges: edge j lies between ring triangles j-1 and j
-  T qe[MX];
-  if (OPT >= 2 && pxq) {  // pxt16: weighted by the distances to the circumcentres
+  T qe[MX], qo[MX];
+  if (kGass) {
+    // gass18: the rhombus PV of each ring edge and of the outer edge of each
+    // ring triangle (the edge that links ring edges j and j+1)
+#pragma unroll
+    for (int j = 0; j < MX; ++j) {
+      qe[j] = gq[ed[j]];
+      qo[j] = gq[gout[j * nv + c]];
+    }
+  } else if (kAll && pxq) {  // pxt16: weighted by the distances to the circumcentres
 #pragma unroll
     for (int j = 0; j < MX; ++j) {
       const int jp = (j == 0) ? (hex ? MX - 1 : MX - 2) : j - 1;
@@ -1000,7 +1048,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
 #pragma unroll
     for (int j = 1; j < MX; ++j) qe[j] = (T)0.5 * (qt[j - 1] + qt[j]);
   }
-  if (OPT >= 2 && apv) {
+  if (kAll && apv) {
     // APVM: q_e -= pvspar dt (grad q(t_{j-1}) + grad q(t_j)) . (v(c) + v(nb_j))
     const V4<T> vs = apv[c];
     V4<T> g[MX];
@@ -1015,7 +1063,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
       qe[j] -= pvsdt * d;
     }
   }
-  if (OPT >= 2 && pxw) {
+  if (kAll && pxw) {
     // pxt16's pered Coriolis term: the Perot reconstruction at this cell of
     // uh (q_e + q_l) / 2 for an edge l is (q_l A + B) / 2 with A = P(uh) and
     // B = P(uh q); fast_edge combines both cells' A and B with q_l
@@ -1038,7 +1086,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     for (int i = 0; i < MX; ++i) {
       if ((i < MX - 1 || hex) && ((A >> (BA_OWN + i)) & 1)) {
         T P = (T)0;
-        if (OPT >= 1 && tang) {  // diffusion's tangential part, as below
+        if (kDiff && tang) {  // diffusion's tangential part, as below
           const int ip = (i == 0) ? (hex ? MX - 1 : MX - 2) : i - 1;
           T fz = (T)4 * kw[i] * (T)(1.0 / (dl[i] * dl[i])) * inv_a;
           if ((A >> (BA_FZ + i)) & 1) fz = -fz;
@@ -1066,7 +1114,20 @@ __global__ __launch_bounds__(BS) void fast_cell(
     for (int j = 0; j < MX; ++j) {
       if (j != i) {
         const T w = ((j > i) ? (S[j] - S[i]) : (tot - S[i] + S[j])) - (T)0.5;
-        acc += G[j] * (qe[i] + qe[j]) * w;
+        T qs = qe[i] + qe[j];
+        if (kGass) {
+          // coriolis_ed gass (swm_operators.f90:1378-1403): the PV of the
+          // edge linking i and j (gethxedgeconnection): the outer edge of
+          // the ring triangle between neighbours, the ring edge between
+          // ring edges two apart, else the trsk average
+          const int n = hex ? MX : MX - 1;
+          const int dj = (j - i + n) % n;
+          if (dj == 1) qs = (T)2 * qo[i];
+          else if (dj == n - 1) qs = (T)2 * qo[j];
+          else if (dj == 2) qs = (T)2 * qe[(i + 1) % n];
+          else if (dj == n - 2) qs = (T)2 * qe[(j + 1) % n];
+        }
+        acc += G[j] * qs * w;
       }
     }
     T P = (T)0.5 * acc;
@@ -1076,7 +1137,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
         // diffusion, tangential part (owner only): the jump of K zeta
         // between the edge's ring triangles i-1 and i, times |e| / (|e*| a)
         // = 4 kw / (|e*|^2 a)
-        if (OPT >= 1 && tang) {  // non-null with diffusion or hyperdiffusion
+        if (kDiff && tang) {  // non-null with diffusion or hyperdiffusion
           const int ip = (i == 0) ? (hex ? MX - 1 : MX - 2) : i - 1;
           T fz = (T)4 * kw[i] * (T)(1.0 / (dl[i] * dl[i])) * inv_a;
           if ((A >> (BA_FZ + i)) & 1) fz = -fz;
@@ -1454,9 +1515,13 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->kremap[0], NT, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta, (const T *)s->x_t_kap, \
       (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, s->f_lin ? 1 : 0,                    \
       (s->f_diff || s->f_hyper) ? (const T *)s->x_kt : nullptr,                                    \
-      (T *)s->x_cz, (T *)s->q, s->flag)
+      (T *)s->x_cz, s->f_gass, (const T *)s->x_g_w, (const T *)s->x_g_a, (T *)s->x_g_E, (T *)s->x_g_H, \
+      (T *)s->q, s->flag)
   IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
 #undef IMX_TRI
+  if (s->f_gass)
+    fast_gass_q<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(s->kremap[2], NE, s->x_g_et, (const T *)s->x_g_E,
+                                                          (const T *)s->x_g_H, (T *)s->x_g_q);
   if (s->f_hyper) {
     fast_hyper_a<T, 256><<<grid_bs(nv, 256), 256, 0, st>>>(s->kremap[1], nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB,
                                                            (const T *)s->x_c_dl, s->x_c_iaa, (const T *)s->x_kh,
@@ -1485,10 +1550,13 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->f_px ? (const V4<T> *)s->x_ap_wc : nullptr, s->f_px ? (const T *)s->x_px_wq : nullptr,    \
       (V4<T> *)s->x_px_ab, (T *)s->x_px_qe,                                                        \
       s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
-      hs, us, s->href, (const T *)s->x_gb,                                                        \
+      (const T *)s->x_g_q, s->x_g_out, hs, us, s->href, (const T *)s->x_gb,                          \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
-  if (s->f_apvm || s->f_px) {
+  if (s->f_gass) {
+    constexpr int OPT = 3;
+    IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
+  } else if (s->f_apvm || s->f_px) {
     constexpr int OPT = 2;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
   } else if (s->f_diff || s->f_hyper) {
@@ -2924,6 +2992,8 @@ int imx_swm_lte_hol(imx_swm *s, const imx_mesh *m, double hollgw, double *err, d
   return imx::lte_analysis(*m, tc, hollgw, hc, s->sc_rec == IMX_REC_PERHX, s->o.erad, s->o.grav, f, err, mim);
 }
 
+}  // extern "C"
+
 namespace imx {
 // The method triple each wrapper stands for (swm_data.f90:443-517): trsk10 =
 // (trsk, trsk, trsk), pxt16 = (bary, pered, perhx), gass18 = (gass, gass,
@@ -2940,7 +3010,80 @@ static double gass_subarea(const imx_mesh_desc &d, int ko, int i) {
   const int eo = d.tr_ed[3 * ko + i] - 1;
   return sphtriarea(d.tr_c + 3 * ko, d.v_p + 3 * (d.ed_v[2 * eo] - 1), d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1));
 }
+
+// gass18 on the fused kernels (mode 1 HTC, 2 HC; 0 switches it off): the
+// sub-triangle areas in fast_tri's slot order with their sum, the triangle
+// areas, the two triangles of each edge slot and the outer edge of each
+// ring triangle of each cell, in the ownership numbering, typed T
+template <typename T>
+static int fast_gass18(imx_swm *s, const imx_mesh_desc &d, int mode) {
+  if (!mode) {
+    s->f_gass = 0;
+    return IMX_OK;
+  }
+  const int nv = s->nv, NE = s->NE, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  std::vector<double> gw((size_t)4 * NT, 0.0), ga(NT, 1.0);
+  for (int ko = 0; ko < d.nt; ++ko) {
+    const int t = tn[ko];
+    int r = -1;  // slot m of the fused kernels is reference edge (r + m) % 3
+    for (int l = 0; l < 3; ++l)
+      if (cn[d.tr_v[3 * ko + l] - 1] == t / 2) r = l;
+    if (r < 0) return set_error(IMX_EINVAL, "imx_swm_set_scheme: triangle owner not found");
+    double sw = 0.0;
+    for (int m = 0; m < 3; ++m) {
+      const double w = gass_subarea(d, ko, (r + m) % 3);
+      gw[(size_t)m * NT + t] = w;
+      sw = sw + w;
+    }
+    gw[(size_t)3 * NT + t] = sw;
+    ga[t] = d.tr_areag[ko];
+  }
+  std::vector<int32_t> et((size_t)2 * NE, -1), out((size_t)MX * nv, 0);
+  for (int eo = 0; eo < d.ne; ++eo) {
+    et[en[eo]] = tn[d.ed_sh[2 * eo] - 1];
+    et[(size_t)NE + en[eo]] = tn[d.ed_sh[2 * eo + 1] - 1];
+  }
+  for (int co = 0; co < nv; ++co) {
+    const int n = d.v_nnb[co];
+    int first = 0;
+    for (int j = 0; j < n; ++j) {
+      // ring triangle j lies between ring edges j and j+1: its third edge
+      const int ko = d.v_tr[MX * co + j] - 1;
+      const int e1 = d.v_ed[MX * co + j] - 1, e2 = d.v_ed[MX * co + (j + 1) % n] - 1;
+      int e3 = -1;
+      for (int l = 0; l < 3; ++l) {
+        const int e = d.tr_ed[3 * ko + l] - 1;
+        if (e != e1 && e != e2) e3 = e;
+      }
+      if (e3 < 0) return set_error(IMX_EINVAL, "imx_swm_set_scheme: ring triangle without an outer edge");
+      out[(size_t)j * nv + cn[co]] = en[e3];
+      if (j == 0) first = en[e3];
+    }
+    for (int j = n; j < MX; ++j) out[(size_t)j * nv + cn[co]] = first;  // a pentagon's padded slot
+  }
+  Alloc &A = s->A;
+  if (!s->x_g_w) {
+    s->x_g_w = A.get<T>((size_t)4 * NT);
+    s->x_g_a = A.get<T>(NT);
+    s->x_g_E = A.get<T>(NT);
+    s->x_g_H = A.get<T>(NT);
+    s->x_g_q = A.get<T>(NE);
+    s->x_g_et = A.get<int32_t>((size_t)2 * NE);
+    s->x_g_out = A.get<int32_t>((size_t)MX * nv);
+    if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
+  }
+  const std::vector<T> gwT = cvt<T>(gw), gaT = cvt<T>(ga);
+  HIPTRY(hipMemcpy(s->x_g_w, gwT.data(), gwT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_g_a, gaT.data(), gaT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_g_et, et.data(), et.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_g_out, out.data(), out.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+  s->f_gass = mode;
+  return IMX_OK;
+}
 }  // namespace imx
+
+extern "C" {
 
 int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc) {
   if (!s || !d || !sc) return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: NULL argument");
@@ -2987,16 +3130,17 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     // the fused kernels carry the trsk10 scheme with HTC (with APVM and K2
     // diffusion) or HC staggering, and pxt16 with HTC
     int fw = -1;
-    for (int w : {IMX_WRAP_TRSK10, IMX_WRAP_PXT16}) {
+    for (int w : {IMX_WRAP_TRSK10, IMX_WRAP_PXT16, IMX_WRAP_GASS18}) {
       int a, b, c;
       imx::wrapper_methods(w, a, b, c);
       if (a == sint && b == cor && c == rec) fw = w;
     }
-    if (fw < 0 || sc->nopv || (hc && fw == IMX_WRAP_PXT16))
+    if (fw < 0 || sc->nopv || (hc && fw == IMX_WRAP_PXT16) || (fw == IMX_WRAP_GASS18 && sc->apvm))
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take trsk10 (HTC or HC) and pxt16 (HTC) with "
-                            "APVM, diffusion and hyperdiffusion; gass18, HC pxt16, the level model and other "
-                            "method triples run on FP64_REF handles");
+                            "APVM, diffusion and hyperdiffusion, and gass18 (HTC or HC) without APVM; HC "
+                            "pxt16, gass18 with APVM, the level model and other method triples run on "
+                            "FP64_REF handles");
     imx_scheme r = *sc;
     r.wrap = fw;
     int rc;
@@ -3009,6 +3153,9 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     rc = s->single ? imx::fast_apvm<float>(s, *d, r) : imx::fast_apvm<double>(s, *d, r);
     if (rc) return rc;
     rc = s->single ? imx::fast_pxt16<float>(s, *d, r) : imx::fast_pxt16<double>(s, *d, r);
+    if (rc) return rc;
+    const int gmode = fw == IMX_WRAP_GASS18 ? (hc ? 2 : 1) : 0;
+    rc = s->single ? imx::fast_gass18<float>(s, *d, gmode) : imx::fast_gass18<double>(s, *d, gmode);
     if (rc) return rc;
     s->sc_wrap = fw;
     s->sc_sint = sint, s->sc_cor = cor, s->sc_rec = rec, s->sc_gcoef = 1.0, s->sc_nopv = false;

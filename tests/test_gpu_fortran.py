@@ -217,6 +217,7 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         p = os.path.join(td, "par", "swm.par")
         swm = open(p).read().splitlines()
         swm[10] = "gass18"
+        swm[32] = "apvm 0.5"
         open(p, "w").write("\n".join(swm) + "\n")
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)
@@ -228,15 +229,17 @@ FUSED_OPTIONS = {
     "mix": ({24: "1e5 align", 26: "1e14 const", 32: "apvm 0.5"},
             {"k2": 1e5, "diffus": "align", "k4": 1e14, "hyperdiffus": "const", "pv": "apvm", "pvspar": 0.5},
             "_pvsapvm0.50_align_diffusion_10to5.000_const_hyperdiffusion_10to14.000"),
+    "gass18": ({10: "gass18"}, {"wrap": "gass18"}, ""),
 }
 
 
 @pytest.mark.parametrize("name", sorted(FUSED_OPTIONS))
 def test_fortran_driver_fp32_options(name):
-    """Diffusion (and, in "mix", hyperdiffusion and APVM) run on the fused
-    FP32 path from par/swm.par: a day of TC2 on glevel 3 against the oracle
-    with the same options (FP32 storage tolerances), and the options' effect
-    over the day is far above the FP32 error."""
+    """Diffusion (and, in "mix", hyperdiffusion and APVM; in "gass18" the
+    gass18 wrapper) run on the fused FP32 path from par/swm.par: a day of
+    TC2 on glevel 3 against the oracle with the same options (FP32 storage
+    tolerances), and the options' effect over the day is far above the FP32
+    error."""
     print(f"[gpu-case] fortran imodel g3 tc2 fp32 {name}", flush=True)
     lines, sc, suffix = FUSED_OPTIONS[name]
     with tempfile.TemporaryDirectory() as td:
@@ -249,7 +252,7 @@ def test_fortran_driver_fp32_options(name):
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)
         assert r.returncode == 0, r.stdout[-2000:]
-        base = "swm_tc2_dt900_HTC_trsk10_areageo" + suffix
+        base = f"swm_tc2_dt900_HTC_{lines.get(10, 'trsk10')}_areageo" + suffix
         st = imxbin.read(os.path.join(td, "data", f"{base}_state_icos_eqs_nopt_3.bin"))
     m = Mesh.icos(3)
     h0, u0 = m.tc2()

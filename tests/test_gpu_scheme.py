@@ -98,8 +98,9 @@ def test_method_refusals():
                    {"wrap": "none", "cor": "dtred"}):
             with pytest.raises(capi.ImxError):
                 s.set_scheme(**kw)
-        # the trsk10 and pxt16 triples spelled out run on the fused kernels
+        # the trsk10, pxt16 and gass18 triples spelled out run on the fused kernels
         s.set_scheme(wrap="none", sint="bary", cor="pered", rec="perhx")
+        s.set_scheme(wrap="none", sint="gass", cor="gass")
         s.set_scheme(wrap="none")
 
 
@@ -150,7 +151,7 @@ def test_scheme_errors():
     m = Mesh.icos(2)
     with Swm(m, precision="fp32", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="FP64_REF"):
-            s.set_scheme(wrap="gass18")
+            s.set_scheme(wrap="gass18", pv="apvm")
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
@@ -197,7 +198,8 @@ FAST_K2 = {"const": {"k2": 1e5}, "align": {"k2": 1e5, "diffus": "align"},
            "hck4": {"stag": "HC", "pv": "apvm", "pvspar": 0.3, "k4": 1e14, "hyperdiffus": "const"},
            "pxt16": {"wrap": "pxt16"},
            "pxk2": {"wrap": "pxt16", "k2": 1e5, "diffus": "align"},
-           "pxmix": {"wrap": "pxt16", "pv": "apvm", "pvspar": 0.5, "k4": 1e14, "hyperdiffus": "const"}}
+           "pxmix": {"wrap": "pxt16", "pv": "apvm", "pvspar": 0.5, "k4": 1e14, "hyperdiffus": "const"},
+           "gass18": {"wrap": "gass18"}, "hcgass": {"stag": "HC", "wrap": "gass18"}}
 
 
 @pytest.mark.parametrize("name", sorted(FAST_K2))
@@ -223,7 +225,8 @@ def test_fast_k2_tendency_vs_oracle(name):
             assert rel_l2(dg, do) < tol_d, (prec, rel_l2(dg, do))
 
 
-@pytest.mark.parametrize("name", ["align", "apvm_k2", "hc_apvm_k2", "mix", "hck4", "pxt16", "pxmix"])
+@pytest.mark.parametrize("name", ["align", "apvm_k2", "hc_apvm_k2", "mix", "hck4", "pxt16", "pxmix", "gass18",
+                                  "hcgass"])
 def test_fast_k2_steps_vs_oracle(name):
     from tests.conftest import rel_l2
     say("fast scheme steps fp64 / fp32", name)
@@ -256,7 +259,8 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"wrap": "gass18"}, {"stag": "HC", "wrap": "pxt16"}, {"wrap": "gass18", "pv": "apvm"}):
+        for kw in ({"stag": "HC", "wrap": "pxt16"}, {"wrap": "gass18", "pv": "apvm"},
+                   {"wrap": "none", "sint": "gass", "cor": "dtred"}):
             with pytest.raises(capi.ImxError, match="fused handles take trsk10"):
                 s.set_scheme(**kw)
 
