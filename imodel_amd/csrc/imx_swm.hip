@@ -759,6 +759,11 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
       gE[t] = eta;
       gH[t] = Hs / At;
     }
+    if (q) {  // APVM's gradient takes q at the triangle, eta / h_t
+      T qq = (T)0;
+      if (use_pv && !(fabs((double)Hs / (double)At) < HTR_GUARD)) qq = eta / (Hs / At);
+      q[t] = qq;
+    }
     return;
   }
   T hsum = t_kap[t] * h[o];
@@ -773,11 +778,23 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
   q[t] = qq;
 }
 
+// Points and vectors of the fused APVM pass, one 16 B (FP32) or 32 B (FP64)
+// record each, so a gather is one vector load instead of three.
+template <typename T>
+struct alignas(4 * sizeof(T)) V4 {
+  T x, y, z, w;
+};
+
 // gass18's PV at every edge slot from the rhombus of its two triangles:
-// q = (E1 + E2) / (H1 + H2)
+// q = (E1 + E2) / (H1 + H2); with APVM (apv non-null) minus
+// pvspar dt (grad q(t1) + grad q(t2)) . (v(c1) + v(c2)) (apvm,
+// swm_operators.f90:1497-1501), from fast_apvm_pre's vectors
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void fast_gass_q(int remap, int NE, const int32_t *__restrict__ e_t,
                                                    const T *__restrict__ gE, const T *__restrict__ gH,
+                                                   const int32_t *__restrict__ e_oth,
+                                                   const V4<T> *__restrict__ apv,
+                                                   const V4<T> *__restrict__ apg, T pvsdt,
                                                    T *__restrict__ gq) {
   const int l = ent_index<BS>(remap);
   if (l >= NE) return;
@@ -785,7 +802,14 @@ __global__ __launch_bounds__(BS) void fast_gass_q(int remap, int NE, const int32
   if (t1 < 0) return;
   const int t2 = e_t[NE + l];
   const T H = gH[t1] + gH[t2];
-  gq[l] = (H != (T)0) ? (gE[t1] + gE[t2]) / H : (T)0;
+  T qe = (H != (T)0) ? (gE[t1] + gE[t2]) / H : (T)0;
+  if (apv) {
+    const int c1 = l / 3, c2 = e_oth[l] & OTH_MASK;
+    const V4<T> g1 = apg[t1], g2 = apg[t2], v1 = apv[c1], v2 = apv[c2];
+    const T d = (g1.x + g2.x) * (v1.x + v2.x) + (g1.y + g2.y) * (v1.y + v2.y) + (g1.z + g2.z) * (v1.z + v2.z);
+    qe -= pvsdt * d;
+  }
+  gq[l] = qe;
 }
 
 // Neumaier's compensated summation step.
@@ -795,13 +819,6 @@ __device__ __forceinline__ void kadd(double &s, double &c, double x) {
   else c += (x - t) + s;
   s = t;
 }
-
-// Points and vectors of the fused APVM pass, one 16 B (FP32) or 32 B (FP64)
-// record each, so a gather is one vector load instead of three.
-template <typename T>
-struct alignas(4 * sizeof(T)) V4 {
-  T x, y, z, w;
-};
 
 // APVM on the fused path, one thread per cell after fast_tri: the Perot
 // velocity at the cell (vector_edhx2hx_perot) and, for the cell's owned
@@ -1516,12 +1533,9 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, s->f_lin ? 1 : 0,                    \
       (s->f_diff || s->f_hyper) ? (const T *)s->x_kt : nullptr,                                    \
       (T *)s->x_cz, s->f_gass, (const T *)s->x_g_w, (const T *)s->x_g_a, (T *)s->x_g_E, (T *)s->x_g_H, \
-      (T *)s->q, s->flag)
+      (s->f_gass && !s->f_apvm) ? nullptr : (T *)s->q, s->flag)
   IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
 #undef IMX_TRI
-  if (s->f_gass)
-    fast_gass_q<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(s->kremap[2], NE, s->x_g_et, (const T *)s->x_g_E,
-                                                          (const T *)s->x_g_H, (T *)s->x_g_q);
   if (s->f_hyper) {
     fast_hyper_a<T, 256><<<grid_bs(nv, 256), 256, 0, st>>>(s->kremap[1], nv, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB,
                                                            (const T *)s->x_c_dl, s->x_c_iaa, (const T *)s->x_kh,
@@ -1539,6 +1553,11 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
     fast_apvm_pre<T, 256><<<grid_bs(nv, 256), 256, 0, st>>>(
         s->kremap[1], nv, NT, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const V4<T> *)s->x_ap_wc, s->x_ap_tnb,
         (const V4<T> *)s->x_ap_wt, us, (const T *)s->q, (V4<T> *)s->x_ap_v, (V4<T> *)s->x_ap_g);
+  if (s->f_gass)  // after fast_apvm_pre, whose vectors correct the rhombus PV
+    fast_gass_q<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(
+        s->kremap[2], NE, s->x_g_et, (const T *)s->x_g_E, (const T *)s->x_g_H, s->x_e_oth,
+        s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt),
+        (T *)s->x_g_q);
   if (evs) (void)hipEventRecord(evs[1], st);
 #define IMX_CELL(B)                                                                           \
   fast_cell<T, B, OPT><<<grid_bs(nv, B), B, 0, st>>>(                                        \
@@ -3127,20 +3146,19 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: APVM and gass K need ed_c, tr_c, v_p and v_tr");
   const bool hc = sc->stag == IMX_STAG_HC;
   if (s->fast) {
-    // the fused kernels carry the trsk10 scheme with HTC (with APVM and K2
-    // diffusion) or HC staggering, and pxt16 with HTC
+    // the fused kernels carry trsk10 with HTC or HC, pxt16 with HTC and
+    // gass18 with HTC or HC
     int fw = -1;
     for (int w : {IMX_WRAP_TRSK10, IMX_WRAP_PXT16, IMX_WRAP_GASS18}) {
       int a, b, c;
       imx::wrapper_methods(w, a, b, c);
       if (a == sint && b == cor && c == rec) fw = w;
     }
-    if (fw < 0 || sc->nopv || (hc && fw == IMX_WRAP_PXT16) || (fw == IMX_WRAP_GASS18 && sc->apvm))
+    if (fw < 0 || sc->nopv || (hc && fw == IMX_WRAP_PXT16))
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take trsk10 (HTC or HC) and pxt16 (HTC) with "
-                            "APVM, diffusion and hyperdiffusion, and gass18 (HTC or HC) without APVM; HC "
-                            "pxt16, gass18 with APVM, the level model and other method triples run on "
-                            "FP64_REF handles");
+                            "APVM, diffusion and hyperdiffusion, and gass18 (HTC or HC) with APVM; HC "
+                            "pxt16, the level model and other method triples run on FP64_REF handles");
     imx_scheme r = *sc;
     r.wrap = fw;
     int rc;

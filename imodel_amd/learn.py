@@ -204,14 +204,16 @@ class Trainer:
         self.gc, self.gf, self.dt_c = gc, gf, dt_c
         self.horizon, self.window, self.skip = int(horizon), int(window), int(skip)
         self.ratio = 2 ** (gf - gc)
+        self.log = log
         self.fine_mesh = Mesh.icos(gf)
         self.coarse_mesh = Mesh.icos(gc)
         self.Rh, self.Ru = restriction(self.fine_mesh, self.coarse_mesh)
+        self.log(f"trainer: meshes glevel {gc} and {gf}, restriction built")
         self.mlp = Mlp(seed=mlp_seed)
         self.coarse = Swm(self.coarse_mesh, precision="fp32", dt=dt_c, h_ref=H0_GEO)
         self.fine = Swm(self.fine_mesh, precision="fp64", dt=dt_c / self.ratio, h_ref=H0_GEO)
         self.learn = Learn(self.coarse, self.mlp, self.coarse_mesh)
-        self.log = log
+        self.log("trainer: handles created")
         self.train_rms = None
 
     def restrict(self):
@@ -284,6 +286,7 @@ class Trainer:
                 raws.append(f)
                 dus.append(du)
                 dhs.append(dh)
+        self.log(f"calibrated on {len(raws)} samples")
         R = np.concatenate(raws)
         mu, sig = R.mean(0), R.std(0)
         sig[sig < 1e-6 * max(1.0, float(np.abs(mu).max()))] = 1.0
@@ -298,9 +301,10 @@ class Trainer:
     def collect(self, seeds, nsamples, spinup=0):
         """Store nsamples (features, targets) of each seeded run on the device."""
         self.learn.reserve(len(seeds) * nsamples)
-        for sd in seeds:
+        for k, sd in enumerate(seeds):
             for _, du, dh in self.trajectory(sd, nsamples, spinup):
                 self.learn.store(du, dh)
+            self.log(f"collected run {k + 1}/{len(seeds)} (seed {sd})")
         self.log(f"stored {len(seeds) * nsamples} samples")
 
     def fit(self, nsteps, lr0=1e-3, lr1=1e-4, wd=1e-4, seed=1, chunks=10):

@@ -265,7 +265,7 @@ class Swm:
         "gass" with gasscoef | "dubos" | "melv") and the level model nopv.
         Every option runs on FP64_REF handles; fused (FP32/FP64) handles take
         trsk10 (HTC or HC) and pxt16 (HTC) with APVM, diffusion and
-        hyperdiffusion, gass18 (HTC or HC) without APVM, and refuse the
+        hyperdiffusion, gass18 (HTC or HC) with or without APVM, and refuse the
         others."""
         if rec == "dubos":  # Gassmann's K with no trsk part (swmpars, src/swm_data.f90:513-516)
             rec, gasscoef = "gass", 0.0

@@ -308,8 +308,8 @@ int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out
    FP64_REF handles take every option. Fused (FP32/FP64) handles take
    trsk10 (HTC or HC) and pxt16 (HTC), also spelled as IMX_WRAP_NONE
    triples, with APVM, diffusion and hyperdiffusion, and gass18 (HTC or HC)
-   without APVM, and return IMX_ENOTSUP for HC pxt16, gass18 with APVM,
-   other method triples and nopv. */
+   with or without APVM, and return IMX_ENOTSUP for HC pxt16, other method
+   triples and nopv. */
 int  imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc);
 int  imx_swm_set_state(imx_swm *s, const double *h, const double *u);
 int  imx_swm_get_state(imx_swm *s, double *h, double *u);

@@ -216,8 +216,8 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         _par(td, 642, 2, 900, 1, 48, "fp32")
         p = os.path.join(td, "par", "swm.par")
         swm = open(p).read().splitlines()
-        swm[10] = "gass18"
-        swm[32] = "apvm 0.5"
+        swm[8] = "HC"
+        swm[10] = "pxt16"
         open(p, "w").write("\n".join(swm) + "\n")
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)
@@ -230,6 +230,7 @@ FUSED_OPTIONS = {
             {"k2": 1e5, "diffus": "align", "k4": 1e14, "hyperdiffus": "const", "pv": "apvm", "pvspar": 0.5},
             "_pvsapvm0.50_align_diffusion_10to5.000_const_hyperdiffusion_10to14.000"),
     "gass18": ({10: "gass18"}, {"wrap": "gass18"}, ""),
+    "gassapvm": ({10: "gass18", 32: "apvm 0.5"}, {"wrap": "gass18", "pv": "apvm", "pvspar": 0.5}, "_pvsapvm0.50"),
 }
 
 
