@@ -233,7 +233,10 @@ def main():
                 "kernel": "RK4 step (one hipGraph launch: 4 x fast_tri, fast_cell, fast_edge)",
                 "algorithmic_bytes_per_launch": B, "launch_ms": ev_step,
                 "copy_probe_GBs": copy_gbs,
-                "frac_of_copy": (achieved / copy_gbs) if copy_gbs else None}
+                # the step's measured bytes (CANON-1 bytes without counter
+                # data) over the rate a streaming copy reaches on this GPU
+                "frac_of_copy": ((hbm_gbs or achieved) / copy_gbs) if copy_gbs else None,
+                "frac_of_copy_basis": "measured bytes" if hbm_gbs else "CANON-1 bytes"}
     kernels = {}
     if kern:
         kb = impl_kernel_bytes(nv, ne, nt)

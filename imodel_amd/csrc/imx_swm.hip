@@ -1364,18 +1364,23 @@ __global__ __launch_bounds__(TPB) void diag_final(int nparts, const double *__re
 // Streaming copy, used to measure the attainable HBM bandwidth and to scale
 // the memory counters of a profiler (bytes moved are known exactly). Each
 // lane issues UNR independent 16-byte loads before its stores, so every
-// wave keeps UNR x 1 KiB of reads in flight.
-template <int UNR>
-__global__ __launch_bounds__(TPB) void copy_probe(const float4 *__restrict__ a, float4 *__restrict__ b,
+// wave keeps UNR x 1 KiB of reads in flight; NT marks the accesses
+// non-temporal (streamed past the caches).
+typedef float probe4 __attribute__((ext_vector_type(4)));
+template <int UNR, bool NT>
+__global__ __launch_bounds__(TPB) void copy_probe(const probe4 *__restrict__ a, probe4 *__restrict__ b,
                                                   size_t n) {
   const size_t stride = (size_t)gridDim.x * TPB;
   size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
   for (; i + (UNR - 1) * stride < n; i += UNR * stride) {
-    float4 x[UNR];
+    probe4 x[UNR];
 #pragma unroll
-    for (int k = 0; k < UNR; ++k) x[k] = a[i + k * stride];
+    for (int k = 0; k < UNR; ++k) x[k] = NT ? __builtin_nontemporal_load(&a[i + k * stride]) : a[i + k * stride];
 #pragma unroll
-    for (int k = 0; k < UNR; ++k) b[i + k * stride] = x[k];
+    for (int k = 0; k < UNR; ++k) {
+      if (NT) __builtin_nontemporal_store(x[k], &b[i + k * stride]);
+      else b[i + k * stride] = x[k];
+    }
   }
   for (; i < n; i += stride) b[i] = a[i];
 }
@@ -3442,7 +3447,7 @@ int imx_swm_info(imx_swm *s, int64_t out[8]) {
 int imx_copy_probe(int64_t bytes, int32_t reps, double *gbs) {
   if (!gbs || bytes < 16 || reps < 1) return imx::set_error(IMX_EINVAL, "imx_copy_probe: bad argument");
   const size_t n = (size_t)bytes / 16;
-  float4 *a = nullptr, *b = nullptr;
+  probe4 *a = nullptr, *b = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = IMX_OK;
   float ms = 0.f;
@@ -3452,14 +3457,14 @@ int imx_copy_probe(int64_t bytes, int32_t reps, double *gbs) {
              hipEventCreate(&e1) != hipSuccess) {
     rc = imx::set_error(IMX_EDEVICE, "imx_copy_probe: setup failed");
   } else {
-    // the best of a few launch shapes: 4 or 8 loads in flight per lane,
-    // 2048 or 8192 blocks (8 or 32 per CU)
+    // the best of a few launch shapes: 8192 or 32768 blocks (32 or 128 per
+    // CU), cached or non-temporal accesses, 4 loads in flight per lane
     double best = 0.0;
     for (int shape = 0; shape < 4 && rc == IMX_OK; ++shape) {
-      const int blocks = (shape & 1) ? 8192 : 2048;
+      const int blocks = (shape & 1) ? 32768 : 8192;
       auto launch = [&]() {
-        if (shape < 2) copy_probe<4><<<blocks, TPB>>>(a, b, n);
-        else copy_probe<8><<<blocks, TPB>>>(a, b, n);
+        if (shape < 2) copy_probe<4, false><<<blocks, TPB>>>(a, b, n);
+        else copy_probe<4, true><<<blocks, TPB>>>(a, b, n);
       };
       launch();
       (void)hipEventRecord(e0, nullptr);

@@ -84,6 +84,37 @@ def test_limiter_bound_zero_is_plain(g3):
     assert L.last_rms() > 0.0
 
 
+def test_learned_step_follows_handle_changes(g3):
+    """The learned step's graph holds the coarse handle's step kernels; it
+    is captured again after the handle's scheme or launch shapes change.
+    With bound 0 the learned step is the handle's plain step, so after
+    set_scheme(K2) it equals the K2 step of a handle that had K2 from the
+    start, bit for bit, and after tune() still does."""
+    say("learn graph follows the handle")
+    h0, u0 = g3.spectral(8)
+    s = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
+    L = Learn(s, Mlp(seed=3), g3)
+    s.set_state(h0, u0)
+    L.step(1, bound=0.0)  # captured with the plain scheme
+    hq, uq = s.get_state()
+    s.set_scheme(k2=1e5, diffus="const")
+    s.set_state(h0, u0)
+    L.step(1, bound=0.0)
+    h1, u1 = s.get_state()
+    p = Swm(g3, precision="fp32", dt=3600.0, h_ref=H0_GEO)
+    p.set_scheme(k2=1e5, diffus="const")
+    p.set_state(h0, u0)
+    p.step(1)
+    hp, up = p.get_state()
+    assert not np.array_equal(up, uq)
+    assert np.array_equal(h1, hp) and np.array_equal(u1, up)
+    s.tune(1)
+    s.set_state(h0, u0)
+    L.step(1, bound=0.0)
+    h2, u2 = s.get_state()
+    assert np.array_equal(h2, hp) and np.array_equal(u2, up)
+
+
 def test_limiter_falls_back_to_plain_step(g3):
     """A correction above the bound (the default limiter) is dropped: the
     corrected step equals the plain step bit for bit and counts as limited;
