@@ -57,14 +57,18 @@ __device__ __forceinline__ float bf2f(short s) {
 //   g(x) = x/2 (1 + tanh(c (x + a x^3))) = x s(x),
 //   s(x) = sigma(2 c (x + a x^3)) = 1 / (1 + exp(-2 c (x + a x^3))),
 // c = sqrt(2/pi), a = 0.044715: one exponential and one reciprocal
-// (v_exp_f32, v_rcp_f32) per activation. Large |x| saturates s to 0 or 1
-// (exp overflows to inf, whose reciprocal is 0).
+// (v_exp_f32, v_rcp_f32) per activation. The exponent is written base 2,
+// exp(-2 c (x + a x^3)) = 2^(x (k1 + k2 x^2)) with k1 = -2 c log2(e) and
+// k2 = a k1, so it costs one multiply and one FMA before v_exp_f32. Large
+// |x| saturates s to 0 or 1 (2^u overflows to inf, whose reciprocal is 0).
 // g'(x) = s + x s (1 - s) 2 c (1 + 3 a x^2).
 #define GELU_C2 1.5957691216057308f  // 2 sqrt(2/pi)
 #define GELU_A 0.044715f
+#define GELU_K1 (-2.302208198144325f)    // -2 sqrt(2/pi) log2(e)
+#define GELU_K2 (-0.1029432395800235f)   // GELU_K1 * 0.044715
 __device__ __forceinline__ float gelu_s(float x) {
-  const float t = GELU_C2 * __builtin_fmaf(GELU_A * x, x * x, x);
-  return __builtin_amdgcn_rcpf(1.0f + __expf(-t));
+  const float u = x * __builtin_fmaf(GELU_K2, x * x, GELU_K1);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
 __device__ __forceinline__ float gelu_d(float x) {
