@@ -118,6 +118,12 @@ struct imx_swm {
   void *x_g_w = nullptr, *x_g_a = nullptr, *x_g_E = nullptr, *x_g_H = nullptr, *x_g_q = nullptr;
   int32_t *x_g_et = nullptr, *x_g_out = nullptr;
   void *x_px_wq = nullptr, *x_px_n = nullptr, *x_px_ab = nullptr, *x_px_qe = nullptr;
+  // pxt16 with HC: h at each Voronoi edge midpoint from the barycentric
+  // weights [3][NE] (typed T) of the cells x_px_hv [3][NE] of the triangle
+  // holding it, and the per-stage result [NE]
+  bool f_pxhc = false;
+  void *x_px_hb = nullptr, *x_px_he = nullptr;
+  int32_t *x_px_hv = nullptr;
   int32_t *x_ap_tnb = nullptr;
   void *x_ap_wc = nullptr, *x_ap_wt = nullptr;
   void *x_ap_v = nullptr, *x_ap_g = nullptr;

@@ -785,6 +785,22 @@ struct alignas(4 * sizeof(T)) V4 {
   T x, y, z, w;
 };
 
+// pxt16 with HC: h at the Voronoi edge midpoint, barycentric in the cells
+// hv of the triangle that holds it (scalar_hx2ed, swm_operators.f90:88-114),
+// one value per edge slot that both end cells read, so the mass flux still
+// cancels exactly between them
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void fast_hc_he(int remap, int NE, const int32_t *__restrict__ e_oth,
+                                                  const T *__restrict__ hb, const int32_t *__restrict__ hv,
+                                                  const T *__restrict__ hs, T href, T *__restrict__ he) {
+  const int e = ent_index<BS>(remap);
+  if (e >= NE || e_oth[e] < 0) return;
+  T s = hb[e] * hs[hv[e]];
+  s += hb[NE + e] * hs[hv[NE + e]];
+  s += hb[2 * NE + e] * hs[hv[2 * NE + e]];
+  he[e] = href + s;
+}
+
 // gass18's PV at every edge slot from the rhombus of its two triangles:
 // q = (E1 + E2) / (H1 + H2); with APVM (apv non-null) minus
 // pvspar dt (grad q(t1) + grad q(t2)) . (v(c1) + v(c2)) (apvm,
@@ -965,7 +981,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ pxw,
     const T *__restrict__ pxq, V4<T> *__restrict__ pxab, T *__restrict__ pxqe, const V4<T> *__restrict__ apv,
     const V4<T> *__restrict__ apg, T pvsdt, const T *__restrict__ gq, const int32_t *__restrict__ gout,
-    const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
+    const T *__restrict__ hce, const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
     T inv_a, T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
     double *__restrict__ dh_out) {
@@ -1001,13 +1017,19 @@ __global__ __launch_bounds__(BS) void fast_cell(
     dl[j] = (double)c_dl[j * nv + c];
     kw[j] = c_kw[j * nv + c];
   }
+  // h at the cell's edges: the average of the two cells, or (pxt16 with HC)
+  // the barycentric value of the edge's own triangle
+  double hE[MX];
+#pragma unroll
+  for (int j = 0; j < MX; ++j)
+    hE[j] = (kAll && hce) ? (double)hce[ed[j]] : href + 0.5 * ((double)hc + (double)hn[j]);
   double dv = 0.0, dvc = 0.0;
   T ke = (T)0;
   T G[MX];
 #pragma unroll
   for (int j = 0; j < MX; ++j) {
     const bool valid = (j < MX - 1) || hex;
-    const double he = href + 0.5 * ((double)hc + (double)hn[j]);
+    const double he = hE[j];
     double g = (double)ue[j] * he * dl[j];
     if ((A >> (BA_TTG + j)) & 1) g = -g;
     if (!valid) g = 0.0;
@@ -1088,7 +1110,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
 #pragma unroll
     for (int j = 0; j < MX; ++j) {
       const V4<T> w = pxw[j * nv + c];
-      const T uh = ue[j] * (T)(href + 0.5 * ((double)hc + (double)hn[j]));
+      const T uh = ue[j] * (T)hE[j];
       const T uq = uh * qe[j];
       a0 += w.x * uh;
       a1 += w.y * uh;
@@ -1558,6 +1580,9 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
     fast_apvm_pre<T, 256><<<grid_bs(nv, 256), 256, 0, st>>>(
         s->kremap[1], nv, NT, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const V4<T> *)s->x_ap_wc, s->x_ap_tnb,
         (const V4<T> *)s->x_ap_wt, us, (const T *)s->q, (V4<T> *)s->x_ap_v, (V4<T> *)s->x_ap_g);
+  if (s->f_pxhc)
+    fast_hc_he<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(s->kremap[2], NE, s->x_e_oth, (const T *)s->x_px_hb,
+                                                         s->x_px_hv, hs, href, (T *)s->x_px_he);
   if (s->f_gass)  // after fast_apvm_pre, whose vectors correct the rhombus PV
     fast_gass_q<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(
         s->kremap[2], NE, s->x_g_et, (const T *)s->x_g_E, (const T *)s->x_g_H, s->x_e_oth,
@@ -1574,7 +1599,8 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->f_px ? (const V4<T> *)s->x_ap_wc : nullptr, s->f_px ? (const T *)s->x_px_wq : nullptr,    \
       (V4<T> *)s->x_px_ab, (T *)s->x_px_qe,                                                        \
       s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
-      (const T *)s->x_g_q, s->x_g_out, hs, us, s->href, (const T *)s->x_gb,                          \
+      (const T *)s->x_g_q, s->x_g_out, s->f_pxhc ? (const T *)s->x_px_he : nullptr, hs, us, s->href,   \
+      (const T *)s->x_gb,                                                                          \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
   if (s->f_gass) {
@@ -2136,10 +2162,11 @@ static int fast_cell_weights(imx_swm *s, const imx_mesh_desc &d) {
   return IMX_OK;
 }
 
-// pxt16 on the fused kernels (HTC): h at triangles from the barycentric
+// pxt16 on the fused kernels: h at triangles from the barycentric
 // coordinates of the circumcentre (t_kap), q at edges weighted by the
-// distances to the two circumcentres, and the tables of the pered term.
-// Switching back restores the kite-area t_kap.
+// distances to the two circumcentres, and the tables of the pered term;
+// with HC also h at the Voronoi edge midpoints (fast_hc_he). Switching back
+// restores the kite-area t_kap.
 template <typename T>
 static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
   const bool px = sc.wrap == IMX_WRAP_PXT16;
@@ -2170,15 +2197,23 @@ static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) 
   const std::vector<T> tkT = cvt<T>(tk);
   HIPTRY(hipMemcpy(s->x_t_kap, tkT.data(), tkT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
   s->f_px = px;
+  s->f_pxhc = false;
   if (!px) return IMX_OK;
   int rc = fast_cell_weights<T>(s, d);
   if (rc) return rc;
   // q_e = (d2 q(sh1) + d1 q(sh2)) / (d1 + d2), d_k the arc from ed%c to the
-  // circumcentre of sh(k), as weights of ring triangles j-1 and j
+  // circumcentre of sh(k), as weights of ring triangles j-1 and j; with HC
+  // the plain average (scalar_tr2ed, swm_operators.f90:152-168)
+  const bool hc = s->sc_hc;
   std::vector<double> wq((size_t)2 * MX * nv, 0.0), pn((size_t)4 * NE, 0.0);
   for (int co = 0; co < nv; ++co) {
     const int c = cn[co], n = d.v_nnb[co];
     for (int j = 0; j < n; ++j) {
+      if (hc) {
+        wq[(size_t)j * nv + c] = 0.5;
+        wq[(size_t)(MX + j) * nv + c] = 0.5;
+        continue;
+      }
       const int eo = d.v_ed[MX * co + j] - 1;
       const int tp = d.v_tr[MX * co + (j + n - 1) % n] - 1;
       const int s1 = d.ed_sh[2 * eo] - 1;
@@ -2189,13 +2224,40 @@ static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) 
       wq[(size_t)(MX + j) * nv + c] = (tp == s1) ? w2 : w1;
     }
   }
-  // pered: minus the ed%nr component of the tangent-plane average at ed%c
+  // pered: HTC, minus the ed%nr component of the tangent-plane average at
+  // ed%c; HC, the edhx%tg component of the average at edhx%c
+  // (coriolis_ed, swm_operators.f90:1330-1376)
+  std::vector<double> hb(hc ? (size_t)3 * NE : 0, 0.0);
+  std::vector<int32_t> hv(hc ? (size_t)3 * NE : 0, 0);
   for (int eo = 0; eo < d.ne; ++eo) {
-    double nr[3];
-    cross3(d.v_p + 3 * (d.ed_v[2 * eo] - 1), d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1), nr);
-    const double nn = norm3(nr);
-    for (int q = 0; q < 3; ++q) nr[q] = -nr[q] / nn;
-    put_projected(&pn[(size_t)4 * en[eo]], nr, d.ed_c + 3 * eo);
+    const int e = en[eo];
+    if (hc) {
+      const double *t1 = d.tr_c + 3 * (d.ed_sh[2 * eo] - 1), *t2 = d.tr_c + 3 * (d.ed_sh[2 * eo + 1] - 1);
+      const double *hxc = d.edhx_c + 3 * eo;
+      double hnr[3], htg[3];
+      cross3(t1, t2, hnr);
+      double nn = norm3(hnr);
+      for (int q = 0; q < 3; ++q) hnr[q] = hnr[q] / nn;
+      cross3(hnr, hxc, htg);
+      nn = norm3(htg);
+      for (int q = 0; q < 3; ++q) htg[q] = htg[q] / nn;
+      put_projected(&pn[(size_t)4 * e], htg, hxc);
+      // h at the dual edge midpoint: barycentric in the triangle that holds
+      // it (initialize_gridprop, swm.f90:1007-1018)
+      const int kt = insidetrmesh(d, hxc, d.ed_sh[2 * eo] - 1) ? d.ed_sh[2 * eo] - 1 : d.ed_sh[2 * eo + 1] - 1;
+      double b[3];
+      bary_at(d, hxc, kt, b);
+      for (int i = 0; i < 3; ++i) {
+        hb[(size_t)i * NE + e] = b[i];
+        hv[(size_t)i * NE + e] = cn[d.tr_v[3 * kt + i] - 1];
+      }
+    } else {
+      double nr[3];
+      cross3(d.v_p + 3 * (d.ed_v[2 * eo] - 1), d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1), nr);
+      const double nn = norm3(nr);
+      for (int q = 0; q < 3; ++q) nr[q] = -nr[q] / nn;
+      put_projected(&pn[(size_t)4 * e], nr, d.ed_c + 3 * eo);
+    }
   }
   Alloc &A = s->A;
   if (!s->x_px_wq) {
@@ -2205,9 +2267,21 @@ static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) 
     s->x_px_qe = A.get<T>(NE);
     if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
   }
+  if (hc && !s->x_px_hb) {
+    s->x_px_hb = A.get<T>((size_t)3 * NE);
+    s->x_px_hv = A.get<int32_t>((size_t)3 * NE);
+    s->x_px_he = A.get<T>(NE);
+    if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
+  }
   const std::vector<T> wqT = cvt<T>(wq), pnT = cvt<T>(pn);
   HIPTRY(hipMemcpy(s->x_px_wq, wqT.data(), wqT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
   HIPTRY(hipMemcpy(s->x_px_n, pnT.data(), pnT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  if (hc) {
+    const std::vector<T> hbT = cvt<T>(hb);
+    HIPTRY(hipMemcpy(s->x_px_hb, hbT.data(), hbT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+    HIPTRY(hipMemcpy(s->x_px_hv, hv.data(), hv.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+  }
+  s->f_pxhc = hc;
   return IMX_OK;
 }
 
@@ -3151,43 +3225,57 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: APVM and gass K need ed_c, tr_c, v_p and v_tr");
   const bool hc = sc->stag == IMX_STAG_HC;
   if (s->fast) {
-    // the fused kernels carry trsk10 with HTC or HC, pxt16 with HTC and
-    // gass18 with HTC or HC
+    // the fused kernels carry trsk10, pxt16 and gass18 with HTC or HC
     int fw = -1;
     for (int w : {IMX_WRAP_TRSK10, IMX_WRAP_PXT16, IMX_WRAP_GASS18}) {
       int a, b, c;
       imx::wrapper_methods(w, a, b, c);
       if (a == sint && b == cor && c == rec) fw = w;
     }
-    if (fw < 0 || sc->nopv || (hc && fw == IMX_WRAP_PXT16))
+    if (fw < 0 || sc->nopv)
       return imx::set_error(IMX_ENOTSUP,
-                            "imx_swm_set_scheme: fused handles take trsk10 (HTC or HC) and pxt16 (HTC) with "
-                            "APVM, diffusion and hyperdiffusion, and gass18 (HTC or HC) with APVM; HC "
-                            "pxt16, the level model and other method triples run on FP64_REF handles");
+                            "imx_swm_set_scheme: fused handles take trsk10 and pxt16 (HTC or HC) with APVM, "
+                            "diffusion and hyperdiffusion, and gass18 (HTC or HC) with or without APVM; the "
+                            "level model and other method triples run on FP64_REF handles");
+    // the tables of each option, in turn; every step rewrites the tables
+    // and flags of its option whether it is on or off
+    auto apply = [&](const imx_scheme &q, int qw, bool qhc) -> int {
+      int rc;
+      if (qhc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, false, false)))) return rc;
+      rc = s->single ? imx::fast_stagger<float>(s, *d, qhc) : imx::fast_stagger<double>(s, *d, qhc);
+      if (rc) return rc;
+      s->sc_hc = qhc;
+      rc = s->single ? imx::fast_diffusion<float>(s, *d, q) : imx::fast_diffusion<double>(s, *d, q);
+      if (rc) return rc;
+      rc = s->single ? imx::fast_apvm<float>(s, *d, q) : imx::fast_apvm<double>(s, *d, q);
+      if (rc) return rc;
+      rc = s->single ? imx::fast_pxt16<float>(s, *d, q) : imx::fast_pxt16<double>(s, *d, q);
+      if (rc) return rc;
+      const int gmode = qw == IMX_WRAP_GASS18 ? (qhc ? 2 : 1) : 0;
+      return s->single ? imx::fast_gass18<float>(s, *d, gmode) : imx::fast_gass18<double>(s, *d, gmode);
+    };
     imx_scheme r = *sc;
     r.wrap = fw;
-    int rc;
-    if (hc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, false, false)))) return rc;
-    rc = s->single ? imx::fast_stagger<float>(s, *d, hc) : imx::fast_stagger<double>(s, *d, hc);
-    if (rc) return rc;
-    s->sc_hc = hc;
-    rc = s->single ? imx::fast_diffusion<float>(s, *d, r) : imx::fast_diffusion<double>(s, *d, r);
-    if (rc) return rc;
-    rc = s->single ? imx::fast_apvm<float>(s, *d, r) : imx::fast_apvm<double>(s, *d, r);
-    if (rc) return rc;
-    rc = s->single ? imx::fast_pxt16<float>(s, *d, r) : imx::fast_pxt16<double>(s, *d, r);
-    if (rc) return rc;
-    const int gmode = fw == IMX_WRAP_GASS18 ? (hc ? 2 : 1) : 0;
-    rc = s->single ? imx::fast_gass18<float>(s, *d, gmode) : imx::fast_gass18<double>(s, *d, gmode);
-    if (rc) return rc;
+    int rc = apply(r, fw, hc);
+    std::string why;
+    if (rc) {
+      // a failure part way (device memory) would leave a mix of the old and
+      // new options: put the handle back on the default scheme instead
+      why = imx_last_error();
+      const imx_scheme def{};
+      (void)apply(def, IMX_WRAP_TRSK10, false);
+      r = def, fw = IMX_WRAP_TRSK10;
+      imx::wrapper_methods(fw, sint, cor, rec);
+    }
     s->sc_wrap = fw;
     s->sc_sint = sint, s->sc_cor = cor, s->sc_rec = rec, s->sc_gcoef = 1.0, s->sc_nopv = false;
     // the option flags size the Jacobian's stencil superset
-    s->sc_apvm = sc->apvm != 0;
-    s->sc_diff = sc->k2 != 0.0;
-    s->sc_hyper = sc->k4 != 0.0;
+    s->sc_apvm = r.apvm != 0;
+    s->sc_diff = r.k2 != 0.0;
+    s->sc_hyper = r.k4 != 0.0;
     s->sc = r;
     imx::drop_graph(s);
+    if (rc) return imx::set_error(rc, why + " (the handle is back on the default scheme)");
     return IMX_OK;
   }
   const int nv = s->nv, NE = s->NE, NT = s->NT;

@@ -474,11 +474,10 @@ contains
       if (fs /= 0) call imx_check(imx_swm_set_coriolis(hnd, fs, fc), 'imx_swm_set_coriolis')
     end block
     if (scheme_options()) then
-      ! the fused fp32/fp64 kernels carry trsk10 and (with HTC) pxt16, with
-      ! APVM, diffusion and hyperdiffusion, and gass18 with or without APVM
-      if (prec /= IMX_PREC_FP64_REF .and. (wrap_code() == 3 .or. inopv > 0 .or. &
-          (wrap_code() == 1 .and. trim(stag) == 'HC'))) then
-        write(*, '(a)') " ERROR: HC with pxt16, wrapper none and noPV run on the fp64_ref path"
+      ! the fused fp32/fp64 kernels carry trsk10 and pxt16 with APVM,
+      ! diffusion and hyperdiffusion, and gass18 with or without APVM
+      if (prec /= IMX_PREC_FP64_REF .and. (wrap_code() == 3 .or. inopv > 0)) then
+        write(*, '(a)') " ERROR: wrapper none and noPV run on the fp64_ref path"
         write(*, '(a)') "        (par/swm_gpu.par)"
         stop 1
       end if

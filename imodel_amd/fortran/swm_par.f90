@@ -6,9 +6,9 @@
 !                 geo-area subset this build runs: test cases 0, 1, 2, 5,
 !                 6, 11, 12, 21, 22, 23, 32-35, 40-42, 51, 52, 53 and the
 !                 seeded state 100;
-!                 trsk10 with HTC or HC and pxt16 with HTC, APVM, diffusion and
+!                 trsk10 and pxt16 with HTC or HC, APVM, diffusion and
 !                 hyperdiffusion, and gass18 with or without APVM, on every
-!                 path; HC pxt16, wrapper
+!                 path; wrapper
 !                 none (sinterpol trsk/bary/gass, Coriolis trsk/pered/gass/
 !                 dtred, reconmtd trsk/perhx/gass/dubos/melv, gradmtd trsk)
 !                 and the level model (noPV) on the fp64_ref path

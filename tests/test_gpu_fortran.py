@@ -216,8 +216,8 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         _par(td, 642, 2, 900, 1, 48, "fp32")
         p = os.path.join(td, "par", "swm.par")
         swm = open(p).read().splitlines()
-        swm[8] = "HC"
-        swm[10] = "pxt16"
+        swm[10] = "none"
+        swm[12] = "melv 1.0"
         open(p, "w").write("\n".join(swm) + "\n")
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)

@@ -151,7 +151,7 @@ def test_scheme_errors():
     m = Mesh.icos(2)
     with Swm(m, precision="fp32", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="FP64_REF"):
-            s.set_scheme(stag="HC", wrap="pxt16")
+            s.set_scheme(wrap="none", rec="melv")
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
@@ -201,7 +201,9 @@ FAST_K2 = {"const": {"k2": 1e5}, "align": {"k2": 1e5, "diffus": "align"},
            "pxmix": {"wrap": "pxt16", "pv": "apvm", "pvspar": 0.5, "k4": 1e14, "hyperdiffus": "const"},
            "gass18": {"wrap": "gass18"}, "hcgass": {"stag": "HC", "wrap": "gass18"},
            "gassapvm": {"wrap": "gass18", "pv": "apvm", "pvspar": 0.5},
-           "hcgassapvm": {"stag": "HC", "wrap": "gass18", "pv": "apvm", "pvspar": 0.4}}
+           "hcgassapvm": {"stag": "HC", "wrap": "gass18", "pv": "apvm", "pvspar": 0.4},
+           "hcpx": {"stag": "HC", "wrap": "pxt16"},
+           "hcpxmix": {"stag": "HC", "wrap": "pxt16", "pv": "apvm", "pvspar": 0.5, "k2": 1e5, "diffus": "align"}}
 
 
 @pytest.mark.parametrize("name", sorted(FAST_K2))
@@ -228,7 +230,7 @@ def test_fast_k2_tendency_vs_oracle(name):
 
 
 @pytest.mark.parametrize("name", ["align", "apvm_k2", "hc_apvm_k2", "mix", "hck4", "pxt16", "pxmix", "gass18",
-                                  "hcgass", "gassapvm", "hcgassapvm"])
+                                  "hcgass", "gassapvm", "hcgassapvm", "hcpx", "hcpxmix"])
 def test_fast_k2_steps_vs_oracle(name):
     from tests.conftest import rel_l2
     say("fast scheme steps fp64 / fp32", name)
@@ -261,7 +263,7 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"stag": "HC", "wrap": "pxt16"}, {"wrap": "none", "rec": "dubos"},
+        for kw in ({"wrap": "none", "rec": "melv"}, {"wrap": "none", "rec": "dubos"},
                    {"wrap": "none", "sint": "gass", "cor": "dtred"}):
             with pytest.raises(capi.ImxError, match="fused handles take trsk10"):
                 s.set_scheme(**kw)
