@@ -231,16 +231,17 @@ FUSED_OPTIONS = {
             "_pvsapvm0.50_align_diffusion_10to5.000_const_hyperdiffusion_10to14.000"),
     "gass18": ({10: "gass18"}, {"wrap": "gass18"}, ""),
     "gassapvm": ({10: "gass18", 32: "apvm 0.5"}, {"wrap": "gass18", "pv": "apvm", "pvspar": 0.5}, "_pvsapvm0.50"),
+    "hcpx": ({8: "HC", 10: "pxt16"}, {"stag": "HC", "wrap": "pxt16"}, ""),
 }
 
 
 @pytest.mark.parametrize("name", sorted(FUSED_OPTIONS))
 def test_fortran_driver_fp32_options(name):
-    """Diffusion (and, in "mix", hyperdiffusion and APVM; in "gass18" the
-    gass18 wrapper) run on the fused FP32 path from par/swm.par: a day of
-    TC2 on glevel 3 against the oracle with the same options (FP32 storage
-    tolerances), and the options' effect over the day is far above the FP32
-    error."""
+    """Diffusion (and, in "mix", hyperdiffusion and APVM; the gass18 wrapper
+    with and without APVM; pxt16 with HC) run on the fused FP32 path from
+    par/swm.par: a day of TC2 on glevel 3 against the oracle with the same
+    options (FP32 storage tolerances), and the options' effect over the day
+    is far above the FP32 error."""
     print(f"[gpu-case] fortran imodel g3 tc2 fp32 {name}", flush=True)
     lines, sc, suffix = FUSED_OPTIONS[name]
     with tempfile.TemporaryDirectory() as td:
@@ -253,12 +254,14 @@ def test_fortran_driver_fp32_options(name):
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)
         assert r.returncode == 0, r.stdout[-2000:]
-        base = f"swm_tc2_dt900_HTC_{lines.get(10, 'trsk10')}_areageo" + suffix
+        stag = lines.get(8, "HTC")
+        base = f"swm_tc2_dt900_{stag}_{lines.get(10, 'trsk10')}_areageo" + suffix
         st = imxbin.read(os.path.join(td, "data", f"{base}_state_icos_eqs_nopt_3.bin"))
     m = Mesh.icos(3)
-    h0, u0 = m.tc2()
+    h0, u0 = m.tc2(stag=stag)
     h, u = Oracle(m.arrays, scheme=dict(sc, dt=900.0)).step(h0, u0, 900.0, 96)
-    hp, up = Oracle(m.arrays).step(h0, u0, 900.0, 96)
+    # the same staggering without the options
+    hp, up = Oracle(m.arrays, scheme={"stag": stag, "dt": 900.0} if stag == "HC" else None).step(h0, u0, 900.0, 96)
     A = m.arrays["hx_areag"]
     eh = np.sqrt(np.sum(A * (st["h"] - h) ** 2) / np.sum(A * h ** 2))
     eu = np.linalg.norm(st["u"] - u) / np.linalg.norm(u)
