@@ -185,6 +185,51 @@ def test_zero_depth_guard_reports_error():
                 s.tendency(np.zeros_like(h), u)
 
 
+@pytest.mark.parametrize("glevel", [0, 1])
+def test_smallest_meshes_vs_oracle(glevel):
+    """The icosahedron itself (12 pentagons, every cell with a padded slot)
+    and glevel 1 (42 cells), both far smaller than one block: FP64_REF equals
+    the oracle bit for bit over 5 steps, the fused paths agree to their
+    tolerances."""
+    say("smallest meshes", glevel)
+    m, h, u = _case(glevel, 31)
+    o = Oracle(m.arrays)
+    mo, fo = o.tendency(h, u)
+    ho, uo = o.step(h, u, 900.0, 5)
+    A = m.arrays["hx_areag"]
+    for prec, tol_f, tol_h, tol_u in (("fp64_ref", 0.0, 0.0, 0.0), ("fp64", 1e-12, 1e-12, 1e-11),
+                                      ("fp32", 5e-5, 1e-7, 1e-5)):
+        with Swm(m, precision=prec, dt=900.0) as s:
+            mg, fg = s.tendency(h, u)
+            s.set_state(h, u)
+            s.step(5)
+            hg, ug = s.get_state()
+        if prec == "fp64_ref":
+            assert np.array_equal(mg, mo) and np.array_equal(fg, fo)
+            assert np.array_equal(hg, ho) and np.array_equal(ug, uo)
+        else:
+            assert rel_l2(mg, mo) < tol_f and rel_l2(fg, fo) < tol_f, (prec, rel_l2(mg, mo), rel_l2(fg, fo))
+            assert rel_l2(hg, ho, A) < tol_h and rel_l2(ug, uo) < tol_u, (prec, rel_l2(hg, ho, A), rel_l2(ug, uo))
+
+
+def test_nonfinite_state_sets_badflag():
+    """A NaN in the state runs through every kernel without a fault and
+    raises the diagnostics' NaN/Inf flag, on the reference-order and the
+    fused paths (the driver's blow-up stop reads it)."""
+    say("nonfinite state")
+    m, h, u = _case(3, 5)
+    u = u.copy()
+    u[17] = np.nan
+    for prec in ("fp64_ref", "fp64", "fp32"):
+        with Swm(m, precision=prec, dt=900.0) as s:
+            s.set_state(h, u)
+            assert s.diagnostics()["badflag"] == 1
+            s.step(2)
+            assert s.diagnostics()["badflag"] == 1
+            hg, ug = s.get_state()
+        assert not np.all(np.isfinite(hg))
+
+
 def test_mass_conservation_fp32_day_g6():
     """One simulated day at glevel 6 in FP32: relative mass drift stays at
     FP32-storage level, energy drift small, no NaN."""
