@@ -122,6 +122,7 @@ struct imx_swm {
   // weights [3][NE] (typed T) of the cells x_px_hv [3][NE] of the triangle
   // holding it, and the per-stage result [NE]
   bool f_pxhc = false;
+  bool f_nopv = false;  // fused level model (trsk10)
   void *x_px_hb = nullptr, *x_px_he = nullptr;
   int32_t *x_px_hv = nullptr;
   int32_t *x_ap_tnb = nullptr;

@@ -711,7 +711,7 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
                                                 const T *__restrict__ t_kap,
                                                 const T *__restrict__ t_f,
                                                 const T *__restrict__ u,
-                                                const T *__restrict__ h, T href, int use_pv, int flin,
+                                                const T *__restrict__ h, T href, int use_pv, int flin, int nopv,
                                                 const T *__restrict__ kt, T *__restrict__ cz,
                                                 int gmode, const T *__restrict__ g_w,
                                                 const T *__restrict__ g_a, T *__restrict__ gE,
@@ -775,7 +775,9 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
     if (fabs((double)ht) < HTR_GUARD) flag[0] = 1;
     else qq = eta / ht;
   }
-  q[t] = qq;
+  // the level model's Coriolis term reads the absolute vorticity, the PV
+  // still guards the depth (scalar_elem_divide runs either way)
+  q[t] = nopv ? eta : qq;
 }
 
 // Points and vectors of the fused APVM pass, one 16 B (FP32) or 32 B (FP64)
@@ -985,7 +987,10 @@ __global__ __launch_bounds__(BS) void fast_cell(
     T inv_a, T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
     double *__restrict__ dh_out) {
-  constexpr bool kDiff = OPT == 1 || OPT == 2, kAll = OPT == 2, kGass = OPT == 3;
+  // OPT 0 plain; 1 diffusion; 2 every option; 3 gass18; 4 the level model
+  // (noPV: the TRiSK sum takes u and eta at edges in place of uh and q),
+  // with diffusion
+  constexpr bool kDiff = OPT == 1 || OPT == 2 || OPT == 4, kAll = OPT == 2, kGass = OPT == 3, kNopv = OPT == 4;
   const int c = ent_index<BS>(remap);
   if (c >= nv) return;
   const int A = c_bitsA[c];
@@ -1034,7 +1039,13 @@ __global__ __launch_bounds__(BS) void fast_cell(
     if ((A >> (BA_TTG + j)) & 1) g = -g;
     if (!valid) g = 0.0;
     kadd(dv, dvc, g);
-    G[j] = (T)g;
+    if (kNopv) {  // coriolis_ed noPV (swm_operators.f90:1297-1300): u, not uh
+      T gu = ue[j] * (T)dl[j];
+      if ((A >> (BA_TTG + j)) & 1) gu = -gu;
+      G[j] = valid ? gu : (T)0;
+    } else {
+      G[j] = (T)g;
+    }
     ke += valid ? kw[j] * ue[j] * ue[j] : (T)0;
   }
   const double dh = -(dv + dvc) * c_iaa[c];
@@ -1557,7 +1568,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
 #define IMX_TRI(B)                                                                          \
   fast_tri<T, B><<<grid_bs(NT, B), B, 0, st>>>(                                            \
       s->kremap[0], NT, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta, (const T *)s->x_t_kap, \
-      (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, s->f_lin ? 1 : 0,                    \
+      (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, s->f_lin ? 1 : 0, s->f_nopv ? 1 : 0,  \
       (s->f_diff || s->f_hyper) ? (const T *)s->x_kt : nullptr,                                    \
       (T *)s->x_cz, s->f_gass, (const T *)s->x_g_w, (const T *)s->x_g_a, (T *)s->x_g_E, (T *)s->x_g_H, \
       (s->f_gass && !s->f_apvm) ? nullptr : (T *)s->q, s->flag)
@@ -1605,6 +1616,9 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
   if (s->f_gass) {
     constexpr int OPT = 3;
+    IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
+  } else if (s->f_nopv) {
+    constexpr int OPT = 4;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
   } else if (s->f_apvm || s->f_px) {
     constexpr int OPT = 2;
@@ -3232,11 +3246,12 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       imx::wrapper_methods(w, a, b, c);
       if (a == sint && b == cor && c == rec) fw = w;
     }
-    if (fw < 0 || sc->nopv)
+    if (fw < 0 || (sc->nopv && (fw != IMX_WRAP_TRSK10 || sc->apvm || sc->k4 > 0.0)))
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take trsk10 and pxt16 (HTC or HC) with APVM, "
-                            "diffusion and hyperdiffusion, and gass18 (HTC or HC) with or without APVM; the "
-                            "level model and other method triples run on FP64_REF handles");
+                            "diffusion and hyperdiffusion, gass18 (HTC or HC) with or without APVM, and the "
+                            "level model with trsk10 (HTC or HC, diffusion allowed); other method triples and "
+                            "the level model with other options run on FP64_REF handles");
     // the tables of each option, in turn; every step rewrites the tables
     // and flags of its option whether it is on or off
     auto apply = [&](const imx_scheme &q, int qw, bool qhc) -> int {
@@ -3257,6 +3272,7 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     imx_scheme r = *sc;
     r.wrap = fw;
     int rc = apply(r, fw, hc);
+    s->f_nopv = rc == IMX_OK && r.nopv != 0;
     std::string why;
     if (rc) {
       // a failure part way (device memory) would leave a mix of the old and
@@ -3268,7 +3284,7 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       imx::wrapper_methods(fw, sint, cor, rec);
     }
     s->sc_wrap = fw;
-    s->sc_sint = sint, s->sc_cor = cor, s->sc_rec = rec, s->sc_gcoef = 1.0, s->sc_nopv = false;
+    s->sc_sint = sint, s->sc_cor = cor, s->sc_rec = rec, s->sc_gcoef = 1.0, s->sc_nopv = s->f_nopv;
     // the option flags size the Jacobian's stencil superset
     s->sc_apvm = r.apvm != 0;
     s->sc_diff = r.k2 != 0.0;

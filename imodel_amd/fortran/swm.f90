@@ -475,9 +475,11 @@ contains
     end block
     if (scheme_options()) then
       ! the fused fp32/fp64 kernels carry trsk10 and pxt16 with APVM,
-      ! diffusion and hyperdiffusion, and gass18 with or without APVM
-      if (prec /= IMX_PREC_FP64_REF .and. (wrap_code() == 3 .or. inopv > 0)) then
-        write(*, '(a)') " ERROR: wrapper none and noPV run on the fp64_ref path"
+      ! diffusion and hyperdiffusion, gass18 with or without APVM, and the
+      ! level model with trsk10 and at most diffusion
+      if (prec /= IMX_PREC_FP64_REF .and. (wrap_code() == 3 .or. (inopv > 0 .and. &
+          (wrap_code() /= 0 .or. trim(pv_stab) == 'apvm' .or. k4_max > 0._r8)))) then
+        write(*, '(a)') " ERROR: wrapper none, and noPV with other than trsk10 and diffusion, run on the fp64_ref path"
         write(*, '(a)') "        (par/swm_gpu.par)"
         stop 1
       end if

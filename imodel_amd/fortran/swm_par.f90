@@ -7,11 +7,12 @@
 !                 6, 11, 12, 21, 22, 23, 32-35, 40-42, 51, 52, 53 and the
 !                 seeded state 100;
 !                 trsk10 and pxt16 with HTC or HC, APVM, diffusion and
-!                 hyperdiffusion, and gass18 with or without APVM, on every
-!                 path; wrapper
-!                 none (sinterpol trsk/bary/gass, Coriolis trsk/pered/gass/
-!                 dtred, reconmtd trsk/perhx/gass/dubos/melv, gradmtd trsk)
-!                 and the level model (noPV) on the fp64_ref path
+!                 hyperdiffusion, gass18 with or without APVM, and the level
+!                 model (noPV) with trsk10 and diffusion, on every path;
+!                 wrapper none (sinterpol trsk/bary/gass, Coriolis trsk/
+!                 pered/gass/dtred, reconmtd trsk/perhx/gass/dubos/melv,
+!                 gradmtd trsk) and the level model with other options on
+!                 the fp64_ref path
 !   par/swm_gpu.par (new, optional): precision, seed, SFC and graph switches
 !=======================================================================
 module swm_par

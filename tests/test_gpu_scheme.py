@@ -94,13 +94,15 @@ def test_method_refusals():
                 s.set_scheme(**kw)
         s.set_scheme(wrap="none", cor="gass", nopv=True)
     with Swm(m, precision="fp32", dt=900.0) as s:
-        for kw in ({"wrap": "none", "rec": "melv"}, {"wrap": "none", "nopv": True},
-                   {"wrap": "none", "cor": "dtred"}):
+        for kw in ({"wrap": "none", "rec": "melv"}, {"wrap": "none", "nopv": True, "pv": "apvm"},
+                   {"wrap": "gass18", "nopv": True}, {"wrap": "none", "cor": "dtred"}):
             with pytest.raises(capi.ImxError):
                 s.set_scheme(**kw)
-        # the trsk10, pxt16 and gass18 triples spelled out run on the fused kernels
+        # the trsk10, pxt16 and gass18 triples spelled out run on the fused
+        # kernels, and so does the level model with trsk10
         s.set_scheme(wrap="none", sint="bary", cor="pered", rec="perhx")
         s.set_scheme(wrap="none", sint="gass", cor="gass")
+        s.set_scheme(wrap="none", nopv=True)
         s.set_scheme(wrap="none")
 
 
@@ -255,6 +257,53 @@ def test_fast_k2_steps_vs_oracle(name):
         with Swm(m, precision=prec, dt=900.0) as p:
             p.set_state(h, u)
             p.step(10)
+            hq, uq = p.get_state()
+        assert np.array_equal(hp, hq) and np.array_equal(up, uq)
+
+
+NOPV = {"nopv": {"nopv": True}, "hcnopv": {"stag": "HC", "nopv": True},
+        "nopvk2": {"nopv": True, "k2": 1e5, "diffus": "align"}}
+
+
+@pytest.mark.parametrize("name", sorted(NOPV))
+def test_fast_level_model_vs_ref(name):
+    """The level model (noPV) with trsk10 on the fused kernels against the
+    FP64_REF handle, which equals the reference's own noPV runs bit for bit:
+    the tendency (FP64 < 1e-12, FP32 < 5e-5) and 10 steps; the noPV term
+    differs from the PV form far above those tolerances; switching back
+    restores the plain fused step bit for bit."""
+    from tests.conftest import rel_l2
+    say("fast level model", name)
+    m = Mesh.icos(4)
+    h, u = m.spectral(24)
+    sc = NOPV[name]
+    base = {k: v for k, v in sc.items() if k != "nopv"}
+    with Swm(m, precision="fp64_ref", dt=900.0) as r:
+        r.set_scheme(**sc)
+        mo, fo = r.tendency(h, u)
+        r.set_state(h, u)
+        r.step(10)
+        ho, uo = r.get_state()
+        r.set_scheme(**base)
+        _, fp = r.tendency(h, u)
+    assert rel_l2(fp, fo) > 1e-4
+    A = m.arrays["hx_areag"]
+    for prec, tol, tol_h, tol_u in (("fp64", 1e-12, 1e-12, 1e-11), ("fp32", 5e-5, 1e-7, 1e-5)):
+        with Swm(m, precision=prec, dt=900.0) as s:
+            s.set_scheme(**sc)
+            mg, fg = s.tendency(h, u)
+            s.set_state(h, u)
+            s.step(10)
+            hg, ug = s.get_state()
+            s.set_scheme()
+            s.set_state(h, u)
+            s.step(3)
+            hp, up = s.get_state()
+        assert rel_l2(mg, mo) < tol and rel_l2(fg, fo) < tol, (prec, rel_l2(mg, mo), rel_l2(fg, fo))
+        assert rel_l2(hg, ho, A) < tol_h and rel_l2(ug, uo) < tol_u, (prec, rel_l2(hg, ho, A), rel_l2(ug, uo))
+        with Swm(m, precision=prec, dt=900.0) as p:
+            p.set_state(h, u)
+            p.step(3)
             hq, uq = p.get_state()
         assert np.array_equal(hp, hq) and np.array_equal(up, uq)
 
