@@ -338,6 +338,40 @@ DRIVER_METHODS = {
 }
 
 
+@pytest.mark.parametrize("stag", ["HTC", "HC"])
+def test_fortran_driver_fused_level_model(golden, stag):
+    """The level-model line (noPV) with trsk10 on the fused FP32 path: 3
+    steps of the seeded state against the reference's own noPV run (HTC) or
+    the driver's FP64_REF run (HC), to the FP32 storage tolerances."""
+    print(f"[gpu-case] fortran imodel g3 seeded fp32 level model {stag}", flush=True)
+
+    def run(prec):
+        with tempfile.TemporaryDirectory() as td:
+            _par(td, 642, 100, 900, 0.03125, 3, prec)
+            p = os.path.join(td, "par", "swm.par")
+            swm = open(p).read().splitlines()
+            swm[8] = stag
+            swm[34] = "1"
+            open(p, "w").write("\n".join(swm) + "\n")
+            p = os.path.join(td, "par", "swm_gpu.par")
+            gpu = open(p).read().splitlines()
+            gpu[3] = "7"
+            open(p, "w").write("\n".join(gpu) + "\n")
+            res = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                 text=True, timeout=120)
+            assert res.returncode == 0, res.stdout[-2000:]
+            f = [x for x in os.listdir(os.path.join(td, "data")) if x.endswith("_state_icos_eqs_nopt_3.bin")]
+            assert len(f) == 1 and "_nopv" in f[0], f
+            return imxbin.read(os.path.join(td, "data", f[0]))
+
+    st = run("fp32")
+    r = golden("ref_run_g3_mnopv.bin") if stag == "HTC" else run("fp64_ref")
+    A = Mesh.icos(3).arrays["hx_areag"]
+    eh = np.sqrt(np.sum(A * (st["h"] - r["h"]) ** 2) / np.sum(A * r["h"] ** 2))
+    eu = np.linalg.norm(st["u"] - r["u"]) / np.linalg.norm(r["u"])
+    assert eh < 1e-7 and eu < 1e-5, (eh, eu)
+
+
 @pytest.mark.parametrize("name", sorted(DRIVER_METHODS))
 def test_fortran_driver_methods_bitexact(golden, name):
     """The individual method lines read as swmpars reads them: the output
