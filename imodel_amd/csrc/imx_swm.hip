@@ -972,7 +972,8 @@ __global__ __launch_bounds__(BS) void fast_hyper_c(int remap, int nv, int NT, co
 
 // OPT selects the code compiled in, and with it the register footprint:
 // 0 the plain scheme, 1 with diffusion and hyperdiffusion, 2 with every
-// option (APVM and pxt16 too). Pointers of options left out are ignored.
+// option (APVM and pxt16 too), 3 gass18, 4 the level model (with
+// diffusion). Pointers of options left out are ignored.
 template <typename T, int BS, int OPT>
 __global__ __launch_bounds__(BS) void fast_cell(
     int remap, int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
@@ -987,9 +988,8 @@ __global__ __launch_bounds__(BS) void fast_cell(
     T inv_a, T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
     double *__restrict__ dh_out) {
-  // OPT 0 plain; 1 diffusion; 2 every option; 3 gass18; 4 the level model
-  // (noPV: the TRiSK sum takes u and eta at edges in place of uh and q),
-  // with diffusion
+  // the level model (OPT 4): the TRiSK sum takes u and eta at edges in
+  // place of uh and q
   constexpr bool kDiff = OPT == 1 || OPT == 2 || OPT == 4, kAll = OPT == 2, kGass = OPT == 3, kNopv = OPT == 4;
   const int c = ent_index<BS>(remap);
   if (c >= nv) return;
