@@ -123,6 +123,11 @@ struct imx_swm {
   // holding it, and the per-stage result [NE]
   bool f_pxhc = false;
   bool f_nopv = false;  // fused level model (trsk10)
+  // Gassmann's kinetic energy on the fused kernels: sub-area weights [3][NT],
+  // per-stage K_tr [NT] (typed T) and gasscoef
+  bool f_gk = false;
+  double gk_coef = 1.0;
+  void *x_gk_w = nullptr, *x_gk_k = nullptr;
   void *x_px_hb = nullptr, *x_px_he = nullptr;
   int32_t *x_px_hv = nullptr;
   int32_t *x_ap_tnb = nullptr;

@@ -715,7 +715,8 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
                                                 const T *__restrict__ kt, T *__restrict__ cz,
                                                 int gmode, const T *__restrict__ g_w,
                                                 const T *__restrict__ g_a, T *__restrict__ gE,
-                                                T *__restrict__ gH, T *__restrict__ q,
+                                                T *__restrict__ gH, const T *__restrict__ gk_w,
+                                                T *__restrict__ gk_k, T *__restrict__ q,
                                                 int *__restrict__ flag) {
   const int t = ent_index<BS>(remap);
   if (t >= NT) return;
@@ -739,6 +740,15 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
     z += u[e1] * t_ceta[NT + t];
     z += u[e2] * t_ceta[2 * NT + t];
     cz[t] = kt[t] * z;
+  }
+  if (gk_w) {
+    // kinetic_energy_tr (Gassmann's kinetic energy): sum of the
+    // (circumcentre, edge) sub-areas times u^2, over A_t
+    const T u0 = u[e0], u1 = u[e1], u2 = u[e2];
+    T k = gk_w[t] * (u0 * u0);
+    k += gk_w[NT + t] * (u1 * u1);
+    k += gk_w[2 * NT + t] * (u2 * u2);
+    gk_k[t] = k;
   }
   if (gmode) {
     // gass18: h at each edge from its two vertices, h_t = sum w_m h_e / A_t
@@ -984,7 +994,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ pxw,
     const T *__restrict__ pxq, V4<T> *__restrict__ pxab, T *__restrict__ pxqe, const V4<T> *__restrict__ apv,
     const V4<T> *__restrict__ apg, T pvsdt, const T *__restrict__ gq, const int32_t *__restrict__ gout,
-    const T *__restrict__ hce, const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
+    const T *__restrict__ hce, const T *__restrict__ gk_k, T gk_c, const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
     T inv_a, T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
     double *__restrict__ dh_out) {
@@ -1050,6 +1060,15 @@ __global__ __launch_bounds__(BS) void fast_cell(
   }
   const double dh = -(dv + dvc) * c_iaa[c];
   T K = ke * c_ika[c];
+  if ((kAll || kGass) && gk_k) {
+    // Gassmann's kinetic energy (kinetic_energy_hx, swm_operators.f90:
+    // 1198-1207): gasscoef K_trsk + (1 - gasscoef) sum_j hxtr_j K_tr(t_j)
+    T kk = K * gk_c;
+#pragma unroll
+    for (int j = 0; j < MX; ++j)
+      if ((j < MX - 1) || hex) kk += ((T)1 - gk_c) * R[j] * gk_k[tr[j]];
+    K = kk;
+  }
   if (kAll && pxw) {  // pxt16: |v|^2 / 2 of the cell's Perot velocity (perhx)
     T v0 = (T)0, v1 = (T)0, v2 = (T)0;
 #pragma unroll
@@ -1571,6 +1590,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, s->f_lin ? 1 : 0, s->f_nopv ? 1 : 0,  \
       (s->f_diff || s->f_hyper) ? (const T *)s->x_kt : nullptr,                                    \
       (T *)s->x_cz, s->f_gass, (const T *)s->x_g_w, (const T *)s->x_g_a, (T *)s->x_g_E, (T *)s->x_g_H, \
+      s->f_gk ? (const T *)s->x_gk_w : nullptr, (T *)s->x_gk_k,                                     \
       (s->f_gass && !s->f_apvm) ? nullptr : (T *)s->q, s->flag)
   IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
 #undef IMX_TRI
@@ -1610,7 +1630,8 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->f_px ? (const V4<T> *)s->x_ap_wc : nullptr, s->f_px ? (const T *)s->x_px_wq : nullptr,    \
       (V4<T> *)s->x_px_ab, (T *)s->x_px_qe,                                                        \
       s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
-      (const T *)s->x_g_q, s->x_g_out, s->f_pxhc ? (const T *)s->x_px_he : nullptr, hs, us, s->href,   \
+      (const T *)s->x_g_q, s->x_g_out, s->f_pxhc ? (const T *)s->x_px_he : nullptr,                  \
+      s->f_gk ? (const T *)s->x_gk_k : nullptr, (T)s->gk_coef, hs, us, s->href,                      \
       (const T *)s->x_gb,                                                                          \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
@@ -1620,7 +1641,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
   } else if (s->f_nopv) {
     constexpr int OPT = 4;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
-  } else if (s->f_apvm || s->f_px) {
+  } else if (s->f_apvm || s->f_px || s->f_gk) {
     constexpr int OPT = 2;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
   } else if (s->f_diff || s->f_hyper) {
@@ -3123,6 +3144,37 @@ static double gass_subarea(const imx_mesh_desc &d, int ko, int i) {
   return sphtriarea(d.tr_c + 3 * ko, d.v_p + 3 * (d.ed_v[2 * eo] - 1), d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1));
 }
 
+// Gassmann's kinetic energy on the fused kernels (on = false switches it
+// off): the (circumcentre, edge) sub-areas over A_t in fast_tri's slot order,
+// and the per-stage K_tr, typed T
+template <typename T>
+static int fast_gassk(imx_swm *s, const imx_mesh_desc &d, bool on, double gcoef) {
+  s->f_gk = false;
+  if (!on) return IMX_OK;
+  const int NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &tn = s->lay.tri_new;
+  std::vector<double> kw((size_t)3 * NT, 0.0);
+  for (int ko = 0; ko < d.nt; ++ko) {
+    const int t = tn[ko];
+    int r = -1;  // slot m of the fused kernels is reference edge (r + m) % 3
+    for (int l = 0; l < 3; ++l)
+      if (cn[d.tr_v[3 * ko + l] - 1] == t / 2) r = l;
+    if (r < 0) return set_error(IMX_EINVAL, "imx_swm_set_scheme: triangle owner not found");
+    for (int m = 0; m < 3; ++m) kw[(size_t)m * NT + t] = gass_subarea(d, ko, (r + m) % 3) / d.tr_areag[ko];
+  }
+  Alloc &A = s->A;
+  if (!s->x_gk_w) {
+    s->x_gk_w = A.get<T>((size_t)3 * NT);
+    s->x_gk_k = A.get<T>(NT);
+    if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
+  }
+  const std::vector<T> kwT = cvt<T>(kw);
+  HIPTRY(hipMemcpy(s->x_gk_w, kwT.data(), kwT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  s->gk_coef = gcoef;
+  s->f_gk = true;
+  return IMX_OK;
+}
+
 // gass18 on the fused kernels (mode 1 HTC, 2 HC; 0 switches it off): the
 // sub-triangle areas in fast_tri's slot order with their sum, the triangle
 // areas, the two triangles of each edge slot and the outer edge of each
@@ -3240,21 +3292,28 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   const bool hc = sc->stag == IMX_STAG_HC;
   if (s->fast) {
     // the fused kernels carry trsk10, pxt16 and gass18 with HTC or HC
+    // a wrapper's triple, or trsk10's or gass18's scalar interpolation and
+    // Coriolis term with Gassmann's kinetic energy (gass with gasscoef;
+    // dubos is gass with 0)
     int fw = -1;
+    bool gk = false;
     for (int w : {IMX_WRAP_TRSK10, IMX_WRAP_PXT16, IMX_WRAP_GASS18}) {
       int a, b, c;
       imx::wrapper_methods(w, a, b, c);
-      if (a == sint && b == cor && c == rec) fw = w;
+      if (a != sint || b != cor) continue;
+      if (c == rec) fw = w, gk = false;
+      else if (rec == IMX_REC_GASS && w != IMX_WRAP_PXT16) fw = w, gk = true;
     }
-    if (fw < 0 || (sc->nopv && (fw != IMX_WRAP_TRSK10 || sc->apvm || sc->k4 > 0.0)))
+    if (fw < 0 || (sc->nopv && (fw != IMX_WRAP_TRSK10 || gk || sc->apvm || sc->k4 > 0.0)))
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take trsk10 and pxt16 (HTC or HC) with APVM, "
-                            "diffusion and hyperdiffusion, gass18 (HTC or HC) with or without APVM, and the "
-                            "level model with trsk10 (HTC or HC, diffusion allowed); other method triples and "
-                            "the level model with other options run on FP64_REF handles");
+                            "diffusion and hyperdiffusion, gass18 (HTC or HC) with or without APVM, trsk10's "
+                            "or gass18's methods with Gassmann's kinetic energy, and the level model with "
+                            "trsk10 (HTC or HC, diffusion allowed); other method triples and the level model "
+                            "with other options run on FP64_REF handles");
     // the tables of each option, in turn; every step rewrites the tables
     // and flags of its option whether it is on or off
-    auto apply = [&](const imx_scheme &q, int qw, bool qhc) -> int {
+    auto apply = [&](const imx_scheme &q, int qw, bool qhc, bool qgk, double qgc) -> int {
       int rc;
       if (qhc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, false, false)))) return rc;
       rc = s->single ? imx::fast_stagger<float>(s, *d, qhc) : imx::fast_stagger<double>(s, *d, qhc);
@@ -3266,12 +3325,14 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       if (rc) return rc;
       rc = s->single ? imx::fast_pxt16<float>(s, *d, q) : imx::fast_pxt16<double>(s, *d, q);
       if (rc) return rc;
+      rc = s->single ? imx::fast_gassk<float>(s, *d, qgk, qgc) : imx::fast_gassk<double>(s, *d, qgk, qgc);
+      if (rc) return rc;
       const int gmode = qw == IMX_WRAP_GASS18 ? (qhc ? 2 : 1) : 0;
       return s->single ? imx::fast_gass18<float>(s, *d, gmode) : imx::fast_gass18<double>(s, *d, gmode);
     };
     imx_scheme r = *sc;
     r.wrap = fw;
-    int rc = apply(r, fw, hc);
+    int rc = apply(r, fw, hc, gk, gcoef);
     s->f_nopv = rc == IMX_OK && r.nopv != 0;
     std::string why;
     if (rc) {
@@ -3279,12 +3340,12 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       // new options: put the handle back on the default scheme instead
       why = imx_last_error();
       const imx_scheme def{};
-      (void)apply(def, IMX_WRAP_TRSK10, false);
-      r = def, fw = IMX_WRAP_TRSK10;
+      (void)apply(def, IMX_WRAP_TRSK10, false, false, 1.0);
+      r = def, fw = IMX_WRAP_TRSK10, gk = false, gcoef = 1.0;
       imx::wrapper_methods(fw, sint, cor, rec);
     }
     s->sc_wrap = fw;
-    s->sc_sint = sint, s->sc_cor = cor, s->sc_rec = rec, s->sc_gcoef = 1.0, s->sc_nopv = s->f_nopv;
+    s->sc_sint = sint, s->sc_cor = cor, s->sc_rec = rec, s->sc_gcoef = gk ? gcoef : 1.0, s->sc_nopv = s->f_nopv;
     // the option flags size the Jacobian's stencil superset
     s->sc_apvm = r.apvm != 0;
     s->sc_diff = r.k2 != 0.0;

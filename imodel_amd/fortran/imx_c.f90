@@ -9,6 +9,7 @@ module imx_c
   implicit none
 
   integer(c_int32_t), parameter :: IMX_PREC_FP64_REF = 0, IMX_PREC_FP64 = 1, IMX_PREC_FP32 = 2
+  integer(c_int), parameter :: IMX_ENOTSUP = -6
   integer(c_int32_t), parameter :: IMX_FIELD_H_ED = 0, IMX_FIELD_ETA = 2, IMX_FIELD_Q_TR = 4, IMX_FIELD_Q_ED = 5, &
                                    IMX_FIELD_DIVUH = 6, IMX_FIELD_KE_HX = 7, IMX_FIELD_DIVU = 11
   integer(c_int32_t), parameter :: IMX_POS_EQS = 0, IMX_POS_POL = 1

@@ -454,6 +454,7 @@ contains
     integer(c_int32_t), intent(in) :: prec
     type(imx_swm_opts) :: o
     type(imx_scheme) :: sc
+    integer(c_int) :: rc
     call imx_swm_default_opts(o)
     o%precision = prec
     o%testcase = merge(2, testcase, testcase == 100)
@@ -474,15 +475,6 @@ contains
       if (fs /= 0) call imx_check(imx_swm_set_coriolis(hnd, fs, fc), 'imx_swm_set_coriolis')
     end block
     if (scheme_options()) then
-      ! the fused fp32/fp64 kernels carry trsk10 and pxt16 with APVM,
-      ! diffusion and hyperdiffusion, gass18 with or without APVM, and the
-      ! level model with trsk10 and at most diffusion
-      if (prec /= IMX_PREC_FP64_REF .and. (wrap_code() == 3 .or. (inopv > 0 .and. &
-          (wrap_code() /= 0 .or. trim(pv_stab) == 'apvm' .or. k4_max > 0._r8)))) then
-        write(*, '(a)') " ERROR: wrapper none, and noPV with other than trsk10 and diffusion, run on the fp64_ref path"
-        write(*, '(a)') "        (par/swm_gpu.par)"
-        stop 1
-      end if
       sc%stag = merge(1, 0, trim(stag) == 'HC')
       sc%wrap = wrap_code()
       sc%apvm = merge(1, 0, trim(pv_stab) == 'apvm')
@@ -496,7 +488,16 @@ contains
       sc%recon = rec_code()
       sc%gasscoef = gass_coef()
       sc%nopv = merge(1, 0, inopv > 0)
-      call imx_check(imx_swm_set_scheme(hnd, mdesc, sc), 'imx_swm_set_scheme')
+      ! the fused fp32/fp64 kernels carry the wrappers, trsk10's and
+      ! gass18's methods with Gassmann's kinetic energy and the level model
+      ! with trsk10; the library names what they refuse
+      rc = imx_swm_set_scheme(hnd, mdesc, sc)
+      if (rc == IMX_ENOTSUP .and. prec /= IMX_PREC_FP64_REF) then
+        write(*, '(a)') " ERROR: these scheme options run on the fp64_ref path (par/swm_gpu.par):"
+        write(*, '(a)') "        "//trim(imx_error_message())
+        stop 1
+      end if
+      call imx_check(rc, 'imx_swm_set_scheme')
     end if
   end subroutine open_handle
 

@@ -338,20 +338,33 @@ DRIVER_METHODS = {
 }
 
 
-@pytest.mark.parametrize("stag", ["HTC", "HC"])
-def test_fortran_driver_fused_level_model(golden, stag):
-    """The level-model line (noPV) with trsk10 on the fused FP32 path: 3
-    steps of the seeded state against the reference's own noPV run (HTC) or
-    the driver's FP64_REF run (HC), to the FP32 storage tolerances."""
-    print(f"[gpu-case] fortran imodel g3 seeded fp32 level model {stag}", flush=True)
+# par/swm.par lines of options the fused kernels take beyond the wrappers,
+# with the reference's own 3-step run of seed 7 (None: the driver's FP64_REF
+# run) and a token of the output name
+FUSED_METHODS = {
+    "nopv": ({34: "1"}, "ref_run_g3_mnopv.bin", "_nopv"),
+    "hcnopv": ({8: "HC", 34: "1"}, None, "_nopv"),
+    "rgass": ({10: "none", 12: "gass 0.75"}, "ref_run_g3_mrgass.bin", "vrecgass0.75"),
+    "rdubos": ({10: "none", 12: "dubos 1.0"}, "ref_run_g3_mrdubos.bin", "vrecdubos"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FUSED_METHODS))
+def test_fortran_driver_fused_level_model(golden, name):
+    """The level-model line (noPV) with trsk10 and Gassmann's kinetic energy
+    (reconmtd gass / dubos) on the fused FP32 path: 3 steps of the seeded
+    state against the reference's own run (or the driver's FP64_REF run), to
+    the FP32 storage tolerances."""
+    print(f"[gpu-case] fortran imodel g3 seeded fp32 fused methods {name}", flush=True)
+    lines, fixture, token = FUSED_METHODS[name]
 
     def run(prec):
         with tempfile.TemporaryDirectory() as td:
             _par(td, 642, 100, 900, 0.03125, 3, prec)
             p = os.path.join(td, "par", "swm.par")
             swm = open(p).read().splitlines()
-            swm[8] = stag
-            swm[34] = "1"
+            for k, v in lines.items():
+                swm[k] = v
             open(p, "w").write("\n".join(swm) + "\n")
             p = os.path.join(td, "par", "swm_gpu.par")
             gpu = open(p).read().splitlines()
@@ -361,11 +374,11 @@ def test_fortran_driver_fused_level_model(golden, stag):
                                  text=True, timeout=120)
             assert res.returncode == 0, res.stdout[-2000:]
             f = [x for x in os.listdir(os.path.join(td, "data")) if x.endswith("_state_icos_eqs_nopt_3.bin")]
-            assert len(f) == 1 and "_nopv" in f[0], f
+            assert len(f) == 1 and token in f[0], f
             return imxbin.read(os.path.join(td, "data", f[0]))
 
     st = run("fp32")
-    r = golden("ref_run_g3_mnopv.bin") if stag == "HTC" else run("fp64_ref")
+    r = golden(fixture) if fixture else run("fp64_ref")
     A = Mesh.icos(3).arrays["hx_areag"]
     eh = np.sqrt(np.sum(A * (st["h"] - r["h"]) ** 2) / np.sum(A * r["h"] ** 2))
     eu = np.linalg.norm(st["u"] - r["u"]) / np.linalg.norm(r["u"])

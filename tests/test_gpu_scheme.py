@@ -84,8 +84,9 @@ def test_methods_match_reference(golden, name, sfc):
 
 def test_method_refusals():
     """nopv with pered or dtred, gass interpolation with diffusion, bad
-    method values; fused handles refuse wrapper-none triples other than
-    trsk10's and pxt16's."""
+    method values; fused handles take the wrappers' triples, trsk10's and
+    gass18's methods with Gassmann's kinetic energy and the level model with
+    trsk10, and refuse the other triples."""
     m = Mesh.icos(3)
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         for kw in ({"wrap": "none", "cor": "pered", "nopv": True}, {"wrap": "none", "cor": "dtred", "nopv": True},
@@ -103,6 +104,8 @@ def test_method_refusals():
         s.set_scheme(wrap="none", sint="bary", cor="pered", rec="perhx")
         s.set_scheme(wrap="none", sint="gass", cor="gass")
         s.set_scheme(wrap="none", nopv=True)
+        s.set_scheme(wrap="none", rec="gass", gasscoef=0.5)
+        s.set_scheme(wrap="none", sint="gass", cor="gass", rec="dubos")
         s.set_scheme(wrap="none")
 
 
@@ -261,23 +264,35 @@ def test_fast_k2_steps_vs_oracle(name):
         assert np.array_equal(hp, hq) and np.array_equal(up, uq)
 
 
-NOPV = {"nopv": {"nopv": True}, "hcnopv": {"stag": "HC", "nopv": True},
-        "nopvk2": {"nopv": True, "k2": 1e5, "diffus": "align"}}
+# options the oracle does not restate (the level model, Gassmann's kinetic
+# energy): (scheme, the same scheme without the option)
+FAST_VS_REF = {
+    "nopv": ({"nopv": True}, {}),
+    "hcnopv": ({"stag": "HC", "nopv": True}, {"stag": "HC"}),
+    "nopvk2": ({"nopv": True, "k2": 1e5, "diffus": "align"}, {"k2": 1e5, "diffus": "align"}),
+    "rgass": ({"wrap": "none", "rec": "gass", "gasscoef": 0.75}, {}),
+    "rdubos": ({"wrap": "none", "rec": "dubos"}, {}),
+    "hcrgass": ({"stag": "HC", "wrap": "none", "rec": "gass", "gasscoef": 0.75}, {"stag": "HC"}),
+    "gassgk": ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "gass", "gasscoef": 0.5}, {"wrap": "gass18"}),
+    "rgassk2": ({"wrap": "none", "rec": "gass", "gasscoef": 0.25, "k2": 1e5, "diffus": "const", "pv": "apvm",
+                 "pvspar": 0.5}, {"k2": 1e5, "diffus": "const", "pv": "apvm", "pvspar": 0.5}),
+}
 
 
-@pytest.mark.parametrize("name", sorted(NOPV))
-def test_fast_level_model_vs_ref(name):
-    """The level model (noPV) with trsk10 on the fused kernels against the
-    FP64_REF handle, which equals the reference's own noPV runs bit for bit:
-    the tendency (FP64 < 1e-12, FP32 < 5e-5) and 10 steps; the noPV term
-    differs from the PV form far above those tolerances; switching back
-    restores the plain fused step bit for bit."""
+@pytest.mark.parametrize("name", sorted(FAST_VS_REF))
+def test_fast_vs_ref(name):
+    """The level model (noPV) with trsk10, and Gassmann's kinetic energy
+    (gass with gasscoef, dubos) with trsk10's or gass18's methods, on the
+    fused kernels against the FP64_REF handle, which equals the reference's
+    own runs with these options bit for bit: the tendency (FP64 < 1e-12,
+    FP32 < 5e-5) and 10 steps; the option changes the tendency far above the
+    FP64 tolerance; switching back restores the plain fused step bit for
+    bit."""
     from tests.conftest import rel_l2
-    say("fast level model", name)
+    say("fast vs ref", name)
     m = Mesh.icos(4)
     h, u = m.spectral(24)
-    sc = NOPV[name]
-    base = {k: v for k, v in sc.items() if k != "nopv"}
+    sc, base = FAST_VS_REF[name]
     with Swm(m, precision="fp64_ref", dt=900.0) as r:
         r.set_scheme(**sc)
         mo, fo = r.tendency(h, u)
@@ -286,7 +301,7 @@ def test_fast_level_model_vs_ref(name):
         ho, uo = r.get_state()
         r.set_scheme(**base)
         _, fp = r.tendency(h, u)
-    assert rel_l2(fp, fo) > 1e-4
+    assert rel_l2(fp, fo) > 1e-6, rel_l2(fp, fo)
     A = m.arrays["hx_areag"]
     for prec, tol, tol_h, tol_u in (("fp64", 1e-12, 1e-12, 1e-11), ("fp32", 5e-5, 1e-7, 1e-5)):
         with Swm(m, precision=prec, dt=900.0) as s:
@@ -312,7 +327,7 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"wrap": "none", "rec": "melv"}, {"wrap": "none", "rec": "dubos"},
+        for kw in ({"wrap": "none", "rec": "melv"}, {"wrap": "none", "sint": "bary", "cor": "pered", "rec": "gass"},
                    {"wrap": "none", "sint": "gass", "cor": "dtred"}):
             with pytest.raises(capi.ImxError, match="fused handles take trsk10"):
                 s.set_scheme(**kw)
