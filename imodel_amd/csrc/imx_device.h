@@ -126,6 +126,7 @@ struct imx_swm {
   // Gassmann's kinetic energy on the fused kernels: sub-area weights [3][NT],
   // per-stage K_tr [NT] (typed T) and gasscoef
   bool f_gk = false;
+  int f_melv = 0;  // Melvin's kinetic energy on the fused kernels: 1 HTC, 2 HC
   double gk_coef = 1.0;
   void *x_gk_w = nullptr, *x_gk_k = nullptr;
   void *x_px_hb = nullptr, *x_px_he = nullptr;

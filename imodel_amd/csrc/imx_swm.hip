@@ -994,7 +994,8 @@ __global__ __launch_bounds__(BS) void fast_cell(
     const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ pxw,
     const T *__restrict__ pxq, V4<T> *__restrict__ pxab, T *__restrict__ pxqe, const V4<T> *__restrict__ apv,
     const V4<T> *__restrict__ apg, T pvsdt, const T *__restrict__ gq, const int32_t *__restrict__ gout,
-    const T *__restrict__ hce, const T *__restrict__ gk_k, T gk_c, const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
+    const T *__restrict__ hce, const T *__restrict__ gk_k, T gk_c, int melv,
+    const V4<T> *__restrict__ mw, const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
     T inv_a, T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
     double *__restrict__ dh_out) {
@@ -1068,6 +1069,33 @@ __global__ __launch_bounds__(BS) void fast_cell(
     for (int j = 0; j < MX; ++j)
       if ((j < MX - 1) || hex) kk += ((T)1 - gk_c) * R[j] * gk_k[tr[j]];
     K = kk;
+  }
+  if (kAll && melv) {
+    // Melvin's kinetic energy (kinetic_energy_hx, swm_operators.f90:
+    // 1174-1197): HTC, sum_j kw_j u_j (u_j h_e) / (h sum_j kw_j); HC, the
+    // Perot vectors of uh and u dotted, over 2 h
+    const double hcell = href + (double)hc;
+    if (mw) {
+      T a0 = (T)0, a1 = (T)0, a2 = (T)0, b0 = (T)0, b1 = (T)0, b2 = (T)0;
+#pragma unroll
+      for (int j = 0; j < MX; ++j) {  // a pentagon's padded slot has a zero weight
+        const V4<T> w = mw[j * nv + c];
+        const T uh = ue[j] * (T)hE[j];
+        a0 += w.x * uh;
+        a1 += w.y * uh;
+        a2 += w.z * uh;
+        b0 += w.x * ue[j];
+        b1 += w.y * ue[j];
+        b2 += w.z * ue[j];
+      }
+      K = (T)((double)(a0 * b0 + a1 * b1 + a2 * b2) / hcell / 2.0);
+    } else {
+      T km = (T)0;
+#pragma unroll
+      for (int j = 0; j < MX; ++j)
+        if ((j < MX - 1) || hex) km += kw[j] * ue[j] * (ue[j] * (T)hE[j]);
+      K = (T)((double)(km * c_ika[c]) / hcell);
+    }
   }
   if (kAll && pxw) {  // pxt16: |v|^2 / 2 of the cell's Perot velocity (perhx)
     T v0 = (T)0, v1 = (T)0, v2 = (T)0;
@@ -1631,7 +1659,8 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       (V4<T> *)s->x_px_ab, (T *)s->x_px_qe,                                                        \
       s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
       (const T *)s->x_g_q, s->x_g_out, s->f_pxhc ? (const T *)s->x_px_he : nullptr,                  \
-      s->f_gk ? (const T *)s->x_gk_k : nullptr, (T)s->gk_coef, hs, us, s->href,                      \
+      s->f_gk ? (const T *)s->x_gk_k : nullptr, (T)s->gk_coef, s->f_melv,                          \
+      s->f_melv == 2 ? (const V4<T> *)s->x_ap_wc : nullptr, hs, us, s->href,                         \
       (const T *)s->x_gb,                                                                          \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
@@ -1641,7 +1670,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
   } else if (s->f_nopv) {
     constexpr int OPT = 4;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
-  } else if (s->f_apvm || s->f_px || s->f_gk) {
+  } else if (s->f_apvm || s->f_px || s->f_gk || s->f_melv) {
     constexpr int OPT = 2;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
   } else if (s->f_diff || s->f_hyper) {
@@ -3294,26 +3323,28 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     // the fused kernels carry trsk10, pxt16 and gass18 with HTC or HC
     // a wrapper's triple, or trsk10's or gass18's scalar interpolation and
     // Coriolis term with Gassmann's kinetic energy (gass with gasscoef;
-    // dubos is gass with 0)
+    // dubos is gass with 0), or trsk10's with Melvin's
     int fw = -1;
-    bool gk = false;
+    bool gk = false, melv = false;
     for (int w : {IMX_WRAP_TRSK10, IMX_WRAP_PXT16, IMX_WRAP_GASS18}) {
       int a, b, c;
       imx::wrapper_methods(w, a, b, c);
       if (a != sint || b != cor) continue;
-      if (c == rec) fw = w, gk = false;
-      else if (rec == IMX_REC_GASS && w != IMX_WRAP_PXT16) fw = w, gk = true;
+      if (c == rec) fw = w, gk = false, melv = false;
+      else if (rec == IMX_REC_GASS && w != IMX_WRAP_PXT16) fw = w, gk = true, melv = false;
+      else if (rec == IMX_REC_MELV && w == IMX_WRAP_TRSK10) fw = w, gk = false, melv = true;
     }
-    if (fw < 0 || (sc->nopv && (fw != IMX_WRAP_TRSK10 || gk || sc->apvm || sc->k4 > 0.0)))
+    if (fw < 0 || (sc->nopv && (fw != IMX_WRAP_TRSK10 || gk || melv || sc->apvm || sc->k4 > 0.0)))
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take trsk10 and pxt16 (HTC or HC) with APVM, "
                             "diffusion and hyperdiffusion, gass18 (HTC or HC) with or without APVM, trsk10's "
-                            "or gass18's methods with Gassmann's kinetic energy, and the level model with "
+                            "or gass18's methods with Gassmann's kinetic energy, trsk10's with Melvin's, and "
+                            "the level model with "
                             "trsk10 (HTC or HC, diffusion allowed); other method triples and the level model "
                             "with other options run on FP64_REF handles");
     // the tables of each option, in turn; every step rewrites the tables
     // and flags of its option whether it is on or off
-    auto apply = [&](const imx_scheme &q, int qw, bool qhc, bool qgk, double qgc) -> int {
+    auto apply = [&](const imx_scheme &q, int qw, bool qhc, bool qgk, double qgc, bool qmelv) -> int {
       int rc;
       if (qhc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, false, false)))) return rc;
       rc = s->single ? imx::fast_stagger<float>(s, *d, qhc) : imx::fast_stagger<double>(s, *d, qhc);
@@ -3327,12 +3358,19 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       if (rc) return rc;
       rc = s->single ? imx::fast_gassk<float>(s, *d, qgk, qgc) : imx::fast_gassk<double>(s, *d, qgk, qgc);
       if (rc) return rc;
+      // Melvin's kinetic energy: HC takes the Perot weights of the cell
+      s->f_melv = 0;
+      if (qmelv && qhc) {
+        rc = s->single ? imx::fast_cell_weights<float>(s, *d) : imx::fast_cell_weights<double>(s, *d);
+        if (rc) return rc;
+      }
+      s->f_melv = qmelv ? (qhc ? 2 : 1) : 0;
       const int gmode = qw == IMX_WRAP_GASS18 ? (qhc ? 2 : 1) : 0;
       return s->single ? imx::fast_gass18<float>(s, *d, gmode) : imx::fast_gass18<double>(s, *d, gmode);
     };
     imx_scheme r = *sc;
     r.wrap = fw;
-    int rc = apply(r, fw, hc, gk, gcoef);
+    int rc = apply(r, fw, hc, gk, gcoef, melv);
     s->f_nopv = rc == IMX_OK && r.nopv != 0;
     std::string why;
     if (rc) {
@@ -3340,8 +3378,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       // new options: put the handle back on the default scheme instead
       why = imx_last_error();
       const imx_scheme def{};
-      (void)apply(def, IMX_WRAP_TRSK10, false, false, 1.0);
-      r = def, fw = IMX_WRAP_TRSK10, gk = false, gcoef = 1.0;
+      (void)apply(def, IMX_WRAP_TRSK10, false, false, 1.0, false);
+      r = def, fw = IMX_WRAP_TRSK10, gk = false, melv = false, gcoef = 1.0;
       imx::wrapper_methods(fw, sint, cor, rec);
     }
     s->sc_wrap = fw;

@@ -217,7 +217,7 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         p = os.path.join(td, "par", "swm.par")
         swm = open(p).read().splitlines()
         swm[10] = "none"
-        swm[12] = "melv 1.0"
+        swm[14] = "dtred"
         open(p, "w").write("\n".join(swm) + "\n")
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)
@@ -346,6 +346,8 @@ FUSED_METHODS = {
     "hcnopv": ({8: "HC", 34: "1"}, None, "_nopv"),
     "rgass": ({10: "none", 12: "gass 0.75"}, "ref_run_g3_mrgass.bin", "vrecgass0.75"),
     "rdubos": ({10: "none", 12: "dubos 1.0"}, "ref_run_g3_mrdubos.bin", "vrecdubos"),
+    "rmelv": ({10: "none", 12: "melv 1.0"}, "ref_run_g3_mrmelv.bin", "vrecmelv"),
+    "rmelvhc": ({8: "HC", 10: "none", 12: "melv 1.0"}, "ref_run_g3_mrmelvhc.bin", "vrecmelv"),
 }
 
 

@@ -85,8 +85,8 @@ def test_methods_match_reference(golden, name, sfc):
 def test_method_refusals():
     """nopv with pered or dtred, gass interpolation with diffusion, bad
     method values; fused handles take the wrappers' triples, trsk10's and
-    gass18's methods with Gassmann's kinetic energy and the level model with
-    trsk10, and refuse the other triples."""
+    gass18's methods with Gassmann's kinetic energy, trsk10's with Melvin's,
+    and the level model with trsk10, and refuse the other triples."""
     m = Mesh.icos(3)
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         for kw in ({"wrap": "none", "cor": "pered", "nopv": True}, {"wrap": "none", "cor": "dtred", "nopv": True},
@@ -95,7 +95,8 @@ def test_method_refusals():
                 s.set_scheme(**kw)
         s.set_scheme(wrap="none", cor="gass", nopv=True)
     with Swm(m, precision="fp32", dt=900.0) as s:
-        for kw in ({"wrap": "none", "rec": "melv"}, {"wrap": "none", "nopv": True, "pv": "apvm"},
+        for kw in ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "melv"},
+                   {"wrap": "none", "nopv": True, "pv": "apvm"},
                    {"wrap": "gass18", "nopv": True}, {"wrap": "none", "cor": "dtred"}):
             with pytest.raises(capi.ImxError):
                 s.set_scheme(**kw)
@@ -105,6 +106,7 @@ def test_method_refusals():
         s.set_scheme(wrap="none", sint="gass", cor="gass")
         s.set_scheme(wrap="none", nopv=True)
         s.set_scheme(wrap="none", rec="gass", gasscoef=0.5)
+        s.set_scheme(wrap="none", rec="melv")
         s.set_scheme(wrap="none", sint="gass", cor="gass", rec="dubos")
         s.set_scheme(wrap="none")
 
@@ -156,7 +158,7 @@ def test_scheme_errors():
     m = Mesh.icos(2)
     with Swm(m, precision="fp32", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="FP64_REF"):
-            s.set_scheme(wrap="none", rec="melv")
+            s.set_scheme(wrap="none", cor="dtred")
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
@@ -276,13 +278,18 @@ FAST_VS_REF = {
     "gassgk": ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "gass", "gasscoef": 0.5}, {"wrap": "gass18"}),
     "rgassk2": ({"wrap": "none", "rec": "gass", "gasscoef": 0.25, "k2": 1e5, "diffus": "const", "pv": "apvm",
                  "pvspar": 0.5}, {"k2": 1e5, "diffus": "const", "pv": "apvm", "pvspar": 0.5}),
+    "rmelv": ({"wrap": "none", "rec": "melv"}, {}),
+    "rmelvhc": ({"stag": "HC", "wrap": "none", "rec": "melv"}, {"stag": "HC"}),
+    "rmelvmix": ({"wrap": "none", "rec": "melv", "k2": 1e5, "diffus": "align", "pv": "apvm", "pvspar": 0.4},
+                 {"k2": 1e5, "diffus": "align", "pv": "apvm", "pvspar": 0.4}),
 }
 
 
 @pytest.mark.parametrize("name", sorted(FAST_VS_REF))
 def test_fast_vs_ref(name):
-    """The level model (noPV) with trsk10, and Gassmann's kinetic energy
-    (gass with gasscoef, dubos) with trsk10's or gass18's methods, on the
+    """The level model (noPV) with trsk10, Gassmann's kinetic energy (gass
+    with gasscoef, dubos) with trsk10's or gass18's methods, and Melvin's
+    kinetic energy (HTC and HC) with trsk10's methods, on the
     fused kernels against the FP64_REF handle, which equals the reference's
     own runs with these options bit for bit: the tendency (FP64 < 1e-12,
     FP32 < 5e-5) and 10 steps; the option changes the tendency far above the
@@ -327,7 +334,7 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"wrap": "none", "rec": "melv"}, {"wrap": "none", "sint": "bary", "cor": "pered", "rec": "gass"},
+        for kw in ({"wrap": "none", "sint": "bary"}, {"wrap": "none", "sint": "bary", "cor": "pered", "rec": "gass"},
                    {"wrap": "none", "sint": "gass", "cor": "dtred"}):
             with pytest.raises(capi.ImxError, match="fused handles take trsk10"):
                 s.set_scheme(**kw)
