@@ -127,6 +127,12 @@ struct imx_swm {
   // per-stage K_tr [NT] (typed T) and gasscoef
   bool f_gk = false;
   int f_melv = 0;  // Melvin's kinetic energy on the fused kernels: 1 HTC, 2 HC
+  // dtred's Coriolis term on the fused kernels: triangle weights [3][NT] and
+  // per-stage vhq [NT] (V4 records), edge interpolation weights [2][NE],
+  // normals [NE] (V4), the two triangles of each edge slot [2][NE]
+  bool f_dtred = false;
+  void *x_dt_w = nullptr, *x_dt_vhq = nullptr, *x_dt_ew = nullptr, *x_dt_n = nullptr;
+  int32_t *x_dt_et = nullptr;
   double gk_coef = 1.0;
   void *x_gk_w = nullptr, *x_gk_k = nullptr;
   void *x_px_hb = nullptr, *x_px_he = nullptr;

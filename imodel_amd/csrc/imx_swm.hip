@@ -704,6 +704,13 @@ __global__ __launch_bounds__(TPB) void ref_rk_final(int n, double *__restrict__ 
 #define OTH_MASK 0x3FFFFFFF
 #define OTH_V2 30
 
+// Points and vectors of the fused APVM pass, one 16 B (FP32) or 32 B (FP64)
+// record each, so a gather is one vector load instead of three.
+template <typename T>
+struct alignas(4 * sizeof(T)) V4 {
+  T x, y, z, w;
+};
+
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t *__restrict__ t_ab,
                                                 const int32_t *__restrict__ t_eb,
@@ -716,7 +723,8 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
                                                 int gmode, const T *__restrict__ g_w,
                                                 const T *__restrict__ g_a, T *__restrict__ gE,
                                                 T *__restrict__ gH, const T *__restrict__ gk_w,
-                                                T *__restrict__ gk_k, T *__restrict__ q,
+                                                T *__restrict__ gk_k, const V4<T> *__restrict__ dt_w,
+                                                V4<T> *__restrict__ dt_vhq, T *__restrict__ q,
                                                 int *__restrict__ flag) {
   const int t = ent_index<BS>(remap);
   if (t >= NT) return;
@@ -788,14 +796,21 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
   // the level model's Coriolis term reads the absolute vorticity, the PV
   // still guards the depth (scalar_elem_divide runs either way)
   q[t] = nopv ? eta : qq;
+  if (dt_w) {
+    // dtred: q times the Perot reconstruction of uh at the circumcentre
+    // (vector_edhx2tr_perot, swm.f90:2501-2503), linear in uh: the host
+    // folds signs, lengths, the cross product with the circumcentre and
+    // 1/A_t into one vector per edge slot; uh takes h at the edge as the
+    // mean of its two cells
+    const T ho = h[o], ha = h[a], hb = h[b];
+    const T uh0 = u[e0] * (href + (T)0.5 * (ho + ha));
+    const T uh1 = u[e1] * (href + (T)0.5 * (ha + hb));
+    const T uh2 = u[e2] * (href + (T)0.5 * (hb + ho));
+    const V4<T> w0 = dt_w[t], w1 = dt_w[NT + t], w2 = dt_w[2 * NT + t];
+    dt_vhq[t] = V4<T>{qq * (uh0 * w0.x + uh1 * w1.x + uh2 * w2.x), qq * (uh0 * w0.y + uh1 * w1.y + uh2 * w2.y),
+                      qq * (uh0 * w0.z + uh1 * w1.z + uh2 * w2.z), (T)0};
+  }
 }
-
-// Points and vectors of the fused APVM pass, one 16 B (FP32) or 32 B (FP64)
-// record each, so a gather is one vector load instead of three.
-template <typename T>
-struct alignas(4 * sizeof(T)) V4 {
-  T x, y, z, w;
-};
 
 // pxt16 with HC: h at the Voronoi edge midpoint, barycentric in the cells
 // hv of the triangle that holds it (scalar_hx2ed, swm_operators.f90:88-114),
@@ -995,7 +1010,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     const T *__restrict__ pxq, V4<T> *__restrict__ pxab, T *__restrict__ pxqe, const V4<T> *__restrict__ apv,
     const V4<T> *__restrict__ apg, T pvsdt, const T *__restrict__ gq, const int32_t *__restrict__ gout,
     const T *__restrict__ hce, const T *__restrict__ gk_k, T gk_c, int melv,
-    const V4<T> *__restrict__ mw, const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
+    const V4<T> *__restrict__ mw, int dtred, const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
     T inv_a, T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
     double *__restrict__ dh_out) {
@@ -1160,25 +1175,30 @@ __global__ __launch_bounds__(BS) void fast_cell(
       qe[j] -= pvsdt * d;
     }
   }
-  if (kAll && pxw) {
-    // pxt16's pered Coriolis term: the Perot reconstruction at this cell of
-    // uh (q_e + q_l) / 2 for an edge l is (q_l A + B) / 2 with A = P(uh) and
-    // B = P(uh q); fast_edge combines both cells' A and B with q_l
-    T a0 = (T)0, a1 = (T)0, a2 = (T)0, b0 = (T)0, b1 = (T)0, b2 = (T)0;
+  if (kAll && (pxw || dtred)) {
+    // the Coriolis term is fast_edge's (pered, dtred); this cell stores the
+    // diffusion part of the partials of the edges it owns
+    if (pxw) {
+      // pxt16's pered Coriolis term: the Perot reconstruction at this cell
+      // of uh (q_e + q_l) / 2 for an edge l is (q_l A + B) / 2 with
+      // A = P(uh) and B = P(uh q); fast_edge combines both cells' A and B
+      // with q_l
+      T a0 = (T)0, a1 = (T)0, a2 = (T)0, b0 = (T)0, b1 = (T)0, b2 = (T)0;
 #pragma unroll
-    for (int j = 0; j < MX; ++j) {
-      const V4<T> w = pxw[j * nv + c];
-      const T uh = ue[j] * (T)hE[j];
-      const T uq = uh * qe[j];
-      a0 += w.x * uh;
-      a1 += w.y * uh;
-      a2 += w.z * uh;
-      b0 += w.x * uq;
-      b1 += w.y * uq;
-      b2 += w.z * uq;
+      for (int j = 0; j < MX; ++j) {
+        const V4<T> w = pxw[j * nv + c];
+        const T uh = ue[j] * (T)hE[j];
+        const T uq = uh * qe[j];
+        a0 += w.x * uh;
+        a1 += w.y * uh;
+        a2 += w.z * uh;
+        b0 += w.x * uq;
+        b1 += w.y * uq;
+        b2 += w.z * uq;
+      }
+      pxab[c] = V4<T>{a0, a1, a2, (T)0};
+      pxab[nv + c] = V4<T>{b0, b1, b2, (T)0};
     }
-    pxab[c] = V4<T>{a0, a1, a2, (T)0};
-    pxab[nv + c] = V4<T>{b0, b1, b2, (T)0};
 #pragma unroll
     for (int i = 0; i < MX; ++i) {
       if ((i < MX - 1 || hex) && ((A >> (BA_OWN + i)) & 1)) {
@@ -1193,7 +1213,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
           P = fz * jump;
         }
         Pown[ed[i]] = P;
-        pxqe[ed[i]] = qe[i];
+        if (pxw) pxqe[ed[i]] = qe[i];
       }
     }
   } else {
@@ -1273,7 +1293,11 @@ __global__ __launch_bounds__(BS) void fast_edge(int remap, int NE, const int32_t
                                                  const T *__restrict__ Poth,
                                                  const V4<T> *__restrict__ pxab,
                                                  const T *__restrict__ pxqe,
-                                                 const V4<T> *__restrict__ pxn, int zero_mom,
+                                                 const V4<T> *__restrict__ pxn,
+                                                 const int32_t *__restrict__ dt_et,
+                                                 const T *__restrict__ dt_ew,
+                                                 const V4<T> *__restrict__ dt_n,
+                                                 const V4<T> *__restrict__ dt_vhq, int zero_mom,
                                                  int stage, T dt_all, const T *__restrict__ dte,
                                                  const T *u0, T *uout, T *__restrict__ accu,
                                                  T *__restrict__ du_out) {
@@ -1295,6 +1319,14 @@ __global__ __launch_bounds__(BS) void fast_edge(int remap, int NE, const int32_t
     const V4<T> A1 = pxab[v1], A2 = pxab[v2], B1 = pxab[nv + v1], B2 = pxab[nv + v2], n = pxn[e];
     const T uhq = (T)0.25 * (n.x * (ql * (A1.x + A2.x) + (B1.x + B2.x)) + n.y * (ql * (A1.y + A2.y) + (B1.y + B2.y)) +
                              n.z * (ql * (A1.z + A2.z) + (B1.z + B2.z)));
+    du = -((phi[v2] - phi[v1]) + Pown[e]) * e_il[e] - uhq;
+  } else if (dt_vhq) {
+    // dtred (coriolis_ed, swm_operators.f90:1310-1329): the two triangles'
+    // vhq interpolated to the edge (distance weights, HTC; the mean, HC),
+    // dotted with the edge normal (projection and sign folded into dt_n)
+    const V4<T> a = dt_vhq[dt_et[e]], b = dt_vhq[dt_et[NE + e]], n = dt_n[e];
+    const T w1 = dt_ew[e], w2 = dt_ew[NE + e];
+    const T uhq = n.x * (w1 * a.x + w2 * b.x) + n.y * (w1 * a.y + w2 * b.y) + n.z * (w1 * a.z + w2 * b.z);
     du = -((phi[v2] - phi[v1]) + Pown[e]) * e_il[e] - uhq;
   } else {
     du = -((phi[v2] - phi[v1]) + (Pown[e] + Poth[e])) * e_il[e];
@@ -1619,6 +1651,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       (s->f_diff || s->f_hyper) ? (const T *)s->x_kt : nullptr,                                    \
       (T *)s->x_cz, s->f_gass, (const T *)s->x_g_w, (const T *)s->x_g_a, (T *)s->x_g_E, (T *)s->x_g_H, \
       s->f_gk ? (const T *)s->x_gk_w : nullptr, (T *)s->x_gk_k,                                     \
+      s->f_dtred ? (const V4<T> *)s->x_dt_w : nullptr, (V4<T> *)s->x_dt_vhq,                        \
       (s->f_gass && !s->f_apvm) ? nullptr : (T *)s->q, s->flag)
   IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
 #undef IMX_TRI
@@ -1660,7 +1693,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
       (const T *)s->x_g_q, s->x_g_out, s->f_pxhc ? (const T *)s->x_px_he : nullptr,                  \
       s->f_gk ? (const T *)s->x_gk_k : nullptr, (T)s->gk_coef, s->f_melv,                          \
-      s->f_melv == 2 ? (const V4<T> *)s->x_ap_wc : nullptr, hs, us, s->href,                         \
+      s->f_melv == 2 ? (const V4<T> *)s->x_ap_wc : nullptr, s->f_dtred ? 1 : 0, hs, us, s->href,     \
       (const T *)s->x_gb,                                                                          \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
@@ -1670,7 +1703,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
   } else if (s->f_nopv) {
     constexpr int OPT = 4;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
-  } else if (s->f_apvm || s->f_px || s->f_gk || s->f_melv) {
+  } else if (s->f_apvm || s->f_px || s->f_gk || s->f_melv || s->f_dtred) {
     constexpr int OPT = 2;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
   } else if (s->f_diff || s->f_hyper) {
@@ -1686,8 +1719,9 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
   fast_edge<T, B><<<grid_bs(NE, B), B, 0, st>>>(                                                \
       s->kremap[2], NE, s->x_e_oth, (const T *)s->x_e_il, (const T *)s->phi, (const T *)s->Pown, \
       (const T *)s->Poth, s->f_px ? (const V4<T> *)s->x_px_ab : nullptr, (const T *)s->x_px_qe,  \
-      (const V4<T> *)s->x_px_n, s->o.testcase <= 1, stage, (T)s->o.dt, (const T *)s->dt_e,      \
-      (const T *)s->u0, uout, (T *)s->accu, (T *)s->du_out)
+      (const V4<T> *)s->x_px_n, s->x_dt_et, (const T *)s->x_dt_ew, (const V4<T> *)s->x_dt_n,      \
+      s->f_dtred ? (const V4<T> *)s->x_dt_vhq : nullptr, s->o.testcase <= 1, stage, (T)s->o.dt,   \
+      (const T *)s->dt_e, (const T *)s->u0, uout, (T *)s->accu, (T *)s->du_out)
   IMX_BS_SWITCH(s->kbs[2], IMX_EDGE)
 #undef IMX_EDGE
   if (evs) (void)hipEventRecord(evs[3], st);
@@ -3173,6 +3207,93 @@ static double gass_subarea(const imx_mesh_desc &d, int ko, int i) {
   return sphtriarea(d.tr_c + 3 * ko, d.v_p + 3 * (d.ed_v[2 * eo] - 1), d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1));
 }
 
+// dtred's Coriolis term on the fused kernels (on = false switches it off):
+// per triangle slot, the vector weight of each edge slot in vhq = q sum_m
+// uh_m W_m, W_m = tg_m leng_m (c x (e_m - c)) / A_t (vector_edhx2tr_perot
+// with the circumcentre c and the edge points e_m: ed%c, HTC; edhx%c, HC);
+// per edge slot, its two triangles, their interpolation weights and the
+// normal the result is dotted with (HTC: minus ed%nr projected at ed%c;
+// HC: edhx%tg), typed T
+template <typename T>
+static int fast_dtred(imx_swm *s, const imx_mesh_desc &d, bool on) {
+  s->f_dtred = false;
+  if (!on) return IMX_OK;
+  const int NE = s->NE, NT = s->NT;
+  const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
+  const bool hc = s->sc_hc;
+  std::vector<int8_t> hct;
+  if (hc) {  // the HC vorticity signs tr%tg s_e, in the layout of ref_tri
+    hct.resize((size_t)3 * NT);
+    HIPTRY(hipMemcpy(hct.data(), s->hc_t, hct.size(), hipMemcpyDeviceToHost), "set_scheme");
+  }
+  std::vector<double> w((size_t)4 * 3 * NT, 0.0), ew((size_t)2 * NE, 0.0), nn((size_t)4 * NE, 0.0);
+  std::vector<int32_t> et((size_t)2 * NE, 0);
+  for (int ko = 0; ko < d.nt; ++ko) {
+    const int t = tn[ko];
+    int r = -1;  // slot m of the fused kernels is reference edge (r + m) % 3
+    for (int l = 0; l < 3; ++l)
+      if (cn[d.tr_v[3 * ko + l] - 1] == t / 2) r = l;
+    if (r < 0) return set_error(IMX_EINVAL, "imx_swm_set_scheme: triangle owner not found");
+    const double *p = d.tr_c + 3 * ko;
+    for (int m = 0; m < 3; ++m) {
+      const int lr = (r + m) % 3;
+      const int eo = d.tr_ed[3 * ko + lr] - 1;
+      const double sg = hc ? (double)hct[(size_t)lr * NT + t] : (double)d.tr_tg[3 * ko + lr];
+      const double *pe = hc ? d.edhx_c + 3 * eo : d.ed_c + 3 * eo;
+      const double f = sg * d.ed_leng[eo] / d.tr_areag[ko];
+      const double v[3] = {pe[0] - p[0], pe[1] - p[1], pe[2] - p[2]};
+      double *o = &w[4 * ((size_t)m * NT + t)];
+      o[0] = f * (p[1] * v[2] - p[2] * v[1]);
+      o[1] = f * (p[2] * v[0] - p[0] * v[2]);
+      o[2] = f * (p[0] * v[1] - p[1] * v[0]);
+    }
+  }
+  for (int eo = 0; eo < d.ne; ++eo) {
+    const int e = en[eo];
+    const int k1 = d.ed_sh[2 * eo] - 1, k2 = d.ed_sh[2 * eo + 1] - 1;
+    et[e] = tn[k1];
+    et[(size_t)NE + e] = tn[k2];
+    if (hc) {
+      const double *t1 = d.tr_c + 3 * k1, *t2 = d.tr_c + 3 * k2, *hxc = d.edhx_c + 3 * eo;
+      double hnr[3], htg[3];
+      cross3(t1, t2, hnr);
+      double n = norm3(hnr);
+      for (int q = 0; q < 3; ++q) hnr[q] = hnr[q] / n;
+      cross3(hnr, hxc, htg);
+      n = norm3(htg);
+      for (int q = 0; q < 3; ++q) nn[(size_t)4 * e + q] = htg[q] / n;
+      ew[e] = 0.5;
+      ew[(size_t)NE + e] = 0.5;
+    } else {
+      const double d1 = arclen(d.ed_c + 3 * eo, d.tr_c + 3 * k1);
+      const double d2 = arclen(d.ed_c + 3 * eo, d.tr_c + 3 * k2);
+      ew[e] = d2 / (d1 + d2);
+      ew[(size_t)NE + e] = d1 / (d1 + d2);
+      double nr[3];
+      cross3(d.v_p + 3 * (d.ed_v[2 * eo] - 1), d.v_p + 3 * (d.ed_v[2 * eo + 1] - 1), nr);
+      const double n = norm3(nr);
+      for (int q = 0; q < 3; ++q) nr[q] = -nr[q] / n;
+      put_projected(&nn[(size_t)4 * e], nr, d.ed_c + 3 * eo);
+    }
+  }
+  Alloc &A = s->A;
+  if (!s->x_dt_w) {
+    s->x_dt_w = A.get<T>((size_t)4 * 3 * NT);
+    s->x_dt_vhq = A.get<T>((size_t)4 * NT);
+    s->x_dt_ew = A.get<T>((size_t)2 * NE);
+    s->x_dt_n = A.get<T>((size_t)4 * NE);
+    s->x_dt_et = A.get<int32_t>((size_t)2 * NE);
+    if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
+  }
+  const std::vector<T> wT = cvt<T>(w), ewT = cvt<T>(ew), nT = cvt<T>(nn);
+  HIPTRY(hipMemcpy(s->x_dt_w, wT.data(), wT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_dt_ew, ewT.data(), ewT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_dt_n, nT.data(), nT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  HIPTRY(hipMemcpy(s->x_dt_et, et.data(), et.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
+  s->f_dtred = true;
+  return IMX_OK;
+}
+
 // Gassmann's kinetic energy on the fused kernels (on = false switches it
 // off): the (circumcentre, edge) sub-areas over A_t in fast_tri's slot order,
 // and the per-stage K_tr, typed T
@@ -3334,17 +3455,24 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       else if (rec == IMX_REC_GASS && w != IMX_WRAP_PXT16) fw = w, gk = true, melv = false;
       else if (rec == IMX_REC_MELV && w == IMX_WRAP_TRSK10) fw = w, gk = false, melv = true;
     }
-    if (fw < 0 || (sc->nopv && (fw != IMX_WRAP_TRSK10 || gk || melv || sc->apvm || sc->k4 > 0.0)))
+    // trsk interpolation with the dtred Coriolis term and the trsk,
+    // Gassmann's or Melvin's kinetic energy
+    bool dtred = false;
+    if (fw < 0 && sint == IMX_SINT_TRSK && cor == IMX_COR_DTRED &&
+        (rec == IMX_REC_TRSK || rec == IMX_REC_GASS || rec == IMX_REC_MELV)) {
+      fw = IMX_WRAP_TRSK10, dtred = true, gk = rec == IMX_REC_GASS, melv = rec == IMX_REC_MELV;
+    }
+    if (fw < 0 || (sc->nopv && (fw != IMX_WRAP_TRSK10 || gk || melv || dtred || sc->apvm || sc->k4 > 0.0)))
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take trsk10 and pxt16 (HTC or HC) with APVM, "
                             "diffusion and hyperdiffusion, gass18 (HTC or HC) with or without APVM, trsk10's "
-                            "or gass18's methods with Gassmann's kinetic energy, trsk10's with Melvin's, and "
-                            "the level model with "
+                            "or gass18's methods with Gassmann's kinetic energy, trsk10's with Melvin's, the "
+                            "dtred Coriolis term with trsk interpolation, and the level model with "
                             "trsk10 (HTC or HC, diffusion allowed); other method triples and the level model "
                             "with other options run on FP64_REF handles");
     // the tables of each option, in turn; every step rewrites the tables
     // and flags of its option whether it is on or off
-    auto apply = [&](const imx_scheme &q, int qw, bool qhc, bool qgk, double qgc, bool qmelv) -> int {
+    auto apply = [&](const imx_scheme &q, int qw, bool qhc, bool qgk, double qgc, bool qmelv, bool qdt) -> int {
       int rc;
       if (qhc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, false, false)))) return rc;
       rc = s->single ? imx::fast_stagger<float>(s, *d, qhc) : imx::fast_stagger<double>(s, *d, qhc);
@@ -3365,12 +3493,14 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
         if (rc) return rc;
       }
       s->f_melv = qmelv ? (qhc ? 2 : 1) : 0;
+      rc = s->single ? imx::fast_dtred<float>(s, *d, qdt) : imx::fast_dtred<double>(s, *d, qdt);
+      if (rc) return rc;
       const int gmode = qw == IMX_WRAP_GASS18 ? (qhc ? 2 : 1) : 0;
       return s->single ? imx::fast_gass18<float>(s, *d, gmode) : imx::fast_gass18<double>(s, *d, gmode);
     };
     imx_scheme r = *sc;
     r.wrap = fw;
-    int rc = apply(r, fw, hc, gk, gcoef, melv);
+    int rc = apply(r, fw, hc, gk, gcoef, melv, dtred);
     s->f_nopv = rc == IMX_OK && r.nopv != 0;
     std::string why;
     if (rc) {
@@ -3378,8 +3508,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       // new options: put the handle back on the default scheme instead
       why = imx_last_error();
       const imx_scheme def{};
-      (void)apply(def, IMX_WRAP_TRSK10, false, false, 1.0, false);
-      r = def, fw = IMX_WRAP_TRSK10, gk = false, melv = false, gcoef = 1.0;
+      (void)apply(def, IMX_WRAP_TRSK10, false, false, 1.0, false, false);
+      r = def, fw = IMX_WRAP_TRSK10, gk = false, melv = false, dtred = false, gcoef = 1.0;
       imx::wrapper_methods(fw, sint, cor, rec);
     }
     s->sc_wrap = fw;

@@ -309,9 +309,10 @@ int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out
    trsk10 and pxt16 (HTC or HC), also spelled as IMX_WRAP_NONE triples, with
    APVM, diffusion and hyperdiffusion, gass18 (HTC or HC) with or without
    APVM, trsk10's or gass18's methods with Gassmann's kinetic energy
-   (IMX_REC_GASS with gasscoef), trsk10's with Melvin's (IMX_REC_MELV), and
-   nopv with trsk10 (HTC or HC, with or without diffusion), and return
-   IMX_ENOTSUP for other method triples and nopv with other options. */
+   (IMX_REC_GASS with gasscoef), trsk10's with Melvin's (IMX_REC_MELV), the
+   dtred Coriolis term with trsk interpolation, and nopv with trsk10 (HTC or
+   HC, with or without diffusion), and return IMX_ENOTSUP for other method
+   triples and nopv with other options. */
 int  imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc);
 int  imx_swm_set_state(imx_swm *s, const double *h, const double *u);
 int  imx_swm_get_state(imx_swm *s, double *h, double *u);

@@ -217,7 +217,7 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         p = os.path.join(td, "par", "swm.par")
         swm = open(p).read().splitlines()
         swm[10] = "none"
-        swm[14] = "dtred"
+        swm[16] = "bary"
         open(p, "w").write("\n".join(swm) + "\n")
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)
@@ -348,6 +348,8 @@ FUSED_METHODS = {
     "rdubos": ({10: "none", 12: "dubos 1.0"}, "ref_run_g3_mrdubos.bin", "vrecdubos"),
     "rmelv": ({10: "none", 12: "melv 1.0"}, "ref_run_g3_mrmelv.bin", "vrecmelv"),
     "rmelvhc": ({8: "HC", 10: "none", 12: "melv 1.0"}, "ref_run_g3_mrmelvhc.bin", "vrecmelv"),
+    "cdtred": ({10: "none", 14: "dtred"}, "ref_run_g3_mcdtred.bin", "crecdtred"),
+    "cdtredhc": ({8: "HC", 10: "none", 14: "dtred"}, "ref_run_g3_mcdtredhc.bin", "crecdtred"),
 }
 
 

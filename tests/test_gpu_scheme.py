@@ -86,7 +86,8 @@ def test_method_refusals():
     """nopv with pered or dtred, gass interpolation with diffusion, bad
     method values; fused handles take the wrappers' triples, trsk10's and
     gass18's methods with Gassmann's kinetic energy, trsk10's with Melvin's,
-    and the level model with trsk10, and refuse the other triples."""
+    the dtred Coriolis term with trsk interpolation, and the level model
+    with trsk10, and refuse the other triples."""
     m = Mesh.icos(3)
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         for kw in ({"wrap": "none", "cor": "pered", "nopv": True}, {"wrap": "none", "cor": "dtred", "nopv": True},
@@ -97,7 +98,7 @@ def test_method_refusals():
     with Swm(m, precision="fp32", dt=900.0) as s:
         for kw in ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "melv"},
                    {"wrap": "none", "nopv": True, "pv": "apvm"},
-                   {"wrap": "gass18", "nopv": True}, {"wrap": "none", "cor": "dtred"}):
+                   {"wrap": "gass18", "nopv": True}, {"wrap": "none", "sint": "gass", "cor": "dtred"}):
             with pytest.raises(capi.ImxError):
                 s.set_scheme(**kw)
         # the trsk10, pxt16 and gass18 triples spelled out run on the fused
@@ -107,6 +108,7 @@ def test_method_refusals():
         s.set_scheme(wrap="none", nopv=True)
         s.set_scheme(wrap="none", rec="gass", gasscoef=0.5)
         s.set_scheme(wrap="none", rec="melv")
+        s.set_scheme(wrap="none", cor="dtred")
         s.set_scheme(wrap="none", sint="gass", cor="gass", rec="dubos")
         s.set_scheme(wrap="none")
 
@@ -158,7 +160,7 @@ def test_scheme_errors():
     m = Mesh.icos(2)
     with Swm(m, precision="fp32", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="FP64_REF"):
-            s.set_scheme(wrap="none", cor="dtred")
+            s.set_scheme(wrap="none", sint="bary")
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
@@ -282,14 +284,19 @@ FAST_VS_REF = {
     "rmelvhc": ({"stag": "HC", "wrap": "none", "rec": "melv"}, {"stag": "HC"}),
     "rmelvmix": ({"wrap": "none", "rec": "melv", "k2": 1e5, "diffus": "align", "pv": "apvm", "pvspar": 0.4},
                  {"k2": 1e5, "diffus": "align", "pv": "apvm", "pvspar": 0.4}),
+    "cdtred": ({"wrap": "none", "cor": "dtred"}, {}),
+    "cdtredhc": ({"stag": "HC", "wrap": "none", "cor": "dtred"}, {"stag": "HC"}),
+    "cdtredmix": ({"wrap": "none", "cor": "dtred", "rec": "gass", "gasscoef": 0.5, "k2": 1e5, "diffus": "const"},
+                  {"k2": 1e5, "diffus": "const"}),
 }
 
 
 @pytest.mark.parametrize("name", sorted(FAST_VS_REF))
 def test_fast_vs_ref(name):
     """The level model (noPV) with trsk10, Gassmann's kinetic energy (gass
-    with gasscoef, dubos) with trsk10's or gass18's methods, and Melvin's
-    kinetic energy (HTC and HC) with trsk10's methods, on the
+    with gasscoef, dubos) with trsk10's or gass18's methods, Melvin's
+    kinetic energy (HTC and HC) with trsk10's methods, and the dtred
+    Coriolis term (HTC and HC) with trsk interpolation, on the
     fused kernels against the FP64_REF handle, which equals the reference's
     own runs with these options bit for bit: the tendency (FP64 < 1e-12,
     FP32 < 5e-5) and 10 steps; the option changes the tendency far above the
