@@ -108,7 +108,9 @@ struct imx_swm {
   // triangles j-1 and j [2][MX][nv], per edge the folded normal of the
   // pered term [NE][4]; per stage A = P(uh) and B = P(uh q) at cells
   // [2][nv][4] and q_e at edges (typed T). The cell Perot weights are x_ap_wc.
-  bool f_px = false;
+  // pxt16's methods on the fused kernels, each on its own: barycentric
+  // interpolation, the pered Coriolis term, the Perot kinetic energy
+  bool f_bary = false, f_pered = false, f_perhx = false;
   // fused handles with gass18 (1 HTC, 2 HC): per triangle slot the areas of
   // (circumcentre, edge m) and their sum [4][NT] and the area [NT]; per
   // edge slot its two triangles [2][NE]; per cell slot the outer edge of

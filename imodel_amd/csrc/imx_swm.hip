@@ -1006,7 +1006,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     const double *__restrict__ c_iaa, const T *__restrict__ c_ika,
     const T *__restrict__ c_dl, const T *__restrict__ c_kw, const T *__restrict__ q,
     const T *__restrict__ kh, const T *__restrict__ cz, const T *__restrict__ hd,
-    const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ pxw,
+    const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ pxw, const V4<T> *__restrict__ pkw,
     const T *__restrict__ pxq, V4<T> *__restrict__ pxab, T *__restrict__ pxqe, const V4<T> *__restrict__ apv,
     const V4<T> *__restrict__ apg, T pvsdt, const T *__restrict__ gq, const int32_t *__restrict__ gout,
     const T *__restrict__ hce, const T *__restrict__ gk_k, T gk_c, int melv,
@@ -1112,11 +1112,11 @@ __global__ __launch_bounds__(BS) void fast_cell(
       K = (T)((double)(km * c_ika[c]) / hcell);
     }
   }
-  if (kAll && pxw) {  // pxt16: |v|^2 / 2 of the cell's Perot velocity (perhx)
+  if (kAll && pkw) {  // perhx (pxt16): |v|^2 / 2 of the cell's Perot velocity
     T v0 = (T)0, v1 = (T)0, v2 = (T)0;
 #pragma unroll
     for (int j = 0; j < MX; ++j) {  // a pentagon's padded slot has a zero weight
-      const V4<T> w = pxw[j * nv + c];
+      const V4<T> w = pkw[j * nv + c];
       v0 += w.x * ue[j];
       v1 += w.y * ue[j];
       v2 += w.z * ue[j];
@@ -1688,7 +1688,8 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->f_diff ? (const T *)s->x_kh : nullptr, (const T *)s->x_cz,                              \
       s->f_hyper ? (const T *)s->x_hy_d : nullptr, s->f_hyper ? (const T *)s->x_hy_z : nullptr,    \
       (s->f_diff || s->f_hyper) ? s->x_kh : nullptr,                                               \
-      s->f_px ? (const V4<T> *)s->x_ap_wc : nullptr, s->f_px ? (const T *)s->x_px_wq : nullptr,    \
+      s->f_pered ? (const V4<T> *)s->x_ap_wc : nullptr, s->f_perhx ? (const V4<T> *)s->x_ap_wc : nullptr, \
+      s->f_bary ? (const T *)s->x_px_wq : nullptr,                                                 \
       (V4<T> *)s->x_px_ab, (T *)s->x_px_qe,                                                        \
       s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
       (const T *)s->x_g_q, s->x_g_out, s->f_pxhc ? (const T *)s->x_px_he : nullptr,                  \
@@ -1703,7 +1704,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
   } else if (s->f_nopv) {
     constexpr int OPT = 4;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
-  } else if (s->f_apvm || s->f_px || s->f_gk || s->f_melv || s->f_dtred) {
+  } else if (s->f_apvm || s->f_bary || s->f_pered || s->f_perhx || s->f_gk || s->f_melv || s->f_dtred) {
     constexpr int OPT = 2;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
   } else if (s->f_diff || s->f_hyper) {
@@ -1718,7 +1719,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
 #define IMX_EDGE(B)                                                                              \
   fast_edge<T, B><<<grid_bs(NE, B), B, 0, st>>>(                                                \
       s->kremap[2], NE, s->x_e_oth, (const T *)s->x_e_il, (const T *)s->phi, (const T *)s->Pown, \
-      (const T *)s->Poth, s->f_px ? (const V4<T> *)s->x_px_ab : nullptr, (const T *)s->x_px_qe,  \
+      (const T *)s->Poth, s->f_pered ? (const V4<T> *)s->x_px_ab : nullptr, (const T *)s->x_px_qe, \
       (const V4<T> *)s->x_px_n, s->x_dt_et, (const T *)s->x_dt_ew, (const V4<T> *)s->x_dt_n,      \
       s->f_dtred ? (const V4<T> *)s->x_dt_vhq : nullptr, s->o.testcase <= 1, stage, (T)s->o.dt,   \
       (const T *)s->dt_e, (const T *)s->u0, uout, (T *)s->accu, (T *)s->du_out)
@@ -1969,7 +1970,7 @@ static int get_vector(imx_swm *s, const double *src, int kind, double *out) {
 // fast_edge then forms -(dphi + P) / |e|, i.e.
 // + (Kd(v2) - Kd(v1)) / (|e| a) - e (Kz(sh2) - Kz(sh1)) / (|e*| a).
 template <typename T>
-static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
+static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc, bool bary) {
   s->f_diff = false;
   s->f_hyper = false;
   if (sc.k2 == 0.0 && sc.k4 == 0.0) return IMX_OK;
@@ -1993,7 +1994,7 @@ static int fast_diffusion(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &
   std::vector<double> hx, tr;
   // with K4 > 0 both terms take the square roots of the hyperdiffusion
   // coefficients, as the reference sets them up
-  diffusion_coef(d, trhx, sc.wrap == IMX_WRAP_PXT16, sc.k2, sc.diffus, sc.k4, sc.hyperdiffus, hx, tr);
+  diffusion_coef(d, trhx, bary, sc.k2, sc.diffus, sc.k4, sc.hyperdiffus, hx, tr);
   std::vector<double> kh(nv, 0.0), kt(NT, 0.0);
   for (int co = 0; co < nv; ++co) kh[cn[co]] = hx[co];
   for (int ko = 0; ko < d.nt; ++ko) kt[tn[ko]] = tr[ko];
@@ -2260,20 +2261,28 @@ static int fast_cell_weights(imx_swm *s, const imx_mesh_desc &d) {
   return IMX_OK;
 }
 
-// pxt16 on the fused kernels: h at triangles from the barycentric
-// coordinates of the circumcentre (t_kap), q at edges weighted by the
-// distances to the two circumcentres, and the tables of the pered term;
-// with HC also h at the Voronoi edge midpoints (fast_hc_he). Switching back
-// restores the kite-area t_kap.
+// pxt16's three methods on the fused kernels, each on its own (bary: the
+// scalar interpolation; pered: the Coriolis term; perhx: the kinetic
+// energy). bary: h at triangles from the barycentric coordinates of the
+// circumcentre (t_kap), q at edges weighted by the distances to the two
+// circumcentres (HC: the plain average) and, with HC, h at the Voronoi edge
+// midpoints (fast_hc_he); switching it off restores the kite-area t_kap.
+// pered: its normals and per-stage buffers. pered and perhx: the cell's
+// Perot weights.
 template <typename T>
-static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) {
-  const bool px = sc.wrap == IMX_WRAP_PXT16;
-  if (!px && !s->f_px) return IMX_OK;
+static int fast_px(imx_swm *s, const imx_mesh_desc &d, bool bary, bool pered, bool perhx) {
+  const bool px = bary;
+  s->f_pered = false;
+  s->f_perhx = false;
+  if (!bary && !s->f_bary) {
+    s->f_pxhc = false;
+    if (!pered && !perhx) return IMX_OK;
+  }
   const int nv = s->nv, NE = s->NE, NT = s->NT;
   const std::vector<int32_t> &cn = s->lay.cell_new, &en = s->lay.edge_new, &tn = s->lay.tri_new;
   std::vector<double> trhx_v, hxtr_v, trskw_v;
   const double *trhx = d.tr_trhx_area;
-  if (!px && !trhx) {
+  if (!px && s->f_bary && !trhx) {  // restoring the kite weights
     trhx_v.assign((size_t)3 * d.nt, 0.0);
     hxtr_v.assign((size_t)MX * d.nv, 0.0);
     trskw_v.assign((size_t)MX * MX * d.nv, 0.0);
@@ -2281,7 +2290,7 @@ static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) 
     trhx = trhx_v.data();
   }
   std::vector<double> tk((size_t)3 * NT, 0.0);
-  for (int ko = 0; ko < d.nt; ++ko) {
+  for (int ko = 0; ko < d.nt && (bary || s->f_bary); ++ko) {
     const int t = tn[ko];
     int r = -1;
     for (int l = 0; l < 3; ++l)
@@ -2292,13 +2301,17 @@ static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) 
     for (int m = 0; m < 3; ++m)
       tk[(size_t)m * NT + t] = px ? b[(r + m) % 3] : trhx[3 * ko + (r + m) % 3] / d.tr_areag[ko];
   }
-  const std::vector<T> tkT = cvt<T>(tk);
-  HIPTRY(hipMemcpy(s->x_t_kap, tkT.data(), tkT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
-  s->f_px = px;
+  if (bary || s->f_bary) {
+    const std::vector<T> tkT = cvt<T>(tk);
+    HIPTRY(hipMemcpy(s->x_t_kap, tkT.data(), tkT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+  }
+  s->f_bary = bary;
   s->f_pxhc = false;
-  if (!px) return IMX_OK;
-  int rc = fast_cell_weights<T>(s, d);
-  if (rc) return rc;
+  if (!bary && !pered && !perhx) return IMX_OK;
+  if (pered || perhx) {
+    const int rc = fast_cell_weights<T>(s, d);
+    if (rc) return rc;
+  }
   // q_e = (d2 q(sh1) + d1 q(sh2)) / (d1 + d2), d_k the arc from ed%c to the
   // circumcentre of sh(k), as weights of ring triangles j-1 and j; with HC
   // the plain average (scalar_tr2ed, swm_operators.f90:152-168)
@@ -2306,7 +2319,7 @@ static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) 
   std::vector<double> wq((size_t)2 * MX * nv, 0.0), pn((size_t)4 * NE, 0.0);
   for (int co = 0; co < nv; ++co) {
     const int c = cn[co], n = d.v_nnb[co];
-    for (int j = 0; j < n; ++j) {
+    for (int j = 0; j < n && bary; ++j) {
       if (hc) {
         wq[(size_t)j * nv + c] = 0.5;
         wq[(size_t)(MX + j) * nv + c] = 0.5;
@@ -2325,9 +2338,10 @@ static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) 
   // pered: HTC, minus the ed%nr component of the tangent-plane average at
   // ed%c; HC, the edhx%tg component of the average at edhx%c
   // (coriolis_ed, swm_operators.f90:1330-1376)
-  std::vector<double> hb(hc ? (size_t)3 * NE : 0, 0.0);
-  std::vector<int32_t> hv(hc ? (size_t)3 * NE : 0, 0);
-  for (int eo = 0; eo < d.ne; ++eo) {
+  const bool hcb = hc && bary;
+  std::vector<double> hb(hcb ? (size_t)3 * NE : 0, 0.0);
+  std::vector<int32_t> hv(hcb ? (size_t)3 * NE : 0, 0);
+  for (int eo = 0; eo < d.ne && (pered || hcb); ++eo) {
     const int e = en[eo];
     if (hc) {
       const double *t1 = d.tr_c + 3 * (d.ed_sh[2 * eo] - 1), *t2 = d.tr_c + 3 * (d.ed_sh[2 * eo + 1] - 1);
@@ -2340,6 +2354,7 @@ static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) 
       nn = norm3(htg);
       for (int q = 0; q < 3; ++q) htg[q] = htg[q] / nn;
       put_projected(&pn[(size_t)4 * e], htg, hxc);
+      if (!hcb) continue;
       // h at the dual edge midpoint: barycentric in the triangle that holds
       // it (initialize_gridprop, swm.f90:1007-1018)
       const int kt = insidetrmesh(d, hxc, d.ed_sh[2 * eo] - 1) ? d.ed_sh[2 * eo] - 1 : d.ed_sh[2 * eo + 1] - 1;
@@ -2365,7 +2380,7 @@ static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) 
     s->x_px_qe = A.get<T>(NE);
     if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
   }
-  if (hc && !s->x_px_hb) {
+  if (hcb && !s->x_px_hb) {
     s->x_px_hb = A.get<T>((size_t)3 * NE);
     s->x_px_hv = A.get<int32_t>((size_t)3 * NE);
     s->x_px_he = A.get<T>(NE);
@@ -2374,12 +2389,14 @@ static int fast_pxt16(imx_swm *s, const imx_mesh_desc &d, const imx_scheme &sc) 
   const std::vector<T> wqT = cvt<T>(wq), pnT = cvt<T>(pn);
   HIPTRY(hipMemcpy(s->x_px_wq, wqT.data(), wqT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
   HIPTRY(hipMemcpy(s->x_px_n, pnT.data(), pnT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
-  if (hc) {
+  if (hcb) {
     const std::vector<T> hbT = cvt<T>(hb);
     HIPTRY(hipMemcpy(s->x_px_hb, hbT.data(), hbT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
     HIPTRY(hipMemcpy(s->x_px_hv, hv.data(), hv.size() * sizeof(int32_t), hipMemcpyHostToDevice), "set_scheme");
   }
-  s->f_pxhc = hc;
+  s->f_pxhc = hcb;
+  s->f_pered = pered;
+  s->f_perhx = perhx;
   return IMX_OK;
 }
 
@@ -3441,66 +3458,72 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     return imx::set_error(IMX_EINVAL, "imx_swm_set_scheme: APVM and gass K need ed_c, tr_c, v_p and v_tr");
   const bool hc = sc->stag == IMX_STAG_HC;
   if (s->fast) {
-    // the fused kernels carry trsk10, pxt16 and gass18 with HTC or HC
-    // a wrapper's triple, or trsk10's or gass18's scalar interpolation and
-    // Coriolis term with Gassmann's kinetic energy (gass with gasscoef;
-    // dubos is gass with 0), or trsk10's with Melvin's
-    int fw = -1;
-    bool gk = false, melv = false;
-    for (int w : {IMX_WRAP_TRSK10, IMX_WRAP_PXT16, IMX_WRAP_GASS18}) {
-      int a, b, c;
-      imx::wrapper_methods(w, a, b, c);
-      if (a != sint || b != cor) continue;
-      if (c == rec) fw = w, gk = false, melv = false;
-      else if (rec == IMX_REC_GASS && w != IMX_WRAP_PXT16) fw = w, gk = true, melv = false;
-      else if (rec == IMX_REC_MELV && w == IMX_WRAP_TRSK10) fw = w, gk = false, melv = true;
+    // the fused kernels take every combination of the scalar interpolation
+    // trsk or bary, the Coriolis term trsk, pered or dtred (dtred with trsk
+    // interpolation) and the kinetic energy trsk, perhx, gass (gasscoef;
+    // dubos is gass with 0) or melv; and gass18's interpolation and
+    // Coriolis term with the trsk or gass kinetic energy; the level model
+    // with trsk10's methods
+    struct Methods {
+      bool gass = false, bary = false, pered = false, perhx = false, gk = false, melv = false, dtred = false;
+    } fm;
+    bool ok = false;
+    if (sint == IMX_SINT_GASS && cor == IMX_COR_GASS) {
+      ok = rec == IMX_REC_TRSK || rec == IMX_REC_GASS;
+      fm.gass = true;
+      fm.gk = rec == IMX_REC_GASS;
+    } else if ((sint == IMX_SINT_TRSK || sint == IMX_SINT_BARY) &&
+               (cor == IMX_COR_TRSK || cor == IMX_COR_PERED || (cor == IMX_COR_DTRED && sint == IMX_SINT_TRSK))) {
+      ok = true;
+      fm.bary = sint == IMX_SINT_BARY;
+      fm.pered = cor == IMX_COR_PERED;
+      fm.dtred = cor == IMX_COR_DTRED;
+      fm.perhx = rec == IMX_REC_PERHX;
+      fm.gk = rec == IMX_REC_GASS;
+      fm.melv = rec == IMX_REC_MELV;
     }
-    // trsk interpolation with the dtred Coriolis term and the trsk,
-    // Gassmann's or Melvin's kinetic energy
-    bool dtred = false;
-    if (fw < 0 && sint == IMX_SINT_TRSK && cor == IMX_COR_DTRED &&
-        (rec == IMX_REC_TRSK || rec == IMX_REC_GASS || rec == IMX_REC_MELV)) {
-      fw = IMX_WRAP_TRSK10, dtred = true, gk = rec == IMX_REC_GASS, melv = rec == IMX_REC_MELV;
-    }
-    if (fw < 0 || (sc->nopv && (fw != IMX_WRAP_TRSK10 || gk || melv || dtred || sc->apvm || sc->k4 > 0.0)))
+    const bool trsk3 = ok && !fm.gass && !fm.bary && !fm.pered && !fm.dtred && !fm.perhx && !fm.gk && !fm.melv;
+    if (!ok || (sc->nopv && (!trsk3 || sc->apvm || sc->k4 > 0.0)))
       return imx::set_error(IMX_ENOTSUP,
-                            "imx_swm_set_scheme: fused handles take trsk10 and pxt16 (HTC or HC) with APVM, "
-                            "diffusion and hyperdiffusion, gass18 (HTC or HC) with or without APVM, trsk10's "
-                            "or gass18's methods with Gassmann's kinetic energy, trsk10's with Melvin's, the "
-                            "dtred Coriolis term with trsk interpolation, and the level model with "
-                            "trsk10 (HTC or HC, diffusion allowed); other method triples and the level model "
-                            "with other options run on FP64_REF handles");
+                            "imx_swm_set_scheme: fused handles take the scalar interpolation trsk or bary, the "
+                            "Coriolis term trsk, pered or dtred (dtred with trsk interpolation) and any kinetic "
+                            "energy, or gass18's interpolation and Coriolis term with the trsk or gass kinetic "
+                            "energy, with APVM, diffusion and hyperdiffusion, HTC or HC; and the level model "
+                            "with trsk10's methods (diffusion allowed); other method triples and the level "
+                            "model with other options run on FP64_REF handles");
+    int fw = fm.gass ? IMX_WRAP_GASS18 : (fm.bary && fm.pered && fm.perhx) ? IMX_WRAP_PXT16 : IMX_WRAP_TRSK10;
     // the tables of each option, in turn; every step rewrites the tables
     // and flags of its option whether it is on or off
-    auto apply = [&](const imx_scheme &q, int qw, bool qhc, bool qgk, double qgc, bool qmelv, bool qdt) -> int {
+    auto apply = [&](const imx_scheme &q, bool qhc, const Methods &m, double qgc) -> int {
       int rc;
       if (qhc && ((rc = imx::scheme_tables(s, d)) || (rc = imx::hc_tables(s, d, false, false)))) return rc;
       rc = s->single ? imx::fast_stagger<float>(s, *d, qhc) : imx::fast_stagger<double>(s, *d, qhc);
       if (rc) return rc;
       s->sc_hc = qhc;
-      rc = s->single ? imx::fast_diffusion<float>(s, *d, q) : imx::fast_diffusion<double>(s, *d, q);
+      rc = s->single ? imx::fast_diffusion<float>(s, *d, q, m.bary) : imx::fast_diffusion<double>(s, *d, q, m.bary);
       if (rc) return rc;
       rc = s->single ? imx::fast_apvm<float>(s, *d, q) : imx::fast_apvm<double>(s, *d, q);
       if (rc) return rc;
-      rc = s->single ? imx::fast_pxt16<float>(s, *d, q) : imx::fast_pxt16<double>(s, *d, q);
+      rc = s->single ? imx::fast_px<float>(s, *d, m.bary, m.pered, m.perhx)
+                     : imx::fast_px<double>(s, *d, m.bary, m.pered, m.perhx);
       if (rc) return rc;
-      rc = s->single ? imx::fast_gassk<float>(s, *d, qgk, qgc) : imx::fast_gassk<double>(s, *d, qgk, qgc);
+      rc = s->single ? imx::fast_gassk<float>(s, *d, m.gk, qgc) : imx::fast_gassk<double>(s, *d, m.gk, qgc);
       if (rc) return rc;
       // Melvin's kinetic energy: HC takes the Perot weights of the cell
       s->f_melv = 0;
-      if (qmelv && qhc) {
+      if (m.melv && qhc) {
         rc = s->single ? imx::fast_cell_weights<float>(s, *d) : imx::fast_cell_weights<double>(s, *d);
         if (rc) return rc;
       }
-      s->f_melv = qmelv ? (qhc ? 2 : 1) : 0;
-      rc = s->single ? imx::fast_dtred<float>(s, *d, qdt) : imx::fast_dtred<double>(s, *d, qdt);
+      s->f_melv = m.melv ? (qhc ? 2 : 1) : 0;
+      rc = s->single ? imx::fast_dtred<float>(s, *d, m.dtred) : imx::fast_dtred<double>(s, *d, m.dtred);
       if (rc) return rc;
-      const int gmode = qw == IMX_WRAP_GASS18 ? (qhc ? 2 : 1) : 0;
+      const int gmode = m.gass ? (qhc ? 2 : 1) : 0;
       return s->single ? imx::fast_gass18<float>(s, *d, gmode) : imx::fast_gass18<double>(s, *d, gmode);
     };
     imx_scheme r = *sc;
     r.wrap = fw;
-    int rc = apply(r, fw, hc, gk, gcoef, melv, dtred);
+    int rc = apply(r, hc, fm, gcoef);
     s->f_nopv = rc == IMX_OK && r.nopv != 0;
     std::string why;
     if (rc) {
@@ -3508,12 +3531,12 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       // new options: put the handle back on the default scheme instead
       why = imx_last_error();
       const imx_scheme def{};
-      (void)apply(def, IMX_WRAP_TRSK10, false, false, 1.0, false, false);
-      r = def, fw = IMX_WRAP_TRSK10, gk = false, melv = false, dtred = false, gcoef = 1.0;
+      (void)apply(def, false, Methods{}, 1.0);
+      r = def, fw = IMX_WRAP_TRSK10, fm = Methods{}, gcoef = 1.0;
       imx::wrapper_methods(fw, sint, cor, rec);
     }
     s->sc_wrap = fw;
-    s->sc_sint = sint, s->sc_cor = cor, s->sc_rec = rec, s->sc_gcoef = gk ? gcoef : 1.0, s->sc_nopv = s->f_nopv;
+    s->sc_sint = sint, s->sc_cor = cor, s->sc_rec = rec, s->sc_gcoef = fm.gk ? gcoef : 1.0, s->sc_nopv = s->f_nopv;
     // the option flags size the Jacobian's stencil superset
     s->sc_apvm = r.apvm != 0;
     s->sc_diff = r.k2 != 0.0;

@@ -264,11 +264,10 @@ class Swm:
         "pered" | "gass" | "dtred"), kinetic energy rec ("trsk" | "perhx" |
         "gass" with gasscoef | "dubos" | "melv") and the level model nopv.
         Every option runs on FP64_REF handles; fused (FP32/FP64) handles take
-        trsk10 and pxt16 (HTC or HC) with APVM, diffusion and hyperdiffusion,
-        gass18 (HTC or HC) with or without APVM, trsk10's or gass18's methods
-        with rec "gass"/"dubos", trsk10's with rec "melv", cor "dtred" with
-        trsk interpolation, nopv with trsk10 (with or without diffusion), and
-        refuse the others."""
+        every triple of sint "trsk"/"bary", cor "trsk"/"pered"/"dtred" (dtred
+        with trsk) and any rec, gass18's sint and cor with rec "trsk"/"gass",
+        and nopv with trsk10's methods, with APVM, diffusion and
+        hyperdiffusion, HTC or HC, and refuse the others."""
         if rec == "dubos":  # Gassmann's K with no trsk part (swmpars, src/swm_data.f90:513-516)
             rec, gasscoef = "gass", 0.0
         sc = capi.Scheme(stag=capi.STAG[stag], wrap=capi.WRAP[wrap], apvm=int(pv == "apvm"), pvspar=float(pvspar), k2=float(k2),

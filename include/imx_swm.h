@@ -305,14 +305,14 @@ void imx_swm_default_opts(imx_swm_opts *o);
 int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out);
 /* Scheme options (imx_scheme); d is the mesh the handle was created on
    (the coefficients and the APVM reconstruction read its geometry).
-   FP64_REF handles take every option. Fused (FP32/FP64) handles take
-   trsk10 and pxt16 (HTC or HC), also spelled as IMX_WRAP_NONE triples, with
-   APVM, diffusion and hyperdiffusion, gass18 (HTC or HC) with or without
-   APVM, trsk10's or gass18's methods with Gassmann's kinetic energy
-   (IMX_REC_GASS with gasscoef), trsk10's with Melvin's (IMX_REC_MELV), the
-   dtred Coriolis term with trsk interpolation, and nopv with trsk10 (HTC or
-   HC, with or without diffusion), and return IMX_ENOTSUP for other method
-   triples and nopv with other options. */
+   FP64_REF handles take every option. Fused (FP32/FP64) handles take, HTC
+   or HC, with APVM, diffusion and hyperdiffusion, every method triple of
+   the scalar interpolation trsk or bary, the Coriolis term trsk, pered or
+   dtred (dtred with trsk interpolation) and any kinetic energy; gass18's
+   interpolation and Coriolis term with the trsk or gass kinetic energy
+   (gass18 itself refuses diffusion, as the reference does); and nopv with
+   trsk10's methods (with or without diffusion); they return IMX_ENOTSUP
+   for other method triples and nopv with other options. */
 int  imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc);
 int  imx_swm_set_state(imx_swm *s, const double *h, const double *u);
 int  imx_swm_get_state(imx_swm *s, double *h, double *u);

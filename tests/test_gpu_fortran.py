@@ -217,7 +217,7 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         p = os.path.join(td, "par", "swm.par")
         swm = open(p).read().splitlines()
         swm[10] = "none"
-        swm[16] = "bary"
+        swm[14] = "gass"
         open(p, "w").write("\n".join(swm) + "\n")
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)
@@ -350,6 +350,7 @@ FUSED_METHODS = {
     "rmelvhc": ({8: "HC", 10: "none", 12: "melv 1.0"}, "ref_run_g3_mrmelvhc.bin", "vrecmelv"),
     "cdtred": ({10: "none", 14: "dtred"}, "ref_run_g3_mcdtred.bin", "crecdtred"),
     "cdtredhc": ({8: "HC", 10: "none", 14: "dtred"}, "ref_run_g3_mcdtredhc.bin", "crecdtred"),
+    "baryperhx": ({10: "none", 12: "perhx 1.0", 16: "bary"}, "ref_run_g3_mbaryperhx.bin", "vrecperhx"),
 }
 
 

@@ -84,10 +84,10 @@ def test_methods_match_reference(golden, name, sfc):
 
 def test_method_refusals():
     """nopv with pered or dtred, gass interpolation with diffusion, bad
-    method values; fused handles take the wrappers' triples, trsk10's and
-    gass18's methods with Gassmann's kinetic energy, trsk10's with Melvin's,
-    the dtred Coriolis term with trsk interpolation, and the level model
-    with trsk10, and refuse the other triples."""
+    method values; fused handles take the triples of trsk or bary
+    interpolation, trsk, pered or dtred Coriolis (dtred with trsk) and any
+    kinetic energy, gass18's methods with the trsk or gass kinetic energy,
+    and the level model with trsk10, and refuse the other triples."""
     m = Mesh.icos(3)
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         for kw in ({"wrap": "none", "cor": "pered", "nopv": True}, {"wrap": "none", "cor": "dtred", "nopv": True},
@@ -109,6 +109,7 @@ def test_method_refusals():
         s.set_scheme(wrap="none", rec="gass", gasscoef=0.5)
         s.set_scheme(wrap="none", rec="melv")
         s.set_scheme(wrap="none", cor="dtred")
+        s.set_scheme(wrap="none", sint="bary", cor="pered", rec="melv")
         s.set_scheme(wrap="none", sint="gass", cor="gass", rec="dubos")
         s.set_scheme(wrap="none")
 
@@ -160,7 +161,7 @@ def test_scheme_errors():
     m = Mesh.icos(2)
     with Swm(m, precision="fp32", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="FP64_REF"):
-            s.set_scheme(wrap="none", sint="bary")
+            s.set_scheme(wrap="none", cor="gass")
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
@@ -288,15 +289,22 @@ FAST_VS_REF = {
     "cdtredhc": ({"stag": "HC", "wrap": "none", "cor": "dtred"}, {"stag": "HC"}),
     "cdtredmix": ({"wrap": "none", "cor": "dtred", "rec": "gass", "gasscoef": 0.5, "k2": 1e5, "diffus": "const"},
                   {"k2": 1e5, "diffus": "const"}),
+    "baryperhx": ({"wrap": "none", "sint": "bary", "rec": "perhx"}, {}),
+    "barytrsk": ({"wrap": "none", "sint": "bary"}, {}),
+    "peredtrsk": ({"wrap": "none", "cor": "pered"}, {}),
+    "hcpered": ({"stag": "HC", "wrap": "none", "cor": "pered", "rec": "perhx"}, {"stag": "HC"}),
+    "pxgk": ({"wrap": "none", "sint": "bary", "cor": "pered", "rec": "gass", "gasscoef": 0.5}, {}),
+    "hcbarymelv": ({"stag": "HC", "wrap": "none", "sint": "bary", "rec": "melv"}, {"stag": "HC"}),
+    "dtredperhx": ({"wrap": "none", "cor": "dtred", "rec": "perhx", "k2": 1e5, "diffus": "diam"},
+                   {"k2": 1e5, "diffus": "diam"}),
 }
 
 
 @pytest.mark.parametrize("name", sorted(FAST_VS_REF))
 def test_fast_vs_ref(name):
-    """The level model (noPV) with trsk10, Gassmann's kinetic energy (gass
-    with gasscoef, dubos) with trsk10's or gass18's methods, Melvin's
-    kinetic energy (HTC and HC) with trsk10's methods, and the dtred
-    Coriolis term (HTC and HC) with trsk interpolation, on the
+    """The level model (noPV) with trsk10, and method triples other than the
+    wrappers' (Gassmann's and Melvin's kinetic energies, the dtred Coriolis
+    term, pxt16's methods mixed with trsk10's), HTC and HC, on the
     fused kernels against the FP64_REF handle, which equals the reference's
     own runs with these options bit for bit: the tendency (FP64 < 1e-12,
     FP32 < 5e-5) and 10 steps; the option changes the tendency far above the
@@ -341,9 +349,9 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"wrap": "none", "sint": "bary"}, {"wrap": "none", "sint": "bary", "cor": "pered", "rec": "gass"},
-                   {"wrap": "none", "sint": "gass", "cor": "dtred"}):
-            with pytest.raises(capi.ImxError, match="fused handles take trsk10"):
+        for kw in ({"wrap": "none", "sint": "gass", "cor": "trsk"}, {"wrap": "none", "sint": "trsk", "cor": "gass"},
+                   {"wrap": "none", "sint": "bary", "cor": "dtred"}):
+            with pytest.raises(capi.ImxError, match="fused handles take"):
                 s.set_scheme(**kw)
 
 
