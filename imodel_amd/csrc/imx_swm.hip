@@ -724,7 +724,8 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
                                                 const T *__restrict__ g_a, T *__restrict__ gE,
                                                 T *__restrict__ gH, const T *__restrict__ gk_w,
                                                 T *__restrict__ gk_k, const V4<T> *__restrict__ dt_w,
-                                                V4<T> *__restrict__ dt_vhq, T *__restrict__ q,
+                                                V4<T> *__restrict__ dt_vhq, const T *__restrict__ dt_he,
+                                                T *__restrict__ q,
                                                 int *__restrict__ flag) {
   const int t = ent_index<BS>(remap);
   if (t >= NT) return;
@@ -801,11 +802,19 @@ __global__ __launch_bounds__(BS) void fast_tri(int remap, int NT, const int32_t 
     // (vector_edhx2tr_perot, swm.f90:2501-2503), linear in uh: the host
     // folds signs, lengths, the cross product with the circumcentre and
     // 1/A_t into one vector per edge slot; uh takes h at the edge as the
-    // mean of its two cells
-    const T ho = h[o], ha = h[a], hb = h[b];
-    const T uh0 = u[e0] * (href + (T)0.5 * (ho + ha));
-    const T uh1 = u[e1] * (href + (T)0.5 * (ha + hb));
-    const T uh2 = u[e2] * (href + (T)0.5 * (hb + ho));
+    // mean of its two cells, or with bary on HC the barycentric depth
+    // fast_hc_he wrote (dt_he)
+    T uh0, uh1, uh2;
+    if (dt_he) {
+      uh0 = u[e0] * dt_he[e0];
+      uh1 = u[e1] * dt_he[e1];
+      uh2 = u[e2] * dt_he[e2];
+    } else {
+      const T ho = h[o], ha = h[a], hb = h[b];
+      uh0 = u[e0] * (href + (T)0.5 * (ho + ha));
+      uh1 = u[e1] * (href + (T)0.5 * (ha + hb));
+      uh2 = u[e2] * (href + (T)0.5 * (hb + ho));
+    }
     const V4<T> w0 = dt_w[t], w1 = dt_w[NT + t], w2 = dt_w[2 * NT + t];
     dt_vhq[t] = V4<T>{qq * (uh0 * w0.x + uh1 * w1.x + uh2 * w2.x), qq * (uh0 * w0.y + uh1 * w1.y + uh2 * w2.y),
                       qq * (uh0 * w0.z + uh1 * w1.z + uh2 * w2.z), (T)0};
@@ -1644,6 +1653,9 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
   hipStream_t st = s->stream;
   const T href = (T)s->href;
   if (evs) (void)hipEventRecord(evs[0], st);
+  if (s->f_pxhc)  // before fast_tri, whose dtred term reads these depths
+    fast_hc_he<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(s->kremap[2], NE, s->x_e_oth, (const T *)s->x_px_hb,
+                                                         s->x_px_hv, hs, href, (T *)s->x_px_he);
 #define IMX_TRI(B)                                                                          \
   fast_tri<T, B><<<grid_bs(NT, B), B, 0, st>>>(                                            \
       s->kremap[0], NT, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta, (const T *)s->x_t_kap, \
@@ -1652,6 +1664,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       (T *)s->x_cz, s->f_gass, (const T *)s->x_g_w, (const T *)s->x_g_a, (T *)s->x_g_E, (T *)s->x_g_H, \
       s->f_gk ? (const T *)s->x_gk_w : nullptr, (T *)s->x_gk_k,                                     \
       s->f_dtred ? (const V4<T> *)s->x_dt_w : nullptr, (V4<T> *)s->x_dt_vhq,                        \
+      (s->f_dtred && s->f_pxhc) ? (const T *)s->x_px_he : nullptr,                                  \
       (s->f_gass && !s->f_apvm) ? nullptr : (T *)s->q, s->flag)
   IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
 #undef IMX_TRI
@@ -1672,9 +1685,6 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
     fast_apvm_pre<T, 256><<<grid_bs(nv, 256), 256, 0, st>>>(
         s->kremap[1], nv, NT, s->x_c_nb, s->x_c_bitsA, s->x_c_bitsB, (const V4<T> *)s->x_ap_wc, s->x_ap_tnb,
         (const V4<T> *)s->x_ap_wt, us, (const T *)s->q, (V4<T> *)s->x_ap_v, (V4<T> *)s->x_ap_g);
-  if (s->f_pxhc)
-    fast_hc_he<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(s->kremap[2], NE, s->x_e_oth, (const T *)s->x_px_hb,
-                                                         s->x_px_hv, hs, href, (T *)s->x_px_he);
   if (s->f_gass)  // after fast_apvm_pre, whose vectors correct the rhombus PV
     fast_gass_q<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(
         s->kremap[2], NE, s->x_g_et, (const T *)s->x_g_E, (const T *)s->x_g_H, s->x_e_oth,
@@ -3459,8 +3469,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   const bool hc = sc->stag == IMX_STAG_HC;
   if (s->fast) {
     // the fused kernels take every combination of the scalar interpolation
-    // trsk or bary, the Coriolis term trsk, pered or dtred (dtred with trsk
-    // interpolation) and the kinetic energy trsk, perhx, gass (gasscoef;
+    // trsk or bary, the Coriolis term trsk, pered or dtred and the kinetic
+    // energy trsk, perhx, gass (gasscoef;
     // dubos is gass with 0) or melv; and gass18's interpolation and
     // Coriolis term with the trsk or gass kinetic energy; the level model
     // with trsk10's methods
@@ -3473,7 +3483,7 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       fm.gass = true;
       fm.gk = rec == IMX_REC_GASS;
     } else if ((sint == IMX_SINT_TRSK || sint == IMX_SINT_BARY) &&
-               (cor == IMX_COR_TRSK || cor == IMX_COR_PERED || (cor == IMX_COR_DTRED && sint == IMX_SINT_TRSK))) {
+               (cor == IMX_COR_TRSK || cor == IMX_COR_PERED || cor == IMX_COR_DTRED)) {
       ok = true;
       fm.bary = sint == IMX_SINT_BARY;
       fm.pered = cor == IMX_COR_PERED;
@@ -3486,7 +3496,7 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     if (!ok || (sc->nopv && (!trsk3 || sc->apvm || sc->k4 > 0.0)))
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take the scalar interpolation trsk or bary, the "
-                            "Coriolis term trsk, pered or dtred (dtred with trsk interpolation) and any kinetic "
+                            "Coriolis term trsk, pered or dtred and any kinetic "
                             "energy, or gass18's interpolation and Coriolis term with the trsk or gass kinetic "
                             "energy, with APVM, diffusion and hyperdiffusion, HTC or HC; and the level model "
                             "with trsk10's methods (diffusion allowed); other method triples and the level "

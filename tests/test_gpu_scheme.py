@@ -85,8 +85,7 @@ def test_methods_match_reference(golden, name, sfc):
 def test_method_refusals():
     """nopv with pered or dtred, gass interpolation with diffusion, bad
     method values; fused handles take the triples of trsk or bary
-    interpolation, trsk, pered or dtred Coriolis (dtred with trsk) and any
-    kinetic energy, gass18's methods with the trsk or gass kinetic energy,
+    interpolation, trsk, pered or dtred Coriolis and any kinetic energy, gass18's methods with the trsk or gass kinetic energy,
     and the level model with trsk10, and refuse the other triples."""
     m = Mesh.icos(3)
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
@@ -109,6 +108,7 @@ def test_method_refusals():
         s.set_scheme(wrap="none", rec="gass", gasscoef=0.5)
         s.set_scheme(wrap="none", rec="melv")
         s.set_scheme(wrap="none", cor="dtred")
+        s.set_scheme(stag="HC", wrap="none", sint="bary", cor="dtred")
         s.set_scheme(wrap="none", sint="bary", cor="pered", rec="melv")
         s.set_scheme(wrap="none", sint="gass", cor="gass", rec="dubos")
         s.set_scheme(wrap="none")
@@ -297,6 +297,11 @@ FAST_VS_REF = {
     "hcbarymelv": ({"stag": "HC", "wrap": "none", "sint": "bary", "rec": "melv"}, {"stag": "HC"}),
     "dtredperhx": ({"wrap": "none", "cor": "dtred", "rec": "perhx", "k2": 1e5, "diffus": "diam"},
                    {"k2": 1e5, "diffus": "diam"}),
+    "barydtred": ({"wrap": "none", "sint": "bary", "cor": "dtred"}, {}),
+    "hcbarydtred": ({"stag": "HC", "wrap": "none", "sint": "bary", "cor": "dtred", "rec": "perhx"},
+                    {"stag": "HC"}),
+    "hcbarydtredk2": ({"stag": "HC", "wrap": "none", "sint": "bary", "cor": "dtred", "k2": 1e5,
+                       "diffus": "const"}, {"stag": "HC", "k2": 1e5, "diffus": "const"}),
 }
 
 
@@ -350,7 +355,7 @@ def test_fast_scheme_refusals():
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
         for kw in ({"wrap": "none", "sint": "gass", "cor": "trsk"}, {"wrap": "none", "sint": "trsk", "cor": "gass"},
-                   {"wrap": "none", "sint": "bary", "cor": "dtred"}):
+                   {"wrap": "none", "sint": "gass", "cor": "dtred"}):
             with pytest.raises(capi.ImxError, match="fused handles take"):
                 s.set_scheme(**kw)
 
