@@ -1007,7 +1007,8 @@ __global__ __launch_bounds__(BS) void fast_hyper_c(int remap, int nv, int NT, co
 // OPT selects the code compiled in, and with it the register footprint:
 // 0 the plain scheme, 1 with diffusion and hyperdiffusion, 2 with every
 // option (APVM and pxt16 too), 3 gass18, 4 the level model (with
-// diffusion). Pointers of options left out are ignored.
+// diffusion), 5 gass18 with the Perot or Melvin kinetic energy. Pointers of
+// options left out are ignored.
 template <typename T, int BS, int OPT>
 __global__ __launch_bounds__(BS) void fast_cell(
     int remap, int nv, const int32_t *__restrict__ c_nb, const int32_t *__restrict__ c_bitsA,
@@ -1025,7 +1026,8 @@ __global__ __launch_bounds__(BS) void fast_cell(
     double *__restrict__ dh_out) {
   // the level model (OPT 4): the TRiSK sum takes u and eta at edges in
   // place of uh and q
-  constexpr bool kDiff = OPT == 1 || OPT == 2 || OPT == 4, kAll = OPT == 2, kGass = OPT == 3, kNopv = OPT == 4;
+  constexpr bool kDiff = OPT == 1 || OPT == 2 || OPT == 4, kAll = OPT == 2, kGass = OPT == 3 || OPT == 5,
+                 kNopv = OPT == 4, kKe = OPT == 2 || OPT == 5;
   const int c = ent_index<BS>(remap);
   if (c >= nv) return;
   const int A = c_bitsA[c];
@@ -1094,7 +1096,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
       if ((j < MX - 1) || hex) kk += ((T)1 - gk_c) * R[j] * gk_k[tr[j]];
     K = kk;
   }
-  if (kAll && melv) {
+  if (kKe && melv) {
     // Melvin's kinetic energy (kinetic_energy_hx, swm_operators.f90:
     // 1174-1197): HTC, sum_j kw_j u_j (u_j h_e) / (h sum_j kw_j); HC, the
     // Perot vectors of uh and u dotted, over 2 h
@@ -1121,7 +1123,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
       K = (T)((double)(km * c_ika[c]) / hcell);
     }
   }
-  if (kAll && pkw) {  // perhx (pxt16): |v|^2 / 2 of the cell's Perot velocity
+  if (kKe && pkw) {  // perhx (pxt16): |v|^2 / 2 of the cell's Perot velocity
     T v0 = (T)0, v1 = (T)0, v2 = (T)0;
 #pragma unroll
     for (int j = 0; j < MX; ++j) {  // a pentagon's padded slot has a zero weight
@@ -1708,7 +1710,10 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       (const T *)s->x_gb,                                                                          \
       (T)s->o.grav, (T)(1.0 / s->o.erad), (T *)s->phi, (T *)s->Pown, (T *)s->Poth, stage,       \
       s->o.dt, s->dt_c, (const T *)s->h0, hout, (T *)s->acch, s->dh_out)
-  if (s->f_gass) {
+  if (s->f_gass && (s->f_melv || s->f_perhx)) {
+    constexpr int OPT = 5;
+    IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
+  } else if (s->f_gass) {
     constexpr int OPT = 3;
     IMX_BS_SWITCH(s->kbs[1], IMX_CELL)
   } else if (s->f_nopv) {
@@ -3470,18 +3475,19 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
   if (s->fast) {
     // the fused kernels take every combination of the scalar interpolation
     // trsk or bary, the Coriolis term trsk, pered or dtred and the kinetic
-    // energy trsk, perhx, gass (gasscoef;
-    // dubos is gass with 0) or melv; and gass18's interpolation and
-    // Coriolis term with the trsk or gass kinetic energy; the level model
-    // with trsk10's methods
+    // energy trsk, perhx, gass (gasscoef; dubos is gass with 0) or melv;
+    // gass18's interpolation and Coriolis term with any of these kinetic
+    // energies; the level model with trsk10's methods
     struct Methods {
       bool gass = false, bary = false, pered = false, perhx = false, gk = false, melv = false, dtred = false;
     } fm;
     bool ok = false;
     if (sint == IMX_SINT_GASS && cor == IMX_COR_GASS) {
-      ok = rec == IMX_REC_TRSK || rec == IMX_REC_GASS;
+      ok = true;
       fm.gass = true;
       fm.gk = rec == IMX_REC_GASS;
+      fm.perhx = rec == IMX_REC_PERHX;
+      fm.melv = rec == IMX_REC_MELV;
     } else if ((sint == IMX_SINT_TRSK || sint == IMX_SINT_BARY) &&
                (cor == IMX_COR_TRSK || cor == IMX_COR_PERED || cor == IMX_COR_DTRED)) {
       ok = true;
@@ -3497,7 +3503,7 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take the scalar interpolation trsk or bary, the "
                             "Coriolis term trsk, pered or dtred and any kinetic "
-                            "energy, or gass18's interpolation and Coriolis term with the trsk or gass kinetic "
+                            "energy, or gass18's interpolation and Coriolis term with any kinetic "
                             "energy, with APVM, diffusion and hyperdiffusion, HTC or HC; and the level model "
                             "with trsk10's methods (diffusion allowed); other method triples and the level "
                             "model with other options run on FP64_REF handles");

@@ -488,9 +488,9 @@ contains
       sc%recon = rec_code()
       sc%gasscoef = gass_coef()
       sc%nopv = merge(1, 0, inopv > 0)
-      ! the fused fp32/fp64 kernels carry the wrappers, trsk10's and
-      ! gass18's methods with Gassmann's kinetic energy and the level model
-      ! with trsk10; the library names what they refuse
+      ! the fused fp32/fp64 kernels carry the wrappers, most method
+      ! triples and the level model with trsk10; the library names what
+      ! they refuse
       rc = imx_swm_set_scheme(hnd, mdesc, sc)
       if (rc == IMX_ENOTSUP .and. prec /= IMX_PREC_FP64_REF) then
         write(*, '(a)') " ERROR: these scheme options run on the fp64_ref path (par/swm_gpu.par):"

@@ -85,8 +85,9 @@ def test_methods_match_reference(golden, name, sfc):
 def test_method_refusals():
     """nopv with pered or dtred, gass interpolation with diffusion, bad
     method values; fused handles take the triples of trsk or bary
-    interpolation, trsk, pered or dtred Coriolis and any kinetic energy, gass18's methods with the trsk or gass kinetic energy,
-    and the level model with trsk10, and refuse the other triples."""
+    interpolation, trsk, pered or dtred Coriolis and any kinetic energy,
+    gass18's methods with any kinetic energy, and the level model with
+    trsk10, and refuse the other triples."""
     m = Mesh.icos(3)
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         for kw in ({"wrap": "none", "cor": "pered", "nopv": True}, {"wrap": "none", "cor": "dtred", "nopv": True},
@@ -95,7 +96,7 @@ def test_method_refusals():
                 s.set_scheme(**kw)
         s.set_scheme(wrap="none", cor="gass", nopv=True)
     with Swm(m, precision="fp32", dt=900.0) as s:
-        for kw in ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "melv"},
+        for kw in ({"wrap": "none", "sint": "gass", "cor": "trsk", "rec": "melv"},
                    {"wrap": "none", "nopv": True, "pv": "apvm"},
                    {"wrap": "gass18", "nopv": True}, {"wrap": "none", "sint": "gass", "cor": "dtred"}):
             with pytest.raises(capi.ImxError):
@@ -111,6 +112,7 @@ def test_method_refusals():
         s.set_scheme(stag="HC", wrap="none", sint="bary", cor="dtred")
         s.set_scheme(wrap="none", sint="bary", cor="pered", rec="melv")
         s.set_scheme(wrap="none", sint="gass", cor="gass", rec="dubos")
+        s.set_scheme(stag="HC", wrap="none", sint="gass", cor="gass", rec="melv")
         s.set_scheme(wrap="none")
 
 
@@ -298,6 +300,13 @@ FAST_VS_REF = {
     "dtredperhx": ({"wrap": "none", "cor": "dtred", "rec": "perhx", "k2": 1e5, "diffus": "diam"},
                    {"k2": 1e5, "diffus": "diam"}),
     "barydtred": ({"wrap": "none", "sint": "bary", "cor": "dtred"}, {}),
+    "gassperhx": ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "perhx"}, {"wrap": "gass18"}),
+    "gassmelv": ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "melv", "pv": "apvm", "pvspar": 0.4},
+                 {"wrap": "gass18", "pv": "apvm", "pvspar": 0.4}),
+    "hcgassmelv": ({"stag": "HC", "wrap": "none", "sint": "gass", "cor": "gass", "rec": "melv"},
+                   {"stag": "HC", "wrap": "gass18"}),
+    "hcgassperhx": ({"stag": "HC", "wrap": "none", "sint": "gass", "cor": "gass", "rec": "perhx", "pv": "apvm",
+                     "pvspar": 0.4}, {"stag": "HC", "wrap": "gass18", "pv": "apvm", "pvspar": 0.4}),
     "hcbarydtred": ({"stag": "HC", "wrap": "none", "sint": "bary", "cor": "dtred", "rec": "perhx"},
                     {"stag": "HC"}),
     "hcbarydtredk2": ({"stag": "HC", "wrap": "none", "sint": "bary", "cor": "dtred", "k2": 1e5,
