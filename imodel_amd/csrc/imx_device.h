@@ -117,6 +117,8 @@ struct imx_swm {
   // ring triangle j [MX][nv]; per stage E and H at triangles and q at edges
   // (typed T)
   int f_gass = 0;
+  bool f_glink = false;              // gass mode: the linking-edge PV of the gass Coriolis term
+  bool f_gtrsk = false;              // gass mode: the edge PV from trsk interpolation
   void *x_g_w = nullptr, *x_g_a = nullptr, *x_g_E = nullptr, *x_g_H = nullptr, *x_g_q = nullptr;
   int32_t *x_g_et = nullptr, *x_g_out = nullptr;
   void *x_px_wq = nullptr, *x_px_n = nullptr, *x_px_ab = nullptr, *x_px_qe = nullptr;

@@ -838,7 +838,8 @@ __global__ __launch_bounds__(BS) void fast_hc_he(int remap, int NE, const int32_
 }
 
 // gass18's PV at every edge slot from the rhombus of its two triangles:
-// q = (E1 + E2) / (H1 + H2); with APVM (apv non-null) minus
+// q = (E1 + E2) / (H1 + H2), or with trsk interpolation (qtr non-null) the
+// mean of the two triangles' PV; with APVM (apv non-null) minus
 // pvspar dt (grad q(t1) + grad q(t2)) . (v(c1) + v(c2)) (apvm,
 // swm_operators.f90:1497-1501), from fast_apvm_pre's vectors
 template <typename T, int BS>
@@ -847,14 +848,19 @@ __global__ __launch_bounds__(BS) void fast_gass_q(int remap, int NE, const int32
                                                    const int32_t *__restrict__ e_oth,
                                                    const V4<T> *__restrict__ apv,
                                                    const V4<T> *__restrict__ apg, T pvsdt,
-                                                   T *__restrict__ gq) {
+                                                   const T *__restrict__ qtr, T *__restrict__ gq) {
   const int l = ent_index<BS>(remap);
   if (l >= NE) return;
   const int t1 = e_t[l];
   if (t1 < 0) return;
   const int t2 = e_t[NE + l];
-  const T H = gH[t1] + gH[t2];
-  T qe = (H != (T)0) ? (gE[t1] + gE[t2]) / H : (T)0;
+  T qe;
+  if (qtr) {  // gass Coriolis term with trsk interpolation: the plain average (scalar_tr2ed)
+    qe = (T)0.5 * (qtr[t1] + qtr[t2]);
+  } else {
+    const T H = gH[t1] + gH[t2];
+    qe = (H != (T)0) ? (gE[t1] + gE[t2]) / H : (T)0;
+  }
   if (apv) {
     const int c1 = l / 3, c2 = e_oth[l] & OTH_MASK;
     const V4<T> g1 = apg[t1], g2 = apg[t2], v1 = apv[c1], v2 = apv[c2];
@@ -1019,7 +1025,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
     const T *__restrict__ hz, const void *tang, const V4<T> *__restrict__ pxw, const V4<T> *__restrict__ pkw,
     const T *__restrict__ pxq, V4<T> *__restrict__ pxab, T *__restrict__ pxqe, const V4<T> *__restrict__ apv,
     const V4<T> *__restrict__ apg, T pvsdt, const T *__restrict__ gq, const int32_t *__restrict__ gout,
-    const T *__restrict__ hce, const T *__restrict__ gk_k, T gk_c, int melv,
+    const T *__restrict__ hce, int glink, const T *__restrict__ gk_k, T gk_c, int melv,
     const V4<T> *__restrict__ mw, int dtred, const T *__restrict__ hs, const T *__restrict__ us, double href, const T *__restrict__ gb, T grav,
     T inv_a, T *__restrict__ phi, T *__restrict__ Pown, T *__restrict__ Poth, int stage, double dt_all,
     const double *__restrict__ dtc, const T *h0, T *hout, T *__restrict__ acch,
@@ -1158,7 +1164,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
 #pragma unroll
     for (int j = 0; j < MX; ++j) {
       qe[j] = gq[ed[j]];
-      qo[j] = gq[gout[j * nv + c]];
+      qo[j] = glink ? gq[gout[j * nv + c]] : (T)0;
     }
   } else if (kAll && pxq) {  // pxt16: weighted by the distances to the circumcentres
 #pragma unroll
@@ -1243,7 +1249,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
       if (j != i) {
         const T w = ((j > i) ? (S[j] - S[i]) : (tot - S[i] + S[j])) - (T)0.5;
         T qs = qe[i] + qe[j];
-        if (kGass) {
+        if (kGass && glink) {
           // coriolis_ed gass (swm_operators.f90:1378-1403): the PV of the
           // edge linking i and j (gethxedgeconnection): the outer edge of
           // the ring triangle between neighbours, the ring edge between
@@ -1663,11 +1669,11 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->kremap[0], NT, s->x_t_ab, s->x_t_eb, (const T *)s->x_t_ceta, (const T *)s->x_t_kap, \
       (const T *)s->x_t_f, us, hs, href, s->o.testcase >= 2, s->f_lin ? 1 : 0, s->f_nopv ? 1 : 0,  \
       (s->f_diff || s->f_hyper) ? (const T *)s->x_kt : nullptr,                                    \
-      (T *)s->x_cz, s->f_gass, (const T *)s->x_g_w, (const T *)s->x_g_a, (T *)s->x_g_E, (T *)s->x_g_H, \
+      (T *)s->x_cz, s->f_gtrsk ? 0 : s->f_gass, (const T *)s->x_g_w, (const T *)s->x_g_a, (T *)s->x_g_E, (T *)s->x_g_H, \
       s->f_gk ? (const T *)s->x_gk_w : nullptr, (T *)s->x_gk_k,                                     \
       s->f_dtred ? (const V4<T> *)s->x_dt_w : nullptr, (V4<T> *)s->x_dt_vhq,                        \
       (s->f_dtred && s->f_pxhc) ? (const T *)s->x_px_he : nullptr,                                  \
-      (s->f_gass && !s->f_apvm) ? nullptr : (T *)s->q, s->flag)
+      (s->f_gass && !s->f_apvm && !s->f_gtrsk) ? nullptr : (T *)s->q, s->flag)
   IMX_BS_SWITCH(s->kbs[0], IMX_TRI)
 #undef IMX_TRI
   if (s->f_hyper) {
@@ -1691,7 +1697,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
     fast_gass_q<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(
         s->kremap[2], NE, s->x_g_et, (const T *)s->x_g_E, (const T *)s->x_g_H, s->x_e_oth,
         s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt),
-        (T *)s->x_g_q);
+        s->f_gtrsk ? (const T *)s->q : nullptr, (T *)s->x_g_q);
   if (evs) (void)hipEventRecord(evs[1], st);
 #define IMX_CELL(B)                                                                           \
   fast_cell<T, B, OPT><<<grid_bs(nv, B), B, 0, st>>>(                                        \
@@ -1704,7 +1710,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
       s->f_bary ? (const T *)s->x_px_wq : nullptr,                                                 \
       (V4<T> *)s->x_px_ab, (T *)s->x_px_qe,                                                        \
       s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt), \
-      (const T *)s->x_g_q, s->x_g_out, s->f_pxhc ? (const T *)s->x_px_he : nullptr,                  \
+      (const T *)s->x_g_q, s->x_g_out, s->f_pxhc ? (const T *)s->x_px_he : nullptr, s->f_glink ? 1 : 0, \
       s->f_gk ? (const T *)s->x_gk_k : nullptr, (T)s->gk_coef, s->f_melv,                          \
       s->f_melv == 2 ? (const V4<T> *)s->x_ap_wc : nullptr, s->f_dtred ? 1 : 0, hs, us, s->href,     \
       (const T *)s->x_gb,                                                                          \
@@ -3476,15 +3482,23 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     // the fused kernels take every combination of the scalar interpolation
     // trsk or bary, the Coriolis term trsk, pered or dtred and the kinetic
     // energy trsk, perhx, gass (gasscoef; dubos is gass with 0) or melv;
-    // gass18's interpolation and Coriolis term with any of these kinetic
-    // energies; the level model with trsk10's methods
+    // gass18's interpolation, Coriolis term or both (trsk's for the other)
+    // with any of these kinetic energies; the level model with trsk10's
+    // methods
     struct Methods {
       bool gass = false, bary = false, pered = false, perhx = false, gk = false, melv = false, dtred = false;
+      bool glink = false, gtrsk = false;  // gass Coriolis term; gass PV from trsk interpolation
     } fm;
     bool ok = false;
-    if (sint == IMX_SINT_GASS && cor == IMX_COR_GASS) {
-      ok = true;
+    if ((sint == IMX_SINT_GASS && (cor == IMX_COR_GASS || cor == IMX_COR_TRSK)) ||
+        (sint == IMX_SINT_TRSK && cor == IMX_COR_GASS)) {
+      // gass18's rhombus PV with the trsk Coriolis sum, or the gass
+      // Coriolis sum over the trsk average of the triangles' PV; fast_cell's
+      // gass mode has no diffusion
+      ok = sc->k2 <= 0.0 && sc->k4 <= 0.0;
       fm.gass = true;
+      fm.glink = cor == IMX_COR_GASS;
+      fm.gtrsk = sint == IMX_SINT_TRSK;
       fm.gk = rec == IMX_REC_GASS;
       fm.perhx = rec == IMX_REC_PERHX;
       fm.melv = rec == IMX_REC_MELV;
@@ -3503,8 +3517,9 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take the scalar interpolation trsk or bary, the "
                             "Coriolis term trsk, pered or dtred and any kinetic "
-                            "energy, or gass18's interpolation and Coriolis term with any kinetic "
-                            "energy, with APVM, diffusion and hyperdiffusion, HTC or HC; and the level model "
+                            "energy, with APVM, diffusion and hyperdiffusion; gass18's interpolation, "
+                            "Coriolis term or both (trsk's for the other) with any kinetic energy and APVM; "
+                            "HTC or HC; and the level model "
                             "with trsk10's methods (diffusion allowed); other method triples and the level "
                             "model with other options run on FP64_REF handles");
     int fw = fm.gass ? IMX_WRAP_GASS18 : (fm.bary && fm.pered && fm.perhx) ? IMX_WRAP_PXT16 : IMX_WRAP_TRSK10;
@@ -3535,6 +3550,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       rc = s->single ? imx::fast_dtred<float>(s, *d, m.dtred) : imx::fast_dtred<double>(s, *d, m.dtred);
       if (rc) return rc;
       const int gmode = m.gass ? (qhc ? 2 : 1) : 0;
+      s->f_glink = m.glink;
+      s->f_gtrsk = m.gtrsk;
       return s->single ? imx::fast_gass18<float>(s, *d, gmode) : imx::fast_gass18<double>(s, *d, gmode);
     };
     imx_scheme r = *sc;

@@ -309,7 +309,8 @@ int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out
    or HC, with APVM, diffusion and hyperdiffusion, every method triple of
    the scalar interpolation trsk or bary, the Coriolis term trsk, pered or
    dtred and any kinetic energy; gass18's
-   interpolation and Coriolis term with any kinetic energy
+   interpolation, Coriolis term or both (trsk's for the other) with any
+   kinetic energy and no diffusion
    (gass18 itself refuses diffusion, as the reference does); and nopv with
    trsk10's methods (with or without diffusion); they return IMX_ENOTSUP
    for other method triples and nopv with other options. */
