@@ -3513,14 +3513,15 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       fm.melv = rec == IMX_REC_MELV;
     }
     const bool trsk3 = ok && !fm.gass && !fm.bary && !fm.pered && !fm.dtred && !fm.perhx && !fm.gk && !fm.melv;
-    if (!ok || (sc->nopv && (!trsk3 || sc->apvm || sc->k4 > 0.0)))
+    if (!ok || (sc->nopv && (!trsk3 || sc->apvm)))
       return imx::set_error(IMX_ENOTSUP,
                             "imx_swm_set_scheme: fused handles take the scalar interpolation trsk or bary, the "
                             "Coriolis term trsk, pered or dtred and any kinetic "
                             "energy, with APVM, diffusion and hyperdiffusion; gass18's interpolation, "
                             "Coriolis term or both (trsk's for the other) with any kinetic energy and APVM; "
                             "HTC or HC; and the level model "
-                            "with trsk10's methods (diffusion allowed); other method triples and the level "
+                            "with trsk10's methods (diffusion and hyperdiffusion allowed); other method "
+                            "triples and the level "
                             "model with other options run on FP64_REF handles");
     int fw = fm.gass ? IMX_WRAP_GASS18 : (fm.bary && fm.pered && fm.perhx) ? IMX_WRAP_PXT16 : IMX_WRAP_TRSK10;
     // the tables of each option, in turn; every step rewrites the tables

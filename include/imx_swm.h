@@ -311,9 +311,10 @@ int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out
    dtred and any kinetic energy; gass18's
    interpolation, Coriolis term or both (trsk's for the other) with any
    kinetic energy and no diffusion
-   (gass18 itself refuses diffusion, as the reference does); and nopv with
-   trsk10's methods (with or without diffusion); they return IMX_ENOTSUP
-   for other method triples and nopv with other options. */
+   (without diffusion, which the reference refuses with gass18); and nopv with
+   trsk10's methods (with or without diffusion and hyperdiffusion); they
+   return IMX_ENOTSUP for other method triples and nopv with other
+   options. */
 int  imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc);
 int  imx_swm_set_state(imx_swm *s, const double *h, const double *u);
 int  imx_swm_get_state(imx_swm *s, double *h, double *u);

@@ -281,6 +281,10 @@ FAST_VS_REF = {
     "nopv": ({"nopv": True}, {}),
     "hcnopv": ({"stag": "HC", "nopv": True}, {"stag": "HC"}),
     "nopvk2": ({"nopv": True, "k2": 1e5, "diffus": "align"}, {"k2": 1e5, "diffus": "align"}),
+    "nopvk4": ({"nopv": True, "k2": 1e5, "diffus": "const", "k4": 1e14, "hyperdiffus": "const"},
+               {"k2": 1e5, "diffus": "const", "k4": 1e14, "hyperdiffus": "const"}),
+    "hcnopvk4": ({"stag": "HC", "nopv": True, "k4": 1e14, "hyperdiffus": "align"},
+                 {"stag": "HC", "k4": 1e14, "hyperdiffus": "align"}),
     "rgass": ({"wrap": "none", "rec": "gass", "gasscoef": 0.75}, {}),
     "rdubos": ({"wrap": "none", "rec": "dubos"}, {}),
     "hcrgass": ({"stag": "HC", "wrap": "none", "rec": "gass", "gasscoef": 0.75}, {"stag": "HC"}),
