@@ -121,6 +121,8 @@ struct imx_swm {
   bool f_gtrsk = false;              // gass mode: the edge PV from trsk interpolation
   void *x_g_w = nullptr, *x_g_a = nullptr, *x_g_E = nullptr, *x_g_H = nullptr, *x_g_q = nullptr;
   int32_t *x_g_et = nullptr, *x_g_out = nullptr;
+  void *x_g_qw = nullptr;              // gass mode with bary (HTC): the two triangles' weights per edge
+  bool f_gqw = false;
   void *x_px_wq = nullptr, *x_px_n = nullptr, *x_px_ab = nullptr, *x_px_qe = nullptr;
   // pxt16 with HC: h at each Voronoi edge midpoint from the barycentric
   // weights [3][NE] (typed T) of the cells x_px_hv [3][NE] of the triangle

@@ -848,7 +848,8 @@ __global__ __launch_bounds__(BS) void fast_gass_q(int remap, int NE, const int32
                                                    const int32_t *__restrict__ e_oth,
                                                    const V4<T> *__restrict__ apv,
                                                    const V4<T> *__restrict__ apg, T pvsdt,
-                                                   const T *__restrict__ qtr, T *__restrict__ gq) {
+                                                   const T *__restrict__ qtr, const T *__restrict__ qw,
+                                                   T *__restrict__ gq) {
   const int l = ent_index<BS>(remap);
   if (l >= NE) return;
   const int t1 = e_t[l];
@@ -856,7 +857,8 @@ __global__ __launch_bounds__(BS) void fast_gass_q(int remap, int NE, const int32
   const int t2 = e_t[NE + l];
   T qe;
   if (qtr) {  // gass Coriolis term with trsk interpolation: the plain average (scalar_tr2ed)
-    qe = (T)0.5 * (qtr[t1] + qtr[t2]);
+    // or with bary (HTC) the weights of the distances to the circumcentres
+    qe = qw ? qw[l] * qtr[t1] + qw[NE + l] * qtr[t2] : (T)0.5 * (qtr[t1] + qtr[t2]);
   } else {
     const T H = gH[t1] + gH[t2];
     qe = (H != (T)0) ? (gE[t1] + gE[t2]) / H : (T)0;
@@ -1697,7 +1699,7 @@ static void launch_fast_stage(imx_swm *s, int stage, const T *hs, const T *us, T
     fast_gass_q<T, 256><<<grid_bs(NE, 256), 256, 0, st>>>(
         s->kremap[2], NE, s->x_g_et, (const T *)s->x_g_E, (const T *)s->x_g_H, s->x_e_oth,
         s->f_apvm ? (const V4<T> *)s->x_ap_v : nullptr, (const V4<T> *)s->x_ap_g, (T)(s->sc.pvspar * s->o.dt),
-        s->f_gtrsk ? (const T *)s->q : nullptr, (T *)s->x_g_q);
+        s->f_gtrsk ? (const T *)s->q : nullptr, s->f_gqw ? (const T *)s->x_g_qw : nullptr, (T *)s->x_g_q);
   if (evs) (void)hipEventRecord(evs[1], st);
 #define IMX_CELL(B)                                                                           \
   fast_cell<T, B, OPT><<<grid_bs(nv, B), B, 0, st>>>(                                        \
@@ -3368,9 +3370,10 @@ static int fast_gassk(imx_swm *s, const imx_mesh_desc &d, bool on, double gcoef)
 // areas, the two triangles of each edge slot and the outer edge of each
 // ring triangle of each cell, in the ownership numbering, typed T
 template <typename T>
-static int fast_gass18(imx_swm *s, const imx_mesh_desc &d, int mode) {
+static int fast_gass18(imx_swm *s, const imx_mesh_desc &d, int mode, bool qwd) {
   if (!mode) {
     s->f_gass = 0;
+    s->f_gqw = false;
     return IMX_OK;
   }
   const int nv = s->nv, NE = s->NE, NT = s->NT;
@@ -3424,6 +3427,27 @@ static int fast_gass18(imx_swm *s, const imx_mesh_desc &d, int mode) {
     s->x_g_et = A.get<int32_t>((size_t)2 * NE);
     s->x_g_out = A.get<int32_t>((size_t)MX * nv);
     if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
+  }
+  s->f_gqw = false;
+  if (qwd && (!d.ed_c || !d.tr_c)) return set_error(IMX_EINVAL, "imx_swm_set_scheme: bary needs ed_c and tr_c");
+  if (qwd) {
+    // the gass Coriolis term over bary-interpolated PV (HTC): q_e =
+    // (d2 q(sh1) + d1 q(sh2)) / (d1 + d2), d_k the arc from ed%c to the
+    // circumcentre of sh(k) (scalar_tr2ed, swm_operators.f90:165-169)
+    std::vector<double> qw((size_t)2 * NE, 0.0);
+    for (int eo = 0; eo < d.ne; ++eo) {
+      const double d1 = arclen(d.ed_c + 3 * eo, d.tr_c + 3 * (d.ed_sh[2 * eo] - 1));
+      const double d2 = arclen(d.ed_c + 3 * eo, d.tr_c + 3 * (d.ed_sh[2 * eo + 1] - 1));
+      qw[en[eo]] = d2 / (d1 + d2);
+      qw[(size_t)NE + en[eo]] = d1 / (d1 + d2);
+    }
+    if (!s->x_g_qw) {
+      s->x_g_qw = A.get<T>((size_t)2 * NE);
+      if (A.err != hipSuccess) return hip_fail(A.err, "imx_swm_set_scheme");
+    }
+    const std::vector<T> qwT = cvt<T>(qw);
+    HIPTRY(hipMemcpy(s->x_g_qw, qwT.data(), qwT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
+    s->f_gqw = true;
   }
   const std::vector<T> gwT = cvt<T>(gw), gaT = cvt<T>(ga);
   HIPTRY(hipMemcpy(s->x_g_w, gwT.data(), gwT.size() * sizeof(T), hipMemcpyHostToDevice), "set_scheme");
@@ -3482,23 +3506,24 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     // the fused kernels take every combination of the scalar interpolation
     // trsk or bary, the Coriolis term trsk, pered or dtred and the kinetic
     // energy trsk, perhx, gass (gasscoef; dubos is gass with 0) or melv;
-    // gass18's interpolation, Coriolis term or both (trsk's for the other)
-    // with any of these kinetic energies; the level model with trsk10's
-    // methods
+    // gass18's interpolation, Coriolis term or both (trsk's for the other;
+    // the gass Coriolis term also with bary on HTC) with any of these
+    // kinetic energies; the level model with trsk10's methods
     struct Methods {
       bool gass = false, bary = false, pered = false, perhx = false, gk = false, melv = false, dtred = false;
       bool glink = false, gtrsk = false;  // gass Coriolis term; gass PV from trsk interpolation
     } fm;
     bool ok = false;
     if ((sint == IMX_SINT_GASS && (cor == IMX_COR_GASS || cor == IMX_COR_TRSK)) ||
-        (sint == IMX_SINT_TRSK && cor == IMX_COR_GASS)) {
+        (sint == IMX_SINT_TRSK && cor == IMX_COR_GASS) || (sint == IMX_SINT_BARY && cor == IMX_COR_GASS && !hc)) {
       // gass18's rhombus PV with the trsk Coriolis sum, or the gass
       // Coriolis sum over the trsk average of the triangles' PV; fast_cell's
       // gass mode has no diffusion
       ok = sc->k2 <= 0.0 && sc->k4 <= 0.0;
       fm.gass = true;
       fm.glink = cor == IMX_COR_GASS;
-      fm.gtrsk = sint == IMX_SINT_TRSK;
+      fm.gtrsk = sint != IMX_SINT_GASS;
+      fm.bary = sint == IMX_SINT_BARY;
       fm.gk = rec == IMX_REC_GASS;
       fm.perhx = rec == IMX_REC_PERHX;
       fm.melv = rec == IMX_REC_MELV;
@@ -3518,7 +3543,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
                             "imx_swm_set_scheme: fused handles take the scalar interpolation trsk or bary, the "
                             "Coriolis term trsk, pered or dtred and any kinetic "
                             "energy, with APVM, diffusion and hyperdiffusion; gass18's interpolation, "
-                            "Coriolis term or both (trsk's for the other) with any kinetic energy and APVM; "
+                            "Coriolis term or both (trsk's for the other; the gass Coriolis term also with "
+                            "bary on HTC) with any kinetic energy and APVM; "
                             "HTC or HC; and the level model "
                             "with trsk10's methods (diffusion and hyperdiffusion allowed); other method "
                             "triples and the level "
@@ -3553,7 +3579,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       const int gmode = m.gass ? (qhc ? 2 : 1) : 0;
       s->f_glink = m.glink;
       s->f_gtrsk = m.gtrsk;
-      return s->single ? imx::fast_gass18<float>(s, *d, gmode) : imx::fast_gass18<double>(s, *d, gmode);
+      const bool qwd = m.gass && m.bary;
+      return s->single ? imx::fast_gass18<float>(s, *d, gmode, qwd) : imx::fast_gass18<double>(s, *d, gmode, qwd);
     };
     imx_scheme r = *sc;
     r.wrap = fw;

@@ -219,6 +219,7 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         swm[10] = "none"
         swm[14] = "gass"
         swm[16] = "bary"
+        swm[8] = "HC"
         open(p, "w").write("\n".join(swm) + "\n")
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)

@@ -86,8 +86,9 @@ def test_method_refusals():
     """nopv with pered or dtred, gass interpolation with diffusion, bad
     method values; fused handles take the triples of trsk or bary
     interpolation, trsk, pered or dtred Coriolis and any kinetic energy,
-    gass18's interpolation, Coriolis term or both (trsk's for the other)
-    with any kinetic energy and no diffusion, and the level model with
+    gass18's interpolation, Coriolis term or both (trsk's for the other;
+    the gass Coriolis term also with bary on HTC) with any kinetic energy
+    and no diffusion, and the level model with
     trsk10, and refuse the other triples."""
     m = Mesh.icos(3)
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
@@ -97,7 +98,7 @@ def test_method_refusals():
                 s.set_scheme(**kw)
         s.set_scheme(wrap="none", cor="gass", nopv=True)
     with Swm(m, precision="fp32", dt=900.0) as s:
-        for kw in ({"wrap": "none", "sint": "bary", "cor": "gass", "rec": "melv"},
+        for kw in ({"stag": "HC", "wrap": "none", "sint": "bary", "cor": "gass", "rec": "melv"},
                    {"wrap": "none", "cor": "gass", "k2": 1e5}, {"wrap": "none", "sint": "gass", "cor": "pered"},
                    {"wrap": "none", "nopv": True, "pv": "apvm"},
                    {"wrap": "gass18", "nopv": True}, {"wrap": "none", "sint": "gass", "cor": "dtred"}):
@@ -167,7 +168,7 @@ def test_scheme_errors():
     m = Mesh.icos(2)
     with Swm(m, precision="fp32", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="FP64_REF"):
-            s.set_scheme(wrap="none", sint="bary", cor="gass")
+            s.set_scheme(stag="HC", wrap="none", sint="bary", cor="gass")
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
@@ -314,6 +315,9 @@ FAST_VS_REF = {
     "trskgass": ({"wrap": "none", "sint": "trsk", "cor": "gass", "rec": "gass", "gasscoef": 0.5}, {}),
     "trskgassapvm": ({"wrap": "none", "sint": "trsk", "cor": "gass", "pv": "apvm", "pvspar": 0.4},
                      {"pv": "apvm", "pvspar": 0.4}),
+    "barygass": ({"wrap": "none", "sint": "bary", "cor": "gass", "pv": "apvm", "pvspar": 0.4},
+                 {"pv": "apvm", "pvspar": 0.4}),
+    "barygassmelv": ({"wrap": "none", "sint": "bary", "cor": "gass", "rec": "melv"}, {}),
     "hctrskgass": ({"stag": "HC", "wrap": "none", "sint": "trsk", "cor": "gass", "rec": "perhx"}, {"stag": "HC"}),
     "gassperhx": ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "perhx"}, {"wrap": "gass18"}),
     "gassmelv": ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "melv", "pv": "apvm", "pvspar": 0.4},
@@ -378,7 +382,8 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"wrap": "none", "sint": "bary", "cor": "gass"}, {"wrap": "none", "sint": "gass", "cor": "dtred"},
+        for kw in ({"stag": "HC", "wrap": "none", "sint": "bary", "cor": "gass"},
+                   {"wrap": "none", "sint": "gass", "cor": "dtred"},
                    {"stag": "HC", "wrap": "none", "cor": "gass", "k4": 1e14},
                    {"wrap": "none", "sint": "gass", "cor": "dtred"}):
             with pytest.raises(capi.ImxError, match="fused handles take"):
