@@ -1072,7 +1072,7 @@ __global__ __launch_bounds__(BS) void fast_cell(
   double hE[MX];
 #pragma unroll
   for (int j = 0; j < MX; ++j)
-    hE[j] = (kAll && hce) ? (double)hce[ed[j]] : href + 0.5 * ((double)hc + (double)hn[j]);
+    hE[j] = ((kAll || kGass) && hce) ? (double)hce[ed[j]] : href + 0.5 * ((double)hc + (double)hn[j]);
   double dv = 0.0, dvc = 0.0;
   T ke = (T)0;
   T G[MX];
@@ -3506,16 +3506,15 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
     // the fused kernels take every combination of the scalar interpolation
     // trsk or bary, the Coriolis term trsk, pered or dtred and the kinetic
     // energy trsk, perhx, gass (gasscoef; dubos is gass with 0) or melv;
-    // gass18's interpolation, Coriolis term or both (trsk's for the other;
-    // the gass Coriolis term also with bary on HTC) with any of these
-    // kinetic energies; the level model with trsk10's methods
+    // gass18's interpolation, Coriolis term or both (trsk's or bary's for
+    // the other) with any of these kinetic energies; the level model with trsk10's methods
     struct Methods {
       bool gass = false, bary = false, pered = false, perhx = false, gk = false, melv = false, dtred = false;
       bool glink = false, gtrsk = false;  // gass Coriolis term; gass PV from trsk interpolation
     } fm;
     bool ok = false;
     if ((sint == IMX_SINT_GASS && (cor == IMX_COR_GASS || cor == IMX_COR_TRSK)) ||
-        (sint == IMX_SINT_TRSK && cor == IMX_COR_GASS) || (sint == IMX_SINT_BARY && cor == IMX_COR_GASS && !hc)) {
+        (sint == IMX_SINT_TRSK && cor == IMX_COR_GASS) || (sint == IMX_SINT_BARY && cor == IMX_COR_GASS)) {
       // gass18's rhombus PV with the trsk Coriolis sum, or the gass
       // Coriolis sum over the trsk average of the triangles' PV; fast_cell's
       // gass mode has no diffusion
@@ -3543,8 +3542,8 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
                             "imx_swm_set_scheme: fused handles take the scalar interpolation trsk or bary, the "
                             "Coriolis term trsk, pered or dtred and any kinetic "
                             "energy, with APVM, diffusion and hyperdiffusion; gass18's interpolation, "
-                            "Coriolis term or both (trsk's for the other; the gass Coriolis term also with "
-                            "bary on HTC) with any kinetic energy and APVM; "
+                            "Coriolis term or both (trsk's or bary's for the other) with any kinetic energy "
+                            "and APVM; "
                             "HTC or HC; and the level model "
                             "with trsk10's methods (diffusion and hyperdiffusion allowed); other method "
                             "triples and the level "
@@ -3579,7 +3578,7 @@ int imx_swm_set_scheme(imx_swm *s, const imx_mesh_desc *d, const imx_scheme *sc)
       const int gmode = m.gass ? (qhc ? 2 : 1) : 0;
       s->f_glink = m.glink;
       s->f_gtrsk = m.gtrsk;
-      const bool qwd = m.gass && m.bary;
+      const bool qwd = m.gass && m.bary && !qhc;  // HC: the plain average
       return s->single ? imx::fast_gass18<float>(s, *d, gmode, qwd) : imx::fast_gass18<double>(s, *d, gmode, qwd);
     };
     imx_scheme r = *sc;

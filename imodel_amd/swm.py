@@ -266,7 +266,7 @@ class Swm:
         Every option runs on FP64_REF handles; fused (FP32/FP64) handles take
         every triple of sint "trsk"/"bary", cor "trsk"/"pered"/"dtred" and
         any rec, sint "gass" with cor "gass"/"trsk" or cor "gass" with sint
-        "trsk" (or "bary" on HTC; no diffusion) with any rec, and nopv with trsk10's methods, with APVM, diffusion and
+        "trsk"/"bary" (no diffusion) with any rec, and nopv with trsk10's methods, with APVM, diffusion and
         hyperdiffusion, HTC or HC, and refuse the others."""
         if rec == "dubos":  # Gassmann's K with no trsk part (swmpars, src/swm_data.f90:513-516)
             rec, gasscoef = "gass", 0.0

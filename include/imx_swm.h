@@ -309,8 +309,7 @@ int  imx_swm_create(const imx_mesh_desc *d, const imx_swm_opts *o, imx_swm **out
    or HC, with APVM, diffusion and hyperdiffusion, every method triple of
    the scalar interpolation trsk or bary, the Coriolis term trsk, pered or
    dtred and any kinetic energy; gass18's interpolation, Coriolis term or
-   both (trsk's for the other; the gass Coriolis term also with bary on
-   HTC) with any kinetic energy and without diffusion; and nopv with
+   both (trsk's or bary's for the other) with any kinetic energy and without diffusion; and nopv with
    trsk10's methods (with or without diffusion and hyperdiffusion); they
    return IMX_ENOTSUP for other method triples and nopv with other
    options. */

@@ -217,9 +217,8 @@ def test_fortran_driver_scheme_needs_fp64_ref():
         p = os.path.join(td, "par", "swm.par")
         swm = open(p).read().splitlines()
         swm[10] = "none"
-        swm[14] = "gass"
-        swm[16] = "bary"
-        swm[8] = "HC"
+        swm[14] = "pered"
+        swm[16] = "gass"
         open(p, "w").write("\n".join(swm) + "\n")
         r = subprocess.run([IMODEL], cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                            text=True, timeout=120)

@@ -86,9 +86,8 @@ def test_method_refusals():
     """nopv with pered or dtred, gass interpolation with diffusion, bad
     method values; fused handles take the triples of trsk or bary
     interpolation, trsk, pered or dtred Coriolis and any kinetic energy,
-    gass18's interpolation, Coriolis term or both (trsk's for the other;
-    the gass Coriolis term also with bary on HTC) with any kinetic energy
-    and no diffusion, and the level model with
+    gass18's interpolation, Coriolis term or both (trsk's or bary's for
+    the other) with any kinetic energy and no diffusion, and the level model with
     trsk10, and refuse the other triples."""
     m = Mesh.icos(3)
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
@@ -98,7 +97,7 @@ def test_method_refusals():
                 s.set_scheme(**kw)
         s.set_scheme(wrap="none", cor="gass", nopv=True)
     with Swm(m, precision="fp32", dt=900.0) as s:
-        for kw in ({"stag": "HC", "wrap": "none", "sint": "bary", "cor": "gass", "rec": "melv"},
+        for kw in ({"wrap": "none", "sint": "bary", "cor": "gass", "k2": 1e5},
                    {"wrap": "none", "cor": "gass", "k2": 1e5}, {"wrap": "none", "sint": "gass", "cor": "pered"},
                    {"wrap": "none", "nopv": True, "pv": "apvm"},
                    {"wrap": "gass18", "nopv": True}, {"wrap": "none", "sint": "gass", "cor": "dtred"}):
@@ -168,7 +167,7 @@ def test_scheme_errors():
     m = Mesh.icos(2)
     with Swm(m, precision="fp32", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="FP64_REF"):
-            s.set_scheme(stag="HC", wrap="none", sint="bary", cor="gass")
+            s.set_scheme(wrap="none", sint="gass", cor="pered")
     with Swm(m, precision="fp64_ref", dt=900.0) as s:
         with pytest.raises(capi.ImxError, match="bad option"):
             s.set_scheme(k2=-1.0)
@@ -318,6 +317,8 @@ FAST_VS_REF = {
     "barygass": ({"wrap": "none", "sint": "bary", "cor": "gass", "pv": "apvm", "pvspar": 0.4},
                  {"pv": "apvm", "pvspar": 0.4}),
     "barygassmelv": ({"wrap": "none", "sint": "bary", "cor": "gass", "rec": "melv"}, {}),
+    "hcbarygass": ({"stag": "HC", "wrap": "none", "sint": "bary", "cor": "gass", "rec": "melv", "pv": "apvm",
+                    "pvspar": 0.4}, {"stag": "HC", "pv": "apvm", "pvspar": 0.4}),
     "hctrskgass": ({"stag": "HC", "wrap": "none", "sint": "trsk", "cor": "gass", "rec": "perhx"}, {"stag": "HC"}),
     "gassperhx": ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "perhx"}, {"wrap": "gass18"}),
     "gassmelv": ({"wrap": "none", "sint": "gass", "cor": "gass", "rec": "melv", "pv": "apvm", "pvspar": 0.4},
@@ -382,7 +383,7 @@ def test_fast_scheme_refusals():
     say("fast scheme refusals")
     m = Mesh.icos(2)
     with Swm(m, precision="fp64", dt=900.0) as s:
-        for kw in ({"stag": "HC", "wrap": "none", "sint": "bary", "cor": "gass"},
+        for kw in ({"wrap": "none", "sint": "gass", "cor": "pered", "rec": "perhx"},
                    {"wrap": "none", "sint": "gass", "cor": "dtred"},
                    {"stag": "HC", "wrap": "none", "cor": "gass", "k4": 1e14},
                    {"wrap": "none", "sint": "gass", "cor": "dtred"}):
