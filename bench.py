@@ -372,14 +372,21 @@ def bench_scheme(args):
     # the options on the fused kernels, FP32 and FP64, against their plain
     # step
     fused = {}
+    fused_sets = (("trsk10", {}), ("k2const", SCHEME_SETS["k2const"]),
+                  ("k2diam", {"k2": 2e5, "diffus": "diam"}), ("apvm", SCHEME_SETS["apvm"]),
+                  ("apvm_k2", dict(SCHEME_SETS["apvm"], **SCHEME_SETS["k2const"])),
+                  ("hc", {"stag": "HC"}), ("k4diam", SCHEME_SETS["k4diam"]),
+                  ("pxt16", SCHEME_SETS["pxt16"]), ("pxmix", SCHEME_SETS["pxmix"]),
+                  ("gass18", SCHEME_SETS["gass18"]), ("hcgass18", {"stag": "HC", "wrap": "gass18"}),
+                  ("hcpxt16", {"stag": "HC", "wrap": "pxt16"}),
+                  # individual methods of wrapper none
+                  ("nopv", {"nopv": True}), ("melv", {"wrap": "none", "rec": "melv"}),
+                  ("gassk", {"wrap": "none", "rec": "gass", "gasscoef": 0.5}),
+                  ("dtred", {"wrap": "none", "cor": "dtred"}),
+                  ("gasscor", {"wrap": "none", "cor": "gass"}),
+                  ("gassperhx", {"wrap": "none", "sint": "gass", "cor": "gass", "rec": "perhx"}))
     for prec in ("fp32", "fp64"):
-        for name, sc in (("trsk10", {}), ("k2const", SCHEME_SETS["k2const"]),
-                         ("k2diam", {"k2": 2e5, "diffus": "diam"}), ("apvm", SCHEME_SETS["apvm"]),
-                         ("apvm_k2", dict(SCHEME_SETS["apvm"], **SCHEME_SETS["k2const"])),
-                         ("hc", {"stag": "HC"}), ("k4diam", SCHEME_SETS["k4diam"]),
-                         ("pxt16", SCHEME_SETS["pxt16"]), ("pxmix", SCHEME_SETS["pxmix"]),
-                         ("gass18", SCHEME_SETS["gass18"]), ("hcgass18", {"stag": "HC", "wrap": "gass18"}),
-                         ("hcpxt16", {"stag": "HC", "wrap": "pxt16"})):
+        for name, sc in fused_sets:
             with Swm(mesh, precision=prec, dt=args.dt) as s:
                 s.set_scheme(**sc)
                 s.set_state(h0, u0)
@@ -390,8 +397,7 @@ def bench_scheme(args):
             fused[key] = {"ms_per_step": ms / steps, "cell_steps_per_s": nv * steps / (ms * 1e-3),
                           "options": sc}
             log(f"fused {key}: {ms / steps:.4f} ms/step")
-        for name in ("k2const", "k2diam", "apvm", "apvm_k2", "hc", "k4diam", "pxt16", "pxmix", "gass18", "hcgass18",
-                     "hcpxt16"):
+        for name, _ in fused_sets[1:]:
             fused[f"{prec}_{name}"]["vs_plain"] = (fused[f"{prec}_trsk10"]["ms_per_step"]
                                                    / fused[f"{prec}_{name}"]["ms_per_step"])
     top = res["pxmix"]
